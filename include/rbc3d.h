@@ -1,0 +1,149 @@
+/*
+ * rbc3d.h -- C ABI of the MI355X (gfx950) implementation of RBC3D's Ewald-split
+ * boundary-integral matrix-vector product.
+ *
+ * Every entry point returns an int status (RBC3D_OK = 0).  Arrays are the
+ * reference's Fortran arrays passed by base address (column-major, indices kept
+ * 1-based where the reference stores indices), so a Fortran ISO_C_BINDING shim can
+ * forward c_loc(array(1,1)) unchanged.  Host pointers unless the name ends in _dev.
+ *
+ * Each declaration names the reference routine it replaces (paths relative to the
+ * RBC3D source tree).  The reference reads module globals (ModConf, ModData); here
+ * they live in an opaque context created once per process/rank and GPU.
+ */
+#ifndef RBC3D_H
+#define RBC3D_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  RBC3D_OK = 0,
+  RBC3D_EINVAL = 1,      /* invalid argument or shape */
+  RBC3D_EHIP = 2,        /* HIP runtime error */
+  RBC3D_ENBR = 3,        /* more than 32 neighbouring cells within rc of a target */
+  RBC3D_ESTATE = 4,      /* call out of order (e.g. no cells set) */
+  RBC3D_ENONFINITE = 5,  /* non-finite input */
+  RBC3D_ENOCONV = 6      /* GMRES did not reach the tolerance within maxit */
+};
+
+typedef struct rbc3d_ctx rbc3d_ctx;
+
+/* Module-global parameters (common/ModConf.F90:14-47).  alpha is the reference's
+ * Ewald parameter (erfc(sqrt(pi/alpha) r), exp(-pi alpha q^2)); rc, Nb and P are
+ * given explicitly (what SetEwaldPrms, common/ModConf.F90:348-408, derives). */
+typedef struct {
+  double Lb[3];
+  double alpha;
+  double rc;
+  int Nb[3];
+  int P;
+  int ntypes;
+  double viscRat[8]; /* initCOEFs, common/ModConf.F90:327-344 -> Acoef, Bcoef */
+  double vBkg[3];
+} rbc3d_conf;
+
+/* --- lifetime: PME_Init/PME_Finalize (common/ModPME.F90:252-345) + the Ewald
+ *     tables of common/ModEwaldFunc.F90:97-107,152-162 built eagerly ---------- */
+int rbc3d_create(const rbc3d_conf *conf, rbc3d_ctx **out);
+int rbc3d_destroy(rbc3d_ctx *ctx);
+const char *rbc3d_last_error(void);
+
+/* --- module data: rbcs(:) and walls(:) of common/ModData.F90:13-20 ---------- */
+/* Cells: RBC_Create grid (common/ModRbc.F90:44-110) for per-cell nlat (nlon = 2 nlat),
+ * geometry x(nlat,nlon,3), a3(nlat,nlon,3), detj(nlat,nlon) concatenated over cells.
+ * Builds the x/a3/detJ surface splines (Rbc_BuildSurfaceSource(xFlag),
+ * common/ModRbc.F90:737-767), the polar patch (RbcPolarPatch_Create,
+ * common/ModPolarPatch.F90:27-75), and the source list + cell list
+ * (SourceList_UpdateCoord + HashTable_Build, common/ModSourceList.F90:92-151). */
+int rbc3d_set_cells(rbc3d_ctx *ctx, int nrbc, const int *nlat0, const int *nlat,
+                    const int *celltype, const double *x, const double *a3, const double *detj);
+/* which = 1: f, 2: g.  Rbc_BuildSurfaceSource(fFlag|gFlag) (common/ModRbc.F90:770-802)
+ * + SourceList_UpdateDensity (common/ModSourceList.F90:160-187). */
+int rbc3d_set_cell_density(rbc3d_ctx *ctx, int which, const double *dens);
+/* One wall: x(nvert,3), e2v(nele,3) 1-based.  Wall_Build_V2V + Wall_ComputeGeometry
+ * (common/ModWall.F90:66-145), global vertex numbering (common/ModData.F90:50-63),
+ * wall source list (common/ModSourceList.F90:130-147). */
+int rbc3d_set_wall(rbc3d_ctx *ctx, int nvert, int nele, const double *x, const int *e2v);
+int rbc3d_set_wall_density(rbc3d_ctx *ctx, const double *f);
+/* PrepareSingIntOnWall (common/ModIntOnWalls.F90:181-308): sparse wall self-matrix */
+int rbc3d_prepare_sing_int_on_wall(rbc3d_ctx *ctx);
+int rbc3d_wall_self_nnz(rbc3d_ctx *ctx, long long *nnz);
+
+/* --- cell list ----------------------------------------------------------- */
+/* HashTable_Build (common/ModHashTable.F90:23-58): hoc(0:Nc1+1,0:Nc2+1,0:Nc3+1),
+ * next(nPoint), 1-based, -1 sentinel; x(ldx,3). Bit-identical to the reference. */
+int rbc3d_hashtable_build(const int Nc[3], const double iLbNc[3], int nPoint, const double *x,
+                          int ldx, int *hoc, int *next);
+
+/* --- hot path: real space ------------------------------------------------- */
+/* AddIntOnRbcs(c1,c2,tlist,v) (common/ModIntOnRbcs.F90:25-158): pair sums over the
+ * cell list, same-surface masking, self-patch singular quadrature, near-singular
+ * corrections, AddLinearInt.  tlist = (nT, x(nT,3), indx(nT,0:2), active(nT) as
+ * 4-byte logicals, Acoef(nT)); v(nT,3) accumulated in place. */
+int rbc3d_add_int_on_rbcs(rbc3d_ctx *ctx, double c1, double c2, int nT, const double *tx,
+                          const int *tindx, const int *active, const double *Acoef, double *v);
+/* AddIntOnWalls(c1,tlist,v) (common/ModIntOnWalls.F90:37-128) */
+int rbc3d_add_int_on_walls(rbc3d_ctx *ctx, double c1, int nT, const double *tx, const int *tindx,
+                           const int *active, const double *Acoef, double *v);
+/* Diagnostics of the last add_int_on_rbcs: in-cutoff pair count and a checksum of
+ * the pair set (sum over pairs of (i+1)*(j+1) mod 2^64, 1-based target/source). */
+int rbc3d_pair_stats(rbc3d_ctx *ctx, long long *npairs, unsigned long long *checksum);
+
+/* --- hot path: PME (common/ModPME.F90) ------------------------------------ */
+/* PME_Distrib_Source(c1,c2,cells,walls) :58-133 (cells/walls = present flags) */
+int rbc3d_pme_distrib_source(rbc3d_ctx *ctx, double c1, double c2, int cells, int walls);
+/* PME_Transform :137-222 (forward FFTs, Green's function, B-factor, inverse FFTs) */
+int rbc3d_pme_transform(rbc3d_ctx *ctx);
+/* PME_Add_Interp_Vel(tlist,v) :228-248 */
+int rbc3d_pme_add_interp_vel(rbc3d_ctx *ctx, int nT, const double *tx, const int *active,
+                             const double *Acoef, double *v);
+/* copy a PME grid to the host: which = 0 ff (3 fields), 1 tt (9), 2 vv (3);
+ * layout ff(0:N1-1,0:N2-1,0:N3-1,field) */
+int rbc3d_pme_grid(rbc3d_ctx *ctx, int which, double *out);
+
+/* --- hot path: spherical harmonics (common/ModSphpk.F90) ------------------ */
+/* ShAnalGau (:74-105, SPHEREPACK shags), ShSynthGau (:240-270, shsgs),
+ * ShSynthEqu (:308-338, shses) with the reference's argument lists. */
+int rbc3d_sh_anal_gau(int nlat, int nlon, int nt, const double *g, int idg, int jdg, double *a,
+                      double *b, int ida, int jda);
+int rbc3d_sh_synth_gau(int nlat, int nlon, int nt, double *g, int idg, int jdg, const double *a,
+                       const double *b, int ida, int jda);
+int rbc3d_sh_synth_equ(int nlat, int nlon, int nt, double *g, int idg, int jdg, const double *a,
+                       const double *b, int ida, int jda);
+/* Glob_Sph_Trans(v,c,direction) (common/ModVelSolver.F90:641-719); direction
+ * 1 = PHYS_TO_FOUR, -1 = FOUR_TO_PHYS.  v(nPoint,3), c(3*sum nlat0^2). */
+int rbc3d_glob_sph_trans(rbc3d_ctx *ctx, double *v, double *c, int direction);
+/* spline tables of cell irbc (0-based): which 0 x, 1 a3, 2 detJ, 3 f*detJ, 4 g*detJ;
+ * each u,u1,u2,u12 (2 nlat, nlon, nvar) as Spline_Build_on_Sphere leaves them */
+int rbc3d_cell_spline(rbc3d_ctx *ctx, int irbc, int which, double *u, double *u1, double *u2,
+                      double *u12);
+
+/* --- composite operators --------------------------------------------------- */
+/* CalcVelocityField(tlist,v) (common/ModPostProcess.F90:29-61) */
+int rbc3d_calc_velocity_field(rbc3d_ctx *ctx, int nT, const double *tx, const int *tindx,
+                              const int *active, const double *Acoef, double *v);
+/* MyMatMult of the cell BIE (common/ModVelSolver.F90:523-601): b = A u, u and b of
+ * length 3*sum(nlat0^2) in Glob_Sph_Trans packing.  The _dev form takes device
+ * pointers and leaves the result on the device (stream-ordered). */
+int rbc3d_cell_dof(rbc3d_ctx *ctx, int *dof);
+int rbc3d_cell_matmult(rbc3d_ctx *ctx, const double *u, double *b);
+int rbc3d_cell_matmult_dev(rbc3d_ctx *ctx, const double *d_u, double *d_b);
+/* Compute_Rhs (common/ModVelSolver.F90:455-515) */
+int rbc3d_cell_rhs(rbc3d_ctx *ctx, double *rhs);
+/* Wall no-slip operator (common/ModNoSlip.F90): MyMatMult :255-308 on the
+ * de-duplicated traction vector (3 per unique vertex), Compute_Wall_Residual_Vel
+ * :153-193 on all wall vertices v(nvert,3). */
+int rbc3d_wall_dof(rbc3d_ctx *ctx, int *dof);
+int rbc3d_wall_matmult(rbc3d_ctx *ctx, const double *u, double *b);
+int rbc3d_wall_residual_vel(rbc3d_ctx *ctx, double *v);
+
+/* --- timing hooks (HIP events on the context's stream) --------------------- */
+int rbc3d_stream(rbc3d_ctx *ctx, void **hip_stream);
+int rbc3d_sync(rbc3d_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

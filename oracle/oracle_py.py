@@ -1,0 +1,303 @@
+"""ctypes binding of the CPU restatement (oracle/rbc3d_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, the smoke check and the CPU-baseline
+leg of bench.py, as the checker.  The product path never imports this module.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "build", "librbc3d_oracle.so")
+
+dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+class OrcConf(C.Structure):
+    _fields_ = [("Lb", C.c_double * 3), ("alpha", C.c_double), ("rc", C.c_double),
+                ("Nb", C.c_int * 3), ("P", C.c_int), ("ntypes", C.c_int),
+                ("viscRat", C.c_double * 8), ("vBkg", C.c_double * 3)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    if not os.path.exists(_LIB):
+        build()
+    L = C.CDLL(_LIB)
+    vp = C.c_void_p
+    sig = {
+        "orc_create": (vp, [C.POINTER(OrcConf)]),
+        "orc_destroy": (None, [vp]),
+        "orc_set_threads": (C.c_int, [C.c_int]),
+        "orc_set_cells": (C.c_int, [vp, C.c_int, ip, ip, ip, dp, dp, dp]),
+        "orc_set_cell_density": (C.c_int, [vp, C.c_int, dp]),
+        "orc_set_wall": (C.c_int, [vp, C.c_int, C.c_int, dp, ip]),
+        "orc_set_wall_density": (C.c_int, [vp, dp]),
+        "orc_prepare_sing_int_on_wall": (C.c_int, [vp]),
+        "orc_ntargets_rbc": (C.c_int, [vp]),
+        "orc_ntargets_wall": (C.c_int, [vp]),
+        "orc_tlist_rbc": (C.c_int, [vp, dp, ip, dp]),
+        "orc_tlist_wall": (C.c_int, [vp, dp, ip, dp]),
+        "orc_add_int_on_rbcs": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, dp, ip, ip, dp, dp,
+                                          C.POINTER(C.c_longlong)]),
+        "orc_add_int_on_walls": (C.c_int, [vp, C.c_double, C.c_int, dp, ip, ip, dp, dp]),
+        "orc_pme_distrib_source": (C.c_int, [vp, C.c_double, C.c_double, C.c_int, C.c_int]),
+        "orc_pme_transform": (C.c_int, [vp]),
+        "orc_pme_add_interp_vel": (C.c_int, [vp, C.c_int, dp, ip, dp, dp]),
+        "orc_calc_velocity_field": (C.c_int, [vp, C.c_int, dp, ip, ip, dp, dp]),
+        "orc_glob_sph_trans": (C.c_int, [vp, dp, dp, C.c_int]),
+        "orc_cell_dof": (C.c_int, [vp]),
+        "orc_cell_matmult": (C.c_int, [vp, dp, dp]),
+        "orc_cell_rhs": (C.c_int, [vp, dp]),
+        "orc_wall_dof": (C.c_int, [vp]),
+        "orc_wall_matmult": (C.c_int, [vp, dp, dp]),
+        "orc_wall_residual_vel": (C.c_int, [vp, dp]),
+        "orc_ewald_sl": (None, [vp, C.c_double, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "orc_ewald_dl": (None, [vp, C.c_double, C.POINTER(C.c_double)]),
+        "orc_mask": (C.c_double, [vp, C.c_double]),
+        "orc_bspline": (None, [C.c_double, C.c_int, C.POINTER(C.c_int), dp]),
+        "orc_gauleg": (None, [C.c_double, C.c_double, C.c_int, dp, dp]),
+        "orc_gauleg_sinh": (None, [C.c_double, C.c_double, C.c_double, C.c_double, C.c_int, dp, dp]),
+        "orc_gaqd": (None, [C.c_int, dp, dp]),
+        "orc_pbar_table": (None, [C.c_int, dp, C.c_int, dp]),
+        "orc_hashtable_build": (C.c_int, [ip, dp, C.c_int, dp, C.c_int, ip, ip]),
+        "orc_sh_anal_gau": (C.c_int, [C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, dp,
+                                      C.c_int, C.c_int]),
+        "orc_sh_synth_gau": (C.c_int, [C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, dp,
+                                       C.c_int, C.c_int]),
+        "orc_sh_synth_equ": (C.c_int, [C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, dp,
+                                       C.c_int, C.c_int]),
+        "orc_fft_diff": (None, [C.c_int, C.c_int, C.c_int, dp, dp, dp]),
+        "orc_min_dist_to_tri": (C.c_double, [dp, dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "orc_tri_int_regular": (None, [vp, dp, dp, dp, dp]),
+        "orc_tri_int_duffy": (None, [vp, dp, dp, dp, C.c_double, C.c_double, dp]),
+        "orc_wall_self_nnz": (C.c_int, [vp]),
+        "orc_wall_self_csr": (C.c_int, [vp, ip, ip, dp]),
+        "orc_pme_grid": (C.c_int, [vp, C.c_int, dp]),
+        "orc_fft3d_forward": (None, [C.c_int, C.c_int, C.c_int, dp, dp]),
+        "orc_fft3d_inverse": (None, [C.c_int, C.c_int, C.c_int, dp, dp]),
+        "orc_cell_spline": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp, dp]),
+        "orc_pair_checksum": (C.c_ulonglong, [vp]),
+        "orc_near_list": (C.c_int, [vp, C.c_int, dp, ip, C.c_int, ip, ip, ip, dp, C.c_int]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    return L
+
+
+_L = None
+
+
+def L():
+    global _L
+    if _L is None:
+        _L = lib()
+    return _L
+
+
+def make_conf(Lb, alpha, rc, Nb, P, viscRat, vBkg=(0, 0, 0)):
+    c = OrcConf()
+    c.Lb[:] = list(Lb)
+    c.alpha = alpha
+    c.rc = rc
+    c.Nb[:] = list(Nb)
+    c.P = P
+    c.ntypes = len(viscRat)
+    for i, v in enumerate(viscRat):
+        c.viscRat[i] = v
+    c.vBkg[:] = list(vBkg)
+    return c
+
+
+class Oracle:
+    """A restated reference system: conf + cells (+ one wall)."""
+
+    def __init__(self, Lb, alpha, rc, Nb, P, viscRat, vBkg=(0, 0, 0)):
+        self.conf = make_conf(Lb, alpha, rc, Nb, P, viscRat, vBkg)
+        self.h = L().orc_create(C.byref(self.conf))
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            L().orc_destroy(self.h)
+            self.h = None
+
+    @staticmethod
+    def threads(n=0):
+        return L().orc_set_threads(n)
+
+    def set_cells(self, cells):
+        r = L().orc_set_cells(self.h, cells.ncell, np.ascontiguousarray(cells.nlat0, np.int32),
+                              np.ascontiguousarray(cells.nlat, np.int32),
+                              np.ascontiguousarray(cells.celltype, np.int32), cells.x, cells.a3,
+                              cells.detj)
+        assert r == 0
+        self.np = L().orc_ntargets_rbc(self.h)
+
+    def set_density(self, which, dens):
+        assert L().orc_set_cell_density(self.h, which, np.ascontiguousarray(dens)) == 0
+
+    def set_wall(self, x, e2v, nvert, nele):
+        assert L().orc_set_wall(self.h, nvert, nele, np.ascontiguousarray(x),
+                                np.ascontiguousarray(e2v, np.int32)) == 0
+
+    def set_wall_density(self, f):
+        assert L().orc_set_wall_density(self.h, np.ascontiguousarray(f)) == 0
+
+    def prepare_wall(self):
+        assert L().orc_prepare_sing_int_on_wall(self.h) == 0
+
+    def tlist_rbc(self):
+        n = L().orc_ntargets_rbc(self.h)
+        tx = np.zeros(3 * n)
+        ti = np.zeros(3 * n, np.int32)
+        Ac = np.zeros(n)
+        L().orc_tlist_rbc(self.h, tx, ti, Ac)
+        return tx, ti, np.ones(n, np.int32), Ac
+
+    def tlist_wall(self):
+        n = L().orc_ntargets_wall(self.h)
+        tx = np.zeros(3 * n)
+        ti = np.zeros(3 * n, np.int32)
+        Ac = np.zeros(n)
+        L().orc_tlist_wall(self.h, tx, ti, Ac)
+        return tx, ti, np.ones(n, np.int32), Ac
+
+    def add_int_on_rbcs(self, c1, c2, tl, v=None):
+        tx, ti, act, Ac = tl
+        n = len(Ac)
+        v = np.zeros(3 * n) if v is None else v
+        npairs = C.c_longlong(0)
+        r = L().orc_add_int_on_rbcs(self.h, c1, c2, n, tx, ti, act, Ac, v, C.byref(npairs))
+        if r:
+            raise RuntimeError(f"orc_add_int_on_rbcs status {r}")
+        return v, npairs.value
+
+    def add_int_on_walls(self, c1, tl, v=None):
+        tx, ti, act, Ac = tl
+        n = len(Ac)
+        v = np.zeros(3 * n) if v is None else v
+        assert L().orc_add_int_on_walls(self.h, c1, n, tx, ti, act, Ac, v) == 0
+        return v
+
+    def pme(self, c1, c2, tl, cells=True, walls=True, v=None):
+        tx, ti, act, Ac = tl
+        n = len(Ac)
+        v = np.zeros(3 * n) if v is None else v
+        L().orc_pme_distrib_source(self.h, c1, c2, int(cells), int(walls))
+        L().orc_pme_transform(self.h)
+        L().orc_pme_add_interp_vel(self.h, n, tx, act, Ac, v)
+        return v
+
+    def calc_velocity_field(self, tl):
+        tx, ti, act, Ac = tl
+        n = len(Ac)
+        v = np.zeros(3 * n)
+        r = L().orc_calc_velocity_field(self.h, n, tx, ti, act, Ac, v)
+        if r:
+            raise RuntimeError(f"orc_calc_velocity_field status {r}")
+        return v
+
+    def cell_dof(self):
+        return L().orc_cell_dof(self.h)
+
+    def glob_sph_trans(self, v=None, c=None, direction=1):
+        if direction == 1:
+            c = np.zeros(self.cell_dof())
+            L().orc_glob_sph_trans(self.h, np.ascontiguousarray(v), c, 1)
+            return c
+        v = np.zeros(3 * self.np)
+        L().orc_glob_sph_trans(self.h, v, np.ascontiguousarray(c), -1)
+        return v
+
+    def cell_matmult(self, u):
+        b = np.zeros(self.cell_dof())
+        r = L().orc_cell_matmult(self.h, np.ascontiguousarray(u), b)
+        if r:
+            raise RuntimeError(f"orc_cell_matmult status {r}")
+        return b
+
+    def cell_rhs(self):
+        b = np.zeros(self.cell_dof())
+        assert L().orc_cell_rhs(self.h, b) == 0
+        return b
+
+    def wall_dof(self):
+        return L().orc_wall_dof(self.h)
+
+    def wall_matmult(self, u):
+        b = np.zeros(self.wall_dof())
+        L().orc_wall_matmult(self.h, np.ascontiguousarray(u), b)
+        return b
+
+    def wall_residual_vel(self):
+        v = np.zeros(3 * L().orc_ntargets_wall(self.h))
+        L().orc_wall_residual_vel(self.h, v)
+        return v
+
+    def wall_csr(self):
+        nnz = L().orc_wall_self_nnz(self.h)
+        nv = L().orc_ntargets_wall(self.h)
+        rp = np.zeros(3 * nv + 1, np.int32)
+        col = np.zeros(nnz, np.int32)
+        val = np.zeros(nnz)
+        L().orc_wall_self_csr(self.h, rp, col, val)
+        return rp, col, val
+
+    def pme_grid(self, which):
+        N = int(np.prod(list(self.conf.Nb)))
+        out = np.zeros((9 if which == 1 else 3) * N)
+        L().orc_pme_grid(self.h, which, out)
+        return out
+
+    def cell_spline(self, irbc, which, m, n, nvar):
+        u, u1, u2, u12 = (np.zeros(m * n * nvar) for _ in range(4))
+        L().orc_cell_spline(self.h, irbc, which, u, u1, u2, u12)
+        return u, u1, u2, u12
+
+    def near_list(self, tl, i, cap=32):
+        tx, ti, act, Ac = tl
+        cells = np.zeros(cap, np.int32)
+        il = np.zeros(cap, np.int32)
+        jl = np.zeros(cap, np.int32)
+        d = np.zeros(cap)
+        n = L().orc_near_list(self.h, len(Ac), tx, ti, i, cells, il, jl, d, cap)
+        return n, cells[:max(n, 0)], il[:max(n, 0)], jl[:max(n, 0)], d[:max(n, 0)]
+
+    def pair_checksum(self):
+        return L().orc_pair_checksum(self.h)
+
+    def ewald_sl(self, r):
+        A, B = C.c_double(), C.c_double()
+        L().orc_ewald_sl(self.h, r, C.byref(A), C.byref(B))
+        return A.value, B.value
+
+    def ewald_dl(self, r):
+        A = C.c_double()
+        L().orc_ewald_dl(self.h, r, C.byref(A))
+        return A.value
+
+    def mask(self, x):
+        return L().orc_mask(self.h, x)
+
+
+def from_workload(w, threads=0):
+    o = Oracle(**w.conf_kwargs())
+    if threads:
+        Oracle.threads(threads)
+    o.set_cells(w.cells)
+    o.set_density(1, w.cells.f)
+    o.set_density(2, w.cells.g)
+    if w.wall is not None:
+        x, e2v, nv, ne = w.wall
+        o.set_wall(x, e2v, nv, ne)
+    return o

@@ -1,0 +1,348 @@
+"""Python host binding of librbc3d_gpu.so, the gfx950 implementation of RBC3D's
+Ewald-split boundary-integral matvec (C ABI: include/rbc3d.h).
+
+The names mirror the reference's Fortran module routines (common/ModIntOnRbcs.F90
+AddIntOnRbcs, common/ModPME.F90 PME_*, common/ModSphpk.F90 Sh*, ...).  Arrays use
+the reference's Fortran layouts, flattened in Fortran order.  There is no CPU
+fallback: when the library is missing this module raises on first use.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librbc3d_gpu.so")
+
+dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+STATUS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "neighbour-cell list overflow (>32)",
+          4: "call out of order", 5: "non-finite input", 6: "GMRES did not converge"}
+
+
+class Rbc3dError(RuntimeError):
+    def __init__(self, fn, code, msg):
+        super().__init__(f"{fn}: status {code} ({STATUS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class Conf(C.Structure):
+    _fields_ = [("Lb", C.c_double * 3), ("alpha", C.c_double), ("rc", C.c_double),
+                ("Nb", C.c_int * 3), ("P", C.c_int), ("ntypes", C.c_int),
+                ("viscRat", C.c_double * 8), ("vBkg", C.c_double * 3)]
+
+
+_SIG = {
+    "rbc3d_create": [C.POINTER(Conf), C.POINTER(C.c_void_p)],
+    "rbc3d_destroy": [C.c_void_p],
+    "rbc3d_set_cells": [C.c_void_p, C.c_int, ip, ip, ip, dp, dp, dp],
+    "rbc3d_set_cell_density": [C.c_void_p, C.c_int, dp],
+    "rbc3d_set_wall": [C.c_void_p, C.c_int, C.c_int, dp, ip],
+    "rbc3d_set_wall_density": [C.c_void_p, dp],
+    "rbc3d_prepare_sing_int_on_wall": [C.c_void_p],
+    "rbc3d_wall_self_nnz": [C.c_void_p, C.POINTER(C.c_longlong)],
+    "rbc3d_hashtable_build": [ip, dp, C.c_int, dp, C.c_int, ip, ip],
+    "rbc3d_add_int_on_rbcs": [C.c_void_p, C.c_double, C.c_double, C.c_int, dp, ip, ip, dp, dp],
+    "rbc3d_add_int_on_walls": [C.c_void_p, C.c_double, C.c_int, dp, ip, ip, dp, dp],
+    "rbc3d_pair_stats": [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_ulonglong)],
+    "rbc3d_pme_distrib_source": [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int],
+    "rbc3d_pme_transform": [C.c_void_p],
+    "rbc3d_pme_add_interp_vel": [C.c_void_p, C.c_int, dp, ip, dp, dp],
+    "rbc3d_pme_grid": [C.c_void_p, C.c_int, dp],
+    "rbc3d_sh_anal_gau": [C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, dp, C.c_int, C.c_int],
+    "rbc3d_sh_synth_gau": [C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, dp, C.c_int, C.c_int],
+    "rbc3d_sh_synth_equ": [C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, dp, C.c_int, C.c_int],
+    "rbc3d_glob_sph_trans": [C.c_void_p, dp, dp, C.c_int],
+    "rbc3d_cell_spline": [C.c_void_p, C.c_int, C.c_int, dp, dp, dp, dp],
+    "rbc3d_calc_velocity_field": [C.c_void_p, C.c_int, dp, ip, ip, dp, dp],
+    "rbc3d_cell_dof": [C.c_void_p, C.POINTER(C.c_int)],
+    "rbc3d_cell_matmult": [C.c_void_p, dp, dp],
+    "rbc3d_cell_matmult_dev": [C.c_void_p, C.c_void_p, C.c_void_p],
+    "rbc3d_cell_rhs": [C.c_void_p, dp],
+    "rbc3d_wall_dof": [C.c_void_p, C.POINTER(C.c_int)],
+    "rbc3d_wall_matmult": [C.c_void_p, dp, dp],
+    "rbc3d_wall_residual_vel": [C.c_void_p, dp],
+    "rbc3d_stream": [C.c_void_p, C.POINTER(C.c_void_p)],
+    "rbc3d_sync": [C.c_void_p],
+}
+
+_lib = None
+
+
+def lib():
+    """Load librbc3d_gpu.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is missing: build it with rbc3d_amd/build.py "
+                              "(there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, args in _SIG.items():
+            fn = getattr(L, name)
+            fn.restype = C.c_int
+            fn.argtypes = args
+        L.rbc3d_last_error.restype = C.c_char_p
+        L.rbc3d_last_error.argtypes = []
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return list(_SIG) + ["rbc3d_last_error"]
+
+
+def _check(name, r):
+    if r != 0:
+        raise Rbc3dError(name, r, lib().rbc3d_last_error().decode())
+
+
+def _call(name, *args):
+    _check(name, getattr(lib(), name)(*args))
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def hashtable_build(Nc, iLbNc, x, nPoint=None):
+    """HashTable_Build (common/ModHashTable.F90:23-58): x is (ldx, 3) Fortran-flat."""
+    x = _f64(x)
+    ldx = len(x) // 3
+    n = ldx if nPoint is None else nPoint
+    Nc = _i32(Nc)
+    hoc = np.zeros((Nc[0] + 2) * (Nc[1] + 2) * (Nc[2] + 2), np.int32)
+    nxt = np.zeros(max(n, 1), np.int32)
+    _call("rbc3d_hashtable_build", Nc, _f64(iLbNc), n, x, ldx, hoc, nxt)
+    return hoc, nxt[:n]
+
+
+def sh_anal_gau(nlat, nlon, g, nt=1, ida=None, jda=None):
+    """ShAnalGau (common/ModSphpk.F90:74-105): g(nlat, nlon, nt) Fortran-flat -> a, b."""
+    ida = ida or nlat
+    jda = jda or nlat
+    a = np.zeros(ida * jda * nt)
+    b = np.zeros(ida * jda * nt)
+    _call("rbc3d_sh_anal_gau", nlat, nlon, nt, _f64(g), nlat, nlon, a, b, ida, jda)
+    return a, b
+
+
+def sh_synth_gau(nlat, nlon, a, b, nt=1, ida=None, jda=None):
+    ida = ida or nlat
+    jda = jda or nlat
+    g = np.zeros(nlat * nlon * nt)
+    _call("rbc3d_sh_synth_gau", nlat, nlon, nt, g, nlat, nlon, _f64(a), _f64(b), ida, jda)
+    return g
+
+
+def sh_synth_equ(nlat, nlon, a, b, nt=1, ida=None, jda=None):
+    ida = ida or nlat
+    jda = jda or nlat
+    g = np.zeros(nlat * nlon * nt)
+    _call("rbc3d_sh_synth_equ", nlat, nlon, nt, g, nlat, nlon, _f64(a), _f64(b), ida, jda)
+    return g
+
+
+class System:
+    """The reference's module state (ModConf + ModData) on one GPU: parameters, cells,
+    an optional wall, and the PME grids."""
+
+    def __init__(self, Lb, alpha, rc, Nb, P, viscRat, vBkg=(0.0, 0.0, 0.0)):
+        c = Conf()
+        c.Lb[:] = list(Lb)
+        c.alpha = alpha
+        c.rc = rc
+        c.Nb[:] = list(Nb)
+        c.P = P
+        c.ntypes = len(viscRat)
+        for i, v in enumerate(viscRat):
+            c.viscRat[i] = v
+        c.vBkg[:] = list(vBkg)
+        self.conf = c
+        h = C.c_void_p()
+        _call("rbc3d_create", C.byref(c), C.byref(h))
+        self.h = h
+        self.np = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().rbc3d_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- module data ---------------------------------------------------------
+    def set_cells(self, cells):
+        _call("rbc3d_set_cells", self.h, cells.ncell, _i32(cells.nlat0), _i32(cells.nlat),
+              _i32(cells.celltype), _f64(cells.x), _f64(cells.a3), _f64(cells.detj))
+        self.np = cells.npoint
+        self.cells = cells
+
+    def set_cell_density(self, which, dens):
+        _call("rbc3d_set_cell_density", self.h, which, _f64(dens))
+
+    def set_wall(self, x, e2v, nvert, nele):
+        _call("rbc3d_set_wall", self.h, nvert, nele, _f64(x), _i32(e2v))
+        self.nvert = nvert
+
+    def set_wall_density(self, f):
+        _call("rbc3d_set_wall_density", self.h, _f64(f))
+
+    def prepare_sing_int_on_wall(self):
+        _call("rbc3d_prepare_sing_int_on_wall", self.h)
+
+    # --- target lists (TargetList_Create/Update, common/ModTargetList.F90) ----
+    def tlist_rbc(self):
+        cells = self.cells
+        xs, idx, Ac = [], [], []
+        Acoef = [1.0 + v if v > 0 else 1.0 for v in self.conf.viscRat[: self.conf.ntypes]]
+        off = 0
+        n = cells.npoint
+        tx = np.zeros(3 * n)
+        ti = np.zeros(3 * n, np.int32)
+        A = np.zeros(n)
+        p = 0
+        for i in range(cells.ncell):
+            nl = int(cells.nlat[i])
+            nn = 2 * nl
+            npt = nl * nn
+            blk = cells.x[3 * off: 3 * (off + npt)].reshape(3, npt)
+            tx[p:p + npt] = blk[0]
+            tx[n + p:n + p + npt] = blk[1]
+            tx[2 * n + p:2 * n + p + npt] = blk[2]
+            q = np.arange(npt)
+            ti[p:p + npt] = i + 1
+            ti[n + p:n + p + npt] = q % nl + 1
+            ti[2 * n + p:2 * n + p + npt] = q // nl + 1
+            A[p:p + npt] = Acoef[cells.celltype[i] - 1]
+            p += npt
+            off += npt
+        return tx, ti, np.ones(n, np.int32), A
+
+    # --- hot path --------------------------------------------------------------
+    def add_int_on_rbcs(self, c1, c2, tl, v=None):
+        tx, ti, act, A = tl
+        n = len(A)
+        v = np.zeros(3 * n) if v is None else _f64(v)
+        _call("rbc3d_add_int_on_rbcs", self.h, c1, c2, n, _f64(tx), _i32(ti), _i32(act), _f64(A), v)
+        return v
+
+    def pair_stats(self):
+        n = C.c_longlong()
+        s = C.c_ulonglong()
+        _call("rbc3d_pair_stats", self.h, C.byref(n), C.byref(s))
+        return n.value, s.value
+
+    def add_int_on_walls(self, c1, tl, v=None):
+        tx, ti, act, A = tl
+        n = len(A)
+        v = np.zeros(3 * n) if v is None else _f64(v)
+        _call("rbc3d_add_int_on_walls", self.h, c1, n, _f64(tx), _i32(ti), _i32(act), _f64(A), v)
+        return v
+
+    def pme_distrib_source(self, c1, c2, cells=True, walls=True):
+        _call("rbc3d_pme_distrib_source", self.h, c1, c2, int(cells), int(walls))
+
+    def pme_transform(self):
+        _call("rbc3d_pme_transform", self.h)
+
+    def pme_add_interp_vel(self, tl, v=None):
+        tx, ti, act, A = tl
+        n = len(A)
+        v = np.zeros(3 * n) if v is None else _f64(v)
+        _call("rbc3d_pme_add_interp_vel", self.h, n, _f64(tx), _i32(act), _f64(A), v)
+        return v
+
+    def pme(self, c1, c2, tl, cells=True, walls=True, v=None):
+        self.pme_distrib_source(c1, c2, cells, walls)
+        self.pme_transform()
+        return self.pme_add_interp_vel(tl, v)
+
+    def pme_grid(self, which):
+        N = int(np.prod(list(self.conf.Nb)))
+        out = np.zeros((9 if which == 1 else 3) * N)
+        _call("rbc3d_pme_grid", self.h, which, out)
+        return out
+
+    def calc_velocity_field(self, tl):
+        tx, ti, act, A = tl
+        n = len(A)
+        v = np.zeros(3 * n)
+        _call("rbc3d_calc_velocity_field", self.h, n, _f64(tx), _i32(ti), _i32(act), _f64(A), v)
+        return v
+
+    def cell_dof(self):
+        d = C.c_int()
+        _call("rbc3d_cell_dof", self.h, C.byref(d))
+        return d.value
+
+    def glob_sph_trans(self, v=None, c=None, direction=1):
+        if direction == 1:
+            c = np.zeros(self.cell_dof())
+            _call("rbc3d_glob_sph_trans", self.h, _f64(v), c, 1)
+            return c
+        v = np.zeros(3 * self.np)
+        _call("rbc3d_glob_sph_trans", self.h, v, _f64(c), -1)
+        return v
+
+    def cell_matmult(self, u):
+        b = np.zeros(self.cell_dof())
+        _call("rbc3d_cell_matmult", self.h, _f64(u), b)
+        return b
+
+    def cell_matmult_dev(self, d_u, d_b):
+        _call("rbc3d_cell_matmult_dev", self.h, C.c_void_p(d_u), C.c_void_p(d_b))
+
+    def cell_rhs(self):
+        b = np.zeros(self.cell_dof())
+        _call("rbc3d_cell_rhs", self.h, b)
+        return b
+
+    def cell_spline(self, irbc, which, m, n, nvar):
+        u, u1, u2, u12 = (np.zeros(m * n * nvar) for _ in range(4))
+        _call("rbc3d_cell_spline", self.h, irbc, which, u, u1, u2, u12)
+        return u, u1, u2, u12
+
+    def wall_dof(self):
+        d = C.c_int()
+        _call("rbc3d_wall_dof", self.h, C.byref(d))
+        return d.value
+
+    def wall_matmult(self, u):
+        b = np.zeros(self.wall_dof())
+        _call("rbc3d_wall_matmult", self.h, _f64(u), b)
+        return b
+
+    def wall_residual_vel(self):
+        v = np.zeros(3 * self.nvert)
+        _call("rbc3d_wall_residual_vel", self.h, v)
+        return v
+
+    def stream(self):
+        s = C.c_void_p()
+        _call("rbc3d_stream", self.h, C.byref(s))
+        return s.value
+
+    def sync(self):
+        _call("rbc3d_sync", self.h)
+
+
+def from_workload(w):
+    s = System(**w.conf_kwargs())
+    s.set_cells(w.cells)
+    s.set_cell_density(1, w.cells.f)
+    s.set_cell_density(2, w.cells.g)
+    if w.wall is not None:
+        x, e2v, nv, ne = w.wall
+        s.set_wall(x, e2v, nv, ne)
+    return s

@@ -1,0 +1,292 @@
+// Internal declarations of the gfx950 implementation of RBC3D's Ewald-split
+// boundary-integral matvec.  Everything here is FP64 ("plain" form).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/rbc3d.h"
+
+#define RBC_NTAB 8192
+#define RBC_NBR_MAX 32
+#define RBC_PI 3.14159265358979323846
+#define RBC_TWO_PI (2 * RBC_PI)
+
+void rbc_set_error(const std::string &msg);
+
+#define HIP_TRY(expr)                                                                    \
+  do {                                                                                   \
+    hipError_t _e = (expr);                                                              \
+    if (_e != hipSuccess) {                                                              \
+      rbc_set_error(std::string(#expr) + ": " + hipGetErrorString(_e));                   \
+      return RBC3D_EHIP;                                                                 \
+    }                                                                                    \
+  } while (0)
+
+#define RBC_TRY(expr)            \
+  do {                           \
+    int _r = (expr);             \
+    if (_r != RBC3D_OK) return _r; \
+  } while (0)
+
+// Device buffer with explicit lifetime (hipMalloc / hipFree); grows on demand.
+template <typename T>
+struct DBuf {
+  T *p = nullptr;
+  size_t n = 0;
+  int ensure(size_t count) {
+    if (count <= n && p) return RBC3D_OK;
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+    if (count == 0) return RBC3D_OK;
+    HIP_TRY(hipMalloc(&p, count * sizeof(T)));
+    n = count;
+    return RBC3D_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+// Tables shared by every cell with the same grid (nlat, nlon = 2 nlat).
+struct GridType {
+  int nlat, nlon, mmax;  // mmax = min(nlat, nlon/2 + 1) (SPHEREPACK Gauss grids)
+  double radius;         // polar patch radius pi/sqrt(nlat)
+  int nrad, nazm;
+  std::vector<double> th, phi, w;  // w includes 2 pi / nlon (RBC_Create)
+  DBuf<double> d_th, d_phi, d_w;
+  DBuf<double> d_pw;     // analysis  [m][n][i] = (2/nlon) wgauss_i Pbar(m,n,th_i)
+  DBuf<double> d_pg;     // synthesis [m][n][i] = Pbar(m,n,th_i) (x 1/2 for m = 0)
+  DBuf<double> d_pe;     // equi synthesis, nlat+1 rows incl. poles, [m][n][ie]
+  DBuf<double> d_cs;     // cos(m phi_j) [m][j] then sin(m phi_j) [m][j], m <= nlat
+  DBuf<double> d_Dm;     // periodic derivative matrix, length 2 nlat, [p][q]
+  DBuf<double> d_Dn;     // periodic derivative matrix, length nlon, [p][q]
+  DBuf<double> d_thG, d_phiG, d_pwt;  // polar patch: [node][irad + nrad*iazm], weights
+  DBuf<double> d_mask;   // same-surface mask [ilat_i][ilat_j][dlon]
+};
+
+// Per-cell record (device array of these).
+struct CellRec {
+  int off;      // first node in the concatenated node arrays
+  int gtype;    // GridType index
+  int nlat, nlon, nlat0;
+  int celltype;  // 1-based
+  int coff;      // offset of the cell's block in the Glob_Sph_Trans vector
+  long long spl;  // offset (doubles) of the cell's spline block
+  double Acoef, Bcoef, area, meshSize, radius;
+};
+
+// Spline block of a cell: 13 variables (x:3, a3:3, detJ:1, f detJ:3, g detJ:3), each
+// (2 nlat) x nlon points of 4 doubles (u, u1, u2, u12), point (i,j) at i + m*j.
+enum { SPL_X = 0, SPL_A3 = 3, SPL_DETJ = 6, SPL_F = 7, SPL_G = 10, SPL_NVAR = 13 };
+
+struct WallData {
+  int nvert = 0, nele = 0, id = 0, ndof = 0;
+  std::vector<double> x;
+  std::vector<int> e2v, v2v, glb;
+  DBuf<double> d_x, d_f, d_area, d_eps, d_cen;
+  DBuf<int> d_e2v, d_glb;
+  // cell list of wall sources (triangle centroids)
+  DBuf<int> d_bin_start, d_order;
+  // self matrix CSR (rows i + k nvert)
+  DBuf<int> d_rowptr, d_col;
+  DBuf<double> d_val;
+  long long nnz = 0;
+  bool prepared = false;
+};
+
+struct rbc3d_ctx {
+  rbc3d_conf conf;
+  double Lb[3], iLb[3];
+  double Acoef[8], Bcoef[8];
+  hipStream_t stream = nullptr;
+  // tables
+  DBuf<double> d_sl1, d_sl2, d_dl1, d_mtab;
+  double r_eps;
+  // cell list geometry
+  int Nc[3];
+  double iLbNc[3];
+  int nbins;
+  // cells
+  int nrbc = 0, np = 0;
+  std::vector<GridType *> gtypes;
+  std::vector<CellRec> cells;
+  DBuf<CellRec> d_cells;
+  DBuf<double> d_x, d_a3, d_detj, d_ds, d_f, d_g, d_fs, d_gs;  // node arrays, [d][np]
+  DBuf<int> d_node_cell, d_node_ll;  // cell index; ilat | ilon << 16
+  DBuf<double> d_spl;
+  long long spl_total = 0;
+  // source cell list (sorted by bin, ascending index inside a bin)
+  DBuf<int> d_src_bin, d_src_order, d_bin_start, d_tmp_i;
+  DBuf<double> d_src_rec;  // sorted source records
+  DBuf<unsigned char> d_cub_tmp;
+  // target scratch
+  DBuf<double> d_tx, d_tA, d_v, d_vacc;
+  DBuf<int> d_tindx, d_tact, d_tbin, d_torder, d_tbin_start, d_chunk_off;
+  DBuf<int> d_nbr_cnt, d_nbr_cell, d_nbr_node;
+  DBuf<double> d_nbr_dist;
+  DBuf<int> d_work, d_work_cnt;
+  DBuf<double> d_dvitem, d_scratch;
+  DBuf<int> d_flag;
+  DBuf<unsigned long long> d_pairstat;
+  // cell bounding spheres binned by centre (neighbour-cell candidates)
+  DBuf<double> d_cell_sph;
+  DBuf<int> d_cellbin_start, d_cellbin_order;
+  double max_cell_radius = 0;
+  // PME
+  bool pme_ready = false;
+  int flag_sl = 0, flag_dl = 0;
+  DBuf<double> d_grid;   // real grids: up to 12 fields (ff 3 | tt 9), later vv 3
+  DBuf<double> d_cgrid;  // complex grids (interleaved) up to 12 fields
+  DBuf<double> d_vgrid;  // velocity grid 3 fields
+  DBuf<double> d_bb;     // B-factor (kx,ky,kz)
+  DBuf<double> d_tw;     // FFT twiddles for the three lengths
+  int tw_off[3];
+  DBuf<int> d_pme_key, d_pme_order, d_pme_start;
+  DBuf<double> d_pme_src;  // sorted spread records
+  // SH scratch
+  DBuf<double> d_shbuf, d_shbuf2;
+  // wall
+  WallData wall;
+  // per-grid-type table pointers for the kernels
+  DBuf<const double *> d_tp_ptr;
+  DBuf<int> d_tp_int;
+  size_t tp_ntypes = 0;
+  // last pair statistics
+  long long last_npairs = 0;
+  unsigned long long last_checksum = 0;
+};
+
+// ---------------------------------------------------------------------------
+// Device helpers shared by several kernels.
+
+// Reference distance, evaluated without FMA contraction so the in-cutoff test
+// matches the reference bit for bit: xx = xj - xi; xx -= nint(xx/Lb) Lb;
+// rr = sqrt((x1*x1 + x2*x2) + x3*x3)   (common/ModIntOnRbcs.F90:68-74)
+__device__ __forceinline__ double ref_min_image_dist(const double xi[3], const double xj[3],
+                                                     const double Lb[3], const double iLb[3],
+                                                     double xx[3]) {
+#pragma clang fp contract(off)
+  for (int d = 0; d < 3; d++) {
+    double t = xj[d] - xi[d];
+    double sh = round(t * iLb[d]) * Lb[d];
+    xx[d] = t - sh;
+  }
+  double s = xx[0] * xx[0] + xx[1] * xx[1];
+  s = s + xx[2] * xx[2];
+  return sqrt(s);
+}
+
+__device__ __forceinline__ int imod_d(int a, int n) {
+  int r = a % n;
+  return r < 0 ? r + n : r;
+}
+
+// EwaldCoeff_SL / EwaldCoeff_DL (common/ModEwaldFunc.F90:86-178) by table interpolation.
+__device__ __forceinline__ void ewald_sl(const double *__restrict__ t1, const double *__restrict__ t2,
+                                         double r, double reps, double rc, double &A, double &B) {
+  if (r < reps) {
+    A = 0;
+    B = 0;
+    return;
+  }
+  double s = RBC_NTAB * r / rc;
+  double fi = floor(s);
+  if (fi >= RBC_NTAB) {
+    A = 0;
+    B = 0;
+    return;
+  }
+  int i = (int)fi;
+  double c1 = t1[i] * (i + 1 - s) + t1[i + 1] * (s - i);
+  double c2 = t2[i] * (i + 1 - s) + t2[i + 1] * (s - i);
+  double ir = 1. / r, ir2 = ir * ir;
+  A = c1 * ir * ir2 + c2 * ir2;
+  B = c1 * ir - c2;
+}
+
+__device__ __forceinline__ double ewald_dl(const double *__restrict__ t, double r, double reps,
+                                           double rc) {
+  if (r < reps) return 0.;
+  double s = RBC_NTAB * r / rc;
+  double fi = floor(s);
+  if (fi >= RBC_NTAB) return 0.;
+  int i = (int)fi;
+  double c1 = t[i] * (i + 1 - s) + t[i + 1] * (s - i);
+  double r2 = r * r;
+  return c1 / (r2 * r2 * r);
+}
+
+// MaskFunc (common/ModBasicMath.F90:351-379)
+__device__ __forceinline__ double mask_fn(const double *__restrict__ mt, double x) {
+  double s = fabs(x) * RBC_NTAB;
+  double fi = floor(s);
+  if (fi >= RBC_NTAB) return 0.;
+  int i = (int)fi;
+  return mt[i] * (i + 1 - s) + mt[i + 1] * (s - i);
+}
+
+// Spline_Interp (common/ModSpline.F90:150-191) on a cell's spline block.
+// blk points at variable v0; nvar consecutive variables are interpolated.
+__device__ __forceinline__ void spline_interp(const double *__restrict__ blk, int m, int n,
+                                              double x, double y, int nvar, double *f) {
+  double hx = RBC_TWO_PI / m, hy = RBC_TWO_PI / n;
+  double ihx = 1. / hx, ihy = 1. / hy;
+  double fi1 = floor(x * ihx), fj1 = floor(y * ihy);
+  double s = x * ihx - fi1, t = y * ihy - fj1;
+  int i1 = imod_d((int)fi1, m), j1 = imod_d((int)fj1, n);
+  int i2 = i1 + 1 == m ? 0 : i1 + 1, j2 = j1 + 1 == n ? 0 : j1 + 1;
+  double cx0 = 1 + s * s * (-3. + 2. * s), cx1 = s * s * (3. - 2. * s);
+  double cx2 = hx * s * (1. + s * (-2. + s)), cx3 = hx * s * s * (-1. + s);
+  double cy0 = 1 + t * t * (-3. + 2. * t), cy1 = t * t * (3. - 2. * t);
+  double cy2 = hy * t * (1. + t * (-2. + t)), cy3 = hy * t * t * (-1. + t);
+  size_t stride = (size_t)m * n * 4;
+  size_t p11 = 4 * (size_t)(i1 + m * j1), p12 = 4 * (size_t)(i1 + m * j2);
+  size_t p21 = 4 * (size_t)(i2 + m * j1), p22 = 4 * (size_t)(i2 + m * j2);
+  for (int l = 0; l < nvar; l++) {
+    const double *b = blk + l * stride;
+    double4 a11 = *(const double4 *)(b + p11), a12 = *(const double4 *)(b + p12);
+    double4 a21 = *(const double4 *)(b + p21), a22 = *(const double4 *)(b + p22);
+    // rows of u(4,4): (u, u, u2, u2)_(i1), (.)_(i2), (u1, u1, u12, u12)_(i1), (.)_(i2)
+    double r0 = a11.x * cy0 + a12.x * cy1 + a11.z * cy2 + a12.z * cy3;
+    double r1 = a21.x * cy0 + a22.x * cy1 + a21.z * cy2 + a22.z * cy3;
+    double r2 = a11.y * cy0 + a12.y * cy1 + a11.w * cy2 + a12.w * cy3;
+    double r3 = a21.y * cy0 + a22.y * cy1 + a21.w * cy2 + a22.w * cy3;
+    f[l] = cx0 * r0 + cx1 * r1 + cx2 * r2 + cx3 * r3;
+  }
+}
+
+// DistOnSphere (common/ModPolarPatch.F90:249-257)
+__device__ __forceinline__ double dist_on_sphere(double th0, double phi0, double th1, double phi1) {
+  double d = cos(th0 - th1) - sin(th0) * sin(th1) * (1. - cos(phi0 - phi1));
+  d = fmin(1., fmax(-1., d));
+  return acos(d);
+}
+
+// Kernel launchers (defined in the .hip files)
+int rbc_sh_build_splines(rbc3d_ctx *c, int which);
+int rbc_sh_glob(rbc3d_ctx *c, const double *d_in, double *d_out, int direction);
+int rbc_sh_anal_batch(int nlat, int nlon, int nt, const double *d_g, int idg, int jdg, double *d_a,
+                      double *d_b, int ida, int jda, hipStream_t st);
+int rbc_sh_synth_batch(int nlat, int nlon, int nt, double *d_g, int idg, int jdg, const double *d_a,
+                       const double *d_b, int ida, int jda, int equi, hipStream_t st);
+int rbc_build_source_list(rbc3d_ctx *c);
+int rbc_update_density(rbc3d_ctx *c, int which);
+int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const double *d_tx,
+                            const int *d_tindx, const int *d_act, const double *d_A, double *d_v);
+int rbc_pme_init(rbc3d_ctx *c);
+int rbc_pme_spread(rbc3d_ctx *c, double c1, double c2, int cells, int walls);
+int rbc_pme_transform(rbc3d_ctx *c);
+int rbc_pme_gather(rbc3d_ctx *c, int nT, const double *d_tx, const int *d_act, const double *d_A,
+                   double *d_v);
+int rbc_wall_setup(rbc3d_ctx *c);
+int rbc_wall_prepare(rbc3d_ctx *c);
+int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double c1, int nT, const double *d_tx, const int *d_tindx,
+                             const int *d_act, const double *d_A, double *d_v);
+int rbc_hash_sort(rbc3d_ctx *c, int n, const double *d_x, int ldx, int *d_bin, int *d_order,
+                  int *d_bin_start);

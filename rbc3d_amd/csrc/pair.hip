@@ -1,0 +1,1287 @@
+// Real-space part of the Ewald-split matvec on cell surfaces:
+//   cell list (HashTable_Build, common/ModHashTable.F90:23-58),
+//   pair sums with same-surface masking (AddIntOnRbcs, common/ModIntOnRbcs.F90:47-112),
+//   neighbouring-cell lists (NbrRbcList_Insert, common/ModRbcSingInt.F90:341-364),
+//   self-patch singular quadrature (RBC_SingInt, common/ModRbcSingInt.F90:29-90),
+//   near-singular corrections (RBC_NearSingInt and helpers, :103-310,
+//     Spline_FindProjection common/ModSpline.F90:203-269),
+//   the periodic linear double-layer term (AddLinearInt, common/ModIntOnRbcs.F90:162-201).
+//
+// Layout: sources are sorted by cell-list bin (stable, so ascending index inside a
+// bin); targets are binned the same way.  The pair kernel gives one workgroup to a
+// chunk of up to 256 targets of one bin and streams the sources of the 27
+// neighbouring bins through LDS tiles.  Each lane owns one target, accumulates in
+// registers and writes once: no atomics, a fixed summation order per target.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+
+#include "internal.h"
+
+#define PAIR_T 256
+#define NBR_T 128
+
+// ---------------------------------------------------------------------------
+// cell list
+__device__ __forceinline__ void hash_index3(const double x[3], const int Nc[3], const double iLbNc[3],
+                                            int &i1, int &i2, int &i3) {
+  // HashTable_Index (common/ModHashTable.F90:67-91), one rank; 1-based bins
+  i1 = imod_d((int)floor(x[0] * iLbNc[0]), Nc[0]) + 1;
+  i2 = imod_d((int)floor(x[1] * iLbNc[1]), Nc[1]) + 1;
+  i3 = imod_d((int)floor(x[2] * iLbNc[2]), Nc[2]) + 1;
+}
+
+struct HashGeom {
+  int Nc[3];
+  double iLbNc[3];
+};
+
+__global__ void k_bins(int n, const double *__restrict__ x, int ldx, HashGeom hg, int *bin, int *idx) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double xi[3] = {x[i], x[i + ldx], x[i + 2 * (size_t)ldx]};
+  int i1, i2, i3;
+  hash_index3(xi, hg.Nc, hg.iLbNc, i1, i2, i3);
+  bin[i] = (i1 - 1) + hg.Nc[0] * ((i2 - 1) + hg.Nc[1] * (i3 - 1));
+  idx[i] = i;
+}
+
+// bin_start[b] = first sorted position with bin >= b (sorted keys), b = 0..nbins
+__global__ void k_bin_start(int n, const int *__restrict__ skey, int nbins, int *bin_start) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nbins) return;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (skey[mid] < b)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  bin_start[b] = lo;
+}
+
+// Sort n points by bin (stable: ascending index inside a bin).  d_bin receives the
+// bin of each point in ORIGINAL order, d_order the sorted point indices.
+int rbc_hash_sort(rbc3d_ctx *c, int n, const double *d_x, int ldx, int *d_bin, int *d_order,
+                  int *d_bin_start) {
+  HashGeom hg;
+  for (int d = 0; d < 3; d++) {
+    hg.Nc[d] = c->Nc[d];
+    hg.iLbNc[d] = c->iLbNc[d];
+  }
+  hipStream_t st = c->stream;
+  DBuf<int> keys_out, idx_in;
+  RBC_TRY(keys_out.ensure(n + 1));
+  RBC_TRY(idx_in.ensure(n + 1));
+  if (n > 0) {
+    k_bins<<<(n + 255) / 256, 256, 0, st>>>(n, d_x, ldx, hg, d_bin, idx_in.p);
+    HIP_TRY(hipGetLastError());
+    int bits = 1;
+    while ((1 << bits) < c->nbins) bits++;
+    size_t tmp = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, d_bin, keys_out.p, idx_in.p, d_order, n,
+                                               0, bits, st));
+    RBC_TRY(c->d_cub_tmp.ensure(tmp + 16));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->d_cub_tmp.p, tmp, d_bin, keys_out.p, idx_in.p,
+                                               d_order, n, 0, bits, st));
+  }
+  k_bin_start<<<(c->nbins + 1 + 255) / 256, 256, 0, st>>>(n, keys_out.p, c->nbins, d_bin_start);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  keys_out.release();
+  idx_in.release();
+  return RBC3D_OK;
+}
+
+// hoc/next from the sorted list, with the ghost layers of HashTable_Build
+__global__ void k_hoc_next(int nbins, HashGeom hg, const int *__restrict__ bin_start,
+                           const int *__restrict__ order, int *hoc, int *next) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= nbins) return;
+  int b1 = b % hg.Nc[0], b2 = (b / hg.Nc[0]) % hg.Nc[1], b3 = b / (hg.Nc[0] * hg.Nc[1]);
+  int s = bin_start[b], e = bin_start[b + 1];
+  // push-front insertion in index order: head = largest index, next -> next smaller
+  int head = e > s ? order[e - 1] + 1 : -1;
+  for (int k = s; k < e; k++) next[order[k]] = k > s ? order[k - 1] + 1 : -1;
+  int n1 = hg.Nc[0] + 2, n2 = hg.Nc[1] + 2;
+  // write the real bin and every ghost image of it: position 0 holds bin Nc, position
+  // Nc+1 holds bin 1 (per axis, corners included, as the slab copies leave them)
+  int pos[3][3], npos[3];
+  int bb[3] = {b1, b2, b3};
+  for (int ax = 0; ax < 3; ax++) {
+    npos[ax] = 0;
+    pos[ax][npos[ax]++] = bb[ax] + 1;
+    if (bb[ax] + 1 == hg.Nc[ax]) pos[ax][npos[ax]++] = 0;
+    if (bb[ax] == 0) pos[ax][npos[ax]++] = hg.Nc[ax] + 1;
+  }
+  for (int a3 = 0; a3 < npos[2]; a3++)
+    for (int a2 = 0; a2 < npos[1]; a2++)
+      for (int a1 = 0; a1 < npos[0]; a1++) hoc[pos[0][a1] + n1 * (pos[1][a2] + n2 * pos[2][a3])] = head;
+}
+
+__global__ void k_fill_i(int *p, int n, int v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+extern "C" int rbc3d_hashtable_build(const int Nc[3], const double iLbNc[3], int nPoint,
+                                     const double *x, int ldx, int *hoc, int *next) {
+  if (!Nc || !iLbNc || nPoint < 0 || (nPoint > 0 && (!x || ldx < nPoint)) || !hoc || !next)
+    return RBC3D_EINVAL;
+  for (int d = 0; d < 3; d++)
+    if (Nc[d] < 1) return RBC3D_EINVAL;
+  // the ghost-layer copies of HashTable_Build equal the periodic images only when
+  // Nc >= 2 in each direction; Nc = 1 stays valid (the ghosts alias the single bin)
+  rbc3d_ctx tmp;
+  for (int d = 0; d < 3; d++) {
+    tmp.Nc[d] = Nc[d];
+    tmp.iLbNc[d] = iLbNc[d];
+  }
+  tmp.nbins = Nc[0] * Nc[1] * Nc[2];
+  HIP_TRY(hipStreamCreateWithFlags(&tmp.stream, hipStreamNonBlocking));
+  DBuf<double> dx;
+  DBuf<int> bin, order, start, dhoc, dnext;
+  int n1 = Nc[0] + 2, n2 = Nc[1] + 2, n3 = Nc[2] + 2, nh = n1 * n2 * n3;
+  int r = RBC3D_OK;
+  if (!r) r = dx.ensure((size_t)3 * nPoint + 1);
+  if (!r) r = bin.ensure(nPoint + 1);
+  if (!r) r = order.ensure(nPoint + 1);
+  if (!r) r = start.ensure(tmp.nbins + 1);
+  if (!r) r = dhoc.ensure(nh);
+  if (!r) r = dnext.ensure(nPoint + 1);
+  if (!r && nPoint > 0)
+    for (int d = 0; d < 3 && !r; d++)
+      if (hipMemcpy(dx.p + (size_t)nPoint * d, x + (size_t)ldx * d, sizeof(double) * nPoint,
+                    hipMemcpyHostToDevice) != hipSuccess)
+        r = RBC3D_EHIP;
+  if (!r) r = rbc_hash_sort(&tmp, nPoint, dx.p, nPoint, bin.p, order.p, start.p);
+  if (!r) {
+    HashGeom hg;
+    for (int d = 0; d < 3; d++) {
+      hg.Nc[d] = Nc[d];
+      hg.iLbNc[d] = iLbNc[d];
+    }
+    k_fill_i<<<(nh + 255) / 256, 256, 0, tmp.stream>>>(dhoc.p, nh, -1);
+    k_hoc_next<<<(tmp.nbins + 255) / 256, 256, 0, tmp.stream>>>(tmp.nbins, hg, start.p, order.p,
+                                                                dhoc.p, dnext.p);
+    if (hipStreamSynchronize(tmp.stream) != hipSuccess) r = RBC3D_EHIP;
+  }
+  if (!r && hipMemcpy(hoc, dhoc.p, sizeof(int) * nh, hipMemcpyDeviceToHost) != hipSuccess) r = RBC3D_EHIP;
+  if (!r && nPoint > 0 &&
+      hipMemcpy(next, dnext.p, sizeof(int) * nPoint, hipMemcpyDeviceToHost) != hipSuccess)
+    r = RBC3D_EHIP;
+  dx.release();
+  bin.release();
+  order.release();
+  start.release();
+  dhoc.release();
+  dnext.release();
+  tmp.d_cub_tmp.release();
+  (void)hipStreamDestroy(tmp.stream);
+  tmp.stream = nullptr;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// source list on cells
+
+// sorted source records, SoA with stride np: x,y,z, f(3), g(3), a3(3), Bcoef  (13)
+#define SREC 13
+
+__global__ void k_src_records(int np, const int *__restrict__ order, const double *__restrict__ x,
+                              const double *__restrict__ fs, const double *__restrict__ gs,
+                              const double *__restrict__ a3, const CellRec *__restrict__ cells,
+                              const int *__restrict__ node_cell, double *__restrict__ rec,
+                              int *__restrict__ rec_meta) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= np) return;
+  int p = order[k];
+  for (int d = 0; d < 3; d++) {
+    rec[k + (size_t)np * d] = x[p + (size_t)np * d];
+    rec[k + (size_t)np * (3 + d)] = fs[p + (size_t)np * d];
+    rec[k + (size_t)np * (6 + d)] = gs[p + (size_t)np * d];
+    rec[k + (size_t)np * (9 + d)] = a3[p + (size_t)np * d];
+  }
+  rec[k + (size_t)np * 12] = cells[node_cell[p]].Bcoef;
+  rec_meta[k] = p;
+}
+
+// slist%f = f dS, slist%g = g dS (SourceList_UpdateDensity, common/ModSourceList.F90:160-187)
+__global__ void k_density(int np, const double *__restrict__ src, const double *__restrict__ ds,
+                          double *__restrict__ dst) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  double s = ds[p];
+  for (int d = 0; d < 3; d++) dst[p + (size_t)np * d] = src[p + (size_t)np * d] * s;
+}
+
+// bounding sphere of each cell (centroid of nodes, max node distance)
+__global__ void k_cell_sphere(int nrbc, const CellRec *__restrict__ cells, const double *__restrict__ x,
+                              int np, double *sph) {
+  int ci = blockIdx.x;
+  if (ci >= nrbc) return;
+  const CellRec cr = cells[ci];
+  int npt = cr.nlat * cr.nlon;
+  __shared__ double red[4][256];
+  double s[3] = {0, 0, 0};
+  for (int t = threadIdx.x; t < npt; t += blockDim.x)
+    for (int d = 0; d < 3; d++) s[d] += x[cr.off + t + (size_t)np * d];
+  for (int d = 0; d < 3; d++) red[d][threadIdx.x] = s[d];
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int d = 0; d < 3; d++) red[d][threadIdx.x] += red[d][threadIdx.x + w];
+    __syncthreads();
+  }
+  double cen[3] = {red[0][0] / npt, red[1][0] / npt, red[2][0] / npt};
+  __syncthreads();
+  double rmax = 0;
+  for (int t = threadIdx.x; t < npt; t += blockDim.x) {
+    double r2 = 0;
+    for (int d = 0; d < 3; d++) {
+      double dd = x[cr.off + t + (size_t)np * d] - cen[d];
+      r2 += dd * dd;
+    }
+    rmax = fmax(rmax, sqrt(r2));
+  }
+  red[3][threadIdx.x] = rmax;
+  __syncthreads();
+  for (int w = blockDim.x / 2; w > 0; w >>= 1) {
+    if (threadIdx.x < w) red[3][threadIdx.x] = fmax(red[3][threadIdx.x], red[3][threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    sph[4 * ci] = cen[0];
+    sph[4 * ci + 1] = cen[1];
+    sph[4 * ci + 2] = cen[2];
+    sph[4 * ci + 3] = red[3][0] * (1 + 1e-12) + 1e-12;
+  }
+}
+
+int rbc_build_source_list(rbc3d_ctx *c) {
+  int np = c->np;
+  RBC_TRY(c->d_src_bin.ensure(np + 1));
+  RBC_TRY(c->d_src_order.ensure(np + 1));
+  RBC_TRY(c->d_bin_start.ensure(c->nbins + 1));
+  RBC_TRY(rbc_hash_sort(c, np, c->d_x.p, np, c->d_src_bin.p, c->d_src_order.p, c->d_bin_start.p));
+  RBC_TRY(c->d_src_rec.ensure((size_t)SREC * np + 1));
+  RBC_TRY(c->d_cell_sph.ensure((size_t)4 * c->nrbc + 4));
+  k_cell_sphere<<<c->nrbc, 256, 0, c->stream>>>(c->nrbc, c->d_cells.p, c->d_x.p, np, c->d_cell_sph.p);
+  HIP_TRY(hipGetLastError());
+  return rbc_update_density(c, 0);
+}
+
+int rbc_update_density(rbc3d_ctx *c, int which) {
+  int np = c->np;
+  hipStream_t st = c->stream;
+  if (which == 1 || which == 2) {
+    RBC_TRY(rbc_sh_build_splines(c, which));
+    k_density<<<(np + 255) / 256, 256, 0, st>>>(np, which == 1 ? c->d_f.p : c->d_g.p, c->d_ds.p,
+                                                which == 1 ? c->d_fs.p : c->d_gs.p);
+    HIP_TRY(hipGetLastError());
+  }
+  RBC_TRY(c->d_work.ensure(np + 1));
+  k_src_records<<<(np + 255) / 256, 256, 0, st>>>(np, c->d_src_order.p, c->d_x.p, c->d_fs.p,
+                                                  c->d_gs.p, c->d_a3.p, c->d_cells.p,
+                                                  c->d_node_cell.p, c->d_src_rec.p, c->d_work.p);
+  HIP_TRY(hipGetLastError());
+  return RBC3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// pair kernel
+
+
+struct PairArgs {
+  double Lb[3], iLb[3];
+  int Nc[3];
+  double rc, reps, c1, c2;
+  int np, nT, nbins, nrbc;
+  const double *rec;     // sorted source records (SREC x np)
+  const int *src_node;   // sorted position -> node index
+  const int *bin_start;  // sources per bin
+  const int *node_cell, *node_ll;
+  const CellRec *cells;
+  const double *sl1, *sl2, *dl1;
+  const double *const *mask;  // per grid type same-surface mask tables
+  const double *tx, *tA;
+  const int *tindx, *tact, *torder, *tbin_start, *chunk_off;
+  double *vpair;
+  unsigned long long *stat;  // [0] in-cutoff pairs, [1] pair-set checksum
+};
+
+__global__ void k_chunks(int nbins, const int *__restrict__ tbin_start, int *chunk_cnt) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nbins) return;
+  chunk_cnt[b] = b < nbins ? (tbin_start[b + 1] - tbin_start[b] + PAIR_T - 1) / PAIR_T : 0;
+}
+
+// Pair sums of AddIntOnRbcs (common/ModIntOnRbcs.F90:47-112) for one chunk of targets
+// of one bin; the 27 neighbouring bins' sources stream through LDS.
+__global__ void __launch_bounds__(PAIR_T) k_pair(PairArgs a) {
+  __shared__ double sx[PAIR_T], sy[PAIR_T], sz[PAIR_T];
+  __shared__ double sf[3][PAIR_T], sg[3][PAIR_T], sn[3][PAIR_T], sB[PAIR_T];
+  __shared__ int snode[PAIR_T];
+  __shared__ int s_bin, s_chunk;
+  const int nbins = a.nbins;
+  if (threadIdx.x == 0) {
+    int blk = blockIdx.x;
+    if (blk >= a.chunk_off[nbins]) {
+      s_bin = -1;
+    } else {
+      int lo = 0, hi = nbins;  // chunk_off[lo] <= blk < chunk_off[hi]
+      while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (a.chunk_off[mid] <= blk)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      s_bin = lo;
+      s_chunk = blk - a.chunk_off[lo];
+    }
+  }
+  __syncthreads();
+  const int bin = s_bin;
+  if (bin < 0) return;
+  const int t0 = a.tbin_start[bin] + s_chunk * PAIR_T, t1 = a.tbin_start[bin + 1];
+  const int k = t0 + threadIdx.x;
+  const bool have = k < t1;
+  const int ti = have ? a.torder[k] : 0;
+  const bool act = have && a.tact[ti] != 0;
+  double xi[3] = {0, 0, 0}, Ai = 1.;
+  int surf_i = -1000, ilat_i = 0, ilon_i = 0, nl_i = 1, nn_i = 1;
+  const double *mtab = nullptr;
+  if (have) {
+    for (int d = 0; d < 3; d++) xi[d] = a.tx[ti + (size_t)a.nT * d];
+    Ai = a.tA[ti];
+    surf_i = a.tindx[ti];
+    if (surf_i >= 1 && surf_i <= a.nrbc) {
+      const CellRec &cr = a.cells[surf_i - 1];
+      nl_i = cr.nlat;
+      nn_i = cr.nlon;
+      mtab = a.mask[cr.gtype];
+      ilat_i = a.tindx[ti + a.nT] - 1;
+      ilon_i = a.tindx[ti + 2 * (size_t)a.nT] - 1;
+    }
+  }
+  const int cell_i = (surf_i >= 1 && surf_i <= a.nrbc) ? surf_i - 1 : -1;
+  double acc0 = 0, acc1 = 0, acc2 = 0;
+  unsigned long long npairs = 0, csum = 0;
+  const int b1 = bin % a.Nc[0], b2 = (bin / a.Nc[0]) % a.Nc[1], b3 = bin / (a.Nc[0] * a.Nc[1]);
+  const size_t np = a.np;
+  for (int nbi = 0; nbi < 27; nbi++) {
+    int o1 = nbi / 9 - 1, o2 = (nbi / 3) % 3 - 1, o3 = nbi % 3 - 1;
+    int n1 = imod_d(b1 + o1, a.Nc[0]), n2 = imod_d(b2 + o2, a.Nc[1]), n3 = imod_d(b3 + o3, a.Nc[2]);
+    int nb = n1 + a.Nc[0] * (n2 + a.Nc[1] * n3);
+    int s0 = a.bin_start[nb], s1 = a.bin_start[nb + 1];
+    for (int base = s0; base < s1; base += PAIR_T) {
+      int cnt = min(PAIR_T, s1 - base);
+      __syncthreads();
+      if (threadIdx.x < cnt) {
+        int q = base + threadIdx.x;
+        sx[threadIdx.x] = a.rec[q];
+        sy[threadIdx.x] = a.rec[q + np];
+        sz[threadIdx.x] = a.rec[q + 2 * np];
+        for (int d = 0; d < 3; d++) {
+          sf[d][threadIdx.x] = a.rec[q + np * (3 + d)];
+          sg[d][threadIdx.x] = a.rec[q + np * (6 + d)];
+          sn[d][threadIdx.x] = a.rec[q + np * (9 + d)];
+        }
+        sB[threadIdx.x] = a.rec[q + np * 12];
+        snode[threadIdx.x] = a.src_node[q];
+      }
+      __syncthreads();
+      if (!act) continue;
+      for (int j = 0; j < cnt; j++) {
+        double xj[3] = {sx[j], sy[j], sz[j]}, xx[3];
+        double rr = ref_min_image_dist(xi, xj, a.Lb, a.iLb, xx);
+        if (rr > a.rc) continue;
+        int node = snode[j];
+        double mask = 0.;
+        if (cell_i >= 0 && a.node_cell[node] == cell_i) {
+          int ll = a.node_ll[node];
+          int jl = ll & 0xffff, jn = ll >> 16;
+          int dl = jn - ilon_i;
+          dl += dl < 0 ? nn_i : 0;
+          mask = mtab[((size_t)ilat_i * nl_i + jl) * nn_i + dl];
+        }
+        npairs++;
+        csum += (unsigned long long)(ti + 1) * (unsigned long long)(node + 1);
+        if (a.c1 != 0) {
+          double EA, EB;
+          ewald_sl(a.sl1, a.sl2, rr, a.reps, a.rc, EA, EB);
+          double f0 = sf[0][j], f1 = sf[1][j], f2 = sf[2][j];
+          double xf = xx[0] * f0 + xx[1] * f1 + xx[2] * f2;
+          double cc = (1. - mask) * a.c1 / Ai;
+          acc0 += cc * (EA * xx[0] * xf + EB * f0);
+          acc1 += cc * (EA * xx[1] * xf + EB * f1);
+          acc2 += cc * (EA * xx[2] * xf + EB * f2);
+        }
+        if (a.c2 != 0) {
+          double EA = ewald_dl(a.dl1, rr, a.reps, a.rc);
+          double xg = xx[0] * sg[0][j] + xx[1] * sg[1][j] + xx[2] * sg[2][j];
+          double xn = xx[0] * sn[0][j] + xx[1] * sn[1][j] + xx[2] * sn[2][j];
+          double cc = (1. - mask) * a.c2 * sB[j] / Ai;
+          acc0 += cc * (EA * xx[0] * xg * xn);
+          acc1 += cc * (EA * xx[1] * xg * xn);
+          acc2 += cc * (EA * xx[2] * xg * xn);
+        }
+      }
+    }
+  }
+  if (act) {
+    a.vpair[ti] = acc0;
+    a.vpair[ti + (size_t)a.nT] = acc1;
+    a.vpair[ti + 2 * (size_t)a.nT] = acc2;
+    atomicAdd(&a.stat[0], npairs);
+    atomicAdd(&a.stat[1], csum);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// neighbouring-cell lists: for every target and every other cell with a node in the
+// cut-off, the closest node (ties: first visited, i.e. lowest 27-bin visit position,
+// then highest source index -- the reference's list order), NbrRbcList_Insert.
+struct NbrArgs {
+  double Lb[3], iLb[3], iLbNc[3];
+  int Nc[3];
+  double rc;
+  int np, nT, nrbc;
+  const double *x;  // node coordinates [d][np]
+  const int *src_bin;
+  const CellRec *cells;
+  const double *sph;
+  const double *tx;
+  const int *tindx, *tact;
+  int *cnt, *slot_cell, *slot_node;
+  double *slot_dist;
+  int *flag;
+};
+
+__global__ void __launch_bounds__(NBR_T) k_nbr(NbrArgs a) {
+  int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ti >= a.nT) return;
+  if (!a.tact[ti]) {
+    a.cnt[ti] = 0;
+    return;
+  }
+  double xi[3] = {a.tx[ti], a.tx[ti + a.nT], a.tx[ti + 2 * (size_t)a.nT]};
+  int surf_i = a.tindx[ti];
+  int t1, t2, t3;
+  hash_index3(xi, a.Nc, a.iLbNc, t1, t2, t3);
+  t1--;
+  t2--;
+  t3--;
+  int n = 0;
+  bool ovf = false;
+  size_t np = a.np;
+  for (int cj = 0; cj < a.nrbc; cj++) {
+    if (cj == surf_i - 1) continue;
+    double cen[3] = {a.sph[4 * cj], a.sph[4 * cj + 1], a.sph[4 * cj + 2]}, dd[3];
+    double rcen = ref_min_image_dist(xi, cen, a.Lb, a.iLb, dd);
+    if (rcen > a.rc + a.sph[4 * cj + 3] + 1e-9 * a.rc) continue;
+    const CellRec cr = a.cells[cj];
+    int npt = cr.nlat * cr.nlon;
+    double best = 0;
+    int bkey = 0, bnode = -1;
+    for (int q = 0; q < npt; q++) {
+      int node = cr.off + q;
+      double xj[3] = {a.x[node], a.x[node + np], a.x[node + 2 * np]}, xx[3];
+      double rr = ref_min_image_dist(xi, xj, a.Lb, a.iLb, xx);
+      if (rr > a.rc) continue;
+      int sb = a.src_bin[node];
+      int s1 = sb % a.Nc[0], s2 = (sb / a.Nc[0]) % a.Nc[1], s3 = sb / (a.Nc[0] * a.Nc[1]);
+      int d1 = imod_d(s1 - t1 + 1, a.Nc[0]), d2 = imod_d(s2 - t2 + 1, a.Nc[1]),
+          d3 = imod_d(s3 - t3 + 1, a.Nc[2]);
+      if (d1 > 2 || d2 > 2 || d3 > 2) continue;  // not in the 27 visited bins
+      int key = 9 * d1 + 3 * d2 + d3;
+      bool take = bnode < 0 || rr < best || (rr == best && (key < bkey || (key == bkey && node > bnode)));
+      if (take) {
+        best = rr;
+        bkey = key;
+        bnode = node;
+      }
+    }
+    if (bnode >= 0) {
+      if (n < RBC_NBR_MAX) {
+        size_t s = (size_t)ti * RBC_NBR_MAX + n;
+        a.slot_cell[s] = cj;
+        a.slot_node[s] = bnode;
+        a.slot_dist[s] = best;
+        n++;
+      } else {
+        ovf = true;
+      }
+    }
+  }
+  a.cnt[ti] = n;
+  if (ovf) atomicOr(a.flag, 1);
+}
+
+// ---------------------------------------------------------------------------
+// helpers for the singular quadratures (FP64 throughout)
+
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// PolarPatch_Build for a single local point (common/ModPolarPatch.F90:99-148)
+__device__ __forceinline__ void patch_point(double s0, double c0, double phi0, double thL,
+                                            double phL, double &thG, double &phG) {
+  double st = sin(thL), ct = cos(thL), sp = sin(phL), cp = cos(phL);
+  double x0 = st * cp, x1 = st * sp, x2 = ct;
+  double y0 = c0 * x0 + 0. * x1 + s0 * x2;
+  double y1 = 0. * x0 + 1. * x1 + 0. * x2;
+  double y2 = -s0 * x0 + 0. * x1 + c0 * x2;
+  y2 = fmax(-1.0, fmin(1.0, y2));
+  double ph = atan2(y1, y0) + phi0;
+  thG = acos(y2);
+  phG = ph - floor(ph * (1. / RBC_TWO_PI)) * RBC_TWO_PI;
+}
+
+// PolarPatch_Map (common/ModPolarPatch.F90:218-242)
+__device__ void patch_map(double th0, double phi0, double dth, double dphi, double &th, double &phi) {
+  double ct0 = cos(th0), st0 = sin(th0), cp0 = cos(phi0), sp0 = sin(phi0);
+  double s1[3] = {ct0 * cp0, ct0 * sp0, -st0};
+  double s2[3] = {-sp0, cp0, 0.};
+  double x0[3] = {st0 * cp0, st0 * sp0, ct0};
+  double xl[3] = {sin(dth) * cos(dphi), sin(dth) * sin(dphi), cos(dth)};
+  double x[3];
+  for (int d = 0; d < 3; d++) x[d] = xl[0] * s1[d] + xl[1] * s2[d] + xl[2] * x0[d];
+  double nrm = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+  for (int d = 0; d < 3; d++) x[d] = x[d] / nrm;
+  x[2] = fmax(-1., fmin(1., x[2]));
+  th = acos(x[2]);
+  phi = atan2(x[1], x[0]);
+  if (phi < 0) phi += RBC_TWO_PI;
+}
+
+// Spline_FindProjection (common/ModSpline.F90:203-269) with QuadFit_2D/Min_Quad_2D
+// (common/ModBasicMath.F90:234-328; dposv as Cholesky + triangular solves)
+__device__ void find_projection(const double *sx, int m, int n, const double xtar[3], double &th0,
+                                double &phi0, double x0[3]) {
+  const int nth = 2, nphi = 8, NQ = nth * nphi + 1;
+  spline_interp(sx, m, n, th0, phi0, 3, x0);
+  double h = fmax(RBC_TWO_PI / m, RBC_TWO_PI / n);
+  for (int iter = 1; iter <= 3; iter++) {
+    double lhs[21], rhs[6];  // packed upper triangle
+    for (int q = 0; q < 21; q++) lhs[q] = 0;
+    for (int q = 0; q < 6; q++) rhs[q] = 0;
+    double s0 = sin(th0), c0 = cos(th0);
+    double xx[3], d20 = 0;
+    spline_interp(sx, m, n, th0, phi0, 3, xx);
+    for (int d = 0; d < 3; d++) xx[d] -= xtar[d];
+    d20 = xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2];
+    for (int i = 0; i < NQ; i++) {
+      double xi_, yi_, fi_;
+      if (i == 0) {
+        xi_ = 0.;
+        yi_ = 0.;
+        fi_ = d20;
+      } else {
+        int it = (i - 1) % nth, ip = (i - 1) / nth;
+        double thL = (it + 1) * h / nth, phL = ip * RBC_TWO_PI / nphi;
+        xi_ = thL * cos(phL);
+        yi_ = thL * sin(phL);
+        double thP, phP;
+        patch_point(s0, c0, phi0, thL, phL, thP, phP);
+        spline_interp(sx, m, n, thP, phP, 3, xx);
+        for (int d = 0; d < 3; d++) xx[d] -= xtar[d];
+        fi_ = xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2];
+      }
+      double u[6] = {1., xi_, yi_, xi_ * xi_, xi_ * yi_, yi_ * yi_};
+      int q = 0;
+      for (int aa = 0; aa < 6; aa++)
+        for (int bb = aa; bb < 6; bb++) lhs[q++] += u[aa] * u[bb];
+      for (int aa = 0; aa < 6; aa++) rhs[aa] += u[aa] * fi_;
+    }
+    // Cholesky A = U^T U on the packed upper triangle (row-major packing)
+    double U[6][6];
+    {
+      int q = 0;
+      for (int aa = 0; aa < 6; aa++)
+        for (int bb = aa; bb < 6; bb++) U[aa][bb] = lhs[q++];
+    }
+    for (int j = 0; j < 6; j++) {
+      double sj = U[j][j];
+      for (int kk = 0; kk < j; kk++) sj -= U[kk][j] * U[kk][j];
+      double ujj = sqrt(sj);
+      U[j][j] = ujj;
+      for (int i = j + 1; i < 6; i++) {
+        double t = U[j][i];
+        for (int kk = 0; kk < j; kk++) t -= U[kk][j] * U[kk][i];
+        U[j][i] = t / ujj;
+      }
+    }
+    for (int i = 0; i < 6; i++) {
+      double t = rhs[i];
+      for (int kk = 0; kk < i; kk++) t -= U[kk][i] * rhs[kk];
+      rhs[i] = t / U[i][i];
+    }
+    for (int i = 5; i >= 0; i--) {
+      double t = rhs[i];
+      for (int kk = i + 1; kk < 6; kk++) t -= U[i][kk] * rhs[kk];
+      rhs[i] = t / U[i][i];
+    }
+    double l11 = 2. * rhs[3], l12 = rhs[4], l21 = rhs[4], l22 = 2. * rhs[5];
+    double det = l11 * l22 - l12 * l21, xm0 = 0., xm1 = 0.;
+    if (det > 0) {
+      double idet = 1. / det, r1 = -rhs[1], r2 = -rhs[2];
+      xm0 = idet * (l22 * r1 - l12 * r2);
+      xm1 = idet * (-l21 * r1 + l11 * r2);
+    }
+    double thMinL = sqrt(xm0 * xm0 + xm1 * xm1), phiMinL = atan2(xm1, xm0);
+    double thMin, phiMin, xMin[3];
+    patch_map(th0, phi0, thMinL, phiMinL, thMin, phiMin);
+    spline_interp(sx, m, n, thMin, phiMin, 3, xMin);
+    for (int d = 0; d < 3; d++) xx[d] = xMin[d] - xtar[d];
+    double d2Min = xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2];
+    if (d2Min > d20) break;
+    th0 = thMin;
+    phi0 = phiMin;
+    for (int d = 0; d < 3; d++) x0[d] = xMin[d];
+    h = 0.5 * h;
+  }
+}
+
+// GauLeg (common/ModQuadRule.F90:135-208), device copy (n <= 32)
+__device__ void gauleg_d(double x1, double x2, int n, double *x, double *w) {
+  int m = (n + 1) / 2;
+  double xm = 0.5 * (x2 + x1), xl = 0.5 * (x2 - x1);
+  for (int j = 0; j < m; j++) {
+    double z = cos(RBC_PI * ((j + 1) - 0.25) / (n + 0.5)), z1, p1 = 0, p2, p3, pp = 0;
+    bool unfinished = true;
+    for (int its = 1; its <= 10 && unfinished; its++) {
+      p1 = 1.0;
+      p2 = 0.0;
+      for (int jj = 1; jj <= n; jj++) {
+        p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * jj - 1.0) * z * p2 - (jj - 1.0) * p3) / jj;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+      z1 = z;
+      z = z1 - p1 / pp;
+      unfinished = fabs(z - z1) > 3.e-14;
+    }
+    x[j] = xm - xl * z;
+    x[n - 1 - j] = xm + xl * z;
+    w[j] = 2.0 * xl / ((1.0 - z * z) * pp * pp);
+    w[n - 1 - j] = w[j];
+  }
+}
+
+// GauLeg_Sinh (common/ModQuadRule.F90:223-263)
+__device__ void gauleg_sinh_d(double xmin, double xmax, double a, double b, int n, double *x, double *w) {
+  double xm = 0.5 * (xmin + xmax), xl = 0.5 * (xmax - xmin);
+  double a0 = (a - xm) / xl, b0 = b * xl;
+  double f1 = (1 + a0) / b0, f2 = (1 - a0) / b0;
+  double u1 = log(f1 + sqrt(1 + f1 * f1)), u2 = log(f2 + sqrt(1 + f2 * f2));
+  double mu = 0.5 * (u1 + u2), eta = 0.5 * (u1 - u2);
+  double s[16];
+  gauleg_d(-1., 1., n, s, w);
+  for (int i = 0; i < n; i++) {
+    x[i] = a0 + b0 * sinh(mu * s[i] - eta);
+    w[i] = w[i] * b0 * mu * cosh(mu * s[i] - eta);
+  }
+  for (int i = 0; i < n; i++) {
+    w[i] = xl * w[i];
+    x[i] = xm + xl * x[i];
+  }
+}
+
+struct SingArgs {
+  double Lb[3], iLb[3];
+  double rc, reps, c1, c2;
+  int np, nT, nrbc;
+  const CellRec *cells;
+  const double *spl;
+  const double *const *th, *const *phi, *const *thG, *const *phiG, *const *pwt;
+  const int *nrad, *nazm;
+  const double *sl1, *sl2, *dl1, *mtab;
+  const double *x, *fs, *gs, *a3, *detj, *w_unused;
+  const double *tx, *tA;
+  const int *tindx, *tact;
+  double *dvself;
+  // near-singular
+  const int *cnt, *slot_cell, *slot_node, *item_off;
+  double *item_rec;  // 12 doubles per item
+  int *work, *work_cnt;
+  double *dvitem;
+};
+
+// RBC_SingInt (common/ModRbcSingInt.F90:29-90): one wavefront per target on a cell
+__global__ void __launch_bounds__(256) k_self(SingArgs a) {
+  int wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  if (wid >= a.nT) return;
+  int ti = wid;
+  int surf = a.tindx[ti];
+  if (!a.tact[ti] || surf < 1 || surf > a.nrbc) {
+    if (lane == 0)
+      for (int d = 0; d < 3; d++) a.dvself[ti + (size_t)a.nT * d] = 0.;
+    return;
+  }
+  const CellRec cr = a.cells[surf - 1];
+  int gt = cr.gtype, m = 2 * cr.nlat, n = cr.nlon;
+  int ilat = a.tindx[ti + a.nT] - 1, ilon = a.tindx[ti + 2 * (size_t)a.nT] - 1;
+  const double *blk = a.spl + cr.spl;
+  size_t vs = (size_t)m * n * 4;
+  double xi[3];
+  spline_interp(blk + vs * SPL_X, m, n, a.th[gt][ilat], a.phi[gt][ilon], 3, xi);
+  int nrad = a.nrad[gt], nazm = a.nazm[gt], npt = nrad * nazm;
+  const double *thG = a.thG[gt] + (size_t)npt * (ilat + cr.nlat * ilon);
+  const double *phG = a.phiG[gt] + (size_t)npt * (ilat + cr.nlat * ilon);
+  const double *pw = a.pwt[gt];
+  double c2m = a.c2 * cr.Bcoef;
+  double dv0 = 0, dv1 = 0, dv2 = 0;
+  for (int p = lane; p < npt; p += 64) {
+    int irad = p % nrad;
+    double thj = thG[p], phj = phG[p], xj[3], xx[3];
+    spline_interp(blk + vs * SPL_X, m, n, thj, phj, 3, xj);
+    for (int d = 0; d < 3; d++) xx[d] = xj[d] - xi[d];
+    double rr = sqrt(xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2]);
+    if (rr >= a.rc) continue;
+    if (a.c1 != 0) {
+      double fj[3], EA, EB;
+      spline_interp(blk + vs * SPL_F, m, n, thj, phj, 3, fj);
+      for (int d = 0; d < 3; d++) fj[d] = pw[irad] * fj[d];
+      ewald_sl(a.sl1, a.sl2, rr, a.reps, a.rc, EA, EB);
+      double xf = xx[0] * fj[0] + xx[1] * fj[1] + xx[2] * fj[2];
+      dv0 += a.c1 * (EA * xx[0] * xf + EB * fj[0]);
+      dv1 += a.c1 * (EA * xx[1] * xf + EB * fj[1]);
+      dv2 += a.c1 * (EA * xx[2] * xf + EB * fj[2]);
+    }
+    if (c2m != 0) {
+      double gj[3], nj[3];
+      spline_interp(blk + vs * SPL_G, m, n, thj, phj, 3, gj);
+      for (int d = 0; d < 3; d++) gj[d] = pw[irad] * gj[d];
+      spline_interp(blk + vs * SPL_A3, m, n, thj, phj, 3, nj);
+      double EA = ewald_dl(a.dl1, rr, a.reps, a.rc);
+      double xg = xx[0] * gj[0] + xx[1] * gj[1] + xx[2] * gj[2];
+      double xn = xx[0] * nj[0] + xx[1] * nj[1] + xx[2] * nj[2];
+      dv0 += c2m * (EA * xx[0] * xg * xn);
+      dv1 += c2m * (EA * xx[1] * xg * xn);
+      dv2 += c2m * (EA * xx[2] * xg * xn);
+    }
+  }
+  dv0 = wave_sum(dv0);
+  dv1 = wave_sum(dv1);
+  dv2 = wave_sum(dv2);
+  if (lane == 0) {
+    a.dvself[ti] = dv0;
+    a.dvself[ti + (size_t)a.nT] = dv1;
+    a.dvself[ti + 2 * (size_t)a.nT] = dv2;
+  }
+}
+
+// Near-singular stage 1: translate the target next to the closest node, project it
+// onto the neighbour's spline surface and apply the distance test of RBC_NearSingInt.
+// item record: xi(3) x0(3) th0 phi0 a30(3) dist
+__global__ void __launch_bounds__(128) k_proj(SingArgs a) {
+  int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  int ti = gid / RBC_NBR_MAX, p = gid % RBC_NBR_MAX;
+  if (ti >= a.nT) return;
+  if (p >= a.cnt[ti]) return;
+  size_t s = (size_t)ti * RBC_NBR_MAX + p;
+  int cj = a.slot_cell[s], node = a.slot_node[s];
+  const CellRec cr = a.cells[cj];
+  int gt = cr.gtype, m = 2 * cr.nlat, n = cr.nlon;
+  int q = node - cr.off, ilat0 = q % cr.nlat, ilon0 = q / cr.nlat;
+  double th0 = a.th[gt][ilat0], phi0 = a.phi[gt][ilon0], x0[3], xt[3];
+  size_t np = a.np;
+  for (int d = 0; d < 3; d++) x0[d] = a.x[node + np * d];
+  for (int d = 0; d < 3; d++) {
+#pragma clang fp contract(off)
+    double xx = a.tx[ti + (size_t)a.nT * d] - x0[d];
+    double sh = round(xx * a.iLb[d]) * a.Lb[d];
+    xx = xx - sh;
+    xt[d] = x0[d] + xx;
+  }
+  const double *blk = a.spl + cr.spl;
+  size_t vs = (size_t)m * n * 4;
+  find_projection(blk + vs * SPL_X, m, n, xt, th0, phi0, x0);
+  double a30[3];
+  spline_interp(blk + vs * SPL_A3, m, n, th0, phi0, 3, a30);
+  double dist = a30[0] * (xt[0] - x0[0]) + a30[1] * (xt[1] - x0[1]) + a30[2] * (xt[2] - x0[2]);
+  size_t it = (size_t)a.item_off[ti] + p;
+  double *r = a.item_rec + 12 * it;
+  for (int d = 0; d < 3; d++) {
+    r[d] = xt[d];
+    r[3 + d] = x0[d];
+    r[8 + d] = a30[d];
+  }
+  r[6] = th0;
+  r[7] = phi0;
+  r[11] = dist;
+  for (int d = 0; d < 3; d++) a.dvitem[3 * it + d] = 0.;
+  if (dist > 2 * cr.meshSize) return;
+  int slot = atomicAdd(a.work_cnt, 1);
+  a.work[2 * slot] = (int)it;
+  a.work[2 * slot + 1] = cj;
+}
+
+// RBC_NearSingInt_ReAdd (common/ModRbcSingInt.F90:232-310), lanes over the 16 x 32 patch
+__device__ void readd_wave(const SingArgs &a, const CellRec &cr, const double *blk, int m, int n,
+                           const double xi[3], const double x0[3], double th0, double phi0,
+                           double radPat, double c1, double c2, double dv[3]) {
+  const int nrad = 16, nazm = 32;
+  double thP[16], wP[16];
+  double d0 = xi[0] - x0[0], d1 = xi[1] - x0[1], d2 = xi[2] - x0[2];
+  double dist = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+  double sizePat = radPat * sqrt(cr.area / (4 * RBC_PI));
+  if (dist > 2.2250738585072014e-308) {
+    dist = dist * (radPat / sizePat);
+    gauleg_sinh_d(0., radPat, 0., dist, nrad, thP, wP);
+  } else {
+    gauleg_d(0., radPat, nrad, thP, wP);
+  }
+  int lane = threadIdx.x & 63;
+  size_t vs = (size_t)m * n * 4;
+  double s0 = sin(th0), c0 = cos(th0);
+  double a0 = 0, a1 = 0, a2 = 0;
+  for (int p = lane; p < nrad * nazm; p += 64) {
+    int irad = p % nrad, iazm = p / nrad;
+    double w = wP[irad] * sin(thP[irad]) * (RBC_TWO_PI / nazm);
+    w = w * mask_fn(a.mtab, thP[irad] / radPat);
+    double phL = iazm * RBC_TWO_PI / nazm, thj, phj;
+    patch_point(s0, c0, phi0, thP[irad], phL, thj, phj);
+    double xj[3], xx[3];
+    spline_interp(blk + vs * SPL_X, m, n, thj, phj, 3, xj);
+    for (int d = 0; d < 3; d++) xx[d] = xj[d] - xi[d];
+    double rr = sqrt(xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2]);
+    if (rr > a.rc) continue;
+    if (c1 != 0) {
+      double fj[3], EA, EB;
+      spline_interp(blk + vs * SPL_F, m, n, thj, phj, 3, fj);
+      for (int d = 0; d < 3; d++) fj[d] = w * fj[d];
+      ewald_sl(a.sl1, a.sl2, rr, a.reps, a.rc, EA, EB);
+      double xf = xx[0] * fj[0] + xx[1] * fj[1] + xx[2] * fj[2];
+      a0 += c1 * (EA * xx[0] * xf + EB * fj[0]);
+      a1 += c1 * (EA * xx[1] * xf + EB * fj[1]);
+      a2 += c1 * (EA * xx[2] * xf + EB * fj[2]);
+    }
+    if (c2 != 0) {
+      double gj[3], nj[3];
+      spline_interp(blk + vs * SPL_A3, m, n, thj, phj, 3, nj);
+      spline_interp(blk + vs * SPL_G, m, n, thj, phj, 3, gj);
+      for (int d = 0; d < 3; d++) gj[d] = w * gj[d];
+      double EA = ewald_dl(a.dl1, rr, a.reps, a.rc);
+      double xg = xx[0] * gj[0] + xx[1] * gj[1] + xx[2] * gj[2];
+      double xn = xx[0] * nj[0] + xx[1] * nj[1] + xx[2] * nj[2];
+      a0 += c2 * (EA * xx[0] * xg * xn);
+      a1 += c2 * (EA * xx[1] * xg * xn);
+      a2 += c2 * (EA * xx[2] * xg * xn);
+    }
+  }
+  dv[0] = wave_sum(a0);
+  dv[1] = wave_sum(a1);
+  dv[2] = wave_sum(a2);
+}
+
+// Near-singular stage 2: one wavefront per item that passed the distance test.
+// RBC_NearSingInt (:103-166) = subtract (:175-226) + re-add, or the interpolation
+// between the re-adds at xi1 and on the surface (with the double-layer jump).
+__global__ void __launch_bounds__(256) k_near(SingArgs a, int nwork) {
+  int wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  if (wid >= nwork) return;
+  int it = a.work[2 * wid], cj = a.work[2 * wid + 1];
+  const CellRec cr = a.cells[cj];
+  int gt = cr.gtype, m = 2 * cr.nlat, n = cr.nlon, nl = cr.nlat, nn = cr.nlon;
+  const double *r = a.item_rec + 12 * (size_t)it;
+  double xi[3] = {r[0], r[1], r[2]}, x0[3] = {r[3], r[4], r[5]}, a30[3] = {r[8], r[9], r[10]};
+  double th0 = r[6], phi0 = r[7], dist = r[11];
+  double radPat = cr.radius;
+  double c1 = a.c1, c2 = a.c2 * cr.Bcoef;
+  const double *blk = a.spl + cr.spl;
+  size_t vs = (size_t)m * n * 4;
+  double sizePat = radPat * sqrt(cr.area / (4 * RBC_PI));
+  double dist1 = copysign(0.01 * sizePat, dist);
+  // subtract: nodes of PolarPatch_FindPoints (common/ModPolarPatch.F90:161-209), with
+  // their multiplicity when the longitude range wraps more than once
+  const double *th = a.th[gt], *ph = a.phi[gt];
+  const double eps = 1.e-10;
+  double h_phi = RBC_TWO_PI / nn, ih_phi = 1.0 / h_phi, cosr0 = cos(radPat);
+  double s0 = 0, s1 = 0, s2 = 0;
+  // first row that fails "ths >= th0 + r0" ends the scan (ths ascending)
+  for (int q = lane; q < nl * nn; q += 64) {
+    int i = q % nl, jl = q / nl;
+    if (th[i] <= th0 - radPat) continue;
+    if (th[i] >= th0 + radPat) continue;
+    double dth = th[i] - th0;
+    double dphi = 1. - (cos(dth) - cosr0) / ((sin(th[i]) + eps) * (sin(th0) + eps));
+    dphi = fmax(-1., fmin(1., dphi));
+    dphi = acos(dphi);
+    int jmin, jmax;
+    if (dphi > RBC_PI - eps) {
+      jmin = 1;
+      jmax = nn;
+    } else {
+      jmin = (int)ceil((phi0 - dphi - ph[0]) * ih_phi) + 1;
+      jmax = (int)floor((phi0 + dphi - ph[0]) * ih_phi) + 1;
+    }
+    // multiplicity of node jl: #{j in [jmin,jmax] : (j-1) mod nn == jl}
+    int lo = jmin - 1 - jl, hi = jmax - 1 - jl;  // need k*nn in [lo, hi]
+    if (hi < lo) continue;
+    int kmin = (int)ceil((double)lo / nn), kmax = (int)floor((double)hi / nn);
+    int mult = kmax - kmin + 1;
+    if (mult <= 0) continue;
+    int node = cr.off + q;
+    size_t np = a.np;
+    double xx[3];
+    for (int d = 0; d < 3; d++) xx[d] = a.x[node + np * d] - xi[d];
+    double rr = sqrt(xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2]);
+    if (rr > a.rc) continue;
+    double dthm = dist_on_sphere(th0, phi0, th[i], ph[jl]);
+    double mask = mask_fn(a.mtab, dthm / radPat);
+    for (int rep = 0; rep < mult; rep++) {
+      if (c1 != 0) {
+        double fj[3] = {a.fs[node], a.fs[node + np], a.fs[node + 2 * np]}, EA, EB;
+        ewald_sl(a.sl1, a.sl2, rr, a.reps, a.rc, EA, EB);
+        double xf = xx[0] * fj[0] + xx[1] * fj[1] + xx[2] * fj[2];
+        s0 -= c1 * mask * (EA * xx[0] * xf + EB * fj[0]);
+        s1 -= c1 * mask * (EA * xx[1] * xf + EB * fj[1]);
+        s2 -= c1 * mask * (EA * xx[2] * xf + EB * fj[2]);
+      }
+      if (c2 != 0) {
+        double gj[3] = {a.gs[node], a.gs[node + np], a.gs[node + 2 * np]};
+        double nj[3] = {a.a3[node], a.a3[node + np], a.a3[node + 2 * np]};
+        double EA = ewald_dl(a.dl1, rr, a.reps, a.rc);
+        double xg = xx[0] * gj[0] + xx[1] * gj[1] + xx[2] * gj[2];
+        double xn = xx[0] * nj[0] + xx[1] * nj[1] + xx[2] * nj[2];
+        s0 -= c2 * mask * (EA * xx[0] * xg * xn);
+        s1 -= c2 * mask * (EA * xx[1] * xg * xn);
+        s2 -= c2 * mask * (EA * xx[2] * xg * xn);
+      }
+    }
+  }
+  double dv[3] = {wave_sum(s0), wave_sum(s1), wave_sum(s2)}, dt[3];
+  if (fabs(dist) >= fabs(dist1)) {
+    readd_wave(a, cr, blk, m, n, xi, x0, th0, phi0, radPat, c1, c2, dt);
+    for (int d = 0; d < 3; d++) dv[d] += dt[d];
+  } else {
+    double xi1[3], dv1[3], dv0[3];
+    for (int d = 0; d < 3; d++) xi1[d] = x0[d] + dist1 * a30[d];
+    readd_wave(a, cr, blk, m, n, xi1, x0, th0, phi0, radPat, c1, c2, dv1);
+    readd_wave(a, cr, blk, m, n, x0, x0, th0, phi0, radPat, c1, c2, dv0);
+    if (c2 != 0) {
+      double detJ0, g0[3];
+      spline_interp(blk + vs * SPL_DETJ, m, n, th0, phi0, 1, &detJ0);
+      spline_interp(blk + vs * SPL_G, m, n, th0, phi0, 3, g0);
+      for (int d = 0; d < 3; d++) g0[d] = g0[d] / detJ0;
+      for (int d = 0; d < 3; d++) dv0[d] += (dist > 0 ? 1. : -1.) * c2 * 4 * RBC_PI * g0[d];
+    }
+    for (int d = 0; d < 3; d++) dv[d] += dv0[d] + dist / dist1 * (dv1[d] - dv0[d]);
+  }
+  if (lane == 0)
+    for (int d = 0; d < 3; d++) a.dvitem[3 * (size_t)it + d] = dv[d];
+}
+
+// AddLinearInt (common/ModIntOnRbcs.F90:162-201): partial sums per block
+__global__ void __launch_bounds__(256) k_linear_partial(int np, const double *__restrict__ x,
+                                                        const double *__restrict__ g,
+                                                        const double *__restrict__ a3,
+                                                        const double *__restrict__ ds,
+                                                        const int *__restrict__ node_cell,
+                                                        const CellRec *__restrict__ cells,
+                                                        double *part) {
+  __shared__ double red[3][256];
+  double s[3] = {0, 0, 0};
+  for (int p = blockIdx.x * blockDim.x + threadIdx.x; p < np; p += gridDim.x * blockDim.x) {
+    double vn = 0;
+    for (int d = 0; d < 3; d++) vn += g[p + (size_t)np * d] * a3[p + (size_t)np * d];
+    double B = cells[node_cell[p]].Bcoef;
+    for (int d = 0; d < 3; d++) s[d] += B * x[p + (size_t)np * d] * vn * ds[p];
+  }
+  for (int d = 0; d < 3; d++) red[d][threadIdx.x] = s[d];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w)
+      for (int d = 0; d < 3; d++) red[d][threadIdx.x] += red[d][threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0)
+    for (int d = 0; d < 3; d++) part[3 * blockIdx.x + d] = red[d][0];
+}
+
+__global__ void k_linear_final(int nb, const double *part, double scale, double *xv) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double s[3] = {0, 0, 0};
+  for (int b = 0; b < nb; b++)
+    for (int d = 0; d < 3; d++) s[d] += part[3 * b + d];
+  for (int d = 0; d < 3; d++) xv[d] = scale * s[d];
+}
+
+struct CombineArgs {
+  int nT;
+  double c2;
+  const int *tact, *cnt, *item_off;
+  const double *tA, *vpair, *dvself, *dvitem, *xv;
+  int has_linear;
+  double *v;
+};
+
+__global__ void k_combine(CombineArgs a) {
+  int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ti >= a.nT || !a.tact[ti]) return;
+  double A = a.tA[ti];
+  for (int d = 0; d < 3; d++) {
+    double s = a.vpair[ti + (size_t)a.nT * d];
+    s += a.dvself[ti + (size_t)a.nT * d] / A;
+    int off = a.item_off[ti];
+    for (int p = 0; p < a.cnt[ti]; p++) s += a.dvitem[3 * (size_t)(off + p) + d] / A;
+    if (a.has_linear) s += a.c2 * a.xv[d] / A;
+    a.v[ti + (size_t)a.nT * d] += s;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// table pointer arrays per grid type, owned by the context (rebuilt when a grid
+// type is added)
+struct TypePtrs {
+  const double *const *mask, *const *th, *const *phi, *const *thG, *const *phiG, *const *pwt;
+  const int *nrad, *nazm;
+};
+static int type_ptrs(rbc3d_ctx *c, TypePtrs *tp) {
+  size_t nt = c->gtypes.size();
+  if (c->tp_ntypes != nt) {
+    std::vector<const double *> ptr(6 * nt);
+    std::vector<int> ni(2 * nt);
+    for (size_t i = 0; i < nt; i++) {
+      GridType *g = c->gtypes[i];
+      ptr[0 * nt + i] = g->d_mask.p;
+      ptr[1 * nt + i] = g->d_th.p;
+      ptr[2 * nt + i] = g->d_phi.p;
+      ptr[3 * nt + i] = g->d_thG.p;
+      ptr[4 * nt + i] = g->d_phiG.p;
+      ptr[5 * nt + i] = g->d_pwt.p;
+      ni[i] = g->nrad;
+      ni[nt + i] = g->nazm;
+    }
+    RBC_TRY(c->d_tp_ptr.ensure(ptr.size()));
+    RBC_TRY(c->d_tp_int.ensure(ni.size()));
+    HIP_TRY(hipMemcpy(c->d_tp_ptr.p, ptr.data(), ptr.size() * sizeof(ptr[0]), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(c->d_tp_int.p, ni.data(), ni.size() * sizeof(int), hipMemcpyHostToDevice));
+    c->tp_ntypes = nt;
+  }
+  const double *const *p = c->d_tp_ptr.p;
+  tp->mask = p;
+  tp->th = p + nt;
+  tp->phi = p + 2 * nt;
+  tp->thG = p + 3 * nt;
+  tp->phiG = p + 4 * nt;
+  tp->pwt = p + 5 * nt;
+  tp->nrad = c->d_tp_int.p;
+  tp->nazm = c->d_tp_int.p + nt;
+  return RBC3D_OK;
+}
+
+static int exclusive_scan(rbc3d_ctx *c, const int *in, int *out, int n) {
+  size_t tmp = 0;
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, n, c->stream));
+  RBC_TRY(c->d_cub_tmp.ensure(tmp + 16));
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->d_cub_tmp.p, tmp, in, out, n, c->stream));
+  return RBC3D_OK;
+}
+
+int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const double *d_tx,
+                            const int *d_tindx, const int *d_act, const double *d_A, double *d_v) {
+  if (c->nrbc == 0 || nT == 0) return RBC3D_OK;
+  hipStream_t st = c->stream;
+  int nb = c->nbins;
+  TypePtrs tpv, *tp = &tpv;
+  RBC_TRY(type_ptrs(c, tp));
+  // targets binned like the sources
+  RBC_TRY(c->d_tbin.ensure(nT + 1));
+  RBC_TRY(c->d_torder.ensure(nT + 1));
+  RBC_TRY(c->d_tbin_start.ensure(nb + 1));
+  RBC_TRY(rbc_hash_sort(c, nT, d_tx, nT, c->d_tbin.p, c->d_torder.p, c->d_tbin_start.p));
+  RBC_TRY(c->d_flag.ensure(2 * (size_t)(nb + 1) + 8));
+  int *chunk_cnt = c->d_flag.p + 8, *chunk_off = chunk_cnt + (nb + 1);
+  k_chunks<<<(nb + 1 + 255) / 256, 256, 0, st>>>(nb, c->d_tbin_start.p, chunk_cnt);
+  RBC_TRY(exclusive_scan(c, chunk_cnt, chunk_off, nb + 1));
+  RBC_TRY(c->d_vacc.ensure((size_t)3 * nT + 1));
+  RBC_TRY(c->d_pairstat.ensure(2));
+  HIP_TRY(hipMemsetAsync(c->d_pairstat.p, 0, 2 * sizeof(unsigned long long), st));
+  HIP_TRY(hipMemsetAsync(c->d_flag.p, 0, 8 * sizeof(int), st));
+  PairArgs pa;
+  for (int d = 0; d < 3; d++) {
+    pa.Lb[d] = c->Lb[d];
+    pa.iLb[d] = c->iLb[d];
+    pa.Nc[d] = c->Nc[d];
+  }
+  pa.rc = c->conf.rc;
+  pa.reps = c->r_eps;
+  pa.c1 = c1;
+  pa.c2 = c2;
+  pa.np = c->np;
+  pa.nT = nT;
+  pa.nbins = nb;
+  pa.nrbc = c->nrbc;
+  pa.rec = c->d_src_rec.p;
+  pa.src_node = c->d_work.p;
+  pa.bin_start = c->d_bin_start.p;
+  pa.node_cell = c->d_node_cell.p;
+  pa.node_ll = c->d_node_ll.p;
+  pa.cells = c->d_cells.p;
+  pa.sl1 = c->d_sl1.p;
+  pa.sl2 = c->d_sl2.p;
+  pa.dl1 = c->d_dl1.p;
+  pa.mask = tp->mask;
+  pa.tx = d_tx;
+  pa.tA = d_A;
+  pa.tindx = d_tindx;
+  pa.tact = d_act;
+  pa.torder = c->d_torder.p;
+  pa.tbin_start = c->d_tbin_start.p;
+  pa.chunk_off = chunk_off;
+  pa.vpair = c->d_vacc.p;
+  pa.stat = c->d_pairstat.p;
+  int nblk = nb + (nT + PAIR_T - 1) / PAIR_T + 1;
+  k_pair<<<nblk, PAIR_T, 0, st>>>(pa);
+  HIP_TRY(hipGetLastError());
+  // neighbouring-cell lists
+  RBC_TRY(c->d_nbr_cnt.ensure(2 * (size_t)nT + 2));
+  RBC_TRY(c->d_nbr_cell.ensure((size_t)RBC_NBR_MAX * nT + 1));
+  RBC_TRY(c->d_nbr_node.ensure((size_t)RBC_NBR_MAX * nT + 1));
+  RBC_TRY(c->d_nbr_dist.ensure((size_t)RBC_NBR_MAX * nT + 1));
+  NbrArgs na;
+  for (int d = 0; d < 3; d++) {
+    na.Lb[d] = c->Lb[d];
+    na.iLb[d] = c->iLb[d];
+    na.iLbNc[d] = c->iLbNc[d];
+    na.Nc[d] = c->Nc[d];
+  }
+  na.rc = c->conf.rc;
+  na.np = c->np;
+  na.nT = nT;
+  na.nrbc = c->nrbc;
+  na.x = c->d_x.p;
+  na.src_bin = c->d_src_bin.p;
+  na.cells = c->d_cells.p;
+  na.sph = c->d_cell_sph.p;
+  na.tx = d_tx;
+  na.tindx = d_tindx;
+  na.tact = d_act;
+  na.cnt = c->d_nbr_cnt.p;
+  na.slot_cell = c->d_nbr_cell.p;
+  na.slot_node = c->d_nbr_node.p;
+  na.slot_dist = c->d_nbr_dist.p;
+  na.flag = c->d_flag.p;
+  k_nbr<<<(nT + NBR_T - 1) / NBR_T, NBR_T, 0, st>>>(na);
+  HIP_TRY(hipGetLastError());
+  int *item_off = c->d_nbr_cnt.p + nT + 1;
+  HIP_TRY(hipMemsetAsync(c->d_nbr_cnt.p + nT, 0, sizeof(int), st));
+  RBC_TRY(exclusive_scan(c, c->d_nbr_cnt.p, item_off, nT + 1));
+  int hflag[2] = {0, 0}, nitems = 0;
+  HIP_TRY(hipMemcpyAsync(hflag, c->d_flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(&nitems, item_off + nT, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (hflag[0]) {
+    rbc_set_error("more than 32 neighbouring cells within the cut-off of a target");
+    return RBC3D_ENBR;
+  }
+  // singular and near-singular corrections
+  RBC_TRY(c->d_scratch.ensure((size_t)12 * nitems + (size_t)3 * nT + 16));
+  RBC_TRY(c->d_dvitem.ensure((size_t)3 * nitems + 3));
+  RBC_TRY(c->d_work_cnt.ensure(2));
+  // d_work holds the sorted-position -> node map; the near-singular worklist is separate
+  DBuf<int> wl;
+  RBC_TRY(wl.ensure(2 * (size_t)nitems + 2));
+  HIP_TRY(hipMemsetAsync(c->d_work_cnt.p, 0, sizeof(int), st));
+  SingArgs sa;
+  for (int d = 0; d < 3; d++) {
+    sa.Lb[d] = c->Lb[d];
+    sa.iLb[d] = c->iLb[d];
+  }
+  sa.rc = c->conf.rc;
+  sa.reps = c->r_eps;
+  sa.c1 = c1;
+  sa.c2 = c2;
+  sa.np = c->np;
+  sa.nT = nT;
+  sa.nrbc = c->nrbc;
+  sa.cells = c->d_cells.p;
+  sa.spl = c->d_spl.p;
+  sa.th = tp->th;
+  sa.phi = tp->phi;
+  sa.thG = tp->thG;
+  sa.phiG = tp->phiG;
+  sa.pwt = tp->pwt;
+  sa.nrad = tp->nrad;
+  sa.nazm = tp->nazm;
+  sa.sl1 = c->d_sl1.p;
+  sa.sl2 = c->d_sl2.p;
+  sa.dl1 = c->d_dl1.p;
+  sa.mtab = c->d_mtab.p;
+  sa.x = c->d_x.p;
+  sa.fs = c->d_fs.p;
+  sa.gs = c->d_gs.p;
+  sa.a3 = c->d_a3.p;
+  sa.detj = c->d_detj.p;
+  sa.w_unused = nullptr;
+  sa.tx = d_tx;
+  sa.tA = d_A;
+  sa.tindx = d_tindx;
+  sa.tact = d_act;
+  double *dvself = c->d_scratch.p + (size_t)12 * nitems;
+  sa.dvself = dvself;
+  sa.cnt = c->d_nbr_cnt.p;
+  sa.slot_cell = c->d_nbr_cell.p;
+  sa.slot_node = c->d_nbr_node.p;
+  sa.item_off = item_off;
+  sa.item_rec = c->d_scratch.p;
+  sa.work = wl.p;
+  sa.work_cnt = c->d_work_cnt.p;
+  sa.dvitem = c->d_dvitem.p;
+  k_self<<<(unsigned)(((size_t)nT * 64 + 255) / 256), 256, 0, st>>>(sa);
+  HIP_TRY(hipGetLastError());
+  if (nitems > 0) {
+    k_proj<<<(unsigned)(((size_t)nT * RBC_NBR_MAX + 127) / 128), 128, 0, st>>>(sa);
+    HIP_TRY(hipGetLastError());
+    int nwork = 0;
+    HIP_TRY(hipMemcpyAsync(&nwork, c->d_work_cnt.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+    if (nwork > 0) {
+      k_near<<<(unsigned)(((size_t)nwork * 64 + 255) / 256), 256, 0, st>>>(sa, nwork);
+      HIP_TRY(hipGetLastError());
+    }
+  }
+  // linear double-layer term
+  double *xv = c->d_scratch.p + (size_t)12 * nitems + (size_t)3 * nT + 8;
+  DBuf<double> part;
+  if (c2 != 0) {
+    int nbk = 256;
+    RBC_TRY(part.ensure(3 * nbk));
+    k_linear_partial<<<nbk, 256, 0, st>>>(c->np, c->d_x.p, c->d_g.p, c->d_a3.p, c->d_ds.p,
+                                          c->d_node_cell.p, c->d_cells.p, part.p);
+    double scale = -8 * RBC_PI * (c->iLb[0] * c->iLb[1] * c->iLb[2]);
+    k_linear_final<<<1, 64, 0, st>>>(nbk, part.p, scale, xv);
+  }
+  CombineArgs ca;
+  ca.nT = nT;
+  ca.c2 = c2;
+  ca.tact = d_act;
+  ca.cnt = c->d_nbr_cnt.p;
+  ca.item_off = item_off;
+  ca.tA = d_A;
+  ca.vpair = c->d_vacc.p;
+  ca.dvself = dvself;
+  ca.dvitem = c->d_dvitem.p;
+  ca.xv = xv;
+  ca.has_linear = c2 != 0;
+  ca.v = d_v;
+  k_combine<<<(nT + 255) / 256, 256, 0, st>>>(ca);
+  HIP_TRY(hipGetLastError());
+  unsigned long long hs[2];
+  HIP_TRY(hipMemcpyAsync(hs, c->d_pairstat.p, sizeof(hs), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  c->last_npairs = (long long)hs[0];
+  c->last_checksum = hs[1];
+  part.release();
+  wl.release();
+  return RBC3D_OK;
+}

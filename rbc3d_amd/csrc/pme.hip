@@ -1,0 +1,726 @@
+// Smooth particle-mesh Ewald on the GPU (common/ModPME.F90, FFT conventions of
+// common/ModPFFTW.F90):
+//   spread  -- owner-computes per grid brick: sources are sorted by the brick of
+//              their B-spline stencil origin; a workgroup owns one brick, stages
+//              the sources of the bricks whose stencils reach it through LDS and
+//              each lane sums the contributions to its own grid points (no atomics);
+//   FFT     -- in-LDS mixed-radix (2,3,4,5) Stockham FFTs over batches of lines:
+//              x r2c, y c2c (sign -1), z c2c (sign +1); inverse z (-1), y (+1), x c2r
+//              keeping the real part of the zero and Nyquist x-frequencies;
+//   Green   -- elementwise k-space multiply with the B-factor;
+//   gather  -- one lane per target, P^3 stencil.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+
+#include "internal.h"
+
+#define SPR_B 8        // grid brick edge
+#define SPR_T 256
+#define FFT_T 256
+#define FFT_ELEMS 2048 // complex elements per FFT workgroup (lines x length)
+
+// ---------------------------------------------------------------------------
+// B-spline weights: BsplineFunc (common/ModBasicMath.F90:392-417)
+__device__ __forceinline__ int bspline_d(double xc, int P, double *w) {
+  double u[12];
+  int imin = (int)floor(xc) - (P - 1);
+  u[0] = imin - (xc - P);
+  for (int j = 1; j < P; j++) u[j] = u[j - 1] + 1;
+  w[0] = 1.;
+  for (int j = 1; j < P; j++) w[j] = 0.;
+  for (int pp = 2; pp <= P; pp++) {
+    for (int j = pp; j >= 2; j--)
+      w[j - 1] = u[j - 1] / (pp - 1.) * w[j - 1] + (pp - u[j - 1]) / (pp - 1.) * w[j - 2];
+    w[0] = u[0] / (pp - 1.) * w[0];
+  }
+  return imin;
+}
+
+static int host_bspline(double xc, int P, double *w) {
+  double u[16];
+  int imin = (int)floor(xc) - (P - 1);
+  u[0] = imin - (xc - P);
+  for (int j = 1; j < P; j++) u[j] = u[j - 1] + 1;
+  w[0] = 1.;
+  for (int j = 1; j < P; j++) w[j] = 0.;
+  for (int pp = 2; pp <= P; pp++) {
+    for (int j = pp; j >= 2; j--)
+      w[j - 1] = u[j - 1] / (pp - 1.) * w[j - 1] + (pp - u[j - 1]) / (pp - 1.) * w[j - 2];
+    w[0] = u[0] / (pp - 1.) * w[0];
+  }
+  return imin;
+}
+
+static void factor_radices(int n, std::vector<int> &r) {
+  r.clear();
+  while (n % 4 == 0) {
+    r.push_back(4);
+    n /= 4;
+  }
+  while (n % 2 == 0) {
+    r.push_back(2);
+    n /= 2;
+  }
+  while (n % 3 == 0) {
+    r.push_back(3);
+    n /= 3;
+  }
+  while (n % 5 == 0) {
+    r.push_back(5);
+    n /= 5;
+  }
+  if (n != 1) r.push_back(-n);  // unsupported factor
+}
+
+// ---------------------------------------------------------------------------
+int rbc_pme_init(rbc3d_ctx *c) {
+  if (c->pme_ready) return RBC3D_OK;
+  int N1 = c->conf.Nb[0], N2 = c->conf.Nb[1], N3 = c->conf.Nb[2], P = c->conf.P;
+  for (int ax = 0; ax < 3; ax++) {
+    std::vector<int> r;
+    factor_radices(c->conf.Nb[ax], r);
+    if (!r.empty() && r.back() < 0) {
+      rbc_set_error("PME grid sizes must factor into 2, 3 and 5");
+      return RBC3D_EINVAL;
+    }
+    if (c->conf.Nb[ax] > FFT_ELEMS) {
+      rbc_set_error("PME grid size too large");
+      return RBC3D_EINVAL;
+    }
+  }
+  int H = N1 / 2 + 1;
+  size_t ng = (size_t)N1 * N2 * N3, nc = (size_t)H * N2 * N3;
+  RBC_TRY(c->d_grid.ensure(12 * ng));
+  RBC_TRY(c->d_cgrid.ensure(2 * 15 * nc));
+  RBC_TRY(c->d_vgrid.ensure(3 * ng));
+  // B-factor per axis (PME_Init :309-336); bb = ((1*bx)*by)*bz, bb(0,0,0) = 0
+  double MP[16];
+  host_bspline(P + 2.220446049250313e-16, P, MP);
+  std::vector<double> fac((size_t)H + N2 + N3);
+  int Ns[3] = {N1, N2, N3}, cnt[3] = {H, N2, N3}, off = 0;
+  for (int ax = 0; ax < 3; ax++) {
+    for (int k = 0; k < cnt[ax]; k++) {
+      std::complex<double> b = 0, I(0, 1);
+      for (int m = 0; m <= P - 2; m++) b += MP[m] * std::exp(RBC_TWO_PI * I * (double)k * (double)m / (double)Ns[ax]);
+      b = std::exp(RBC_TWO_PI * I * (double)k * (P - 1.) / (double)Ns[ax]) / b;
+      double bv = std::abs(b);
+      fac[off + k] = bv * bv;
+    }
+    off += cnt[ax];
+  }
+  RBC_TRY(c->d_bb.ensure(fac.size()));
+  HIP_TRY(hipMemcpy(c->d_bb.p, fac.data(), fac.size() * 8, hipMemcpyHostToDevice));
+  // twiddles exp(-2 pi i m / N) for the three lengths
+  std::vector<double> tw;
+  for (int ax = 0; ax < 3; ax++) {
+    c->tw_off[ax] = (int)(tw.size() / 2);
+    int N = Ns[ax];
+    for (int m = 0; m < N; m++) {
+      long double ang = 2.0L * 3.141592653589793238462643383279502884L * m / N;
+      tw.push_back((double)cosl(ang));
+      tw.push_back((double)-sinl(ang));
+    }
+  }
+  RBC_TRY(c->d_tw.ensure(tw.size()));
+  HIP_TRY(hipMemcpy(c->d_tw.p, tw.data(), tw.size() * 8, hipMemcpyHostToDevice));
+  c->pme_ready = true;
+  return RBC3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// spread
+// record layout (doubles): [0..2] imin,jmin,kmin (as doubles) [3..3+3P) weights,
+// then nf field values (c1*f for single layer, c2*g_i*(a3_j Bcoef) for double layer)
+__global__ void k_spread_prep(int ns, int ncell, const double *__restrict__ x, int ldx,
+                              const double *__restrict__ fs, const double *__restrict__ gs,
+                              const double *__restrict__ a3, const double *__restrict__ Bn,
+                              const int *__restrict__ node_cell, const CellRec *__restrict__ cells,
+                              int nele, const double *__restrict__ wx, const double *__restrict__ wf,
+                              const int *__restrict__ e2v, int nvert, const double *__restrict__ warea,
+                              double ih0, double ih1, double ih2, int N1, int N2, int N3, int P,
+                              int flag_sl, int flag_dl, int nbx, int nby, int nbz, double *rec,
+                              int reclen, int *key, int *idx) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= ns) return;
+  double xs[3], f[3] = {0, 0, 0}, t[9];
+  for (int q = 0; q < 9; q++) t[q] = 0;
+  if (p < ncell) {
+    for (int d = 0; d < 3; d++) xs[d] = x[p + (size_t)ldx * d];
+    if (flag_sl)
+      for (int d = 0; d < 3; d++) f[d] = fs[p + (size_t)ldx * d];
+    if (flag_dl) {
+      double B = cells[node_cell[p]].Bcoef, g[3], n[3];
+      for (int d = 0; d < 3; d++) {
+        g[d] = gs[p + (size_t)ldx * d];
+        n[d] = a3[p + (size_t)ldx * d] * B;
+      }
+      for (int ii = 0; ii < 3; ii++)
+        for (int jj = 0; jj < 3; jj++) t[ii + 3 * jj] = g[ii] * n[jj];
+    }
+  } else {
+    // wall triangle centroid, f = mean vertex traction * area (PME_Distrib_Source :108-130)
+    int e = p - ncell;
+    double xe[3][3], fe[3][3];
+    for (int l = 0; l < 3; l++) {
+      int iv = e2v[e + (size_t)nele * l] - 1;
+      for (int d = 0; d < 3; d++) {
+        xe[l][d] = wx[iv + (size_t)nvert * d];
+        fe[l][d] = wf[iv + (size_t)nvert * d];
+      }
+    }
+    for (int d = 0; d < 3; d++) xs[d] = (1.0 / 3) * (xe[0][d] + xe[1][d] + xe[2][d]);
+    if (flag_sl)
+      for (int d = 0; d < 3; d++) f[d] = (1.0 / 3) * (fe[0][d] + fe[1][d] + fe[2][d]) * warea[e];
+  }
+  double *r = rec + (size_t)reclen * p;
+  double w[12];
+  int imin = bspline_d(xs[0] * ih0, P, w);
+  for (int q = 0; q < P; q++) r[3 + q] = w[q];
+  int jmin = bspline_d(xs[1] * ih1, P, w);
+  for (int q = 0; q < P; q++) r[3 + P + q] = w[q];
+  int kmin = bspline_d(xs[2] * ih2, P, w);
+  for (int q = 0; q < P; q++) r[3 + 2 * P + q] = w[q];
+  int i0 = imod_d(imin, N1), j0 = imod_d(jmin, N2), k0 = imod_d(kmin, N3);
+  r[0] = i0;
+  r[1] = j0;
+  r[2] = k0;
+  int o = 3 + 3 * P;
+  for (int d = 0; d < 3; d++) r[o + d] = f[d];
+  for (int q = 0; q < 9; q++) r[o + 3 + q] = t[q];
+  // brick of the stencil origin
+  key[p] = (i0 / SPR_B) + nbx * ((j0 / SPR_B) + nby * (k0 / SPR_B));
+  idx[p] = p;
+  (void)nbz;
+}
+
+struct SpreadArgs {
+  int N1, N2, N3, P, nbx, nby, nbz, reclen, ns;
+  double c1, c2;
+  int flag_sl, flag_dl;
+  const double *rec;
+  const int *order, *brick_start;
+  double *grid;  // 12 fields [f][k][j][i]
+};
+
+// One workgroup per brick of the grid.  A source with stencil origin g covers
+// g..g+P-1 (mod N), so brick [b0, b0+B) receives sources with origin in
+// [b0-P+1, b0+B-1] (mod N): the bricks that overlap that range on each axis.
+__global__ void __launch_bounds__(SPR_T) k_spread(SpreadArgs a) {
+  extern __shared__ double srec[];  // SPR_T records
+  int bid = blockIdx.x;
+  int bx = bid % a.nbx, by = (bid / a.nbx) % a.nby, bz = bid / (a.nbx * a.nby);
+  int ib0 = bx * SPR_B, jb0 = by * SPR_B, kb0 = bz * SPR_B;
+  int P = a.P;
+  // lists of source bricks per axis (deduplicated, at most 3 + wrap)
+  int lst[3][4], nl[3];
+  int Ns[3] = {a.N1, a.N2, a.N3}, b0s[3] = {ib0, jb0, kb0}, nbs[3] = {a.nbx, a.nby, a.nbz};
+  for (int ax = 0; ax < 3; ax++) {
+    nl[ax] = 0;
+    int lo = b0s[ax] - P + 1, hi = b0s[ax] + SPR_B - 1;
+    for (int g = lo; g <= hi; g++) {
+      int gg = imod_d(g, Ns[ax]);
+      int br = gg / SPR_B;
+      bool dup = false;
+      for (int q = 0; q < nl[ax]; q++) dup |= lst[ax][q] == br;
+      if (!dup && nl[ax] < 4) lst[ax][nl[ax]++] = br;
+    }
+  }
+  // my grid points (two per lane for B = 8)
+  const int npts = SPR_B * SPR_B * SPR_B;
+  double acc[2][12];
+  int gi[2], gj[2], gk[2];
+  bool valid[2];
+  for (int s = 0; s < 2; s++) {
+    int pt = threadIdx.x + s * SPR_T;
+    gi[s] = ib0 + pt % SPR_B;
+    gj[s] = jb0 + (pt / SPR_B) % SPR_B;
+    gk[s] = kb0 + pt / (SPR_B * SPR_B);
+    valid[s] = pt < npts && gi[s] < a.N1 && gj[s] < a.N2 && gk[s] < a.N3;
+    for (int f = 0; f < 12; f++) acc[s][f] = 0;
+  }
+  int rl = a.reclen;
+  for (int qz = 0; qz < nl[2]; qz++)
+    for (int qy = 0; qy < nl[1]; qy++)
+      for (int qx = 0; qx < nl[0]; qx++) {
+        int sb = lst[0][qx] + a.nbx * (lst[1][qy] + a.nby * lst[2][qz]);
+        int s0 = a.brick_start[sb], s1 = a.brick_start[sb + 1];
+        for (int base = s0; base < s1; base += SPR_T) {
+          int cnt = min(SPR_T, s1 - base);
+          __syncthreads();
+          for (int t = threadIdx.x; t < cnt * rl; t += SPR_T) {
+            int q = t / rl, e = t % rl;
+            srec[t] = a.rec[(size_t)rl * a.order[base + q] + e];
+          }
+          __syncthreads();
+          for (int q = 0; q < cnt; q++) {
+            const double *r = srec + q * rl;
+            int i0 = (int)r[0], j0 = (int)r[1], k0 = (int)r[2];
+            for (int s = 0; s < 2; s++) {
+              if (!valid[s]) continue;
+              int di = gi[s] - i0, dj = gj[s] - j0, dk = gk[s] - k0;
+              di += di < 0 ? a.N1 : 0;
+              dj += dj < 0 ? a.N2 : 0;
+              dk += dk < 0 ? a.N3 : 0;
+              if (di >= P || dj >= P || dk >= P) continue;
+              double wxyz = r[3 + di] * r[3 + P + dj] * r[3 + 2 * P + dk];
+              const double *fv = r + 3 + 3 * P;
+              if (a.flag_sl) {
+                double cw = a.c1 * wxyz;
+                for (int f = 0; f < 3; f++) acc[s][f] += cw * fv[f];
+              }
+              if (a.flag_dl) {
+                double cw = a.c2 * wxyz;
+                for (int f = 0; f < 9; f++) acc[s][3 + f] += cw * fv[3 + f];
+              }
+            }
+          }
+        }
+      }
+  size_t ng = (size_t)a.N1 * a.N2 * a.N3;
+  for (int s = 0; s < 2; s++) {
+    if (!valid[s]) continue;
+    size_t o = gi[s] + (size_t)a.N1 * (gj[s] + (size_t)a.N2 * gk[s]);
+    for (int f = 0; f < 12; f++) a.grid[o + ng * f] = acc[s][f];
+  }
+}
+
+// brick_start[b] = first sorted position with brick >= b, b = 0..nbr
+__global__ void k_brick_start(int n, const int *__restrict__ skey, int nbr, int *start) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nbr) return;
+  int lo = 0, hi = n;
+  while (lo < hi) {
+    int mid = (lo + hi) >> 1;
+    if (skey[mid] < b)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  start[b] = lo;
+}
+
+int rbc_pme_spread(rbc3d_ctx *c, double c1, double c2, int cells, int walls) {
+  hipStream_t st = c->stream;
+  int N1 = c->conf.Nb[0], N2 = c->conf.Nb[1], N3 = c->conf.Nb[2], P = c->conf.P;
+  c->flag_sl = fabs(c1) > 1.e-10;
+  c->flag_dl = fabs(c2) > 1.e-10;
+  int ncell = (cells && c->nrbc) ? c->np : 0;
+  int nele = (walls && c->wall.nvert) ? c->wall.nele : 0;
+  int ns = ncell + nele;
+  int nbx = (N1 + SPR_B - 1) / SPR_B, nby = (N2 + SPR_B - 1) / SPR_B, nbz = (N3 + SPR_B - 1) / SPR_B;
+  int nbr = nbx * nby * nbz;
+  int reclen = 3 + 3 * P + 12;
+  RBC_TRY(c->d_pme_src.ensure((size_t)reclen * ns + 1));
+  RBC_TRY(c->d_pme_key.ensure(2 * (size_t)ns + 2));
+  RBC_TRY(c->d_pme_order.ensure(2 * (size_t)ns + 2));
+  RBC_TRY(c->d_pme_start.ensure(nbr + 1));
+  int *key = c->d_pme_key.p, *key2 = c->d_pme_key.p + ns + 1;
+  int *idx = c->d_pme_order.p + ns + 1, *order = c->d_pme_order.p;
+  if (ns > 0) {
+    k_spread_prep<<<(ns + 255) / 256, 256, 0, st>>>(
+        ns, ncell, c->d_x.p, c->np, c->d_fs.p, c->d_gs.p, c->d_a3.p, nullptr, c->d_node_cell.p,
+        c->d_cells.p, nele, c->wall.d_x.p, c->wall.d_f.p, c->wall.d_e2v.p, c->wall.nvert,
+        c->wall.d_area.p, N1 / c->Lb[0], N2 / c->Lb[1], N3 / c->Lb[2], N1, N2, N3, P, c->flag_sl,
+        c->flag_dl, nbx, nby, nbz, c->d_pme_src.p, reclen, key, idx);
+    HIP_TRY(hipGetLastError());
+    int bits = 1;
+    while ((1 << bits) < nbr) bits++;
+    size_t tmp = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key, key2, idx, order, ns, 0, bits, st));
+    RBC_TRY(c->d_cub_tmp.ensure(tmp + 16));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->d_cub_tmp.p, tmp, key, key2, idx, order, ns, 0,
+                                               bits, st));
+  }
+  k_brick_start<<<(nbr + 1 + 255) / 256, 256, 0, st>>>(ns, key2, nbr, c->d_pme_start.p);
+  HIP_TRY(hipGetLastError());
+  SpreadArgs a;
+  a.N1 = N1;
+  a.N2 = N2;
+  a.N3 = N3;
+  a.P = P;
+  a.nbx = nbx;
+  a.nby = nby;
+  a.nbz = nbz;
+  a.reclen = reclen;
+  a.ns = ns;
+  a.c1 = c1;
+  a.c2 = c2;
+  a.flag_sl = c->flag_sl;
+  a.flag_dl = c->flag_dl;
+  a.rec = c->d_pme_src.p;
+  a.order = order;
+  a.brick_start = c->d_pme_start.p;
+  a.grid = c->d_grid.p;
+  size_t lds = sizeof(double) * SPR_T * reclen;
+  k_spread<<<nbr, SPR_T, lds, st>>>(a);
+  HIP_TRY(hipGetLastError());
+  return RBC3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// FFT: batches of lines, Stockham autosort in LDS.
+struct FftPlan {
+  int N, nrad, rad[16];
+  int sign;
+  const double2 *tw;  // exp(-2 pi i m/N)
+};
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 twid(const double2 *tw, int m, int N, int sign) {
+  double2 t = tw[m % N];
+  if (sign > 0) t.y = -t.y;
+  return t;
+}
+
+// FFT of TL lines of length N stored line-major in buf (line l at l*N); uses tmp.
+// Returns pointer to the buffer holding the result.
+__device__ double2 *fft_lds(double2 *buf, double2 *tmp, int TL, const FftPlan &pl) {
+  int N = pl.N, Ns = 1;
+  double2 *in = buf, *out = tmp;
+  for (int s = 0; s < pl.nrad; s++) {
+    int R = pl.rad[s], NR = N / R;
+    for (int t = threadIdx.x; t < TL * NR; t += blockDim.x) {
+      int l = t / NR, j = t % NR;
+      const double2 *src = in + (size_t)l * N;
+      double2 *dst = out + (size_t)l * N;
+      int k = j % Ns;
+      double2 v[5];
+      for (int r = 0; r < R; r++) {
+        double2 x = src[j + r * NR];
+        if (r && k) x = cmul(x, twid(pl.tw, r * k * (N / (Ns * R)), N, pl.sign));
+        v[r] = x;
+      }
+      int base = (j / Ns) * Ns * R + k;
+      if (R == 2) {
+        dst[base] = make_double2(v[0].x + v[1].x, v[0].y + v[1].y);
+        dst[base + Ns] = make_double2(v[0].x - v[1].x, v[0].y - v[1].y);
+      } else if (R == 4) {
+        double2 a0 = make_double2(v[0].x + v[2].x, v[0].y + v[2].y);
+        double2 a1 = make_double2(v[0].x - v[2].x, v[0].y - v[2].y);
+        double2 b0 = make_double2(v[1].x + v[3].x, v[1].y + v[3].y);
+        double2 b1 = make_double2(v[1].x - v[3].x, v[1].y - v[3].y);
+        // multiply b1 by (sign * i)
+        double2 jb1 = pl.sign < 0 ? make_double2(b1.y, -b1.x) : make_double2(-b1.y, b1.x);
+        dst[base] = make_double2(a0.x + b0.x, a0.y + b0.y);
+        dst[base + Ns] = make_double2(a1.x + jb1.x, a1.y + jb1.y);
+        dst[base + 2 * Ns] = make_double2(a0.x - b0.x, a0.y - b0.y);
+        dst[base + 3 * Ns] = make_double2(a1.x - jb1.x, a1.y - jb1.y);
+      } else {
+        for (int q = 0; q < R; q++) {
+          double2 sum = v[0];
+          for (int r = 1; r < R; r++) {
+            double2 w = twid(pl.tw, ((q * r) % R) * NR, N, pl.sign);
+            double2 p = cmul(v[r], w);
+            sum.x += p.x;
+            sum.y += p.y;
+          }
+          dst[base + q * Ns] = sum;
+        }
+      }
+    }
+    __syncthreads();
+    double2 *t2 = in;
+    in = out;
+    out = t2;
+    Ns *= R;
+  }
+  return in;
+}
+
+// complex lines: base(l) = (l % A)*sa + (l / A)*sb, element stride es
+struct LineMap {
+  int A;
+  long long sa, sb, es;
+  int nlines;
+};
+
+__global__ void __launch_bounds__(FFT_T) k_fft_c2c(double2 *data, FftPlan pl, LineMap lm, int TL) {
+  extern __shared__ double2 lds[];
+  double2 *buf = lds, *tmp = lds + (size_t)TL * pl.N;
+  int l0 = blockIdx.x * TL;
+  int N = pl.N;
+  // load: lines fastest (adjacent lines are adjacent in memory when sa = 1)
+  for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
+    int l = t % TL, e = t / TL, L = l0 + l;
+    if (L >= lm.nlines) continue;
+    long long o = (long long)(L % lm.A) * lm.sa + (long long)(L / lm.A) * lm.sb + (long long)e * lm.es;
+    buf[(size_t)l * N + e] = data[o];
+  }
+  __syncthreads();
+  double2 *res = fft_lds(buf, tmp, TL, pl);
+  for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
+    int l = t % TL, e = t / TL, L = l0 + l;
+    if (L >= lm.nlines) continue;
+    long long o = (long long)(L % lm.A) * lm.sa + (long long)(L / lm.A) * lm.sb + (long long)e * lm.es;
+    data[o] = res[(size_t)l * N + e];
+  }
+}
+
+// x r2c: real rows of N1 -> complex rows of H = N1/2+1
+__global__ void __launch_bounds__(FFT_T) k_fft_r2c(const double *in, double2 *out, FftPlan pl,
+                                                   int nrows, int TL) {
+  extern __shared__ double2 lds[];
+  double2 *buf = lds, *tmp = lds + (size_t)TL * pl.N;
+  int N = pl.N, H = N / 2 + 1, r0 = blockIdx.x * TL;
+  for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
+    int l = t / N, e = t % N, R = r0 + l;
+    if (R >= nrows) continue;
+    buf[t] = make_double2(in[(size_t)R * N + e], 0.);
+  }
+  __syncthreads();
+  double2 *res = fft_lds(buf, tmp, TL, pl);
+  for (int t = threadIdx.x; t < TL * H; t += blockDim.x) {
+    int l = t / H, e = t % H, R = r0 + l;
+    if (R >= nrows) continue;
+    out[(size_t)R * H + e] = res[(size_t)l * N + e];
+  }
+}
+
+// x c2r: complex rows of H -> real rows of N1 (real part of the zero and Nyquist
+// frequencies only, the halfcomplex convention of FFTW's c2r)
+__global__ void __launch_bounds__(FFT_T) k_fft_c2r(const double2 *in, double *out, FftPlan pl,
+                                                   int nrows, int TL) {
+  extern __shared__ double2 lds[];
+  double2 *buf = lds, *tmp = lds + (size_t)TL * pl.N;
+  int N = pl.N, H = N / 2 + 1, r0 = blockIdx.x * TL;
+  for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
+    int l = t / N, e = t % N, R = r0 + l;
+    if (R >= nrows) continue;
+    double2 v;
+    if (e == 0) {
+      v = make_double2(in[(size_t)R * H].x, 0.);
+    } else if (2 * e == N) {
+      v = make_double2(in[(size_t)R * H + e].x, 0.);
+    } else if (e < H) {
+      v = in[(size_t)R * H + e];
+    } else {
+      double2 w = in[(size_t)R * H + (N - e)];
+      v = make_double2(w.x, -w.y);
+    }
+    buf[t] = v;
+  }
+  __syncthreads();
+  double2 *res = fft_lds(buf, tmp, TL, pl);
+  for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
+    int l = t / N, e = t % N, R = r0 + l;
+    if (R >= nrows) continue;
+    out[(size_t)R * N + e] = res[(size_t)l * N + e].x;
+  }
+}
+
+static FftPlan make_plan(rbc3d_ctx *c, int ax, int sign) {
+  FftPlan p;
+  p.N = c->conf.Nb[ax];
+  std::vector<int> r;
+  factor_radices(p.N, r);
+  p.nrad = (int)r.size();
+  for (int i = 0; i < p.nrad; i++) p.rad[i] = r[i];
+  p.sign = sign;
+  p.tw = (const double2 *)(c->d_tw.p + 2 * (size_t)c->tw_off[ax]);
+  return p;
+}
+
+static int lines_per_block(int N) { return std::max(1, FFT_ELEMS / N); }
+
+static int fft_c2c(rbc3d_ctx *c, double2 *data, int ax, int sign, int nf) {
+  int N1 = c->conf.Nb[0], N2 = c->conf.Nb[1], N3 = c->conf.Nb[2], H = N1 / 2 + 1;
+  FftPlan pl = make_plan(c, ax, sign);
+  LineMap lm;
+  if (ax == 1) {  // lines over (kx, kz, f), stride H
+    lm.A = H;
+    lm.sa = 1;
+    lm.sb = (long long)H * N2;
+    lm.es = H;
+    lm.nlines = H * N3 * nf;
+  } else {  // ax == 2: lines over (kx, ky, f), stride H*N2
+    lm.A = H * N2;
+    lm.sa = 1;
+    lm.sb = (long long)H * N2 * N3;
+    lm.es = (long long)H * N2;
+    lm.nlines = H * N2 * nf;
+  }
+  int TL = lines_per_block(pl.N);
+  size_t lds = 2 * sizeof(double2) * TL * pl.N;
+  k_fft_c2c<<<(lm.nlines + TL - 1) / TL, FFT_T, lds, c->stream>>>(data, pl, lm, TL);
+  HIP_TRY(hipGetLastError());
+  return RBC3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Green's function multiply (PME_Transform :159-210), output fields 12..14
+struct GreenArgs {
+  int N1, N2, N3, H;
+  double iLb[3], alpha, vol;
+  int flag_sl, flag_dl;
+  const double *bb;  // bx(H), by(N2), bz(N3)
+  double2 *C;        // 15 complex fields
+};
+
+__global__ void k_green(GreenArgs a) {
+  size_t nc = (size_t)a.H * a.N2 * a.N3;
+  size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (o >= nc) return;
+  int i = (int)(o % a.H), j = (int)((o / a.H) % a.N2), k = (int)(o / ((size_t)a.H * a.N2));
+  double2 vc[3] = {{0, 0}, {0, 0}, {0, 0}};
+  if (o != 0) {
+    double q[3];
+    q[0] = i * a.iLb[0];
+    q[1] = (j < a.N2 / 2) ? j * a.iLb[1] : (j - a.N2) * a.iLb[1];
+    double q3 = (k < a.N3 / 2) ? k * a.iLb[2] : (k - a.N3) * a.iLb[2];
+    q[2] = -q3;
+    double spa = sqrt(RBC_PI * a.alpha), qt[3];
+    for (int d = 0; d < 3; d++) qt[d] = spa * q[d];
+    double q2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+    double q2t = RBC_PI * a.alpha * q2, e = exp(-q2t);
+    double phi0 = e / q2t, phi1 = (e + phi0) / q2t;
+    if (a.flag_sl) {
+      double2 f[3];
+      for (int d = 0; d < 3; d++) f[d] = a.C[o + nc * d];
+      double2 qf = make_double2(qt[0] * f[0].x + qt[1] * f[1].x + qt[2] * f[2].x,
+                                qt[0] * f[0].y + qt[1] * f[1].y + qt[2] * f[2].y);
+      double s = 2 * a.alpha / a.vol * phi1;
+      for (int d = 0; d < 3; d++) {
+        vc[d].x += s * (q2t * f[d].x - qt[d] * qf.x);
+        vc[d].y += s * (q2t * f[d].y - qt[d] * qf.y);
+      }
+    }
+    if (a.flag_dl) {
+      double2 T[3][3];
+      for (int ii = 0; ii < 3; ii++)
+        for (int jj = 0; jj < 3; jj++) T[ii][jj] = a.C[o + nc * (3 + ii + 3 * jj)];
+      double2 tr = make_double2(T[0][0].x + T[1][1].x + T[2][2].x, T[0][0].y + T[1][1].y + T[2][2].y);
+      double2 qT[3], Tq[3];
+      for (int jj = 0; jj < 3; jj++)
+        qT[jj] = make_double2(q[0] * T[0][jj].x + q[1] * T[1][jj].x + q[2] * T[2][jj].x,
+                              q[0] * T[0][jj].y + q[1] * T[1][jj].y + q[2] * T[2][jj].y);
+      for (int ii = 0; ii < 3; ii++)
+        Tq[ii] = make_double2(T[ii][0].x * q[0] + T[ii][1].x * q[1] + T[ii][2].x * q[2],
+                              T[ii][0].y * q[0] + T[ii][1].y * q[1] + T[ii][2].y * q[2]);
+      double2 qTq = make_double2(q[0] * Tq[0].x + q[1] * Tq[1].x + q[2] * Tq[2].x,
+                                 q[0] * Tq[0].y + q[1] * Tq[1].y + q[2] * Tq[2].y);
+      double s0 = 4 * RBC_PI * a.alpha / a.vol * phi0;
+      double s1 = 8 * RBC_PI * RBC_PI * a.alpha * a.alpha / a.vol * phi1;
+      for (int d = 0; d < 3; d++) {
+        // u = q tr + qT + Tq;  dv = i s0 u - i s1 qTq q;  vc -= dv
+        double ux = q[d] * tr.x + qT[d].x + Tq[d].x, uy = q[d] * tr.y + qT[d].y + Tq[d].y;
+        double wx = qTq.x * q[d], wy = qTq.y * q[d];
+        // i*s0*u = (-s0*uy, s0*ux); i*s1*w = (-s1*wy, s1*wx)
+        double dvx = -s0 * uy - (-s1 * wy), dvy = s0 * ux - s1 * wx;
+        vc[d].x -= dvx;
+        vc[d].y -= dvy;
+      }
+    }
+    double b = ((1.0 * a.bb[i]) * a.bb[a.H + j]) * a.bb[a.H + a.N2 + k];
+    for (int d = 0; d < 3; d++) {
+      vc[d].x *= b;
+      vc[d].y *= b;
+    }
+  }
+  for (int d = 0; d < 3; d++) a.C[o + nc * (12 + d)] = vc[d];
+}
+
+int rbc_pme_transform(rbc3d_ctx *c) {
+  hipStream_t st = c->stream;
+  int N1 = c->conf.Nb[0], N2 = c->conf.Nb[1], N3 = c->conf.Nb[2], H = N1 / 2 + 1;
+  size_t ng = (size_t)N1 * N2 * N3, nc = (size_t)H * N2 * N3;
+  double2 *C = (double2 *)c->d_cgrid.p;
+  int f0 = c->flag_sl ? 0 : 3, f1 = c->flag_dl ? 12 : 3;
+  int nf = f1 - f0;
+  if (nf > 0) {
+    FftPlan px = make_plan(c, 0, -1);
+    int TL = lines_per_block(N1), nrows = N2 * N3 * nf;
+    size_t lds = 2 * sizeof(double2) * TL * N1;
+    k_fft_r2c<<<(nrows + TL - 1) / TL, FFT_T, lds, st>>>(c->d_grid.p + ng * f0, C + nc * f0, px, nrows,
+                                                         TL);
+    HIP_TRY(hipGetLastError());
+    RBC_TRY(fft_c2c(c, C + nc * f0, 1, -1, nf));
+    RBC_TRY(fft_c2c(c, C + nc * f0, 2, +1, nf));
+  }
+  GreenArgs g;
+  g.N1 = N1;
+  g.N2 = N2;
+  g.N3 = N3;
+  g.H = H;
+  for (int d = 0; d < 3; d++) g.iLb[d] = c->iLb[d];
+  g.alpha = c->conf.alpha;
+  g.vol = c->Lb[0] * c->Lb[1] * c->Lb[2];
+  g.flag_sl = c->flag_sl;
+  g.flag_dl = c->flag_dl;
+  g.bb = c->d_bb.p;
+  g.C = C;
+  k_green<<<(unsigned)((nc + 255) / 256), 256, 0, st>>>(g);
+  HIP_TRY(hipGetLastError());
+  RBC_TRY(fft_c2c(c, C + nc * 12, 2, -1, 3));
+  RBC_TRY(fft_c2c(c, C + nc * 12, 1, +1, 3));
+  FftPlan pxi = make_plan(c, 0, +1);
+  int TL = lines_per_block(N1), nrows = N2 * N3 * 3;
+  size_t lds = 2 * sizeof(double2) * TL * N1;
+  k_fft_c2r<<<(nrows + TL - 1) / TL, FFT_T, lds, st>>>(C + nc * 12, c->d_vgrid.p, pxi, nrows, TL);
+  HIP_TRY(hipGetLastError());
+  return RBC3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// gather: Interp_Vel (:450-489) + PME_Add_Interp_Vel (:228-248)
+struct GatherArgs {
+  int N1, N2, N3, P, nT;
+  double ih[3];
+  const double *vv, *tx, *tA;
+  const int *tact;
+  double *v;
+};
+
+__global__ void __launch_bounds__(256) k_gather(GatherArgs a) {
+  int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ti >= a.nT || !a.tact[ti]) return;
+  double x[3] = {a.tx[ti], a.tx[ti + a.nT], a.tx[ti + 2 * (size_t)a.nT]};
+  double wx[12], wy[12], wz[12];
+  int P = a.P;
+  int kmin = bspline_d(x[2] * a.ih[2], P, wz);
+  int jmin = bspline_d(x[1] * a.ih[1], P, wy);
+  int imin = bspline_d(x[0] * a.ih[0], P, wx);
+  size_t ng = (size_t)a.N1 * a.N2 * a.N3;
+  double v0 = 0, v1 = 0, v2 = 0;
+  for (int k0 = 0; k0 < P; k0++) {
+    int k = imod_d(kmin + k0, a.N3);
+    for (int j0 = 0; j0 < P; j0++) {
+      int j = imod_d(jmin + j0, a.N2);
+      for (int i0 = 0; i0 < P; i0++) {
+        int i = imod_d(imin + i0, a.N1);
+        double w = wx[i0] * wy[j0] * wz[k0];
+        size_t o = i + (size_t)a.N1 * (j + (size_t)a.N2 * k);
+        v0 += w * a.vv[o];
+        v1 += w * a.vv[o + ng];
+        v2 += w * a.vv[o + 2 * ng];
+      }
+    }
+  }
+  double A = a.tA[ti];
+  a.v[ti] += v0 / A;
+  a.v[ti + (size_t)a.nT] += v1 / A;
+  a.v[ti + 2 * (size_t)a.nT] += v2 / A;
+}
+
+int rbc_pme_gather(rbc3d_ctx *c, int nT, const double *d_tx, const int *d_act, const double *d_A,
+                   double *d_v) {
+  GatherArgs g;
+  g.N1 = c->conf.Nb[0];
+  g.N2 = c->conf.Nb[1];
+  g.N3 = c->conf.Nb[2];
+  g.P = c->conf.P;
+  g.nT = nT;
+  for (int d = 0; d < 3; d++) g.ih[d] = c->conf.Nb[d] / c->Lb[d];
+  g.vv = c->d_vgrid.p;
+  g.tx = d_tx;
+  g.tA = d_A;
+  g.tact = d_act;
+  g.v = d_v;
+  k_gather<<<(nT + 255) / 256, 256, 0, c->stream>>>(g);
+  HIP_TRY(hipGetLastError());
+  return RBC3D_OK;
+}

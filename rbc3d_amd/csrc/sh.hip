@@ -1,0 +1,483 @@
+// Spherical-harmonic transforms on the cell grids (SPHEREPACK conventions, as
+// used through common/ModSphpk.F90), the per-cell surface splines built from
+// them (Rbc_BuildSurfaceSource, common/ModRbc.F90:716-807; Spline_Build_on_Sphere,
+// common/ModSpline.F90:121-142; FFT_Diff, common/ModFFT.F90:25-93) and the global
+// packing of Glob_Sph_Trans (common/ModVelSolver.F90:641-719).
+//
+// One workgroup per cell.  A cell's data is a few KB, so each stage (longitude
+// DFT, Legendre sums, equi-spaced synthesis, derivative operators) runs out of LDS
+// as small dense contractions with precomputed tables.
+#include <algorithm>
+#include <cmath>
+
+#include "internal.h"
+
+#define SH_THREADS 256
+
+// C[m][i] = sum_j f[i + ld*j] cos(m phi_j),  S[m][i] = sum_j f sin(m phi_j)
+__device__ void lon_dft(const double *f, int ld, int nl, int nn, int mmax, const double *cs,
+                        int cs_ld_m, double *C, double *S) {
+  const double *cosT = cs, *sinT = cs + (size_t)cs_ld_m * nn;
+  for (int t = threadIdx.x; t < mmax * nl; t += blockDim.x) {
+    int m = t / nl, i = t % nl;
+    double a = 0, b = 0;
+    for (int j = 0; j < nn; j++) {
+      double v = f[i + ld * j];
+      a += v * cosT[m * nn + j];
+      b += v * sinT[m * nn + j];
+    }
+    C[t] = a;
+    S[t] = b;
+  }
+}
+
+// a[m + lda*n] = sum_i pw[m][n][i] C[m][i];  b = -sum_i pw S   (n in [m, nmax))
+__device__ void leg_anal(const double *C, const double *S, const double *pw, int nl, int mmax,
+                         int nmax, double *a, double *b, int lda) {
+  for (int t = threadIdx.x; t < mmax * nmax; t += blockDim.x) {
+    int m = t / nmax, n = t % nmax;
+    double sa = 0, sb = 0;
+    if (n >= m) {
+      const double *p = pw + ((size_t)m * nl + n) * nl;
+      for (int i = 0; i < nl; i++) {
+        sa += p[i] * C[m * nl + i];
+        sb += p[i] * S[m * nl + i];
+      }
+    }
+    a[m + lda * n] = sa;
+    b[m + lda * n] = -sb;
+  }
+}
+
+// G[m][i] = sum_{n=m}^{nmax-1} pt[m][n][i] a[m + lda n]  (pt includes the 1/2 at m = 0)
+__device__ void leg_synth(const double *a, const double *b, int lda, const double *pt, int nlt,
+                          int nrow, int mmax, int nmax, double *Ga, double *Gb) {
+  for (int t = threadIdx.x; t < mmax * nrow; t += blockDim.x) {
+    int m = t / nrow, i = t % nrow;
+    double sa = 0, sb = 0;
+    for (int n = m; n < nmax; n++) {
+      double p = pt[((size_t)m * nlt + n) * nrow + i];
+      sa += p * a[m + lda * n];
+      sb += p * b[m + lda * n];
+    }
+    Ga[t] = sa;
+    Gb[t] = sb;
+  }
+}
+
+// g[i + ld*j] = sum_m Ga[m][i] cos(m phi_j) - Gb[m][i] sin(m phi_j)
+__device__ void lon_synth(const double *Ga, const double *Gb, int nrow, int nn, int mmax,
+                          const double *cs, int cs_ld_m, double *g, int ld) {
+  const double *cosT = cs, *sinT = cs + (size_t)cs_ld_m * nn;
+  for (int t = threadIdx.x; t < nrow * nn; t += blockDim.x) {
+    int i = t % nrow, j = t / nrow;
+    double v = 0;
+    for (int m = 0; m < mmax; m++) v += Ga[m * nrow + i] * cosT[m * nn + j] - Gb[m * nrow + i] * sinT[m * nn + j];
+    g[i + ld * j] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Surface splines of a cell: which = 0 -> x, a3, detJ; 1 -> f detJ; 2 -> g detJ
+__global__ void __launch_bounds__(SH_THREADS) k_build_splines(
+    const CellRec *__restrict__ cells, int cell0, int gtype, const double *__restrict__ pw,
+    const double *__restrict__ pe, const double *__restrict__ cs, const double *__restrict__ Dm,
+    const double *__restrict__ Dn, const double *__restrict__ x, const double *__restrict__ a3,
+    const double *__restrict__ detj, const double *__restrict__ fd, int np, int which,
+    double *__restrict__ spl, const int *__restrict__ cell_list) {
+  extern __shared__ double sm[];
+  int ci = cell_list[cell0 + blockIdx.x];
+  const CellRec cr = cells[ci];
+  if (cr.gtype != gtype) return;
+  int nl = cr.nlat, nn = cr.nlon, n0 = cr.nlat0, ne = nl + 1, m2 = 2 * nl;
+  int mmax = min(nl, nn / 2 + 1);
+  int R1 = ne * nn, R2 = 2 * ne * nl;
+  double *r1 = sm, *r2 = sm + R1, *u = r2 + R2, *u1 = u + m2 * nn;
+  int nvar = which == 0 ? 7 : 3;
+  for (int v = 0; v < nvar; v++) {
+    // field values on the Gauss grid
+    const double *src;
+    int comp, var;
+    bool mul = false;
+    if (which == 0) {
+      if (v < 3) {
+        src = x;
+        comp = v;
+        var = SPL_X + v;
+      } else if (v < 6) {
+        src = a3;
+        comp = v - 3;
+        var = SPL_A3 + v - 3;
+      } else {
+        src = detj;
+        comp = 0;
+        var = SPL_DETJ;
+      }
+    } else {
+      src = fd;
+      comp = v;
+      var = (which == 1 ? SPL_F : SPL_G) + v;
+      mul = true;
+    }
+    for (int t = threadIdx.x; t < nl * nn; t += blockDim.x) {
+      double val = src[cr.off + t + (size_t)np * comp];
+      if (mul) val *= detj[cr.off + t];
+      r1[t] = val;  // (ilat, ilon) at ilat + nl*ilon
+    }
+    __syncthreads();
+    // analysis: C,S in r2
+    lon_dft(r1, nl, nl, nn, mmax, cs, nl + 1, r2, r2 + mmax * nl);
+    __syncthreads();
+    // a, b (n < nlat0: ShFilter) in r1, lda = n0
+    leg_anal(r2, r2 + mmax * nl, pw, nl, min(mmax, n0), n0, r1, r1 + n0 * n0, n0);
+    __syncthreads();
+    // equi-spaced synthesis on nl+1 rows: Ga, Gb in r2
+    int mme = min(n0, min(ne, nn / 2 + 1));
+    leg_synth(r1, r1 + n0 * n0, n0, pe, ne, ne, mme, n0, r2, r2 + mme * ne);
+    __syncthreads();
+    lon_synth(r2, r2 + mme * ne, ne, nn, mme, cs, nl + 1, r1, ne);  // g_e[ie + ne*j] in r1
+    __syncthreads();
+    // doubled sphere: u(i, j), i < 2 nl
+    for (int t = threadIdx.x; t < m2 * nn; t += blockDim.x) {
+      int i = t % m2, j = t / m2;
+      u[t] = i < nl ? r1[i + ne * j] : r1[(m2 - i) + ne * ((j + nn / 2) % nn)];
+    }
+    __syncthreads();
+    // u1 = D_m u (first index)
+    for (int t = threadIdx.x; t < m2 * nn; t += blockDim.x) {
+      int i = t % m2, j = t / m2;
+      double s = 0;
+      for (int q = 0; q < m2; q++) s += Dm[i * m2 + q] * u[q + m2 * j];
+      u1[t] = s;
+    }
+    __syncthreads();
+    // u2 = derivative of u along j, u12 = derivative of u1 along j; write block
+    double *out = spl + cr.spl + (size_t)var * m2 * nn * 4;
+    for (int t = threadIdx.x; t < m2 * nn; t += blockDim.x) {
+      int i = t % m2, j = t / m2;
+      double s2 = 0, s12 = 0;
+      for (int q = 0; q < nn; q++) {
+        double d = Dn[j * nn + q];
+        s2 += d * u[i + m2 * q];
+        s12 += d * u1[i + m2 * q];
+      }
+      double4 o;
+      o.x = u[t];
+      o.y = u1[t];
+      o.z = s2;
+      o.w = s12;
+      *(double4 *)(out + 4 * (size_t)t) = o;
+    }
+    __syncthreads();
+  }
+}
+
+static size_t spline_lds(int nl) {
+  int ne = nl + 1, nn = 2 * nl;
+  return sizeof(double) * ((size_t)ne * nn + 2 * (size_t)ne * nl + 2 * (size_t)(2 * nl) * nn);
+}
+
+// grouped launches: one per grid type (cells of one type share the tables)
+static int cells_of_type(rbc3d_ctx *c, int gt, std::vector<int> &ids) {
+  ids.clear();
+  for (int i = 0; i < c->nrbc; i++)
+    if (c->cells[i].gtype == gt) ids.push_back(i);
+  return (int)ids.size();
+}
+
+static int upload_ids(rbc3d_ctx *c) {
+  // d_tmp_i holds, per grid type, the list of cells of that type (concatenated)
+  std::vector<int> all;
+  std::vector<int> ids;
+  for (size_t gt = 0; gt < c->gtypes.size(); gt++) {
+    cells_of_type(c, (int)gt, ids);
+    all.insert(all.end(), ids.begin(), ids.end());
+  }
+  RBC_TRY(c->d_tmp_i.ensure(all.size() + 1));
+  if (!all.empty())
+    HIP_TRY(hipMemcpyAsync(c->d_tmp_i.p, all.data(), all.size() * sizeof(int), hipMemcpyHostToDevice,
+                           c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RBC3D_OK;
+}
+
+int rbc_sh_build_splines(rbc3d_ctx *c, int which) {
+  if (c->nrbc == 0) return RBC3D_OK;
+  if (which == 0) RBC_TRY(upload_ids(c));
+  std::vector<int> ids;
+  int base = 0;
+  for (size_t gt = 0; gt < c->gtypes.size(); gt++) {
+    int n = cells_of_type(c, (int)gt, ids);
+    if (!n) continue;
+    GridType *g = c->gtypes[gt];
+    size_t lds = spline_lds(g->nlat);
+    const double *fd = which == 1 ? c->d_f.p : which == 2 ? c->d_g.p : nullptr;
+    hipLaunchKernelGGL(k_build_splines, dim3(n), dim3(SH_THREADS), lds, c->stream, c->d_cells.p,
+                       base, (int)gt, g->d_pw.p, g->d_pe.p, g->d_cs.p, g->d_Dm.p, g->d_Dn.p,
+                       c->d_x.p, c->d_a3.p, c->d_detj.p, fd, c->np, which, c->d_spl.p,
+                       c->d_tmp_i.p);
+    HIP_TRY(hipGetLastError());
+    base += n;
+  }
+  return RBC3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Glob_Sph_Trans: analysis (v -> packed coefficients) and synthesis.
+__device__ __forceinline__ int pack_a(int m, int n, int n0) {
+  // a(m,n), m = 0..n0-1, n = m..n0-1, in (m outer, n inner) order
+  return m * n0 - m * (m - 1) / 2 + (n - m);
+}
+__device__ __forceinline__ int pack_b(int m, int n, int n0) {
+  int nb0 = n0 * (n0 + 1) / 2;
+  int mm = m - 1;  // m = 1..n0-1
+  return nb0 + mm * (n0 - 1) - mm * (mm - 1) / 2 + (n - m);
+}
+
+__global__ void __launch_bounds__(SH_THREADS) k_glob_anal(
+    const CellRec *__restrict__ cells, int cell0, int gtype, const double *__restrict__ pw,
+    const double *__restrict__ cs, const double *__restrict__ v, int np, double *__restrict__ cf,
+    const int *__restrict__ cell_list) {
+  extern __shared__ double sm[];
+  int ci = cell_list[cell0 + blockIdx.x];
+  const CellRec cr = cells[ci];
+  if (cr.gtype != gtype) return;
+  int nl = cr.nlat, nn = cr.nlon, n0 = cr.nlat0;
+  int mmax = min(nl, nn / 2 + 1);
+  double *f = sm, *C = f + nl * nn, *a = C + 2 * mmax * nl;
+  for (int d = 0; d < 3; d++) {
+    for (int t = threadIdx.x; t < nl * nn; t += blockDim.x) f[t] = v[cr.off + t + (size_t)np * d];
+    __syncthreads();
+    lon_dft(f, nl, nl, nn, mmax, cs, nl + 1, C, C + mmax * nl);
+    __syncthreads();
+    leg_anal(C, C + mmax * nl, pw, nl, min(mmax, n0), n0, a, a + n0 * n0, n0);
+    __syncthreads();
+    for (int t = threadIdx.x; t < n0 * n0; t += blockDim.x) {
+      int m = t % n0, n = t / n0;
+      if (n < m) continue;
+      cf[cr.coff + 3 * pack_a(m, n, n0) + d] = a[m + n0 * n];
+      if (m >= 1) cf[cr.coff + 3 * pack_b(m, n, n0) + d] = a[n0 * n0 + m + n0 * n];
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void __launch_bounds__(SH_THREADS) k_glob_synth(
+    const CellRec *__restrict__ cells, int cell0, int gtype, const double *__restrict__ pg,
+    const double *__restrict__ cs, const double *__restrict__ cf, int np, double *__restrict__ v,
+    const int *__restrict__ cell_list) {
+  extern __shared__ double sm[];
+  int ci = cell_list[cell0 + blockIdx.x];
+  const CellRec cr = cells[ci];
+  if (cr.gtype != gtype) return;
+  int nl = cr.nlat, nn = cr.nlon, n0 = cr.nlat0;
+  int mm = min(n0, min(nl, nn / 2 + 1));
+  double *a = sm, *G = a + 2 * n0 * n0, *g = G + 2 * mm * nl;
+  for (int d = 0; d < 3; d++) {
+    for (int t = threadIdx.x; t < n0 * n0; t += blockDim.x) {
+      int m = t % n0, n = t / n0;
+      a[t] = (n >= m) ? cf[cr.coff + 3 * pack_a(m, n, n0) + d] : 0.;
+      a[n0 * n0 + t] = (n >= m && m >= 1) ? cf[cr.coff + 3 * pack_b(m, n, n0) + d] : 0.;
+    }
+    __syncthreads();
+    leg_synth(a, a + n0 * n0, n0, pg, nl, nl, mm, n0, G, G + mm * nl);
+    __syncthreads();
+    lon_synth(G, G + mm * nl, nl, nn, mm, cs, nl + 1, g, nl);
+    __syncthreads();
+    for (int t = threadIdx.x; t < nl * nn; t += blockDim.x) v[cr.off + t + (size_t)np * d] = g[t];
+    __syncthreads();
+  }
+}
+
+int rbc_sh_glob(rbc3d_ctx *c, const double *d_in, double *d_out, int direction) {
+  std::vector<int> ids;
+  int base = 0;
+  for (size_t gt = 0; gt < c->gtypes.size(); gt++) {
+    int n = cells_of_type(c, (int)gt, ids);
+    if (!n) continue;
+    GridType *g = c->gtypes[gt];
+    int nl = g->nlat, nn = g->nlon;
+    size_t lds = sizeof(double) * ((size_t)nl * nn + 2 * (size_t)nl * nl + 2 * (size_t)nl * nl + 16);
+    if (direction == 1)
+      hipLaunchKernelGGL(k_glob_anal, dim3(n), dim3(SH_THREADS), lds, c->stream, c->d_cells.p, base,
+                         (int)gt, g->d_pw.p, g->d_cs.p, d_in, c->np, d_out, c->d_tmp_i.p);
+    else
+      hipLaunchKernelGGL(k_glob_synth, dim3(n), dim3(SH_THREADS), lds, c->stream, c->d_cells.p,
+                         base, (int)gt, g->d_pg.p, g->d_cs.p, d_in, c->np, d_out, c->d_tmp_i.p);
+    HIP_TRY(hipGetLastError());
+    base += n;
+  }
+  return RBC3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Standalone ShAnalGau / ShSynthGau / ShSynthEqu on nt fields with the reference's
+// leading dimensions (general nlat, nlon).  Tables are built per call.
+static void host_gauss(int n, std::vector<double> &th, std::vector<double> &w) {
+  th.resize(n);
+  w.resize(n);
+  auto leg = [n](double x, double &dp) {
+    double p0 = 1, p1 = x;
+    for (int k = 2; k <= n; k++) {
+      double p2 = ((2 * k - 1) * x * p1 - (k - 1) * p0) / k;
+      p0 = p1;
+      p1 = p2;
+    }
+    dp = n * (x * p1 - p0) / (x * x - 1);
+    return p1;
+  };
+  for (int k = 0; k < n; k++) {
+    double t = RBC_PI * (k + 0.75) / (n + 0.5), dp, p;
+    for (int it = 0; it < 100; it++) {
+      p = leg(cos(t), dp);
+      double dt = p / (-sin(t) * dp);
+      t -= dt;
+      if (fabs(dt) < 1e-17) break;
+    }
+    p = leg(cos(t), dp);
+    t -= p / (-sin(t) * dp);
+    p = leg(cos(t), dp);
+    th[k] = t;
+    double st = sin(t);
+    w[k] = 2. / (st * st * dp * dp);
+  }
+}
+
+static void host_pbar(int nl, int mmax, const std::vector<double> &th, std::vector<double> &out) {
+  int nt = (int)th.size();
+  out.assign((size_t)mmax * nl * nt, 0.);
+  for (int i = 0; i < nt; i++) {
+    double c = cos(th[i]), s = sin(th[i]), pmm = sqrt(0.5);
+    for (int m = 0; m < mmax; m++) {
+      if (m > 0) pmm = sqrt((2.0 * m + 1) / (2.0 * m)) * s * pmm;
+      if (m >= nl) break;
+      out[((size_t)m * nl + m) * nt + i] = pmm;
+      if (m + 1 < nl) {
+        double pa = pmm, pb = sqrt(2.0 * m + 3) * c * pmm;
+        out[((size_t)m * nl + m + 1) * nt + i] = pb;
+        for (int n = m + 2; n < nl; n++) {
+          double nn = n, mm = m;
+          double an = sqrt((4 * nn * nn - 1) / (nn * nn - mm * mm));
+          double bn = sqrt(((2 * nn + 1) * ((nn - 1) * (nn - 1) - mm * mm)) /
+                           ((2 * nn - 3) * (nn * nn - mm * mm)));
+          double pn = an * c * pb - bn * pa;
+          out[((size_t)m * nl + n) * nt + i] = pn;
+          pa = pb;
+          pb = pn;
+        }
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(SH_THREADS) k_anal_batch(int nl, int nn, int mmax,
+                                                         const double *__restrict__ pw,
+                                                         const double *__restrict__ cs,
+                                                         const double *__restrict__ g, int idg,
+                                                         int jdg, double *a, double *b, int ida,
+                                                         int jda) {
+  extern __shared__ double sm[];
+  int k = blockIdx.x;
+  double *f = sm, *C = f + nl * nn, *ab = C + 2 * mmax * nl;
+  for (int t = threadIdx.x; t < nl * nn; t += blockDim.x) {
+    int i = t % nl, j = t / nl;
+    f[t] = g[(size_t)k * idg * jdg + i + (size_t)idg * j];
+  }
+  __syncthreads();
+  lon_dft(f, nl, nl, nn, mmax, cs, mmax, C, C + mmax * nl);
+  __syncthreads();
+  leg_anal(C, C + mmax * nl, pw, nl, mmax, nl, ab, ab + mmax * nl, mmax);
+  __syncthreads();
+  for (int t = threadIdx.x; t < mmax * nl; t += blockDim.x) {
+    int m = t % mmax, n = t / mmax;
+    if (n < m) continue;
+    a[(size_t)k * ida * jda + m + (size_t)ida * n] = ab[t];
+    b[(size_t)k * ida * jda + m + (size_t)ida * n] = ab[mmax * nl + t];
+  }
+}
+
+__global__ void __launch_bounds__(SH_THREADS) k_synth_batch(int nl, int nn, int mmax, int nmax,
+                                                          const double *__restrict__ pt,
+                                                          const double *__restrict__ cs, double *g,
+                                                          int idg, int jdg, const double *a,
+                                                          const double *b, int ida, int jda) {
+  extern __shared__ double sm[];
+  int k = blockIdx.x;
+  double *ab = sm, *G = ab + 2 * mmax * nmax, *gg = G + 2 * mmax * nl;
+  for (int t = threadIdx.x; t < mmax * nmax; t += blockDim.x) {
+    int m = t % mmax, n = t / mmax;
+    ab[t] = n >= m ? a[(size_t)k * ida * jda + m + (size_t)ida * n] : 0.;
+    ab[mmax * nmax + t] = n >= m ? b[(size_t)k * ida * jda + m + (size_t)ida * n] : 0.;
+  }
+  __syncthreads();
+  leg_synth(ab, ab + mmax * nmax, mmax, pt, nl, nl, mmax, nmax, G, G + mmax * nl);
+  __syncthreads();
+  lon_synth(G, G + mmax * nl, nl, nn, mmax, cs, mmax, gg, nl);
+  __syncthreads();
+  for (int t = threadIdx.x; t < nl * nn; t += blockDim.x) {
+    int i = t % nl, j = t / nl;
+    g[(size_t)k * idg * jdg + i + (size_t)idg * j] = gg[t];
+  }
+}
+
+static int batch_tables(int nl, int nn, int mmax, bool equi, bool anal, double **d_p, double **d_cs) {
+  std::vector<double> th, w, P;
+  if (equi) {
+    th.resize(nl);
+    for (int i = 0; i < nl; i++) th[i] = RBC_PI * i / (nl - 1);
+  } else {
+    host_gauss(nl, th, w);
+  }
+  host_pbar(nl, mmax, th, P);
+  for (int m = 0; m < mmax; m++)
+    for (int n = 0; n < nl; n++)
+      for (int i = 0; i < nl; i++) {
+        size_t o = ((size_t)m * nl + n) * nl + i;
+        P[o] = anal ? (2.0 / nn) * w[i] * P[o] : (m == 0 ? 0.5 : 1.0) * P[o];
+      }
+  std::vector<double> cs((size_t)2 * mmax * nn);
+  for (int m = 0; m < mmax; m++)
+    for (int j = 0; j < nn; j++) {
+      double ph = RBC_TWO_PI * j / nn;
+      cs[(size_t)m * nn + j] = cos(m * ph);
+      cs[(size_t)mmax * nn + m * nn + j] = sin(m * ph);
+    }
+  HIP_TRY(hipMalloc(d_p, P.size() * 8));
+  HIP_TRY(hipMalloc(d_cs, cs.size() * 8));
+  HIP_TRY(hipMemcpy(*d_p, P.data(), P.size() * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(*d_cs, cs.data(), cs.size() * 8, hipMemcpyHostToDevice));
+  return RBC3D_OK;
+}
+
+int rbc_sh_anal_batch(int nlat, int nlon, int nt, const double *d_g, int idg, int jdg, double *d_a,
+                      double *d_b, int ida, int jda, hipStream_t st) {
+  int mmax = std::min(nlat, nlon / 2 + 1);
+  double *p, *cs;
+  RBC_TRY(batch_tables(nlat, nlon, mmax, false, true, &p, &cs));
+  size_t lds = sizeof(double) * ((size_t)nlat * nlon + 4 * (size_t)mmax * nlat);
+  if (lds > 160 * 1024) return RBC3D_EINVAL;
+  hipLaunchKernelGGL(k_anal_batch, dim3(nt), dim3(SH_THREADS), lds, st, nlat, nlon, mmax, p, cs, d_g,
+                     idg, jdg, d_a, d_b, ida, jda);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  (void)hipFree(p);
+  (void)hipFree(cs);
+  return RBC3D_OK;
+}
+
+int rbc_sh_synth_batch(int nlat, int nlon, int nt, double *d_g, int idg, int jdg, const double *d_a,
+                       const double *d_b, int ida, int jda, int equi, hipStream_t st) {
+  int mmax = std::min(std::min(nlat, nlon / 2 + 1), ida);
+  int nmax = std::min(nlat, jda);
+  double *p, *cs;
+  RBC_TRY(batch_tables(nlat, nlon, mmax, equi, false, &p, &cs));
+  size_t lds = sizeof(double) * (2 * (size_t)mmax * nmax + 2 * (size_t)mmax * nlat + (size_t)nlat * nlon);
+  if (lds > 160 * 1024) return RBC3D_EINVAL;
+  hipLaunchKernelGGL(k_synth_batch, dim3(nt), dim3(SH_THREADS), lds, st, nlat, nlon, mmax, nmax, p,
+                     cs, d_g, idg, jdg, d_a, d_b, ida, jda);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  (void)hipFree(p);
+  (void)hipFree(cs);
+  return RBC3D_OK;
+}
