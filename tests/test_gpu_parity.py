@@ -85,8 +85,11 @@ def test_surface_splines(c1pair):
         for irbc in range(2):
             ug = g.cell_spline(irbc, which, 2 * nl, 2 * nl, nvar)
             uo = o.cell_spline(irbc, which, 2 * nl, 2 * nl, nvar)
+            # one scale per spline: a derivative that vanishes identically (detJ of an
+            # axisymmetric shape along phi) is pure round-off on both sides
+            scale = max(np.abs(b).max() for b in uo)
             for a, b in zip(ug, uo):
-                assert relerr(a, b) < TOL, (which, irbc)
+                assert np.abs(a - b).max() < TOL * scale, (which, irbc)
 
 
 @pytest.mark.parametrize("c1,c2", [(1 / (4 * np.pi), 0.0), (0.0, -1 / (4 * np.pi)),
