@@ -139,6 +139,13 @@ int rbc3d_wall_dof(rbc3d_ctx *ctx, int *dof);
 int rbc3d_wall_matmult(rbc3d_ctx *ctx, const double *u, double *b);
 int rbc3d_wall_residual_vel(rbc3d_ctx *ctx, double *v);
 
+/* --- diagnostics (no counterpart in the reference, which only prints a per-step
+ *     MPI_WTime, common/ModTimeInt.F90:122,167) ---------------------------------
+ * Per-stage GPU time from HIP events recorded on the context's stream around each
+ * kernel stage.  enable: 1 on, 0 off, -1 reset and off.  Stages are numbered
+ * 0..rbc3d_stage_time() returns RBC3D_EINVAL past the last one. */
+int rbc3d_timing(rbc3d_ctx *ctx, int enable);
+int rbc3d_stage_time(rbc3d_ctx *ctx, int stage, double *ms, long long *count, const char **name);
 /* --- timing hooks (HIP events on the context's stream) --------------------- */
 int rbc3d_stream(rbc3d_ctx *ctx, void **hip_stream);
 int rbc3d_sync(rbc3d_ctx *ctx);

@@ -67,6 +67,9 @@ _SIG = {
     "rbc3d_wall_residual_vel": [C.c_void_p, dp],
     "rbc3d_stream": [C.c_void_p, C.POINTER(C.c_void_p)],
     "rbc3d_sync": [C.c_void_p],
+    "rbc3d_timing": [C.c_void_p, C.c_int],
+    "rbc3d_stage_time": [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
+                         C.POINTER(C.c_char_p)],
 }
 
 _lib = None
@@ -335,6 +338,23 @@ class System:
 
     def sync(self):
         _call("rbc3d_sync", self.h)
+
+    def timing(self, enable):
+        _call("rbc3d_timing", self.h, int(enable))
+
+    def stage_times(self):
+        """{stage name: (total ms, launches)} from the HIP-event stage timer."""
+        out = {}
+        s = 0
+        while True:
+            ms = C.c_double()
+            n = C.c_longlong()
+            nm = C.c_char_p()
+            if lib().rbc3d_stage_time(self.h, s, C.byref(ms), C.byref(n), C.byref(nm)) != 0:
+                break
+            out[nm.value.decode()] = (ms.value, n.value)
+            s += 1
+        return out
 
 
 def from_workload(w):

@@ -252,6 +252,78 @@ static int make_gtype(rbc3d_ctx *c, int nlat, GridType **out) {
 }
 
 // ---------------------------------------------------------------------------
+// stage timing
+static hipEvent_t ev_get(StageTimer &t) {
+  if (!t.pool.empty()) {
+    hipEvent_t e = t.pool.back();
+    t.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+void rbc_stage_begin(rbc3d_ctx *c, int stage) {
+  if (!c->timer.on) return;
+  StageTimer::Rec r;
+  r.stage = stage;
+  r.a = ev_get(c->timer);
+  r.b = ev_get(c->timer);
+  (void)hipEventRecord(r.a, c->stream);
+  c->timer.pending.push_back(r);
+}
+void rbc_stage_end(rbc3d_ctx *c, int stage) {
+  if (!c->timer.on) return;
+  for (auto it = c->timer.pending.rbegin(); it != c->timer.pending.rend(); ++it)
+    if (it->stage == stage) {
+      (void)hipEventRecord(it->b, c->stream);
+      return;
+    }
+}
+void rbc_stage_flush(rbc3d_ctx *c) {
+  if (c->timer.pending.empty()) return;
+  (void)hipStreamSynchronize(c->stream);
+  for (auto &r : c->timer.pending) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, r.a, r.b) == hipSuccess) {
+      c->timer.ms[r.stage] += ms;
+      c->timer.cnt[r.stage]++;
+    }
+    c->timer.pool.push_back(r.a);
+    c->timer.pool.push_back(r.b);
+  }
+  c->timer.pending.clear();
+}
+
+static const char *kStageNames[ST_NSTAGE] = {
+    "target_sort", "pair", "nbr_list", "self_sing", "near_proj", "near_sing", "linear_dl", "combine",
+    "sh_glob", "sh_spline", "spread_prep", "spread", "fft", "green", "gather", "wall"};
+
+extern "C" int rbc3d_timing(rbc3d_ctx *c, int enable) {
+  if (!c) return RBC3D_EINVAL;
+  rbc_stage_flush(c);
+  c->timer.on = enable != 0;
+  if (enable < 0) {
+    for (int s = 0; s < ST_NSTAGE; s++) {
+      c->timer.ms[s] = 0;
+      c->timer.cnt[s] = 0;
+    }
+    c->timer.on = false;
+  }
+  return RBC3D_OK;
+}
+
+extern "C" int rbc3d_stage_time(rbc3d_ctx *c, int stage, double *ms, long long *count,
+                                const char **name) {
+  if (!c || stage < 0 || stage >= ST_NSTAGE) return RBC3D_EINVAL;
+  rbc_stage_flush(c);
+  if (ms) *ms = c->timer.ms[stage];
+  if (count) *count = c->timer.cnt[stage];
+  if (name) *name = kStageNames[stage];
+  return RBC3D_OK;
+}
+
+// ---------------------------------------------------------------------------
 // C ABI
 
 extern "C" int rbc3d_create(const rbc3d_conf *conf, rbc3d_ctx **out) {
@@ -356,6 +428,8 @@ extern "C" int rbc3d_destroy(rbc3d_ctx *c) {
   c->d_cub_tmp.release();
   c->d_tp_ptr.release();
   c->d_tp_int.release();
+  rbc_stage_flush(c);
+  for (auto e : c->timer.pool) (void)hipEventDestroy(e);
   c->d_pairstat.release();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

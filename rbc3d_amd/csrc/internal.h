@@ -70,6 +70,23 @@ struct GridType {
   DBuf<double> d_mask;   // same-surface mask [ilat_i][ilat_j][dlon]
 };
 
+// Stage timing (HIP events on the context's stream, enabled on request)
+enum {
+  ST_TSORT = 0, ST_PAIR, ST_NBR, ST_SELF, ST_PROJ, ST_NEAR, ST_LINEAR, ST_COMBINE, ST_SH_GLOB,
+  ST_SPLINE, ST_SPREAD_PREP, ST_SPREAD, ST_FFT, ST_GREEN, ST_GATHER, ST_WALL, ST_NSTAGE
+};
+struct StageTimer {
+  bool on = false;
+  struct Rec {
+    int stage;
+    hipEvent_t a, b;
+  };
+  std::vector<Rec> pending;
+  std::vector<hipEvent_t> pool;
+  double ms[ST_NSTAGE] = {0};
+  long long cnt[ST_NSTAGE] = {0};
+};
+
 // Per-cell record (device array of these).
 struct CellRec {
   int off;      // first node in the concatenated node arrays
@@ -157,6 +174,7 @@ struct rbc3d_ctx {
   DBuf<const double *> d_tp_ptr;
   DBuf<int> d_tp_int;
   size_t tp_ntypes = 0;
+  StageTimer timer;
   // last pair statistics
   long long last_npairs = 0;
   unsigned long long last_checksum = 0;
@@ -267,6 +285,10 @@ __device__ __forceinline__ double dist_on_sphere(double th0, double phi0, double
   d = fmin(1., fmax(-1., d));
   return acos(d);
 }
+
+void rbc_stage_begin(rbc3d_ctx *c, int stage);
+void rbc_stage_end(rbc3d_ctx *c, int stage);
+void rbc_stage_flush(rbc3d_ctx *c);
 
 // Kernel launchers (defined in the .hip files)
 int rbc_sh_build_splines(rbc3d_ctx *c, int which);

@@ -1098,7 +1098,9 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   RBC_TRY(c->d_tbin.ensure(nT + 1));
   RBC_TRY(c->d_torder.ensure(nT + 1));
   RBC_TRY(c->d_tbin_start.ensure(nb + 1));
+  rbc_stage_begin(c, ST_TSORT);
   RBC_TRY(rbc_hash_sort(c, nT, d_tx, nT, c->d_tbin.p, c->d_torder.p, c->d_tbin_start.p));
+  rbc_stage_end(c, ST_TSORT);
   RBC_TRY(c->d_flag.ensure(2 * (size_t)(nb + 1) + 8));
   int *chunk_cnt = c->d_flag.p + 8, *chunk_off = chunk_cnt + (nb + 1);
   k_chunks<<<(nb + 1 + 255) / 256, 256, 0, st>>>(nb, c->d_tbin_start.p, chunk_cnt);
@@ -1141,7 +1143,9 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   pa.vpair = c->d_vacc.p;
   pa.stat = c->d_pairstat.p;
   int nblk = nb + (nT + PAIR_T - 1) / PAIR_T + 1;
+  rbc_stage_begin(c, ST_PAIR);
   k_pair<<<nblk, PAIR_T, 0, st>>>(pa);
+  rbc_stage_end(c, ST_PAIR);
   HIP_TRY(hipGetLastError());
   // neighbouring-cell lists
   RBC_TRY(c->d_nbr_cnt.ensure(2 * (size_t)nT + 2));
@@ -1171,7 +1175,9 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   na.slot_node = c->d_nbr_node.p;
   na.slot_dist = c->d_nbr_dist.p;
   na.flag = c->d_flag.p;
+  rbc_stage_begin(c, ST_NBR);
   k_nbr<<<(nT + NBR_T - 1) / NBR_T, NBR_T, 0, st>>>(na);
+  rbc_stage_end(c, ST_NBR);
   HIP_TRY(hipGetLastError());
   int *item_off = c->d_nbr_cnt.p + nT + 1;
   HIP_TRY(hipMemsetAsync(c->d_nbr_cnt.p + nT, 0, sizeof(int), st));
@@ -1237,16 +1243,22 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   sa.work = wl.p;
   sa.work_cnt = c->d_work_cnt.p;
   sa.dvitem = c->d_dvitem.p;
+  rbc_stage_begin(c, ST_SELF);
   k_self<<<(unsigned)(((size_t)nT * 64 + 255) / 256), 256, 0, st>>>(sa);
+  rbc_stage_end(c, ST_SELF);
   HIP_TRY(hipGetLastError());
   if (nitems > 0) {
+    rbc_stage_begin(c, ST_PROJ);
     k_proj<<<(unsigned)(((size_t)nT * RBC_NBR_MAX + 127) / 128), 128, 0, st>>>(sa);
+    rbc_stage_end(c, ST_PROJ);
     HIP_TRY(hipGetLastError());
     int nwork = 0;
     HIP_TRY(hipMemcpyAsync(&nwork, c->d_work_cnt.p, sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
     if (nwork > 0) {
+      rbc_stage_begin(c, ST_NEAR);
       k_near<<<(unsigned)(((size_t)nwork * 64 + 255) / 256), 256, 0, st>>>(sa, nwork);
+      rbc_stage_end(c, ST_NEAR);
       HIP_TRY(hipGetLastError());
     }
   }
@@ -1256,10 +1268,14 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   if (c2 != 0) {
     int nbk = 256;
     RBC_TRY(part.ensure(3 * nbk));
+    rbc_stage_begin(c, ST_LINEAR);
     k_linear_partial<<<nbk, 256, 0, st>>>(c->np, c->d_x.p, c->d_g.p, c->d_a3.p, c->d_ds.p,
                                           c->d_node_cell.p, c->d_cells.p, part.p);
+    rbc_stage_end(c, ST_LINEAR);
     double scale = -8 * RBC_PI * (c->iLb[0] * c->iLb[1] * c->iLb[2]);
+    rbc_stage_begin(c, ST_LINEAR);
     k_linear_final<<<1, 64, 0, st>>>(nbk, part.p, scale, xv);
+    rbc_stage_end(c, ST_LINEAR);
   }
   CombineArgs ca;
   ca.nT = nT;
@@ -1274,7 +1290,9 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   ca.xv = xv;
   ca.has_linear = c2 != 0;
   ca.v = d_v;
+  rbc_stage_begin(c, ST_COMBINE);
   k_combine<<<(nT + 255) / 256, 256, 0, st>>>(ca);
+  rbc_stage_end(c, ST_COMBINE);
   HIP_TRY(hipGetLastError());
   unsigned long long hs[2];
   HIP_TRY(hipMemcpyAsync(hs, c->d_pairstat.p, sizeof(hs), hipMemcpyDeviceToHost, st));

@@ -320,11 +320,13 @@ int rbc_pme_spread(rbc3d_ctx *c, double c1, double c2, int cells, int walls) {
   int *key = c->d_pme_key.p, *key2 = c->d_pme_key.p + ns + 1;
   int *idx = c->d_pme_order.p + ns + 1, *order = c->d_pme_order.p;
   if (ns > 0) {
+    rbc_stage_begin(c, ST_SPREAD_PREP);
     k_spread_prep<<<(ns + 255) / 256, 256, 0, st>>>(
         ns, ncell, c->d_x.p, c->np, c->d_fs.p, c->d_gs.p, c->d_a3.p, nullptr, c->d_node_cell.p,
         c->d_cells.p, nele, c->wall.d_x.p, c->wall.d_f.p, c->wall.d_e2v.p, c->wall.nvert,
         c->wall.d_area.p, N1 / c->Lb[0], N2 / c->Lb[1], N3 / c->Lb[2], N1, N2, N3, P, c->flag_sl,
         c->flag_dl, nbx, nby, nbz, c->d_pme_src.p, reclen, key, idx);
+    rbc_stage_end(c, ST_SPREAD_PREP);
     HIP_TRY(hipGetLastError());
     int bits = 1;
     while ((1 << bits) < nbr) bits++;
@@ -355,7 +357,9 @@ int rbc_pme_spread(rbc3d_ctx *c, double c1, double c2, int cells, int walls) {
   a.brick_start = c->d_pme_start.p;
   a.grid = c->d_grid.p;
   size_t lds = sizeof(double) * SPR_T * reclen;
+  rbc_stage_begin(c, ST_SPREAD);
   k_spread<<<nbr, SPR_T, lds, st>>>(a);
+  rbc_stage_end(c, ST_SPREAD);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;
 }
@@ -631,6 +635,7 @@ int rbc_pme_transform(rbc3d_ctx *c) {
   double2 *C = (double2 *)c->d_cgrid.p;
   int f0 = c->flag_sl ? 0 : 3, f1 = c->flag_dl ? 12 : 3;
   int nf = f1 - f0;
+  rbc_stage_begin(c, ST_FFT);
   if (nf > 0) {
     FftPlan px = make_plan(c, 0, -1);
     int TL = lines_per_block(N1), nrows = N2 * N3 * nf;
@@ -641,6 +646,7 @@ int rbc_pme_transform(rbc3d_ctx *c) {
     RBC_TRY(fft_c2c(c, C + nc * f0, 1, -1, nf));
     RBC_TRY(fft_c2c(c, C + nc * f0, 2, +1, nf));
   }
+  rbc_stage_end(c, ST_FFT);
   GreenArgs g;
   g.N1 = N1;
   g.N2 = N2;
@@ -653,14 +659,18 @@ int rbc_pme_transform(rbc3d_ctx *c) {
   g.flag_dl = c->flag_dl;
   g.bb = c->d_bb.p;
   g.C = C;
+  rbc_stage_begin(c, ST_GREEN);
   k_green<<<(unsigned)((nc + 255) / 256), 256, 0, st>>>(g);
+  rbc_stage_end(c, ST_GREEN);
   HIP_TRY(hipGetLastError());
+  rbc_stage_begin(c, ST_FFT);
   RBC_TRY(fft_c2c(c, C + nc * 12, 2, -1, 3));
   RBC_TRY(fft_c2c(c, C + nc * 12, 1, +1, 3));
   FftPlan pxi = make_plan(c, 0, +1);
   int TL = lines_per_block(N1), nrows = N2 * N3 * 3;
   size_t lds = 2 * sizeof(double2) * TL * N1;
   k_fft_c2r<<<(nrows + TL - 1) / TL, FFT_T, lds, st>>>(C + nc * 12, c->d_vgrid.p, pxi, nrows, TL);
+  rbc_stage_end(c, ST_FFT);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;
 }
@@ -720,7 +730,9 @@ int rbc_pme_gather(rbc3d_ctx *c, int nT, const double *d_tx, const int *d_act, c
   g.tA = d_A;
   g.tact = d_act;
   g.v = d_v;
+  rbc_stage_begin(c, ST_GATHER);
   k_gather<<<(nT + 255) / 256, 256, 0, c->stream>>>(g);
+  rbc_stage_end(c, ST_GATHER);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;
 }

@@ -212,10 +212,12 @@ int rbc_sh_build_splines(rbc3d_ctx *c, int which) {
     GridType *g = c->gtypes[gt];
     size_t lds = spline_lds(g->nlat);
     const double *fd = which == 1 ? c->d_f.p : which == 2 ? c->d_g.p : nullptr;
+    rbc_stage_begin(c, ST_SPLINE);
     hipLaunchKernelGGL(k_build_splines, dim3(n), dim3(SH_THREADS), lds, c->stream, c->d_cells.p,
                        base, (int)gt, g->d_pw.p, g->d_pe.p, g->d_cs.p, g->d_Dm.p, g->d_Dn.p,
                        c->d_x.p, c->d_a3.p, c->d_detj.p, fd, c->np, which, c->d_spl.p,
                        c->d_tmp_i.p);
+    rbc_stage_end(c, ST_SPLINE);
     HIP_TRY(hipGetLastError());
     base += n;
   }
@@ -298,12 +300,14 @@ int rbc_sh_glob(rbc3d_ctx *c, const double *d_in, double *d_out, int direction) 
     GridType *g = c->gtypes[gt];
     int nl = g->nlat, nn = g->nlon;
     size_t lds = sizeof(double) * ((size_t)nl * nn + 2 * (size_t)nl * nl + 2 * (size_t)nl * nl + 16);
+    rbc_stage_begin(c, ST_SH_GLOB);
     if (direction == 1)
       hipLaunchKernelGGL(k_glob_anal, dim3(n), dim3(SH_THREADS), lds, c->stream, c->d_cells.p, base,
                          (int)gt, g->d_pw.p, g->d_cs.p, d_in, c->np, d_out, c->d_tmp_i.p);
     else
       hipLaunchKernelGGL(k_glob_synth, dim3(n), dim3(SH_THREADS), lds, c->stream, c->d_cells.p,
                          base, (int)gt, g->d_pg.p, g->d_cs.p, d_in, c->np, d_out, c->d_tmp_i.p);
+    rbc_stage_end(c, ST_SH_GLOB);
     HIP_TRY(hipGetLastError());
     base += n;
   }
