@@ -334,8 +334,8 @@ extern "C" int rbc3d_create(const rbc3d_conf *conf, rbc3d_ctx **out) {
     return RBC3D_EINVAL;
   }
   for (int d = 0; d < 3; d++)
-    if (conf->Nb[d] < conf->P || conf->Lb[d] <= 0) {
-      rbc_set_error("rbc3d_create: invalid box or PME grid");
+    if (conf->Nb[d] < conf->P || conf->Lb[d] <= 0 || conf->rc >= 0.5 * conf->Lb[d]) {
+      rbc_set_error("rbc3d_create: invalid box, cut-off (rc must be below Lb/2) or PME grid");
       return RBC3D_EINVAL;
     }
   int ndev = 0;
@@ -385,6 +385,43 @@ extern "C" int rbc3d_create(const rbc3d_conf *conf, rbc3d_ctx **out) {
   c->iLbNc[1] = c->iLb[1] * c->Nc[1];
   c->iLbNc[2] = c->Nc[2] / (c->Lb[2] - 0.);
   c->nbins = c->Nc[0] * c->Nc[1] * c->Nc[2];
+  // pair-kernel grid: twice the reference's resolution, offsets within +-R bins
+  // whose box distance to the target bin is within rc (a small margin keeps the
+  // enumeration conservative); an axis with 2R+1 >= Nf enumerates every bin once
+  {
+    int R[3], lo[3], hi[3];
+    double wmin[3];
+    for (int d = 0; d < 3; d++) {
+      c->Nf[d] = 2 * c->Nc[d];
+      c->iLbNf[d] = c->iLb[d] * c->Nf[d];
+      wmin[d] = c->Lb[d] / c->Nf[d];
+      R[d] = (int)ceil(rc / wmin[d] * (1 + 1e-9));
+      if (2 * R[d] + 1 >= c->Nf[d]) {
+        lo[d] = 0;
+        hi[d] = c->Nf[d] - 1;
+      } else {
+        lo[d] = -R[d];
+        hi[d] = R[d];
+      }
+    }
+    std::vector<int> offs;
+    for (int a = lo[0]; a <= hi[0]; a++)
+      for (int b = lo[1]; b <= hi[1]; b++)
+        for (int q = lo[2]; q <= hi[2]; q++) {
+          int o[3] = {a, b, q};
+          double g2 = 0;
+          for (int d = 0; d < 3; d++) {
+            bool all = (lo[d] == 0 && hi[d] == c->Nf[d] - 1);
+            int k = all ? 0 : std::max(std::abs(o[d]) - 1, 0);
+            g2 += (k * wmin[d]) * (k * wmin[d]);
+          }
+          if (g2 > rc * rc * (1 + 1e-9)) continue;
+          offs.push_back((a + 128) | ((b + 128) << 8) | ((q + 128) << 16));
+        }
+    c->nfbins = c->Nf[0] * c->Nf[1] * c->Nf[2];
+    c->noffs = offs.size();
+    if (!r) r = upload(c->d_offs, offs, c->stream);
+  }
   if (!r) r = hipStreamSynchronize(c->stream) == hipSuccess ? 0 : RBC3D_EHIP;
   if (r) {
     rbc3d_destroy(c);
@@ -428,6 +465,9 @@ extern "C" int rbc3d_destroy(rbc3d_ctx *c) {
   c->d_cub_tmp.release();
   c->d_tp_ptr.release();
   c->d_tp_int.release();
+  c->d_offs.release();
+  c->d_tmp_f.release();
+  c->d_fbin_start.release();
   rbc_stage_flush(c);
   for (auto e : c->timer.pool) (void)hipEventDestroy(e);
   c->d_pairstat.release();

@@ -129,6 +129,12 @@ struct rbc3d_ctx {
   int Nc[3];
   double iLbNc[3];
   int nbins;
+  // finer grid of the pair kernel (width >= rc/2) and its pruned source-bin offsets
+  int Nf[3];
+  double iLbNf[3];
+  int nfbins;
+  size_t noffs = 0;
+  DBuf<int> d_offs, d_tmp_f, d_fbin_start;
   // cells
   int nrbc = 0, np = 0;
   std::vector<GridType *> gtypes;

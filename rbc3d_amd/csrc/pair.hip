@@ -19,8 +19,7 @@
 
 #include "internal.h"
 
-#define PAIR_T 256
-#define NBR_T 128
+#define NBR_T 256
 
 // ---------------------------------------------------------------------------
 // cell list
@@ -64,13 +63,8 @@ __global__ void k_bin_start(int n, const int *__restrict__ skey, int nbins, int 
 
 // Sort n points by bin (stable: ascending index inside a bin).  d_bin receives the
 // bin of each point in ORIGINAL order, d_order the sorted point indices.
-int rbc_hash_sort(rbc3d_ctx *c, int n, const double *d_x, int ldx, int *d_bin, int *d_order,
-                  int *d_bin_start) {
-  HashGeom hg;
-  for (int d = 0; d < 3; d++) {
-    hg.Nc[d] = c->Nc[d];
-    hg.iLbNc[d] = c->iLbNc[d];
-  }
+static int hash_sort_grid(rbc3d_ctx *c, const HashGeom &hg, int nbins, int n, const double *d_x,
+                          int ldx, int *d_bin, int *d_order, int *d_bin_start) {
   hipStream_t st = c->stream;
   DBuf<int> keys_out, idx_in;
   RBC_TRY(keys_out.ensure(n + 1));
@@ -79,7 +73,7 @@ int rbc_hash_sort(rbc3d_ctx *c, int n, const double *d_x, int ldx, int *d_bin, i
     k_bins<<<(n + 255) / 256, 256, 0, st>>>(n, d_x, ldx, hg, d_bin, idx_in.p);
     HIP_TRY(hipGetLastError());
     int bits = 1;
-    while ((1 << bits) < c->nbins) bits++;
+    while ((1 << bits) < nbins) bits++;
     size_t tmp = 0;
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, d_bin, keys_out.p, idx_in.p, d_order, n,
                                                0, bits, st));
@@ -87,12 +81,34 @@ int rbc_hash_sort(rbc3d_ctx *c, int n, const double *d_x, int ldx, int *d_bin, i
     HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->d_cub_tmp.p, tmp, d_bin, keys_out.p, idx_in.p,
                                                d_order, n, 0, bits, st));
   }
-  k_bin_start<<<(c->nbins + 1 + 255) / 256, 256, 0, st>>>(n, keys_out.p, c->nbins, d_bin_start);
+  k_bin_start<<<(nbins + 1 + 255) / 256, 256, 0, st>>>(n, keys_out.p, nbins, d_bin_start);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st));
   keys_out.release();
   idx_in.release();
   return RBC3D_OK;
+}
+
+// the reference's cell list (coarse bins)
+int rbc_hash_sort(rbc3d_ctx *c, int n, const double *d_x, int ldx, int *d_bin, int *d_order,
+                  int *d_bin_start) {
+  HashGeom hg;
+  for (int d = 0; d < 3; d++) {
+    hg.Nc[d] = c->Nc[d];
+    hg.iLbNc[d] = c->iLbNc[d];
+  }
+  return hash_sort_grid(c, hg, c->nbins, n, d_x, ldx, d_bin, d_order, d_bin_start);
+}
+
+// the pair kernel's grid (fine bins)
+static int fine_sort(rbc3d_ctx *c, int n, const double *d_x, int ldx, int *d_bin, int *d_order,
+                     int *d_bin_start) {
+  HashGeom hg;
+  for (int d = 0; d < 3; d++) {
+    hg.Nc[d] = c->Nf[d];
+    hg.iLbNc[d] = c->iLbNf[d];
+  }
+  return hash_sort_grid(c, hg, c->nfbins, n, d_x, ldx, d_bin, d_order, d_bin_start);
 }
 
 // hoc/next from the sorted list, with the ghost layers of HashTable_Build
@@ -266,6 +282,9 @@ int rbc_build_source_list(rbc3d_ctx *c) {
   RBC_TRY(c->d_src_order.ensure(np + 1));
   RBC_TRY(c->d_bin_start.ensure(c->nbins + 1));
   RBC_TRY(rbc_hash_sort(c, np, c->d_x.p, np, c->d_src_bin.p, c->d_src_order.p, c->d_bin_start.p));
+  RBC_TRY(c->d_tmp_f.ensure(np + 1));
+  RBC_TRY(c->d_fbin_start.ensure(c->nfbins + 1));
+  RBC_TRY(fine_sort(c, np, c->d_x.p, np, c->d_tmp_f.p, c->d_src_order.p, c->d_fbin_start.p));
   RBC_TRY(c->d_src_rec.ensure((size_t)SREC * np + 1));
   RBC_TRY(c->d_cell_sph.ensure((size_t)4 * c->nrbc + 4));
   k_cell_sphere<<<c->nrbc, 256, 0, c->stream>>>(c->nrbc, c->d_cells.p, c->d_x.p, np, c->d_cell_sph.p);
@@ -292,16 +311,27 @@ int rbc_update_density(rbc3d_ctx *c, int which) {
 
 // ---------------------------------------------------------------------------
 // pair kernel
+//
+// Targets and sources are binned on a grid twice as fine as the reference's cell
+// list (width >= rc/2).  With one rank the reference's visited set contains every
+// minimum-image pair with rr <= rc (its bins are at least rc wide, or there are
+// three of them and all are visited), so the pair set is {rr <= rc} and any
+// conservative enumeration reproduces it: here the source bins within +-R of the
+// target bin whose box distance does not exceed rc.  A workgroup is one wavefront
+// owning up to 64 targets of one fine bin; sources stream through LDS in tiles.
 
+#define PT 64    // targets per workgroup
+#define PS 256   // sources per LDS tile
 
 struct PairArgs {
   double Lb[3], iLb[3];
-  int Nc[3];
-  double rc, reps, c1, c2;
-  int np, nT, nbins, nrbc;
-  const double *rec;     // sorted source records (SREC x np)
+  int Nf[3];
+  double rc, rc2hi, reps, c1, c2;
+  int np, nT, nfbins, nrbc, noffs;
+  const int *offs;       // packed source-bin offsets
+  const double *rec;     // sources sorted by fine bin (SREC x np)
   const int *src_node;   // sorted position -> node index
-  const int *bin_start;  // sources per bin
+  const int *bin_start;  // sources per fine bin
   const int *node_cell, *node_ll;
   const CellRec *cells;
   const double *sl1, *sl2, *dl1;
@@ -315,47 +345,36 @@ struct PairArgs {
 __global__ void k_chunks(int nbins, const int *__restrict__ tbin_start, int *chunk_cnt) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b > nbins) return;
-  chunk_cnt[b] = b < nbins ? (tbin_start[b + 1] - tbin_start[b] + PAIR_T - 1) / PAIR_T : 0;
+  chunk_cnt[b] = b < nbins ? (tbin_start[b + 1] - tbin_start[b] + PT - 1) / PT : 0;
 }
 
-// Pair sums of AddIntOnRbcs (common/ModIntOnRbcs.F90:47-112) for one chunk of targets
-// of one bin; the 27 neighbouring bins' sources stream through LDS.
-__global__ void __launch_bounds__(PAIR_T) k_pair(PairArgs a) {
-  __shared__ double sx[PAIR_T], sy[PAIR_T], sz[PAIR_T];
-  __shared__ double sf[3][PAIR_T], sg[3][PAIR_T], sn[3][PAIR_T], sB[PAIR_T];
-  __shared__ int snode[PAIR_T];
-  __shared__ int s_bin, s_chunk;
-  const int nbins = a.nbins;
-  if (threadIdx.x == 0) {
-    int blk = blockIdx.x;
-    if (blk >= a.chunk_off[nbins]) {
-      s_bin = -1;
-    } else {
-      int lo = 0, hi = nbins;  // chunk_off[lo] <= blk < chunk_off[hi]
-      while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (a.chunk_off[mid] <= blk)
-          lo = mid;
-        else
-          hi = mid;
-      }
-      s_bin = lo;
-      s_chunk = blk - a.chunk_off[lo];
-    }
+__global__ void __launch_bounds__(PT) k_pair(PairArgs a) {
+  __shared__ double sx[PS], sy[PS], sz[PS];
+  __shared__ double sf[3][PS], sg[3][PS], sn[3][PS], sB[PS];
+  __shared__ int snode[PS];
+  const int nb = a.nfbins, blk = blockIdx.x;
+  if (blk >= a.chunk_off[nb]) return;
+  int lo = 0, hi = nb;  // chunk_off[lo] <= blk < chunk_off[hi]
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (a.chunk_off[mid] <= blk)
+      lo = mid;
+    else
+      hi = mid;
   }
-  __syncthreads();
-  const int bin = s_bin;
-  if (bin < 0) return;
-  const int t0 = a.tbin_start[bin] + s_chunk * PAIR_T, t1 = a.tbin_start[bin + 1];
+  const int bin = lo, chunk = blk - a.chunk_off[lo];
+  const int t0 = a.tbin_start[bin] + chunk * PT, t1 = a.tbin_start[bin + 1];
   const int k = t0 + threadIdx.x;
   const bool have = k < t1;
   const int ti = have ? a.torder[k] : 0;
   const bool act = have && a.tact[ti] != 0;
-  double xi[3] = {0, 0, 0}, Ai = 1.;
+  double xi0 = 0, xi1 = 0, xi2 = 0, Ai = 1.;
   int surf_i = -1000, ilat_i = 0, ilon_i = 0, nl_i = 1, nn_i = 1;
   const double *mtab = nullptr;
   if (have) {
-    for (int d = 0; d < 3; d++) xi[d] = a.tx[ti + (size_t)a.nT * d];
+    xi0 = a.tx[ti];
+    xi1 = a.tx[ti + (size_t)a.nT];
+    xi2 = a.tx[ti + 2 * (size_t)a.nT];
     Ai = a.tA[ti];
     surf_i = a.tindx[ti];
     if (surf_i >= 1 && surf_i <= a.nrbc) {
@@ -368,36 +387,53 @@ __global__ void __launch_bounds__(PAIR_T) k_pair(PairArgs a) {
     }
   }
   const int cell_i = (surf_i >= 1 && surf_i <= a.nrbc) ? surf_i - 1 : -1;
+  const double L0 = a.Lb[0], L1 = a.Lb[1], L2 = a.Lb[2];
+  const double iL0 = a.iLb[0], iL1 = a.iLb[1], iL2 = a.iLb[2];
   double acc0 = 0, acc1 = 0, acc2 = 0;
   unsigned long long npairs = 0, csum = 0;
-  const int b1 = bin % a.Nc[0], b2 = (bin / a.Nc[0]) % a.Nc[1], b3 = bin / (a.Nc[0] * a.Nc[1]);
+  const int b1 = bin % a.Nf[0], b2 = (bin / a.Nf[0]) % a.Nf[1], b3 = bin / (a.Nf[0] * a.Nf[1]);
   const size_t np = a.np;
-  for (int nbi = 0; nbi < 27; nbi++) {
-    int o1 = nbi / 9 - 1, o2 = (nbi / 3) % 3 - 1, o3 = nbi % 3 - 1;
-    int n1 = imod_d(b1 + o1, a.Nc[0]), n2 = imod_d(b2 + o2, a.Nc[1]), n3 = imod_d(b3 + o3, a.Nc[2]);
-    int nb = n1 + a.Nc[0] * (n2 + a.Nc[1] * n3);
-    int s0 = a.bin_start[nb], s1 = a.bin_start[nb + 1];
-    for (int base = s0; base < s1; base += PAIR_T) {
-      int cnt = min(PAIR_T, s1 - base);
+  for (int oi = 0; oi < a.noffs; oi++) {
+    int code = a.offs[oi];
+    int o1 = (code & 255) - 128, o2 = ((code >> 8) & 255) - 128, o3 = ((code >> 16) & 255) - 128;
+    int n1 = imod_d(b1 + o1, a.Nf[0]), n2 = imod_d(b2 + o2, a.Nf[1]), n3 = imod_d(b3 + o3, a.Nf[2]);
+    int nbn = n1 + a.Nf[0] * (n2 + a.Nf[1] * n3);
+    int s0 = a.bin_start[nbn], s1 = a.bin_start[nbn + 1];
+    for (int base = s0; base < s1; base += PS) {
+      int cnt = min(PS, s1 - base);
       __syncthreads();
-      if (threadIdx.x < cnt) {
-        int q = base + threadIdx.x;
-        sx[threadIdx.x] = a.rec[q];
-        sy[threadIdx.x] = a.rec[q + np];
-        sz[threadIdx.x] = a.rec[q + 2 * np];
+      for (int t = threadIdx.x; t < cnt; t += PT) {
+        int q = base + t;
+        sx[t] = a.rec[q];
+        sy[t] = a.rec[q + np];
+        sz[t] = a.rec[q + 2 * np];
         for (int d = 0; d < 3; d++) {
-          sf[d][threadIdx.x] = a.rec[q + np * (3 + d)];
-          sg[d][threadIdx.x] = a.rec[q + np * (6 + d)];
-          sn[d][threadIdx.x] = a.rec[q + np * (9 + d)];
+          sf[d][t] = a.rec[q + np * (3 + d)];
+          sg[d][t] = a.rec[q + np * (6 + d)];
+          sn[d][t] = a.rec[q + np * (9 + d)];
         }
-        sB[threadIdx.x] = a.rec[q + np * 12];
-        snode[threadIdx.x] = a.src_node[q];
+        sB[t] = a.rec[q + np * 12];
+        snode[t] = a.src_node[q];
       }
       __syncthreads();
       if (!act) continue;
       for (int j = 0; j < cnt; j++) {
-        double xj[3] = {sx[j], sy[j], sz[j]}, xx[3];
-        double rr = ref_min_image_dist(xi, xj, a.Lb, a.iLb, xx);
+        // reference distance without FMA contraction (common/ModIntOnRbcs.F90:68-74);
+        // rint differs from NINT only at |x/Lb| = 1/2, where |x| = Lb/2 > rc
+        double d0, d1, d2, s;
+        {
+#pragma clang fp contract(off)
+          d0 = sx[j] - xi0;
+          d1 = sy[j] - xi1;
+          d2 = sz[j] - xi2;
+          d0 = d0 - rint(d0 * iL0) * L0;
+          d1 = d1 - rint(d1 * iL1) * L1;
+          d2 = d2 - rint(d2 * iL2) * L2;
+          s = d0 * d0 + d1 * d1;
+          s = s + d2 * d2;
+        }
+        if (s > a.rc2hi) continue;
+        double rr = sqrt(s);
         if (rr > a.rc) continue;
         int node = snode[j];
         double mask = 0.;
@@ -414,20 +450,20 @@ __global__ void __launch_bounds__(PAIR_T) k_pair(PairArgs a) {
           double EA, EB;
           ewald_sl(a.sl1, a.sl2, rr, a.reps, a.rc, EA, EB);
           double f0 = sf[0][j], f1 = sf[1][j], f2 = sf[2][j];
-          double xf = xx[0] * f0 + xx[1] * f1 + xx[2] * f2;
+          double xf = d0 * f0 + d1 * f1 + d2 * f2;
           double cc = (1. - mask) * a.c1 / Ai;
-          acc0 += cc * (EA * xx[0] * xf + EB * f0);
-          acc1 += cc * (EA * xx[1] * xf + EB * f1);
-          acc2 += cc * (EA * xx[2] * xf + EB * f2);
+          acc0 += cc * (EA * d0 * xf + EB * f0);
+          acc1 += cc * (EA * d1 * xf + EB * f1);
+          acc2 += cc * (EA * d2 * xf + EB * f2);
         }
         if (a.c2 != 0) {
           double EA = ewald_dl(a.dl1, rr, a.reps, a.rc);
-          double xg = xx[0] * sg[0][j] + xx[1] * sg[1][j] + xx[2] * sg[2][j];
-          double xn = xx[0] * sn[0][j] + xx[1] * sn[1][j] + xx[2] * sn[2][j];
+          double xg = d0 * sg[0][j] + d1 * sg[1][j] + d2 * sg[2][j];
+          double xn = d0 * sn[0][j] + d1 * sn[1][j] + d2 * sn[2][j];
           double cc = (1. - mask) * a.c2 * sB[j] / Ai;
-          acc0 += cc * (EA * xx[0] * xg * xn);
-          acc1 += cc * (EA * xx[1] * xg * xn);
-          acc2 += cc * (EA * xx[2] * xg * xn);
+          acc0 += cc * (EA * d0 * xg * xn);
+          acc1 += cc * (EA * d1 * xg * xn);
+          acc2 += cc * (EA * d2 * xg * xn);
         }
       }
     }
@@ -436,6 +472,13 @@ __global__ void __launch_bounds__(PAIR_T) k_pair(PairArgs a) {
     a.vpair[ti] = acc0;
     a.vpair[ti + (size_t)a.nT] = acc1;
     a.vpair[ti + 2 * (size_t)a.nT] = acc2;
+  }
+  // one atomic per wavefront for the pair statistics
+  for (int o = 32; o > 0; o >>= 1) {
+    npairs += __shfl_xor(npairs, o, 64);
+    csum += __shfl_xor(csum, o, 64);
+  }
+  if (threadIdx.x == 0) {
     atomicAdd(&a.stat[0], npairs);
     atomicAdd(&a.stat[1], csum);
   }
@@ -443,8 +486,10 @@ __global__ void __launch_bounds__(PAIR_T) k_pair(PairArgs a) {
 
 // ---------------------------------------------------------------------------
 // neighbouring-cell lists: for every target and every other cell with a node in the
-// cut-off, the closest node (ties: first visited, i.e. lowest 27-bin visit position,
-// then highest source index -- the reference's list order), NbrRbcList_Insert.
+// cut-off, the closest node (ties: first visited, i.e. lowest 27-bin visit position
+// of the reference's cell list, then highest source index), NbrRbcList_Insert.
+// One wavefront per target: lanes test the cells' bounding spheres, then sweep the
+// nodes of each candidate cell and reduce (distance, visit key, node).
 struct NbrArgs {
   double Lb[3], iLb[3], iLbNc[3];
   int Nc[3];
@@ -461,15 +506,24 @@ struct NbrArgs {
   int *flag;
 };
 
+__device__ __forceinline__ bool nbr_better(double r1, int k1, int n1, double r2, int k2, int n2) {
+  if (n1 < 0) return false;
+  if (n2 < 0) return true;
+  if (r1 != r2) return r1 < r2;
+  if (k1 != k2) return k1 < k2;
+  return n1 > n2;
+}
+
 __global__ void __launch_bounds__(NBR_T) k_nbr(NbrArgs a) {
-  int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  const int ti = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
   if (ti >= a.nT) return;
   if (!a.tact[ti]) {
-    a.cnt[ti] = 0;
+    if (lane == 0) a.cnt[ti] = 0;
     return;
   }
   double xi[3] = {a.tx[ti], a.tx[ti + a.nT], a.tx[ti + 2 * (size_t)a.nT]};
-  int surf_i = a.tindx[ti];
+  const int surf_i = a.tindx[ti];
   int t1, t2, t3;
   hash_index3(xi, a.Nc, a.iLbNc, t1, t2, t3);
   t1--;
@@ -477,48 +531,69 @@ __global__ void __launch_bounds__(NBR_T) k_nbr(NbrArgs a) {
   t3--;
   int n = 0;
   bool ovf = false;
-  size_t np = a.np;
-  for (int cj = 0; cj < a.nrbc; cj++) {
-    if (cj == surf_i - 1) continue;
-    double cen[3] = {a.sph[4 * cj], a.sph[4 * cj + 1], a.sph[4 * cj + 2]}, dd[3];
-    double rcen = ref_min_image_dist(xi, cen, a.Lb, a.iLb, dd);
-    if (rcen > a.rc + a.sph[4 * cj + 3] + 1e-9 * a.rc) continue;
-    const CellRec cr = a.cells[cj];
-    int npt = cr.nlat * cr.nlon;
-    double best = 0;
-    int bkey = 0, bnode = -1;
-    for (int q = 0; q < npt; q++) {
-      int node = cr.off + q;
-      double xj[3] = {a.x[node], a.x[node + np], a.x[node + 2 * np]}, xx[3];
-      double rr = ref_min_image_dist(xi, xj, a.Lb, a.iLb, xx);
-      if (rr > a.rc) continue;
-      int sb = a.src_bin[node];
-      int s1 = sb % a.Nc[0], s2 = (sb / a.Nc[0]) % a.Nc[1], s3 = sb / (a.Nc[0] * a.Nc[1]);
-      int d1 = imod_d(s1 - t1 + 1, a.Nc[0]), d2 = imod_d(s2 - t2 + 1, a.Nc[1]),
-          d3 = imod_d(s3 - t3 + 1, a.Nc[2]);
-      if (d1 > 2 || d2 > 2 || d3 > 2) continue;  // not in the 27 visited bins
-      int key = 9 * d1 + 3 * d2 + d3;
-      bool take = bnode < 0 || rr < best || (rr == best && (key < bkey || (key == bkey && node > bnode)));
-      if (take) {
-        best = rr;
-        bkey = key;
-        bnode = node;
-      }
+  const size_t np = a.np;
+  for (int cbase = 0; cbase < a.nrbc; cbase += 64) {
+    int cj = cbase + lane;
+    bool pass = false;
+    if (cj < a.nrbc && cj != surf_i - 1) {
+      double cen[3] = {a.sph[4 * cj], a.sph[4 * cj + 1], a.sph[4 * cj + 2]}, dd[3];
+      double rcen = ref_min_image_dist(xi, cen, a.Lb, a.iLb, dd);
+      pass = !(rcen > a.rc + a.sph[4 * cj + 3] + 1e-9 * a.rc);
     }
-    if (bnode >= 0) {
-      if (n < RBC_NBR_MAX) {
-        size_t s = (size_t)ti * RBC_NBR_MAX + n;
-        a.slot_cell[s] = cj;
-        a.slot_node[s] = bnode;
-        a.slot_dist[s] = best;
-        n++;
-      } else {
-        ovf = true;
+    unsigned long long m = __ballot(pass);
+    while (m) {
+      int kk = __ffsll((long long)m) - 1;
+      m &= m - 1;
+      int c2 = cbase + kk;
+      const CellRec cr = a.cells[c2];
+      int npt = cr.nlat * cr.nlon;
+      double best = 0;
+      int bkey = 0, bnode = -1;
+      for (int q = lane; q < npt; q += 64) {
+        int node = cr.off + q;
+        double xj[3] = {a.x[node], a.x[node + np], a.x[node + 2 * np]}, xx[3];
+        double rr = ref_min_image_dist(xi, xj, a.Lb, a.iLb, xx);
+        if (rr > a.rc) continue;
+        int sb = a.src_bin[node];
+        int s1 = sb % a.Nc[0], s2 = (sb / a.Nc[0]) % a.Nc[1], s3 = sb / (a.Nc[0] * a.Nc[1]);
+        int d1 = imod_d(s1 - t1 + 1, a.Nc[0]), d2 = imod_d(s2 - t2 + 1, a.Nc[1]),
+            d3 = imod_d(s3 - t3 + 1, a.Nc[2]);
+        if (d1 > 2 || d2 > 2 || d3 > 2) continue;  // not in the 27 visited bins
+        int key = 9 * d1 + 3 * d2 + d3;
+        if (nbr_better(rr, key, node, best, bkey, bnode)) {
+          best = rr;
+          bkey = key;
+          bnode = node;
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        double ob = __shfl_xor(best, o, 64);
+        int ok = __shfl_xor(bkey, o, 64), on = __shfl_xor(bnode, o, 64);
+        if (nbr_better(ob, ok, on, best, bkey, bnode)) {
+          best = ob;
+          bkey = ok;
+          bnode = on;
+        }
+      }
+      if (bnode >= 0) {
+        if (n < RBC_NBR_MAX) {
+          if (lane == 0) {
+            size_t s = (size_t)ti * RBC_NBR_MAX + n;
+            a.slot_cell[s] = c2;
+            a.slot_node[s] = bnode;
+            a.slot_dist[s] = best;
+          }
+          n++;
+        } else {
+          ovf = true;
+        }
       }
     }
   }
-  a.cnt[ti] = n;
-  if (ovf) atomicOr(a.flag, 1);
+  if (lane == 0) {
+    a.cnt[ti] = n;
+    if (ovf) atomicOr(a.flag, 1);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -558,94 +633,6 @@ __device__ void patch_map(double th0, double phi0, double dth, double dphi, doub
   th = acos(x[2]);
   phi = atan2(x[1], x[0]);
   if (phi < 0) phi += RBC_TWO_PI;
-}
-
-// Spline_FindProjection (common/ModSpline.F90:203-269) with QuadFit_2D/Min_Quad_2D
-// (common/ModBasicMath.F90:234-328; dposv as Cholesky + triangular solves)
-__device__ void find_projection(const double *sx, int m, int n, const double xtar[3], double &th0,
-                                double &phi0, double x0[3]) {
-  const int nth = 2, nphi = 8, NQ = nth * nphi + 1;
-  spline_interp(sx, m, n, th0, phi0, 3, x0);
-  double h = fmax(RBC_TWO_PI / m, RBC_TWO_PI / n);
-  for (int iter = 1; iter <= 3; iter++) {
-    double lhs[21], rhs[6];  // packed upper triangle
-    for (int q = 0; q < 21; q++) lhs[q] = 0;
-    for (int q = 0; q < 6; q++) rhs[q] = 0;
-    double s0 = sin(th0), c0 = cos(th0);
-    double xx[3], d20 = 0;
-    spline_interp(sx, m, n, th0, phi0, 3, xx);
-    for (int d = 0; d < 3; d++) xx[d] -= xtar[d];
-    d20 = xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2];
-    for (int i = 0; i < NQ; i++) {
-      double xi_, yi_, fi_;
-      if (i == 0) {
-        xi_ = 0.;
-        yi_ = 0.;
-        fi_ = d20;
-      } else {
-        int it = (i - 1) % nth, ip = (i - 1) / nth;
-        double thL = (it + 1) * h / nth, phL = ip * RBC_TWO_PI / nphi;
-        xi_ = thL * cos(phL);
-        yi_ = thL * sin(phL);
-        double thP, phP;
-        patch_point(s0, c0, phi0, thL, phL, thP, phP);
-        spline_interp(sx, m, n, thP, phP, 3, xx);
-        for (int d = 0; d < 3; d++) xx[d] -= xtar[d];
-        fi_ = xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2];
-      }
-      double u[6] = {1., xi_, yi_, xi_ * xi_, xi_ * yi_, yi_ * yi_};
-      int q = 0;
-      for (int aa = 0; aa < 6; aa++)
-        for (int bb = aa; bb < 6; bb++) lhs[q++] += u[aa] * u[bb];
-      for (int aa = 0; aa < 6; aa++) rhs[aa] += u[aa] * fi_;
-    }
-    // Cholesky A = U^T U on the packed upper triangle (row-major packing)
-    double U[6][6];
-    {
-      int q = 0;
-      for (int aa = 0; aa < 6; aa++)
-        for (int bb = aa; bb < 6; bb++) U[aa][bb] = lhs[q++];
-    }
-    for (int j = 0; j < 6; j++) {
-      double sj = U[j][j];
-      for (int kk = 0; kk < j; kk++) sj -= U[kk][j] * U[kk][j];
-      double ujj = sqrt(sj);
-      U[j][j] = ujj;
-      for (int i = j + 1; i < 6; i++) {
-        double t = U[j][i];
-        for (int kk = 0; kk < j; kk++) t -= U[kk][j] * U[kk][i];
-        U[j][i] = t / ujj;
-      }
-    }
-    for (int i = 0; i < 6; i++) {
-      double t = rhs[i];
-      for (int kk = 0; kk < i; kk++) t -= U[kk][i] * rhs[kk];
-      rhs[i] = t / U[i][i];
-    }
-    for (int i = 5; i >= 0; i--) {
-      double t = rhs[i];
-      for (int kk = i + 1; kk < 6; kk++) t -= U[i][kk] * rhs[kk];
-      rhs[i] = t / U[i][i];
-    }
-    double l11 = 2. * rhs[3], l12 = rhs[4], l21 = rhs[4], l22 = 2. * rhs[5];
-    double det = l11 * l22 - l12 * l21, xm0 = 0., xm1 = 0.;
-    if (det > 0) {
-      double idet = 1. / det, r1 = -rhs[1], r2 = -rhs[2];
-      xm0 = idet * (l22 * r1 - l12 * r2);
-      xm1 = idet * (-l21 * r1 + l11 * r2);
-    }
-    double thMinL = sqrt(xm0 * xm0 + xm1 * xm1), phiMinL = atan2(xm1, xm0);
-    double thMin, phiMin, xMin[3];
-    patch_map(th0, phi0, thMinL, phiMinL, thMin, phiMin);
-    spline_interp(sx, m, n, thMin, phiMin, 3, xMin);
-    for (int d = 0; d < 3; d++) xx[d] = xMin[d] - xtar[d];
-    double d2Min = xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2];
-    if (d2Min > d20) break;
-    th0 = thMin;
-    phi0 = phiMin;
-    for (int d = 0; d < 3; d++) x0[d] = xMin[d];
-    h = 0.5 * h;
-  }
 }
 
 // GauLeg (common/ModQuadRule.F90:135-208), device copy (n <= 32)
@@ -779,20 +766,36 @@ __global__ void __launch_bounds__(256) k_self(SingArgs a) {
 }
 
 // Near-singular stage 1: translate the target next to the closest node, project it
-// onto the neighbour's spline surface and apply the distance test of RBC_NearSingInt.
+// onto the neighbour's spline surface (Spline_FindProjection) and apply the distance
+// test of RBC_NearSingInt.  Sixteen lanes per item: each lane evaluates one of the 16
+// patch points of an iteration; the normal equations of the quadratic fit are
+// reduced across the group and solved redundantly by every lane.
 // item record: xi(3) x0(3) th0 phi0 a30(3) dist
-__global__ void __launch_bounds__(128) k_proj(SingArgs a) {
-  int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  int ti = gid / RBC_NBR_MAX, p = gid % RBC_NBR_MAX;
-  if (ti >= a.nT) return;
-  if (p >= a.cnt[ti]) return;
-  size_t s = (size_t)ti * RBC_NBR_MAX + p;
-  int cj = a.slot_cell[s], node = a.slot_node[s];
+__device__ __forceinline__ double grp_sum16(double v) {
+  for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
+  return v;
+}
+
+__global__ void __launch_bounds__(256) k_proj(SingArgs a, int nitems) {
+  const int gid = (blockIdx.x * blockDim.x + threadIdx.x) >> 4, sub = threadIdx.x & 15;
+  if (gid >= nitems) return;
+  // item -> (target, slot): item_off is an exclusive scan over targets
+  int lo = 0, hi = a.nT;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (a.item_off[mid] <= gid)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const int ti = lo, p = gid - a.item_off[lo];
+  const size_t s = (size_t)ti * RBC_NBR_MAX + p;
+  const int cj = a.slot_cell[s], node = a.slot_node[s];
   const CellRec cr = a.cells[cj];
-  int gt = cr.gtype, m = 2 * cr.nlat, n = cr.nlon;
-  int q = node - cr.off, ilat0 = q % cr.nlat, ilon0 = q / cr.nlat;
+  const int gt = cr.gtype, m = 2 * cr.nlat, n = cr.nlon;
+  const int q = node - cr.off, ilat0 = q % cr.nlat, ilon0 = q / cr.nlat;
   double th0 = a.th[gt][ilat0], phi0 = a.phi[gt][ilon0], x0[3], xt[3];
-  size_t np = a.np;
+  const size_t np = a.np;
   for (int d = 0; d < 3; d++) x0[d] = a.x[node + np * d];
   for (int d = 0; d < 3; d++) {
 #pragma clang fp contract(off)
@@ -802,13 +805,82 @@ __global__ void __launch_bounds__(128) k_proj(SingArgs a) {
     xt[d] = x0[d] + xx;
   }
   const double *blk = a.spl + cr.spl;
-  size_t vs = (size_t)m * n * 4;
-  find_projection(blk + vs * SPL_X, m, n, xt, th0, phi0, x0);
+  const size_t vs = (size_t)m * n * 4;
+  const double *sxp = blk + vs * SPL_X;
+  // Spline_FindProjection (common/ModSpline.F90:203-269)
+  spline_interp(sxp, m, n, th0, phi0, 3, x0);
+  double h = fmax(RBC_TWO_PI / m, RBC_TWO_PI / n);
+  const int nth = 2, nphi = 8;
+  for (int iter = 1; iter <= 3; iter++) {
+    double s0 = sin(th0), c0 = cos(th0), xx[3];
+    spline_interp(sxp, m, n, th0, phi0, 3, xx);
+    for (int d = 0; d < 3; d++) xx[d] -= xt[d];
+    double d20 = xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2];
+    // this lane's patch point i = sub + 1
+    int it = sub % nth, ip = sub / nth;
+    double thL = (it + 1) * h / nth, phL = ip * RBC_TWO_PI / nphi;
+    double xi_ = thL * cos(phL), yi_ = thL * sin(phL), thP, phP;
+    patch_point(s0, c0, phi0, thL, phL, thP, phP);
+    spline_interp(sxp, m, n, thP, phP, 3, xx);
+    for (int d = 0; d < 3; d++) xx[d] -= xt[d];
+    double fi_ = xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2];
+    double u[6] = {1., xi_, yi_, xi_ * xi_, xi_ * yi_, yi_ * yi_};
+    double U[6][6], rhs[6];
+    for (int aa = 0; aa < 6; aa++) {
+      for (int bb = aa; bb < 6; bb++) U[aa][bb] = grp_sum16(u[aa] * u[bb]);
+      rhs[aa] = grp_sum16(u[aa] * fi_);
+    }
+    // the patch centre (0,0) with f = d20
+    U[0][0] += 1.;
+    rhs[0] += d20;
+    // dposv('U'): A = U^T U, then two triangular solves
+    for (int j = 0; j < 6; j++) {
+      double sj = U[j][j];
+      for (int kk = 0; kk < j; kk++) sj -= U[kk][j] * U[kk][j];
+      double ujj = sqrt(sj);
+      U[j][j] = ujj;
+      for (int i = j + 1; i < 6; i++) {
+        double t = U[j][i];
+        for (int kk = 0; kk < j; kk++) t -= U[kk][j] * U[kk][i];
+        U[j][i] = t / ujj;
+      }
+    }
+    for (int i = 0; i < 6; i++) {
+      double t = rhs[i];
+      for (int kk = 0; kk < i; kk++) t -= U[kk][i] * rhs[kk];
+      rhs[i] = t / U[i][i];
+    }
+    for (int i = 5; i >= 0; i--) {
+      double t = rhs[i];
+      for (int kk = i + 1; kk < 6; kk++) t -= U[i][kk] * rhs[kk];
+      rhs[i] = t / U[i][i];
+    }
+    // Min_Quad_2D (common/ModBasicMath.F90:296-328)
+    double l11 = 2. * rhs[3], l12 = rhs[4], l21 = rhs[4], l22 = 2. * rhs[5];
+    double det = l11 * l22 - l12 * l21, xm0 = 0., xm1 = 0.;
+    if (det > 0) {
+      double idet = 1. / det, r1 = -rhs[1], r2 = -rhs[2];
+      xm0 = idet * (l22 * r1 - l12 * r2);
+      xm1 = idet * (-l21 * r1 + l11 * r2);
+    }
+    double thMinL = sqrt(xm0 * xm0 + xm1 * xm1), phiMinL = atan2(xm1, xm0);
+    double thMin, phiMin, xMin[3];
+    patch_map(th0, phi0, thMinL, phiMinL, thMin, phiMin);
+    spline_interp(sxp, m, n, thMin, phiMin, 3, xMin);
+    for (int d = 0; d < 3; d++) xx[d] = xMin[d] - xt[d];
+    double d2Min = xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2];
+    if (d2Min > d20) break;
+    th0 = thMin;
+    phi0 = phiMin;
+    for (int d = 0; d < 3; d++) x0[d] = xMin[d];
+    h = 0.5 * h;
+  }
   double a30[3];
   spline_interp(blk + vs * SPL_A3, m, n, th0, phi0, 3, a30);
   double dist = a30[0] * (xt[0] - x0[0]) + a30[1] * (xt[1] - x0[1]) + a30[2] * (xt[2] - x0[2]);
-  size_t it = (size_t)a.item_off[ti] + p;
-  double *r = a.item_rec + 12 * it;
+  if (sub != 0) return;
+  const size_t itm = (size_t)gid;
+  double *r = a.item_rec + 12 * itm;
   for (int d = 0; d < 3; d++) {
     r[d] = xt[d];
     r[3 + d] = x0[d];
@@ -817,10 +889,10 @@ __global__ void __launch_bounds__(128) k_proj(SingArgs a) {
   r[6] = th0;
   r[7] = phi0;
   r[11] = dist;
-  for (int d = 0; d < 3; d++) a.dvitem[3 * it + d] = 0.;
+  for (int d = 0; d < 3; d++) a.dvitem[3 * itm + d] = 0.;
   if (dist > 2 * cr.meshSize) return;
   int slot = atomicAdd(a.work_cnt, 1);
-  a.work[2 * slot] = (int)it;
+  a.work[2 * slot] = (int)itm;
   a.work[2 * slot + 1] = cj;
 }
 
@@ -1091,7 +1163,7 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
                             const int *d_tindx, const int *d_act, const double *d_A, double *d_v) {
   if (c->nrbc == 0 || nT == 0) return RBC3D_OK;
   hipStream_t st = c->stream;
-  int nb = c->nbins;
+  int nb = c->nfbins;
   TypePtrs tpv, *tp = &tpv;
   RBC_TRY(type_ptrs(c, tp));
   // targets binned like the sources
@@ -1113,19 +1185,22 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   for (int d = 0; d < 3; d++) {
     pa.Lb[d] = c->Lb[d];
     pa.iLb[d] = c->iLb[d];
-    pa.Nc[d] = c->Nc[d];
+    pa.Nf[d] = c->Nf[d];
   }
   pa.rc = c->conf.rc;
+  pa.rc2hi = c->conf.rc * c->conf.rc * (1.0 + 1e-12);
+  pa.noffs = (int)c->noffs;
+  pa.offs = c->d_offs.p;
   pa.reps = c->r_eps;
   pa.c1 = c1;
   pa.c2 = c2;
   pa.np = c->np;
   pa.nT = nT;
-  pa.nbins = nb;
+  pa.nfbins = nb;
   pa.nrbc = c->nrbc;
   pa.rec = c->d_src_rec.p;
   pa.src_node = c->d_work.p;
-  pa.bin_start = c->d_bin_start.p;
+  pa.bin_start = c->d_fbin_start.p;
   pa.node_cell = c->d_node_cell.p;
   pa.node_ll = c->d_node_ll.p;
   pa.cells = c->d_cells.p;
@@ -1142,9 +1217,9 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   pa.chunk_off = chunk_off;
   pa.vpair = c->d_vacc.p;
   pa.stat = c->d_pairstat.p;
-  int nblk = nb + (nT + PAIR_T - 1) / PAIR_T + 1;
+  int nblk = nb + (nT + PT - 1) / PT + 1;
   rbc_stage_begin(c, ST_PAIR);
-  k_pair<<<nblk, PAIR_T, 0, st>>>(pa);
+  k_pair<<<nblk, PT, 0, st>>>(pa);
   rbc_stage_end(c, ST_PAIR);
   HIP_TRY(hipGetLastError());
   // neighbouring-cell lists
@@ -1176,7 +1251,7 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   na.slot_dist = c->d_nbr_dist.p;
   na.flag = c->d_flag.p;
   rbc_stage_begin(c, ST_NBR);
-  k_nbr<<<(nT + NBR_T - 1) / NBR_T, NBR_T, 0, st>>>(na);
+  k_nbr<<<(unsigned)(((size_t)nT * 64 + NBR_T - 1) / NBR_T), NBR_T, 0, st>>>(na);
   rbc_stage_end(c, ST_NBR);
   HIP_TRY(hipGetLastError());
   int *item_off = c->d_nbr_cnt.p + nT + 1;
@@ -1249,7 +1324,7 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   HIP_TRY(hipGetLastError());
   if (nitems > 0) {
     rbc_stage_begin(c, ST_PROJ);
-    k_proj<<<(unsigned)(((size_t)nT * RBC_NBR_MAX + 127) / 128), 128, 0, st>>>(sa);
+    k_proj<<<(unsigned)(((size_t)nitems * 16 + 255) / 256), 256, 0, st>>>(sa, nitems);
     rbc_stage_end(c, ST_PROJ);
     HIP_TRY(hipGetLastError());
     int nwork = 0;
