@@ -140,8 +140,10 @@ static void patch_build(double th0, double phi0, int nth, const double *thL, int
 
 template <typename T>
 static int upload(DBuf<T> &b, const std::vector<T> &h, hipStream_t st) {
+  // synchronous: the host vectors are temporaries of the caller
   RBC_TRY(b.ensure(h.size()));
-  if (!h.empty()) HIP_TRY(hipMemcpyAsync(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  if (!h.empty()) HIP_TRY(hipMemcpy(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
   return RBC3D_OK;
 }
 

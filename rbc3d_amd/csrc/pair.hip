@@ -1171,7 +1171,7 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   RBC_TRY(c->d_torder.ensure(nT + 1));
   RBC_TRY(c->d_tbin_start.ensure(nb + 1));
   rbc_stage_begin(c, ST_TSORT);
-  RBC_TRY(rbc_hash_sort(c, nT, d_tx, nT, c->d_tbin.p, c->d_torder.p, c->d_tbin_start.p));
+  RBC_TRY(fine_sort(c, nT, d_tx, nT, c->d_tbin.p, c->d_torder.p, c->d_tbin_start.p));
   rbc_stage_end(c, ST_TSORT);
   RBC_TRY(c->d_flag.ensure(2 * (size_t)(nb + 1) + 8));
   int *chunk_cnt = c->d_flag.p + 8, *chunk_off = chunk_cnt + (nb + 1);
