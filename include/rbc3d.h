@@ -138,6 +138,24 @@ int rbc3d_cell_rhs(rbc3d_ctx *ctx, double *rhs);
 int rbc3d_wall_dof(rbc3d_ctx *ctx, int *dof);
 int rbc3d_wall_matmult(rbc3d_ctx *ctx, const double *u, double *b);
 int rbc3d_wall_residual_vel(rbc3d_ctx *ctx, double *v);
+int rbc3d_get_wall_density(rbc3d_ctx *ctx, double *f);
+/* Compute_Rhs into a device vector (common/ModVelSolver.F90:455-515) */
+int rbc3d_cell_rhs_dev(rbc3d_ctx *ctx, double *d_rhs);
+
+/* --- the two GMRES solves (PETSc KSPGMRES semantics: restart 30, classical
+ *     Gram-Schmidt, no preconditioner, x += V y at restarts) ------------------ */
+/* NoSlipWall (common/ModNoSlip.F90:45-150): residual wall velocity, GMRES for the
+ * traction increment (rtol relative to the initial residual, maxit), f = f0 + finc.
+ * finc0: initial guess of the increment on the de-duplicated unknowns (NULL: zero,
+ * the reference's VecZeroEntries at :117).  vres_max (optional): max |v| of the
+ * residual velocity after the update (the monitor of :140-145). */
+int rbc3d_noslip_wall(rbc3d_ctx *ctx, double rtol, int maxit, const double *finc0, int *niter,
+                      double *rnorm, int *converged, double *vres_max);
+/* Solve_RBC_Vel (common/ModVelSolver.F90:44-132): v(nPoint,3) cell velocities; with
+ * all viscosity ratios 1 the solution is the right-hand side (no GMRES); otherwise
+ * GMRES to 1e-11 (maxit 200) from the previous solution.  The cells' double-layer
+ * density is then set to v (common/ModTimeInt.F90:773-790).  v may be NULL. */
+int rbc3d_solve_rbc_vel(rbc3d_ctx *ctx, double *v, int *niter, double *rnorm);
 
 /* --- diagnostics (no counterpart in the reference, which only prints a per-step
  *     MPI_WTime, common/ModTimeInt.F90:122,167) ---------------------------------

@@ -86,6 +86,10 @@ def lib():
         "orc_fft3d_inverse": (None, [C.c_int, C.c_int, C.c_int, dp, dp]),
         "orc_cell_spline": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp, dp]),
         "orc_pair_checksum": (C.c_ulonglong, [vp]),
+        "orc_noslip_wall": (C.c_int, [vp, C.c_double, C.c_int, C.c_void_p, C.POINTER(C.c_int),
+                                      C.POINTER(C.c_double), C.POINTER(C.c_double)]),
+        "orc_get_wall_density": (C.c_int, [vp, dp]),
+        "orc_solve_rbc_vel": (C.c_int, [vp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
         "orc_near_list": (C.c_int, [vp, C.c_int, dp, ip, C.c_int, ip, ip, ip, dp, C.c_int]),
     }
     for name, (res, args) in sig.items():
@@ -272,6 +276,31 @@ class Oracle:
         d = np.zeros(cap)
         n = L().orc_near_list(self.h, len(Ac), tx, ti, i, cells, il, jl, d, cap)
         return n, cells[:max(n, 0)], il[:max(n, 0)], jl[:max(n, 0)], d[:max(n, 0)]
+
+    def noslip_wall(self, rtol=1e-8, maxit=60, finc0=None):
+        ni = C.c_int()
+        rn = C.c_double()
+        vm = C.c_double()
+        x0p = None
+        if finc0 is not None:
+            x0 = np.ascontiguousarray(finc0, dtype=np.float64)
+            x0p = x0.ctypes.data_as(C.c_void_p)
+        conv = L().orc_noslip_wall(self.h, rtol, maxit, x0p, C.byref(ni), C.byref(rn), C.byref(vm))
+        return ni.value, rn.value, bool(conv), vm.value
+
+    def wall_density(self):
+        f = np.zeros(3 * L().orc_ntargets_wall(self.h))
+        L().orc_get_wall_density(self.h, f)
+        return f
+
+    def solve_rbc_vel(self):
+        if not hasattr(self, "_sol"):
+            self._sol = np.zeros(self.cell_dof())
+        v = np.zeros(3 * self.np)
+        ni = C.c_int()
+        rn = C.c_double()
+        L().orc_solve_rbc_vel(self.h, self._sol, v, C.byref(ni), C.byref(rn))
+        return v, ni.value, rn.value
 
     def pair_checksum(self):
         return L().orc_pair_checksum(self.h)

@@ -2424,3 +2424,185 @@ int orc_wall_residual_vel(orc_sys *s, double *v) {
   free(act);
   return rc;
 }
+
+/* ======================================================================== */
+/* GMRES(30), PETSc KSPGMRES semantics as used by common/ModVelSolver.F90:74-90
+ * and common/ModNoSlip.F90:70-87: classical Gram-Schmidt (no refinement), no
+ * preconditioner, convergence when the Hessenberg residual estimate <=
+ * max(rtol*||r0||, atol), explicit residual at restarts, divergence at 1e5*||r0||. */
+typedef int (*orc_matvec_fn)(orc_sys *, const double *, double *);
+
+static double dotp(int n, const double *a, const double *b) {
+  double s = 0;
+  for (int i = 0; i < n; i++) s += a[i] * b[i];
+  return s;
+}
+
+static int orc_gmres(orc_sys *s, orc_matvec_fn mv, int n, const double *b, double *x, int x0nz,
+                     double rtol, int maxit, int *niter, double *rnorm) {
+  const int m = 30;
+  double *V = malloc(sizeof(double) * (size_t)n * (m + 1));
+  double *H = calloc((size_t)(m + 1) * m, sizeof(double)), cs[30], sn[30], g[31], h[31];
+  int its = 0, reason = 0, first = 1;
+  double ttol = 0, rnorm0 = 0, res = 0;
+  if (!x0nz) memset(x, 0, sizeof(double) * n);
+  while (1) {
+    if (x0nz || !first) {
+      mv(s, x, V);
+      for (int i = 0; i < n; i++) V[i] = b[i] - V[i];
+    } else {
+      memcpy(V, b, sizeof(double) * n);
+    }
+    res = sqrt(dotp(n, V, V));
+    if (first) {
+      rnorm0 = res;
+      ttol = fmax(rtol * rnorm0, 1e-50);
+    }
+    first = 0;
+    if (res == 0.0 || res <= ttol) {
+      reason = 1;
+      break;
+    }
+    for (int i = 0; i < n; i++) V[i] *= 1.0 / res;
+    for (int i = 0; i <= m; i++) g[i] = 0;
+    g[0] = res;
+    int it = 0, happy = 0;
+    while (it < m && its < maxit) {
+      double *vj = V + (size_t)n * it, *vn = V + (size_t)n * (it + 1);
+      mv(s, vj, vn);
+      for (int i = 0; i <= it; i++) h[i] = dotp(n, V + (size_t)n * i, vn);
+      for (int q = 0; q < n; q++) {
+        double t = vn[q];
+        for (int i = 0; i <= it; i++) t -= h[i] * V[(size_t)n * i + q];
+        vn[q] = t;
+      }
+      double tt = sqrt(dotp(n, vn, vn));
+      for (int i = 0; i <= it; i++) H[(size_t)i * m + it] = h[i];
+      H[(size_t)(it + 1) * m + it] = tt;
+      if (tt > fabs(g[it]) * 1.0e-30)
+        for (int q = 0; q < n; q++) vn[q] *= 1.0 / tt;
+      else
+        happy = 1;
+      for (int i = 0; i < it; i++) {
+        double a = H[(size_t)i * m + it], bb = H[(size_t)(i + 1) * m + it];
+        H[(size_t)i * m + it] = cs[i] * a + sn[i] * bb;
+        H[(size_t)(i + 1) * m + it] = -sn[i] * a + cs[i] * bb;
+      }
+      double a = H[(size_t)it * m + it], bb = H[(size_t)(it + 1) * m + it];
+      double tden = sqrt(a * a + bb * bb);
+      if (tden != 0.0) {
+        cs[it] = a / tden;
+        sn[it] = bb / tden;
+        H[(size_t)it * m + it] = tden;
+        H[(size_t)(it + 1) * m + it] = 0.;
+        g[it + 1] = -sn[it] * g[it];
+        g[it] = cs[it] * g[it];
+      } else {
+        cs[it] = 1;
+        sn[it] = 0;
+        g[it + 1] = 0;
+      }
+      res = fabs(g[it + 1]);
+      it++;
+      its++;
+      if (res <= ttol) {
+        reason = 1;
+        break;
+      }
+      if (res >= 1e5 * rnorm0) {
+        reason = -1;
+        break;
+      }
+      if (happy) {
+        reason = 1;
+        break;
+      }
+    }
+    double y[30];
+    for (int i = it - 1; i >= 0; i--) {
+      double t = g[i];
+      for (int k = i + 1; k < it; k++) t -= H[(size_t)i * m + k] * y[k];
+      y[i] = t / H[(size_t)i * m + i];
+    }
+    for (int q = 0; q < n; q++) {
+      double t = x[q];
+      for (int j = 0; j < it; j++) t += y[j] * V[(size_t)n * j + q];
+      x[q] = t;
+    }
+    if (reason != 0 || its >= maxit) break;
+  }
+  free(V);
+  free(H);
+  *niter = its;
+  *rnorm = res;
+  return reason > 0;
+}
+
+/* NoSlipWall: common/ModNoSlip.F90:45-150 */
+int orc_noslip_wall(orc_sys *s, double rtol, int maxit, const double *finc0, int *niter,
+                    double *rnorm, double *vres_max) {
+  orc_wall *w = &s->wall;
+  int nv = w->nvert, nd = w->ndof;
+  if (!w->rowptr) orc_prepare_sing_int_on_wall(s);
+  double *f0 = malloc(sizeof(double) * 3 * nv), *v = malloc(sizeof(double) * 3 * nv);
+  double *rhs = malloc(sizeof(double) * nd), *x = calloc(nd, sizeof(double));
+  memcpy(f0, w->f, sizeof(double) * 3 * nv);
+  orc_wall_residual_vel(s, v);
+  for (int i = 0; i < 3 * nv; i++) v[i] = -v[i];
+  assemble(w, v, rhs, 1);
+  if (finc0) memcpy(x, finc0, sizeof(double) * nd);
+  int conv = orc_gmres(s, orc_wall_matmult, nd, rhs, x, finc0 != NULL, rtol, maxit, niter, rnorm);
+  double *finc = malloc(sizeof(double) * 3 * nv);
+  assemble(w, finc, x, -1);
+  for (int i = 0; i < 3 * nv; i++) w->f[i] = f0[i] + finc[i];
+  if (vres_max) {
+    orc_wall_residual_vel(s, v);
+    double mx = 0;
+    for (int i = 0; i < 3 * nv; i++) mx = fmax(mx, fabs(v[i]));
+    *vres_max = mx;
+  }
+  free(f0);
+  free(v);
+  free(rhs);
+  free(x);
+  free(finc);
+  return conv;
+}
+
+int orc_get_wall_density(const orc_sys *s, double *f) {
+  memcpy(f, s->wall.f, sizeof(double) * 3 * s->wall.nvert);
+  return 0;
+}
+
+/* Solve_RBC_Vel: common/ModVelSolver.F90:44-132 (+ g := v, common/ModTimeInt.F90:773-790);
+ * sol0: previous solution (in/out, length cell_dof), the nonzero initial guess */
+int orc_solve_rbc_vel(orc_sys *s, double *sol, double *v, int *niter, double *rnorm) {
+  int dof = orc_cell_dof(s), nP = s->slr.nPoint;
+  double *rhs = malloc(sizeof(double) * dof);
+  orc_cell_rhs(s, rhs);
+  int all_one = 1;
+  for (int l = 0; l < s->conf.ntypes; l++) all_one = all_one && s->conf.viscRat[l] == 1.0;
+  *niter = 0;
+  *rnorm = 0;
+  if (all_one)
+    memcpy(sol, rhs, sizeof(double) * dof);
+  else
+    orc_gmres(s, orc_cell_matmult, dof, rhs, sol, 1, 1e-11, 200, niter, rnorm);
+  double *c = malloc(sizeof(double) * dof);
+  memcpy(c, sol, sizeof(double) * dof);
+  orc_glob_sph_trans(s, v, c, -1);
+  /* g := v */
+  double *gc = malloc(sizeof(double) * 3 * nP);
+  size_t off = 0;
+  for (int ir = 0; ir < s->nrbc; ir++) {
+    int nvert = s->rbcs[ir].nlat * s->rbcs[ir].nlon;
+    for (int d = 0; d < 3; d++)
+      for (int q = 0; q < nvert; q++) gc[3 * off + q + (size_t)nvert * d] = v[off + q + (size_t)nP * d];
+    off += nvert;
+  }
+  orc_set_cell_density(s, 2, gc);
+  free(gc);
+  free(c);
+  free(rhs);
+  return 0;
+}

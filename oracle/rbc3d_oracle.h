@@ -108,6 +108,10 @@ void orc_fft3d_inverse(int N1, int N2, int N3, const double *in_cplx, double *ou
 int orc_cell_spline(const orc_sys *s, int irbc, int which, double *u, double *u1, double *u2,
                     double *u12);
 unsigned long long orc_pair_checksum(const orc_sys *s);
+int orc_noslip_wall(orc_sys *s, double rtol, int maxit, const double *finc0, int *niter,
+                    double *rnorm, double *vres_max);
+int orc_get_wall_density(const orc_sys *s, double *f);
+int orc_solve_rbc_vel(orc_sys *s, double *sol, double *v, int *niter, double *rnorm);
 int orc_near_list(orc_sys *s, int nT, const double *tx, const int *tindx, int i, int *cells,
                   int *ilat, int *ilon, double *dist, int cap);
 

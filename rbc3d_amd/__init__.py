@@ -67,6 +67,11 @@ _SIG = {
     "rbc3d_wall_residual_vel": [C.c_void_p, dp],
     "rbc3d_stream": [C.c_void_p, C.POINTER(C.c_void_p)],
     "rbc3d_sync": [C.c_void_p],
+    "rbc3d_get_wall_density": [C.c_void_p, dp],
+    "rbc3d_cell_rhs_dev": [C.c_void_p, C.c_void_p],
+    "rbc3d_noslip_wall": [C.c_void_p, C.c_double, C.c_int, C.c_void_p, C.POINTER(C.c_int),
+                          C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double)],
+    "rbc3d_solve_rbc_vel": [C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)],
     "rbc3d_timing": [C.c_void_p, C.c_int],
     "rbc3d_stage_time": [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                          C.POINTER(C.c_char_p)],
@@ -330,6 +335,34 @@ class System:
         v = np.zeros(3 * self.nvert)
         _call("rbc3d_wall_residual_vel", self.h, v)
         return v
+
+    def wall_density(self):
+        f = np.zeros(3 * self.nvert)
+        _call("rbc3d_get_wall_density", self.h, f)
+        return f
+
+    def noslip_wall(self, rtol=1e-8, maxit=60, finc0=None):
+        """NoSlipWall: returns (niter, residual estimate, converged, max |v| after)."""
+        ni = C.c_int()
+        rn = C.c_double()
+        cv = C.c_int()
+        vm = C.c_double()
+        x0 = None
+        if finc0 is not None:
+            x0 = _f64(finc0)
+            x0p = x0.ctypes.data_as(C.c_void_p)
+        else:
+            x0p = None
+        _call("rbc3d_noslip_wall", self.h, rtol, maxit, x0p, C.byref(ni), C.byref(rn), C.byref(cv),
+              C.byref(vm))
+        return ni.value, rn.value, bool(cv.value), vm.value
+
+    def solve_rbc_vel(self):
+        v = np.zeros(3 * self.np)
+        ni = C.c_int()
+        rn = C.c_double()
+        _call("rbc3d_solve_rbc_vel", self.h, v.ctypes.data_as(C.c_void_p), C.byref(ni), C.byref(rn))
+        return v, ni.value, rn.value
 
     def stream(self):
         s = C.c_void_p()

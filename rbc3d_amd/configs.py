@@ -348,6 +348,15 @@ def _placement(key, make):
     return cs, rots
 
 
+def make_small(ncell=1, nlat=7, L=8.0, Nb=(16, 16, 16), P=6, xi=2.0, viscRat=5.0, seed=1):
+    """A few coarse cells in a cube (test-sized operators)."""
+    rng = np.random.default_rng(seed)
+    cs = place_centres(rng, ncell, (L, L, L), 2.2)
+    cells = build_cells(rng, ["disc"] * ncell, [nlat] * ncell, [1.0] * ncell, cs, [1] * ncell)
+    return Workload(f"small{ncell}x{nlat}", (L, L, L), alpha_ref_from_xi(xi), 2.0, tuple(Nb), P,
+                    (viscRat,), (0.0, 0.0, 0.0), cells)
+
+
 def make_workload(name: str, seed: int = 1, ncell: int = None, Nb: tuple = None) -> Workload:
     """The benchmark shapes.  ``ncell``/``Nb`` shrink a shape for quick tests."""
     rng = np.random.default_rng(seed)
@@ -379,6 +388,16 @@ def make_workload(name: str, seed: int = 1, ncell: int = None, Nb: tuple = None)
         xi = xi_for_tolerance(2.0, 1e-8)
         wall = make_tube()
         return Workload("C3", Lb, alpha_ref_from_xi(xi), 2.0, Nb or (96, 96, 180), 6,
+                        (1.0,), (0.0, 0.0, 1.0), cells, wall)
+    if name == "C3s":
+        # a small tube for parity tests: radius 4, length 16, 40 x 50 quads
+        n = ncell or 8
+        Lb = (9.0, 9.0, 16.0)
+        cs = place_centres(rng, n, Lb, 2.2, cyl=(4.5, 4.5, 4.0 - 1.3))
+        cells = build_cells(rng, ["disc"] * n, [13] * n, [1.0] * n, cs, [1] * n)
+        xi = xi_for_tolerance(2.0, 1e-8)
+        wall = make_tube(R=4.0, Lz=16.0, nazm=40, nax=50, centre=(4.5, 4.5))
+        return Workload("C3s", Lb, alpha_ref_from_xi(xi), 2.0, Nb or (32, 32, 60), 6,
                         (1.0,), (0.0, 0.0, 1.0), cells, wall)
     if name == "C5":
         n1, n2 = (3584, 512) if ncell is None else (ncell - ncell // 8, ncell // 8)

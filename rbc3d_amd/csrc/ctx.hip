@@ -468,6 +468,7 @@ extern "C" int rbc3d_destroy(rbc3d_ctx *c) {
   c->d_tp_ptr.release();
   c->d_tp_int.release();
   c->d_offs.release();
+  c->d_cell_sol.release();
   c->d_tmp_f.release();
   c->d_fbin_start.release();
   rbc_stage_flush(c);
@@ -808,7 +809,7 @@ __global__ void k_add(double *v, const double *__restrict__ g, size_t n) {
 }
 
 // scatter node-ordered g (np,3) into the cell density and rebuild splines + list density
-static int set_g_from_nodes_dev(rbc3d_ctx *c, const double *d_gnodes) {
+int rbc_set_g_nodes(rbc3d_ctx *c, const double *d_gnodes) {
   HIP_TRY(hipMemcpyAsync(c->d_g.p, d_gnodes, sizeof(double) * 3 * c->np, hipMemcpyDeviceToDevice,
                          c->stream));
   return rbc_update_density(c, 2);
@@ -837,7 +838,7 @@ extern "C" int rbc3d_cell_matmult_dev(rbc3d_ctx *c, const double *d_u, double *d
   RBC_TRY(c->d_vacc.ensure((size_t)3 * np + 1));
   // g = synthesis(u); rbc%g <- g; Rbc_BuildSurfaceSource(gFlag); SourceList_UpdateDensity(G)
   RBC_TRY(rbc_sh_glob(c, d_u, c->d_shbuf.p, -1));
-  RBC_TRY(set_g_from_nodes_dev(c, c->d_shbuf.p));
+  RBC_TRY(rbc_set_g_nodes(c, c->d_shbuf.p));
   RBC_TRY(ensure_rbc_targets(c));
   HIP_TRY(hipMemsetAsync(c->d_v.p, 0, sizeof(double) * 3 * np, st));
   double c1 = 0., c2 = -1. / (4. * RBC_PI);
@@ -867,8 +868,8 @@ extern "C" int rbc3d_cell_matmult(rbc3d_ctx *c, const double *u, double *b) {
   return RBC3D_OK;
 }
 
-extern "C" int rbc3d_cell_rhs(rbc3d_ctx *c, double *rhs) {
-  if (!c || !rhs) return RBC3D_EINVAL;
+extern "C" int rbc3d_cell_rhs_dev(rbc3d_ctx *c, double *d_rhs) {
+  if (!c || !d_rhs) return RBC3D_EINVAL;
   if (c->nrbc == 0) return RBC3D_ESTATE;
   int np = c->np, dof;
   rbc3d_cell_dof(c, &dof);
@@ -887,10 +888,18 @@ extern "C" int rbc3d_cell_rhs(rbc3d_ctx *c, double *rhs) {
   RBC_TRY(rbc_pme_gather(c, np, c->d_tx.p, c->d_tact.p, c->d_tA.p, c->d_v.p));
   k_axpy_bkg<<<(np + 255) / 256, 256, 0, st>>>(c->d_v.p, np, c->d_tA.p, 2. * c->conf.vBkg[0],
                                                 2. * c->conf.vBkg[1], 2. * c->conf.vBkg[2], 1);
+  RBC_TRY(rbc_sh_glob(c, c->d_v.p, d_rhs, 1));
+  return RBC3D_OK;
+}
+
+extern "C" int rbc3d_cell_rhs(rbc3d_ctx *c, double *rhs) {
+  if (!c || !rhs) return RBC3D_EINVAL;
+  int dof;
+  rbc3d_cell_dof(c, &dof);
   RBC_TRY(c->d_shbuf2.ensure(dof + 1));
-  RBC_TRY(rbc_sh_glob(c, c->d_v.p, c->d_shbuf2.p, 1));
-  HIP_TRY(hipMemcpyAsync(rhs, c->d_shbuf2.p, (size_t)dof * 8, hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
+  RBC_TRY(rbc3d_cell_rhs_dev(c, c->d_shbuf2.p));
+  HIP_TRY(hipMemcpyAsync(rhs, c->d_shbuf2.p, (size_t)dof * 8, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return RBC3D_OK;
 }
 

@@ -1,0 +1,338 @@
+// Restarted GMRES on the device with PETSc KSPGMRES semantics (the solver both
+// reference solves use: common/ModVelSolver.F90:74-90, common/ModNoSlip.F90:70-87):
+// restart 30, classical Gram-Schmidt without refinement, no preconditioner, the
+// residual estimate from the Givens-rotated Hessenberg system, convergence when the
+// estimate drops to max(rtol * ||r0||, atol), explicit residual b - A x at restarts.
+//
+// Vectors stay in HBM; the (restart+1) x restart Hessenberg matrix and the
+// rotations live on the host (a few hundred doubles).
+#include <cmath>
+#include <functional>
+#include <vector>
+
+#include "internal.h"
+
+#define G_T 256
+#define G_MAXV 64
+
+// partial dot products of w with V[0..k) (column stride ld), one row of partials per block
+__global__ void __launch_bounds__(G_T) k_mdot(int n, int k, const double *__restrict__ V, size_t ld,
+                                              const double *__restrict__ w, double *part) {
+  __shared__ double red[G_T];
+  for (int j = 0; j < k; j++) {
+    double s = 0;
+    for (int i = blockIdx.x * G_T + threadIdx.x; i < n; i += gridDim.x * G_T) s += V[j * ld + i] * w[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = G_T / 2; o > 0; o >>= 1) {
+      if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[(size_t)j * gridDim.x + blockIdx.x] = red[0];
+    __syncthreads();
+  }
+}
+
+__global__ void k_mdot_final(int k, int nb, const double *__restrict__ part, double *out) {
+  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= k) return;
+  double s = 0;
+  for (int b = 0; b < nb; b++) s += part[(size_t)j * nb + b];
+  out[j] = s;
+}
+
+// w -= sum_j h_j V_j
+__global__ void k_maxpy(int n, int k, const double *__restrict__ V, size_t ld,
+                        const double *__restrict__ h, double *w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = w[i];
+  for (int j = 0; j < k; j++) s -= h[j] * V[j * ld + i];
+  w[i] = s;
+}
+
+// x += sum_j y_j V_j
+__global__ void k_update(int n, int k, const double *__restrict__ V, size_t ld,
+                         const double *__restrict__ y, double *x) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double s = x[i];
+  for (int j = 0; j < k; j++) s += y[j] * V[j * ld + i];
+  x[i] = s;
+}
+
+__global__ void k_scale(int n, double a, const double *__restrict__ x, double *y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = a * x[i];
+}
+
+__global__ void k_sub(int n, const double *__restrict__ b, double *r) {  // r = b - r
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) r[i] = b[i] - r[i];
+}
+
+struct GmresWork {
+  DBuf<double> V, w, part, hd;
+};
+
+static int dots(hipStream_t st, GmresWork &ws, int n, int k, const double *V, size_t ld,
+                const double *w, double *h_host) {
+  int nb = std::min(1024, (n + G_T - 1) / G_T);
+  RBC_TRY(ws.part.ensure((size_t)nb * G_MAXV + 1));
+  RBC_TRY(ws.hd.ensure(G_MAXV + 1));
+  k_mdot<<<nb, G_T, 0, st>>>(n, k, V, ld, w, ws.part.p);
+  k_mdot_final<<<1, 64, 0, st>>>(k, nb, ws.part.p, ws.hd.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(h_host, ws.hd.p, sizeof(double) * k, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return RBC3D_OK;
+}
+
+int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, double *)> &matvec,
+              const double *d_b, double *d_x, bool x0_nonzero, double rtol, double atol,
+              double dtol, int maxit, int restart, int *niter, double *rnorm, int *reason,
+              std::vector<double> *history) {
+  hipStream_t st = c->stream;
+  if (restart < 1 || restart > G_MAXV - 1) return RBC3D_EINVAL;
+  GmresWork ws;
+  size_t ld = (size_t)n + 64;
+  RBC_TRY(ws.V.ensure(ld * (restart + 1)));
+  RBC_TRY(ws.w.ensure(n + 1));
+  double *V = ws.V.p;
+  const int nb = (n + 255) / 256;
+  int its = 0;
+  double ttol = 0, rnorm0 = 0, res = 0;
+  *reason = 0;
+  if (!x0_nonzero) HIP_TRY(hipMemsetAsync(d_x, 0, sizeof(double) * n, st));
+  std::vector<double> H((size_t)(restart + 1) * restart), cs(restart), sn(restart), g(restart + 1),
+      h(restart + 1);
+  bool first = true;
+  while (true) {
+    // r = b - A x  (KSPInitialResidual); with x = 0 this is b
+    if (x0_nonzero || !first) {
+      RBC_TRY(matvec(d_x, V));
+      k_sub<<<nb, 256, 0, st>>>(n, d_b, V);
+    } else {
+      HIP_TRY(hipMemcpyAsync(V, d_b, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    }
+    double nrm2;
+    RBC_TRY(dots(st, ws, n, 1, V, ld, V, &nrm2));
+    res = sqrt(nrm2);
+    if (first) {
+      rnorm0 = res;
+      ttol = std::max(rtol * rnorm0, atol);  // KSPConvergedDefault
+      if (history) history->push_back(res);
+    }
+    first = false;
+    if (res == 0.0 || res <= ttol) {
+      *reason = 1;
+      break;
+    }
+    k_scale<<<nb, 256, 0, st>>>(n, 1.0 / res, V, V);
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = res;
+    int it = 0;
+    bool happy = false;
+    while (it < restart && its < maxit) {
+      double *vj = V + ld * it, *vn = V + ld * (it + 1);
+      RBC_TRY(matvec(vj, vn));
+      // classical Gram-Schmidt: h = V^T w, w -= V h
+      RBC_TRY(dots(st, ws, n, it + 1, V, ld, vn, h.data()));
+      HIP_TRY(hipMemcpyAsync(ws.hd.p, h.data(), sizeof(double) * (it + 1), hipMemcpyHostToDevice, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      k_maxpy<<<nb, 256, 0, st>>>(n, it + 1, V, ld, ws.hd.p, vn);
+      double tt2;
+      RBC_TRY(dots(st, ws, n, 1, vn, ld, vn, &tt2));
+      double tt = sqrt(tt2);
+      for (int i = 0; i <= it; i++) H[(size_t)i * restart + it] = h[i];
+      H[(size_t)(it + 1) * restart + it] = tt;
+      // happy breakdown bound as in PETSc: haptol * |g(it)|
+      double hapbnd = fabs(g[it]) * 1.0e-30;
+      if (tt > hapbnd) {
+        k_scale<<<nb, 256, 0, st>>>(n, 1.0 / tt, vn, vn);
+      } else {
+        happy = true;
+      }
+      // apply the previous rotations and form a new one (KSPGMRESUpdateHessenberg)
+      for (int i = 0; i < it; i++) {
+        double a = H[(size_t)i * restart + it], b = H[(size_t)(i + 1) * restart + it];
+        H[(size_t)i * restart + it] = cs[i] * a + sn[i] * b;
+        H[(size_t)(i + 1) * restart + it] = -sn[i] * a + cs[i] * b;
+      }
+      double a = H[(size_t)it * restart + it], b = H[(size_t)(it + 1) * restart + it];
+      double tden = sqrt(a * a + b * b);
+      if (tden != 0.0) {
+        cs[it] = a / tden;
+        sn[it] = b / tden;
+        H[(size_t)it * restart + it] = tden;
+        H[(size_t)(it + 1) * restart + it] = 0.;
+        g[it + 1] = -sn[it] * g[it];
+        g[it] = cs[it] * g[it];
+      } else {
+        cs[it] = 1.;
+        sn[it] = 0.;
+        g[it + 1] = 0.;
+      }
+      res = fabs(g[it + 1]);
+      it++;
+      its++;
+      if (history) history->push_back(res);
+      if (res <= ttol) {
+        *reason = 1;
+        break;
+      }
+      if (dtol > 0 && res >= dtol * rnorm0) {
+        *reason = -1;
+        break;
+      }
+      if (happy) {
+        *reason = 1;
+        break;
+      }
+    }
+    // x += V y with H y = g (upper triangular)
+    std::vector<double> y(it, 0.);
+    for (int i = it - 1; i >= 0; i--) {
+      double s = g[i];
+      for (int k = i + 1; k < it; k++) s -= H[(size_t)i * restart + k] * y[k];
+      y[i] = s / H[(size_t)i * restart + i];
+    }
+    if (it > 0) {
+      HIP_TRY(hipMemcpyAsync(ws.hd.p, y.data(), sizeof(double) * it, hipMemcpyHostToDevice, st));
+      HIP_TRY(hipStreamSynchronize(st));
+      k_update<<<nb, 256, 0, st>>>(n, it, V, ld, ws.hd.p, d_x);
+      HIP_TRY(hipGetLastError());
+    }
+    if (*reason != 0 || its >= maxit) break;
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  *niter = its;
+  *rnorm = res;
+  if (*reason == 0) *reason = -3;  // KSP_DIVERGED_ITS
+  ws.V.release();
+  ws.w.release();
+  ws.part.release();
+  ws.hd.release();
+  return RBC3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+int rbc_wall_matmult_dev(rbc3d_ctx *c, const double *d_u, double *d_b);
+int rbc_wall_residual_dev(rbc3d_ctx *c);
+
+__global__ void k_neg_assemble(int nuniq, int nvert, const int *__restrict__ last,
+                               const double *__restrict__ v, double *u1d) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nuniq) return;
+  int i = last[g];
+  for (int d = 0; d < 3; d++) u1d[3 * g + d] = -v[i + (size_t)nvert * d];
+}
+__global__ void k_add_assembled(int nvert, const int *__restrict__ glb, const double *__restrict__ f0,
+                                const double *__restrict__ u1d, double *f) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nvert) return;
+  int g = glb[i] - 1;
+  for (int d = 0; d < 3; d++) f[i + (size_t)nvert * d] = f0[i + (size_t)nvert * d] + u1d[3 * g + d];
+}
+__global__ void k_absmax(int n, const double *__restrict__ v, double *out) {
+  __shared__ double red[256];
+  double m = 0;
+  for (int i = threadIdx.x; i < n; i += 256) m = fmax(m, fabs(v[i]));
+  red[threadIdx.x] = m;
+  __syncthreads();
+  for (int o = 128; o > 0; o >>= 1) {
+    if (threadIdx.x < o) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + o]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// NoSlipWall (common/ModNoSlip.F90:45-150)
+extern "C" int rbc3d_noslip_wall(rbc3d_ctx *c, double rtol, int maxit, const double *finc0,
+                                 int *niter, double *rnorm, int *converged, double *vres_max) {
+  if (!c || !niter || !rnorm || !converged) return RBC3D_EINVAL;
+  WallData &w = c->wall;
+  if (!w.nvert) return RBC3D_ESTATE;
+  hipStream_t st = c->stream;
+  int nv = w.nvert, nd = w.ndof, nu = nd / 3;
+  DBuf<double> f0, rhs, x, tmpv;
+  RBC_TRY(f0.ensure((size_t)3 * nv));
+  RBC_TRY(rhs.ensure(nd + 1));
+  RBC_TRY(x.ensure(nd + 1));
+  HIP_TRY(hipMemcpyAsync(f0.p, w.d_f.p, sizeof(double) * 3 * nv, hipMemcpyDeviceToDevice, st));
+  // rhs = -(residual velocity), assembled on the unique vertices
+  RBC_TRY(rbc_wall_residual_dev(c));
+  k_neg_assemble<<<(nu + 255) / 256, 256, 0, st>>>(nu, nv, w.d_glb.p + nv, c->d_v.p, rhs.p);
+  HIP_TRY(hipGetLastError());
+  bool x0nz = finc0 != nullptr;
+  if (x0nz) HIP_TRY(hipMemcpyAsync(x.p, finc0, sizeof(double) * nd, hipMemcpyHostToDevice, st));
+  int reason = 0;
+  auto mv = [c](const double *u, double *b) { return rbc_wall_matmult_dev(c, u, b); };
+  RBC_TRY(rbc_gmres(c, nd, mv, rhs.p, x.p, x0nz, rtol, 1e-50, 1e5, maxit, 30, niter, rnorm, &reason,
+                    nullptr));
+  *converged = reason > 0;
+  // f = f0 + finc (AssembleArray(finc, finc1D, -1))
+  k_add_assembled<<<(nv + 255) / 256, 256, 0, st>>>(nv, w.d_glb.p, f0.p, x.p, w.d_f.p);
+  HIP_TRY(hipGetLastError());
+  if (vres_max) {
+    RBC_TRY(rbc_wall_residual_dev(c));
+    RBC_TRY(tmpv.ensure(1));
+    k_absmax<<<1, 256, 0, st>>>(3 * nv, c->d_v.p, tmpv.p);
+    HIP_TRY(hipMemcpyAsync(vres_max, tmpv.p, sizeof(double), hipMemcpyDeviceToHost, st));
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  f0.release();
+  rhs.release();
+  x.release();
+  tmpv.release();
+  return RBC3D_OK;
+}
+
+extern "C" int rbc3d_get_wall_density(rbc3d_ctx *c, double *f) {
+  if (!c || !f) return RBC3D_EINVAL;
+  if (!c->wall.nvert) return RBC3D_ESTATE;
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  HIP_TRY(hipMemcpy(f, c->wall.d_f.p, sizeof(double) * 3 * c->wall.nvert, hipMemcpyDeviceToHost));
+  return RBC3D_OK;
+}
+
+// Solve_RBC_Vel (common/ModVelSolver.F90:44-132): v(nPoint,3) on return; the cell
+// double-layer density is then set to v (common/ModTimeInt.F90:773-790)
+extern "C" int rbc3d_cell_rhs_dev(rbc3d_ctx *c, double *d_rhs);
+extern "C" int rbc3d_cell_matmult_dev(rbc3d_ctx *c, const double *d_u, double *d_b);
+int rbc_set_g_nodes(rbc3d_ctx *c, const double *d_gnodes);
+
+extern "C" int rbc3d_solve_rbc_vel(rbc3d_ctx *c, double *v, int *niter, double *rnorm) {
+  if (!c || !niter || !rnorm) return RBC3D_EINVAL;
+  if (c->nrbc == 0) return RBC3D_ESTATE;
+  hipStream_t st = c->stream;
+  int dof;
+  RBC_TRY(rbc3d_cell_dof(c, &dof));
+  DBuf<double> rhs;
+  RBC_TRY(rhs.ensure(dof + 1));
+  RBC_TRY(c->d_cell_sol.ensure(dof + 1));
+  if (!c->cell_sol_init) {
+    HIP_TRY(hipMemsetAsync(c->d_cell_sol.p, 0, sizeof(double) * dof, st));
+    c->cell_sol_init = true;
+  }
+  RBC_TRY(rbc3d_cell_rhs_dev(c, rhs.p));
+  bool all_one = true;
+  for (int l = 0; l < c->conf.ntypes; l++) all_one = all_one && c->conf.viscRat[l] == 1.0;
+  *niter = 0;
+  *rnorm = 0;
+  if (all_one) {
+    HIP_TRY(hipMemcpyAsync(c->d_cell_sol.p, rhs.p, sizeof(double) * dof, hipMemcpyDeviceToDevice, st));
+  } else {
+    int reason = 0;
+    auto mv = [c](const double *u, double *b) { return rbc3d_cell_matmult_dev(c, u, b); };
+    RBC_TRY(rbc_gmres(c, dof, mv, rhs.p, c->d_cell_sol.p, true, 1e-11, 1e-50, 1e5, 200, 30, niter,
+                      rnorm, &reason, nullptr));
+  }
+  RBC_TRY(c->d_shbuf.ensure((size_t)3 * c->np + 1));
+  RBC_TRY(rbc_sh_glob(c, c->d_cell_sol.p, c->d_shbuf.p, -1));
+  RBC_TRY(rbc_set_g_nodes(c, c->d_shbuf.p));
+  if (v) HIP_TRY(hipMemcpyAsync(v, c->d_shbuf.p, sizeof(double) * 3 * c->np, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  rhs.release();
+  return RBC3D_OK;
+}

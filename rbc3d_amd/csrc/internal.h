@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -181,6 +182,9 @@ struct rbc3d_ctx {
   DBuf<int> d_tp_int;
   size_t tp_ntypes = 0;
   StageTimer timer;
+  // persistent cell-BIE solution (KSPSetInitialGuessNonzero, common/ModVelSolver.F90:87)
+  DBuf<double> d_cell_sol;
+  bool cell_sol_init = false;
   // last pair statistics
   long long last_npairs = 0;
   unsigned long long last_checksum = 0;
@@ -316,5 +320,9 @@ int rbc_wall_setup(rbc3d_ctx *c);
 int rbc_wall_prepare(rbc3d_ctx *c);
 int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double c1, int nT, const double *d_tx, const int *d_tindx,
                              const int *d_act, const double *d_A, double *d_v);
+int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, double *)> &matvec,
+              const double *d_b, double *d_x, bool x0_nonzero, double rtol, double atol,
+              double dtol, int maxit, int restart, int *niter, double *rnorm, int *reason,
+              std::vector<double> *history);
 int rbc_hash_sort(rbc3d_ctx *c, int n, const double *d_x, int ldx, int *d_bin, int *d_order,
                   int *d_bin_start);

@@ -1,46 +1,801 @@
-// Wall (vessel) surfaces: placeholder entry points until the wall kernels land.
+// Wall (vessel) surfaces as Tri3 meshes (common/ModIntOnWalls.F90, ModWall.F90,
+// ModNoSlip.F90):
+//   * the wall source list (triangle centroids) on the reference's cell list, so the
+//     visited (target, triangle) set is the reference's 27-bin set;
+//   * AddIntOnWalls for targets off the wall: one lane per target, 7-point rule or the
+//     Duffy rule near the triangle (Tri_Int_Regular/Tri_Int_Duffy, MinDistToTri);
+//   * PrepareSingIntOnWall as a 3x3-block sparse matrix assembled on the GPU: count,
+//     fill per-contribution blocks, segmented sort by column, sum duplicates;
+//   * the wall no-slip operator (MyMatMult :255-308) = block SpMV + wall-only PME,
+//     and Compute_Wall_Residual_Vel :153-193.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+
 #include "internal.h"
 
-int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double, int, const double *, const int *, const int *,
-                             const double *, double *) {
-  (void)c;
-  rbc_set_error("wall integrals are not available in this build");
-  return RBC3D_EINVAL;
+struct TriRules {
+  double r7[7][2], w7[7];  // gqTri7 (common/ModQuadRule.F90:70-94)
+  double rd[4], wd[4];     // GauLeg(0,1,4) for the Duffy rule (Tri_Int_Duffy :398-404)
+};
+
+static void gauleg_h(double x1, double x2, int n, double *x, double *w) {
+  int m = (n + 1) / 2;
+  double xm = 0.5 * (x2 + x1), xl = 0.5 * (x2 - x1);
+  for (int j = 0; j < m; j++) {
+    double z = cos(RBC_PI * ((j + 1) - 0.25) / (n + 0.5)), z1, p1 = 0, p2, p3, pp = 0;
+    bool unfinished = true;
+    for (int its = 1; its <= 10 && unfinished; its++) {
+      p1 = 1.0;
+      p2 = 0.0;
+      for (int jj = 1; jj <= n; jj++) {
+        p3 = p2;
+        p2 = p1;
+        p1 = ((2.0 * jj - 1.0) * z * p2 - (jj - 1.0) * p3) / jj;
+      }
+      pp = n * (z * p1 - p2) / (z * z - 1.0);
+      z1 = z;
+      z = z1 - p1 / pp;
+      unfinished = fabs(z - z1) > 3.e-14;
+    }
+    x[j] = xm - xl * z;
+    x[n - 1 - j] = xm + xl * z;
+    w[j] = 2.0 * xl / ((1.0 - z * z) * pp * pp);
+    w[n - 1 - j] = w[j];
+  }
 }
 
-extern "C" int rbc3d_set_wall(rbc3d_ctx *c, int, int, const double *, const int *) {
-  (void)c;
-  rbc_set_error("walls are not available in this build");
-  return RBC3D_EINVAL;
+static TriRules make_rules() {
+  TriRules t;
+  double r = (6. - sqrt(15.)) / 21., s = r, u = 1 - r - s;
+  double a1[3] = {r, s, u}, b1[3] = {s, u, r};
+  for (int k = 0; k < 3; k++) {
+    t.r7[k][0] = a1[k];
+    t.r7[k][1] = b1[k];
+    t.w7[k] = (155. - sqrt(15.)) / 2400.;
+  }
+  r = (6. + sqrt(15.)) / 21.;
+  s = r;
+  u = 1 - r - s;
+  double a2[3] = {r, s, u}, b2[3] = {s, u, r};
+  for (int k = 0; k < 3; k++) {
+    t.r7[3 + k][0] = a2[k];
+    t.r7[3 + k][1] = b2[k];
+    t.w7[3 + k] = (155. + sqrt(15.)) / 2400.;
+  }
+  t.r7[6][0] = 1. / 3.;
+  t.r7[6][1] = 1. / 3.;
+  t.w7[6] = 9. / 80.;
+  gauleg_h(0., 1., 4, t.rd, t.wd);
+  return t;
 }
-extern "C" int rbc3d_set_wall_density(rbc3d_ctx *c, const double *) {
-  (void)c;
-  return RBC3D_EINVAL;
+
+// MinDistToTri (common/ModIntOnWalls.F90:480-577); x[l][d]
+__device__ double min_dist_to_tri(const double xt[3], const double x[3][3], double &s0, double &t0) {
+  double x12[3], x13[3], x1T[3];
+  for (int d = 0; d < 3; d++) {
+    x12[d] = x[1][d] - x[0][d];
+    x13[d] = x[2][d] - x[0][d];
+    x1T[d] = x[0][d] - xt[d];
+  }
+  double a = x12[0] * x12[0] + x12[1] * x12[1] + x12[2] * x12[2];
+  double b = x12[0] * x13[0] + x12[1] * x13[1] + x12[2] * x13[2];
+  double c = x13[0] * x13[0] + x13[1] * x13[1] + x13[2] * x13[2];
+  double det = a * c - b * b, invDet = 1. / det;
+  double d = x12[0] * x1T[0] + x12[1] * x1T[1] + x12[2] * x1T[2];
+  double e = x13[0] * x1T[0] + x13[1] * x1T[1] + x13[2] * x1T[2];
+  double f = x1T[0] * x1T[0] + x1T[1] * x1T[1] + x1T[2] * x1T[2];
+  double s = b * e - c * d, t = b * d - a * e;
+  int reg;
+  if (s + t <= det) {
+    if (s < 0)
+      reg = t < 0 ? 4 : 3;
+    else if (t < 0)
+      reg = 5;
+    else
+      reg = 0;
+  } else {
+    if (s < 0)
+      reg = 2;
+    else if (t < 0)
+      reg = 6;
+    else
+      reg = 1;
+  }
+  if (reg == 2)
+    reg = (-(c + e) < 0) ? 3 : 1;
+  else if (reg == 4)
+    reg = (d < 0) ? 5 : 3;
+  else if (reg == 6)
+    reg = (b + e - a - d < 0) ? 1 : 5;
+  if (reg == 0) {
+    s = invDet * s;
+    t = invDet * t;
+  } else if (reg == 1) {
+    s = (c + e - b - d) / (a - 2 * b + c);
+    s = fmin(1., fmax(0., s));
+    t = 1. - s;
+  } else if (reg == 3) {
+    s = 0.;
+    t = -e / c;
+    t = fmin(1., fmax(0., t));
+  } else if (reg == 5) {
+    t = 0.;
+    s = -d / a;
+    s = fmin(1., fmax(0., s));
+  }
+  s0 = s;
+  t0 = t;
+  return sqrt(a * s * s + 2 * b * s * t + c * t * t + 2 * d * s + 2 * e * t + f);
 }
+
+struct EwT {
+  const double *sl1, *sl2;
+  double reps, rc;
+};
+
+// Tri_Int_Regular (:319-363) / Tri_Int_Duffy (:373-465).  With lhs != nullptr the
+// influence blocks lhs[l][a][b] are accumulated (and f is ignored).
+__device__ void tri_int(const TriRules &R, const EwT &ew, const double x[3][3], const double f[3][3],
+                        const double xt[3], bool duffy, double s0, double t0, double rhs[3],
+                        double (*lhs)[3][3]) {
+  rhs[0] = rhs[1] = rhs[2] = 0;
+  if (lhs)
+    for (int l = 0; l < 3; l++)
+      for (int a = 0; a < 3; a++)
+        for (int b = 0; b < 3; b++) lhs[l][a][b] = 0;
+  if (!duffy) {
+    double e1[3], e2[3], cr[3];
+    for (int d = 0; d < 3; d++) {
+      e1[d] = x[1][d] - x[0][d];
+      e2[d] = x[2][d] - x[0][d];
+    }
+    cr[0] = e1[1] * e2[2] - e1[2] * e2[1];
+    cr[1] = e1[2] * e2[0] - e1[0] * e2[2];
+    cr[2] = e1[0] * e2[1] - e1[1] * e2[0];
+    double detJ = 2 * (0.5 * sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]));
+    for (int q = 0; q < 7; q++) {
+      double s = R.r7[q][0], t = R.r7[q][1];
+      double xg[3], fg[3], xx[3], EA, EB;
+      for (int d = 0; d < 3; d++) {
+        xg[d] = (1. - s - t) * x[0][d] + s * x[1][d] + t * x[2][d];
+        fg[d] = (1. - s - t) * f[0][d] + s * f[1][d] + t * f[2][d];
+      }
+      double ds = R.w7[q] * detJ;
+      for (int d = 0; d < 3; d++) {
+        fg[d] = ds * fg[d];
+        xx[d] = xt[d] - xg[d];
+      }
+      double rr = sqrt(xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2]);
+      ewald_sl(ew.sl1, ew.sl2, rr, ew.reps, ew.rc, EA, EB);
+      double xf = xx[0] * fg[0] + xx[1] * fg[1] + xx[2] * fg[2];
+      for (int d = 0; d < 3; d++) rhs[d] += EA * xx[d] * xf + EB * fg[d];
+      if (lhs) {
+        double wv[3] = {1 - s - t, s, t};
+        for (int a = 0; a < 3; a++)
+          for (int b = 0; b < 3; b++) {
+            double L = ds * (EA * xx[a] * xx[b] + (a == b ? EB : 0.));
+            for (int l = 0; l < 3; l++) lhs[l][a][b] += wv[l] * L;
+          }
+      }
+    }
+    return;
+  }
+  double x0[3], f0[3];
+  for (int d = 0; d < 3; d++) {
+    x0[d] = (1 - s0 - t0) * x[0][d] + s0 * x[1][d] + t0 * x[2][d];
+    f0[d] = (1 - s0 - t0) * f[0][d] + s0 * f[1][d] + t0 * f[2][d];
+  }
+  const double st1[3][2] = {{0, 0}, {1, 0}, {0, 1}}, st2[3][2] = {{1, 0}, {0, 1}, {0, 0}};
+  for (int n = 0; n < 3; n++) {
+    int n2 = n == 2 ? 0 : n + 1;
+    double a[3], b[3], nr[3];
+    for (int d = 0; d < 3; d++) {
+      a[d] = x[n][d] - x0[d];
+      b[d] = x[n2][d] - x0[d];
+    }
+    nr[0] = a[1] * b[2] - a[2] * b[1];
+    nr[1] = a[2] * b[0] - a[0] * b[2];
+    nr[2] = a[0] * b[1] - a[1] * b[0];
+    double detJ = sqrt(nr[0] * nr[0] + nr[1] * nr[1] + nr[2] * nr[2]);
+    for (int i = 0; i < 4; i++)
+      for (int j = 0; j < 4; j++) {
+        double s = R.rd[i], t = s * R.rd[j];
+        double xg[3], fg[3], xx[3], EA, EB;
+        for (int d = 0; d < 3; d++) {
+          xg[d] = (1. - s) * x0[d] + (s - t) * x[n][d] + t * x[n2][d];
+          fg[d] = (1. - s) * f0[d] + (s - t) * f[n][d] + t * f[n2][d];
+        }
+        double ds = R.wd[i] * R.wd[j] * detJ * s;
+        for (int d = 0; d < 3; d++) {
+          fg[d] = ds * fg[d];
+          xx[d] = xt[d] - xg[d];
+        }
+        double rr = sqrt(xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2]);
+        ewald_sl(ew.sl1, ew.sl2, rr, ew.reps, ew.rc, EA, EB);
+        double xf = xx[0] * fg[0] + xx[1] * fg[1] + xx[2] * fg[2];
+        for (int d = 0; d < 3; d++) rhs[d] += EA * xx[d] * xf + EB * fg[d];
+        if (lhs) {
+          double sG = (1. - s) * s0 + (s - t) * st1[n][0] + t * st2[n][0];
+          double tG = (1. - s) * t0 + (s - t) * st1[n][1] + t * st2[n][1];
+          double wv[3] = {1. - sG - tG, sG, tG};
+          for (int aa = 0; aa < 3; aa++)
+            for (int bb = 0; bb < 3; bb++) {
+              double L = ds * (EA * xx[aa] * xx[bb] + (aa == bb ? EB : 0.));
+              for (int l = 0; l < 3; l++) lhs[l][aa][bb] += wv[l] * L;
+            }
+        }
+      }
+  }
+}
+
+struct WallArgs {
+  double Lb[3], iLb[3], iLbNc[3];
+  int Nc[3], nvert, nele, wall_id;
+  double rc;
+  TriRules R;
+  EwT ew;
+  const double *wx, *wf, *eps;  // vertices (nvert,3), traction (nvert,3), epsDist
+  const int *e2v;               // (nele,3) 1-based
+  const int *bin_start, *order; // centroid cell list (coarse bins)
+};
+
+__device__ __forceinline__ void load_tri(const WallArgs &a, int e, double x[3][3], double f[3][3]) {
+  for (int l = 0; l < 3; l++) {
+    int iv = a.e2v[e + (size_t)a.nele * l] - 1;
+    for (int d = 0; d < 3; d++) {
+      x[l][d] = a.wx[iv + (size_t)a.nvert * d];
+      if (f) f[l][d] = a.wf[iv + (size_t)a.nvert * d];
+    }
+  }
+}
+
+__device__ __forceinline__ void hash_bins(const WallArgs &a, const double xi[3], int &i1, int &i2, int &i3) {
+  i1 = imod_d((int)floor(xi[0] * a.iLbNc[0]), a.Nc[0]);
+  i2 = imod_d((int)floor(xi[1] * a.iLbNc[1]), a.Nc[1]);
+  i3 = imod_d((int)floor(xi[2] * a.iLbNc[2]), a.Nc[2]);
+}
+
+// translate the triangle next to xi (the shift of vertex 1, :103-108)
+__device__ __forceinline__ void shift_tri(const WallArgs &a, const double xi[3], double x[3][3]) {
+  for (int d = 0; d < 3; d++) {
+    double sh = round((xi[d] - x[0][d]) * a.iLb[d]) * a.Lb[d];
+    for (int l = 0; l < 3; l++) x[l][d] += sh;
+  }
+}
+
+// AddIntOnWalls for targets that are not wall vertices (common/ModIntOnWalls.F90:80-127)
+__global__ void __launch_bounds__(128) k_wall_targets(WallArgs a, double c1, int nT,
+                                                      const double *__restrict__ tx,
+                                                      const int *__restrict__ tindx,
+                                                      const int *__restrict__ tact,
+                                                      const double *__restrict__ tA, double *v) {
+  int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ti >= nT || !tact[ti]) return;
+  double xi[3] = {tx[ti], tx[ti + nT], tx[ti + 2 * (size_t)nT]};
+  if (tindx[ti] == a.wall_id) return;
+  int i1, i2, i3;
+  hash_bins(a, xi, i1, i2, i3);
+  double acc[3] = {0, 0, 0};
+  for (int nb = 0; nb < 27; nb++) {
+    int n1 = imod_d(i1 + nb / 9 - 1, a.Nc[0]), n2 = imod_d(i2 + (nb / 3) % 3 - 1, a.Nc[1]),
+        n3 = imod_d(i3 + nb % 3 - 1, a.Nc[2]);
+    int b = n1 + a.Nc[0] * (n2 + a.Nc[1] * n3);
+    for (int k = a.bin_start[b]; k < a.bin_start[b + 1]; k++) {
+      int e = a.order[k];
+      double x[3][3], f[3][3], dv[3], s0, t0;
+      load_tri(a, e, x, f);
+      shift_tri(a, xi, x);
+      double rr = min_dist_to_tri(xi, x, s0, t0);
+      if (rr > a.rc) continue;
+      tri_int(a.R, a.ew, x, f, xi, rr < a.eps[e], s0, t0, dv, nullptr);
+      for (int d = 0; d < 3; d++) acc[d] += c1 * dv[d] / tA[ti];
+    }
+  }
+  for (int d = 0; d < 3; d++) v[ti + (size_t)nT * d] += acc[d];
+}
+
+// Self-matrix assembly (PrepareSingIntOnWall :181-308), pass 1: triangles per vertex
+__global__ void __launch_bounds__(128) k_wall_count(WallArgs a, int *cnt) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.nvert) return;
+  double xi[3] = {a.wx[i], a.wx[i + a.nvert], a.wx[i + 2 * (size_t)a.nvert]};
+  int i1, i2, i3, n = 0;
+  hash_bins(a, xi, i1, i2, i3);
+  for (int nb = 0; nb < 27; nb++) {
+    int n1 = imod_d(i1 + nb / 9 - 1, a.Nc[0]), n2 = imod_d(i2 + (nb / 3) % 3 - 1, a.Nc[1]),
+        n3 = imod_d(i3 + nb % 3 - 1, a.Nc[2]);
+    int b = n1 + a.Nc[0] * (n2 + a.Nc[1] * n3);
+    for (int k = a.bin_start[b]; k < a.bin_start[b + 1]; k++) {
+      int e = a.order[k];
+      double x[3][3], s0, t0;
+      load_tri(a, e, x, nullptr);
+      shift_tri(a, xi, x);
+      if (!(min_dist_to_tri(xi, x, s0, t0) > a.rc)) n++;
+    }
+  }
+  cnt[i] = n;
+}
+
+// pass 2: one 3x3 block per (vertex row, triangle, triangle vertex)
+__global__ void __launch_bounds__(128) k_wall_fill(WallArgs a, const int *__restrict__ off,
+                                                   int *ekey, int *eidx, double *eval) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.nvert) return;
+  double xi[3] = {a.wx[i], a.wx[i + a.nvert], a.wx[i + 2 * (size_t)a.nvert]};
+  int i1, i2, i3;
+  hash_bins(a, xi, i1, i2, i3);
+  long long p = 3LL * off[i];
+  for (int nb = 0; nb < 27; nb++) {
+    int n1 = imod_d(i1 + nb / 9 - 1, a.Nc[0]), n2 = imod_d(i2 + (nb / 3) % 3 - 1, a.Nc[1]),
+        n3 = imod_d(i3 + nb % 3 - 1, a.Nc[2]);
+    int b = n1 + a.Nc[0] * (n2 + a.Nc[1] * n3);
+    for (int k = a.bin_start[b]; k < a.bin_start[b + 1]; k++) {
+      int e = a.order[k];
+      double x[3][3], f[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, s0, t0, rhs[3], lhs[3][3][3];
+      load_tri(a, e, x, nullptr);
+      shift_tri(a, xi, x);
+      double rr = min_dist_to_tri(xi, x, s0, t0);
+      if (rr > a.rc) continue;
+      tri_int(a.R, a.ew, x, f, xi, !(rr > a.eps[e]), s0, t0, rhs, lhs);
+      for (int l = 0; l < 3; l++) {
+        int iv = a.e2v[e + (size_t)a.nele * l] - 1;
+        ekey[p] = iv;
+        eidx[p] = (int)p;
+        for (int q = 0; q < 9; q++) eval[9 * p + q] = lhs[l][q / 3][q % 3];
+        p++;
+      }
+    }
+  }
+}
+
+// pass 3 (after a segmented sort of each row by column): sum equal columns
+__global__ void __launch_bounds__(128) k_wall_reduce(int nvert, const int *__restrict__ off,
+                                                     const int *__restrict__ skey,
+                                                     const int *__restrict__ sidx,
+                                                     const double *__restrict__ eval, int *rowcnt,
+                                                     int *bcol, double *bval, int pass) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nvert) return;
+  long long s = 3LL * off[i], e = 3LL * off[i + 1];
+  int n = 0;
+  long long out = pass ? rowcnt[i] : 0;
+  for (long long k = s; k < e; k++) {
+    bool first = (k == s) || skey[k] != skey[k - 1];
+    if (first) {
+      if (pass) {
+        bcol[out + n] = skey[k];
+        for (int q = 0; q < 9; q++) bval[9 * (out + n) + q] = 0.;
+      }
+      n++;
+    }
+    if (pass) {
+      const double *src = eval + 9 * (size_t)sidx[k];
+      for (int q = 0; q < 9; q++) bval[9 * (out + n - 1) + q] += src[q];
+    }
+  }
+  if (!pass) rowcnt[i] = n;
+}
+
+// v_i += c1 (L f)_i / Acoef_i over wall vertices (SingIntOnWall :136-172)
+__global__ void __launch_bounds__(256) k_wall_spmv(int nvert, double c1, const int *__restrict__ rowptr,
+                                                   const int *__restrict__ bcol,
+                                                   const double *__restrict__ bval,
+                                                   const double *__restrict__ f,
+                                                   const double *__restrict__ tA, double *v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nvert) return;
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
+    int j = bcol[k];
+    const double *B = bval + 9 * (size_t)k;
+    double f0 = f[j], f1 = f[j + nvert], f2 = f[j + 2 * (size_t)nvert];
+    s0 += B[0] * f0 + B[1] * f1 + B[2] * f2;
+    s1 += B[3] * f0 + B[4] * f1 + B[5] * f2;
+    s2 += B[6] * f0 + B[7] * f1 + B[8] * f2;
+  }
+  double A = tA[i];
+  v[i] += c1 * s0 / A;
+  v[i + nvert] += c1 * s1 / A;
+  v[i + 2 * (size_t)nvert] += c1 * s2 / A;
+}
+
+static WallArgs wall_args(rbc3d_ctx *c) {
+  WallArgs a;
+  WallData &w = c->wall;
+  for (int d = 0; d < 3; d++) {
+    a.Lb[d] = c->Lb[d];
+    a.iLb[d] = c->iLb[d];
+    a.iLbNc[d] = c->iLbNc[d];
+    a.Nc[d] = c->Nc[d];
+  }
+  a.nvert = w.nvert;
+  a.nele = w.nele;
+  a.wall_id = w.id;
+  a.rc = c->conf.rc;
+  a.R = make_rules();
+  a.ew.sl1 = c->d_sl1.p;
+  a.ew.sl2 = c->d_sl2.p;
+  a.ew.reps = c->r_eps;
+  a.ew.rc = c->conf.rc;
+  a.wx = w.d_x.p;
+  a.wf = w.d_f.p;
+  a.eps = w.d_eps.p;
+  a.e2v = w.d_e2v.p;
+  a.bin_start = w.d_bin_start.p;
+  a.order = w.d_order.p;
+  return a;
+}
+
+// ---------------------------------------------------------------------------
+extern "C" int rbc3d_set_wall(rbc3d_ctx *c, int nvert, int nele, const double *x, const int *e2v) {
+  if (!c || nvert < 3 || nele < 1 || !x || !e2v) return RBC3D_EINVAL;
+  for (int i = 0; i < 3 * nele; i++)
+    if (e2v[i] < 1 || e2v[i] > nvert) {
+      rbc_set_error("rbc3d_set_wall: e2v out of range");
+      return RBC3D_EINVAL;
+    }
+  for (int i = 0; i < 3 * nvert; i++)
+    if (!std::isfinite(x[i])) return RBC3D_ENONFINITE;
+  WallData &w = c->wall;
+  w.nvert = nvert;
+  w.nele = nele;
+  w.id = c->nrbc + 1;  // GlobData_Init (common/ModData.F90:40-49)
+  w.x.assign(x, x + 3 * (size_t)nvert);
+  w.e2v.assign(e2v, e2v + 3 * (size_t)nele);
+  // Wall_Build_V2V (common/ModWall.F90:66-114)
+  w.v2v.assign(nvert, 0);
+  {
+    double eps = std::min(c->Lb[0], std::min(c->Lb[1], c->Lb[2])) * 1.e-5;
+    double xmin[3], xmax[3];
+    for (int d = 0; d < 3; d++) {
+      xmin[d] = xmax[d] = x[(size_t)nvert * d];
+      for (int i = 0; i < nvert; i++) {
+        xmin[d] = std::min(xmin[d], x[i + (size_t)nvert * d]);
+        xmax[d] = std::max(xmax[d], x[i + (size_t)nvert * d]);
+      }
+    }
+    std::vector<int> bd;
+    for (int i = 0; i < nvert; i++) {
+      double m1 = 1e300, m2 = 1e300;
+      for (int d = 0; d < 3; d++) {
+        m1 = std::min(m1, fabs(x[i + (size_t)nvert * d] - xmin[d]));
+        m2 = std::min(m2, fabs(x[i + (size_t)nvert * d] - xmax[d]));
+      }
+      if (m1 < eps || m2 < eps) bd.push_back(i);
+    }
+    for (size_t p1 = 0; p1 + 1 < bd.size(); p1++) {
+      int v1 = bd[p1];
+      if (w.v2v[v1] > 0) continue;
+      for (size_t p2 = p1 + 1; p2 < bd.size(); p2++) {
+        int v2 = bd[p2];
+        double mx = 0;
+        for (int d = 0; d < 3; d++) {
+          double xx = x[v2 + (size_t)nvert * d] - x[v1 + (size_t)nvert * d];
+          xx = xx - round(xx * c->iLb[d]) * c->Lb[d];
+          mx = std::max(mx, fabs(xx));
+        }
+        if (mx < eps) w.v2v[v2] = v1 + 1;
+      }
+    }
+  }
+  // global vertex numbering (common/ModData.F90:50-63)
+  w.glb.assign(nvert, 0);
+  int p = 0;
+  for (int i = 0; i < nvert; i++) w.glb[i] = w.v2v[i] == 0 ? ++p : w.glb[w.v2v[i] - 1];
+  w.ndof = 3 * p;
+  // Wall_ComputeGeometry (common/ModWall.F90:118-145) and centroids
+  std::vector<double> area(nele), eps(nele), cen((size_t)3 * nele);
+  for (int e = 0; e < nele; e++) {
+    double xe[3][3];
+    for (int l = 0; l < 3; l++)
+      for (int d = 0; d < 3; d++) xe[l][d] = x[(e2v[e + (size_t)nele * l] - 1) + (size_t)nvert * d];
+    double a[3], b[3], cr[3];
+    for (int d = 0; d < 3; d++) {
+      a[d] = xe[1][d] - xe[0][d];
+      b[d] = xe[2][d] - xe[0][d];
+    }
+    cr[0] = a[1] * b[2] - a[2] * b[1];
+    cr[1] = a[2] * b[0] - a[0] * b[2];
+    cr[2] = a[0] * b[1] - a[1] * b[0];
+    double n = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
+    area[e] = 0.5 * n;
+    eps[e] = sqrt(area[e]);
+    for (int d = 0; d < 3; d++) cen[e + (size_t)nele * d] = (1.0 / 3) * ((xe[0][d] + xe[1][d]) + xe[2][d]);
+  }
+  // dedup map for AssembleArray (common/ModNoSlip.F90:361-384): the last vertex with a
+  // given global index supplies that unknown's value
+  std::vector<int> last(p, -1);
+  for (int i = 0; i < nvert; i++) last[w.glb[i] - 1] = i;
+  hipStream_t st = c->stream;
+  RBC_TRY(w.d_x.ensure((size_t)3 * nvert));
+  RBC_TRY(w.d_f.ensure((size_t)3 * nvert));
+  RBC_TRY(w.d_area.ensure(nele));
+  RBC_TRY(w.d_eps.ensure(nele));
+  RBC_TRY(w.d_cen.ensure((size_t)3 * nele));
+  RBC_TRY(w.d_e2v.ensure((size_t)3 * nele));
+  RBC_TRY(w.d_glb.ensure(nvert + p + 1));
+  HIP_TRY(hipMemcpy(w.d_x.p, x, sizeof(double) * 3 * nvert, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemset(w.d_f.p, 0, sizeof(double) * 3 * nvert));
+  HIP_TRY(hipMemcpy(w.d_area.p, area.data(), sizeof(double) * nele, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(w.d_eps.p, eps.data(), sizeof(double) * nele, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(w.d_cen.p, cen.data(), sizeof(double) * 3 * nele, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(w.d_e2v.p, e2v, sizeof(int) * 3 * nele, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(w.d_glb.p, w.glb.data(), sizeof(int) * nvert, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(w.d_glb.p + nvert, last.data(), sizeof(int) * p, hipMemcpyHostToDevice));
+  // wall source list on the reference's cell list
+  DBuf<int> bins;
+  RBC_TRY(bins.ensure(nele + 1));
+  RBC_TRY(w.d_order.ensure(nele + 1));
+  RBC_TRY(w.d_bin_start.ensure(c->nbins + 1));
+  RBC_TRY(rbc_hash_sort(c, nele, w.d_cen.p, nele, bins.p, w.d_order.p, w.d_bin_start.p));
+  bins.release();
+  w.prepared = false;
+  HIP_TRY(hipStreamSynchronize(st));
+  return RBC3D_OK;
+}
+
+extern "C" int rbc3d_set_wall_density(rbc3d_ctx *c, const double *f) {
+  if (!c || !f) return RBC3D_EINVAL;
+  if (!c->wall.nvert) return RBC3D_ESTATE;
+  HIP_TRY(hipMemcpy(c->wall.d_f.p, f, sizeof(double) * 3 * c->wall.nvert, hipMemcpyHostToDevice));
+  return RBC3D_OK;
+}
+
+int rbc_wall_prepare(rbc3d_ctx *c) {
+  WallData &w = c->wall;
+  if (!w.nvert) return RBC3D_ESTATE;
+  hipStream_t st = c->stream;
+  WallArgs a = wall_args(c);
+  int nv = w.nvert;
+  DBuf<int> cnt, off;
+  RBC_TRY(cnt.ensure(nv + 1));
+  RBC_TRY(off.ensure(nv + 1));
+  k_wall_count<<<(nv + 127) / 128, 128, 0, st>>>(a, cnt.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemsetAsync(cnt.p + nv, 0, sizeof(int), st));
+  size_t tmp = 0;
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, cnt.p, off.p, nv + 1, st));
+  RBC_TRY(c->d_cub_tmp.ensure(tmp + 16));
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->d_cub_tmp.p, tmp, cnt.p, off.p, nv + 1, st));
+  int ntri = 0;
+  HIP_TRY(hipMemcpyAsync(&ntri, off.p + nv, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  long long ne = 3LL * ntri;
+  if (ne > 0x7fffffffLL) {
+    rbc_set_error("wall self-matrix too large");
+    return RBC3D_EINVAL;
+  }
+  DBuf<int> ekey, eidx, skey, sidx, segoff, rowcnt;
+  DBuf<double> eval;
+  RBC_TRY(ekey.ensure(ne + 1));
+  RBC_TRY(eidx.ensure(ne + 1));
+  RBC_TRY(skey.ensure(ne + 1));
+  RBC_TRY(sidx.ensure(ne + 1));
+  RBC_TRY(eval.ensure(9 * ne + 9));
+  k_wall_fill<<<(nv + 127) / 128, 128, 0, st>>>(a, off.p, ekey.p, eidx.p, eval.p);
+  HIP_TRY(hipGetLastError());
+  // segment offsets in entries: 3*off
+  RBC_TRY(segoff.ensure(nv + 1));
+  {
+    std::vector<int> h(nv + 1);
+    HIP_TRY(hipMemcpy(h.data(), off.p, sizeof(int) * (nv + 1), hipMemcpyDeviceToHost));
+    for (auto &v : h) v *= 3;
+    HIP_TRY(hipMemcpy(segoff.p, h.data(), sizeof(int) * (nv + 1), hipMemcpyHostToDevice));
+  }
+  tmp = 0;
+  int bits = 1;
+  while ((1 << bits) < nv) bits++;
+  HIP_TRY(hipcub::DeviceSegmentedRadixSort::SortPairs(nullptr, tmp, ekey.p, skey.p, eidx.p, sidx.p,
+                                                      (int)ne, nv, segoff.p, segoff.p + 1, 0, bits, st));
+  RBC_TRY(c->d_cub_tmp.ensure(tmp + 16));
+  HIP_TRY(hipcub::DeviceSegmentedRadixSort::SortPairs(c->d_cub_tmp.p, tmp, ekey.p, skey.p, eidx.p,
+                                                      sidx.p, (int)ne, nv, segoff.p, segoff.p + 1,
+                                                      0, bits, st));
+  RBC_TRY(rowcnt.ensure(nv + 1));
+  k_wall_reduce<<<(nv + 127) / 128, 128, 0, st>>>(nv, off.p, skey.p, sidx.p, eval.p, rowcnt.p,
+                                                  nullptr, nullptr, 0);
+  HIP_TRY(hipMemsetAsync(rowcnt.p + nv, 0, sizeof(int), st));
+  RBC_TRY(w.d_rowptr.ensure(nv + 1));
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, rowcnt.p, w.d_rowptr.p, nv + 1, st));
+  RBC_TRY(c->d_cub_tmp.ensure(tmp + 16));
+  HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->d_cub_tmp.p, tmp, rowcnt.p, w.d_rowptr.p, nv + 1, st));
+  int nb = 0;
+  HIP_TRY(hipMemcpyAsync(&nb, w.d_rowptr.p + nv, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  RBC_TRY(w.d_col.ensure(nb + 1));
+  RBC_TRY(w.d_val.ensure(9 * (size_t)nb + 9));
+  k_wall_reduce<<<(nv + 127) / 128, 128, 0, st>>>(nv, off.p, skey.p, sidx.p, eval.p, w.d_rowptr.p,
+                                                  w.d_col.p, w.d_val.p, 1);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  w.nnz = 9LL * nb;
+  w.prepared = true;
+  return RBC3D_OK;
+}
+
 extern "C" int rbc3d_prepare_sing_int_on_wall(rbc3d_ctx *c) {
-  (void)c;
-  return RBC3D_EINVAL;
+  if (!c) return RBC3D_EINVAL;
+  return rbc_wall_prepare(c);
 }
+
 extern "C" int rbc3d_wall_self_nnz(rbc3d_ctx *c, long long *nnz) {
-  (void)c;
-  if (nnz) *nnz = 0;
-  return RBC3D_EINVAL;
+  if (!c || !nnz) return RBC3D_EINVAL;
+  *nnz = c->wall.nnz;
+  return RBC3D_OK;
 }
-extern "C" int rbc3d_add_int_on_walls(rbc3d_ctx *c, double, int, const double *, const int *,
-                                      const int *, const double *, double *) {
-  (void)c;
-  return RBC3D_EINVAL;
+
+int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double c1, int nT, const double *d_tx, const int *d_tindx,
+                             const int *d_act, const double *d_A, double *d_v) {
+  WallData &w = c->wall;
+  if (!w.nvert || nT == 0) return RBC3D_OK;
+  hipStream_t st = c->stream;
+  int first = 0;
+  HIP_TRY(hipMemcpyAsync(&first, d_tindx, sizeof(int), hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  rbc_stage_begin(c, ST_WALL);
+  if (first == w.id) {
+    // wall vertices: the precomputed self-interaction; with one wall that is all
+    if (!w.prepared) RBC_TRY(rbc_wall_prepare(c));
+    k_wall_spmv<<<(w.nvert + 255) / 256, 256, 0, st>>>(w.nvert, c1, w.d_rowptr.p, w.d_col.p, w.d_val.p,
+                                                       w.d_f.p, d_A, d_v);
+  } else {
+    k_wall_targets<<<(nT + 127) / 128, 128, 0, st>>>(wall_args(c), c1, nT, d_tx, d_tindx, d_act, d_A,
+                                                     d_v);
+  }
+  rbc_stage_end(c, ST_WALL);
+  HIP_TRY(hipGetLastError());
+  return RBC3D_OK;
 }
+
+extern "C" int rbc3d_add_int_on_walls(rbc3d_ctx *c, double c1, int nT, const double *tx,
+                                      const int *tindx, const int *active, const double *Acoef,
+                                      double *v) {
+  if (!c || nT < 0 || (nT > 0 && (!tx || !tindx || !active || !Acoef || !v))) return RBC3D_EINVAL;
+  if (nT == 0 || !c->wall.nvert) return RBC3D_OK;
+  hipStream_t st = c->stream;
+  RBC_TRY(c->d_tx.ensure((size_t)3 * nT + 1));
+  RBC_TRY(c->d_tA.ensure(nT + 1));
+  RBC_TRY(c->d_v.ensure((size_t)3 * nT + 1));
+  RBC_TRY(c->d_tindx.ensure((size_t)3 * nT + 1));
+  RBC_TRY(c->d_tact.ensure(nT + 1));
+  HIP_TRY(hipMemcpy(c->d_tx.p, tx, sizeof(double) * 3 * nT, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_tA.p, Acoef, sizeof(double) * nT, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_v.p, v, sizeof(double) * 3 * nT, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_tindx.p, tindx, sizeof(int) * 3 * nT, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_tact.p, active, sizeof(int) * nT, hipMemcpyHostToDevice));
+  RBC_TRY(rbc_add_int_on_walls_dev(c, c1, nT, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
+                                   c->d_v.p));
+  HIP_TRY(hipMemcpyAsync(v, c->d_v.p, sizeof(double) * 3 * nT, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  return RBC3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// no-slip operator on the de-duplicated traction vector
+__global__ void k_wall_tlist(int nvert, int id, const double *__restrict__ x, double *tx, int *tindx,
+                             int *act, double *A) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nvert) return;
+  for (int d = 0; d < 3; d++) tx[i + (size_t)nvert * d] = x[i + (size_t)nvert * d];
+  tindx[i] = id;
+  tindx[i + nvert] = i + 1;
+  tindx[i + 2 * (size_t)nvert] = -1;
+  act[i] = 1;
+  A[i] = 2.;
+}
+
+// AssembleArray(u, u1D, -1): u(i,:) = u1D(3 glb(i) - 2 : 3 glb(i))
+__global__ void k_assemble_in(int nvert, const int *__restrict__ glb, const double *__restrict__ u1d,
+                              double *u) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nvert) return;
+  int g = glb[i] - 1;
+  for (int d = 0; d < 3; d++) u[i + (size_t)nvert * d] = u1d[3 * g + d];
+}
+// AssembleArray(u, u1D, +1): the last vertex of each global index supplies it
+__global__ void k_assemble_out(int nuniq, int nvert, const int *__restrict__ last,
+                               const double *__restrict__ u, double *u1d) {
+  int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nuniq) return;
+  int i = last[g];
+  for (int d = 0; d < 3; d++) u1d[3 * g + d] = u[i + (size_t)nvert * d];
+}
+
+static int wall_targets_dev(rbc3d_ctx *c) {
+  WallData &w = c->wall;
+  int nv = w.nvert;
+  RBC_TRY(c->d_tx.ensure((size_t)3 * nv + 1));
+  RBC_TRY(c->d_tA.ensure(nv + 1));
+  RBC_TRY(c->d_v.ensure((size_t)3 * nv + 1));
+  RBC_TRY(c->d_tindx.ensure((size_t)3 * nv + 1));
+  RBC_TRY(c->d_tact.ensure(nv + 1));
+  k_wall_tlist<<<(nv + 255) / 256, 256, 0, c->stream>>>(nv, w.id, w.d_x.p, c->d_tx.p, c->d_tindx.p,
+                                                        c->d_tact.p, c->d_tA.p);
+  HIP_TRY(hipGetLastError());
+  return RBC3D_OK;
+}
+
+int rbc_wall_matmult_dev(rbc3d_ctx *c, const double *d_u, double *d_b) {
+  WallData &w = c->wall;
+  if (!w.nvert) return RBC3D_ESTATE;
+  if (!w.prepared) RBC_TRY(rbc_wall_prepare(c));
+  hipStream_t st = c->stream;
+  int nv = w.nvert, nu = w.ndof / 3;
+  RBC_TRY(wall_targets_dev(c));
+  k_assemble_in<<<(nv + 255) / 256, 256, 0, st>>>(nv, w.d_glb.p, d_u, w.d_f.p);
+  HIP_TRY(hipMemsetAsync(c->d_v.p, 0, sizeof(double) * 3 * nv, st));
+  double c1 = 1. / (4. * RBC_PI);
+  RBC_TRY(rbc_add_int_on_walls_dev(c, c1, nv, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
+                                   c->d_v.p));
+  RBC_TRY(rbc_pme_init(c));
+  RBC_TRY(rbc_pme_spread(c, c1, 0., 0, 1));
+  RBC_TRY(rbc_pme_transform(c));
+  RBC_TRY(rbc_pme_gather(c, nv, c->d_tx.p, c->d_tact.p, c->d_tA.p, c->d_v.p));
+  k_assemble_out<<<(nu + 255) / 256, 256, 0, st>>>(nu, nv, w.d_glb.p + nv, c->d_v.p, d_b);
+  HIP_TRY(hipGetLastError());
+  return RBC3D_OK;
+}
+
 extern "C" int rbc3d_wall_dof(rbc3d_ctx *c, int *dof) {
-  (void)c;
-  if (dof) *dof = 0;
-  return RBC3D_EINVAL;
+  if (!c || !dof) return RBC3D_EINVAL;
+  *dof = c->wall.ndof;
+  return RBC3D_OK;
 }
-extern "C" int rbc3d_wall_matmult(rbc3d_ctx *c, const double *, double *) {
-  (void)c;
-  return RBC3D_EINVAL;
+
+extern "C" int rbc3d_wall_matmult(rbc3d_ctx *c, const double *u, double *b) {
+  if (!c || !u || !b) return RBC3D_EINVAL;
+  WallData &w = c->wall;
+  if (!w.nvert) return RBC3D_ESTATE;
+  int nd = w.ndof;
+  RBC_TRY(c->d_shbuf2.ensure(2 * (size_t)nd + 2));
+  double *du = c->d_shbuf2.p, *db = du + nd + 1;
+  HIP_TRY(hipMemcpy(du, u, sizeof(double) * nd, hipMemcpyHostToDevice));
+  RBC_TRY(rbc_wall_matmult_dev(c, du, db));
+  HIP_TRY(hipMemcpyAsync(b, db, sizeof(double) * nd, hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RBC3D_OK;
 }
-extern "C" int rbc3d_wall_residual_vel(rbc3d_ctx *c, double *) {
-  (void)c;
-  return RBC3D_EINVAL;
+
+__global__ void k_add_const(double *v, int n, double s0, double s1, double s2) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  v[i] += s0;
+  v[i + n] += s1;
+  v[i + 2 * (size_t)n] += s2;
+}
+
+// Compute_Wall_Residual_Vel (common/ModNoSlip.F90:153-193) into d_v (nvert,3)
+int rbc_wall_residual_dev(rbc3d_ctx *c) {
+  WallData &w = c->wall;
+  if (!w.nvert) return RBC3D_ESTATE;
+  if (!w.prepared) RBC_TRY(rbc_wall_prepare(c));
+  hipStream_t st = c->stream;
+  int nv = w.nvert;
+  RBC_TRY(wall_targets_dev(c));
+  HIP_TRY(hipMemsetAsync(c->d_v.p, 0, sizeof(double) * 3 * nv, st));
+  double c1 = 1. / (4. * RBC_PI), c2 = 1. / (4. * RBC_PI);
+  if (c->nrbc)
+    RBC_TRY(rbc_add_int_on_rbcs_dev(c, c1, c2, nv, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
+                                    c->d_v.p));
+  RBC_TRY(rbc_add_int_on_walls_dev(c, c1, nv, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
+                                   c->d_v.p));
+  RBC_TRY(rbc_pme_init(c));
+  RBC_TRY(rbc_pme_spread(c, c1, c2, 1, 1));
+  RBC_TRY(rbc_pme_transform(c));
+  RBC_TRY(rbc_pme_gather(c, nv, c->d_tx.p, c->d_tact.p, c->d_tA.p, c->d_v.p));
+  k_add_const<<<(nv + 255) / 256, 256, 0, st>>>(c->d_v.p, nv, c->conf.vBkg[0], c->conf.vBkg[1],
+                                                c->conf.vBkg[2]);
+  HIP_TRY(hipGetLastError());
+  return RBC3D_OK;
+}
+
+extern "C" int rbc3d_wall_residual_vel(rbc3d_ctx *c, double *v) {
+  if (!c || !v) return RBC3D_EINVAL;
+  RBC_TRY(rbc_wall_residual_dev(c));
+  HIP_TRY(hipMemcpyAsync(v, c->d_v.p, sizeof(double) * 3 * c->wall.nvert, hipMemcpyDeviceToHost,
+                         c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RBC3D_OK;
 }
