@@ -80,6 +80,58 @@ def cpu_baseline(w, sample_every=32, threads=None):
     return total, t, threads
 
 
+def timestep_bench(steps, warmup):
+    """One timestep of the 200-cell tube at viscosity ratio 1 (ModTimeInt.F90:716-808 +
+    ModNoSlip.F90:45-150): Solve_RBC_Vel (right-hand side: cell single layer, walls,
+    PME; no cell GMRES at ratio 1; density g := v) then NoSlipWall (residual velocity,
+    wall-traction GMRES to 1e-8 relative residual, at most 60 iterations, x0 = 0, update,
+    monitoring residual).  The wall traction is reset to zero before every step, as at
+    the first step of a run, so every step does the same work."""
+    import ctypes as C
+    import torch
+    import rbc3d_amd
+    from rbc3d_amd import configs
+    w = configs.make_workload("C3", seed=1)
+    g = rbc3d_amd.from_workload(w)
+    nv = w.wall[2]
+    f0 = np.zeros(3 * nv)
+    g.prepare_sing_int_on_wall()
+    nnz = C.c_longlong()
+    rbc3d_amd._call("rbc3d_wall_self_nnz", g.h, C.byref(nnz))
+    stream = torch.cuda.ExternalStream(g.stream())
+    ni, rn = C.c_int(), C.c_double()
+    total_ms, its = 0.0, []
+    res = None
+    for k in range(warmup + steps):
+        g.set_wall_density(f0)
+        g.sync()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        e0.record(stream)
+        rbc3d_amd._call("rbc3d_solve_rbc_vel", g.h, None, C.byref(ni), C.byref(rn))
+        res = g.noslip_wall(1e-8, 60)
+        e1.record(stream)
+        g.sync()
+        wall = time.perf_counter() - t0
+        if k >= warmup:
+            total_ms += max(e0.elapsed_time(e1), 1e3 * wall)
+            its.append(res[0])
+    cells = w.cells
+    return {
+        "workload": f"C3: {cells.ncell} biconcave cells (radius 1.722, nlat 13x26, 25% "
+                    f"haematocrit) in an R=8, Lz=32 periodic tube, {nv} wall vertices "
+                    f"({w.wall[3]} Tri3), box {w.Lb}, PME {w.Nb}, viscosity ratio 1",
+        "s_per_timestep": round(total_ms / steps * 1e-3, 5),
+        "steps": steps,
+        "gmres_iterations": its,
+        "gmres_residual": res[1],
+        "gmres_converged": res[2],
+        "wall_velocity_after": res[3],
+        "wall_self_matrix_nnz": int(nnz.value),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +141,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-only", action="store_true")
     ap.add_argument("--sample-every", type=int, default=32)
+    ap.add_argument("--no-timestep", action="store_true",
+                    help="skip the tube timestep (GMRES seconds per timestep)")
+    ap.add_argument("--timestep-steps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
@@ -138,6 +193,7 @@ def main():
     sysg.timing(0)
     ms_per_step = max(gpu_ms, 1e3 * wall) / args.steps
     npairs, _ = sysg.pair_stats()
+    nproj, nnear = sysg.near_stats()
 
     # dominant kernel and its roofline: the real-space pair kernel is FP64-compute
     # bound; algorithmic work = 28 flop per in-cutoff (target, source) pair for the
@@ -173,7 +229,10 @@ def main():
         "roofline": roofline,
         "dominant_stage": {"name": dom[0], "ms_per_step": round(dom[1][0] / args.steps, 4)},
         "stages_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in stages.items() if v[1]},
+        "near_singular": {"projections": int(nproj), "corrections": int(nnear)},
     }
+    if not args.no_timestep:
+        out["timestep"] = timestep_bench(args.timestep_steps, 1)
     if not args.no_cpu:
         tot, parts, thr = cpu_baseline(w, args.sample_every)
         out["cpu_baseline"] = {"value": round(ncell / tot, 4), "unit": "cell-matvecs/s",

@@ -90,6 +90,10 @@ int rbc3d_add_int_on_walls(rbc3d_ctx *ctx, double c1, int nT, const double *tx, 
 /* Diagnostics of the last add_int_on_rbcs: in-cutoff pair count and a checksum of
  * the pair set (sum over pairs of (i+1)*(j+1) mod 2^64, 1-based target/source). */
 int rbc3d_pair_stats(rbc3d_ctx *ctx, long long *npairs, unsigned long long *checksum);
+/* Diagnostics of the last add_int_on_rbcs (no counterpart in the reference): the
+ * number of (target, neighbouring cell) projections of RBC_NearSingInt
+ * (common/ModRbcSingInt.F90:103-166) and how many passed its distance test. */
+int rbc3d_near_stats(rbc3d_ctx *ctx, long long *nitems, long long *nwork);
 
 /* --- hot path: PME (common/ModPME.F90) ------------------------------------ */
 /* PME_Distrib_Source(c1,c2,cells,walls) :58-133 (cells/walls = present flags) */

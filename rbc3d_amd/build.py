@@ -30,6 +30,33 @@ def _compile(src):
     return obj
 
 
+FLANG = os.environ.get("FC", "/opt/rocm/bin/amdflang")
+FDIR = os.path.join(HERE, "fortran")
+FBUILD = os.path.join(FDIR, "build")
+DRIVER_SRC = os.path.join(os.path.dirname(HERE), "tests", "fortran", "velocity_driver.f90")
+DRIVER = os.path.join(os.path.dirname(HERE), "tests", "fortran", "velocity_driver")
+
+
+def build_fortran(verbose=True):
+    """The ISO_C_BINDING module (rbc3d_bind) and the Fortran host driver the GPU
+    tests run; the driver links librbc3d_gpu.so through an $ORIGIN-relative rpath."""
+    os.makedirs(FBUILD, exist_ok=True)
+    mod_obj = os.path.join(FBUILD, "rbc3d_bind.o")
+    cmds = [
+        [FLANG, "-O2", "-fPIC", "-c", os.path.join(FDIR, "rbc3d_bind.f90"), "-J", FBUILD,
+         "-o", mod_obj],
+        [FLANG, "-O2", "-I", FBUILD, DRIVER_SRC, mod_obj, "-L", HERE, "-lrbc3d_gpu",
+         "-Wl,-rpath,$ORIGIN/../../rbc3d_amd", "-o", DRIVER],
+    ]
+    for cmd in cmds:
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"amdflang failed:\n{' '.join(cmd)}\n{r.stderr[-4000:]}")
+    if verbose:
+        print(f"built {DRIVER}")
+    return DRIVER
+
+
 def build(verbose=True):
     os.makedirs(OBJ, exist_ok=True)
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
@@ -42,6 +69,7 @@ def build(verbose=True):
     os.replace(tmp, OUT)
     if verbose:
         print(f"built {OUT}")
+    build_fortran(verbose)
     return OUT
 
 

@@ -28,6 +28,7 @@ import numpy as np
 
 EF_C = (0.2072, 2.0026, -1.1228)
 SURFACE_GAP = 0.1
+WALL_GAP = 0.25  # closest approach of a cell node to the tube wall
 
 
 def gauss_colat(nlat: int):
@@ -130,8 +131,8 @@ def shape_param(kind: str, nlat: int, radius: float = 1.0):
 def cell_volume(kind: str, nlat: int, radius: float = 1.0) -> float:
     th, w = gauss_colat(nlat)
     _, x, xt, xp = shape_param(kind, nlat, radius)
-    n = np.cross(xt, xp)
-    wt = w[:, None] * (2 * np.pi / (2 * nlat))
+    n = np.cross(xt, xp)  # |x_th x x_phi| dth dphi; the Gauss weights carry sin(th) dth
+    wt = w[:, None] * (2 * np.pi / (2 * nlat)) / np.sin(th)[:, None]
     return float((wt * (n * x).sum(-1)).sum() / 3.0)
 
 
@@ -162,11 +163,13 @@ def place_centres(rng, n, L, minsep, cyl=None, max_tries=2_000_000):
     return cs
 
 
-def place_cells_gap(rng, shapes, L, gap, max_tries=2_000_000):
+def place_cells_gap(rng, shapes, L, gap, max_tries=2_000_000, cyl=None):
     """Random sequential placement of rigidly rotated cells in a periodic box with a
     minimum surface gap: a candidate (uniform centre, uniform rotation) is rejected
     when any of its surface nodes comes closer than ``gap`` to a node of a placed
     cell.  ``shapes``: list of (body-frame nodes (n,3), bounding radius).
+    ``cyl = (xc, yc, Rmax)``: every node must also lie within distance Rmax of the
+    z-parallel axis through (xc, yc) (cells inside a tube).
     Placed nodes live in a periodic k-d tree, rebuilt every few insertions; the
     cells placed since the last rebuild are checked directly."""
     from scipy.spatial import cKDTree
@@ -181,8 +184,13 @@ def place_cells_gap(rng, shapes, L, gap, max_tries=2_000_000):
             if tries > max_tries:
                 raise RuntimeError(f"could not place {len(shapes)} cells (placed {len(cs)})")
             c = rng.random(3) * L
+            if cyl is not None and np.hypot(c[0] - cyl[0], c[1] - cyl[1]) > cyl[2] - 0.5 * rad:
+                continue
             Q = random_rotation(rng)
-            mine = np.mod(pts @ Q.T + c, L)
+            body = pts @ Q.T + c
+            if cyl is not None and np.hypot(body[:, 0] - cyl[0], body[:, 1] - cyl[1]).max() > cyl[2]:
+                continue
+            mine = np.mod(body, L)
             ok = True
             if tree is not None:
                 d, _ = tree.query(mine, k=1, distance_upper_bound=gap)
@@ -374,17 +382,22 @@ def make_workload(name: str, seed: int = 1, ncell: int = None, Nb: tuple = None)
         # a 2.2 minimum centre separation is infeasible at 20% volume fraction
         # (exclusion spheres would fill 71% of the box): use a surface gap instead
         body = shape_param("disc", 17)[1].reshape(-1, 3)
-        cs, rots = _placement(f"C2_n{n}_s{seed}", lambda: place_cells_gap(
+        cs, rots = _placement(f"C2_n{n}_L{L:.3f}_s{seed}", lambda: place_cells_gap(
             rng, [(body, 1.0)] * n, (L, L, L), SURFACE_GAP))
         cells = build_cells(rng, ["disc"] * n, [17] * n, [1.0] * n, cs, [1] * n, rots=rots)
         xi = xi_for_tolerance(2.0, 1e-8)
         return Workload("C2", (L, L, L), alpha_ref_from_xi(xi), 2.0, Nb or (128, 128, 128), 6,
                         (5.0,), (0.0, 0.0, 0.0), cells)
     if name == "C3":
+        # 200 discs fill the R=8, Lz=32 tube to 25% haematocrit when their radius is
+        # (0.25 * pi * 64 * 32 / (200 * V_disc(1)))^(1/3) = 1.722
         n = ncell or 200
         Lb = (16.5, 16.5, 32.0)
-        cs = place_centres(rng, n, Lb, 2.2, cyl=(8.25, 8.25, 8.0 - 1.3))
-        cells = build_cells(rng, ["disc"] * n, [13] * n, [1.0] * n, cs, [1] * n)
+        rad = (0.25 * math.pi * 64.0 * 32.0 / (200 * cell_volume("disc", 13))) ** (1.0 / 3.0)
+        body = shape_param("disc", 13, rad)[1].reshape(-1, 3)
+        cs, rots = _placement(f"C3_n{n}_R{rad:.3f}_s{seed}", lambda: place_cells_gap(
+            rng, [(body, rad)] * n, Lb, SURFACE_GAP, cyl=(8.25, 8.25, 8.0 - WALL_GAP)))
+        cells = build_cells(rng, ["disc"] * n, [13] * n, [rad] * n, cs, [1] * n, rots=rots)
         xi = xi_for_tolerance(2.0, 1e-8)
         wall = make_tube()
         return Workload("C3", Lb, alpha_ref_from_xi(xi), 2.0, Nb or (96, 96, 180), 6,

@@ -249,6 +249,61 @@ static int make_gtype(rbc3d_ctx *c, int nlat, GridType **out) {
         mk[((size_t)a * nl + b) * nn + d] = mask_tab(mt, acos(dd) / g->radius);
       }
   RBC_TRY(upload(g->d_mask, mk, st));
+  // Spline_FindProjection (common/ModSpline.F90:203-269): in each of its 3 iterations
+  // the 17-point quadratic fit of QuadFit_2D has a design fixed by the patch size h
+  // alone, so its least-squares solve is done here once: c = sum_k W_k d2_k with
+  // W = A^-1 [u_0 .. u_16], A = sum_k u_k u_k^T, u = (1, x, y, x^2, xy, y^2).  The
+  // local patch directions (sin th cos ph, sin th sin ph, cos th) are tabulated too.
+  {
+    std::vector<double> pj(RBC_PJ_SIZE, 0.);
+    double hp = std::max(RBC_TWO_PI / (2 * nl), RBC_TWO_PI / nn);
+    for (int it3 = 0; it3 < 3; it3++) {
+      double xy[17][2] = {{0., 0.}};
+      for (int ip = 0; ip < 8; ip++)
+        for (int it = 0; it < 2; it++) {
+          int i = (it + 1) + 2 * ip;
+          double thL = (it + 1) * hp / 2, phL = ip * RBC_TWO_PI / 8;
+          xy[i][0] = thL * cos(phL);
+          xy[i][1] = thL * sin(phL);
+          double *d = &pj[RBC_PJ_DIR + (size_t)(it3 * 16 + (i - 1)) * 3];
+          d[0] = sin(thL) * cos(phL);
+          d[1] = sin(thL) * sin(phL);
+          d[2] = cos(thL);
+        }
+      long double A[6][12] = {};
+      double U[17][6];
+      for (int k = 0; k < 17; k++) {
+        double x = xy[k][0], y = xy[k][1];
+        double u[6] = {1., x, y, x * x, x * y, y * y};
+        for (int a = 0; a < 6; a++) {
+          U[k][a] = u[a];
+          for (int b = 0; b < 6; b++) A[a][b] += (long double)u[a] * u[b];
+        }
+      }
+      for (int a = 0; a < 6; a++) A[a][6 + a] = 1;
+      for (int col = 0; col < 6; col++) {  // Gauss-Jordan, partial pivoting
+        int pv = col;
+        for (int r = col + 1; r < 6; r++)
+          if (fabsl(A[r][col]) > fabsl(A[pv][col])) pv = r;
+        for (int k = 0; k < 12; k++) std::swap(A[col][k], A[pv][k]);
+        long double d = A[col][col];
+        for (int k = 0; k < 12; k++) A[col][k] /= d;
+        for (int r = 0; r < 6; r++)
+          if (r != col) {
+            long double f = A[r][col];
+            for (int k = 0; k < 12; k++) A[r][k] -= f * A[col][k];
+          }
+      }
+      for (int k = 0; k < 17; k++)
+        for (int a = 0; a < 6; a++) {
+          long double w = 0;
+          for (int b = 0; b < 6; b++) w += A[a][6 + b] * U[k][b];
+          pj[RBC_PJ_W + (size_t)(it3 * 17 + k) * 6 + a] = (double)w;
+        }
+      hp = 0.5 * hp;
+    }
+    RBC_TRY(upload(g->d_proj, pj, st));
+  }
   *out = g;
   return RBC3D_OK;
 }
@@ -406,10 +461,14 @@ extern "C" int rbc3d_create(const rbc3d_conf *conf, rbc3d_ctx **out) {
         hi[d] = R[d];
       }
     }
+    // kept offsets grouped into x-rows: for each (b, q) the admissible a form one
+    // contiguous range (the box distance grows with |a|), and the source bins of a
+    // row are contiguous in the bin-sorted source order
     std::vector<int> offs;
-    for (int a = lo[0]; a <= hi[0]; a++)
-      for (int b = lo[1]; b <= hi[1]; b++)
-        for (int q = lo[2]; q <= hi[2]; q++) {
+    for (int b = lo[1]; b <= hi[1]; b++)
+      for (int q = lo[2]; q <= hi[2]; q++) {
+        int amin = 1 << 20, amax = -(1 << 20);
+        for (int a = lo[0]; a <= hi[0]; a++) {
           int o[3] = {a, b, q};
           double g2 = 0;
           for (int d = 0; d < 3; d++) {
@@ -418,8 +477,14 @@ extern "C" int rbc3d_create(const rbc3d_conf *conf, rbc3d_ctx **out) {
             g2 += (k * wmin[d]) * (k * wmin[d]);
           }
           if (g2 > rc * rc * (1 + 1e-9)) continue;
-          offs.push_back((a + 128) | ((b + 128) << 8) | ((q + 128) << 16));
+          amin = std::min(amin, a);
+          amax = std::max(amax, a);
         }
+        if (amin > amax) continue;
+        unsigned code = (unsigned)(amin + 128) | ((unsigned)(amax + 128) << 8) |
+                        ((unsigned)(b + 128) << 16) | ((unsigned)(q + 128) << 24);
+        offs.push_back((int)code);
+      }
     c->nfbins = c->Nf[0] * c->Nf[1] * c->Nf[2];
     c->noffs = offs.size();
     if (!r) r = upload(c->d_offs, offs, c->stream);
@@ -443,7 +508,8 @@ extern "C" int rbc3d_destroy(rbc3d_ctx *c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto *g : c->gtypes) {
     for (DBuf<double> *b : {&g->d_th, &g->d_phi, &g->d_w, &g->d_pw, &g->d_pg, &g->d_pe, &g->d_cs,
-                            &g->d_Dm, &g->d_Dn, &g->d_thG, &g->d_phiG, &g->d_pwt, &g->d_mask})
+                            &g->d_Dm, &g->d_Dn, &g->d_thG, &g->d_phiG, &g->d_pwt, &g->d_mask,
+                            &g->d_proj})
       b->release();
     delete g;
   }
@@ -471,6 +537,11 @@ extern "C" int rbc3d_destroy(rbc3d_ctx *c) {
   c->d_cell_sol.release();
   c->d_tmp_f.release();
   c->d_fbin_start.release();
+  c->d_worklist.release();
+  c->d_ikey.release();
+  c->d_ival.release();
+  c->d_islot.release();
+  c->d_linpart.release();
   rbc_stage_flush(c);
   for (auto e : c->timer.pool) (void)hipEventDestroy(e);
   c->d_pairstat.release();
@@ -661,6 +732,13 @@ extern "C" int rbc3d_pair_stats(rbc3d_ctx *c, long long *npairs, unsigned long l
   if (!c) return RBC3D_EINVAL;
   if (npairs) *npairs = c->last_npairs;
   if (checksum) *checksum = c->last_checksum;
+  return RBC3D_OK;
+}
+
+extern "C" int rbc3d_near_stats(rbc3d_ctx *c, long long *nitems, long long *nwork) {
+  if (!c) return RBC3D_EINVAL;
+  if (nitems) *nitems = c->last_nitems;
+  if (nwork) *nwork = c->last_nwork;
   return RBC3D_OK;
 }
 

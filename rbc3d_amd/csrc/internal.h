@@ -12,6 +12,11 @@
 
 #define RBC_NTAB 8192
 #define RBC_NBR_MAX 32
+// closest-point projection tables per grid type: least-squares weights W[3][17][6]
+// then local patch directions [3][16][3]
+#define RBC_PJ_W 0
+#define RBC_PJ_DIR 306
+#define RBC_PJ_SIZE 450
 #define RBC_PI 3.14159265358979323846
 #define RBC_TWO_PI (2 * RBC_PI)
 
@@ -37,6 +42,10 @@ template <typename T>
 struct DBuf {
   T *p = nullptr;
   size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf &) = delete;
+  DBuf &operator=(const DBuf &) = delete;
+  ~DBuf() { release(); }
   int ensure(size_t count) {
     if (count <= n && p) return RBC3D_OK;
     if (p) (void)hipFree(p);
@@ -69,6 +78,7 @@ struct GridType {
   DBuf<double> d_Dn;     // periodic derivative matrix, length nlon, [p][q]
   DBuf<double> d_thG, d_phiG, d_pwt;  // polar patch: [node][irad + nrad*iazm], weights
   DBuf<double> d_mask;   // same-surface mask [ilat_i][ilat_j][dlon]
+  DBuf<double> d_proj;   // projection tables (RBC_PJ_*)
 };
 
 // Stage timing (HIP events on the context's stream, enabled on request)
@@ -186,7 +196,9 @@ struct rbc3d_ctx {
   DBuf<double> d_cell_sol;
   bool cell_sol_init = false;
   // last pair statistics
-  long long last_npairs = 0;
+  long long last_npairs = 0, last_nitems = 0, last_nwork = 0;
+  DBuf<int> d_worklist, d_ikey, d_ival, d_islot;
+  DBuf<double> d_linpart;
   unsigned long long last_checksum = 0;
 };
 
@@ -263,6 +275,10 @@ __device__ __forceinline__ double mask_fn(const double *__restrict__ mt, double 
 // blk points at variable v0; nvar consecutive variables are interpolated.
 __device__ __forceinline__ void spline_interp(const double *__restrict__ blk, int m, int n,
                                               double x, double y, int nvar, double *f) {
+  // evaluated without FMA contraction, operation for operation as the reference's
+  // compiled Fortran: the double-layer near-singular correction amplifies ulp-level
+  // differences of these values about a thousandfold
+#pragma clang fp contract(off)
   double hx = RBC_TWO_PI / m, hy = RBC_TWO_PI / n;
   double ihx = 1. / hx, ihy = 1. / hy;
   double fi1 = floor(x * ihx), fj1 = floor(y * ihy);
@@ -276,6 +292,7 @@ __device__ __forceinline__ void spline_interp(const double *__restrict__ blk, in
   size_t stride = (size_t)m * n * 4;
   size_t p11 = 4 * (size_t)(i1 + m * j1), p12 = 4 * (size_t)(i1 + m * j2);
   size_t p21 = 4 * (size_t)(i2 + m * j1), p22 = 4 * (size_t)(i2 + m * j2);
+#pragma nounroll
   for (int l = 0; l < nvar; l++) {
     const double *b = blk + l * stride;
     double4 a11 = *(const double4 *)(b + p11), a12 = *(const double4 *)(b + p12);

@@ -320,15 +320,16 @@ int rbc_update_density(rbc3d_ctx *c, int which) {
 // target bin whose box distance does not exceed rc.  A workgroup is one wavefront
 // owning up to 64 targets of one fine bin; sources stream through LDS in tiles.
 
-#define PT 64    // targets per workgroup
+#define PT 64    // targets per workgroup, one per lane
+#define PW 4     // wavefronts per workgroup, each taking every PW-th source of a tile
 #define PS 256   // sources per LDS tile
 
 struct PairArgs {
   double Lb[3], iLb[3];
   int Nf[3];
   double rc, rc2hi, reps, c1, c2;
-  int np, nT, nfbins, nrbc, noffs;
-  const int *offs;       // packed source-bin offsets
+  int np, nT, nfbins, nrbc, nrows;
+  const int *rows;       // x-rows of source bins: packed (alo, ahi, o2, o3)
   const double *rec;     // sources sorted by fine bin (SREC x np)
   const int *src_node;   // sorted position -> node index
   const int *bin_start;  // sources per fine bin
@@ -348,10 +349,18 @@ __global__ void k_chunks(int nbins, const int *__restrict__ tbin_start, int *chu
   chunk_cnt[b] = b < nbins ? (tbin_start[b + 1] - tbin_start[b] + PT - 1) / PT : 0;
 }
 
-__global__ void __launch_bounds__(PT) k_pair(PairArgs a) {
+// SL: single-layer term present (c1 != 0); DL: double-layer term present (c2 != 0).
+// The workgroup's four wavefronts hold the same 64 targets and split every source
+// tile between them; their partial sums are added in wavefront order at the end.
+template <bool SL, bool DL>
+__global__ void __launch_bounds__(PT *PW) k_pair(PairArgs a) {
+  constexpr int NP = (SL ? 3 : 0) + (DL ? 6 : 0);
+  constexpr int PD = SL ? 3 : 0;  // first DL payload slot
   __shared__ double sx[PS], sy[PS], sz[PS];
-  __shared__ double sf[3][PS], sg[3][PS], sn[3][PS], sB[PS];
-  __shared__ int snode[PS];
+  __shared__ double sp[NP > 0 ? NP : 1][PS];
+  __shared__ int snode[PS], scell[PS], sll[PS];
+  __shared__ double red[PW - 1][3][PT];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int nb = a.nfbins, blk = blockIdx.x;
   if (blk >= a.chunk_off[nb]) return;
   int lo = 0, hi = nb;  // chunk_off[lo] <= blk < chunk_off[hi]
@@ -364,21 +373,24 @@ __global__ void __launch_bounds__(PT) k_pair(PairArgs a) {
   }
   const int bin = lo, chunk = blk - a.chunk_off[lo];
   const int t0 = a.tbin_start[bin] + chunk * PT, t1 = a.tbin_start[bin + 1];
-  const int k = t0 + threadIdx.x;
+  const int k = t0 + lane;
   const bool have = k < t1;
   const int ti = have ? a.torder[k] : 0;
   const bool act = have && a.tact[ti] != 0;
-  double xi0 = 0, xi1 = 0, xi2 = 0, Ai = 1.;
-  int surf_i = -1000, ilat_i = 0, ilon_i = 0, nl_i = 1, nn_i = 1;
+  double xi0 = 0, xi1 = 0, xi2 = 0, cA1 = 0, cA2 = 0;
+  int ilat_i = 0, ilon_i = 0, nl_i = 1, nn_i = 1, cell_i = -1;
   const double *mtab = nullptr;
   if (have) {
     xi0 = a.tx[ti];
     xi1 = a.tx[ti + (size_t)a.nT];
     xi2 = a.tx[ti + 2 * (size_t)a.nT];
-    Ai = a.tA[ti];
-    surf_i = a.tindx[ti];
+    double Ai = a.tA[ti];
+    cA1 = a.c1 / Ai;
+    cA2 = a.c2 / Ai;
+    int surf_i = a.tindx[ti];
     if (surf_i >= 1 && surf_i <= a.nrbc) {
       const CellRec &cr = a.cells[surf_i - 1];
+      cell_i = surf_i - 1;
       nl_i = cr.nlat;
       nn_i = cr.nlon;
       mtab = a.mask[cr.gtype];
@@ -386,89 +398,123 @@ __global__ void __launch_bounds__(PT) k_pair(PairArgs a) {
       ilon_i = a.tindx[ti + 2 * (size_t)a.nT] - 1;
     }
   }
-  const int cell_i = (surf_i >= 1 && surf_i <= a.nrbc) ? surf_i - 1 : -1;
   const double L0 = a.Lb[0], L1 = a.Lb[1], L2 = a.Lb[2];
   const double iL0 = a.iLb[0], iL1 = a.iLb[1], iL2 = a.iLb[2];
+  const int Nf0 = a.Nf[0], Nf1 = a.Nf[1];
   double acc0 = 0, acc1 = 0, acc2 = 0;
   unsigned long long npairs = 0, csum = 0;
-  const int b1 = bin % a.Nf[0], b2 = (bin / a.Nf[0]) % a.Nf[1], b3 = bin / (a.Nf[0] * a.Nf[1]);
+  const int b1 = bin % Nf0, b2 = (bin / Nf0) % Nf1, b3 = bin / (Nf0 * Nf1);
   const size_t np = a.np;
-  for (int oi = 0; oi < a.noffs; oi++) {
-    int code = a.offs[oi];
-    int o1 = (code & 255) - 128, o2 = ((code >> 8) & 255) - 128, o3 = ((code >> 16) & 255) - 128;
-    int n1 = imod_d(b1 + o1, a.Nf[0]), n2 = imod_d(b2 + o2, a.Nf[1]), n3 = imod_d(b3 + o3, a.Nf[2]);
-    int nbn = n1 + a.Nf[0] * (n2 + a.Nf[1] * n3);
-    int s0 = a.bin_start[nbn], s1 = a.bin_start[nbn + 1];
-    for (int base = s0; base < s1; base += PS) {
-      int cnt = min(PS, s1 - base);
-      __syncthreads();
-      for (int t = threadIdx.x; t < cnt; t += PT) {
-        int q = base + t;
-        sx[t] = a.rec[q];
-        sy[t] = a.rec[q + np];
-        sz[t] = a.rec[q + 2 * np];
-        for (int d = 0; d < 3; d++) {
-          sf[d][t] = a.rec[q + np * (3 + d)];
-          sg[d][t] = a.rec[q + np * (6 + d)];
-          sn[d][t] = a.rec[q + np * (9 + d)];
-        }
-        sB[t] = a.rec[q + np * 12];
-        snode[t] = a.src_node[q];
+  for (int ri = 0; ri < a.nrows; ri++) {
+    const unsigned code = (unsigned)a.rows[ri];
+    const int alo = (int)(code & 255) - 128, ahi = (int)((code >> 8) & 255) - 128;
+    const int o2 = (int)((code >> 16) & 255) - 128, o3 = (int)(code >> 24) - 128;
+    const int rowbase = Nf0 * (imod_d(b2 + o2, Nf1) + Nf1 * imod_d(b3 + o3, a.Nf[2]));
+    // the row's x-bins as one or two runs (a range shorter than the row wraps at most once)
+    int rlo[2], rhi[2], nrun = 1;
+    if (ahi - alo + 1 >= Nf0) {
+      rlo[0] = 0;
+      rhi[0] = Nf0 - 1;
+    } else {
+      const int x0 = b1 + alo, x1 = b1 + ahi;
+      if (x0 < 0) {
+        rlo[0] = x0 + Nf0, rhi[0] = Nf0 - 1, rlo[1] = 0, rhi[1] = x1, nrun = 2;
+      } else if (x1 >= Nf0) {
+        rlo[0] = x0, rhi[0] = Nf0 - 1, rlo[1] = 0, rhi[1] = x1 - Nf0, nrun = 2;
+      } else {
+        rlo[0] = x0, rhi[0] = x1;
       }
-      __syncthreads();
-      if (!act) continue;
-      for (int j = 0; j < cnt; j++) {
-        // reference distance without FMA contraction (common/ModIntOnRbcs.F90:68-74);
-        // rint differs from NINT only at |x/Lb| = 1/2, where |x| = Lb/2 > rc
-        double d0, d1, d2, s;
-        {
+    }
+    for (int ru = 0; ru < nrun; ru++) {
+      const int s0 = a.bin_start[rowbase + rlo[ru]], s1 = a.bin_start[rowbase + rhi[ru] + 1];
+      for (int base = s0; base < s1; base += PS) {
+        const int cnt = min(PS, s1 - base);
+        __syncthreads();
+        for (int t = threadIdx.x; t < cnt; t += PT * PW) {
+          const int q = base + t;
+          sx[t] = a.rec[q];
+          sy[t] = a.rec[q + np];
+          sz[t] = a.rec[q + 2 * np];
+          if (SL)
+            for (int d = 0; d < 3; d++) sp[d][t] = a.rec[q + np * (3 + d)];
+          if (DL) {
+            const double Bj = a.rec[q + np * 12];
+            for (int d = 0; d < 3; d++) {
+              sp[PD + d][t] = a.rec[q + np * (6 + d)] * Bj;
+              sp[PD + 3 + d][t] = a.rec[q + np * (9 + d)];
+            }
+          }
+          const int node = a.src_node[q];
+          snode[t] = node;
+          scell[t] = a.node_cell[node];
+          sll[t] = a.node_ll[node];
+        }
+        __syncthreads();
+        if (!act) continue;
+        for (int j = wv; j < cnt; j += PW) {
+          // reference distance without FMA contraction (common/ModIntOnRbcs.F90:68-74);
+          // rint differs from NINT only at |x/Lb| = 1/2, where |x| = Lb/2 > rc
+          double d0, d1, d2, s2;
+          {
 #pragma clang fp contract(off)
-          d0 = sx[j] - xi0;
-          d1 = sy[j] - xi1;
-          d2 = sz[j] - xi2;
-          d0 = d0 - rint(d0 * iL0) * L0;
-          d1 = d1 - rint(d1 * iL1) * L1;
-          d2 = d2 - rint(d2 * iL2) * L2;
-          s = d0 * d0 + d1 * d1;
-          s = s + d2 * d2;
-        }
-        if (s > a.rc2hi) continue;
-        double rr = sqrt(s);
-        if (rr > a.rc) continue;
-        int node = snode[j];
-        double mask = 0.;
-        if (cell_i >= 0 && a.node_cell[node] == cell_i) {
-          int ll = a.node_ll[node];
-          int jl = ll & 0xffff, jn = ll >> 16;
-          int dl = jn - ilon_i;
-          dl += dl < 0 ? nn_i : 0;
-          mask = mtab[((size_t)ilat_i * nl_i + jl) * nn_i + dl];
-        }
-        npairs++;
-        csum += (unsigned long long)(ti + 1) * (unsigned long long)(node + 1);
-        if (a.c1 != 0) {
-          double EA, EB;
-          ewald_sl(a.sl1, a.sl2, rr, a.reps, a.rc, EA, EB);
-          double f0 = sf[0][j], f1 = sf[1][j], f2 = sf[2][j];
-          double xf = d0 * f0 + d1 * f1 + d2 * f2;
-          double cc = (1. - mask) * a.c1 / Ai;
-          acc0 += cc * (EA * d0 * xf + EB * f0);
-          acc1 += cc * (EA * d1 * xf + EB * f1);
-          acc2 += cc * (EA * d2 * xf + EB * f2);
-        }
-        if (a.c2 != 0) {
-          double EA = ewald_dl(a.dl1, rr, a.reps, a.rc);
-          double xg = d0 * sg[0][j] + d1 * sg[1][j] + d2 * sg[2][j];
-          double xn = d0 * sn[0][j] + d1 * sn[1][j] + d2 * sn[2][j];
-          double cc = (1. - mask) * a.c2 * sB[j] / Ai;
-          acc0 += cc * (EA * d0 * xg * xn);
-          acc1 += cc * (EA * d1 * xg * xn);
-          acc2 += cc * (EA * d2 * xg * xn);
+            d0 = sx[j] - xi0;
+            d1 = sy[j] - xi1;
+            d2 = sz[j] - xi2;
+            d0 = d0 - rint(d0 * iL0) * L0;
+            d1 = d1 - rint(d1 * iL1) * L1;
+            d2 = d2 - rint(d2 * iL2) * L2;
+            s2 = d0 * d0 + d1 * d1;
+            s2 = s2 + d2 * d2;
+          }
+          if (s2 > a.rc2hi) continue;
+          const double rr = sqrt(s2);
+          if (rr > a.rc) continue;
+          const int node = snode[j];
+          double wm = 1.;
+          if (scell[j] == cell_i) {
+            const int ll = sll[j];
+            const int jl = ll & 0xffff, jn = ll >> 16;
+            int dl = jn - ilon_i;
+            dl += dl < 0 ? nn_i : 0;
+            wm = 1. - mtab[((size_t)ilat_i * nl_i + jl) * nn_i + dl];
+          }
+          npairs++;
+          csum += (unsigned long long)(ti + 1) * (unsigned long long)(node + 1);
+          if (SL) {
+            double EA, EB;
+            ewald_sl(a.sl1, a.sl2, rr, a.reps, a.rc, EA, EB);
+            const double f0 = sp[0][j], f1 = sp[1][j], f2 = sp[2][j];
+            const double xf = d0 * f0 + d1 * f1 + d2 * f2;
+            const double cc = wm * cA1;
+            acc0 += cc * (EA * d0 * xf + EB * f0);
+            acc1 += cc * (EA * d1 * xf + EB * f1);
+            acc2 += cc * (EA * d2 * xf + EB * f2);
+          }
+          if (DL) {
+            const double EA = ewald_dl(a.dl1, rr, a.reps, a.rc);
+            const double xg = d0 * sp[PD][j] + d1 * sp[PD + 1][j] + d2 * sp[PD + 2][j];
+            const double xn = d0 * sp[PD + 3][j] + d1 * sp[PD + 4][j] + d2 * sp[PD + 5][j];
+            const double cc = wm * cA2 * EA * xg * xn;
+            acc0 += cc * d0;
+            acc1 += cc * d1;
+            acc2 += cc * d2;
+          }
         }
       }
     }
   }
-  if (act) {
+  if (wv > 0) {
+    red[wv - 1][0][lane] = acc0;
+    red[wv - 1][1][lane] = acc1;
+    red[wv - 1][2][lane] = acc2;
+  }
+  __syncthreads();
+  if (wv == 0 && act) {
+    for (int w = 0; w < PW - 1; w++) {
+      acc0 += red[w][0][lane];
+      acc1 += red[w][1][lane];
+      acc2 += red[w][2][lane];
+    }
     a.vpair[ti] = acc0;
     a.vpair[ti + (size_t)a.nT] = acc1;
     a.vpair[ti + 2 * (size_t)a.nT] = acc2;
@@ -478,7 +524,7 @@ __global__ void __launch_bounds__(PT) k_pair(PairArgs a) {
     npairs += __shfl_xor(npairs, o, 64);
     csum += __shfl_xor(csum, o, 64);
   }
-  if (threadIdx.x == 0) {
+  if (lane == 0) {
     atomicAdd(&a.stat[0], npairs);
     atomicAdd(&a.stat[1], csum);
   }
@@ -696,6 +742,8 @@ struct SingArgs {
   double *dvself;
   // near-singular
   const int *cnt, *slot_cell, *slot_node, *item_off;
+  const double *const *proj;               // projection tables per grid type
+  const int *item_order, *item_slot;       // items sorted by neighbouring cell; item -> slot
   double *item_rec;  // 12 doubles per item
   int *work, *work_cnt;
   double *dvitem;
@@ -767,29 +815,20 @@ __global__ void __launch_bounds__(256) k_self(SingArgs a) {
 
 // Near-singular stage 1: translate the target next to the closest node, project it
 // onto the neighbour's spline surface (Spline_FindProjection) and apply the distance
-// test of RBC_NearSingInt.  Sixteen lanes per item: each lane evaluates one of the 16
-// patch points of an iteration; the normal equations of the quadratic fit are
-// reduced across the group and solved redundantly by every lane.
+// test of RBC_NearSingInt.  One lane per (target, neighbouring cell) item; items are
+// processed in neighbouring-cell order, and consecutive blocks of that order are
+// placed on one XCD, so a cell's spline block is fetched into few L2s.  The quadratic
+// fit uses the per-grid-type least-squares weights (c = sum_k W_k d2_k).
 // item record: xi(3) x0(3) th0 phi0 a30(3) dist
-__device__ __forceinline__ double grp_sum16(double v) {
-  for (int o = 8; o > 0; o >>= 1) v += __shfl_xor(v, o, 16);
-  return v;
-}
-
-__global__ void __launch_bounds__(256) k_proj(SingArgs a, int nitems) {
-  const int gid = (blockIdx.x * blockDim.x + threadIdx.x) >> 4, sub = threadIdx.x & 15;
-  if (gid >= nitems) return;
-  // item -> (target, slot): item_off is an exclusive scan over targets
-  int lo = 0, hi = a.nT;
-  while (hi - lo > 1) {
-    int mid = (lo + hi) >> 1;
-    if (a.item_off[mid] <= gid)
-      lo = mid;
-    else
-      hi = mid;
-  }
-  const int ti = lo, p = gid - a.item_off[lo];
-  const size_t s = (size_t)ti * RBC_NBR_MAX + p;
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_proj(SingArgs a, int nitems) {
+  const int nblk = (nitems + 255) / 256, per = (nblk + 7) / 8;
+  const int lblk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
+  if (lblk >= nblk) return;
+  const int gidx = lblk * 256 + threadIdx.x;
+  if (gidx >= nitems) return;
+  const int itm = a.item_order[gidx];
+  const int s = a.item_slot[itm];
+  const int ti = s / RBC_NBR_MAX;
   const int cj = a.slot_cell[s], node = a.slot_node[s];
   const CellRec cr = a.cells[cj];
   const int gt = cr.gtype, m = 2 * cr.nlat, n = cr.nlon;
@@ -807,59 +846,47 @@ __global__ void __launch_bounds__(256) k_proj(SingArgs a, int nitems) {
   const double *blk = a.spl + cr.spl;
   const size_t vs = (size_t)m * n * 4;
   const double *sxp = blk + vs * SPL_X;
+  const double *pj = a.proj[gt];
   // Spline_FindProjection (common/ModSpline.F90:203-269)
   spline_interp(sxp, m, n, th0, phi0, 3, x0);
-  double h = fmax(RBC_TWO_PI / m, RBC_TWO_PI / n);
-  const int nth = 2, nphi = 8;
-  for (int iter = 1; iter <= 3; iter++) {
-    double s0 = sin(th0), c0 = cos(th0), xx[3];
-    spline_interp(sxp, m, n, th0, phi0, 3, xx);
-    for (int d = 0; d < 3; d++) xx[d] -= xt[d];
-    double d20 = xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2];
-    // this lane's patch point i = sub + 1
-    int it = sub % nth, ip = sub / nth;
-    double thL = (it + 1) * h / nth, phL = ip * RBC_TWO_PI / nphi;
-    double xi_ = thL * cos(phL), yi_ = thL * sin(phL), thP, phP;
-    patch_point(s0, c0, phi0, thL, phL, thP, phP);
-    spline_interp(sxp, m, n, thP, phP, 3, xx);
-    for (int d = 0; d < 3; d++) xx[d] -= xt[d];
-    double fi_ = xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2];
-    double u[6] = {1., xi_, yi_, xi_ * xi_, xi_ * yi_, yi_ * yi_};
-    double U[6][6], rhs[6];
-    for (int aa = 0; aa < 6; aa++) {
-      for (int bb = aa; bb < 6; bb++) U[aa][bb] = grp_sum16(u[aa] * u[bb]);
-      rhs[aa] = grp_sum16(u[aa] * fi_);
-    }
-    // the patch centre (0,0) with f = d20
-    U[0][0] += 1.;
-    rhs[0] += d20;
-    // dposv('U'): A = U^T U, then two triangular solves
-    for (int j = 0; j < 6; j++) {
-      double sj = U[j][j];
-      for (int kk = 0; kk < j; kk++) sj -= U[kk][j] * U[kk][j];
-      double ujj = sqrt(sj);
-      U[j][j] = ujj;
-      for (int i = j + 1; i < 6; i++) {
-        double t = U[j][i];
-        for (int kk = 0; kk < j; kk++) t -= U[kk][j] * U[kk][i];
-        U[j][i] = t / ujj;
+#pragma nounroll
+  for (int iter = 0; iter < 3; iter++) {
+    const double s0 = sin(th0), c0 = cos(th0);
+    double xx[3];
+    for (int d = 0; d < 3; d++) xx[d] = x0[d] - xt[d];
+    const double d20 = xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2];
+    const double *W = pj + RBC_PJ_W + iter * 17 * 6;
+    double c1 = W[1] * d20, c2 = W[2] * d20, c3 = W[3] * d20, c4 = W[4] * d20, c5 = W[5] * d20;
+#pragma nounroll
+    for (int k = 0; k < 16; k++) {
+      // PolarPatch_Build (common/ModPolarPatch.F90:99-148) for patch point k
+      const double *dr = pj + RBC_PJ_DIR + (iter * 16 + k) * 3;
+      double y0, y1, y2;
+      {
+#pragma clang fp contract(off)
+        y0 = c0 * dr[0] + 0. * dr[1] + s0 * dr[2];
+        y1 = 0. * dr[0] + 1. * dr[1] + 0. * dr[2];
+        y2 = -s0 * dr[0] + 0. * dr[1] + c0 * dr[2];
       }
-    }
-    for (int i = 0; i < 6; i++) {
-      double t = rhs[i];
-      for (int kk = 0; kk < i; kk++) t -= U[kk][i] * rhs[kk];
-      rhs[i] = t / U[i][i];
-    }
-    for (int i = 5; i >= 0; i--) {
-      double t = rhs[i];
-      for (int kk = i + 1; kk < 6; kk++) t -= U[i][kk] * rhs[kk];
-      rhs[i] = t / U[i][i];
+      y2 = fmax(-1.0, fmin(1.0, y2));
+      double ph = atan2(y1, y0) + phi0;
+      const double thP = acos(y2);
+      const double phP = ph - floor(ph * (1. / RBC_TWO_PI)) * RBC_TWO_PI;
+      spline_interp(sxp, m, n, thP, phP, 3, xx);
+      for (int d = 0; d < 3; d++) xx[d] -= xt[d];
+      const double fk = xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2];
+      const double *w = W + (k + 1) * 6;
+      c1 += w[1] * fk;
+      c2 += w[2] * fk;
+      c3 += w[3] * fk;
+      c4 += w[4] * fk;
+      c5 += w[5] * fk;
     }
     // Min_Quad_2D (common/ModBasicMath.F90:296-328)
-    double l11 = 2. * rhs[3], l12 = rhs[4], l21 = rhs[4], l22 = 2. * rhs[5];
+    double l11 = 2. * c3, l12 = c4, l21 = c4, l22 = 2. * c5;
     double det = l11 * l22 - l12 * l21, xm0 = 0., xm1 = 0.;
     if (det > 0) {
-      double idet = 1. / det, r1 = -rhs[1], r2 = -rhs[2];
+      double idet = 1. / det, r1 = -c1, r2 = -c2;
       xm0 = idet * (l22 * r1 - l12 * r2);
       xm1 = idet * (-l21 * r1 + l11 * r2);
     }
@@ -873,14 +900,11 @@ __global__ void __launch_bounds__(256) k_proj(SingArgs a, int nitems) {
     th0 = thMin;
     phi0 = phiMin;
     for (int d = 0; d < 3; d++) x0[d] = xMin[d];
-    h = 0.5 * h;
   }
   double a30[3];
   spline_interp(blk + vs * SPL_A3, m, n, th0, phi0, 3, a30);
   double dist = a30[0] * (xt[0] - x0[0]) + a30[1] * (xt[1] - x0[1]) + a30[2] * (xt[2] - x0[2]);
-  if (sub != 0) return;
-  const size_t itm = (size_t)gid;
-  double *r = a.item_rec + 12 * itm;
+  double *r = a.item_rec + 12 * (size_t)itm;
   for (int d = 0; d < 3; d++) {
     r[d] = xt[d];
     r[3 + d] = x0[d];
@@ -889,11 +913,25 @@ __global__ void __launch_bounds__(256) k_proj(SingArgs a, int nitems) {
   r[6] = th0;
   r[7] = phi0;
   r[11] = dist;
-  for (int d = 0; d < 3; d++) a.dvitem[3 * itm + d] = 0.;
+  for (int d = 0; d < 3; d++) a.dvitem[3 * (size_t)itm + d] = 0.;
   if (dist > 2 * cr.meshSize) return;
   int slot = atomicAdd(a.work_cnt, 1);
-  a.work[2 * slot] = (int)itm;
+  a.work[2 * slot] = itm;
   a.work[2 * slot + 1] = cj;
+}
+
+// items in target order: (neighbouring cell, item) keys and the item -> slot map
+__global__ void k_items(int nT, const int *__restrict__ cnt, const int *__restrict__ item_off,
+                        const int *__restrict__ slot_cell, int *key, int *val, int *item_slot) {
+  int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ti >= nT) return;
+  int o = item_off[ti];
+  for (int p = 0; p < cnt[ti]; p++) {
+    int s = ti * RBC_NBR_MAX + p;
+    key[o + p] = slot_cell[s];
+    val[o + p] = o + p;
+    item_slot[o + p] = s;
+  }
 }
 
 // RBC_NearSingInt_ReAdd (common/ModRbcSingInt.F90:232-310), lanes over the 16 x 32 patch
@@ -1114,13 +1152,14 @@ __global__ void k_combine(CombineArgs a) {
 // table pointer arrays per grid type, owned by the context (rebuilt when a grid
 // type is added)
 struct TypePtrs {
-  const double *const *mask, *const *th, *const *phi, *const *thG, *const *phiG, *const *pwt;
+  const double *const *mask, *const *th, *const *phi, *const *thG, *const *phiG, *const *pwt,
+      *const *proj;
   const int *nrad, *nazm;
 };
 static int type_ptrs(rbc3d_ctx *c, TypePtrs *tp) {
   size_t nt = c->gtypes.size();
   if (c->tp_ntypes != nt) {
-    std::vector<const double *> ptr(6 * nt);
+    std::vector<const double *> ptr(7 * nt);
     std::vector<int> ni(2 * nt);
     for (size_t i = 0; i < nt; i++) {
       GridType *g = c->gtypes[i];
@@ -1130,6 +1169,7 @@ static int type_ptrs(rbc3d_ctx *c, TypePtrs *tp) {
       ptr[3 * nt + i] = g->d_thG.p;
       ptr[4 * nt + i] = g->d_phiG.p;
       ptr[5 * nt + i] = g->d_pwt.p;
+      ptr[6 * nt + i] = g->d_proj.p;
       ni[i] = g->nrad;
       ni[nt + i] = g->nazm;
     }
@@ -1146,6 +1186,7 @@ static int type_ptrs(rbc3d_ctx *c, TypePtrs *tp) {
   tp->thG = p + 3 * nt;
   tp->phiG = p + 4 * nt;
   tp->pwt = p + 5 * nt;
+  tp->proj = p + 6 * nt;
   tp->nrad = c->d_tp_int.p;
   tp->nazm = c->d_tp_int.p + nt;
   return RBC3D_OK;
@@ -1189,8 +1230,8 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   }
   pa.rc = c->conf.rc;
   pa.rc2hi = c->conf.rc * c->conf.rc * (1.0 + 1e-12);
-  pa.noffs = (int)c->noffs;
-  pa.offs = c->d_offs.p;
+  pa.nrows = (int)c->noffs;
+  pa.rows = c->d_offs.p;
   pa.reps = c->r_eps;
   pa.c1 = c1;
   pa.c2 = c2;
@@ -1219,7 +1260,14 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   pa.stat = c->d_pairstat.p;
   int nblk = nb + (nT + PT - 1) / PT + 1;
   rbc_stage_begin(c, ST_PAIR);
-  k_pair<<<nblk, PT, 0, st>>>(pa);
+  if (c1 != 0 && c2 != 0)
+    k_pair<true, true><<<nblk, PT * PW, 0, st>>>(pa);
+  else if (c1 != 0)
+    k_pair<true, false><<<nblk, PT * PW, 0, st>>>(pa);
+  else if (c2 != 0)
+    k_pair<false, true><<<nblk, PT * PW, 0, st>>>(pa);
+  else
+    k_pair<false, false><<<nblk, PT * PW, 0, st>>>(pa);
   rbc_stage_end(c, ST_PAIR);
   HIP_TRY(hipGetLastError());
   // neighbouring-cell lists
@@ -1265,12 +1313,14 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
     rbc_set_error("more than 32 neighbouring cells within the cut-off of a target");
     return RBC3D_ENBR;
   }
+  c->last_nitems = nitems;
+  c->last_nwork = 0;
   // singular and near-singular corrections
   RBC_TRY(c->d_scratch.ensure((size_t)12 * nitems + (size_t)3 * nT + 16));
   RBC_TRY(c->d_dvitem.ensure((size_t)3 * nitems + 3));
   RBC_TRY(c->d_work_cnt.ensure(2));
   // d_work holds the sorted-position -> node map; the near-singular worklist is separate
-  DBuf<int> wl;
+  DBuf<int> &wl = c->d_worklist;
   RBC_TRY(wl.ensure(2 * (size_t)nitems + 2));
   HIP_TRY(hipMemsetAsync(c->d_work_cnt.p, 0, sizeof(int), st));
   SingArgs sa;
@@ -1324,12 +1374,32 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   HIP_TRY(hipGetLastError());
   if (nitems > 0) {
     rbc_stage_begin(c, ST_PROJ);
-    k_proj<<<(unsigned)(((size_t)nitems * 16 + 255) / 256), 256, 0, st>>>(sa, nitems);
+    RBC_TRY(c->d_ikey.ensure(2 * (size_t)nitems));
+    RBC_TRY(c->d_ival.ensure(2 * (size_t)nitems));
+    RBC_TRY(c->d_islot.ensure(nitems));
+    int *k_in = c->d_ikey.p, *k_out = k_in + nitems, *v_in = c->d_ival.p, *v_out = v_in + nitems;
+    k_items<<<(nT + 255) / 256, 256, 0, st>>>(nT, c->d_nbr_cnt.p, item_off, c->d_nbr_cell.p, k_in,
+                                              v_in, c->d_islot.p);
+    HIP_TRY(hipGetLastError());
+    int bits = 1;
+    while ((1 << bits) < c->nrbc) bits++;
+    size_t tmp = 0;
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, k_in, k_out, v_in, v_out, nitems, 0,
+                                               bits, st));
+    RBC_TRY(c->d_cub_tmp.ensure(tmp + 16));
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->d_cub_tmp.p, tmp, k_in, k_out, v_in, v_out, nitems,
+                                               0, bits, st));
+    sa.item_order = v_out;
+    sa.item_slot = c->d_islot.p;
+    sa.proj = tp->proj;
+    const int nblk = (nitems + 255) / 256, per = (nblk + 7) / 8;
+    k_proj<<<8 * per, 256, 0, st>>>(sa, nitems);
     rbc_stage_end(c, ST_PROJ);
     HIP_TRY(hipGetLastError());
     int nwork = 0;
     HIP_TRY(hipMemcpyAsync(&nwork, c->d_work_cnt.p, sizeof(int), hipMemcpyDeviceToHost, st));
     HIP_TRY(hipStreamSynchronize(st));
+    c->last_nwork = nwork;
     if (nwork > 0) {
       rbc_stage_begin(c, ST_NEAR);
       k_near<<<(unsigned)(((size_t)nwork * 64 + 255) / 256), 256, 0, st>>>(sa, nwork);
@@ -1339,7 +1409,7 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   }
   // linear double-layer term
   double *xv = c->d_scratch.p + (size_t)12 * nitems + (size_t)3 * nT + 8;
-  DBuf<double> part;
+  DBuf<double> &part = c->d_linpart;
   if (c2 != 0) {
     int nbk = 256;
     RBC_TRY(part.ensure(3 * nbk));
