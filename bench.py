@@ -103,6 +103,9 @@ def timestep_bench(steps, warmup):
     total_ms, its = 0.0, []
     res = None
     for k in range(warmup + steps):
+        if k == warmup:
+            g.timing(-1)
+            g.timing(1)
         g.set_wall_density(f0)
         g.sync()
         e0 = torch.cuda.Event(enable_timing=True)
@@ -117,6 +120,8 @@ def timestep_bench(steps, warmup):
         if k >= warmup:
             total_ms += max(e0.elapsed_time(e1), 1e3 * wall)
             its.append(res[0])
+    stages = g.stage_times()
+    g.timing(0)
     cells = w.cells
     return {
         "workload": f"C3: {cells.ncell} biconcave cells (radius 1.722, nlat 13x26, 25% "
@@ -129,6 +134,7 @@ def timestep_bench(steps, warmup):
         "gmres_converged": res[2],
         "wall_velocity_after": res[3],
         "wall_self_matrix_nnz": int(nnz.value),
+        "stages_ms_per_step": {k: round(v[0] / steps, 4) for k, v in stages.items() if v[1]},
     }
 
 
@@ -144,6 +150,9 @@ def main():
     ap.add_argument("--no-timestep", action="store_true",
                     help="skip the tube timestep (GMRES seconds per timestep)")
     ap.add_argument("--timestep-steps", type=int, default=3)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run one plain 512 MiB device copy first (a known-size kernel for "
+                         "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
@@ -170,6 +179,12 @@ def main():
     torch.cuda.synchronize()
     stream = torch.cuda.ExternalStream(sysg.stream())
 
+    if args.copy_calibration:
+        src = torch.ones(64 << 20, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     for _ in range(args.warmup):
         sysg.cell_matmult_dev(u.data_ptr(), b.data_ptr())
     sysg.sync()

@@ -516,7 +516,7 @@ extern "C" int rbc3d_destroy(rbc3d_ctx *c) {
   for (DBuf<double> *b : {&c->d_sl1, &c->d_sl2, &c->d_dl1, &c->d_mtab, &c->d_x, &c->d_a3, &c->d_detj,
                           &c->d_ds, &c->d_f, &c->d_g, &c->d_fs, &c->d_gs, &c->d_spl, &c->d_src_rec,
                           &c->d_tx, &c->d_tA, &c->d_v, &c->d_vacc, &c->d_nbr_dist, &c->d_dvitem,
-                          &c->d_scratch, &c->d_cell_sph, &c->d_grid, &c->d_cgrid, &c->d_vgrid,
+                          &c->d_scratch, &c->d_cell_sph, &c->d_chunk_sph, &c->d_grid, &c->d_cgrid, &c->d_vgrid,
                           &c->d_bb, &c->d_tw, &c->d_pme_src, &c->d_shbuf, &c->d_shbuf2,
                           &c->wall.d_x, &c->wall.d_f, &c->wall.d_area, &c->wall.d_eps,
                           &c->wall.d_cen, &c->wall.d_val})
@@ -557,7 +557,7 @@ extern "C" int rbc3d_set_cells(rbc3d_ctx *c, int nrbc, const int *nlat0, const i
     return RBC3D_EINVAL;
   c->nrbc = nrbc;
   c->cells.resize(nrbc);
-  int np = 0, coff = 0;
+  int np = 0, coff = 0, choff = 0;
   long long spl = 0;
   for (int i = 0; i < nrbc; i++) {
     if (nlat[i] < 2 || nlat[i] > 64 || nlat0[i] < 1 || nlat0[i] > nlat[i] || celltype[i] < 1 ||
@@ -580,6 +580,8 @@ extern "C" int rbc3d_set_cells(rbc3d_ctx *c, int nrbc, const int *nlat0, const i
     r.spl = spl;
     r.coff = coff;
     coff += 3 * r.nlat0 * r.nlat0;
+    r.choff = choff;
+    choff += (r.nlat * r.nlon + 63) / 64;
     int npt = g->nlat * g->nlon;
     // area and mesh size (RBC_ComputeGeometry, common/ModRbc.F90:482-505)
     double area = 0;
@@ -592,6 +594,7 @@ extern "C" int rbc3d_set_cells(rbc3d_ctx *c, int nrbc, const int *nlat0, const i
     spl += (long long)SPL_NVAR * 2 * g->nlat * g->nlon * 4;
   }
   c->np = np;
+  c->nchunks = choff;
   c->spl_total = spl;
   hipStream_t st = c->stream;
   RBC_TRY(upload(c->d_cells, c->cells, st));
@@ -959,7 +962,7 @@ extern "C" int rbc3d_cell_rhs_dev(rbc3d_ctx *c, double *d_rhs) {
                                   c->d_v.p));
   if (c->wall.nvert)
     RBC_TRY(rbc_add_int_on_walls_dev(c, c1, np, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
-                                     c->d_v.p));
+                                     c->d_v.p, false));
   RBC_TRY(rbc_pme_init(c));
   RBC_TRY(rbc_pme_spread(c, c1, c2, 1, 1));
   RBC_TRY(rbc_pme_transform(c));
@@ -993,7 +996,7 @@ extern "C" int rbc3d_calc_velocity_field(rbc3d_ctx *c, int nT, const double *tx,
                                     c->d_v.p));
   if (c->wall.nvert)
     RBC_TRY(rbc_add_int_on_walls_dev(c, c1, nT, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
-                                     c->d_v.p));
+                                     c->d_v.p, tindx[0] == c->wall.id));
   RBC_TRY(rbc_pme_init(c));
   RBC_TRY(rbc_pme_spread(c, c1, c2, 1, 1));
   RBC_TRY(rbc_pme_transform(c));

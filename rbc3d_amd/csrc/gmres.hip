@@ -71,9 +71,6 @@ __global__ void k_sub(int n, const double *__restrict__ b, double *r) {  // r = 
   if (i < n) r[i] = b[i] - r[i];
 }
 
-struct GmresWork {
-  DBuf<double> V, w, part, hd;
-};
 
 static int dots(hipStream_t st, GmresWork &ws, int n, int k, const double *V, size_t ld,
                 const double *w, double *h_host) {
@@ -94,7 +91,7 @@ int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, doubl
               std::vector<double> *history) {
   hipStream_t st = c->stream;
   if (restart < 1 || restart > G_MAXV - 1) return RBC3D_EINVAL;
-  GmresWork ws;
+  GmresWork &ws = c->gmres;
   size_t ld = (size_t)n + 64;
   RBC_TRY(ws.V.ensure(ld * (restart + 1)));
   RBC_TRY(ws.w.ensure(n + 1));
@@ -137,9 +134,8 @@ int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, doubl
       double *vj = V + ld * it, *vn = V + ld * (it + 1);
       RBC_TRY(matvec(vj, vn));
       // classical Gram-Schmidt: h = V^T w, w -= V h
+      // (the projections stay on the device in ws.hd for the update)
       RBC_TRY(dots(st, ws, n, it + 1, V, ld, vn, h.data()));
-      HIP_TRY(hipMemcpyAsync(ws.hd.p, h.data(), sizeof(double) * (it + 1), hipMemcpyHostToDevice, st));
-      HIP_TRY(hipStreamSynchronize(st));
       k_maxpy<<<nb, 256, 0, st>>>(n, it + 1, V, ld, ws.hd.p, vn);
       double tt2;
       RBC_TRY(dots(st, ws, n, 1, vn, ld, vn, &tt2));

@@ -81,6 +81,11 @@ struct GridType {
   DBuf<double> d_proj;   // projection tables (RBC_PJ_*)
 };
 
+// GMRES work space (Krylov basis, partial dot products), kept across solves
+struct GmresWork {
+  DBuf<double> V, w, part, hd;
+};
+
 // Stage timing (HIP events on the context's stream, enabled on request)
 enum {
   ST_TSORT = 0, ST_PAIR, ST_NBR, ST_SELF, ST_PROJ, ST_NEAR, ST_LINEAR, ST_COMBINE, ST_SH_GLOB,
@@ -105,6 +110,7 @@ struct CellRec {
   int nlat, nlon, nlat0;
   int celltype;  // 1-based
   int coff;      // offset of the cell's block in the Glob_Sph_Trans vector
+  int choff;     // first 64-node chunk of the cell (chunk bounding spheres)
   long long spl;  // offset (doubles) of the cell's spline block
   double Acoef, Bcoef, area, meshSize, radius;
 };
@@ -170,6 +176,8 @@ struct rbc3d_ctx {
   DBuf<unsigned long long> d_pairstat;
   // cell bounding spheres binned by centre (neighbour-cell candidates)
   DBuf<double> d_cell_sph;
+  DBuf<double> d_chunk_sph;  // per 64-node chunk of every cell: centre(3), radius
+  int nchunks = 0;
   DBuf<int> d_cellbin_start, d_cellbin_order;
   double max_cell_radius = 0;
   // PME
@@ -198,6 +206,7 @@ struct rbc3d_ctx {
   // last pair statistics
   long long last_npairs = 0, last_nitems = 0, last_nwork = 0;
   DBuf<int> d_worklist, d_ikey, d_ival, d_islot;
+  GmresWork gmres;
   DBuf<double> d_linpart;
   unsigned long long last_checksum = 0;
 };
@@ -336,7 +345,7 @@ int rbc_pme_gather(rbc3d_ctx *c, int nT, const double *d_tx, const int *d_act, c
 int rbc_wall_setup(rbc3d_ctx *c);
 int rbc_wall_prepare(rbc3d_ctx *c);
 int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double c1, int nT, const double *d_tx, const int *d_tindx,
-                             const int *d_act, const double *d_A, double *d_v);
+                             const int *d_act, const double *d_A, double *d_v, bool wall_targets);
 int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, double *)> &matvec,
               const double *d_b, double *d_x, bool x0_nonzero, double rtol, double atol,
               double dtol, int maxit, int restart, int *niter, double *rnorm, int *reason,

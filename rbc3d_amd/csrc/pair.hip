@@ -276,6 +276,39 @@ __global__ void k_cell_sphere(int nrbc, const CellRec *__restrict__ cells, const
   }
 }
 
+// bounding spheres of each cell's consecutive 64-node chunks: one wavefront per chunk
+__global__ void k_chunk_sphere(int nrbc, const CellRec *__restrict__ cells,
+                               const double *__restrict__ x, int np, double *sph) {
+  const int ci = blockIdx.x;
+  if (ci >= nrbc) return;
+  const CellRec cr = cells[ci];
+  const int npt = cr.nlat * cr.nlon, nch = (npt + 63) / 64;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  for (int k = wv; k < nch; k += nw) {
+    const int q = 64 * k + lane, cnt = min(64, npt - 64 * k);
+    double p[3] = {0, 0, 0};
+    if (q < npt)
+      for (int d = 0; d < 3; d++) p[d] = x[cr.off + q + (size_t)np * d];
+    double cen[3];
+    for (int d = 0; d < 3; d++) {
+      double v = p[d];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      cen[d] = v / cnt;
+    }
+    double r2 = 0;
+    if (q < npt)
+      for (int d = 0; d < 3; d++) r2 += (p[d] - cen[d]) * (p[d] - cen[d]);
+    for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
+    if (lane == 0) {
+      double *s = sph + 4 * (size_t)(cr.choff + k);
+      s[0] = cen[0];
+      s[1] = cen[1];
+      s[2] = cen[2];
+      s[3] = sqrt(r2) * (1 + 1e-12) + 1e-12;
+    }
+  }
+}
+
 int rbc_build_source_list(rbc3d_ctx *c) {
   int np = c->np;
   RBC_TRY(c->d_src_bin.ensure(np + 1));
@@ -288,6 +321,10 @@ int rbc_build_source_list(rbc3d_ctx *c) {
   RBC_TRY(c->d_src_rec.ensure((size_t)SREC * np + 1));
   RBC_TRY(c->d_cell_sph.ensure((size_t)4 * c->nrbc + 4));
   k_cell_sphere<<<c->nrbc, 256, 0, c->stream>>>(c->nrbc, c->d_cells.p, c->d_x.p, np, c->d_cell_sph.p);
+  HIP_TRY(hipGetLastError());
+  RBC_TRY(c->d_chunk_sph.ensure((size_t)4 * c->nchunks + 4));
+  k_chunk_sphere<<<c->nrbc, 256, 0, c->stream>>>(c->nrbc, c->d_cells.p, c->d_x.p, np,
+                                                 c->d_chunk_sph.p);
   HIP_TRY(hipGetLastError());
   return rbc_update_density(c, 0);
 }
@@ -544,7 +581,7 @@ struct NbrArgs {
   const double *x;  // node coordinates [d][np]
   const int *src_bin;
   const CellRec *cells;
-  const double *sph;
+  const double *sph, *chunk_sph;
   const double *tx;
   const int *tindx, *tact;
   int *cnt, *slot_cell, *slot_node;
@@ -592,10 +629,24 @@ __global__ void __launch_bounds__(NBR_T) k_nbr(NbrArgs a) {
       m &= m - 1;
       int c2 = cbase + kk;
       const CellRec cr = a.cells[c2];
-      int npt = cr.nlat * cr.nlon;
+      int npt = cr.nlat * cr.nlon, nch = (npt + 63) / 64;
       double best = 0;
       int bkey = 0, bnode = -1;
-      for (int q = lane; q < npt; q += 64) {
+      // only the 64-node chunks whose bounding sphere reaches within rc of the target
+      for (int cb = 0; cb < nch; cb += 64) {
+        bool cp = false;
+        if (cb + lane < nch) {
+          const double *s = a.chunk_sph + 4 * (size_t)(cr.choff + cb + lane);
+          double cen[3] = {s[0], s[1], s[2]}, dd[3];
+          double rcen = ref_min_image_dist(xi, cen, a.Lb, a.iLb, dd);
+          cp = !(rcen > a.rc + s[3] + 1e-9 * a.rc);
+        }
+        unsigned long long cm = __ballot(cp);
+        while (cm) {
+        const int kc = cb + __ffsll((long long)cm) - 1;
+        cm &= cm - 1;
+        const int q = 64 * kc + lane;
+        if (q >= npt) continue;
         int node = cr.off + q;
         double xj[3] = {a.x[node], a.x[node + np], a.x[node + 2 * np]}, xx[3];
         double rr = ref_min_image_dist(xi, xj, a.Lb, a.iLb, xx);
@@ -610,6 +661,7 @@ __global__ void __launch_bounds__(NBR_T) k_nbr(NbrArgs a) {
           best = rr;
           bkey = key;
           bnode = node;
+        }
         }
       }
       for (int o = 32; o > 0; o >>= 1) {
@@ -1290,6 +1342,7 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   na.src_bin = c->d_src_bin.p;
   na.cells = c->d_cells.p;
   na.sph = c->d_cell_sph.p;
+  na.chunk_sph = c->d_chunk_sph.p;
   na.tx = d_tx;
   na.tindx = d_tindx;
   na.tact = d_act;

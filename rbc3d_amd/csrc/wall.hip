@@ -627,15 +627,12 @@ extern "C" int rbc3d_wall_self_nnz(rbc3d_ctx *c, long long *nnz) {
 }
 
 int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double c1, int nT, const double *d_tx, const int *d_tindx,
-                             const int *d_act, const double *d_A, double *d_v) {
+                             const int *d_act, const double *d_A, double *d_v, bool wall_targets) {
   WallData &w = c->wall;
   if (!w.nvert || nT == 0) return RBC3D_OK;
   hipStream_t st = c->stream;
-  int first = 0;
-  HIP_TRY(hipMemcpyAsync(&first, d_tindx, sizeof(int), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
   rbc_stage_begin(c, ST_WALL);
-  if (first == w.id) {
+  if (wall_targets) {
     // wall vertices: the precomputed self-interaction; with one wall that is all
     if (!w.prepared) RBC_TRY(rbc_wall_prepare(c));
     k_wall_spmv<<<(w.nvert + 255) / 256, 256, 0, st>>>(w.nvert, c1, w.d_rowptr.p, w.d_col.p, w.d_val.p,
@@ -665,8 +662,10 @@ extern "C" int rbc3d_add_int_on_walls(rbc3d_ctx *c, double c1, int nT, const dou
   HIP_TRY(hipMemcpy(c->d_v.p, v, sizeof(double) * 3 * nT, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->d_tindx.p, tindx, sizeof(int) * 3 * nT, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(c->d_tact.p, active, sizeof(int) * nT, hipMemcpyHostToDevice));
+  // a target list that starts with this wall's id is the wall's own vertex list
+  // (TargetList_Create order: cells, then walls)
   RBC_TRY(rbc_add_int_on_walls_dev(c, c1, nT, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
-                                   c->d_v.p));
+                                   c->d_v.p, tindx[0] == c->wall.id));
   HIP_TRY(hipMemcpyAsync(v, c->d_v.p, sizeof(double) * 3 * nT, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   return RBC3D_OK;
@@ -728,7 +727,7 @@ int rbc_wall_matmult_dev(rbc3d_ctx *c, const double *d_u, double *d_b) {
   HIP_TRY(hipMemsetAsync(c->d_v.p, 0, sizeof(double) * 3 * nv, st));
   double c1 = 1. / (4. * RBC_PI);
   RBC_TRY(rbc_add_int_on_walls_dev(c, c1, nv, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
-                                   c->d_v.p));
+                                   c->d_v.p, true));
   RBC_TRY(rbc_pme_init(c));
   RBC_TRY(rbc_pme_spread(c, c1, 0., 0, 1));
   RBC_TRY(rbc_pme_transform(c));
@@ -780,7 +779,7 @@ int rbc_wall_residual_dev(rbc3d_ctx *c) {
     RBC_TRY(rbc_add_int_on_rbcs_dev(c, c1, c2, nv, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
                                     c->d_v.p));
   RBC_TRY(rbc_add_int_on_walls_dev(c, c1, nv, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
-                                   c->d_v.p));
+                                   c->d_v.p, true));
   RBC_TRY(rbc_pme_init(c));
   RBC_TRY(rbc_pme_spread(c, c1, c2, 1, 1));
   RBC_TRY(rbc_pme_transform(c));
