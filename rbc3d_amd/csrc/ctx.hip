@@ -776,7 +776,12 @@ extern "C" int rbc3d_pme_grid(rbc3d_ctx *c, int which, double *out) {
   if (!c->pme_ready) return RBC3D_ESTATE;
   size_t ng = (size_t)c->conf.Nb[0] * c->conf.Nb[1] * c->conf.Nb[2];
   HIP_TRY(hipStreamSynchronize(c->stream));
-  if (which == 0)
+  // the spread writes only the fields its coefficients make nonzero
+  if (which == 0 && !c->flag_sl)
+    std::fill(out, out + 3 * ng, 0.);
+  else if (which == 1 && !c->flag_dl)
+    std::fill(out, out + 9 * ng, 0.);
+  else if (which == 0)
     HIP_TRY(hipMemcpy(out, c->d_grid.p, 3 * ng * 8, hipMemcpyDeviceToHost));
   else if (which == 1)
     HIP_TRY(hipMemcpy(out, c->d_grid.p + 3 * ng, 9 * ng * 8, hipMemcpyDeviceToHost));
@@ -792,10 +797,11 @@ static int sh_host(int kind, int nlat, int nlon, int nt, double *g, int idg, int
   int mmax = std::min(nlat, nlon / 2 + 1);
   if (ida < mmax || jda < nlat) return RBC3D_EINVAL;
   size_t ng = (size_t)idg * jdg * nt, na = (size_t)ida * jda * nt;
-  double *dg, *da, *db;
-  HIP_TRY(hipMalloc(&dg, ng * 8));
-  HIP_TRY(hipMalloc(&da, na * 8));
-  HIP_TRY(hipMalloc(&db, na * 8));
+  DBuf<double> bg, ba, bb;  // released on every return path
+  RBC_TRY(bg.ensure(ng));
+  RBC_TRY(ba.ensure(na));
+  RBC_TRY(bb.ensure(na));
+  double *dg = bg.p, *da = ba.p, *db = bb.p;
   int r;
   if (kind == 0) {
     HIP_TRY(hipMemcpy(dg, g, ng * 8, hipMemcpyHostToDevice));
@@ -811,9 +817,6 @@ static int sh_host(int kind, int nlat, int nlon, int nt, double *g, int idg, int
     r = rbc_sh_synth_batch(nlat, nlon, nt, dg, idg, jdg, da, db, ida, jda, kind == 2, 0);
     if (!r) HIP_TRY(hipMemcpy(g, dg, ng * 8, hipMemcpyDeviceToHost));
   }
-  (void)hipFree(dg);
-  (void)hipFree(da);
-  (void)hipFree(db);
   return r;
 }
 extern "C" int rbc3d_sh_anal_gau(int nlat, int nlon, int nt, const double *g, int idg, int jdg,

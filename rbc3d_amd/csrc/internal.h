@@ -280,6 +280,26 @@ __device__ __forceinline__ double mask_fn(const double *__restrict__ mt, double 
   return mt[i] * (i + 1 - s) + mt[i + 1] * (s - i);
 }
 
+// Division of a 32-bit unsigned n by a divisor fixed at launch, by multiply-high and
+// shift (Granlund and Montgomery, "Division by invariant integers using
+// multiplication", 1994, round-up method): q = (umulhi(n, m) + n) >> s with
+// s = ceil(log2 d), m = floor(2^32 (2^s - d) / d) + 1; exact for every n < 2^32.
+struct FastDiv {
+  unsigned d, m, s;
+};
+inline FastDiv make_fastdiv(unsigned d) {
+  FastDiv f;
+  f.d = d;
+  f.s = 0;
+  while ((1ull << f.s) < d) f.s++;
+  f.m = (unsigned)(((1ull << 32) * ((1ull << f.s) - d)) / d + 1);
+  return f;
+}
+__host__ __device__ __forceinline__ unsigned fd_div(unsigned n, const FastDiv &f) {
+  unsigned t = (unsigned)(((unsigned long long)n * f.m) >> 32);  // multiply-high
+  return (unsigned)(((unsigned long long)t + n) >> f.s);
+}
+
 // Spline_Interp (common/ModSpline.F90:150-191) on a cell's spline block.
 // blk points at variable v0; nvar consecutive variables are interpolated.
 __device__ __forceinline__ void spline_interp(const double *__restrict__ blk, int m, int n,

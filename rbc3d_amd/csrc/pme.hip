@@ -198,6 +198,7 @@ __global__ void k_spread_prep(int ns, int ncell, const double *__restrict__ x, i
 
 struct SpreadArgs {
   int N1, N2, N3, P, nbx, nby, nbz, reclen, ns;
+  FastDiv frl;  // LDS record length
   double c1, c2;
   int flag_sl, flag_dl;
   const double *rec;
@@ -208,15 +209,22 @@ struct SpreadArgs {
 // One workgroup per brick of the grid.  A source with stencil origin g covers
 // g..g+P-1 (mod N), so brick [b0, b0+B) receives sources with origin in
 // [b0-P+1, b0+B-1] (mod N): the bricks that overlap that range on each axis.
+// SL / DL select the single-layer (3) and double-layer (9) fields; sources stream
+// through LDS in tiles of SPR_TILE records holding only what the fields need.
+#define SPR_TILE 64
+template <bool SL, bool DL>
 __global__ void __launch_bounds__(SPR_T) k_spread(SpreadArgs a) {
-  extern __shared__ double srec[];  // SPR_T records
-  int bid = blockIdx.x;
-  int bx = bid % a.nbx, by = (bid / a.nbx) % a.nby, bz = bid / (a.nbx * a.nby);
-  int ib0 = bx * SPR_B, jb0 = by * SPR_B, kb0 = bz * SPR_B;
-  int P = a.P;
+  constexpr int NF = (SL ? 3 : 0) + (DL ? 9 : 0);
+  constexpr int MAXP = 12;
+  __shared__ double srec[SPR_TILE * (3 + 3 * MAXP + NF)];
+  const int bid = blockIdx.x;
+  const int bx = bid % a.nbx, by = (bid / a.nbx) % a.nby, bz = bid / (a.nbx * a.nby);
+  const int ib0 = bx * SPR_B, jb0 = by * SPR_B, kb0 = bz * SPR_B;
+  const int P = a.P;
+  const int rl = 3 + 3 * P + NF;  // LDS record: origin, weights, fields
   // lists of source bricks per axis (deduplicated, at most 3 + wrap)
   int lst[3][4], nl[3];
-  int Ns[3] = {a.N1, a.N2, a.N3}, b0s[3] = {ib0, jb0, kb0}, nbs[3] = {a.nbx, a.nby, a.nbz};
+  int Ns[3] = {a.N1, a.N2, a.N3}, b0s[3] = {ib0, jb0, kb0};
   for (int ax = 0; ax < 3; ax++) {
     nl[ax] = 0;
     int lo = b0s[ax] - P + 1, hi = b0s[ax] + SPR_B - 1;
@@ -230,7 +238,7 @@ __global__ void __launch_bounds__(SPR_T) k_spread(SpreadArgs a) {
   }
   // my grid points (two per lane for B = 8)
   const int npts = SPR_B * SPR_B * SPR_B;
-  double acc[2][12];
+  double acc[2][NF > 0 ? NF : 1];
   int gi[2], gj[2], gk[2];
   bool valid[2];
   for (int s = 0; s < 2; s++) {
@@ -239,25 +247,27 @@ __global__ void __launch_bounds__(SPR_T) k_spread(SpreadArgs a) {
     gj[s] = jb0 + (pt / SPR_B) % SPR_B;
     gk[s] = kb0 + pt / (SPR_B * SPR_B);
     valid[s] = pt < npts && gi[s] < a.N1 && gj[s] < a.N2 && gk[s] < a.N3;
-    for (int f = 0; f < 12; f++) acc[s][f] = 0;
+    for (int f = 0; f < NF; f++) acc[s][f] = 0;
   }
-  int rl = a.reclen;
+  const int grl = a.reclen;  // global record: origin, weights, f(3), t(9)
+  const int goff = SL ? 0 : 3;  // first global field slot used
   for (int qz = 0; qz < nl[2]; qz++)
     for (int qy = 0; qy < nl[1]; qy++)
       for (int qx = 0; qx < nl[0]; qx++) {
         int sb = lst[0][qx] + a.nbx * (lst[1][qy] + a.nby * lst[2][qz]);
         int s0 = a.brick_start[sb], s1 = a.brick_start[sb + 1];
-        for (int base = s0; base < s1; base += SPR_T) {
-          int cnt = min(SPR_T, s1 - base);
+        for (int base = s0; base < s1; base += SPR_TILE) {
+          const int cnt = min(SPR_TILE, s1 - base);
           __syncthreads();
           for (int t = threadIdx.x; t < cnt * rl; t += SPR_T) {
-            int q = t / rl, e = t % rl;
-            srec[t] = a.rec[(size_t)rl * a.order[base + q] + e];
+            const int q = fd_div(t, a.frl), e = t - q * rl;
+            const int ge = e < 3 + 3 * P ? e : e + goff;
+            srec[t] = a.rec[(size_t)grl * a.order[base + q] + ge];
           }
           __syncthreads();
           for (int q = 0; q < cnt; q++) {
             const double *r = srec + q * rl;
-            int i0 = (int)r[0], j0 = (int)r[1], k0 = (int)r[2];
+            const int i0 = (int)r[0], j0 = (int)r[1], k0 = (int)r[2];
             for (int s = 0; s < 2; s++) {
               if (!valid[s]) continue;
               int di = gi[s] - i0, dj = gj[s] - j0, dk = gk[s] - k0;
@@ -265,25 +275,26 @@ __global__ void __launch_bounds__(SPR_T) k_spread(SpreadArgs a) {
               dj += dj < 0 ? a.N2 : 0;
               dk += dk < 0 ? a.N3 : 0;
               if (di >= P || dj >= P || dk >= P) continue;
-              double wxyz = r[3 + di] * r[3 + P + dj] * r[3 + 2 * P + dk];
+              const double wxyz = r[3 + di] * r[3 + P + dj] * r[3 + 2 * P + dk];
               const double *fv = r + 3 + 3 * P;
-              if (a.flag_sl) {
-                double cw = a.c1 * wxyz;
+              if (SL) {
+                const double cw = a.c1 * wxyz;
                 for (int f = 0; f < 3; f++) acc[s][f] += cw * fv[f];
               }
-              if (a.flag_dl) {
-                double cw = a.c2 * wxyz;
-                for (int f = 0; f < 9; f++) acc[s][3 + f] += cw * fv[3 + f];
+              if (DL) {
+                const double cw = a.c2 * wxyz;
+                constexpr int o = SL ? 3 : 0;
+                for (int f = 0; f < 9; f++) acc[s][o + f] += cw * fv[o + f];
               }
             }
           }
         }
       }
-  size_t ng = (size_t)a.N1 * a.N2 * a.N3;
+  const size_t ng = (size_t)a.N1 * a.N2 * a.N3;
   for (int s = 0; s < 2; s++) {
     if (!valid[s]) continue;
-    size_t o = gi[s] + (size_t)a.N1 * (gj[s] + (size_t)a.N2 * gk[s]);
-    for (int f = 0; f < 12; f++) a.grid[o + ng * f] = acc[s][f];
+    const size_t o = gi[s] + (size_t)a.N1 * (gj[s] + (size_t)a.N2 * gk[s]);
+    for (int f = 0; f < NF; f++) a.grid[o + ng * (goff + f)] = acc[s][f];
   }
 }
 
@@ -356,9 +367,18 @@ int rbc_pme_spread(rbc3d_ctx *c, double c1, double c2, int cells, int walls) {
   a.order = order;
   a.brick_start = c->d_pme_start.p;
   a.grid = c->d_grid.p;
-  size_t lds = sizeof(double) * SPR_T * reclen;
+  if (P > 12) {
+    rbc_set_error("B-spline order above 12");
+    return RBC3D_EINVAL;
+  }
+  a.frl = make_fastdiv(3 + 3 * P + (c->flag_sl ? 3 : 0) + (c->flag_dl ? 9 : 0));
   rbc_stage_begin(c, ST_SPREAD);
-  k_spread<<<nbr, SPR_T, lds, st>>>(a);
+  if (c->flag_sl && c->flag_dl)
+    k_spread<true, true><<<nbr, SPR_T, 0, st>>>(a);
+  else if (c->flag_sl)
+    k_spread<true, false><<<nbr, SPR_T, 0, st>>>(a);
+  else if (c->flag_dl)
+    k_spread<false, true><<<nbr, SPR_T, 0, st>>>(a);
   rbc_stage_end(c, ST_SPREAD);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;
@@ -370,13 +390,14 @@ struct FftPlan {
   int N, nrad, rad[16];
   int sign;
   const double2 *tw;  // exp(-2 pi i m/N)
+  FastDiv fNR[16], fNs[16];  // N / rad[s] and the span Ns before pass s
 };
 
 __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
-__device__ __forceinline__ double2 twid(const double2 *tw, int m, int N, int sign) {
-  double2 t = tw[m % N];
+__device__ __forceinline__ double2 twid(const double2 *tw, int m, int sign) {  // m < N
+  double2 t = tw[m];
   if (sign > 0) t.y = -t.y;
   return t;
 }
@@ -387,19 +408,20 @@ __device__ double2 *fft_lds(double2 *buf, double2 *tmp, int TL, const FftPlan &p
   int N = pl.N, Ns = 1;
   double2 *in = buf, *out = tmp;
   for (int s = 0; s < pl.nrad; s++) {
-    int R = pl.rad[s], NR = N / R;
+    const int R = pl.rad[s], NR = pl.fNR[s].d;
+    const int tws = NR / Ns;  // twiddle stride N / (Ns R)
     for (int t = threadIdx.x; t < TL * NR; t += blockDim.x) {
-      int l = t / NR, j = t % NR;
+      const int l = fd_div(t, pl.fNR[s]), j = t - l * NR;
       const double2 *src = in + (size_t)l * N;
       double2 *dst = out + (size_t)l * N;
-      int k = j % Ns;
+      const int jq = fd_div(j, pl.fNs[s]), k = j - jq * Ns;
       double2 v[5];
       for (int r = 0; r < R; r++) {
         double2 x = src[j + r * NR];
-        if (r && k) x = cmul(x, twid(pl.tw, r * k * (N / (Ns * R)), N, pl.sign));
+        if (r && k) x = cmul(x, twid(pl.tw, r * k * tws, pl.sign));
         v[r] = x;
       }
-      int base = (j / Ns) * Ns * R + k;
+      int base = jq * Ns * R + k;
       if (R == 2) {
         dst[base] = make_double2(v[0].x + v[1].x, v[0].y + v[1].y);
         dst[base + Ns] = make_double2(v[0].x - v[1].x, v[0].y - v[1].y);
@@ -417,8 +439,11 @@ __device__ double2 *fft_lds(double2 *buf, double2 *tmp, int TL, const FftPlan &p
       } else {
         for (int q = 0; q < R; q++) {
           double2 sum = v[0];
+          int qr = 0;  // (q r) mod R
           for (int r = 1; r < R; r++) {
-            double2 w = twid(pl.tw, ((q * r) % R) * NR, N, pl.sign);
+            qr += q;
+            qr -= qr >= R ? R : 0;
+            double2 w = twid(pl.tw, qr * NR, pl.sign);
             double2 p = cmul(v[r], w);
             sum.x += p.x;
             sum.y += p.y;
@@ -438,48 +463,51 @@ __device__ double2 *fft_lds(double2 *buf, double2 *tmp, int TL, const FftPlan &p
 
 // complex lines: base(l) = (l % A)*sa + (l / A)*sb, element stride es
 struct LineMap {
-  int A;
+  FastDiv A;  // lines per outer index
   long long sa, sb, es;
   int nlines;
 };
 
-__global__ void __launch_bounds__(FFT_T) k_fft_c2c(double2 *data, FftPlan pl, LineMap lm, int TL) {
+__global__ void __launch_bounds__(FFT_T) k_fft_c2c(double2 *data, FftPlan pl, LineMap lm, int TL,
+                                                   FastDiv fTL) {
   extern __shared__ double2 lds[];
   double2 *buf = lds, *tmp = lds + (size_t)TL * pl.N;
-  int l0 = blockIdx.x * TL;
-  int N = pl.N;
+  const int l0 = blockIdx.x * TL;
+  const int N = pl.N;
   // load: lines fastest (adjacent lines are adjacent in memory when sa = 1)
   for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
-    int l = t % TL, e = t / TL, L = l0 + l;
+    const int e = fd_div(t, fTL), l = t - e * TL, L = l0 + l;
     if (L >= lm.nlines) continue;
-    long long o = (long long)(L % lm.A) * lm.sa + (long long)(L / lm.A) * lm.sb + (long long)e * lm.es;
+    const int Lq = fd_div(L, lm.A), Lr = L - Lq * (int)lm.A.d;
+    long long o = (long long)Lr * lm.sa + (long long)Lq * lm.sb + (long long)e * lm.es;
     buf[(size_t)l * N + e] = data[o];
   }
   __syncthreads();
   double2 *res = fft_lds(buf, tmp, TL, pl);
   for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
-    int l = t % TL, e = t / TL, L = l0 + l;
+    const int e = fd_div(t, fTL), l = t - e * TL, L = l0 + l;
     if (L >= lm.nlines) continue;
-    long long o = (long long)(L % lm.A) * lm.sa + (long long)(L / lm.A) * lm.sb + (long long)e * lm.es;
+    const int Lq = fd_div(L, lm.A), Lr = L - Lq * (int)lm.A.d;
+    long long o = (long long)Lr * lm.sa + (long long)Lq * lm.sb + (long long)e * lm.es;
     data[o] = res[(size_t)l * N + e];
   }
 }
 
 // x r2c: real rows of N1 -> complex rows of H = N1/2+1
 __global__ void __launch_bounds__(FFT_T) k_fft_r2c(const double *in, double2 *out, FftPlan pl,
-                                                   int nrows, int TL) {
+                                                   int nrows, int TL, FastDiv fN, FastDiv fH) {
   extern __shared__ double2 lds[];
   double2 *buf = lds, *tmp = lds + (size_t)TL * pl.N;
-  int N = pl.N, H = N / 2 + 1, r0 = blockIdx.x * TL;
+  const int N = pl.N, H = N / 2 + 1, r0 = blockIdx.x * TL;
+  // the block's rows are contiguous in memory: element t of the block is row t / N
   for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
-    int l = t / N, e = t % N, R = r0 + l;
-    if (R >= nrows) continue;
-    buf[t] = make_double2(in[(size_t)R * N + e], 0.);
+    if (r0 + fd_div(t, fN) >= nrows) continue;
+    buf[t] = make_double2(in[(size_t)r0 * N + t], 0.);
   }
   __syncthreads();
   double2 *res = fft_lds(buf, tmp, TL, pl);
   for (int t = threadIdx.x; t < TL * H; t += blockDim.x) {
-    int l = t / H, e = t % H, R = r0 + l;
+    const int l = fd_div(t, fH), e = t - l * H, R = r0 + l;
     if (R >= nrows) continue;
     out[(size_t)R * H + e] = res[(size_t)l * N + e];
   }
@@ -488,12 +516,12 @@ __global__ void __launch_bounds__(FFT_T) k_fft_r2c(const double *in, double2 *ou
 // x c2r: complex rows of H -> real rows of N1 (real part of the zero and Nyquist
 // frequencies only, the halfcomplex convention of FFTW's c2r)
 __global__ void __launch_bounds__(FFT_T) k_fft_c2r(const double2 *in, double *out, FftPlan pl,
-                                                   int nrows, int TL) {
+                                                   int nrows, int TL, FastDiv fN) {
   extern __shared__ double2 lds[];
   double2 *buf = lds, *tmp = lds + (size_t)TL * pl.N;
-  int N = pl.N, H = N / 2 + 1, r0 = blockIdx.x * TL;
+  const int N = pl.N, H = N / 2 + 1, r0 = blockIdx.x * TL;
   for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
-    int l = t / N, e = t % N, R = r0 + l;
+    const int l = fd_div(t, fN), e = t - l * N, R = r0 + l;
     if (R >= nrows) continue;
     double2 v;
     if (e == 0) {
@@ -511,9 +539,8 @@ __global__ void __launch_bounds__(FFT_T) k_fft_c2r(const double2 *in, double *ou
   __syncthreads();
   double2 *res = fft_lds(buf, tmp, TL, pl);
   for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
-    int l = t / N, e = t % N, R = r0 + l;
-    if (R >= nrows) continue;
-    out[(size_t)R * N + e] = res[(size_t)l * N + e].x;
+    if (r0 + fd_div(t, fN) >= nrows) continue;
+    out[(size_t)r0 * N + t] = res[t].x;
   }
 }
 
@@ -526,6 +553,12 @@ static FftPlan make_plan(rbc3d_ctx *c, int ax, int sign) {
   for (int i = 0; i < p.nrad; i++) p.rad[i] = r[i];
   p.sign = sign;
   p.tw = (const double2 *)(c->d_tw.p + 2 * (size_t)c->tw_off[ax]);
+  int Ns = 1;
+  for (int i = 0; i < p.nrad; i++) {
+    p.fNR[i] = make_fastdiv(p.N / p.rad[i]);
+    p.fNs[i] = make_fastdiv(Ns);
+    Ns *= p.rad[i];
+  }
   return p;
 }
 
@@ -536,13 +569,13 @@ static int fft_c2c(rbc3d_ctx *c, double2 *data, int ax, int sign, int nf) {
   FftPlan pl = make_plan(c, ax, sign);
   LineMap lm;
   if (ax == 1) {  // lines over (kx, kz, f), stride H
-    lm.A = H;
+    lm.A = make_fastdiv(H);
     lm.sa = 1;
     lm.sb = (long long)H * N2;
     lm.es = H;
     lm.nlines = H * N3 * nf;
   } else {  // ax == 2: lines over (kx, ky, f), stride H*N2
-    lm.A = H * N2;
+    lm.A = make_fastdiv(H * N2);
     lm.sa = 1;
     lm.sb = (long long)H * N2 * N3;
     lm.es = (long long)H * N2;
@@ -550,7 +583,8 @@ static int fft_c2c(rbc3d_ctx *c, double2 *data, int ax, int sign, int nf) {
   }
   int TL = lines_per_block(pl.N);
   size_t lds = 2 * sizeof(double2) * TL * pl.N;
-  k_fft_c2c<<<(lm.nlines + TL - 1) / TL, FFT_T, lds, c->stream>>>(data, pl, lm, TL);
+  k_fft_c2c<<<(lm.nlines + TL - 1) / TL, FFT_T, lds, c->stream>>>(data, pl, lm, TL,
+                                                                    make_fastdiv(TL));
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;
 }
@@ -641,7 +675,7 @@ int rbc_pme_transform(rbc3d_ctx *c) {
     int TL = lines_per_block(N1), nrows = N2 * N3 * nf;
     size_t lds = 2 * sizeof(double2) * TL * N1;
     k_fft_r2c<<<(nrows + TL - 1) / TL, FFT_T, lds, st>>>(c->d_grid.p + ng * f0, C + nc * f0, px, nrows,
-                                                         TL);
+                                                         TL, make_fastdiv(N1), make_fastdiv(H));
     HIP_TRY(hipGetLastError());
     RBC_TRY(fft_c2c(c, C + nc * f0, 1, -1, nf));
     RBC_TRY(fft_c2c(c, C + nc * f0, 2, +1, nf));
@@ -669,7 +703,8 @@ int rbc_pme_transform(rbc3d_ctx *c) {
   FftPlan pxi = make_plan(c, 0, +1);
   int TL = lines_per_block(N1), nrows = N2 * N3 * 3;
   size_t lds = 2 * sizeof(double2) * TL * N1;
-  k_fft_c2r<<<(nrows + TL - 1) / TL, FFT_T, lds, st>>>(C + nc * 12, c->d_vgrid.p, pxi, nrows, TL);
+  k_fft_c2r<<<(nrows + TL - 1) / TL, FFT_T, lds, st>>>(C + nc * 12, c->d_vgrid.p, pxi, nrows, TL,
+                                                        make_fastdiv(N1));
   rbc_stage_end(c, ST_FFT);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;

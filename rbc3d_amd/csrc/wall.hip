@@ -379,26 +379,36 @@ __global__ void __launch_bounds__(128) k_wall_reduce(int nvert, const int *__res
 }
 
 // v_i += c1 (L f)_i / Acoef_i over wall vertices (SingIntOnWall :136-172)
+// v_i += c1/A_i * sum_k B_k f_(col k): one wavefront per vertex row; the row's 3x3
+// blocks are read as one contiguous run of doubles (lane l takes element l, l+64, ...
+// of the run), so every load instruction is coalesced; element e of block k belongs to
+// output component (e % 9) / 3 and multiplies f component e % 3 of vertex bcol[k].
 __global__ void __launch_bounds__(256) k_wall_spmv(int nvert, double c1, const int *__restrict__ rowptr,
                                                    const int *__restrict__ bcol,
                                                    const double *__restrict__ bval,
                                                    const double *__restrict__ f,
                                                    const double *__restrict__ tA, double *v) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
   if (i >= nvert) return;
-  double s0 = 0, s1 = 0, s2 = 0;
-  for (int k = rowptr[i]; k < rowptr[i + 1]; k++) {
-    int j = bcol[k];
-    const double *B = bval + 9 * (size_t)k;
-    double f0 = f[j], f1 = f[j + nvert], f2 = f[j + 2 * (size_t)nvert];
-    s0 += B[0] * f0 + B[1] * f1 + B[2] * f2;
-    s1 += B[3] * f0 + B[4] * f1 + B[5] * f2;
-    s2 += B[6] * f0 + B[7] * f1 + B[8] * f2;
+  const int e0 = 9 * rowptr[i], e1 = 9 * rowptr[i + 1];  // 9 nnz < 2^31 (checked at assembly)
+  double s[3] = {0, 0, 0};
+  for (int e = e0 + lane; e < e1; e += 64) {
+    const int k = e / 9, r = e - 9 * k;
+    const int j = bcol[k];
+    const double t = bval[e] * f[j + (size_t)nvert * (r % 3)];
+    const int row = r / 3;
+    s[0] += row == 0 ? t : 0.;
+    s[1] += row == 1 ? t : 0.;
+    s[2] += row == 2 ? t : 0.;
   }
-  double A = tA[i];
-  v[i] += c1 * s0 / A;
-  v[i + nvert] += c1 * s1 / A;
-  v[i + 2 * (size_t)nvert] += c1 * s2 / A;
+  for (int d = 0; d < 3; d++)
+    for (int o = 32; o > 0; o >>= 1) s[d] += __shfl_xor(s[d], o, 64);
+  if (lane == 0) {
+    const double A = tA[i];
+    v[i] += c1 * s[0] / A;
+    v[i + nvert] += c1 * s[1] / A;
+    v[i + 2 * (size_t)nvert] += c1 * s[2] / A;
+  }
 }
 
 static WallArgs wall_args(rbc3d_ctx *c) {
@@ -604,6 +614,10 @@ int rbc_wall_prepare(rbc3d_ctx *c) {
   int nb = 0;
   HIP_TRY(hipMemcpyAsync(&nb, w.d_rowptr.p + nv, sizeof(int), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
+  if (9LL * nb > 0x7fffffffLL) {
+    rbc_set_error("wall self-matrix too large");
+    return RBC3D_EINVAL;
+  }
   RBC_TRY(w.d_col.ensure(nb + 1));
   RBC_TRY(w.d_val.ensure(9 * (size_t)nb + 9));
   k_wall_reduce<<<(nv + 127) / 128, 128, 0, st>>>(nv, off.p, skey.p, sidx.p, eval.p, w.d_rowptr.p,
@@ -635,8 +649,8 @@ int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double c1, int nT, const double *d_tx
   if (wall_targets) {
     // wall vertices: the precomputed self-interaction; with one wall that is all
     if (!w.prepared) RBC_TRY(rbc_wall_prepare(c));
-    k_wall_spmv<<<(w.nvert + 255) / 256, 256, 0, st>>>(w.nvert, c1, w.d_rowptr.p, w.d_col.p, w.d_val.p,
-                                                       w.d_f.p, d_A, d_v);
+    k_wall_spmv<<<(unsigned)(((size_t)w.nvert * 64 + 255) / 256), 256, 0, st>>>(
+        w.nvert, c1, w.d_rowptr.p, w.d_col.p, w.d_val.p, w.d_f.p, d_A, d_v);
   } else {
     k_wall_targets<<<(nT + 127) / 128, 128, 0, st>>>(wall_args(c), c1, nT, d_tx, d_tindx, d_act, d_A,
                                                      d_v);
