@@ -487,6 +487,15 @@ extern "C" int rbc3d_create(const rbc3d_conf *conf, rbc3d_ctx **out) {
       }
     c->nfbins = c->Nf[0] * c->Nf[1] * c->Nf[2];
     c->noffs = offs.size();
+    // a source bin at offset o (|o| <= R) from the target bin holds sources at
+    // displacement o*w + delta, |delta| < w, plus the run's image m*Lb; when
+    // (R + 1) w stays below Lb/2 on every axis, NINT(x/Lb) = m for every such pair, so
+    // the kernel can subtract m*Lb instead of rounding per pair
+    c->run_shift = 1;
+    for (int d = 0; d < 3; d++) {
+      bool all = (lo[d] == 0 && hi[d] == c->Nf[d] - 1);
+      if (all || 2.0 * (R[d] + 1) * wmin[d] >= c->Lb[d] * (1 - 1e-9)) c->run_shift = 0;
+    }
     if (!r) r = upload(c->d_offs, offs, c->stream);
   }
   if (!r) r = hipStreamSynchronize(c->stream) == hipSuccess ? 0 : RBC3D_EHIP;

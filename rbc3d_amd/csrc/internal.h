@@ -151,6 +151,7 @@ struct rbc3d_ctx {
   double iLbNf[3];
   int nfbins;
   size_t noffs = 0;
+  int run_shift = 0;  // pair kernel may take the periodic image from the bin run
   DBuf<int> d_offs, d_tmp_f, d_fbin_start;
   // cells
   int nrbc = 0, np = 0;
@@ -237,14 +238,53 @@ __device__ __forceinline__ int imod_d(int a, int n) {
 }
 
 // EwaldCoeff_SL / EwaldCoeff_DL (common/ModEwaldFunc.F90:86-178) by table interpolation.
-__device__ __forceinline__ void ewald_sl(const double *__restrict__ t1, const double *__restrict__ t2,
-                                         double r, double reps, double rc, double &A, double &B) {
-  if (r < reps) {
-    A = 0;
-    B = 0;
-    return;
-  }
-  double s = RBC_NTAB * r / rc;
+// XCD-aware block order: workgroups are dealt to the 8 XCDs round-robin
+// (blockIdx % 8), so logical block (blockIdx % 8) * per + blockIdx / 8 gives each XCD
+// one contiguous range of the logical order and keeps that range's data in one L2.
+// Launch 8 * per blocks, per = ceil(nblk / 8); returns -1 for the padding blocks.
+#define RBC_NXCD 8
+__device__ __forceinline__ int xcd_block(int nblk) {
+  const int per = (nblk + RBC_NXCD - 1) / RBC_NXCD;
+  const int b = (int)(blockIdx.x % RBC_NXCD) * per + (int)(blockIdx.x / RBC_NXCD);
+  return b < nblk ? b : -1;
+}
+inline unsigned xcd_grid(size_t nblk) {
+  return (unsigned)(RBC_NXCD * ((nblk + RBC_NXCD - 1) / RBC_NXCD));
+}
+
+// 1/x from the hardware reciprocal estimate and two Newton steps (x normal, finite)
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+
+// 1/sqrt(x) from the hardware estimate and two Newton steps (x > 0 normal)
+__device__ __forceinline__ double rsq_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double e = fma(-x * y, y, 1.0);
+  y = fma(0.5 * y, e, y);
+  e = fma(-x * y, y, 1.0);
+  return fma(0.5 * y, e, y);
+}
+
+// Ewald coefficients at r with 1/r supplied (r >= reps already established)
+__device__ __forceinline__ double ewald_dl_ir(const double *__restrict__ t, double r, double ir,
+                                              double tabscale) {
+  double s = r * tabscale;
+  double fi = floor(s);
+  if (fi >= RBC_NTAB) return 0.;
+  int i = (int)fi;
+  double c1 = t[i] * (i + 1 - s) + t[i + 1] * (s - i);
+  double ir2 = ir * ir;
+  return c1 * (ir2 * ir2 * ir);
+}
+__device__ __forceinline__ void ewald_sl_ir(const double *__restrict__ t1,
+                                            const double *__restrict__ t2, double r, double ir,
+                                            double tabscale, double &A, double &B) {
+  double s = r * tabscale;
   double fi = floor(s);
   if (fi >= RBC_NTAB) {
     A = 0;
@@ -254,7 +294,32 @@ __device__ __forceinline__ void ewald_sl(const double *__restrict__ t1, const do
   int i = (int)fi;
   double c1 = t1[i] * (i + 1 - s) + t1[i + 1] * (s - i);
   double c2 = t2[i] * (i + 1 - s) + t2[i + 1] * (s - i);
-  double ir = 1. / r, ir2 = ir * ir;
+  double ir2 = ir * ir;
+  A = c1 * ir * ir2 + c2 * ir2;
+  B = c1 * ir - c2;
+}
+
+// The table position is r * (N / rc): the factor is loop-invariant for the callers'
+// loops, and the interpolated value is continuous in it, so this differs from the
+// reference's N * r / rc only at the level of rounding.
+__device__ __forceinline__ void ewald_sl(const double *__restrict__ t1, const double *__restrict__ t2,
+                                         double r, double reps, double rc, double &A, double &B) {
+  if (r < reps) {
+    A = 0;
+    B = 0;
+    return;
+  }
+  double s = r * (RBC_NTAB / rc);
+  double fi = floor(s);
+  if (fi >= RBC_NTAB) {
+    A = 0;
+    B = 0;
+    return;
+  }
+  int i = (int)fi;
+  double c1 = t1[i] * (i + 1 - s) + t1[i + 1] * (s - i);
+  double c2 = t2[i] * (i + 1 - s) + t2[i + 1] * (s - i);
+  double ir = rcp_nr(r), ir2 = ir * ir;
   A = c1 * ir * ir2 + c2 * ir2;
   B = c1 * ir - c2;
 }
@@ -262,13 +327,13 @@ __device__ __forceinline__ void ewald_sl(const double *__restrict__ t1, const do
 __device__ __forceinline__ double ewald_dl(const double *__restrict__ t, double r, double reps,
                                            double rc) {
   if (r < reps) return 0.;
-  double s = RBC_NTAB * r / rc;
+  double s = r * (RBC_NTAB / rc);
   double fi = floor(s);
   if (fi >= RBC_NTAB) return 0.;
   int i = (int)fi;
   double c1 = t[i] * (i + 1 - s) + t[i + 1] * (s - i);
-  double r2 = r * r;
-  return c1 / (r2 * r2 * r);
+  double ir = rcp_nr(r), ir2 = ir * ir;
+  return c1 * (ir2 * ir2 * ir);
 }
 
 // MaskFunc (common/ModBasicMath.F90:351-379)

@@ -209,8 +209,8 @@ extern "C" int rbc3d_hashtable_build(const int Nc[3], const double iLbNc[3], int
 __global__ void k_src_records(int np, const int *__restrict__ order, const double *__restrict__ x,
                               const double *__restrict__ fs, const double *__restrict__ gs,
                               const double *__restrict__ a3, const CellRec *__restrict__ cells,
-                              const int *__restrict__ node_cell, double *__restrict__ rec,
-                              int *__restrict__ rec_meta) {
+                              const int *__restrict__ node_cell, const int *__restrict__ node_ll,
+                              double *__restrict__ rec, int *__restrict__ rec_meta) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= np) return;
   int p = order[k];
@@ -221,7 +221,10 @@ __global__ void k_src_records(int np, const int *__restrict__ order, const doubl
     rec[k + (size_t)np * (9 + d)] = a3[p + (size_t)np * d];
   }
   rec[k + (size_t)np * 12] = cells[node_cell[p]].Bcoef;
+  // sorted-position metadata [3][np]: node, cell, (ilat | ilon << 16)
   rec_meta[k] = p;
+  rec_meta[k + (size_t)np] = node_cell[p];
+  rec_meta[k + 2 * (size_t)np] = node_ll[p];
 }
 
 // slist%f = f dS, slist%g = g dS (SourceList_UpdateDensity, common/ModSourceList.F90:160-187)
@@ -338,10 +341,11 @@ int rbc_update_density(rbc3d_ctx *c, int which) {
                                                 which == 1 ? c->d_fs.p : c->d_gs.p);
     HIP_TRY(hipGetLastError());
   }
-  RBC_TRY(c->d_work.ensure(np + 1));
+  RBC_TRY(c->d_work.ensure(3 * (size_t)np + 1));
   k_src_records<<<(np + 255) / 256, 256, 0, st>>>(np, c->d_src_order.p, c->d_x.p, c->d_fs.p,
                                                   c->d_gs.p, c->d_a3.p, c->d_cells.p,
-                                                  c->d_node_cell.p, c->d_src_rec.p, c->d_work.p);
+                                                  c->d_node_cell.p, c->d_node_ll.p, c->d_src_rec.p,
+                                                  c->d_work.p);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;
 }
@@ -364,11 +368,12 @@ int rbc_update_density(rbc3d_ctx *c, int which) {
 struct PairArgs {
   double Lb[3], iLb[3];
   int Nf[3];
-  double rc, rc2hi, reps, c1, c2;
+  double rc, rc2hi, rc2lo, reps2, tabscale, c1, c2;
   int np, nT, nfbins, nrbc, nrows;
+  int run_shift;  // 1: each bin run has one periodic image on every axis (see k_pair)
   const int *rows;       // x-rows of source bins: packed (alo, ahi, o2, o3)
   const double *rec;     // sources sorted by fine bin (SREC x np)
-  const int *src_node;   // sorted position -> node index
+  const int *src_node;   // sorted position -> node, then cell, then (ilat | ilon << 16)
   const int *bin_start;  // sources per fine bin
   const int *node_cell, *node_ll;
   const CellRec *cells;
@@ -446,9 +451,15 @@ __global__ void __launch_bounds__(PT *PW) k_pair(PairArgs a) {
     const unsigned code = (unsigned)a.rows[ri];
     const int alo = (int)(code & 255) - 128, ahi = (int)((code >> 8) & 255) - 128;
     const int o2 = (int)((code >> 16) & 255) - 128, o3 = (int)(code >> 24) - 128;
-    const int rowbase = Nf0 * (imod_d(b2 + o2, Nf1) + Nf1 * imod_d(b3 + o3, a.Nf[2]));
+    const int y2 = b2 + o2, y3 = b3 + o3;
+    const int rowbase = Nf0 * (imod_d(y2, Nf1) + Nf1 * imod_d(y3, a.Nf[2]));
+    // periodic image of the row on y and z: the reference's NINT(x/Lb) for every pair
+    // of this row that can lie within rc (see run_shift)
+    const double sh1 = (y2 < 0 ? 1 : (y2 >= Nf1 ? -1 : 0)) * L1;
+    const double sh2 = (y3 < 0 ? 1 : (y3 >= a.Nf[2] ? -1 : 0)) * L2;
     // the row's x-bins as one or two runs (a range shorter than the row wraps at most once)
     int rlo[2], rhi[2], nrun = 1;
+    double sh0r[2] = {0., 0.};
     if (ahi - alo + 1 >= Nf0) {
       rlo[0] = 0;
       rhi[0] = Nf0 - 1;
@@ -456,13 +467,16 @@ __global__ void __launch_bounds__(PT *PW) k_pair(PairArgs a) {
       const int x0 = b1 + alo, x1 = b1 + ahi;
       if (x0 < 0) {
         rlo[0] = x0 + Nf0, rhi[0] = Nf0 - 1, rlo[1] = 0, rhi[1] = x1, nrun = 2;
+        sh0r[0] = L0;
       } else if (x1 >= Nf0) {
         rlo[0] = x0, rhi[0] = Nf0 - 1, rlo[1] = 0, rhi[1] = x1 - Nf0, nrun = 2;
+        sh0r[1] = -L0;
       } else {
         rlo[0] = x0, rhi[0] = x1;
       }
     }
     for (int ru = 0; ru < nrun; ru++) {
+      const double sh0 = sh0r[ru];
       const int s0 = a.bin_start[rowbase + rlo[ru]], s1 = a.bin_start[rowbase + rhi[ru] + 1];
       for (int base = s0; base < s1; base += PS) {
         const int cnt = min(PS, s1 - base);
@@ -481,31 +495,50 @@ __global__ void __launch_bounds__(PT *PW) k_pair(PairArgs a) {
               sp[PD + 3 + d][t] = a.rec[q + np * (9 + d)];
             }
           }
-          const int node = a.src_node[q];
-          snode[t] = node;
-          scell[t] = a.node_cell[node];
-          sll[t] = a.node_ll[node];
+          snode[t] = a.src_node[q];
+          scell[t] = a.src_node[q + np];
+          sll[t] = a.src_node[q + 2 * np];
         }
         __syncthreads();
         if (!act) continue;
         for (int j = wv; j < cnt; j += PW) {
           // reference distance without FMA contraction (common/ModIntOnRbcs.F90:68-74);
-          // rint differs from NINT only at |x/Lb| = 1/2, where |x| = Lb/2 > rc
+          // rint differs from NINT only at |x/Lb| = 1/2, where |x| = Lb/2 > rc.  With
+          // run_shift the image multiple is the run's, which equals rint(x/Lb) for every
+          // pair that can be within rc, so the same d - m*Lb is formed.
           double d0, d1, d2, s2;
           {
 #pragma clang fp contract(off)
             d0 = sx[j] - xi0;
             d1 = sy[j] - xi1;
             d2 = sz[j] - xi2;
-            d0 = d0 - rint(d0 * iL0) * L0;
-            d1 = d1 - rint(d1 * iL1) * L1;
-            d2 = d2 - rint(d2 * iL2) * L2;
+            if (a.run_shift) {
+              d0 = d0 - sh0;
+              d1 = d1 - sh1;
+              d2 = d2 - sh2;
+            } else {
+              d0 = d0 - rint(d0 * iL0) * L0;
+              d1 = d1 - rint(d1 * iL1) * L1;
+              d2 = d2 - rint(d2 * iL2) * L2;
+            }
             s2 = d0 * d0 + d1 * d1;
             s2 = s2 + d2 * d2;
           }
           if (s2 > a.rc2hi) continue;
-          const double rr = sqrt(s2);
-          if (rr > a.rc) continue;
+          // sqrt(s2) <= rc is certain below rc^2 (1 - 1e-12); in the thin shell above, the
+          // correctly rounded square root decides, as in the reference
+          double rr, ir;
+          if (s2 >= a.rc2lo) {
+            rr = sqrt(s2);
+            if (rr > a.rc) continue;
+            ir = rcp_nr(rr);
+          } else if (s2 >= a.reps2) {
+            ir = rsq_nr(s2);
+            rr = s2 * ir;
+          } else {
+            rr = 0.;  // below r_eps the Ewald coefficients are zero
+            ir = 0.;
+          }
           const int node = snode[j];
           double wm = 1.;
           if (scell[j] == cell_i) {
@@ -517,9 +550,9 @@ __global__ void __launch_bounds__(PT *PW) k_pair(PairArgs a) {
           }
           npairs++;
           csum += (unsigned long long)(ti + 1) * (unsigned long long)(node + 1);
-          if (SL) {
+          if (SL && ir != 0.) {
             double EA, EB;
-            ewald_sl(a.sl1, a.sl2, rr, a.reps, a.rc, EA, EB);
+            ewald_sl_ir(a.sl1, a.sl2, rr, ir, a.tabscale, EA, EB);
             const double f0 = sp[0][j], f1 = sp[1][j], f2 = sp[2][j];
             const double xf = d0 * f0 + d1 * f1 + d2 * f2;
             const double cc = wm * cA1;
@@ -527,8 +560,8 @@ __global__ void __launch_bounds__(PT *PW) k_pair(PairArgs a) {
             acc1 += cc * (EA * d1 * xf + EB * f1);
             acc2 += cc * (EA * d2 * xf + EB * f2);
           }
-          if (DL) {
-            const double EA = ewald_dl(a.dl1, rr, a.reps, a.rc);
+          if (DL && ir != 0.) {
+            const double EA = ewald_dl_ir(a.dl1, rr, ir, a.tabscale);
             const double xg = d0 * sp[PD][j] + d1 * sp[PD + 1][j] + d2 * sp[PD + 2][j];
             const double xn = d0 * sp[PD + 3][j] + d1 * sp[PD + 4][j] + d2 * sp[PD + 5][j];
             const double cc = wm * cA2 * EA * xg * xn;
@@ -599,7 +632,7 @@ __device__ __forceinline__ bool nbr_better(double r1, int k1, int n1, double r2,
 
 __global__ void __launch_bounds__(NBR_T) k_nbr(NbrArgs a) {
   const int lane = threadIdx.x & 63;
-  const int ti = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int ti = (blockIdx.x * NBR_T + (int)threadIdx.x) >> 6;
   if (ti >= a.nT) return;
   if (!a.tact[ti]) {
     if (lane == 0) a.cnt[ti] = 0;
@@ -733,50 +766,46 @@ __device__ void patch_map(double th0, double phi0, double dth, double dphi, doub
   if (phi < 0) phi += RBC_TWO_PI;
 }
 
-// GauLeg (common/ModQuadRule.F90:135-208), device copy (n <= 32)
-__device__ void gauleg_d(double x1, double x2, int n, double *x, double *w) {
-  int m = (n + 1) / 2;
+// GauLeg (common/ModQuadRule.F90:135-208), node k (ascending) and its weight: each root's Newton iteration is
+// independent of the others, so one lane computes one node, as GauLeg does for it.
+__device__ void gauleg_node(double x1, double x2, int n, int k, double &x, double &w) {
+  const int m = (n + 1) / 2;
+  const int j = k < m ? k : n - 1 - k;
   double xm = 0.5 * (x2 + x1), xl = 0.5 * (x2 - x1);
-  for (int j = 0; j < m; j++) {
-    double z = cos(RBC_PI * ((j + 1) - 0.25) / (n + 0.5)), z1, p1 = 0, p2, p3, pp = 0;
-    bool unfinished = true;
-    for (int its = 1; its <= 10 && unfinished; its++) {
-      p1 = 1.0;
-      p2 = 0.0;
-      for (int jj = 1; jj <= n; jj++) {
-        p3 = p2;
-        p2 = p1;
-        p1 = ((2.0 * jj - 1.0) * z * p2 - (jj - 1.0) * p3) / jj;
-      }
-      pp = n * (z * p1 - p2) / (z * z - 1.0);
-      z1 = z;
-      z = z1 - p1 / pp;
-      unfinished = fabs(z - z1) > 3.e-14;
+  double z = cos(RBC_PI * ((j + 1) - 0.25) / (n + 0.5)), z1, p1 = 0, p2, p3, pp = 0;
+  bool unfinished = true;
+  for (int its = 1; its <= 10 && unfinished; its++) {
+    p1 = 1.0;
+    p2 = 0.0;
+    for (int jj = 1; jj <= n; jj++) {
+      p3 = p2;
+      p2 = p1;
+      p1 = ((2.0 * jj - 1.0) * z * p2 - (jj - 1.0) * p3) / jj;
     }
-    x[j] = xm - xl * z;
-    x[n - 1 - j] = xm + xl * z;
-    w[j] = 2.0 * xl / ((1.0 - z * z) * pp * pp);
-    w[n - 1 - j] = w[j];
+    pp = n * (z * p1 - p2) / (z * z - 1.0);
+    z1 = z;
+    z = z1 - p1 / pp;
+    unfinished = fabs(z - z1) > 3.e-14;
   }
+  // GauLeg writes x(j) then x(n-1-j); for the middle node of odd n the second wins
+  x = (k == n - 1 - j) ? xm + xl * z : xm - xl * z;
+  w = 2.0 * xl / ((1.0 - z * z) * pp * pp);
 }
 
-// GauLeg_Sinh (common/ModQuadRule.F90:223-263)
-__device__ void gauleg_sinh_d(double xmin, double xmax, double a, double b, int n, double *x, double *w) {
+// node k of GauLeg_Sinh (common/ModQuadRule.F90:223-263)
+__device__ void gauleg_sinh_node(double xmin, double xmax, double a, double b, int n, int k,
+                                 double &x, double &w) {
   double xm = 0.5 * (xmin + xmax), xl = 0.5 * (xmax - xmin);
   double a0 = (a - xm) / xl, b0 = b * xl;
   double f1 = (1 + a0) / b0, f2 = (1 - a0) / b0;
   double u1 = log(f1 + sqrt(1 + f1 * f1)), u2 = log(f2 + sqrt(1 + f2 * f2));
   double mu = 0.5 * (u1 + u2), eta = 0.5 * (u1 - u2);
-  double s[16];
-  gauleg_d(-1., 1., n, s, w);
-  for (int i = 0; i < n; i++) {
-    x[i] = a0 + b0 * sinh(mu * s[i] - eta);
-    w[i] = w[i] * b0 * mu * cosh(mu * s[i] - eta);
-  }
-  for (int i = 0; i < n; i++) {
-    w[i] = xl * w[i];
-    x[i] = xm + xl * x[i];
-  }
+  double sk, wk;
+  gauleg_node(-1., 1., n, k, sk, wk);
+  double xk = a0 + b0 * sinh(mu * sk - eta);
+  wk = wk * b0 * mu * cosh(mu * sk - eta);
+  w = xl * wk;
+  x = xm + xl * xk;
 }
 
 struct SingArgs {
@@ -803,7 +832,7 @@ struct SingArgs {
 
 // RBC_SingInt (common/ModRbcSingInt.F90:29-90): one wavefront per target on a cell
 __global__ void __launch_bounds__(256) k_self(SingArgs a) {
-  int wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
+  int wid = (blockIdx.x * 256 + (int)threadIdx.x) >> 6, lane = threadIdx.x & 63;
   if (wid >= a.nT) return;
   int ti = wid;
   int surf = a.tindx[ti];
@@ -873,9 +902,8 @@ __global__ void __launch_bounds__(256) k_self(SingArgs a) {
 // fit uses the per-grid-type least-squares weights (c = sum_k W_k d2_k).
 // item record: xi(3) x0(3) th0 phi0 a30(3) dist
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_proj(SingArgs a, int nitems) {
-  const int nblk = (nitems + 255) / 256, per = (nblk + 7) / 8;
-  const int lblk = (blockIdx.x & 7) * per + (blockIdx.x >> 3);
-  if (lblk >= nblk) return;
+  const int lblk = xcd_block((nitems + 255) / 256);
+  if (lblk < 0) return;
   const int gidx = lblk * 256 + threadIdx.x;
   if (gidx >= nitems) return;
   const int itm = a.item_order[gidx];
@@ -991,26 +1019,28 @@ __device__ void readd_wave(const SingArgs &a, const CellRec &cr, const double *b
                            const double xi[3], const double x0[3], double th0, double phi0,
                            double radPat, double c1, double c2, double dv[3]) {
   const int nrad = 16, nazm = 32;
-  double thP[16], wP[16];
+  // lane l always takes radial node l % 16 (64 is a multiple of nrad): its node and
+  // masked weight are computed once
+  const int lane = threadIdx.x & 63, irad = lane & (nrad - 1);
   double d0 = xi[0] - x0[0], d1 = xi[1] - x0[1], d2 = xi[2] - x0[2];
   double dist = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
   double sizePat = radPat * sqrt(cr.area / (4 * RBC_PI));
+  double thl, wl;
   if (dist > 2.2250738585072014e-308) {
     dist = dist * (radPat / sizePat);
-    gauleg_sinh_d(0., radPat, 0., dist, nrad, thP, wP);
+    gauleg_sinh_node(0., radPat, 0., dist, nrad, irad, thl, wl);
   } else {
-    gauleg_d(0., radPat, nrad, thP, wP);
+    gauleg_node(0., radPat, nrad, irad, thl, wl);
   }
-  int lane = threadIdx.x & 63;
+  double w = wl * sin(thl) * (RBC_TWO_PI / nazm);
+  w = w * mask_fn(a.mtab, thl / radPat);
   size_t vs = (size_t)m * n * 4;
   double s0 = sin(th0), c0 = cos(th0);
   double a0 = 0, a1 = 0, a2 = 0;
   for (int p = lane; p < nrad * nazm; p += 64) {
-    int irad = p % nrad, iazm = p / nrad;
-    double w = wP[irad] * sin(thP[irad]) * (RBC_TWO_PI / nazm);
-    w = w * mask_fn(a.mtab, thP[irad] / radPat);
+    int iazm = p / nrad;
     double phL = iazm * RBC_TWO_PI / nazm, thj, phj;
-    patch_point(s0, c0, phi0, thP[irad], phL, thj, phj);
+    patch_point(s0, c0, phi0, thl, phL, thj, phj);
     double xj[3], xx[3];
     spline_interp(blk + vs * SPL_X, m, n, thj, phj, 3, xj);
     for (int d = 0; d < 3; d++) xx[d] = xj[d] - xi[d];
@@ -1047,7 +1077,7 @@ __device__ void readd_wave(const SingArgs &a, const CellRec &cr, const double *b
 // Near-singular stage 2: one wavefront per item that passed the distance test.
 // RBC_NearSingInt (:103-166) = subtract (:175-226) + re-add, or the interpolation
 // between the re-adds at xi1 and on the surface (with the double-layer jump).
-__global__ void __launch_bounds__(256) k_near(SingArgs a, int nwork) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k_near(SingArgs a, int nwork) {
   int wid = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
   if (wid >= nwork) return;
   int it = a.work[2 * wid], cj = a.work[2 * wid + 1];
@@ -1189,14 +1219,15 @@ struct CombineArgs {
 __global__ void k_combine(CombineArgs a) {
   int ti = blockIdx.x * blockDim.x + threadIdx.x;
   if (ti >= a.nT || !a.tact[ti]) return;
-  double A = a.tA[ti];
+  const double A = a.tA[ti];
+  double s[3];
+  for (int d = 0; d < 3; d++) s[d] = a.vpair[ti + (size_t)a.nT * d] + a.dvself[ti + (size_t)a.nT * d] / A;
+  const int off = a.item_off[ti], n = a.cnt[ti];
+  for (int p = 0; p < n; p++)
+    for (int d = 0; d < 3; d++) s[d] += a.dvitem[3 * (size_t)(off + p) + d] / A;
   for (int d = 0; d < 3; d++) {
-    double s = a.vpair[ti + (size_t)a.nT * d];
-    s += a.dvself[ti + (size_t)a.nT * d] / A;
-    int off = a.item_off[ti];
-    for (int p = 0; p < a.cnt[ti]; p++) s += a.dvitem[3 * (size_t)(off + p) + d] / A;
-    if (a.has_linear) s += a.c2 * a.xv[d] / A;
-    a.v[ti + (size_t)a.nT * d] += s;
+    if (a.has_linear) s[d] += a.c2 * a.xv[d] / A;
+    a.v[ti + (size_t)a.nT * d] += s[d];
   }
 }
 
@@ -1282,9 +1313,12 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   }
   pa.rc = c->conf.rc;
   pa.rc2hi = c->conf.rc * c->conf.rc * (1.0 + 1e-12);
+  pa.rc2lo = c->conf.rc * c->conf.rc * (1.0 - 1e-12);
+  pa.reps2 = c->r_eps * c->r_eps;
+  pa.tabscale = RBC_NTAB / c->conf.rc;
+  pa.run_shift = c->run_shift;
   pa.nrows = (int)c->noffs;
   pa.rows = c->d_offs.p;
-  pa.reps = c->r_eps;
   pa.c1 = c1;
   pa.c2 = c2;
   pa.np = c->np;
@@ -1445,8 +1479,7 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
     sa.item_order = v_out;
     sa.item_slot = c->d_islot.p;
     sa.proj = tp->proj;
-    const int nblk = (nitems + 255) / 256, per = (nblk + 7) / 8;
-    k_proj<<<8 * per, 256, 0, st>>>(sa, nitems);
+    k_proj<<<xcd_grid(((size_t)nitems + 255) / 256), 256, 0, st>>>(sa, nitems);
     rbc_stage_end(c, ST_PROJ);
     HIP_TRY(hipGetLastError());
     int nwork = 0;

@@ -404,54 +404,71 @@ __device__ __forceinline__ double2 twid(const double2 *tw, int m, int sign) {  /
 
 // FFT of TL lines of length N stored line-major in buf (line l at l*N); uses tmp.
 // Returns pointer to the buffer holding the result.
+// One Stockham pass of radix R (compile-time, so the butterfly lives in registers).
+template <int R>
+__device__ __forceinline__ void fft_pass(const double2 *in, double2 *out, int TL, int N, int Ns,
+                                         const FastDiv &fNR, const FastDiv &fNs, const FftPlan &pl) {
+  const int NR = fNR.d;
+  const int tws = NR / Ns;  // twiddle stride N / (Ns R)
+  for (int t = threadIdx.x; t < TL * NR; t += blockDim.x) {
+    const int l = fd_div(t, fNR), j = t - l * NR;
+    const double2 *src = in + (size_t)l * N;
+    double2 *dst = out + (size_t)l * N;
+    const int jq = fd_div(j, fNs), k = j - jq * Ns;
+    double2 v[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      double2 x = src[j + r * NR];
+      if (r && k) x = cmul(x, twid(pl.tw, r * k * tws, pl.sign));
+      v[r] = x;
+    }
+    const int base = jq * Ns * R + k;
+    if (R == 2) {
+      dst[base] = make_double2(v[0].x + v[1].x, v[0].y + v[1].y);
+      dst[base + Ns] = make_double2(v[0].x - v[1].x, v[0].y - v[1].y);
+    } else if (R == 4) {
+      double2 a0 = make_double2(v[0].x + v[2].x, v[0].y + v[2].y);
+      double2 a1 = make_double2(v[0].x - v[2].x, v[0].y - v[2].y);
+      double2 b0 = make_double2(v[1].x + v[3].x, v[1].y + v[3].y);
+      double2 b1 = make_double2(v[1].x - v[3].x, v[1].y - v[3].y);
+      // multiply b1 by (sign * i)
+      double2 jb1 = pl.sign < 0 ? make_double2(b1.y, -b1.x) : make_double2(-b1.y, b1.x);
+      dst[base] = make_double2(a0.x + b0.x, a0.y + b0.y);
+      dst[base + Ns] = make_double2(a1.x + jb1.x, a1.y + jb1.y);
+      dst[base + 2 * Ns] = make_double2(a0.x - b0.x, a0.y - b0.y);
+      dst[base + 3 * Ns] = make_double2(a1.x - jb1.x, a1.y - jb1.y);
+    } else {
+#pragma unroll
+      for (int q = 0; q < R; q++) {
+        double2 sum = v[0];
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+          double2 w = twid(pl.tw, ((q * r) % R) * NR, pl.sign);
+          double2 p = cmul(v[r], w);
+          sum.x += p.x;
+          sum.y += p.y;
+        }
+        dst[base + q * Ns] = sum;
+      }
+    }
+  }
+}
+
+// FFT of TL lines of length N stored line-major in buf (line l at l*N); uses tmp.
+// Returns pointer to the buffer holding the result.
 __device__ double2 *fft_lds(double2 *buf, double2 *tmp, int TL, const FftPlan &pl) {
   int N = pl.N, Ns = 1;
   double2 *in = buf, *out = tmp;
   for (int s = 0; s < pl.nrad; s++) {
-    const int R = pl.rad[s], NR = pl.fNR[s].d;
-    const int tws = NR / Ns;  // twiddle stride N / (Ns R)
-    for (int t = threadIdx.x; t < TL * NR; t += blockDim.x) {
-      const int l = fd_div(t, pl.fNR[s]), j = t - l * NR;
-      const double2 *src = in + (size_t)l * N;
-      double2 *dst = out + (size_t)l * N;
-      const int jq = fd_div(j, pl.fNs[s]), k = j - jq * Ns;
-      double2 v[5];
-      for (int r = 0; r < R; r++) {
-        double2 x = src[j + r * NR];
-        if (r && k) x = cmul(x, twid(pl.tw, r * k * tws, pl.sign));
-        v[r] = x;
-      }
-      int base = jq * Ns * R + k;
-      if (R == 2) {
-        dst[base] = make_double2(v[0].x + v[1].x, v[0].y + v[1].y);
-        dst[base + Ns] = make_double2(v[0].x - v[1].x, v[0].y - v[1].y);
-      } else if (R == 4) {
-        double2 a0 = make_double2(v[0].x + v[2].x, v[0].y + v[2].y);
-        double2 a1 = make_double2(v[0].x - v[2].x, v[0].y - v[2].y);
-        double2 b0 = make_double2(v[1].x + v[3].x, v[1].y + v[3].y);
-        double2 b1 = make_double2(v[1].x - v[3].x, v[1].y - v[3].y);
-        // multiply b1 by (sign * i)
-        double2 jb1 = pl.sign < 0 ? make_double2(b1.y, -b1.x) : make_double2(-b1.y, b1.x);
-        dst[base] = make_double2(a0.x + b0.x, a0.y + b0.y);
-        dst[base + Ns] = make_double2(a1.x + jb1.x, a1.y + jb1.y);
-        dst[base + 2 * Ns] = make_double2(a0.x - b0.x, a0.y - b0.y);
-        dst[base + 3 * Ns] = make_double2(a1.x - jb1.x, a1.y - jb1.y);
-      } else {
-        for (int q = 0; q < R; q++) {
-          double2 sum = v[0];
-          int qr = 0;  // (q r) mod R
-          for (int r = 1; r < R; r++) {
-            qr += q;
-            qr -= qr >= R ? R : 0;
-            double2 w = twid(pl.tw, qr * NR, pl.sign);
-            double2 p = cmul(v[r], w);
-            sum.x += p.x;
-            sum.y += p.y;
-          }
-          dst[base + q * Ns] = sum;
-        }
-      }
-    }
+    const int R = pl.rad[s];
+    if (R == 2)
+      fft_pass<2>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl);
+    else if (R == 3)
+      fft_pass<3>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl);
+    else if (R == 4)
+      fft_pass<4>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl);
+    else
+      fft_pass<5>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl);
     __syncthreads();
     double2 *t2 = in;
     in = out;
@@ -562,7 +579,9 @@ static FftPlan make_plan(rbc3d_ctx *c, int ax, int sign) {
   return p;
 }
 
-static int lines_per_block(int N) { return std::max(1, FFT_ELEMS / N); }
+// ~1024 complex elements per workgroup: 32 KiB of LDS for the two buffers, so five
+// workgroups fit a CU
+static int lines_per_block(int N) { return std::max(1, 1024 / N); }
 
 static int fft_c2c(rbc3d_ctx *c, double2 *data, int ax, int sign, int nf) {
   int N1 = c->conf.Nb[0], N2 = c->conf.Nb[1], N3 = c->conf.Nb[2], H = N1 / 2 + 1;
@@ -720,12 +739,15 @@ struct GatherArgs {
   double *v;
 };
 
+
+// PC: the B-spline order when it is known at compile time (0: runtime a.P)
+template <int PC>
 __global__ void __launch_bounds__(256) k_gather(GatherArgs a) {
-  int ti = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ti = blockIdx.x * blockDim.x + threadIdx.x;
   if (ti >= a.nT || !a.tact[ti]) return;
   double x[3] = {a.tx[ti], a.tx[ti + a.nT], a.tx[ti + 2 * (size_t)a.nT]};
   double wx[12], wy[12], wz[12];
-  int P = a.P;
+  const int P = PC ? PC : a.P;
   int kmin = bspline_d(x[2] * a.ih[2], P, wz);
   int jmin = bspline_d(x[1] * a.ih[1], P, wy);
   int imin = bspline_d(x[0] * a.ih[0], P, wx);
@@ -753,6 +775,7 @@ __global__ void __launch_bounds__(256) k_gather(GatherArgs a) {
 
 int rbc_pme_gather(rbc3d_ctx *c, int nT, const double *d_tx, const int *d_act, const double *d_A,
                    double *d_v) {
+  if (nT <= 0) return RBC3D_OK;
   GatherArgs g;
   g.N1 = c->conf.Nb[0];
   g.N2 = c->conf.Nb[1];
@@ -766,7 +789,10 @@ int rbc_pme_gather(rbc3d_ctx *c, int nT, const double *d_tx, const int *d_act, c
   g.tact = d_act;
   g.v = d_v;
   rbc_stage_begin(c, ST_GATHER);
-  k_gather<<<(nT + 255) / 256, 256, 0, c->stream>>>(g);
+  if (g.P == 6)
+    k_gather<6><<<(nT + 255) / 256, 256, 0, c->stream>>>(g);
+  else
+    k_gather<0><<<(nT + 255) / 256, 256, 0, c->stream>>>(g);
   rbc_stage_end(c, ST_GATHER);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;
