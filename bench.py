@@ -27,6 +27,10 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per k_pair launch on C2 from two rocprofv3 counter runs (FETCH_SIZE,
+# WRITE_SIZE), scaled by the counters' reading for a plain 512 MiB copy (fetch x2.0,
+# write x1.0); commands and table in profiles/cell_matvec_C2_fp64_hbm_traffic.md
+K_PAIR_C2_TRAFFIC_BYTES = int((5306.2 + 12.6) * 1e6)
 
 
 def host_threads():
@@ -150,6 +154,8 @@ def main():
     ap.add_argument("--no-timestep", action="store_true",
                     help="skip the tube timestep (GMRES seconds per timestep)")
     ap.add_argument("--timestep-steps", type=int, default=3)
+    ap.add_argument("--timestep-only", action="store_true",
+                    help="only the tube timestep (prints its JSON object)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run one plain 512 MiB device copy first (a known-size kernel for "
                          "scaling rocprofv3 FETCH_SIZE/WRITE_SIZE counters)")
@@ -158,6 +164,10 @@ def main():
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported by this build "
               "(the multi-GPU sharding is a design note only)", file=sys.stderr)
         sys.exit(2)
+
+    if args.timestep_only:
+        print(json.dumps(timestep_bench(args.timestep_steps, args.warmup)))
+        return
 
     from rbc3d_amd import configs
     w = configs.make_workload(args.config, seed=1)
@@ -185,28 +195,36 @@ def main():
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    for _ in range(args.warmup):
-        sysg.cell_matmult_dev(u.data_ptr(), b.data_ptr())
-    sysg.sync()
+    def timed_matvecs(reuse, steps, warmup):
+        """K matvecs bracketed by HIP events on the library stream (and the host clock)."""
+        sysg.set_geometry_reuse(reuse)
+        for _ in range(warmup):
+            sysg.cell_matmult_dev(u.data_ptr(), b.data_ptr())
+        sysg.sync()
+        sysg.timing(-1)
+        sysg.timing(1)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        sysg.sync()
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(steps):
+            sysg.cell_matmult_dev(u.data_ptr(), b.data_ptr())
+        e1.record(stream)
+        sysg.sync()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        st = sysg.stage_times()
+        sysg.timing(0)
+        return max(e0.elapsed_time(e1), 1e3 * wall) / steps, st
 
-    sysg.timing(-1)
-    sysg.timing(1)
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    sysg.sync()
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(args.steps):
-        sysg.cell_matmult_dev(u.data_ptr(), b.data_ptr())
-    e1.record(stream)
-    sysg.sync()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
-    gpu_ms = e0.elapsed_time(e1)
-    stages = sysg.stage_times()
-    sysg.timing(0)
-    ms_per_step = max(gpu_ms, 1e3 * wall) / args.steps
+    # the headline: every matvec does all of its work, geometry included (as the
+    # reference's MyMatMult does)
+    ms_per_step, stages = timed_matvecs(False, args.steps, args.warmup)
+    # within one GMRES solve the geometry is fixed: neighbour lists and projections
+    # computed by the first matvec are reused by the others
+    ms_reuse, stages_reuse = timed_matvecs(True, args.steps, 1)
     npairs, _ = sysg.pair_stats()
     nproj, nnear = sysg.near_stats()
 
@@ -220,7 +238,9 @@ def main():
     roofline = {"kernel": "k_pair (double-layer real-space pair sums)", "bound": "mfma",
                 "compute": "fp64 VALU (same 78.6 TFLOP/s peak as fp64 matrix)",
                 "achieved": round(pair_tflops, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(pair_tflops / PEAK_FP64_TFLOPS, 4), "traffic": None,
+                "frac": round(pair_tflops / PEAK_FP64_TFLOPS, 4),
+                "traffic": K_PAIR_C2_TRAFFIC_BYTES if args.config == "C2" else None,
+                "traffic_unit": "bytes per launch (HBM read + write, rocprofv3 counters)",
                 "avg_launch_ms": round(pair_avg_ms, 4), "pairs_per_launch": int(npairs),
                 "flop_per_pair": 28}
 
@@ -245,6 +265,13 @@ def main():
         "dominant_stage": {"name": dom[0], "ms_per_step": round(dom[1][0] / args.steps, 4)},
         "stages_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in stages.items() if v[1]},
         "near_singular": {"projections": int(nproj), "corrections": int(nnear)},
+        "geometry_reused": {
+            "what": "matvecs 2..K of a GMRES solve at fixed geometry: neighbouring-cell "
+                    "lists, target binning and closest-point projections from the first",
+            "ms_per_step": round(ms_reuse, 4),
+            "cell_matvecs_per_s": round(ncell / (ms_reuse * 1e-3), 3),
+            "stages_ms_per_step": {k: round(v[0] / args.steps, 4)
+                                   for k, v in stages_reuse.items() if v[1]}},
     }
     if not args.no_timestep:
         out["timestep"] = timestep_bench(args.timestep_steps, 1)

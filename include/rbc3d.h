@@ -146,6 +146,14 @@ int rbc3d_get_wall_density(rbc3d_ctx *ctx, double *f);
 /* Compute_Rhs into a device vector (common/ModVelSolver.F90:455-515) */
 int rbc3d_cell_rhs_dev(rbc3d_ctx *ctx, double *d_rhs);
 
+/* Geometry reuse (no counterpart in the reference, which recomputes everything in
+ * each MyMatMult): the cell BIE matvec and right-hand side keep their geometry-only
+ * intermediate results -- target binning, neighbouring-cell lists (NbrRbcList), the
+ * closest-point projections of RBC_NearSingInt and its distance test -- until the
+ * cells are set again, so the matvecs of one GMRES solve compute them once.  Results
+ * are identical either way.  enable: 1 (default) or 0. */
+int rbc3d_set_geometry_reuse(rbc3d_ctx *ctx, int enable);
+
 /* --- the two GMRES solves (PETSc KSPGMRES semantics: restart 30, classical
  *     Gram-Schmidt, no preconditioner, x += V y at restarts) ------------------ */
 /* NoSlipWall (common/ModNoSlip.F90:45-150): residual wall velocity, GMRES for the

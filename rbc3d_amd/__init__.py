@@ -49,6 +49,7 @@ _SIG = {
     "rbc3d_add_int_on_walls": [C.c_void_p, C.c_double, C.c_int, dp, ip, ip, dp, dp],
     "rbc3d_pair_stats": [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_ulonglong)],
     "rbc3d_near_stats": [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
+    "rbc3d_set_geometry_reuse": [C.c_void_p, C.c_int],
     "rbc3d_pme_distrib_source": [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int],
     "rbc3d_pme_transform": [C.c_void_p],
     "rbc3d_pme_add_interp_vel": [C.c_void_p, C.c_int, dp, ip, dp, dp],
@@ -251,6 +252,9 @@ class System:
         s = C.c_ulonglong()
         _call("rbc3d_pair_stats", self.h, C.byref(n), C.byref(s))
         return n.value, s.value
+
+    def set_geometry_reuse(self, enable):
+        _call("rbc3d_set_geometry_reuse", self.h, int(bool(enable)))
 
     def near_stats(self):
         """(projections, near-singular corrections) of the last add_int_on_rbcs."""

@@ -438,6 +438,11 @@ extern "C" int rbc3d_create(const rbc3d_conf *conf, rbc3d_ctx **out) {
   if (!r) r = upload(c->d_mtab, mt, c->stream);
   // HashTable_ComputeNumBlocks (common/ModHashTable.F90:99-126), one rank
   for (int d = 0; d < 3; d++) c->Nc[d] = std::max((int)floor(c->Lb[d] / rc), 3);
+  if (c->Nc[0] > 1023 || c->Nc[1] > 1023 || c->Nc[2] > 1023) {
+    rbc_set_error("rbc3d_create: more than 1023 cell-list bins on an axis");
+    rbc3d_destroy(c);
+    return RBC3D_EINVAL;
+  }
   c->iLbNc[0] = c->iLb[0] * c->Nc[0];
   c->iLbNc[1] = c->iLb[1] * c->Nc[1];
   c->iLbNc[2] = c->Nc[2] / (c->Lb[2] - 0.);
@@ -487,15 +492,6 @@ extern "C" int rbc3d_create(const rbc3d_conf *conf, rbc3d_ctx **out) {
       }
     c->nfbins = c->Nf[0] * c->Nf[1] * c->Nf[2];
     c->noffs = offs.size();
-    // a source bin at offset o (|o| <= R) from the target bin holds sources at
-    // displacement o*w + delta, |delta| < w, plus the run's image m*Lb; when
-    // (R + 1) w stays below Lb/2 on every axis, NINT(x/Lb) = m for every such pair, so
-    // the kernel can subtract m*Lb instead of rounding per pair
-    c->run_shift = 1;
-    for (int d = 0; d < 3; d++) {
-      bool all = (lo[d] == 0 && hi[d] == c->Nf[d] - 1);
-      if (all || 2.0 * (R[d] + 1) * wmin[d] >= c->Lb[d] * (1 - 1e-9)) c->run_shift = 0;
-    }
     if (!r) r = upload(c->d_offs, offs, c->stream);
   }
   if (!r) r = hipStreamSynchronize(c->stream) == hipSuccess ? 0 : RBC3D_EHIP;
@@ -604,6 +600,7 @@ extern "C" int rbc3d_set_cells(rbc3d_ctx *c, int nrbc, const int *nlat0, const i
   }
   c->np = np;
   c->nchunks = choff;
+  c->geom_epoch++;
   c->spl_total = spl;
   hipStream_t st = c->stream;
   RBC_TRY(upload(c->d_cells, c->cells, st));
@@ -736,7 +733,7 @@ extern "C" int rbc3d_add_int_on_rbcs(rbc3d_ctx *c, double c1, double c2, int nT,
   if (c->nrbc == 0 || nT == 0) return RBC3D_OK;
   RBC_TRY(upload_targets(c, nT, tx, tindx, active, Acoef, v));
   RBC_TRY(rbc_add_int_on_rbcs_dev(c, c1, c2, nT, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
-                                  c->d_v.p));
+                                  c->d_v.p, false));
   return download_v(c, nT, v);
 }
 
@@ -744,6 +741,13 @@ extern "C" int rbc3d_pair_stats(rbc3d_ctx *c, long long *npairs, unsigned long l
   if (!c) return RBC3D_EINVAL;
   if (npairs) *npairs = c->last_npairs;
   if (checksum) *checksum = c->last_checksum;
+  return RBC3D_OK;
+}
+
+extern "C" int rbc3d_set_geometry_reuse(rbc3d_ctx *c, int enable) {
+  if (!c) return RBC3D_EINVAL;
+  c->geom_reuse = enable != 0;
+  c->cache_epoch = -1;
   return RBC3D_OK;
 }
 
@@ -936,7 +940,7 @@ extern "C" int rbc3d_cell_matmult_dev(rbc3d_ctx *c, const double *d_u, double *d
   HIP_TRY(hipMemsetAsync(c->d_v.p, 0, sizeof(double) * 3 * np, st));
   double c1 = 0., c2 = -1. / (4. * RBC_PI);
   RBC_TRY(rbc_add_int_on_rbcs_dev(c, c1, c2, np, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
-                                  c->d_v.p));
+                                  c->d_v.p, true));
   RBC_TRY(rbc_pme_init(c));
   RBC_TRY(rbc_pme_spread(c, c1, c2, 1, 0));
   RBC_TRY(rbc_pme_transform(c));
@@ -971,7 +975,7 @@ extern "C" int rbc3d_cell_rhs_dev(rbc3d_ctx *c, double *d_rhs) {
   HIP_TRY(hipMemsetAsync(c->d_v.p, 0, sizeof(double) * 3 * np, st));
   double c1 = 1. / (4. * RBC_PI), c2 = 0.;
   RBC_TRY(rbc_add_int_on_rbcs_dev(c, c1, c2, np, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
-                                  c->d_v.p));
+                                  c->d_v.p, true));
   if (c->wall.nvert)
     RBC_TRY(rbc_add_int_on_walls_dev(c, c1, np, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
                                      c->d_v.p, false));
@@ -1005,7 +1009,7 @@ extern "C" int rbc3d_calc_velocity_field(rbc3d_ctx *c, int nT, const double *tx,
   double c1 = 1.0 / (4. * RBC_PI), c2 = 1.0 / (4. * RBC_PI);
   if (c->nrbc)
     RBC_TRY(rbc_add_int_on_rbcs_dev(c, c1, c2, nT, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
-                                    c->d_v.p));
+                                    c->d_v.p, false));
   if (c->wall.nvert)
     RBC_TRY(rbc_add_int_on_walls_dev(c, c1, nT, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
                                      c->d_v.p, tindx[0] == c->wall.id));

@@ -151,7 +151,6 @@ struct rbc3d_ctx {
   double iLbNf[3];
   int nfbins;
   size_t noffs = 0;
-  int run_shift = 0;  // pair kernel may take the periodic image from the bin run
   DBuf<int> d_offs, d_tmp_f, d_fbin_start;
   // cells
   int nrbc = 0, np = 0;
@@ -178,6 +177,7 @@ struct rbc3d_ctx {
   // cell bounding spheres binned by centre (neighbour-cell candidates)
   DBuf<double> d_cell_sph;
   DBuf<double> d_chunk_sph;  // per 64-node chunk of every cell: centre(3), radius
+  DBuf<int> d_src_cc;        // coarse cell-list coordinates per node (packed)
   int nchunks = 0;
   DBuf<int> d_cellbin_start, d_cellbin_order;
   double max_cell_radius = 0;
@@ -206,6 +206,10 @@ struct rbc3d_ctx {
   bool cell_sol_init = false;
   // last pair statistics
   long long last_npairs = 0, last_nitems = 0, last_nwork = 0;
+  // geometry epoch (advanced by set_cells) and the epoch of the geometry-only results
+  // kept by the last cacheable add_int_on_rbcs (-1: none)
+  int geom_epoch = 0, cache_epoch = -1, cache_nT = 0, cache_nitems = 0, cache_nwork = 0;
+  bool geom_reuse = true;  // rbc3d_set_geometry_reuse
   DBuf<int> d_worklist, d_ikey, d_ival, d_islot;
   GmresWork gmres;
   DBuf<double> d_linpart;
@@ -421,7 +425,8 @@ int rbc_sh_synth_batch(int nlat, int nlon, int nt, double *d_g, int idg, int jdg
 int rbc_build_source_list(rbc3d_ctx *c);
 int rbc_update_density(rbc3d_ctx *c, int which);
 int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const double *d_tx,
-                            const int *d_tindx, const int *d_act, const double *d_A, double *d_v);
+                            const int *d_tindx, const int *d_act, const double *d_A, double *d_v,
+                            bool cacheable);
 int rbc_pme_init(rbc3d_ctx *c);
 int rbc_pme_spread(rbc3d_ctx *c, double c1, double c2, int cells, int walls);
 int rbc_pme_transform(rbc3d_ctx *c);

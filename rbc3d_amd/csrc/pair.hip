@@ -312,6 +312,16 @@ __global__ void k_chunk_sphere(int nrbc, const CellRec *__restrict__ cells,
   }
 }
 
+// coarse cell-list coordinates of every node, HashTable_Index (common/ModHashTable.F90:67-91)
+__global__ void k_coarse_coords(int np, const double *__restrict__ x, HashGeom hg, int *cc) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  double xp[3] = {x[p], x[p + (size_t)np], x[p + 2 * (size_t)np]};
+  int i1, i2, i3;
+  hash_index3(xp, hg.Nc, hg.iLbNc, i1, i2, i3);
+  cc[p] = (i1 - 1) | ((i2 - 1) << 10) | ((i3 - 1) << 20);
+}
+
 int rbc_build_source_list(rbc3d_ctx *c) {
   int np = c->np;
   RBC_TRY(c->d_src_bin.ensure(np + 1));
@@ -325,6 +335,16 @@ int rbc_build_source_list(rbc3d_ctx *c) {
   RBC_TRY(c->d_cell_sph.ensure((size_t)4 * c->nrbc + 4));
   k_cell_sphere<<<c->nrbc, 256, 0, c->stream>>>(c->nrbc, c->d_cells.p, c->d_x.p, np, c->d_cell_sph.p);
   HIP_TRY(hipGetLastError());
+  RBC_TRY(c->d_src_cc.ensure(np + 1));
+  {
+    HashGeom hg;
+    for (int d = 0; d < 3; d++) {
+      hg.Nc[d] = c->Nc[d];
+      hg.iLbNc[d] = c->iLbNc[d];
+    }
+    k_coarse_coords<<<(np + 255) / 256, 256, 0, c->stream>>>(np, c->d_x.p, hg, c->d_src_cc.p);
+    HIP_TRY(hipGetLastError());
+  }
   RBC_TRY(c->d_chunk_sph.ensure((size_t)4 * c->nchunks + 4));
   k_chunk_sphere<<<c->nrbc, 256, 0, c->stream>>>(c->nrbc, c->d_cells.p, c->d_x.p, np,
                                                  c->d_chunk_sph.p);
@@ -370,7 +390,6 @@ struct PairArgs {
   int Nf[3];
   double rc, rc2hi, rc2lo, reps2, tabscale, c1, c2;
   int np, nT, nfbins, nrbc, nrows;
-  int run_shift;  // 1: each bin run has one periodic image on every axis (see k_pair)
   const int *rows;       // x-rows of source bins: packed (alo, ahi, o2, o3)
   const double *rec;     // sources sorted by fine bin (SREC x np)
   const int *src_node;   // sorted position -> node, then cell, then (ilat | ilon << 16)
@@ -451,15 +470,9 @@ __global__ void __launch_bounds__(PT *PW) k_pair(PairArgs a) {
     const unsigned code = (unsigned)a.rows[ri];
     const int alo = (int)(code & 255) - 128, ahi = (int)((code >> 8) & 255) - 128;
     const int o2 = (int)((code >> 16) & 255) - 128, o3 = (int)(code >> 24) - 128;
-    const int y2 = b2 + o2, y3 = b3 + o3;
-    const int rowbase = Nf0 * (imod_d(y2, Nf1) + Nf1 * imod_d(y3, a.Nf[2]));
-    // periodic image of the row on y and z: the reference's NINT(x/Lb) for every pair
-    // of this row that can lie within rc (see run_shift)
-    const double sh1 = (y2 < 0 ? 1 : (y2 >= Nf1 ? -1 : 0)) * L1;
-    const double sh2 = (y3 < 0 ? 1 : (y3 >= a.Nf[2] ? -1 : 0)) * L2;
+    const int rowbase = Nf0 * (imod_d(b2 + o2, Nf1) + Nf1 * imod_d(b3 + o3, a.Nf[2]));
     // the row's x-bins as one or two runs (a range shorter than the row wraps at most once)
     int rlo[2], rhi[2], nrun = 1;
-    double sh0r[2] = {0., 0.};
     if (ahi - alo + 1 >= Nf0) {
       rlo[0] = 0;
       rhi[0] = Nf0 - 1;
@@ -467,16 +480,13 @@ __global__ void __launch_bounds__(PT *PW) k_pair(PairArgs a) {
       const int x0 = b1 + alo, x1 = b1 + ahi;
       if (x0 < 0) {
         rlo[0] = x0 + Nf0, rhi[0] = Nf0 - 1, rlo[1] = 0, rhi[1] = x1, nrun = 2;
-        sh0r[0] = L0;
       } else if (x1 >= Nf0) {
         rlo[0] = x0, rhi[0] = Nf0 - 1, rlo[1] = 0, rhi[1] = x1 - Nf0, nrun = 2;
-        sh0r[1] = -L0;
       } else {
         rlo[0] = x0, rhi[0] = x1;
       }
     }
     for (int ru = 0; ru < nrun; ru++) {
-      const double sh0 = sh0r[ru];
       const int s0 = a.bin_start[rowbase + rlo[ru]], s1 = a.bin_start[rowbase + rhi[ru] + 1];
       for (int base = s0; base < s1; base += PS) {
         const int cnt = min(PS, s1 - base);
@@ -503,24 +513,16 @@ __global__ void __launch_bounds__(PT *PW) k_pair(PairArgs a) {
         if (!act) continue;
         for (int j = wv; j < cnt; j += PW) {
           // reference distance without FMA contraction (common/ModIntOnRbcs.F90:68-74);
-          // rint differs from NINT only at |x/Lb| = 1/2, where |x| = Lb/2 > rc.  With
-          // run_shift the image multiple is the run's, which equals rint(x/Lb) for every
-          // pair that can be within rc, so the same d - m*Lb is formed.
+          // rint differs from NINT only at |x/Lb| = 1/2, where |x| = Lb/2 > rc
           double d0, d1, d2, s2;
           {
 #pragma clang fp contract(off)
             d0 = sx[j] - xi0;
             d1 = sy[j] - xi1;
             d2 = sz[j] - xi2;
-            if (a.run_shift) {
-              d0 = d0 - sh0;
-              d1 = d1 - sh1;
-              d2 = d2 - sh2;
-            } else {
-              d0 = d0 - rint(d0 * iL0) * L0;
-              d1 = d1 - rint(d1 * iL1) * L1;
-              d2 = d2 - rint(d2 * iL2) * L2;
-            }
+            d0 = d0 - rint(d0 * iL0) * L0;
+            d1 = d1 - rint(d1 * iL1) * L1;
+            d2 = d2 - rint(d2 * iL2) * L2;
             s2 = d0 * d0 + d1 * d1;
             s2 = s2 + d2 * d2;
           }
@@ -612,7 +614,7 @@ struct NbrArgs {
   double rc;
   int np, nT, nrbc;
   const double *x;  // node coordinates [d][np]
-  const int *src_bin;
+  const int *src_cc;  // coarse bin coordinates per node, packed 10 bits each
   const CellRec *cells;
   const double *sph, *chunk_sph;
   const double *tx;
@@ -630,6 +632,10 @@ __device__ __forceinline__ bool nbr_better(double r1, int k1, int n1, double r2,
   return n1 > n2;
 }
 
+// One wavefront per target: lanes test the cells' bounding spheres; for each cell
+// that passes they test its 64-node chunk spheres and sweep the nodes of the chunks
+// that reach within rc, then reduce (distance, visit key, node) over the wavefront.
+// Cells enter the list in increasing index order.
 __global__ void __launch_bounds__(NBR_T) k_nbr(NbrArgs a) {
   const int lane = threadIdx.x & 63;
   const int ti = (blockIdx.x * NBR_T + (int)threadIdx.x) >> 6;
@@ -676,25 +682,26 @@ __global__ void __launch_bounds__(NBR_T) k_nbr(NbrArgs a) {
         }
         unsigned long long cm = __ballot(cp);
         while (cm) {
-        const int kc = cb + __ffsll((long long)cm) - 1;
-        cm &= cm - 1;
-        const int q = 64 * kc + lane;
-        if (q >= npt) continue;
-        int node = cr.off + q;
-        double xj[3] = {a.x[node], a.x[node + np], a.x[node + 2 * np]}, xx[3];
-        double rr = ref_min_image_dist(xi, xj, a.Lb, a.iLb, xx);
-        if (rr > a.rc) continue;
-        int sb = a.src_bin[node];
-        int s1 = sb % a.Nc[0], s2 = (sb / a.Nc[0]) % a.Nc[1], s3 = sb / (a.Nc[0] * a.Nc[1]);
-        int d1 = imod_d(s1 - t1 + 1, a.Nc[0]), d2 = imod_d(s2 - t2 + 1, a.Nc[1]),
-            d3 = imod_d(s3 - t3 + 1, a.Nc[2]);
-        if (d1 > 2 || d2 > 2 || d3 > 2) continue;  // not in the 27 visited bins
-        int key = 9 * d1 + 3 * d2 + d3;
-        if (nbr_better(rr, key, node, best, bkey, bnode)) {
-          best = rr;
-          bkey = key;
-          bnode = node;
-        }
+          const int kc = cb + __ffsll((long long)cm) - 1;
+          cm &= cm - 1;
+          const int q = 64 * kc + lane;
+          if (q >= npt) continue;
+          const int node = cr.off + q;
+          double xj[3] = {a.x[node], a.x[node + np], a.x[node + 2 * np]}, xx[3];
+          double rr = ref_min_image_dist(xi, xj, a.Lb, a.iLb, xx);
+          if (rr > a.rc) continue;
+          const int cc = a.src_cc[node];
+          int d1 = (cc & 1023) - t1 + 1, d2 = ((cc >> 10) & 1023) - t2 + 1, d3 = (cc >> 20) - t3 + 1;
+          d1 += d1 < 0 ? a.Nc[0] : (d1 >= a.Nc[0] ? -a.Nc[0] : 0);
+          d2 += d2 < 0 ? a.Nc[1] : (d2 >= a.Nc[1] ? -a.Nc[1] : 0);
+          d3 += d3 < 0 ? a.Nc[2] : (d3 >= a.Nc[2] ? -a.Nc[2] : 0);
+          if (d1 > 2 || d2 > 2 || d3 > 2) continue;  // not in the 27 visited bins
+          const int key = 9 * d1 + 3 * d2 + d3;
+          if (nbr_better(rr, key, node, best, bkey, bnode)) {
+            best = rr;
+            bkey = key;
+            bnode = node;
+          }
         }
       }
       for (int o = 32; o > 0; o >>= 1) {
@@ -1283,24 +1290,34 @@ static int exclusive_scan(rbc3d_ctx *c, const int *in, int *out, int n) {
   return RBC3D_OK;
 }
 
+// cacheable: the targets are the cells' own nodes (the cell-BIE target list).  Target
+// binning, neighbouring-cell lists and closest-point projections depend on the
+// geometry only; they are kept from the previous cacheable call while the cells have
+// not been set again, and any other call overwrites (and so drops) them.
 int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const double *d_tx,
-                            const int *d_tindx, const int *d_act, const double *d_A, double *d_v) {
+                            const int *d_tindx, const int *d_act, const double *d_A, double *d_v,
+                            bool cacheable) {
   if (c->nrbc == 0 || nT == 0) return RBC3D_OK;
   hipStream_t st = c->stream;
   int nb = c->nfbins;
   TypePtrs tpv, *tp = &tpv;
   RBC_TRY(type_ptrs(c, tp));
-  // targets binned like the sources
-  RBC_TRY(c->d_tbin.ensure(nT + 1));
-  RBC_TRY(c->d_torder.ensure(nT + 1));
-  RBC_TRY(c->d_tbin_start.ensure(nb + 1));
-  rbc_stage_begin(c, ST_TSORT);
-  RBC_TRY(fine_sort(c, nT, d_tx, nT, c->d_tbin.p, c->d_torder.p, c->d_tbin_start.p));
-  rbc_stage_end(c, ST_TSORT);
+  cacheable = cacheable && c->geom_reuse;
+  const bool hit = cacheable && c->cache_epoch == c->geom_epoch && c->cache_nT == nT;
+  c->cache_epoch = -1;  // valid again only when this call completes as cacheable
   RBC_TRY(c->d_flag.ensure(2 * (size_t)(nb + 1) + 8));
   int *chunk_cnt = c->d_flag.p + 8, *chunk_off = chunk_cnt + (nb + 1);
-  k_chunks<<<(nb + 1 + 255) / 256, 256, 0, st>>>(nb, c->d_tbin_start.p, chunk_cnt);
-  RBC_TRY(exclusive_scan(c, chunk_cnt, chunk_off, nb + 1));
+  if (!hit) {
+    // targets binned like the sources
+    RBC_TRY(c->d_tbin.ensure(nT + 1));
+    RBC_TRY(c->d_torder.ensure(nT + 1));
+    RBC_TRY(c->d_tbin_start.ensure(nb + 1));
+    rbc_stage_begin(c, ST_TSORT);
+    RBC_TRY(fine_sort(c, nT, d_tx, nT, c->d_tbin.p, c->d_torder.p, c->d_tbin_start.p));
+    rbc_stage_end(c, ST_TSORT);
+    k_chunks<<<(nb + 1 + 255) / 256, 256, 0, st>>>(nb, c->d_tbin_start.p, chunk_cnt);
+    RBC_TRY(exclusive_scan(c, chunk_cnt, chunk_off, nb + 1));
+  }
   RBC_TRY(c->d_vacc.ensure((size_t)3 * nT + 1));
   RBC_TRY(c->d_pairstat.ensure(2));
   HIP_TRY(hipMemsetAsync(c->d_pairstat.p, 0, 2 * sizeof(unsigned long long), st));
@@ -1316,7 +1333,6 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   pa.rc2lo = c->conf.rc * c->conf.rc * (1.0 - 1e-12);
   pa.reps2 = c->r_eps * c->r_eps;
   pa.tabscale = RBC_NTAB / c->conf.rc;
-  pa.run_shift = c->run_shift;
   pa.nrows = (int)c->noffs;
   pa.rows = c->d_offs.p;
   pa.c1 = c1;
@@ -1357,6 +1373,8 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   rbc_stage_end(c, ST_PAIR);
   HIP_TRY(hipGetLastError());
   // neighbouring-cell lists
+  int *item_off = nullptr, nitems = 0;
+  if (!hit) {
   RBC_TRY(c->d_nbr_cnt.ensure(2 * (size_t)nT + 2));
   RBC_TRY(c->d_nbr_cell.ensure((size_t)RBC_NBR_MAX * nT + 1));
   RBC_TRY(c->d_nbr_node.ensure((size_t)RBC_NBR_MAX * nT + 1));
@@ -1373,7 +1391,7 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   na.nT = nT;
   na.nrbc = c->nrbc;
   na.x = c->d_x.p;
-  na.src_bin = c->d_src_bin.p;
+  na.src_cc = c->d_src_cc.p;
   na.cells = c->d_cells.p;
   na.sph = c->d_cell_sph.p;
   na.chunk_sph = c->d_chunk_sph.p;
@@ -1389,10 +1407,10 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   k_nbr<<<(unsigned)(((size_t)nT * 64 + NBR_T - 1) / NBR_T), NBR_T, 0, st>>>(na);
   rbc_stage_end(c, ST_NBR);
   HIP_TRY(hipGetLastError());
-  int *item_off = c->d_nbr_cnt.p + nT + 1;
+  item_off = c->d_nbr_cnt.p + nT + 1;
   HIP_TRY(hipMemsetAsync(c->d_nbr_cnt.p + nT, 0, sizeof(int), st));
   RBC_TRY(exclusive_scan(c, c->d_nbr_cnt.p, item_off, nT + 1));
-  int hflag[2] = {0, 0}, nitems = 0;
+  int hflag[2] = {0, 0};
   HIP_TRY(hipMemcpyAsync(hflag, c->d_flag.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipMemcpyAsync(&nitems, item_off + nT, sizeof(int), hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
@@ -1400,8 +1418,12 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
     rbc_set_error("more than 32 neighbouring cells within the cut-off of a target");
     return RBC3D_ENBR;
   }
+  } else {
+    item_off = c->d_nbr_cnt.p + nT + 1;
+    nitems = c->cache_nitems;
+  }
   c->last_nitems = nitems;
-  c->last_nwork = 0;
+  if (!hit) c->last_nwork = 0;
   // singular and near-singular corrections
   RBC_TRY(c->d_scratch.ensure((size_t)12 * nitems + (size_t)3 * nT + 16));
   RBC_TRY(c->d_dvitem.ensure((size_t)3 * nitems + 3));
@@ -1459,7 +1481,16 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   k_self<<<(unsigned)(((size_t)nT * 64 + 255) / 256), 256, 0, st>>>(sa);
   rbc_stage_end(c, ST_SELF);
   HIP_TRY(hipGetLastError());
-  if (nitems > 0) {
+  if (nitems > 0 && hit) {
+    const int nwork = c->cache_nwork;
+    c->last_nwork = nwork;
+    if (nwork > 0) {
+      rbc_stage_begin(c, ST_NEAR);
+      k_near<<<(unsigned)(((size_t)nwork * 64 + 255) / 256), 256, 0, st>>>(sa, nwork);
+      rbc_stage_end(c, ST_NEAR);
+      HIP_TRY(hipGetLastError());
+    }
+  } else if (nitems > 0) {
     rbc_stage_begin(c, ST_PROJ);
     RBC_TRY(c->d_ikey.ensure(2 * (size_t)nitems));
     RBC_TRY(c->d_ival.ensure(2 * (size_t)nitems));
@@ -1530,7 +1561,11 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   HIP_TRY(hipStreamSynchronize(st));
   c->last_npairs = (long long)hs[0];
   c->last_checksum = hs[1];
-  part.release();
-  wl.release();
+  if (cacheable) {
+    c->cache_epoch = c->geom_epoch;
+    c->cache_nT = nT;
+    c->cache_nitems = nitems;
+    c->cache_nwork = c->last_nwork;
+  }
   return RBC3D_OK;
 }

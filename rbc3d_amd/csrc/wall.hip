@@ -791,7 +791,7 @@ int rbc_wall_residual_dev(rbc3d_ctx *c) {
   double c1 = 1. / (4. * RBC_PI), c2 = 1. / (4. * RBC_PI);
   if (c->nrbc)
     RBC_TRY(rbc_add_int_on_rbcs_dev(c, c1, c2, nv, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
-                                    c->d_v.p));
+                                    c->d_v.p, false));
   RBC_TRY(rbc_add_int_on_walls_dev(c, c1, nv, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
                                    c->d_v.p, true));
   RBC_TRY(rbc_pme_init(c));

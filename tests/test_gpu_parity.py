@@ -156,3 +156,39 @@ def test_dense_subset(R, orc):
     assert relerr(vg, vo) < TOL
     u = np.random.default_rng(2).standard_normal(o.cell_dof())
     assert relerr(g.cell_matmult(u), o.cell_matmult(u)) < TOL
+
+
+def test_geometry_reuse(R, orc):
+    """Matvecs that reuse the geometry-only results (neighbour lists, projections) of
+    the first one give exactly the values of matvecs that recompute them; after the
+    cells are set again the reuse starts over, and the result matches the oracle."""
+    from rbc3d_amd import configs
+    w = configs.make_workload("C2", seed=3, ncell=48, Nb=(48, 48, 48))
+    g = R.from_workload(w)
+    rng = np.random.default_rng(21)
+    dof = g.cell_dof()
+    u1, u2 = rng.standard_normal(dof), rng.standard_normal(dof)
+    g.set_geometry_reuse(True)
+    g.cell_matmult(u1)
+    b_reuse = g.cell_matmult(u2)
+    assert g.near_stats()[1] > 0  # near-singular corrections are exercised
+    g.set_geometry_reuse(False)
+    b_full = g.cell_matmult(u2)
+    assert np.array_equal(b_reuse, b_full)
+    # move the cells: a stale reuse would keep the old neighbour lists and projections
+    g.set_geometry_reuse(True)
+    c = w.cells
+    shift = np.zeros_like(c.x)
+    off = 0
+    for nl in c.nlat:
+        npt = int(nl) * 2 * int(nl)
+        shift[3 * off: 3 * off + npt] = 0.05  # x components of the cell block
+        off += npt
+    from dataclasses import replace
+    c2 = replace(c, x=c.x + shift)
+    g.set_cells(c2)
+    g.set_cell_density(1, c2.f)
+    g.set_cell_density(2, c2.g)
+    b_moved = g.cell_matmult(u2)
+    o = orc.from_workload(replace(w, cells=c2))
+    assert relerr(b_moved, o.cell_matmult(u2)) < 2e-12
