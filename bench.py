@@ -142,6 +142,45 @@ def timestep_bench(steps, warmup):
     }
 
 
+def c5_bench(steps, warmup):
+    """Cell-BIE matvec on the large mixed batch: 3,584 biconcave discs (nlat 13) and 512
+    spheres of radius 1.4 (nlat 17) at 30% volume fraction, PME 256^3, P=6, viscosity
+    ratio 5; two grid types in one launch of every kernel."""
+    import torch
+    import rbc3d_amd
+    from rbc3d_amd import configs
+    w = configs.make_workload("C5", seed=1)
+    g = rbc3d_amd.from_workload(w)
+    g.set_geometry_reuse(False)
+    dof = g.cell_dof()
+    dev = torch.device("cuda:0")
+    u = torch.randn(dof, dtype=torch.float64, device=dev,
+                    generator=torch.Generator(device=dev).manual_seed(5))
+    b = torch.empty(dof, dtype=torch.float64, device=dev)
+    stream = torch.cuda.ExternalStream(g.stream())
+    for _ in range(warmup):
+        g.cell_matmult_dev(u.data_ptr(), b.data_ptr())
+    g.sync()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(stream)
+    for _ in range(steps):
+        g.cell_matmult_dev(u.data_ptr(), b.data_ptr())
+    e1.record(stream)
+    g.sync()
+    torch.cuda.synchronize()
+    ms = max(e0.elapsed_time(e1), 1e3 * (time.perf_counter() - t0)) / steps
+    npairs, _ = g.pair_stats()
+    return {"workload": f"C5: {w.cells.ncell} cells (3584 discs nlat 13, 512 spheres R 1.4 "
+                        f"nlat 17), box {w.Lb[0]:.3f}^3 (30% volume fraction), PME {w.Nb[0]}^3, "
+                        f"P={w.P}, lambda=5, one cell-BIE matvec",
+            "nodes": w.cells.npoint, "ms_per_step": round(ms, 4),
+            "cell_matvecs_per_s": round(w.cells.ncell / (ms * 1e-3), 3),
+            "pairs_per_matvec": int(npairs), "steps": steps}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -154,6 +193,8 @@ def main():
     ap.add_argument("--no-timestep", action="store_true",
                     help="skip the tube timestep (GMRES seconds per timestep)")
     ap.add_argument("--timestep-steps", type=int, default=3)
+    ap.add_argument("--c5", action="store_true",
+                    help="also time the 4,096-cell mixed-size matvec (C5)")
     ap.add_argument("--timestep-only", action="store_true",
                     help="only the tube timestep (prints its JSON object)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -275,6 +316,8 @@ def main():
     }
     if not args.no_timestep:
         out["timestep"] = timestep_bench(args.timestep_steps, 1)
+    if args.c5:
+        out["C5"] = c5_bench(3, 1)
     if not args.no_cpu:
         tot, parts, thr = cpu_baseline(w, args.sample_every)
         out["cpu_baseline"] = {"value": round(ncell / tot, 4), "unit": "cell-matvecs/s",

@@ -33,12 +33,14 @@ __global__ void __launch_bounds__(G_T) k_mdot(int n, int k, const double *__rest
   }
 }
 
+// out[j] = sum over the nb partial sums of vector j: one wavefront per vector
 __global__ void k_mdot_final(int k, int nb, const double *__restrict__ part, double *out) {
-  int j = blockIdx.x * blockDim.x + threadIdx.x;
+  const int j = blockIdx.x, lane = threadIdx.x;
   if (j >= k) return;
   double s = 0;
-  for (int b = 0; b < nb; b++) s += part[(size_t)j * nb + b];
-  out[j] = s;
+  for (int b = lane; b < nb; b += 64) s += part[(size_t)j * nb + b];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  if (lane == 0) out[j] = s;
 }
 
 // w -= sum_j h_j V_j
@@ -72,17 +74,33 @@ __global__ void k_sub(int n, const double *__restrict__ b, double *r) {  // r = 
 }
 
 
-static int dots(hipStream_t st, GmresWork &ws, int n, int k, const double *V, size_t ld,
-                const double *w, double *h_host) {
+// out[j] = <V_j, w>, j < k, left on the device
+static int dots_dev(hipStream_t st, GmresWork &ws, int n, int k, const double *V, size_t ld,
+                    const double *w, double *out) {
   int nb = std::min(1024, (n + G_T - 1) / G_T);
   RBC_TRY(ws.part.ensure((size_t)nb * G_MAXV + 1));
-  RBC_TRY(ws.hd.ensure(G_MAXV + 1));
   k_mdot<<<nb, G_T, 0, st>>>(n, k, V, ld, w, ws.part.p);
-  k_mdot_final<<<1, 64, 0, st>>>(k, nb, ws.part.p, ws.hd.p);
+  k_mdot_final<<<k, 64, 0, st>>>(k, nb, ws.part.p, out);
   HIP_TRY(hipGetLastError());
+  return RBC3D_OK;
+}
+
+static int dots(hipStream_t st, GmresWork &ws, int n, int k, const double *V, size_t ld,
+                const double *w, double *h_host) {
+  RBC_TRY(ws.hd.ensure(2 * G_MAXV + 2));
+  RBC_TRY(dots_dev(st, ws, n, k, V, ld, w, ws.hd.p));
   HIP_TRY(hipMemcpyAsync(h_host, ws.hd.p, sizeof(double) * k, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   return RBC3D_OK;
+}
+
+// w /= sqrt(t2[0]) when sqrt(t2[0]) > bound (the host decides happy breakdown from the
+// same value afterwards)
+__global__ void k_scale_norm(int n, const double *__restrict__ t2, double bound, double *w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double tt = sqrt(t2[0]);
+  if (tt > bound) w[i] = (1.0 / tt) * w[i];
 }
 
 int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, double *)> &matvec,
@@ -134,21 +152,25 @@ int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, doubl
       double *vj = V + ld * it, *vn = V + ld * (it + 1);
       RBC_TRY(matvec(vj, vn));
       // classical Gram-Schmidt: h = V^T w, w -= V h
-      // (the projections stay on the device in ws.hd for the update)
-      RBC_TRY(dots(st, ws, n, it + 1, V, ld, vn, h.data()));
+      // the projections h stay on the device for the update; |w|^2 follows in
+      // hd[G_MAXV], the normalisation runs on the device, and the host reads h and
+      // |w|^2 once per iteration
+      RBC_TRY(ws.hd.ensure(2 * G_MAXV + 2));
+      RBC_TRY(dots_dev(st, ws, n, it + 1, V, ld, vn, ws.hd.p));
       k_maxpy<<<nb, 256, 0, st>>>(n, it + 1, V, ld, ws.hd.p, vn);
+      RBC_TRY(dots_dev(st, ws, n, 1, vn, ld, vn, ws.hd.p + G_MAXV));
+      // happy breakdown bound as in PETSc: haptol * |g(it)|
+      double hapbnd = fabs(g[it]) * 1.0e-30;
+      k_scale_norm<<<nb, 256, 0, st>>>(n, ws.hd.p + G_MAXV, hapbnd, vn);
+      HIP_TRY(hipGetLastError());
+      HIP_TRY(hipMemcpyAsync(h.data(), ws.hd.p, sizeof(double) * (it + 1), hipMemcpyDeviceToHost, st));
       double tt2;
-      RBC_TRY(dots(st, ws, n, 1, vn, ld, vn, &tt2));
+      HIP_TRY(hipMemcpyAsync(&tt2, ws.hd.p + G_MAXV, sizeof(double), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipStreamSynchronize(st));
       double tt = sqrt(tt2);
       for (int i = 0; i <= it; i++) H[(size_t)i * restart + it] = h[i];
       H[(size_t)(it + 1) * restart + it] = tt;
-      // happy breakdown bound as in PETSc: haptol * |g(it)|
-      double hapbnd = fabs(g[it]) * 1.0e-30;
-      if (tt > hapbnd) {
-        k_scale<<<nb, 256, 0, st>>>(n, 1.0 / tt, vn, vn);
-      } else {
-        happy = true;
-      }
+      if (!(tt > hapbnd)) happy = true;
       // apply the previous rotations and form a new one (KSPGMRESUpdateHessenberg)
       for (int i = 0; i < it; i++) {
         double a = H[(size_t)i * restart + it], b = H[(size_t)(i + 1) * restart + it];

@@ -741,36 +741,49 @@ struct GatherArgs {
 
 
 // PC: the B-spline order when it is known at compile time (0: runtime a.P)
+#define GATHER_L 8  // lanes per target
 template <int PC>
 __global__ void __launch_bounds__(256) k_gather(GatherArgs a) {
-  const int ti = blockIdx.x * blockDim.x + threadIdx.x;
-  if (ti >= a.nT || !a.tact[ti]) return;
-  double x[3] = {a.tx[ti], a.tx[ti + a.nT], a.tx[ti + 2 * (size_t)a.nT]};
-  double wx[12], wy[12], wz[12];
-  const int P = PC ? PC : a.P;
-  int kmin = bspline_d(x[2] * a.ih[2], P, wz);
-  int jmin = bspline_d(x[1] * a.ih[1], P, wy);
-  int imin = bspline_d(x[0] * a.ih[0], P, wx);
-  size_t ng = (size_t)a.N1 * a.N2 * a.N3;
+  // GATHER_L lanes per target share its P x P (y, z) stencil rows; each row is P
+  // consecutive x points.  The partial sums are combined with shuffles.
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ti = gid / GATHER_L, sub = gid % GATHER_L;
+  const bool live = ti < a.nT && a.tact[ti];
   double v0 = 0, v1 = 0, v2 = 0;
-  for (int k0 = 0; k0 < P; k0++) {
-    int k = imod_d(kmin + k0, a.N3);
-    for (int j0 = 0; j0 < P; j0++) {
-      int j = imod_d(jmin + j0, a.N2);
+  if (live) {
+    double x[3] = {a.tx[ti], a.tx[ti + a.nT], a.tx[ti + 2 * (size_t)a.nT]};
+    double wx[12], wy[12], wz[12];
+    const int P = PC ? PC : a.P;
+    int kmin = bspline_d(x[2] * a.ih[2], P, wz);
+    int jmin = bspline_d(x[1] * a.ih[1], P, wy);
+    int imin = bspline_d(x[0] * a.ih[0], P, wx);
+    const size_t ng = (size_t)a.N1 * a.N2 * a.N3;
+    for (int r = sub; r < P * P; r += GATHER_L) {
+      const int j0 = r % P, k0 = r / P;
+      const int k = imod_d(kmin + k0, a.N3), j = imod_d(jmin + j0, a.N2);
+      const double wjk = wy[j0] * wz[k0];
+      const size_t row = (size_t)a.N1 * (j + (size_t)a.N2 * k);
       for (int i0 = 0; i0 < P; i0++) {
-        int i = imod_d(imin + i0, a.N1);
-        double w = wx[i0] * wy[j0] * wz[k0];
-        size_t o = i + (size_t)a.N1 * (j + (size_t)a.N2 * k);
+        const int i = imod_d(imin + i0, a.N1);
+        const double w = wx[i0] * wjk;
+        const size_t o = row + i;
         v0 += w * a.vv[o];
         v1 += w * a.vv[o + ng];
         v2 += w * a.vv[o + 2 * ng];
       }
     }
   }
-  double A = a.tA[ti];
-  a.v[ti] += v0 / A;
-  a.v[ti + (size_t)a.nT] += v1 / A;
-  a.v[ti + 2 * (size_t)a.nT] += v2 / A;
+  for (int o = GATHER_L / 2; o > 0; o >>= 1) {
+    v0 += __shfl_xor(v0, o, GATHER_L);
+    v1 += __shfl_xor(v1, o, GATHER_L);
+    v2 += __shfl_xor(v2, o, GATHER_L);
+  }
+  if (live && sub == 0) {
+    const double A = a.tA[ti];
+    a.v[ti] += v0 / A;
+    a.v[ti + (size_t)a.nT] += v1 / A;
+    a.v[ti + 2 * (size_t)a.nT] += v2 / A;
+  }
 }
 
 int rbc_pme_gather(rbc3d_ctx *c, int nT, const double *d_tx, const int *d_act, const double *d_A,
@@ -789,10 +802,11 @@ int rbc_pme_gather(rbc3d_ctx *c, int nT, const double *d_tx, const int *d_act, c
   g.tact = d_act;
   g.v = d_v;
   rbc_stage_begin(c, ST_GATHER);
+  const unsigned nblk = (unsigned)(((size_t)nT * GATHER_L + 255) / 256);
   if (g.P == 6)
-    k_gather<6><<<(nT + 255) / 256, 256, 0, c->stream>>>(g);
+    k_gather<6><<<nblk, 256, 0, c->stream>>>(g);
   else
-    k_gather<0><<<(nT + 255) / 256, 256, 0, c->stream>>>(g);
+    k_gather<0><<<nblk, 256, 0, c->stream>>>(g);
   rbc_stage_end(c, ST_GATHER);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;

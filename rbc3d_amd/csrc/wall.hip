@@ -379,10 +379,9 @@ __global__ void __launch_bounds__(128) k_wall_reduce(int nvert, const int *__res
 }
 
 // v_i += c1 (L f)_i / Acoef_i over wall vertices (SingIntOnWall :136-172)
-// v_i += c1/A_i * sum_k B_k f_(col k): one wavefront per vertex row; the row's 3x3
-// blocks are read as one contiguous run of doubles (lane l takes element l, l+64, ...
-// of the run), so every load instruction is coalesced; element e of block k belongs to
-// output component (e % 9) / 3 and multiplies f component e % 3 of vertex bcol[k].
+// v_i += c1/A_i * sum_k B_k f_(col k): one wavefront per vertex row, lanes over the
+// row's 3x3 blocks (each lane reads its blocks' 9 values, which lie next to those of
+// the neighbouring lanes, and the 3 traction components of the block's column).
 __global__ void __launch_bounds__(256) k_wall_spmv(int nvert, double c1, const int *__restrict__ rowptr,
                                                    const int *__restrict__ bcol,
                                                    const double *__restrict__ bval,
@@ -390,24 +389,25 @@ __global__ void __launch_bounds__(256) k_wall_spmv(int nvert, double c1, const i
                                                    const double *__restrict__ tA, double *v) {
   const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
   if (i >= nvert) return;
-  const int e0 = 9 * rowptr[i], e1 = 9 * rowptr[i + 1];  // 9 nnz < 2^31 (checked at assembly)
-  double s[3] = {0, 0, 0};
-  for (int e = e0 + lane; e < e1; e += 64) {
-    const int k = e / 9, r = e - 9 * k;
+  double s0 = 0, s1 = 0, s2 = 0;
+  for (int k = rowptr[i] + lane; k < rowptr[i + 1]; k += 64) {
     const int j = bcol[k];
-    const double t = bval[e] * f[j + (size_t)nvert * (r % 3)];
-    const int row = r / 3;
-    s[0] += row == 0 ? t : 0.;
-    s[1] += row == 1 ? t : 0.;
-    s[2] += row == 2 ? t : 0.;
+    const double *B = bval + 9 * (size_t)k;
+    const double f0 = f[j], f1 = f[j + nvert], f2 = f[j + 2 * (size_t)nvert];
+    s0 += B[0] * f0 + B[1] * f1 + B[2] * f2;
+    s1 += B[3] * f0 + B[4] * f1 + B[5] * f2;
+    s2 += B[6] * f0 + B[7] * f1 + B[8] * f2;
   }
-  for (int d = 0; d < 3; d++)
-    for (int o = 32; o > 0; o >>= 1) s[d] += __shfl_xor(s[d], o, 64);
+  for (int o = 32; o > 0; o >>= 1) {
+    s0 += __shfl_xor(s0, o, 64);
+    s1 += __shfl_xor(s1, o, 64);
+    s2 += __shfl_xor(s2, o, 64);
+  }
   if (lane == 0) {
     const double A = tA[i];
-    v[i] += c1 * s[0] / A;
-    v[i + nvert] += c1 * s[1] / A;
-    v[i + 2 * (size_t)nvert] += c1 * s[2] / A;
+    v[i] += c1 * s0 / A;
+    v[i + nvert] += c1 * s1 / A;
+    v[i + 2 * (size_t)nvert] += c1 * s2 / A;
   }
 }
 
