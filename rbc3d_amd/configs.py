@@ -422,13 +422,14 @@ def make_workload(name: str, seed: int = 1, ncell: int = None, Nb: tuple = None)
         rad = [1.0] * n1 + [1.4] * n2
         bd = shape_param("disc", 13)[1].reshape(-1, 3)
         bs = shape_param("sphere", 17, 1.4)[1].reshape(-1, 3)
-        order = rng.permutation(n1 + n2)
-        shapes = [(bs, 1.4) if order[k] >= n1 else (bd, 1.0) for k in range(n1 + n2)]
-        kinds = ["sphere" if order[k] >= n1 else "disc" for k in range(n1 + n2)]
-        nl = [17 if order[k] >= n1 else 13 for k in range(n1 + n2)]
-        rad = [1.4 if order[k] >= n1 else 1.0 for k in range(n1 + n2)]
-        cs, rots = _placement(f"C5_n{n1 + n2}_s{seed}", lambda: place_cells_gap(
-            rng, shapes, (L, L, L), SURFACE_GAP))
+        # random sequential addition jams below 30% with a 0.1 gap and mixed order; the
+        # large spheres go first and the gap is 0.05
+        shapes = [(bs, 1.4)] * n2 + [(bd, 1.0)] * n1
+        kinds = ["sphere"] * n2 + ["disc"] * n1
+        nl = [17] * n2 + [13] * n1
+        rad = [1.4] * n2 + [1.0] * n1
+        cs, rots = _placement(f"C5_n{n1 + n2}_g05_s{seed}", lambda: place_cells_gap(
+            rng, shapes, (L, L, L), 0.05, max_tries=50_000_000))
         cells = build_cells(rng, kinds, nl, rad, cs, [1] * (n1 + n2), rots=rots)
         xi = xi_for_tolerance(2.0, 1e-8)
         return Workload("C5", (L, L, L), alpha_ref_from_xi(xi), 2.0, Nb or (256, 256, 256), 6,

@@ -178,6 +178,7 @@ struct rbc3d_ctx {
   DBuf<double> d_cell_sph;
   DBuf<double> d_chunk_sph;  // per 64-node chunk of every cell: centre(3), radius
   DBuf<int> d_src_cc;        // coarse cell-list coordinates per node (packed)
+  DBuf<double> d_shx;        // SH Legendre GEMM operands (m, row, col) x 2
   int nchunks = 0;
   DBuf<int> d_cellbin_start, d_cellbin_order;
   double max_cell_radius = 0;

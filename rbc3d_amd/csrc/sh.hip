@@ -236,58 +236,122 @@ __device__ __forceinline__ int pack_b(int m, int n, int n0) {
   return nb0 + mm * (n0 - 1) - mm * (mm - 1) / 2 + (n - m);
 }
 
-__global__ void __launch_bounds__(SH_THREADS) k_glob_anal(
-    const CellRec *__restrict__ cells, int cell0, int gtype, const double *__restrict__ pw,
-    const double *__restrict__ cs, const double *__restrict__ v, int np, double *__restrict__ cf,
-    const int *__restrict__ cell_list) {
-  extern __shared__ double sm[];
-  int ci = cell_list[cell0 + blockIdx.x];
-  const CellRec cr = cells[ci];
-  if (cr.gtype != gtype) return;
-  int nl = cr.nlat, nn = cr.nlon, n0 = cr.nlat0;
-  int mmax = min(nl, nn / 2 + 1);
-  double *f = sm, *C = f + nl * nn, *a = C + 2 * mmax * nl;
-  for (int d = 0; d < 3; d++) {
-    for (int t = threadIdx.x; t < nl * nn; t += blockDim.x) f[t] = v[cr.off + t + (size_t)np * d];
-    __syncthreads();
-    lon_dft(f, nl, nl, nn, mmax, cs, nl + 1, C, C + mmax * nl);
-    __syncthreads();
-    leg_anal(C, C + mmax * nl, pw, nl, min(mmax, n0), n0, a, a + n0 * n0, n0);
-    __syncthreads();
-    for (int t = threadIdx.x; t < n0 * n0; t += blockDim.x) {
-      int m = t % n0, n = t / n0;
-      if (n < m) continue;
-      cf[cr.coff + 3 * pack_a(m, n, n0) + d] = a[m + n0 * n];
-      if (m >= 1) cf[cr.coff + 3 * pack_b(m, n, n0) + d] = a[n0 * n0 + m + n0 * n];
+// ---------------------------------------------------------------------------
+// Glob_Sph_Trans with the Legendre stage as batched GEMMs on the matrix cores.
+// For every m the Legendre sums of all cells of one grid type form one product:
+//   analysis   Y_m(n, col) = sum_i pw[m][n][i] X_m(i, col)
+//   synthesis  G_m(i, col) = sum_n pg[m][n][i] Z_m(n, col)
+// with col = 6 * cell + 2 * component + (0: cos, 1: sin).  The longitude DFTs and the
+// coefficient (un)packing stay per cell.  Matrices are (m, row, col) with col fastest,
+// ld = ncolp (6 * ncell rounded up to 16).
+typedef double rbc_d4 __attribute__((ext_vector_type(4)));
+
+// X_m(i, col) = sum_j v(i, j) cos(m phi_j) / sin(m phi_j) for the 3 components
+__global__ void __launch_bounds__(SH_THREADS) k_sh_lon_anal(
+    const CellRec *__restrict__ cells, const int *__restrict__ cell_list, int cell0, int mmax,
+    const double *__restrict__ cs, const double *__restrict__ v, int np, int ncolp, double *X) {
+  const int c = blockIdx.x;
+  const CellRec cr = cells[cell_list[cell0 + c]];
+  const int nl = cr.nlat, nn = cr.nlon;
+  const double *cosT = cs, *sinT = cs + (size_t)(nl + 1) * nn;
+  for (int t = threadIdx.x; t < 3 * mmax * nl; t += blockDim.x) {
+    const int d = t / (mmax * nl), r = t - d * mmax * nl, m = r / nl, i = r - m * nl;
+    const double *f = v + cr.off + (size_t)np * d;
+    double a = 0, b = 0;
+    for (int j = 0; j < nn; j++) {
+      const double x = f[i + nl * j];
+      a += x * cosT[m * nn + j];
+      b += x * sinT[m * nn + j];
     }
-    __syncthreads();
+    double *Xm = X + ((size_t)m * nl + i) * ncolp + 6 * c + 2 * d;
+    Xm[0] = a;
+    Xm[1] = b;
   }
 }
 
-__global__ void __launch_bounds__(SH_THREADS) k_glob_synth(
-    const CellRec *__restrict__ cells, int cell0, int gtype, const double *__restrict__ pg,
-    const double *__restrict__ cs, const double *__restrict__ cf, int np, double *__restrict__ v,
-    const int *__restrict__ cell_list) {
-  extern __shared__ double sm[];
-  int ci = cell_list[cell0 + blockIdx.x];
-  const CellRec cr = cells[ci];
-  if (cr.gtype != gtype) return;
-  int nl = cr.nlat, nn = cr.nlon, n0 = cr.nlat0;
-  int mm = min(n0, min(nl, nn / 2 + 1));
-  double *a = sm, *G = a + 2 * n0 * n0, *g = G + 2 * mm * nl;
-  for (int d = 0; d < 3; d++) {
-    for (int t = threadIdx.x; t < n0 * n0; t += blockDim.x) {
-      int m = t % n0, n = t / n0;
-      a[t] = (n >= m) ? cf[cr.coff + 3 * pack_a(m, n, n0) + d] : 0.;
-      a[n0 * n0 + t] = (n >= m && m >= 1) ? cf[cr.coff + 3 * pack_b(m, n, n0) + d] : 0.;
+// One wavefront per 16 x 16 output tile of one m: D += A(16 x 4) B(4 x 16) with the
+// FP64 MFMA (v_mfma_f64_16x16x4f64: lane l supplies A[l % 16][l / 16] and
+// B[l / 16][l % 16] and holds D[4 i + l / 16][l % 16], i < 4).
+// TRANS_A: A(row, k) = P[m][k][row] (synthesis) instead of P[m][row][k] (analysis).
+template <bool TRANS_A>
+__global__ void __launch_bounds__(256) k_sh_leg_mfma(int nl, int mmax, int ncolp, int RT, int CT,
+                                                     const double *__restrict__ P,
+                                                     const double *__restrict__ X, double *Y) {
+  const int wid = (int)((blockIdx.x * 256 + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  if (wid >= mmax * RT * CT) return;
+  const int ct = wid % CT, rt = (wid / CT) % RT, m = wid / (CT * RT);
+  const int row = rt * 16 + (lane & 15), kq = lane >> 4, col = ct * 16 + (lane & 15);
+  const double *Pm = P + (size_t)m * nl * nl;
+  const double *Xm = X + (size_t)m * nl * ncolp;
+  rbc_d4 acc = {0., 0., 0., 0.};
+  for (int k0 = 0; k0 < nl; k0 += 4) {
+    const int k = k0 + kq;
+    double a = 0., b = 0.;
+    if (k < nl) {
+      if (row < nl) a = TRANS_A ? Pm[(size_t)k * nl + row] : Pm[(size_t)row * nl + k];
+      b = Xm[(size_t)k * ncolp + col];
     }
-    __syncthreads();
-    leg_synth(a, a + n0 * n0, n0, pg, nl, nl, mm, n0, G, G + mm * nl);
-    __syncthreads();
-    lon_synth(G, G + mm * nl, nl, nn, mm, cs, nl + 1, g, nl);
-    __syncthreads();
-    for (int t = threadIdx.x; t < nl * nn; t += blockDim.x) v[cr.off + t + (size_t)np * d] = g[t];
-    __syncthreads();
+    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+  }
+  double *Ym = Y + (size_t)m * nl * ncolp;
+  for (int i = 0; i < 4; i++) {
+    const int r = rt * 16 + 4 * i + (lane >> 4);
+    if (r < nl) Ym[(size_t)r * ncolp + col] = acc[i];
+  }
+}
+
+// coefficients of Y into the Glob_Sph_Trans vector: a = Y(cos), b = -Y(sin)
+__global__ void k_sh_pack(const CellRec *__restrict__ cells, const int *__restrict__ cell_list,
+                          int cell0, int nl, int mmax, int ncolp, const double *__restrict__ Y,
+                          double *cf) {
+  const int c = blockIdx.x;
+  const CellRec cr = cells[cell_list[cell0 + c]];
+  const int n0 = cr.nlat0, mm = min(mmax, n0);
+  for (int t = threadIdx.x; t < 3 * n0 * n0; t += blockDim.x) {
+    const int d = t / (n0 * n0), r = t - d * n0 * n0, m = r % n0, n = r / n0;
+    if (m >= mm || n < m) continue;
+    const double *y = Y + ((size_t)m * nl + n) * ncolp + 6 * c + 2 * d;
+    cf[cr.coff + 3 * pack_a(m, n, n0) + d] = y[0];
+    if (m >= 1) cf[cr.coff + 3 * pack_b(m, n, n0) + d] = -y[1];
+  }
+}
+
+// Z_m(n, col) from the Glob_Sph_Trans vector (zero outside m <= n < nlat0, m < mm)
+__global__ void k_sh_unpack(const CellRec *__restrict__ cells, const int *__restrict__ cell_list,
+                            int cell0, int nl, int mmax, int ncolp, const double *__restrict__ cf,
+                            double *Z) {
+  const int c = blockIdx.x;
+  const CellRec cr = cells[cell_list[cell0 + c]];
+  const int n0 = cr.nlat0, mm = min(n0, mmax);
+  for (int t = threadIdx.x; t < 3 * mmax * nl; t += blockDim.x) {
+    const int d = t / (mmax * nl), r = t - d * mmax * nl, m = r / nl, n = r - m * nl;
+    double a = 0., b = 0.;
+    if (m < mm && n >= m && n < n0) {
+      a = cf[cr.coff + 3 * pack_a(m, n, n0) + d];
+      if (m >= 1) b = cf[cr.coff + 3 * pack_b(m, n, n0) + d];
+    }
+    double *z = Z + ((size_t)m * nl + n) * ncolp + 6 * c + 2 * d;
+    z[0] = a;
+    z[1] = b;
+  }
+}
+
+// v(i, j) = sum_m G_m(i, cos) cos(m phi_j) - G_m(i, sin) sin(m phi_j)
+__global__ void __launch_bounds__(SH_THREADS) k_sh_lon_synth(
+    const CellRec *__restrict__ cells, const int *__restrict__ cell_list, int cell0, int mmax,
+    const double *__restrict__ cs, const double *__restrict__ G, int ncolp, int np, double *v) {
+  const int c = blockIdx.x;
+  const CellRec cr = cells[cell_list[cell0 + c]];
+  const int nl = cr.nlat, nn = cr.nlon, mm = min(cr.nlat0, mmax);
+  const double *cosT = cs, *sinT = cs + (size_t)(nl + 1) * nn;
+  for (int t = threadIdx.x; t < 3 * nl * nn; t += blockDim.x) {
+    const int d = t / (nl * nn), q = t - d * nl * nn, i = q % nl, j = q / nl;
+    double x = 0;
+    for (int m = 0; m < mm; m++) {
+      const double *g = G + ((size_t)m * nl + i) * ncolp + 6 * c + 2 * d;
+      x += g[0] * cosT[m * nn + j] - g[1] * sinT[m * nn + j];
+    }
+    v[cr.off + q + (size_t)np * d] = x;
   }
 }
 
@@ -298,21 +362,33 @@ int rbc_sh_glob(rbc3d_ctx *c, const double *d_in, double *d_out, int direction) 
     int n = cells_of_type(c, (int)gt, ids);
     if (!n) continue;
     GridType *g = c->gtypes[gt];
-    int nl = g->nlat, nn = g->nlon;
-    size_t lds = sizeof(double) * ((size_t)nl * nn + 2 * (size_t)nl * nl + 2 * (size_t)nl * nl + 16);
+    const int nl = g->nlat, nn = g->nlon, mmax = g->mmax;
+    const int ncolp = (6 * n + 15) / 16 * 16, RT = (nl + 15) / 16, CT = ncolp / 16;
+    const size_t mat = (size_t)mmax * nl * ncolp;
+    RBC_TRY(c->d_shx.ensure(2 * mat));
+    double *X = c->d_shx.p, *Y = X + mat;
+    const unsigned gblk = (unsigned)(((size_t)mmax * RT * CT * 64 + 255) / 256);
     rbc_stage_begin(c, ST_SH_GLOB);
-    if (direction == 1)
-      hipLaunchKernelGGL(k_glob_anal, dim3(n), dim3(SH_THREADS), lds, c->stream, c->d_cells.p, base,
-                         (int)gt, g->d_pw.p, g->d_cs.p, d_in, c->np, d_out, c->d_tmp_i.p);
-    else
-      hipLaunchKernelGGL(k_glob_synth, dim3(n), dim3(SH_THREADS), lds, c->stream, c->d_cells.p,
-                         base, (int)gt, g->d_pg.p, g->d_cs.p, d_in, c->np, d_out, c->d_tmp_i.p);
+    if (direction == 1) {
+      k_sh_lon_anal<<<n, SH_THREADS, 0, c->stream>>>(c->d_cells.p, c->d_tmp_i.p, base, mmax,
+                                                       g->d_cs.p, d_in, c->np, ncolp, X);
+      k_sh_leg_mfma<false><<<gblk, 256, 0, c->stream>>>(nl, mmax, ncolp, RT, CT, g->d_pw.p, X, Y);
+      k_sh_pack<<<n, 256, 0, c->stream>>>(c->d_cells.p, c->d_tmp_i.p, base, nl, mmax, ncolp, Y,
+                                          d_out);
+    } else {
+      k_sh_unpack<<<n, 256, 0, c->stream>>>(c->d_cells.p, c->d_tmp_i.p, base, nl, mmax, ncolp, d_in,
+                                            X);
+      k_sh_leg_mfma<true><<<gblk, 256, 0, c->stream>>>(nl, mmax, ncolp, RT, CT, g->d_pg.p, X, Y);
+      k_sh_lon_synth<<<n, SH_THREADS, 0, c->stream>>>(c->d_cells.p, c->d_tmp_i.p, base, mmax,
+                                                      g->d_cs.p, Y, ncolp, c->np, d_out);
+    }
     rbc_stage_end(c, ST_SH_GLOB);
     HIP_TRY(hipGetLastError());
     base += n;
   }
   return RBC3D_OK;
 }
+
 
 // ---------------------------------------------------------------------------
 // Standalone ShAnalGau / ShSynthGau / ShSynthEqu on nt fields with the reference's
