@@ -154,6 +154,18 @@ int rbc3d_cell_rhs_dev(rbc3d_ctx *ctx, double *d_rhs);
  * are identical either way.  enable: 1 (default) or 0. */
 int rbc3d_set_geometry_reuse(rbc3d_ctx *ctx, int enable);
 
+/* Precision of the hot path (no counterpart in the reference, which is FP64
+ * throughout).  RBC3D_PREC_FP64, the default, is the plain form: every stage in FP64,
+ * the reference's arithmetic.  RBC3D_PREC_SPLIT is the split form: real-space pair
+ * sums in FP32 on target-bin-relative coordinates with FP32 accumulation; PME
+ * spread, FFTs, Green's function and gather on FP32 grids; the Legendre GEMMs of the
+ * global SH transforms as three-way split BF16 products on the matrix cores with
+ * FP32 accumulation; the wall self-matrix stored in FP32.  The self-patch and
+ * near-singular quadratures, the neighbour lists and projections, the periodic
+ * linear term, the surface splines and the GMRES Arnoldi process stay FP64. */
+enum { RBC3D_PREC_FP64 = 0, RBC3D_PREC_SPLIT = 1 };
+int rbc3d_set_precision(rbc3d_ctx *ctx, int mode);
+
 /* --- the two GMRES solves (PETSc KSPGMRES semantics: restart 30, classical
  *     Gram-Schmidt, no preconditioner, x += V y at restarts) ------------------ */
 /* NoSlipWall (common/ModNoSlip.F90:45-150): residual wall velocity, GMRES for the

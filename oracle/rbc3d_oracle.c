@@ -2429,7 +2429,8 @@ int orc_wall_residual_vel(orc_sys *s, double *v) {
 /* GMRES(30), PETSc KSPGMRES semantics as used by common/ModVelSolver.F90:74-90
  * and common/ModNoSlip.F90:70-87: classical Gram-Schmidt (no refinement), no
  * preconditioner, convergence when the Hessenberg residual estimate <=
- * max(rtol*||r0||, atol), explicit residual at restarts, divergence at 1e5*||r0||. */
+ * max(rtol*||r0||, atol) (KSPConvergedDefault: ||r0|| is ||b|| when the initial
+ * guess is nonzero), explicit residual at restarts, divergence at 1e5*||r0||. */
 typedef int (*orc_matvec_fn)(orc_sys *, const double *, double *);
 
 static double dotp(int n, const double *a, const double *b) {
@@ -2456,7 +2457,7 @@ static int orc_gmres(orc_sys *s, orc_matvec_fn mv, int n, const double *b, doubl
     res = sqrt(dotp(n, V, V));
     if (first) {
       rnorm0 = res;
-      ttol = fmax(rtol * rnorm0, 1e-50);
+      ttol = fmax(rtol * (x0nz ? sqrt(dotp(n, b, b)) : rnorm0), 1e-50);
     }
     first = 0;
     if (res == 0.0 || res <= ttol) {

@@ -50,6 +50,7 @@ _SIG = {
     "rbc3d_pair_stats": [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_ulonglong)],
     "rbc3d_near_stats": [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
     "rbc3d_set_geometry_reuse": [C.c_void_p, C.c_int],
+    "rbc3d_set_precision": [C.c_void_p, C.c_int],
     "rbc3d_pme_distrib_source": [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int],
     "rbc3d_pme_transform": [C.c_void_p],
     "rbc3d_pme_add_interp_vel": [C.c_void_p, C.c_int, dp, ip, dp, dp],
@@ -255,6 +256,14 @@ class System:
 
     def set_geometry_reuse(self, enable):
         _call("rbc3d_set_geometry_reuse", self.h, int(bool(enable)))
+
+    PREC_FP64 = 0
+    PREC_SPLIT = 1
+
+    def set_precision(self, mode):
+        """0: plain FP64 (default); 1: split form (FP32 pair sums and PME, split-BF16
+        Legendre GEMMs, FP32 wall self-matrix; quadratures and GMRES in FP64)."""
+        _call("rbc3d_set_precision", self.h, int(mode))
 
     def near_stats(self):
         """(projections, near-singular corrections) of the last add_int_on_rbcs."""

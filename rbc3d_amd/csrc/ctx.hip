@@ -249,6 +249,8 @@ static int make_gtype(rbc3d_ctx *c, int nlat, GridType **out) {
         mk[((size_t)a * nl + b) * nn + d] = mask_tab(mt, acos(dd) / g->radius);
       }
   RBC_TRY(upload(g->d_mask, mk, st));
+  std::vector<float> mkf(mk.begin(), mk.end());  // the split form's pair kernel
+  RBC_TRY(upload(g->d_maskf, mkf, st));
   // Spline_FindProjection (common/ModSpline.F90:203-269): in each of its 3 iterations
   // the 17-point quadratic fit of QuadFit_2D has a design fixed by the patch size h
   // alone, so its least-squares solve is done here once: c = sum_k W_k d2_k with
@@ -436,6 +438,21 @@ extern "C" int rbc3d_create(const rbc3d_conf *conf, rbc3d_ctx **out) {
   if (!r) r = upload(c->d_sl2, t2, c->stream);
   if (!r) r = upload(c->d_dl1, t3, c->stream);
   if (!r) r = upload(c->d_mtab, mt, c->stream);
+  {
+    // the same tables in FP32 for the split form's pair kernel
+    std::vector<float> tf(3 * (size_t)(RBC_NTAB + 1));
+    for (int i = 0; i <= RBC_NTAB; i++) {
+      tf[i] = (float)t1[i];
+      tf[(RBC_NTAB + 1) + i] = (float)t2[i];
+      tf[2 * (RBC_NTAB + 1) + i] = (float)t3[i];
+    }
+    if (!r) r = upload(c->d_tabf, tf, c->stream);
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+        ncu > 0)
+      c->ncu = ncu;
+  }
   // HashTable_ComputeNumBlocks (common/ModHashTable.F90:99-126), one rank
   for (int d = 0; d < 3; d++) c->Nc[d] = std::max((int)floor(c->Lb[d] / rc), 3);
   if (c->Nc[0] > 1023 || c->Nc[1] > 1023 || c->Nc[2] > 1023) {
@@ -744,6 +761,15 @@ extern "C" int rbc3d_pair_stats(rbc3d_ctx *c, long long *npairs, unsigned long l
   return RBC3D_OK;
 }
 
+extern "C" int rbc3d_set_precision(rbc3d_ctx *c, int mode) {
+  if (!c || (mode != RBC3D_PREC_FP64 && mode != RBC3D_PREC_SPLIT)) return RBC3D_EINVAL;
+  if (c->prec != mode) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->prec = mode;
+  }
+  return RBC3D_OK;
+}
+
 extern "C" int rbc3d_set_geometry_reuse(rbc3d_ctx *c, int enable) {
   if (!c) return RBC3D_EINVAL;
   c->geom_reuse = enable != 0;
@@ -794,7 +820,14 @@ extern "C" int rbc3d_pme_grid(rbc3d_ctx *c, int which, double *out) {
     std::fill(out, out + 3 * ng, 0.);
   else if (which == 1 && !c->flag_dl)
     std::fill(out, out + 9 * ng, 0.);
-  else if (which == 0)
+  else if (c->pme_grid_prec == RBC3D_PREC_SPLIT) {
+    // FP32 grids of the split form, widened on the host
+    const size_t off = which == 1 ? 3 * ng : 0, n = (which == 1 ? 9 : 3) * ng;
+    const float *src = which == 2 ? (const float *)c->d_vgrid.p : (const float *)c->d_grid.p + off;
+    std::vector<float> tmp(n);
+    HIP_TRY(hipMemcpy(tmp.data(), src, n * 4, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) out[i] = tmp[i];
+  } else if (which == 0)
     HIP_TRY(hipMemcpy(out, c->d_grid.p, 3 * ng * 8, hipMemcpyDeviceToHost));
   else if (which == 1)
     HIP_TRY(hipMemcpy(out, c->d_grid.p + 3 * ng, 9 * ng * 8, hipMemcpyDeviceToHost));

@@ -2,7 +2,9 @@
 // reference solves use: common/ModVelSolver.F90:74-90, common/ModNoSlip.F90:70-87):
 // restart 30, classical Gram-Schmidt without refinement, no preconditioner, the
 // residual estimate from the Givens-rotated Hessenberg system, convergence when the
-// estimate drops to max(rtol * ||r0||, atol), explicit residual b - A x at restarts.
+// estimate drops to max(rtol * ||r0||, atol) -- with ||r0|| = ||b|| when the initial
+// guess is nonzero, as KSPConvergedDefault does -- explicit residual b - A x at
+// restarts.
 //
 // Vectors stay in HBM; the (restart+1) x restart Hessenberg matrix and the
 // rotations live on the host (a few hundred doubles).
@@ -135,7 +137,15 @@ int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, doubl
     res = sqrt(nrm2);
     if (first) {
       rnorm0 = res;
-      ttol = std::max(rtol * rnorm0, atol);  // KSPConvergedDefault
+      double bnorm = res;
+      if (x0_nonzero) {
+        // KSPConvergedDefault with a nonzero initial guess: the tolerance is relative
+        // to the norm of the right-hand side (PCNONE: B b = b)
+        double b2;
+        RBC_TRY(dots(st, ws, n, 1, d_b, (size_t)n, d_b, &b2));
+        bnorm = sqrt(b2);
+      }
+      ttol = std::max(rtol * bnorm, atol);
       if (history) history->push_back(res);
     }
     first = false;

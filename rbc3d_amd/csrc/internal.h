@@ -1,5 +1,7 @@
 // Internal declarations of the gfx950 implementation of RBC3D's Ewald-split
-// boundary-integral matvec.  Everything here is FP64 ("plain" form).
+// boundary-integral matvec.  The plain form is FP64 throughout; the split form
+// (rbc3d_set_precision) swaps the pair sums, PME, SH Legendre GEMMs and the wall
+// self-matrix for FP32 / split-BF16 kernels over the same data.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -78,7 +80,10 @@ struct GridType {
   DBuf<double> d_Dn;     // periodic derivative matrix, length nlon, [p][q]
   DBuf<double> d_thG, d_phiG, d_pwt;  // polar patch: [node][irad + nrad*iazm], weights
   DBuf<double> d_mask;   // same-surface mask [ilat_i][ilat_j][dlon]
+  DBuf<float> d_maskf;   // the same in FP32
   DBuf<double> d_proj;   // projection tables (RBC_PJ_*)
+  // split form: the Legendre tables as three BF16 terms, [3][mmax][nlat][nlat] each
+  DBuf<unsigned short> d_pw3, d_pg3;
 };
 
 // GMRES work space (Krylov basis, partial dot products), kept across solves
@@ -130,8 +135,9 @@ struct WallData {
   // self matrix CSR (rows i + k nvert)
   DBuf<int> d_rowptr, d_col;
   DBuf<double> d_val;
+  DBuf<float> d_valf;  // split form: the 3x3 blocks in FP32
   long long nnz = 0;
-  bool prepared = false;
+  bool prepared = false, valf_ready = false;
 };
 
 struct rbc3d_ctx {
@@ -139,8 +145,11 @@ struct rbc3d_ctx {
   double Lb[3], iLb[3];
   double Acoef[8], Bcoef[8];
   hipStream_t stream = nullptr;
+  int prec = RBC3D_PREC_FP64;  // rbc3d_set_precision
+  int ncu = 256;               // compute units of the device
   // tables
   DBuf<double> d_sl1, d_sl2, d_dl1, d_mtab;
+  DBuf<float> d_tabf;  // split form: sl1, sl2, dl1 in FP32, (RBC_NTAB + 1) each
   double r_eps;
   // cell list geometry
   int Nc[3];
@@ -185,11 +194,13 @@ struct rbc3d_ctx {
   // PME
   bool pme_ready = false;
   int flag_sl = 0, flag_dl = 0;
+  int pme_grid_prec = RBC3D_PREC_FP64;  // scalar of the grids the last transform wrote
   DBuf<double> d_grid;   // real grids: up to 12 fields (ff 3 | tt 9), later vv 3
   DBuf<double> d_cgrid;  // complex grids (interleaved) up to 12 fields
   DBuf<double> d_vgrid;  // velocity grid 3 fields
   DBuf<double> d_bb;     // B-factor (kx,ky,kz)
   DBuf<double> d_tw;     // FFT twiddles for the three lengths
+  DBuf<float> d_twf;     // the same in FP32 (split form)
   int tw_off[3];
   DBuf<int> d_pme_key, d_pme_order, d_pme_start;
   DBuf<double> d_pme_src;  // sorted spread records
@@ -199,6 +210,7 @@ struct rbc3d_ctx {
   WallData wall;
   // per-grid-type table pointers for the kernels
   DBuf<const double *> d_tp_ptr;
+  DBuf<const float *> d_tp_maskf;  // FP32 mask table per grid type
   DBuf<int> d_tp_int;
   size_t tp_ntypes = 0;
   StageTimer timer;

@@ -603,6 +603,354 @@ __global__ void __launch_bounds__(PT *PW) k_pair(PairArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Split form of the pair kernel: FP32 arithmetic and accumulation.
+//
+// Coordinates are taken relative to the centre of the workgroup's target bin in FP64
+// (minimum image about that centre) and split into an FP32 head and an FP32 tail, so
+// the FP32 difference of two nearby points is as accurate as the FP64 one; the
+// kernel values, products and sums are FP32.  Pairs closer than RBC_SPLIT_NEAR rc
+// (about 1.5% of the pairs of a dense packing) are evaluated in FP64 into an FP64
+// accumulator: their terms are the largest, and the near-singular correction
+// subtracts their FP64 values again, so an FP32 error there would not cancel.  Each
+// wavefront first tests its 64 sources of a tile against its target with the heads
+// alone and keeps the in-cutoff ones as two bit masks (far, near); it then evaluates
+// only those, so the loop that evaluates the Stokeslet / stresslet runs as many
+// times as the busiest lane has in-cutoff sources instead of once per candidate.  The Ewald tables (FP32) are held
+// in LDS; workgroups are persistent, stride over the target chunks, and each XCD
+// takes one contiguous range of chunks (neighbouring chunks share source tiles in
+// that XCD's L2).
+#define PS32 (64 * PW)  // sources per tile: 64 per wavefront
+#define RBC_SPLIT_NEAR 0.25  // pairs with r < RBC_SPLIT_NEAR rc run in FP64
+
+struct Pair32Args {
+  double Lb[3], iLb[3], w[3];  // box, 1/box, fine-bin width
+  float fLb[3], fiLb[3];
+  int Nf[3];
+  float rc2, reps2, tabscale, c1, c2;
+  double c1d, c2d, rcd;
+  float near2;    // (RBC_SPLIT_NEAR rc)^2
+  double reps2d;  // r_eps^2 for the FP64 near pairs
+  const double *sl1, *sl2, *dl1;  // FP64 tables for the near pairs
+  int wrap;  // 1: the box is too small for the minimum image about the bin centre
+  int np, nT, nfbins, nrbc, nrows;
+  const int *rows;
+  const double *rec;
+  const int *src_node;
+  const int *bin_start;
+  const CellRec *cells;
+  const float *tab;  // sl1 | sl2 | dl1 (FP32), RBC_NTAB + 1 each
+  const double *const *mask;
+  const float *const *maskf;
+  const double *tx, *tA;
+  const int *tindx, *tact, *torder, *tbin_start, *chunk_off;
+  double *vpair;
+  unsigned long long *stat;
+};
+
+// split x (FP64) into head + tail (FP32)
+__device__ __forceinline__ void split_f(double x, float &h, float &l) {
+  h = (float)x;
+  l = (float)(x - (double)h);
+}
+
+// Ewald table value at s = r N / rc by linear interpolation from an LDS table; s is
+// clamped to the last interval (at r = rc the coefficient is erfc(xi rc) ~ 1e-8 of
+// the near field, where the FP64 form returns zero)
+__device__ __forceinline__ float tab_interp(const float *t, float st) {
+  const int it = min((int)st, RBC_NTAB - 1);
+  const float fr = st - (float)it;
+  return fmaf(fr, t[it + 1] - t[it], t[it]);
+}
+
+// The pass-1 masks of one 32-source half slice.  Positions beyond the tile are
+// parked far away (+1e30) and inactive targets at -1e30, so without the wrap no
+// bound test is needed; with the wrap (tiny boxes) both are tested explicitly.
+template <bool WRAP>
+__device__ __forceinline__ void pair_masks(const float4 *__restrict__ sph, int j0, int jn, bool act,
+                                           float xh0, float xh1, float xh2, float rc2, float near2,
+                                           const float *fLb, const float *fiLb, unsigned &mf,
+                                           unsigned &mn) {
+  mf = 0;
+  mn = 0;
+#pragma unroll
+  for (int q = 0; q < 32; q++) {
+    const float4 p = sph[j0 + q];
+    float d0 = p.x - xh0, d1 = p.y - xh1, d2 = p.z - xh2;
+    if (WRAP) {
+      d0 -= rintf(d0 * fiLb[0]) * fLb[0];
+      d1 -= rintf(d1 * fiLb[1]) * fLb[1];
+      d2 -= rintf(d2 * fiLb[2]) * fLb[2];
+    }
+    const float s2 = fmaf(d2, d2, fmaf(d1, d1, d0 * d0));
+    bool in = s2 <= rc2;
+    if (WRAP) in = in && act && q < jn;
+    mf |= (in && s2 >= near2) ? (1u << q) : 0u;
+    mn |= (in && s2 < near2) ? (1u << q) : 0u;
+  }
+}
+
+template <bool SL, bool DL, bool WRAP>
+__global__ void __launch_bounds__(PT *PW) k_pair32(Pair32Args a) {
+  constexpr int NTB = (SL ? 2 : 0) + (DL ? 1 : 0);
+  constexpr int TDL = SL ? 2 : 0;  // LDS slot of the DL table
+  __shared__ float tab[NTB > 0 ? NTB : 1][RBC_NTAB + 1];
+  // head of the relative position (w: source cell as int bits) and its tail
+  // (w: ilat | ilon << 16 as int bits)
+  __shared__ float4 spos[2][PS32];
+  float4 *sph = spos[0], *spl = spos[1];
+  __shared__ float4 sfv[SL ? PS32 : 1];   // f dS
+  __shared__ float4 sgn[DL ? 2 * PS32 : 1];  // (g dS Bcoef).xyz, a3.x | a3.yz
+  // the wavefronts' partial sums reuse the position buffers once a chunk's tiles are done
+  static_assert(sizeof(double) * (PW - 1) * 3 * PT <= sizeof(spos), "red in spos");
+  double(*red)[3][PT] = reinterpret_cast<double(*)[3][PT]>(&spos[0][0]);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int t = threadIdx.x; t < NTB * (RBC_NTAB + 1); t += PT * PW) {
+    const int q = t / (RBC_NTAB + 1), e = t - q * (RBC_NTAB + 1);
+    const int g = SL ? q : 2;  // global table: sl1, sl2, dl1
+    tab[q][e] = a.tab[(size_t)g * (RBC_NTAB + 1) + e];
+  }
+  const int nb = a.nfbins;
+  const int nchunk = a.chunk_off[nb];
+  const int xcd = (int)(blockIdx.x % RBC_NXCD), nper = (int)(gridDim.x / RBC_NXCD);
+  const int per = (nchunk + RBC_NXCD - 1) / RBC_NXCD;
+  const int c_lo = xcd * per, c_hi = min(nchunk, c_lo + per);
+  const int Nf0 = a.Nf[0], Nf1 = a.Nf[1];
+  const size_t np = a.np;
+  const double tsd = (double)RBC_NTAB / a.rcd;
+  unsigned long long npairs = 0;
+  for (int blk = c_lo + (int)(blockIdx.x / RBC_NXCD); blk < c_hi; blk += nper) {
+    int lo = 0, hi = nb;  // chunk_off[lo] <= blk < chunk_off[hi]
+    while (hi - lo > 1) {
+      const int mid = (lo + hi) >> 1;
+      if (a.chunk_off[mid] <= blk)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    const int bin = lo, chunk = blk - a.chunk_off[lo];
+    const int b1 = bin % Nf0, b2 = (bin / Nf0) % Nf1, b3 = bin / (Nf0 * Nf1);
+    const double ctr0 = (b1 + 0.5) * a.w[0], ctr1 = (b2 + 0.5) * a.w[1], ctr2 = (b3 + 0.5) * a.w[2];
+    const int t0 = a.tbin_start[bin] + chunk * PT, t1 = a.tbin_start[bin + 1];
+    const int k = t0 + lane;
+    const bool have = k < t1;
+    const int ti = have ? a.torder[k] : 0;
+    const bool act = have && a.tact[ti] != 0;
+    float xh0 = -1e30f, xh1 = -1e30f, xh2 = -1e30f, xl0 = 0, xl1 = 0, xl2 = 0, cA1 = 0, cA2 = 0;
+    double cA1d = 0, cA2d = 0;
+    int ilat_i = 0, ilon_i = 0, nl_i = 1, nn_i = 1, cell_i = -1;
+    const float *mtab = nullptr;
+    const double *mtabd = nullptr;
+    if (act) {
+      double r0 = a.tx[ti] - ctr0, r1 = a.tx[ti + (size_t)a.nT] - ctr1,
+             r2 = a.tx[ti + 2 * (size_t)a.nT] - ctr2;
+      r0 -= rint(r0 * a.iLb[0]) * a.Lb[0];
+      r1 -= rint(r1 * a.iLb[1]) * a.Lb[1];
+      r2 -= rint(r2 * a.iLb[2]) * a.Lb[2];
+      split_f(r0, xh0, xl0);
+      split_f(r1, xh1, xl1);
+      split_f(r2, xh2, xl2);
+      const double Ai = a.tA[ti];
+      cA1d = a.c1d / Ai;
+      cA2d = a.c2d / Ai;
+      cA1 = (float)cA1d;
+      cA2 = (float)cA2d;
+      const int surf_i = a.tindx[ti];
+      if (surf_i >= 1 && surf_i <= a.nrbc) {
+        const CellRec &cr = a.cells[surf_i - 1];
+        cell_i = surf_i - 1;
+        nl_i = cr.nlat;
+        nn_i = cr.nlon;
+        mtab = a.maskf[cr.gtype];
+        mtabd = a.mask[cr.gtype];
+        ilat_i = a.tindx[ti + a.nT] - 1;
+        ilon_i = a.tindx[ti + 2 * (size_t)a.nT] - 1;
+      }
+    }
+    float acc0 = 0, acc1 = 0, acc2 = 0;
+    double accd0 = 0, accd1 = 0, accd2 = 0;
+    for (int ri = 0; ri < a.nrows; ri++) {
+      const unsigned code = (unsigned)a.rows[ri];
+      const int alo = (int)(code & 255) - 128, ahi = (int)((code >> 8) & 255) - 128;
+      const int o2 = (int)((code >> 16) & 255) - 128, o3 = (int)(code >> 24) - 128;
+      const int rowbase = Nf0 * (imod_d(b2 + o2, Nf1) + Nf1 * imod_d(b3 + o3, a.Nf[2]));
+      int rlo[2], rhi[2], nrun = 1;
+      if (ahi - alo + 1 >= Nf0) {
+        rlo[0] = 0;
+        rhi[0] = Nf0 - 1;
+      } else {
+        const int x0 = b1 + alo, x1 = b1 + ahi;
+        if (x0 < 0) {
+          rlo[0] = x0 + Nf0, rhi[0] = Nf0 - 1, rlo[1] = 0, rhi[1] = x1, nrun = 2;
+        } else if (x1 >= Nf0) {
+          rlo[0] = x0, rhi[0] = Nf0 - 1, rlo[1] = 0, rhi[1] = x1 - Nf0, nrun = 2;
+        } else {
+          rlo[0] = x0, rhi[0] = x1;
+        }
+      }
+      for (int ru = 0; ru < nrun; ru++) {
+        const int s0 = a.bin_start[rowbase + rlo[ru]], s1 = a.bin_start[rowbase + rhi[ru] + 1];
+        for (int base = s0; base < s1; base += PS32) {
+          const int cnt = min(PS32, s1 - base);
+          __syncthreads();
+          {
+            const int t = threadIdx.x;  // one source per thread (PS32 = threads)
+            if (t < cnt) {
+              const int q = base + t;
+              double r0 = a.rec[q] - ctr0, r1 = a.rec[q + np] - ctr1, r2 = a.rec[q + 2 * np] - ctr2;
+              r0 -= rint(r0 * a.iLb[0]) * a.Lb[0];
+              r1 -= rint(r1 * a.iLb[1]) * a.Lb[1];
+              r2 -= rint(r2 * a.iLb[2]) * a.Lb[2];
+              float4 h, l;
+              split_f(r0, h.x, l.x);
+              split_f(r1, h.y, l.y);
+              split_f(r2, h.z, l.z);
+              h.w = __int_as_float(a.src_node[q + np]);
+              l.w = __int_as_float(a.src_node[q + 2 * np]);
+              sph[t] = h;
+              spl[t] = l;
+              if (SL)
+                sfv[t] = make_float4((float)a.rec[q + np * 3], (float)a.rec[q + np * 4],
+                                     (float)a.rec[q + np * 5], 0.f);
+              if (DL) {
+                const double Bj = a.rec[q + np * 12];
+                sgn[2 * t] = make_float4((float)(a.rec[q + np * 6] * Bj),
+                                         (float)(a.rec[q + np * 7] * Bj),
+                                         (float)(a.rec[q + np * 8] * Bj), (float)a.rec[q + np * 9]);
+                sgn[2 * t + 1] = make_float4((float)a.rec[q + np * 10], (float)a.rec[q + np * 11],
+                                             0.f, 0.f);
+              }
+            } else {
+              sph[t] = make_float4(1e30f, 1e30f, 1e30f, __int_as_float(-2));
+            }
+          }
+          __syncthreads();
+          const int jb = wv * 64;
+          for (int half = 0; half < 2; half++) {
+            const int j0 = jb + 32 * half;
+            unsigned mf, mn;
+            pair_masks<WRAP>(sph, j0, cnt - j0, act, xh0, xh1, xh2, a.rc2, a.near2, a.fLb, a.fiLb,
+                             mf, mn);
+            npairs += (unsigned)(__popc(mf) + __popc(mn));
+            // far pairs: FP32
+            while (mf) {
+              const int j = j0 + __builtin_ctz(mf);
+              mf &= mf - 1;
+              const float4 ph = sph[j], pl = spl[j];
+              float d0 = (ph.x - xh0) + (pl.x - xl0), d1 = (ph.y - xh1) + (pl.y - xl1),
+                    d2 = (ph.z - xh2) + (pl.z - xl2);
+              if (WRAP) {
+                d0 -= rintf(d0 * a.fiLb[0]) * a.fLb[0];
+                d1 -= rintf(d1 * a.fiLb[1]) * a.fLb[1];
+                d2 -= rintf(d2 * a.fiLb[2]) * a.fLb[2];
+              }
+              const float s2 = fmaf(d2, d2, fmaf(d1, d1, d0 * d0));
+              // 1/r from the hardware estimate and one Newton step
+              float ir = __builtin_amdgcn_rsqf(s2);
+              ir = ir * fmaf(-0.5f * s2 * ir, ir, 1.5f);
+              const float st = s2 * ir * a.tabscale;
+              float wm = 1.f;
+              if (__float_as_int(ph.w) == cell_i) {
+                const int ll = __float_as_int(pl.w);
+                int dl = (ll >> 16) - ilon_i;
+                dl += dl < 0 ? nn_i : 0;
+                wm = 1.f - mtab[((size_t)ilat_i * nl_i + (ll & 0xffff)) * nn_i + dl];
+              }
+              const float ir2 = ir * ir;
+              if (SL) {
+                const float c1v = tab_interp(tab[0], st), c2v = tab_interp(tab[1], st);
+                const float EA = fmaf(c1v * ir, ir2, c2v * ir2), EB = fmaf(c1v, ir, -c2v);
+                const float4 f = sfv[j];
+                const float xf = fmaf(d2, f.z, fmaf(d1, f.y, d0 * f.x));
+                const float cc = wm * cA1, ax = EA * xf;
+                acc0 = fmaf(cc, fmaf(ax, d0, EB * f.x), acc0);
+                acc1 = fmaf(cc, fmaf(ax, d1, EB * f.y), acc1);
+                acc2 = fmaf(cc, fmaf(ax, d2, EB * f.z), acc2);
+              }
+              if (DL) {
+                const float EA = tab_interp(tab[TDL], st) * (ir2 * ir2 * ir);
+                const float4 g = sgn[2 * j], n2 = sgn[2 * j + 1];
+                const float xg = fmaf(d2, g.z, fmaf(d1, g.y, d0 * g.x));
+                const float xn = fmaf(d2, n2.y, fmaf(d1, n2.x, d0 * g.w));
+                const float cc = wm * cA2 * EA * xg * xn;
+                acc0 = fmaf(cc, d0, acc0);
+                acc1 = fmaf(cc, d1, acc1);
+                acc2 = fmaf(cc, d2, acc2);
+              }
+            }
+            // near pairs: FP64 (heads and tails recombined exactly), FP64 tables
+            while (mn) {
+              const int j = j0 + __builtin_ctz(mn);
+              mn &= mn - 1;
+              const float4 ph = sph[j], pl = spl[j];
+              double d0 = ((double)ph.x - (double)xh0) + ((double)pl.x - (double)xl0);
+              double d1 = ((double)ph.y - (double)xh1) + ((double)pl.y - (double)xl1);
+              double d2 = ((double)ph.z - (double)xh2) + ((double)pl.z - (double)xl2);
+              if (WRAP) {
+                d0 -= rint(d0 * a.iLb[0]) * a.Lb[0];
+                d1 -= rint(d1 * a.iLb[1]) * a.Lb[1];
+                d2 -= rint(d2 * a.iLb[2]) * a.Lb[2];
+              }
+              const double s2 = d0 * d0 + d1 * d1 + d2 * d2;
+              if (s2 < a.reps2d) continue;  // below r_eps the Ewald coefficients are zero
+              const double ir = rsq_nr(s2), rr = s2 * ir;
+              double wm = 1.;
+              if (__float_as_int(ph.w) == cell_i) {
+                const int ll = __float_as_int(pl.w);
+                int dl = (ll >> 16) - ilon_i;
+                dl += dl < 0 ? nn_i : 0;
+                wm = 1. - mtabd[((size_t)ilat_i * nl_i + (ll & 0xffff)) * nn_i + dl];
+              }
+              if (SL) {
+                double EA, EB;
+                ewald_sl_ir(a.sl1, a.sl2, rr, ir, tsd, EA, EB);
+                const float4 f = sfv[j];
+                const double xf = d0 * f.x + d1 * f.y + d2 * f.z;
+                const double cc = wm * cA1d;
+                accd0 += cc * (EA * d0 * xf + EB * f.x);
+                accd1 += cc * (EA * d1 * xf + EB * f.y);
+                accd2 += cc * (EA * d2 * xf + EB * f.z);
+              }
+              if (DL) {
+                const double EA = ewald_dl_ir(a.dl1, rr, ir, tsd);
+                const float4 g = sgn[2 * j], n2 = sgn[2 * j + 1];
+                const double xg = d0 * g.x + d1 * g.y + d2 * g.z;
+                const double xn = d0 * g.w + d1 * n2.x + d2 * n2.y;
+                const double cc = wm * cA2d * EA * xg * xn;
+                accd0 += cc * d0;
+                accd1 += cc * d1;
+                accd2 += cc * d2;
+              }
+            }
+          }
+        }
+      }
+    }
+    __syncthreads();  // every wavefront is done with the position buffers
+    accd0 += acc0;
+    accd1 += acc1;
+    accd2 += acc2;
+    if (wv > 0) {
+      red[wv - 1][0][lane] = accd0;
+      red[wv - 1][1][lane] = accd1;
+      red[wv - 1][2][lane] = accd2;
+    }
+    __syncthreads();
+    if (wv == 0 && act) {
+      for (int w = 0; w < PW - 1; w++) {
+        accd0 += red[w][0][lane];
+        accd1 += red[w][1][lane];
+        accd2 += red[w][2][lane];
+      }
+      a.vpair[ti] = accd0;
+      a.vpair[ti + (size_t)a.nT] = accd1;
+      a.vpair[ti + 2 * (size_t)a.nT] = accd2;
+    }
+  }
+  for (int o = 32; o > 0; o >>= 1) npairs += __shfl_xor(npairs, o, 64);
+  if (lane == 0) atomicAdd(&a.stat[0], npairs);
+}
+
+// ---------------------------------------------------------------------------
 // neighbouring-cell lists: for every target and every other cell with a node in the
 // cut-off, the closest node (ties: first visited, i.e. lowest 27-bin visit position
 // of the reference's cell list, then highest source index), NbrRbcList_Insert.
@@ -1244,6 +1592,7 @@ __global__ void k_combine(CombineArgs a) {
 struct TypePtrs {
   const double *const *mask, *const *th, *const *phi, *const *thG, *const *phiG, *const *pwt,
       *const *proj;
+  const float *const *maskf;
   const int *nrad, *nazm;
 };
 static int type_ptrs(rbc3d_ctx *c, TypePtrs *tp) {
@@ -1263,6 +1612,10 @@ static int type_ptrs(rbc3d_ctx *c, TypePtrs *tp) {
       ni[i] = g->nrad;
       ni[nt + i] = g->nazm;
     }
+    std::vector<const float *> pf(nt);
+    for (size_t i = 0; i < nt; i++) pf[i] = c->gtypes[i]->d_maskf.p;
+    RBC_TRY(c->d_tp_maskf.ensure(nt));
+    HIP_TRY(hipMemcpy(c->d_tp_maskf.p, pf.data(), nt * sizeof(pf[0]), hipMemcpyHostToDevice));
     RBC_TRY(c->d_tp_ptr.ensure(ptr.size()));
     RBC_TRY(c->d_tp_int.ensure(ni.size()));
     HIP_TRY(hipMemcpy(c->d_tp_ptr.p, ptr.data(), ptr.size() * sizeof(ptr[0]), hipMemcpyHostToDevice));
@@ -1277,6 +1630,7 @@ static int type_ptrs(rbc3d_ctx *c, TypePtrs *tp) {
   tp->phiG = p + 4 * nt;
   tp->pwt = p + 5 * nt;
   tp->proj = p + 6 * nt;
+  tp->maskf = c->d_tp_maskf.p;
   tp->nrad = c->d_tp_int.p;
   tp->nazm = c->d_tp_int.p + nt;
   return RBC3D_OK;
@@ -1287,6 +1641,82 @@ static int exclusive_scan(rbc3d_ctx *c, const int *in, int *out, int n) {
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, in, out, n, c->stream));
   RBC_TRY(c->d_cub_tmp.ensure(tmp + 16));
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->d_cub_tmp.p, tmp, in, out, n, c->stream));
+  return RBC3D_OK;
+}
+
+// launch the split-form pair kernel with the arguments of the FP64 one: persistent
+// workgroups, as many per CU as the LDS (tables + tiles) allows
+static int launch_pair32(rbc3d_ctx *c, const PairArgs &pa, int nblk_max,
+                         const float *const *maskf) {
+  Pair32Args a;
+  bool wrap = false;
+  for (int d = 0; d < 3; d++) {
+    a.Lb[d] = pa.Lb[d];
+    a.iLb[d] = pa.iLb[d];
+    a.w[d] = c->Lb[d] / c->Nf[d];
+    a.fLb[d] = (float)pa.Lb[d];
+    a.fiLb[d] = (float)pa.iLb[d];
+    a.Nf[d] = pa.Nf[d];
+    // sources within rc of a target of the bin lie within rc + w/2 of its centre
+    wrap = wrap || !(c->Lb[d] - a.w[d] > 2 * c->conf.rc * (1 + 1e-6));
+  }
+  a.wrap = wrap;
+  a.rc2 = (float)(c->conf.rc * c->conf.rc);
+  a.reps2 = (float)(c->r_eps * c->r_eps);
+  a.tabscale = (float)pa.tabscale;
+  a.c1 = (float)pa.c1;
+  a.c2 = (float)pa.c2;
+  a.c1d = pa.c1;
+  a.c2d = pa.c2;
+  a.rcd = c->conf.rc;
+  a.near2 = (float)(RBC_SPLIT_NEAR * RBC_SPLIT_NEAR * c->conf.rc * c->conf.rc);
+  a.reps2d = c->r_eps * c->r_eps;
+  a.sl1 = pa.sl1;
+  a.sl2 = pa.sl2;
+  a.dl1 = pa.dl1;
+  a.np = pa.np;
+  a.nT = pa.nT;
+  a.nfbins = pa.nfbins;
+  a.nrbc = pa.nrbc;
+  a.nrows = pa.nrows;
+  a.rows = pa.rows;
+  a.rec = pa.rec;
+  a.src_node = pa.src_node;
+  a.bin_start = pa.bin_start;
+  a.cells = pa.cells;
+  a.tab = c->d_tabf.p;
+  a.mask = pa.mask;
+  a.maskf = maskf;
+  a.tx = pa.tx;
+  a.tA = pa.tA;
+  a.tindx = pa.tindx;
+  a.tact = pa.tact;
+  a.torder = pa.torder;
+  a.tbin_start = pa.tbin_start;
+  a.chunk_off = pa.chunk_off;
+  a.vpair = pa.vpair;
+  a.stat = pa.stat;
+  const bool sl = pa.c1 != 0, dl = pa.c2 != 0;
+  const int per_cu = sl && dl ? 1 : (sl ? 2 : 3);
+  int grid = std::min(nblk_max, c->ncu * per_cu);
+  grid = (grid + RBC_NXCD - 1) / RBC_NXCD * RBC_NXCD;
+  hipStream_t st = c->stream;
+  if (wrap) {
+    if (sl && dl)
+      k_pair32<true, true, true><<<grid, PT * PW, 0, st>>>(a);
+    else if (sl)
+      k_pair32<true, false, true><<<grid, PT * PW, 0, st>>>(a);
+    else if (dl)
+      k_pair32<false, true, true><<<grid, PT * PW, 0, st>>>(a);
+  } else {
+    if (sl && dl)
+      k_pair32<true, true, false><<<grid, PT * PW, 0, st>>>(a);
+    else if (sl)
+      k_pair32<true, false, false><<<grid, PT * PW, 0, st>>>(a);
+    else if (dl)
+      k_pair32<false, true, false><<<grid, PT * PW, 0, st>>>(a);
+  }
+  HIP_TRY(hipGetLastError());
   return RBC3D_OK;
 }
 
@@ -1362,7 +1792,9 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   pa.stat = c->d_pairstat.p;
   int nblk = nb + (nT + PT - 1) / PT + 1;
   rbc_stage_begin(c, ST_PAIR);
-  if (c1 != 0 && c2 != 0)
+  if (c->prec == RBC3D_PREC_SPLIT) {
+    RBC_TRY(launch_pair32(c, pa, nblk, tp->maskf));
+  } else if (c1 != 0 && c2 != 0)
     k_pair<true, true><<<nblk, PT * PW, 0, st>>>(pa);
   else if (c1 != 0)
     k_pair<true, false><<<nblk, PT * PW, 0, st>>>(pa);

@@ -9,6 +9,9 @@
 //              keeping the real part of the zero and Nyquist x-frequencies;
 //   Green   -- elementwise k-space multiply with the B-factor;
 //   gather  -- one lane per target, P^3 stencil.
+// Every stage is templated on the grid scalar: double for the plain form, float for
+// the split form (same kernels, half the bytes; the B-spline weights and the
+// Green's-function factors are formed in FP64 and rounded).
 #include <hipcub/hipcub.hpp>
 
 #include <algorithm>
@@ -21,6 +24,17 @@
 #define SPR_T 256
 #define FFT_T 256
 #define FFT_ELEMS 2048 // complex elements per FFT workgroup (lines x length)
+
+template <typename T>
+struct CxT;
+template <>
+struct CxT<double> {
+  typedef double2 t;
+};
+template <>
+struct CxT<float> {
+  typedef float2 t;
+};
 
 // ---------------------------------------------------------------------------
 // B-spline weights: BsplineFunc (common/ModBasicMath.F90:392-417)
@@ -126,6 +140,9 @@ int rbc_pme_init(rbc3d_ctx *c) {
   }
   RBC_TRY(c->d_tw.ensure(tw.size()));
   HIP_TRY(hipMemcpy(c->d_tw.p, tw.data(), tw.size() * 8, hipMemcpyHostToDevice));
+  std::vector<float> twf(tw.begin(), tw.end());
+  RBC_TRY(c->d_twf.ensure(twf.size()));
+  HIP_TRY(hipMemcpy(c->d_twf.p, twf.data(), twf.size() * 4, hipMemcpyHostToDevice));
   c->pme_ready = true;
   return RBC3D_OK;
 }
@@ -134,6 +151,7 @@ int rbc_pme_init(rbc3d_ctx *c) {
 // spread
 // record layout (doubles): [0..2] imin,jmin,kmin (as doubles) [3..3+3P) weights,
 // then nf field values (c1*f for single layer, c2*g_i*(a3_j Bcoef) for double layer)
+template <typename R>
 __global__ void k_spread_prep(int ns, int ncell, const double *__restrict__ x, int ldx,
                               const double *__restrict__ fs, const double *__restrict__ gs,
                               const double *__restrict__ a3, const double *__restrict__ Bn,
@@ -141,7 +159,7 @@ __global__ void k_spread_prep(int ns, int ncell, const double *__restrict__ x, i
                               int nele, const double *__restrict__ wx, const double *__restrict__ wf,
                               const int *__restrict__ e2v, int nvert, const double *__restrict__ warea,
                               double ih0, double ih1, double ih2, int N1, int N2, int N3, int P,
-                              int flag_sl, int flag_dl, int nbx, int nby, int nbz, double *rec,
+                              int flag_sl, int flag_dl, int nbx, int nby, int nbz, R *rec,
                               int reclen, int *key, int *idx) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= ns) return;
@@ -175,7 +193,7 @@ __global__ void k_spread_prep(int ns, int ncell, const double *__restrict__ x, i
     if (flag_sl)
       for (int d = 0; d < 3; d++) f[d] = (1.0 / 3) * (fe[0][d] + fe[1][d] + fe[2][d]) * warea[e];
   }
-  double *r = rec + (size_t)reclen * p;
+  R *r = rec + (size_t)reclen * p;
   double w[12];
   int imin = bspline_d(xs[0] * ih0, P, w);
   for (int q = 0; q < P; q++) r[3 + q] = w[q];
@@ -196,14 +214,15 @@ __global__ void k_spread_prep(int ns, int ncell, const double *__restrict__ x, i
   (void)nbz;
 }
 
+template <typename T>
 struct SpreadArgs {
   int N1, N2, N3, P, nbx, nby, nbz, reclen, ns;
   FastDiv frl;  // LDS record length
-  double c1, c2;
+  T c1, c2;
   int flag_sl, flag_dl;
-  const double *rec;
+  const T *rec;
   const int *order, *brick_start;
-  double *grid;  // 12 fields [f][k][j][i]
+  T *grid;  // 12 fields [f][k][j][i]
 };
 
 // One workgroup per brick of the grid.  A source with stencil origin g covers
@@ -212,11 +231,11 @@ struct SpreadArgs {
 // SL / DL select the single-layer (3) and double-layer (9) fields; sources stream
 // through LDS in tiles of SPR_TILE records holding only what the fields need.
 #define SPR_TILE 64
-template <bool SL, bool DL>
-__global__ void __launch_bounds__(SPR_T) k_spread(SpreadArgs a) {
+template <bool SL, bool DL, typename T>
+__global__ void __launch_bounds__(SPR_T) k_spread(SpreadArgs<T> a) {
   constexpr int NF = (SL ? 3 : 0) + (DL ? 9 : 0);
   constexpr int MAXP = 12;
-  __shared__ double srec[SPR_TILE * (3 + 3 * MAXP + NF)];
+  __shared__ T srec[SPR_TILE * (3 + 3 * MAXP + NF)];
   const int bid = blockIdx.x;
   const int bx = bid % a.nbx, by = (bid / a.nbx) % a.nby, bz = bid / (a.nbx * a.nby);
   const int ib0 = bx * SPR_B, jb0 = by * SPR_B, kb0 = bz * SPR_B;
@@ -238,7 +257,7 @@ __global__ void __launch_bounds__(SPR_T) k_spread(SpreadArgs a) {
   }
   // my grid points (two per lane for B = 8)
   const int npts = SPR_B * SPR_B * SPR_B;
-  double acc[2][NF > 0 ? NF : 1];
+  T acc[2][NF > 0 ? NF : 1];
   int gi[2], gj[2], gk[2];
   bool valid[2];
   for (int s = 0; s < 2; s++) {
@@ -266,7 +285,7 @@ __global__ void __launch_bounds__(SPR_T) k_spread(SpreadArgs a) {
           }
           __syncthreads();
           for (int q = 0; q < cnt; q++) {
-            const double *r = srec + q * rl;
+            const T *r = srec + q * rl;
             const int i0 = (int)r[0], j0 = (int)r[1], k0 = (int)r[2];
             for (int s = 0; s < 2; s++) {
               if (!valid[s]) continue;
@@ -275,14 +294,14 @@ __global__ void __launch_bounds__(SPR_T) k_spread(SpreadArgs a) {
               dj += dj < 0 ? a.N2 : 0;
               dk += dk < 0 ? a.N3 : 0;
               if (di >= P || dj >= P || dk >= P) continue;
-              const double wxyz = r[3 + di] * r[3 + P + dj] * r[3 + 2 * P + dk];
-              const double *fv = r + 3 + 3 * P;
+              const T wxyz = r[3 + di] * r[3 + P + dj] * r[3 + 2 * P + dk];
+              const T *fv = r + 3 + 3 * P;
               if (SL) {
-                const double cw = a.c1 * wxyz;
+                const T cw = a.c1 * wxyz;
                 for (int f = 0; f < 3; f++) acc[s][f] += cw * fv[f];
               }
               if (DL) {
-                const double cw = a.c2 * wxyz;
+                const T cw = a.c2 * wxyz;
                 constexpr int o = SL ? 3 : 0;
                 for (int f = 0; f < 9; f++) acc[s][o + f] += cw * fv[o + f];
               }
@@ -313,7 +332,8 @@ __global__ void k_brick_start(int n, const int *__restrict__ skey, int nbr, int 
   start[b] = lo;
 }
 
-int rbc_pme_spread(rbc3d_ctx *c, double c1, double c2, int cells, int walls) {
+template <typename T>
+static int pme_spread_t(rbc3d_ctx *c, double c1, double c2, int cells, int walls) {
   hipStream_t st = c->stream;
   int N1 = c->conf.Nb[0], N2 = c->conf.Nb[1], N3 = c->conf.Nb[2], P = c->conf.P;
   c->flag_sl = fabs(c1) > 1.e-10;
@@ -332,11 +352,11 @@ int rbc_pme_spread(rbc3d_ctx *c, double c1, double c2, int cells, int walls) {
   int *idx = c->d_pme_order.p + ns + 1, *order = c->d_pme_order.p;
   if (ns > 0) {
     rbc_stage_begin(c, ST_SPREAD_PREP);
-    k_spread_prep<<<(ns + 255) / 256, 256, 0, st>>>(
+    k_spread_prep<T><<<(ns + 255) / 256, 256, 0, st>>>(
         ns, ncell, c->d_x.p, c->np, c->d_fs.p, c->d_gs.p, c->d_a3.p, nullptr, c->d_node_cell.p,
         c->d_cells.p, nele, c->wall.d_x.p, c->wall.d_f.p, c->wall.d_e2v.p, c->wall.nvert,
         c->wall.d_area.p, N1 / c->Lb[0], N2 / c->Lb[1], N3 / c->Lb[2], N1, N2, N3, P, c->flag_sl,
-        c->flag_dl, nbx, nby, nbz, c->d_pme_src.p, reclen, key, idx);
+        c->flag_dl, nbx, nby, nbz, (T *)c->d_pme_src.p, reclen, key, idx);
     rbc_stage_end(c, ST_SPREAD_PREP);
     HIP_TRY(hipGetLastError());
     int bits = 1;
@@ -349,7 +369,7 @@ int rbc_pme_spread(rbc3d_ctx *c, double c1, double c2, int cells, int walls) {
   }
   k_brick_start<<<(nbr + 1 + 255) / 256, 256, 0, st>>>(ns, key2, nbr, c->d_pme_start.p);
   HIP_TRY(hipGetLastError());
-  SpreadArgs a;
+  SpreadArgs<T> a;
   a.N1 = N1;
   a.N2 = N2;
   a.N3 = N3;
@@ -359,14 +379,14 @@ int rbc_pme_spread(rbc3d_ctx *c, double c1, double c2, int cells, int walls) {
   a.nbz = nbz;
   a.reclen = reclen;
   a.ns = ns;
-  a.c1 = c1;
-  a.c2 = c2;
+  a.c1 = (T)c1;
+  a.c2 = (T)c2;
   a.flag_sl = c->flag_sl;
   a.flag_dl = c->flag_dl;
-  a.rec = c->d_pme_src.p;
+  a.rec = (const T *)c->d_pme_src.p;
   a.order = order;
   a.brick_start = c->d_pme_start.p;
-  a.grid = c->d_grid.p;
+  a.grid = (T *)c->d_grid.p;
   if (P > 12) {
     rbc_set_error("B-spline order above 12");
     return RBC3D_EINVAL;
@@ -374,77 +394,97 @@ int rbc_pme_spread(rbc3d_ctx *c, double c1, double c2, int cells, int walls) {
   a.frl = make_fastdiv(3 + 3 * P + (c->flag_sl ? 3 : 0) + (c->flag_dl ? 9 : 0));
   rbc_stage_begin(c, ST_SPREAD);
   if (c->flag_sl && c->flag_dl)
-    k_spread<true, true><<<nbr, SPR_T, 0, st>>>(a);
+    k_spread<true, true, T><<<nbr, SPR_T, 0, st>>>(a);
   else if (c->flag_sl)
-    k_spread<true, false><<<nbr, SPR_T, 0, st>>>(a);
+    k_spread<true, false, T><<<nbr, SPR_T, 0, st>>>(a);
   else if (c->flag_dl)
-    k_spread<false, true><<<nbr, SPR_T, 0, st>>>(a);
+    k_spread<false, true, T><<<nbr, SPR_T, 0, st>>>(a);
   rbc_stage_end(c, ST_SPREAD);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;
 }
 
+int rbc_pme_spread(rbc3d_ctx *c, double c1, double c2, int cells, int walls) {
+  return c->prec == RBC3D_PREC_SPLIT ? pme_spread_t<float>(c, c1, c2, cells, walls)
+                                     : pme_spread_t<double>(c, c1, c2, cells, walls);
+}
+
 // ---------------------------------------------------------------------------
 // FFT: batches of lines, Stockham autosort in LDS.
+template <typename T>
 struct FftPlan {
   int N, nrad, rad[16];
   int sign;
-  const double2 *tw;  // exp(-2 pi i m/N)
+  const typename CxT<T>::t *tw;  // exp(-2 pi i m/N)
   FastDiv fNR[16], fNs[16];  // N / rad[s] and the span Ns before pass s
 };
 
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+template <typename C>
+__device__ __forceinline__ C cmul(C a, C b) {
+  C r;
+  r.x = a.x * b.x - a.y * b.y;
+  r.y = a.x * b.y + a.y * b.x;
+  return r;
 }
-__device__ __forceinline__ double2 twid(const double2 *tw, int m, int sign) {  // m < N
-  double2 t = tw[m];
+template <typename C>
+__device__ __forceinline__ C twid(const C *tw, int m, int sign) {  // m < N
+  C t = tw[m];
   if (sign > 0) t.y = -t.y;
   return t;
+}
+template <typename C>
+__device__ __forceinline__ C cpx(decltype(C().x) x, decltype(C().x) y) {
+  C r;
+  r.x = x;
+  r.y = y;
+  return r;
 }
 
 // FFT of TL lines of length N stored line-major in buf (line l at l*N); uses tmp.
 // Returns pointer to the buffer holding the result.
 // One Stockham pass of radix R (compile-time, so the butterfly lives in registers).
-template <int R>
-__device__ __forceinline__ void fft_pass(const double2 *in, double2 *out, int TL, int N, int Ns,
-                                         const FastDiv &fNR, const FastDiv &fNs, const FftPlan &pl) {
+template <int R, typename T>
+__device__ __forceinline__ void fft_pass(const typename CxT<T>::t *in, typename CxT<T>::t *out,
+                                         int TL, int N, int Ns, const FastDiv &fNR,
+                                         const FastDiv &fNs, const FftPlan<T> &pl) {
+  typedef typename CxT<T>::t C;
   const int NR = fNR.d;
   const int tws = NR / Ns;  // twiddle stride N / (Ns R)
   for (int t = threadIdx.x; t < TL * NR; t += blockDim.x) {
     const int l = fd_div(t, fNR), j = t - l * NR;
-    const double2 *src = in + (size_t)l * N;
-    double2 *dst = out + (size_t)l * N;
+    const C *src = in + (size_t)l * N;
+    C *dst = out + (size_t)l * N;
     const int jq = fd_div(j, fNs), k = j - jq * Ns;
-    double2 v[R];
+    C v[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
-      double2 x = src[j + r * NR];
+      C x = src[j + r * NR];
       if (r && k) x = cmul(x, twid(pl.tw, r * k * tws, pl.sign));
       v[r] = x;
     }
     const int base = jq * Ns * R + k;
-    if (R == 2) {
-      dst[base] = make_double2(v[0].x + v[1].x, v[0].y + v[1].y);
-      dst[base + Ns] = make_double2(v[0].x - v[1].x, v[0].y - v[1].y);
-    } else if (R == 4) {
-      double2 a0 = make_double2(v[0].x + v[2].x, v[0].y + v[2].y);
-      double2 a1 = make_double2(v[0].x - v[2].x, v[0].y - v[2].y);
-      double2 b0 = make_double2(v[1].x + v[3].x, v[1].y + v[3].y);
-      double2 b1 = make_double2(v[1].x - v[3].x, v[1].y - v[3].y);
+    if constexpr (R == 2) {
+      dst[base] = cpx<C>(v[0].x + v[1].x, v[0].y + v[1].y);
+      dst[base + Ns] = cpx<C>(v[0].x - v[1].x, v[0].y - v[1].y);
+    } else if constexpr (R == 4) {
+      C a0 = cpx<C>(v[0].x + v[2].x, v[0].y + v[2].y);
+      C a1 = cpx<C>(v[0].x - v[2].x, v[0].y - v[2].y);
+      C b0 = cpx<C>(v[1].x + v[3].x, v[1].y + v[3].y);
+      C b1 = cpx<C>(v[1].x - v[3].x, v[1].y - v[3].y);
       // multiply b1 by (sign * i)
-      double2 jb1 = pl.sign < 0 ? make_double2(b1.y, -b1.x) : make_double2(-b1.y, b1.x);
-      dst[base] = make_double2(a0.x + b0.x, a0.y + b0.y);
-      dst[base + Ns] = make_double2(a1.x + jb1.x, a1.y + jb1.y);
-      dst[base + 2 * Ns] = make_double2(a0.x - b0.x, a0.y - b0.y);
-      dst[base + 3 * Ns] = make_double2(a1.x - jb1.x, a1.y - jb1.y);
+      C jb1 = pl.sign < 0 ? cpx<C>(b1.y, -b1.x) : cpx<C>(-b1.y, b1.x);
+      dst[base] = cpx<C>(a0.x + b0.x, a0.y + b0.y);
+      dst[base + Ns] = cpx<C>(a1.x + jb1.x, a1.y + jb1.y);
+      dst[base + 2 * Ns] = cpx<C>(a0.x - b0.x, a0.y - b0.y);
+      dst[base + 3 * Ns] = cpx<C>(a1.x - jb1.x, a1.y - jb1.y);
     } else {
 #pragma unroll
       for (int q = 0; q < R; q++) {
-        double2 sum = v[0];
+        C sum = v[0];
 #pragma unroll
         for (int r = 1; r < R; r++) {
-          double2 w = twid(pl.tw, ((q * r) % R) * NR, pl.sign);
-          double2 p = cmul(v[r], w);
+          C w = twid(pl.tw, ((q * r) % R) * NR, pl.sign);
+          C p = cmul(v[r], w);
           sum.x += p.x;
           sum.y += p.y;
         }
@@ -456,21 +496,24 @@ __device__ __forceinline__ void fft_pass(const double2 *in, double2 *out, int TL
 
 // FFT of TL lines of length N stored line-major in buf (line l at l*N); uses tmp.
 // Returns pointer to the buffer holding the result.
-__device__ double2 *fft_lds(double2 *buf, double2 *tmp, int TL, const FftPlan &pl) {
+template <typename T>
+__device__ typename CxT<T>::t *fft_lds(typename CxT<T>::t *buf, typename CxT<T>::t *tmp, int TL,
+                                       const FftPlan<T> &pl) {
+  typedef typename CxT<T>::t C;
   int N = pl.N, Ns = 1;
-  double2 *in = buf, *out = tmp;
+  C *in = buf, *out = tmp;
   for (int s = 0; s < pl.nrad; s++) {
     const int R = pl.rad[s];
     if (R == 2)
-      fft_pass<2>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl);
+      fft_pass<2, T>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl);
     else if (R == 3)
-      fft_pass<3>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl);
+      fft_pass<3, T>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl);
     else if (R == 4)
-      fft_pass<4>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl);
+      fft_pass<4, T>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl);
     else
-      fft_pass<5>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl);
+      fft_pass<5, T>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl);
     __syncthreads();
-    double2 *t2 = in;
+    C *t2 = in;
     in = out;
     out = t2;
     Ns *= R;
@@ -485,10 +528,13 @@ struct LineMap {
   int nlines;
 };
 
-__global__ void __launch_bounds__(FFT_T) k_fft_c2c(double2 *data, FftPlan pl, LineMap lm, int TL,
-                                                   FastDiv fTL) {
-  extern __shared__ double2 lds[];
-  double2 *buf = lds, *tmp = lds + (size_t)TL * pl.N;
+template <typename T>
+__global__ void __launch_bounds__(FFT_T) k_fft_c2c(typename CxT<T>::t *data, FftPlan<T> pl,
+                                                   LineMap lm, int TL, FastDiv fTL) {
+  typedef typename CxT<T>::t C;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  C *lds = reinterpret_cast<C *>(lds_raw);
+  C *buf = lds, *tmp = lds + (size_t)TL * pl.N;
   const int l0 = blockIdx.x * TL;
   const int N = pl.N;
   // load: lines fastest (adjacent lines are adjacent in memory when sa = 1)
@@ -500,7 +546,7 @@ __global__ void __launch_bounds__(FFT_T) k_fft_c2c(double2 *data, FftPlan pl, Li
     buf[(size_t)l * N + e] = data[o];
   }
   __syncthreads();
-  double2 *res = fft_lds(buf, tmp, TL, pl);
+  C *res = fft_lds<T>(buf, tmp, TL, pl);
   for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
     const int e = fd_div(t, fTL), l = t - e * TL, L = l0 + l;
     if (L >= lm.nlines) continue;
@@ -511,18 +557,22 @@ __global__ void __launch_bounds__(FFT_T) k_fft_c2c(double2 *data, FftPlan pl, Li
 }
 
 // x r2c: real rows of N1 -> complex rows of H = N1/2+1
-__global__ void __launch_bounds__(FFT_T) k_fft_r2c(const double *in, double2 *out, FftPlan pl,
-                                                   int nrows, int TL, FastDiv fN, FastDiv fH) {
-  extern __shared__ double2 lds[];
-  double2 *buf = lds, *tmp = lds + (size_t)TL * pl.N;
+template <typename T>
+__global__ void __launch_bounds__(FFT_T) k_fft_r2c(const T *in, typename CxT<T>::t *out,
+                                                   FftPlan<T> pl, int nrows, int TL, FastDiv fN,
+                                                   FastDiv fH) {
+  typedef typename CxT<T>::t C;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  C *lds = reinterpret_cast<C *>(lds_raw);
+  C *buf = lds, *tmp = lds + (size_t)TL * pl.N;
   const int N = pl.N, H = N / 2 + 1, r0 = blockIdx.x * TL;
   // the block's rows are contiguous in memory: element t of the block is row t / N
   for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
     if (r0 + fd_div(t, fN) >= nrows) continue;
-    buf[t] = make_double2(in[(size_t)r0 * N + t], 0.);
+    buf[t] = cpx<C>(in[(size_t)r0 * N + t], 0.);
   }
   __syncthreads();
-  double2 *res = fft_lds(buf, tmp, TL, pl);
+  C *res = fft_lds<T>(buf, tmp, TL, pl);
   for (int t = threadIdx.x; t < TL * H; t += blockDim.x) {
     const int l = fd_div(t, fH), e = t - l * H, R = r0 + l;
     if (R >= nrows) continue;
@@ -532,44 +582,51 @@ __global__ void __launch_bounds__(FFT_T) k_fft_r2c(const double *in, double2 *ou
 
 // x c2r: complex rows of H -> real rows of N1 (real part of the zero and Nyquist
 // frequencies only, the halfcomplex convention of FFTW's c2r)
-__global__ void __launch_bounds__(FFT_T) k_fft_c2r(const double2 *in, double *out, FftPlan pl,
-                                                   int nrows, int TL, FastDiv fN) {
-  extern __shared__ double2 lds[];
-  double2 *buf = lds, *tmp = lds + (size_t)TL * pl.N;
+template <typename T>
+__global__ void __launch_bounds__(FFT_T) k_fft_c2r(const typename CxT<T>::t *in, T *out,
+                                                   FftPlan<T> pl, int nrows, int TL, FastDiv fN) {
+  typedef typename CxT<T>::t C;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  C *lds = reinterpret_cast<C *>(lds_raw);
+  C *buf = lds, *tmp = lds + (size_t)TL * pl.N;
   const int N = pl.N, H = N / 2 + 1, r0 = blockIdx.x * TL;
   for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
     const int l = fd_div(t, fN), e = t - l * N, R = r0 + l;
     if (R >= nrows) continue;
-    double2 v;
+    C v;
     if (e == 0) {
-      v = make_double2(in[(size_t)R * H].x, 0.);
+      v = cpx<C>(in[(size_t)R * H].x, 0.);
     } else if (2 * e == N) {
-      v = make_double2(in[(size_t)R * H + e].x, 0.);
+      v = cpx<C>(in[(size_t)R * H + e].x, 0.);
     } else if (e < H) {
       v = in[(size_t)R * H + e];
     } else {
-      double2 w = in[(size_t)R * H + (N - e)];
-      v = make_double2(w.x, -w.y);
+      C w = in[(size_t)R * H + (N - e)];
+      v = cpx<C>(w.x, -w.y);
     }
     buf[t] = v;
   }
   __syncthreads();
-  double2 *res = fft_lds(buf, tmp, TL, pl);
+  C *res = fft_lds<T>(buf, tmp, TL, pl);
   for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
     if (r0 + fd_div(t, fN) >= nrows) continue;
     out[(size_t)r0 * N + t] = res[t].x;
   }
 }
 
-static FftPlan make_plan(rbc3d_ctx *c, int ax, int sign) {
-  FftPlan p;
+template <typename T>
+static FftPlan<T> make_plan(rbc3d_ctx *c, int ax, int sign) {
+  FftPlan<T> p;
   p.N = c->conf.Nb[ax];
   std::vector<int> r;
   factor_radices(p.N, r);
   p.nrad = (int)r.size();
   for (int i = 0; i < p.nrad; i++) p.rad[i] = r[i];
   p.sign = sign;
-  p.tw = (const double2 *)(c->d_tw.p + 2 * (size_t)c->tw_off[ax]);
+  if (sizeof(T) == 8)
+    p.tw = (const typename CxT<T>::t *)(c->d_tw.p + 2 * (size_t)c->tw_off[ax]);
+  else
+    p.tw = (const typename CxT<T>::t *)(c->d_twf.p + 2 * (size_t)c->tw_off[ax]);
   int Ns = 1;
   for (int i = 0; i < p.nrad; i++) {
     p.fNR[i] = make_fastdiv(p.N / p.rad[i]);
@@ -579,13 +636,15 @@ static FftPlan make_plan(rbc3d_ctx *c, int ax, int sign) {
   return p;
 }
 
-// ~1024 complex elements per workgroup: 32 KiB of LDS for the two buffers, so five
-// workgroups fit a CU
-static int lines_per_block(int N) { return std::max(1, 1024 / N); }
+// ~1024 complex elements per workgroup (2048 in FP32): 32 KiB of LDS for the two
+// buffers, so five workgroups fit a CU
+template <typename T>
+static int lines_per_block(int N) { return std::max(1, (int)(8192 / sizeof(T)) / N); }
 
-static int fft_c2c(rbc3d_ctx *c, double2 *data, int ax, int sign, int nf) {
+template <typename T>
+static int fft_c2c(rbc3d_ctx *c, typename CxT<T>::t *data, int ax, int sign, int nf) {
   int N1 = c->conf.Nb[0], N2 = c->conf.Nb[1], N3 = c->conf.Nb[2], H = N1 / 2 + 1;
-  FftPlan pl = make_plan(c, ax, sign);
+  FftPlan<T> pl = make_plan<T>(c, ax, sign);
   LineMap lm;
   if (ax == 1) {  // lines over (kx, kz, f), stride H
     lm.A = make_fastdiv(H);
@@ -600,25 +659,29 @@ static int fft_c2c(rbc3d_ctx *c, double2 *data, int ax, int sign, int nf) {
     lm.es = (long long)H * N2;
     lm.nlines = H * N2 * nf;
   }
-  int TL = lines_per_block(pl.N);
-  size_t lds = 2 * sizeof(double2) * TL * pl.N;
-  k_fft_c2c<<<(lm.nlines + TL - 1) / TL, FFT_T, lds, c->stream>>>(data, pl, lm, TL,
-                                                                    make_fastdiv(TL));
+  int TL = lines_per_block<T>(pl.N);
+  size_t lds = 2 * sizeof(typename CxT<T>::t) * TL * pl.N;
+  k_fft_c2c<T><<<(lm.nlines + TL - 1) / TL, FFT_T, lds, c->stream>>>(data, pl, lm, TL,
+                                                                       make_fastdiv(TL));
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;
 }
 
 // ---------------------------------------------------------------------------
 // Green's function multiply (PME_Transform :159-210), output fields 12..14
+template <typename T>
 struct GreenArgs {
   int N1, N2, N3, H;
   double iLb[3], alpha, vol;
   int flag_sl, flag_dl;
-  const double *bb;  // bx(H), by(N2), bz(N3)
-  double2 *C;        // 15 complex fields
+  const double *bb;         // bx(H), by(N2), bz(N3)
+  typename CxT<T>::t *C;    // 15 complex fields
 };
 
-__global__ void k_green(GreenArgs a) {
+// The k-dependent factors are formed in FP64 for either grid scalar T.
+template <typename T>
+__global__ void k_green(GreenArgs<T> a) {
+  typedef typename CxT<T>::t CT;
   size_t nc = (size_t)a.H * a.N2 * a.N3;
   size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (o >= nc) return;
@@ -637,7 +700,10 @@ __global__ void k_green(GreenArgs a) {
     double phi0 = e / q2t, phi1 = (e + phi0) / q2t;
     if (a.flag_sl) {
       double2 f[3];
-      for (int d = 0; d < 3; d++) f[d] = a.C[o + nc * d];
+      for (int d = 0; d < 3; d++) {
+        const CT v = a.C[o + nc * d];
+        f[d] = make_double2(v.x, v.y);
+      }
       double2 qf = make_double2(qt[0] * f[0].x + qt[1] * f[1].x + qt[2] * f[2].x,
                                 qt[0] * f[0].y + qt[1] * f[1].y + qt[2] * f[2].y);
       double s = 2 * a.alpha / a.vol * phi1;
@@ -647,17 +713,20 @@ __global__ void k_green(GreenArgs a) {
       }
     }
     if (a.flag_dl) {
-      double2 T[3][3];
+      double2 Tm[3][3];
       for (int ii = 0; ii < 3; ii++)
-        for (int jj = 0; jj < 3; jj++) T[ii][jj] = a.C[o + nc * (3 + ii + 3 * jj)];
-      double2 tr = make_double2(T[0][0].x + T[1][1].x + T[2][2].x, T[0][0].y + T[1][1].y + T[2][2].y);
+        for (int jj = 0; jj < 3; jj++) {
+          const CT v = a.C[o + nc * (3 + ii + 3 * jj)];
+          Tm[ii][jj] = make_double2(v.x, v.y);
+        }
+      double2 tr = make_double2(Tm[0][0].x + Tm[1][1].x + Tm[2][2].x, Tm[0][0].y + Tm[1][1].y + Tm[2][2].y);
       double2 qT[3], Tq[3];
       for (int jj = 0; jj < 3; jj++)
-        qT[jj] = make_double2(q[0] * T[0][jj].x + q[1] * T[1][jj].x + q[2] * T[2][jj].x,
-                              q[0] * T[0][jj].y + q[1] * T[1][jj].y + q[2] * T[2][jj].y);
+        qT[jj] = make_double2(q[0] * Tm[0][jj].x + q[1] * Tm[1][jj].x + q[2] * Tm[2][jj].x,
+                              q[0] * Tm[0][jj].y + q[1] * Tm[1][jj].y + q[2] * Tm[2][jj].y);
       for (int ii = 0; ii < 3; ii++)
-        Tq[ii] = make_double2(T[ii][0].x * q[0] + T[ii][1].x * q[1] + T[ii][2].x * q[2],
-                              T[ii][0].y * q[0] + T[ii][1].y * q[1] + T[ii][2].y * q[2]);
+        Tq[ii] = make_double2(Tm[ii][0].x * q[0] + Tm[ii][1].x * q[1] + Tm[ii][2].x * q[2],
+                              Tm[ii][0].y * q[0] + Tm[ii][1].y * q[1] + Tm[ii][2].y * q[2]);
       double2 qTq = make_double2(q[0] * Tq[0].x + q[1] * Tq[1].x + q[2] * Tq[2].x,
                                  q[0] * Tq[0].y + q[1] * Tq[1].y + q[2] * Tq[2].y);
       double s0 = 4 * RBC_PI * a.alpha / a.vol * phi0;
@@ -678,29 +747,37 @@ __global__ void k_green(GreenArgs a) {
       vc[d].y *= b;
     }
   }
-  for (int d = 0; d < 3; d++) a.C[o + nc * (12 + d)] = vc[d];
+  for (int d = 0; d < 3; d++) {
+    CT v;
+    v.x = (decltype(v.x))vc[d].x;
+    v.y = (decltype(v.y))vc[d].y;
+    a.C[o + nc * (12 + d)] = v;
+  }
 }
 
-int rbc_pme_transform(rbc3d_ctx *c) {
+template <typename T>
+static int pme_transform_t(rbc3d_ctx *c) {
+  typedef typename CxT<T>::t CT;
   hipStream_t st = c->stream;
   int N1 = c->conf.Nb[0], N2 = c->conf.Nb[1], N3 = c->conf.Nb[2], H = N1 / 2 + 1;
   size_t ng = (size_t)N1 * N2 * N3, nc = (size_t)H * N2 * N3;
-  double2 *C = (double2 *)c->d_cgrid.p;
+  CT *C = (CT *)c->d_cgrid.p;
   int f0 = c->flag_sl ? 0 : 3, f1 = c->flag_dl ? 12 : 3;
   int nf = f1 - f0;
   rbc_stage_begin(c, ST_FFT);
   if (nf > 0) {
-    FftPlan px = make_plan(c, 0, -1);
-    int TL = lines_per_block(N1), nrows = N2 * N3 * nf;
-    size_t lds = 2 * sizeof(double2) * TL * N1;
-    k_fft_r2c<<<(nrows + TL - 1) / TL, FFT_T, lds, st>>>(c->d_grid.p + ng * f0, C + nc * f0, px, nrows,
-                                                         TL, make_fastdiv(N1), make_fastdiv(H));
+    FftPlan<T> px = make_plan<T>(c, 0, -1);
+    int TL = lines_per_block<T>(N1), nrows = N2 * N3 * nf;
+    size_t lds = 2 * sizeof(CT) * TL * N1;
+    k_fft_r2c<T><<<(nrows + TL - 1) / TL, FFT_T, lds, st>>>((const T *)c->d_grid.p + ng * f0,
+                                                            C + nc * f0, px, nrows, TL,
+                                                            make_fastdiv(N1), make_fastdiv(H));
     HIP_TRY(hipGetLastError());
-    RBC_TRY(fft_c2c(c, C + nc * f0, 1, -1, nf));
-    RBC_TRY(fft_c2c(c, C + nc * f0, 2, +1, nf));
+    RBC_TRY(fft_c2c<T>(c, C + nc * f0, 1, -1, nf));
+    RBC_TRY(fft_c2c<T>(c, C + nc * f0, 2, +1, nf));
   }
   rbc_stage_end(c, ST_FFT);
-  GreenArgs g;
+  GreenArgs<T> g;
   g.N1 = N1;
   g.N2 = N2;
   g.N3 = N3;
@@ -713,28 +790,35 @@ int rbc_pme_transform(rbc3d_ctx *c) {
   g.bb = c->d_bb.p;
   g.C = C;
   rbc_stage_begin(c, ST_GREEN);
-  k_green<<<(unsigned)((nc + 255) / 256), 256, 0, st>>>(g);
+  k_green<T><<<(unsigned)((nc + 255) / 256), 256, 0, st>>>(g);
   rbc_stage_end(c, ST_GREEN);
   HIP_TRY(hipGetLastError());
   rbc_stage_begin(c, ST_FFT);
-  RBC_TRY(fft_c2c(c, C + nc * 12, 2, -1, 3));
-  RBC_TRY(fft_c2c(c, C + nc * 12, 1, +1, 3));
-  FftPlan pxi = make_plan(c, 0, +1);
-  int TL = lines_per_block(N1), nrows = N2 * N3 * 3;
-  size_t lds = 2 * sizeof(double2) * TL * N1;
-  k_fft_c2r<<<(nrows + TL - 1) / TL, FFT_T, lds, st>>>(C + nc * 12, c->d_vgrid.p, pxi, nrows, TL,
-                                                        make_fastdiv(N1));
+  RBC_TRY(fft_c2c<T>(c, C + nc * 12, 2, -1, 3));
+  RBC_TRY(fft_c2c<T>(c, C + nc * 12, 1, +1, 3));
+  FftPlan<T> pxi = make_plan<T>(c, 0, +1);
+  int TL = lines_per_block<T>(N1), nrows = N2 * N3 * 3;
+  size_t lds = 2 * sizeof(CT) * TL * N1;
+  k_fft_c2r<T><<<(nrows + TL - 1) / TL, FFT_T, lds, st>>>(C + nc * 12, (T *)c->d_vgrid.p, pxi,
+                                                           nrows, TL, make_fastdiv(N1));
   rbc_stage_end(c, ST_FFT);
   HIP_TRY(hipGetLastError());
+  c->pme_grid_prec = c->prec;
   return RBC3D_OK;
+}
+
+int rbc_pme_transform(rbc3d_ctx *c) {
+  return c->prec == RBC3D_PREC_SPLIT ? pme_transform_t<float>(c) : pme_transform_t<double>(c);
 }
 
 // ---------------------------------------------------------------------------
 // gather: Interp_Vel (:450-489) + PME_Add_Interp_Vel (:228-248)
+template <typename T>
 struct GatherArgs {
   int N1, N2, N3, P, nT;
   double ih[3];
-  const double *vv, *tx, *tA;
+  const T *vv;
+  const double *tx, *tA;
   const int *tact;
   double *v;
 };
@@ -742,8 +826,8 @@ struct GatherArgs {
 
 // PC: the B-spline order when it is known at compile time (0: runtime a.P)
 #define GATHER_L 8  // lanes per target
-template <int PC>
-__global__ void __launch_bounds__(256) k_gather(GatherArgs a) {
+template <int PC, typename T>
+__global__ void __launch_bounds__(256) k_gather(GatherArgs<T> a) {
   // GATHER_L lanes per target share its P x P (y, z) stencil rows; each row is P
   // consecutive x points.  The partial sums are combined with shuffles.
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -786,17 +870,17 @@ __global__ void __launch_bounds__(256) k_gather(GatherArgs a) {
   }
 }
 
-int rbc_pme_gather(rbc3d_ctx *c, int nT, const double *d_tx, const int *d_act, const double *d_A,
-                   double *d_v) {
-  if (nT <= 0) return RBC3D_OK;
-  GatherArgs g;
+template <typename T>
+static int pme_gather_t(rbc3d_ctx *c, int nT, const double *d_tx, const int *d_act,
+                        const double *d_A, double *d_v) {
+  GatherArgs<T> g;
   g.N1 = c->conf.Nb[0];
   g.N2 = c->conf.Nb[1];
   g.N3 = c->conf.Nb[2];
   g.P = c->conf.P;
   g.nT = nT;
   for (int d = 0; d < 3; d++) g.ih[d] = c->conf.Nb[d] / c->Lb[d];
-  g.vv = c->d_vgrid.p;
+  g.vv = (const T *)c->d_vgrid.p;
   g.tx = d_tx;
   g.tA = d_A;
   g.tact = d_act;
@@ -804,10 +888,18 @@ int rbc_pme_gather(rbc3d_ctx *c, int nT, const double *d_tx, const int *d_act, c
   rbc_stage_begin(c, ST_GATHER);
   const unsigned nblk = (unsigned)(((size_t)nT * GATHER_L + 255) / 256);
   if (g.P == 6)
-    k_gather<6><<<nblk, 256, 0, c->stream>>>(g);
+    k_gather<6, T><<<nblk, 256, 0, c->stream>>>(g);
   else
-    k_gather<0><<<nblk, 256, 0, c->stream>>>(g);
+    k_gather<0, T><<<nblk, 256, 0, c->stream>>>(g);
   rbc_stage_end(c, ST_GATHER);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;
+}
+
+int rbc_pme_gather(rbc3d_ctx *c, int nT, const double *d_tx, const int *d_act, const double *d_A,
+                   double *d_v) {
+  if (nT <= 0) return RBC3D_OK;
+  // the velocity grid holds the scalar of the transform that wrote it
+  return c->pme_grid_prec == RBC3D_PREC_SPLIT ? pme_gather_t<float>(c, nT, d_tx, d_act, d_A, d_v)
+                                              : pme_gather_t<double>(c, nT, d_tx, d_act, d_A, d_v);
 }

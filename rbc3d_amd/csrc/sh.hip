@@ -300,6 +300,105 @@ __global__ void __launch_bounds__(256) k_sh_leg_mfma(int nl, int mmax, int ncolp
   }
 }
 
+// Split form of the Legendre GEMM: three-way split BF16 products with FP32
+// accumulation on the BF16 matrix cores.  Each operand is carried as three BF16
+// terms (x = x0 + x1 + x2, 24 significant bits); the product keeps the six terms
+// x_i y_j with i + j <= 2, the largest in one FP32 accumulator and the five smaller
+// ones in another, so the result is FP32-accurate.  The Legendre tables are split on
+// the host; the staged X operand (FP64) is split on the fly.
+// v_mfma_f32_16x16x16bf16_1k: lane l supplies A[l % 16][4 (l / 16) + j] and
+// B[4 (l / 16) + j][l % 16], j < 4, and holds D[4 (l / 16) + i][l % 16], i < 4.
+typedef short rbc_s4 __attribute__((ext_vector_type(4)));
+typedef float rbc_f4 __attribute__((ext_vector_type(4)));
+
+// round to nearest even FP32 -> BF16 (finite inputs)
+__host__ __device__ __forceinline__ unsigned short bf16_rne(float f) {
+  unsigned u;
+  __builtin_memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (unsigned short)(u >> 16);
+}
+__host__ __device__ __forceinline__ float bf16_to_f(unsigned short h) {
+  unsigned u = (unsigned)h << 16;
+  float f;
+  __builtin_memcpy(&f, &u, 4);
+  return f;
+}
+__host__ __device__ __forceinline__ void bf16_split3(double x, unsigned short &h0, unsigned short &h1,
+                                                     unsigned short &h2) {
+  h0 = bf16_rne((float)x);
+  const double r1 = x - (double)bf16_to_f(h0);
+  h1 = bf16_rne((float)r1);
+  const double r2 = r1 - (double)bf16_to_f(h1);
+  h2 = bf16_rne((float)r2);
+}
+
+template <bool TRANS_A>
+__global__ void __launch_bounds__(256) k_sh_leg_bf16x3(int nl, int mmax, int ncolp, int RT, int CT,
+                                                       const unsigned short *__restrict__ P3,
+                                                       const double *__restrict__ X, double *Y) {
+  const int wid = (int)((blockIdx.x * 256 + threadIdx.x) >> 6), lane = threadIdx.x & 63;
+  if (wid >= mmax * RT * CT) return;
+  const int ct = wid % CT, rt = (wid / CT) % RT, m = wid / (CT * RT);
+  const int row = rt * 16 + (lane & 15), kq = lane >> 4, col = ct * 16 + (lane & 15);
+  const size_t psz = (size_t)mmax * nl * nl;
+  const unsigned short *Pm = P3 + (size_t)m * nl * nl;
+  const double *Xm = X + (size_t)m * nl * ncolp;
+  rbc_f4 hi = {0.f, 0.f, 0.f, 0.f}, lo = {0.f, 0.f, 0.f, 0.f};
+  for (int k0 = 0; k0 < nl; k0 += 16) {
+    rbc_s4 a0, a1, a2, b0, b1, b2;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+      const int k = k0 + 4 * kq + j;
+      unsigned short p0 = 0, p1 = 0, p2 = 0, x0 = 0, x1 = 0, x2 = 0;
+      if (k < nl) {
+        if (row < nl) {
+          const size_t o = TRANS_A ? (size_t)k * nl + row : (size_t)row * nl + k;
+          p0 = Pm[o];
+          p1 = Pm[psz + o];
+          p2 = Pm[2 * psz + o];
+        }
+        bf16_split3(Xm[(size_t)k * ncolp + col], x0, x1, x2);
+      }
+      a0[j] = (short)p0;
+      a1[j] = (short)p1;
+      a2[j] = (short)p2;
+      b0[j] = (short)x0;
+      b1[j] = (short)x1;
+      b2[j] = (short)x2;
+    }
+    hi = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b0, hi, 0, 0, 0);
+    lo = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b2, lo, 0, 0, 0);
+    lo = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1, lo, 0, 0, 0);
+    lo = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2, b0, lo, 0, 0, 0);
+    lo = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b1, lo, 0, 0, 0);
+    lo = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b0, lo, 0, 0, 0);
+  }
+  double *Ym = Y + (size_t)m * nl * ncolp;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const int r = rt * 16 + 4 * (lane >> 4) + i;
+    if (r < nl) Ym[(size_t)r * ncolp + col] = (double)hi[i] + (double)lo[i];
+  }
+}
+
+// the split tables of one grid type: [3][mmax][nlat][nlat] BF16 terms of pw and pg
+static int split_tables(GridType *g) {
+  if (g->d_pw3.p) return RBC3D_OK;
+  const size_t n = (size_t)g->mmax * g->nlat * g->nlat;
+  std::vector<double> h(n);
+  std::vector<unsigned short> t(3 * n);
+  for (int which = 0; which < 2; which++) {
+    DBuf<double> &src = which ? g->d_pg : g->d_pw;
+    DBuf<unsigned short> &dst = which ? g->d_pg3 : g->d_pw3;
+    HIP_TRY(hipMemcpy(h.data(), src.p, n * 8, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) bf16_split3(h[i], t[i], t[n + i], t[2 * n + i]);
+    RBC_TRY(dst.ensure(3 * n));
+    HIP_TRY(hipMemcpy(dst.p, t.data(), 3 * n * 2, hipMemcpyHostToDevice));
+  }
+  return RBC3D_OK;
+}
+
 // coefficients of Y into the Glob_Sph_Trans vector: a = Y(cos), b = -Y(sin)
 __global__ void k_sh_pack(const CellRec *__restrict__ cells, const int *__restrict__ cell_list,
                           int cell0, int nl, int mmax, int ncolp, const double *__restrict__ Y,
@@ -368,17 +467,28 @@ int rbc_sh_glob(rbc3d_ctx *c, const double *d_in, double *d_out, int direction) 
     RBC_TRY(c->d_shx.ensure(2 * mat));
     double *X = c->d_shx.p, *Y = X + mat;
     const unsigned gblk = (unsigned)(((size_t)mmax * RT * CT * 64 + 255) / 256);
+    const bool split = c->prec == RBC3D_PREC_SPLIT;
+    if (split && !g->d_pw3.p) {  // once per grid type
+      HIP_TRY(hipStreamSynchronize(c->stream));
+      RBC_TRY(split_tables(g));
+    }
     rbc_stage_begin(c, ST_SH_GLOB);
     if (direction == 1) {
       k_sh_lon_anal<<<n, SH_THREADS, 0, c->stream>>>(c->d_cells.p, c->d_tmp_i.p, base, mmax,
                                                        g->d_cs.p, d_in, c->np, ncolp, X);
-      k_sh_leg_mfma<false><<<gblk, 256, 0, c->stream>>>(nl, mmax, ncolp, RT, CT, g->d_pw.p, X, Y);
+      if (split)
+        k_sh_leg_bf16x3<false><<<gblk, 256, 0, c->stream>>>(nl, mmax, ncolp, RT, CT, g->d_pw3.p, X, Y);
+      else
+        k_sh_leg_mfma<false><<<gblk, 256, 0, c->stream>>>(nl, mmax, ncolp, RT, CT, g->d_pw.p, X, Y);
       k_sh_pack<<<n, 256, 0, c->stream>>>(c->d_cells.p, c->d_tmp_i.p, base, nl, mmax, ncolp, Y,
                                           d_out);
     } else {
       k_sh_unpack<<<n, 256, 0, c->stream>>>(c->d_cells.p, c->d_tmp_i.p, base, nl, mmax, ncolp, d_in,
                                             X);
-      k_sh_leg_mfma<true><<<gblk, 256, 0, c->stream>>>(nl, mmax, ncolp, RT, CT, g->d_pg.p, X, Y);
+      if (split)
+        k_sh_leg_bf16x3<true><<<gblk, 256, 0, c->stream>>>(nl, mmax, ncolp, RT, CT, g->d_pg3.p, X, Y);
+      else
+        k_sh_leg_mfma<true><<<gblk, 256, 0, c->stream>>>(nl, mmax, ncolp, RT, CT, g->d_pg.p, X, Y);
       k_sh_lon_synth<<<n, SH_THREADS, 0, c->stream>>>(c->d_cells.p, c->d_tmp_i.p, base, mmax,
                                                       g->d_cs.p, Y, ncolp, c->np, d_out);
     }

@@ -382,9 +382,11 @@ __global__ void __launch_bounds__(128) k_wall_reduce(int nvert, const int *__res
 // v_i += c1/A_i * sum_k B_k f_(col k): one wavefront per vertex row, lanes over the
 // row's 3x3 blocks (each lane reads its blocks' 9 values, which lie next to those of
 // the neighbouring lanes, and the 3 traction components of the block's column).
+// V: the block scalar (double, or float in the split form); sums are FP64 either way.
+template <typename V>
 __global__ void __launch_bounds__(256) k_wall_spmv(int nvert, double c1, const int *__restrict__ rowptr,
                                                    const int *__restrict__ bcol,
-                                                   const double *__restrict__ bval,
+                                                   const V *__restrict__ bval,
                                                    const double *__restrict__ f,
                                                    const double *__restrict__ tA, double *v) {
   const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
@@ -392,7 +394,7 @@ __global__ void __launch_bounds__(256) k_wall_spmv(int nvert, double c1, const i
   double s0 = 0, s1 = 0, s2 = 0;
   for (int k = rowptr[i] + lane; k < rowptr[i + 1]; k += 64) {
     const int j = bcol[k];
-    const double *B = bval + 9 * (size_t)k;
+    const V *B = bval + 9 * (size_t)k;
     const double f0 = f[j], f1 = f[j + nvert], f2 = f[j + 2 * (size_t)nvert];
     s0 += B[0] * f0 + B[1] * f1 + B[2] * f2;
     s1 += B[3] * f0 + B[4] * f1 + B[5] * f2;
@@ -409,6 +411,11 @@ __global__ void __launch_bounds__(256) k_wall_spmv(int nvert, double c1, const i
     v[i + nvert] += c1 * s1 / A;
     v[i + 2 * (size_t)nvert] += c1 * s2 / A;
   }
+}
+
+__global__ void k_to_float(const double *__restrict__ x, size_t n, float *y) {
+  size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (float)x[i];
 }
 
 static WallArgs wall_args(rbc3d_ctx *c) {
@@ -542,6 +549,7 @@ extern "C" int rbc3d_set_wall(rbc3d_ctx *c, int nvert, int nele, const double *x
   RBC_TRY(rbc_hash_sort(c, nele, w.d_cen.p, nele, bins.p, w.d_order.p, w.d_bin_start.p));
   bins.release();
   w.prepared = false;
+  w.valf_ready = false;
   HIP_TRY(hipStreamSynchronize(st));
   return RBC3D_OK;
 }
@@ -626,6 +634,7 @@ int rbc_wall_prepare(rbc3d_ctx *c) {
   HIP_TRY(hipStreamSynchronize(st));
   w.nnz = 9LL * nb;
   w.prepared = true;
+  w.valf_ready = false;
   return RBC3D_OK;
 }
 
@@ -649,8 +658,21 @@ int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double c1, int nT, const double *d_tx
   if (wall_targets) {
     // wall vertices: the precomputed self-interaction; with one wall that is all
     if (!w.prepared) RBC_TRY(rbc_wall_prepare(c));
-    k_wall_spmv<<<(unsigned)(((size_t)w.nvert * 64 + 255) / 256), 256, 0, st>>>(
-        w.nvert, c1, w.d_rowptr.p, w.d_col.p, w.d_val.p, w.d_f.p, d_A, d_v);
+    const unsigned nblk = (unsigned)(((size_t)w.nvert * 64 + 255) / 256);
+    if (c->prec == RBC3D_PREC_SPLIT) {
+      if (!w.valf_ready) {
+        const size_t nv9 = (size_t)w.nnz;  // 9 values per 3x3 block
+        RBC_TRY(w.d_valf.ensure(nv9 + 1));
+        k_to_float<<<(unsigned)((nv9 + 255) / 256), 256, 0, st>>>(w.d_val.p, nv9, w.d_valf.p);
+        HIP_TRY(hipGetLastError());
+        w.valf_ready = true;
+      }
+      k_wall_spmv<float><<<nblk, 256, 0, st>>>(w.nvert, c1, w.d_rowptr.p, w.d_col.p, w.d_valf.p,
+                                               w.d_f.p, d_A, d_v);
+    } else {
+      k_wall_spmv<double><<<nblk, 256, 0, st>>>(w.nvert, c1, w.d_rowptr.p, w.d_col.p, w.d_val.p,
+                                                w.d_f.p, d_A, d_v);
+    }
   } else {
     k_wall_targets<<<(nT + 127) / 128, 128, 0, st>>>(wall_args(c), c1, nT, d_tx, d_tindx, d_act, d_A,
                                                      d_v);
