@@ -175,11 +175,54 @@ int rbc3d_set_precision(rbc3d_ctx *ctx, int mode);
  * residual velocity after the update (the monitor of :140-145). */
 int rbc3d_noslip_wall(rbc3d_ctx *ctx, double rtol, int maxit, const double *finc0, int *niter,
                       double *rnorm, int *converged, double *vres_max);
+/* Residual history of the last NoSlipWall solve (no counterpart in the reference,
+ * which prints only the final niter and residual, common/ModNoSlip.F90:120-126):
+ * the initial residual norm, then the GMRES estimate after every iteration.  *n is
+ * the length of the history; at most maxn values are copied into h. */
+int rbc3d_gmres_history(rbc3d_ctx *ctx, double *h, int maxn, int *n);
 /* Solve_RBC_Vel (common/ModVelSolver.F90:44-132): v(nPoint,3) cell velocities; with
  * all viscosity ratios 1 the solution is the right-hand side (no GMRES); otherwise
  * GMRES to 1e-11 (maxit 200) from the previous solution.  The cells' double-layer
  * density is then set to v (common/ModTimeInt.F90:773-790).  v may be NULL. */
 int rbc3d_solve_rbc_vel(rbc3d_ctx *ctx, double *v, int *niter, double *rnorm);
+
+/* --- the learned warm start of NoSlipWall (no counterpart in the reference) -----
+ * A per-wall-vertex MLP, 9 -> 64 -> 64 -> 64 -> 3 with GELU, BF16 weights and
+ * activations, FP32 accumulation.  Inputs per vertex: the cells' velocity there
+ * (rbc3d_wall_cell_velocity), the previous traction and the vertex normal, each
+ * standardised (x - mean) / std; the output times scale_out is the predicted traction
+ * increment.  Parameters: an FP32 vector of rbc3d_mlp_nparams() values,
+ * W1[64][9] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[3][64] b4[3] (row-major).
+ * Training: mean squared error of the scaled increments, Adam (beta 0.9, 0.999,
+ * eps 1e-8).  Device arrays: x(n,9) and t(n,3) row-major FP32. */
+typedef struct rbc3d_mlp rbc3d_mlp;
+int rbc3d_mlp_create(rbc3d_mlp **out);
+int rbc3d_mlp_destroy(rbc3d_mlp *m);
+int rbc3d_mlp_nparams(void);
+int rbc3d_mlp_set_params(rbc3d_mlp *m, const float *params);
+int rbc3d_mlp_get_params(rbc3d_mlp *m, float *params);
+int rbc3d_mlp_set_norm(rbc3d_mlp *m, const float *mean_in, const float *std_in,
+                       const float *scale_out);
+/* y(n,3) = the predicted increments */
+int rbc3d_mlp_forward_dev(rbc3d_mlp *m, int n, const float *d_x, float *d_y);
+/* gradient of the loss w.r.t. the parameters into d_grad (may be NULL), and the loss */
+int rbc3d_mlp_grad_dev(rbc3d_mlp *m, int n, const float *d_x, const float *d_t, float *d_grad,
+                       double *loss);
+/* one Adam step on the batch; loss (may be NULL) is the batch loss before the step */
+int rbc3d_mlp_adam_step_dev(rbc3d_mlp *m, int n, const float *d_x, const float *d_t, float lr,
+                            double *loss);
+/* The cells' part of Compute_Wall_Residual_Vel (common/ModNoSlip.F90:153-193) at the
+ * wall vertices, v(nvert,3), and the area-weighted vertex normals n(nvert,3) (the
+ * reference keeps element normals only, common/ModWall.F90:134-138). */
+int rbc3d_wall_cell_velocity(rbc3d_ctx *ctx, double *v);
+int rbc3d_wall_vertex_normals(rbc3d_ctx *ctx, double *n);
+/* NoSlipWall (common/ModNoSlip.F90:45-150) with the network's prediction as GMRES's
+ * initial guess (tolerance rtol * ||b||, as for a zero guess); plain GMRES from zero
+ * is the fallback (*fallback = 1) when the prediction's residual is not below the
+ * right-hand side's or the warm-started solve does not converge.  *niter counts the
+ * iterations of both solves. */
+int rbc3d_noslip_wall_learned(rbc3d_ctx *ctx, rbc3d_mlp *m, double rtol, int maxit, int *niter,
+                              double *rnorm, int *converged, double *vres_max, int *fallback);
 
 /* --- diagnostics (no counterpart in the reference, which only prints a per-step
  *     MPI_WTime, common/ModTimeInt.F90:122,167) ---------------------------------

@@ -17,6 +17,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "librbc3d_gpu.so")
 
 dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "neighbour-cell list overflow (>32)",
@@ -51,6 +52,22 @@ _SIG = {
     "rbc3d_near_stats": [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
     "rbc3d_set_geometry_reuse": [C.c_void_p, C.c_int],
     "rbc3d_set_precision": [C.c_void_p, C.c_int],
+    "rbc3d_gmres_history": [C.c_void_p, dp, C.c_int, C.POINTER(C.c_int)],
+    "rbc3d_mlp_create": [C.POINTER(C.c_void_p)],
+    "rbc3d_mlp_destroy": [C.c_void_p],
+    "rbc3d_mlp_set_params": [C.c_void_p, fp],
+    "rbc3d_mlp_get_params": [C.c_void_p, fp],
+    "rbc3d_mlp_set_norm": [C.c_void_p, fp, fp, fp],
+    "rbc3d_mlp_forward_dev": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p],
+    "rbc3d_mlp_grad_dev": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_void_p,
+                           C.POINTER(C.c_double)],
+    "rbc3d_mlp_adam_step_dev": [C.c_void_p, C.c_int, C.c_void_p, C.c_void_p, C.c_float,
+                                C.POINTER(C.c_double)],
+    "rbc3d_wall_cell_velocity": [C.c_void_p, dp],
+    "rbc3d_wall_vertex_normals": [C.c_void_p, dp],
+    "rbc3d_noslip_wall_learned": [C.c_void_p, C.c_void_p, C.c_double, C.c_int, C.POINTER(C.c_int),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                  C.POINTER(C.c_double), C.POINTER(C.c_int)],
     "rbc3d_pme_distrib_source": [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_int],
     "rbc3d_pme_transform": [C.c_void_p],
     "rbc3d_pme_add_interp_vel": [C.c_void_p, C.c_int, dp, ip, dp, dp],
@@ -97,12 +114,14 @@ def lib():
             fn.argtypes = args
         L.rbc3d_last_error.restype = C.c_char_p
         L.rbc3d_last_error.argtypes = []
+        L.rbc3d_mlp_nparams.restype = C.c_int
+        L.rbc3d_mlp_nparams.argtypes = []
         _lib = L
     return _lib
 
 
 def exported_symbols():
-    return list(_SIG) + ["rbc3d_last_error"]
+    return list(_SIG) + ["rbc3d_last_error", "rbc3d_mlp_nparams"]
 
 
 def _check(name, r):
@@ -194,6 +213,7 @@ class System:
 
     # --- module data ---------------------------------------------------------
     def set_cells(self, cells):
+        self.cells = cells  # the geometry last set (for callers that move cells)
         _call("rbc3d_set_cells", self.h, cells.ncell, _i32(cells.nlat0), _i32(cells.nlat),
               _i32(cells.celltype), _f64(cells.x), _f64(cells.a3), _f64(cells.detj))
         self.np = cells.npoint
@@ -378,6 +398,34 @@ class System:
               C.byref(vm))
         return ni.value, rn.value, bool(cv.value), vm.value
 
+    def gmres_history(self):
+        """Residual history of the last NoSlipWall solve (initial norm, then one
+        estimate per iteration)."""
+        n = C.c_int()
+        _call("rbc3d_gmres_history", self.h, np.zeros(1), 0, C.byref(n))
+        h = np.zeros(max(n.value, 1))
+        _call("rbc3d_gmres_history", self.h, h, n.value, C.byref(n))
+        return h[:n.value]
+
+    def wall_cell_velocity(self):
+        """The cells' velocity at the wall vertices, (nvert,3) flattened in Fortran order."""
+        v = np.zeros(3 * self.nvert)
+        _call("rbc3d_wall_cell_velocity", self.h, v)
+        return v
+
+    def wall_vertex_normals(self):
+        n = np.zeros(3 * self.nvert)
+        _call("rbc3d_wall_vertex_normals", self.h, n)
+        return n
+
+    def noslip_wall_learned(self, mlp, rtol=1e-8, maxit=60):
+        """NoSlipWall warm-started by the network: (niter, residual, converged, max |v|
+        after, fallback used)."""
+        ni, rn, cv, vm, fb = C.c_int(), C.c_double(), C.c_int(), C.c_double(), C.c_int()
+        _call("rbc3d_noslip_wall_learned", self.h, mlp.h, rtol, maxit, C.byref(ni), C.byref(rn),
+              C.byref(cv), C.byref(vm), C.byref(fb))
+        return ni.value, rn.value, bool(cv.value), vm.value, bool(fb.value)
+
     def solve_rbc_vel(self):
         v = np.zeros(3 * self.np)
         ni = C.c_int()
@@ -409,6 +457,56 @@ class System:
             out[nm.value.decode()] = (ms.value, n.value)
             s += 1
         return out
+
+
+class Mlp:
+    """The per-wall-node network of the learned NoSlipWall warm start (rbc3d_mlp_*).
+    Device arrays are passed as raw pointers (e.g. torch tensor data_ptr())."""
+
+    def __init__(self):
+        self.h = C.c_void_p()
+        _call("rbc3d_mlp_create", C.byref(self.h))
+        self.nparams = lib().rbc3d_mlp_nparams()
+
+    def close(self):
+        if self.h:
+            lib().rbc3d_mlp_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.nparams
+        _call("rbc3d_mlp_set_params", self.h, p)
+
+    def get_params(self):
+        p = np.zeros(self.nparams, np.float32)
+        _call("rbc3d_mlp_get_params", self.h, p)
+        return p
+
+    def set_norm(self, mean_in, std_in, scale_out):
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+        _call("rbc3d_mlp_set_norm", self.h, f(mean_in), f(std_in), f(scale_out))
+
+    def forward_dev(self, n, d_x, d_y):
+        _call("rbc3d_mlp_forward_dev", self.h, n, C.c_void_p(d_x), C.c_void_p(d_y))
+
+    def grad_dev(self, n, d_x, d_t, d_grad):
+        loss = C.c_double()
+        _call("rbc3d_mlp_grad_dev", self.h, n, C.c_void_p(d_x), C.c_void_p(d_t),
+              C.c_void_p(d_grad), C.byref(loss))
+        return loss.value
+
+    def adam_step_dev(self, n, d_x, d_t, lr, want_loss=False):
+        loss = C.c_double()
+        _call("rbc3d_mlp_adam_step_dev", self.h, n, C.c_void_p(d_x), C.c_void_p(d_t),
+              C.c_float(lr), C.byref(loss) if want_loss else None)
+        return loss.value if want_loss else None
 
 
 def from_workload(w):

@@ -296,8 +296,9 @@ extern "C" int rbc3d_noslip_wall(rbc3d_ctx *c, double rtol, int maxit, const dou
   if (x0nz) HIP_TRY(hipMemcpyAsync(x.p, finc0, sizeof(double) * nd, hipMemcpyHostToDevice, st));
   int reason = 0;
   auto mv = [c](const double *u, double *b) { return rbc_wall_matmult_dev(c, u, b); };
+  c->gmres.history.clear();
   RBC_TRY(rbc_gmres(c, nd, mv, rhs.p, x.p, x0nz, rtol, 1e-50, 1e5, maxit, 30, niter, rnorm, &reason,
-                    nullptr));
+                    &c->gmres.history));
   *converged = reason > 0;
   // f = f0 + finc (AssembleArray(finc, finc1D, -1))
   k_add_assembled<<<(nv + 255) / 256, 256, 0, st>>>(nv, w.d_glb.p, f0.p, x.p, w.d_f.p);
@@ -313,6 +314,220 @@ extern "C" int rbc3d_noslip_wall(rbc3d_ctx *c, double rtol, int maxit, const dou
   rhs.release();
   x.release();
   tmpv.release();
+  return RBC3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// NoSlipWall with the learned warm start (no counterpart in the reference): the
+// per-wall-node MLP predicts the converged traction from the cells' velocity at the
+// vertex, the previous traction and the vertex normal.  GMRES's initial guess is the
+// combination of the predicted increment p and the three uniform tractions e_x, e_y,
+// e_z with the least residual, x0 = c0 p + sum_d c_d e_d minimising |b - L x0| (the
+// images L e_d are computed once per wall and precision): a per-vertex network is
+// accurate locally, while a small global bias of its traction would show up in the
+// residual at full strength.  Plain GMRES from zero is the fallback when that
+// residual is not below |b| or the warm-started solve does not converge.
+int rbc_mlp_forward(rbc3d_mlp *m, int n, const float *d_x, float *d_y, hipStream_t st);
+int rbc_wall_cell_vel_dev(rbc3d_ctx *c, double *d_out);
+int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double c1, int nT, const double *d_tx, const int *d_tindx,
+                             const int *d_act, const double *d_A, double *d_v, bool wall_targets);
+
+__global__ void k_mlp_features(int nv, const double *__restrict__ uc, const double *__restrict__ f0,
+                               const double *__restrict__ vn, float *X) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nv) return;
+  for (int d = 0; d < 3; d++) {
+    X[(size_t)9 * i + d] = (float)uc[i + (size_t)nv * d];
+    X[(size_t)9 * i + 3 + d] = (float)f0[i + (size_t)nv * d];
+    X[(size_t)9 * i + 6 + d] = (float)vn[i + (size_t)nv * d];
+  }
+}
+// x0 on the unique unknowns: the last vertex of each global index supplies it
+__global__ void k_mlp_x0(int nu, const int *__restrict__ last, const float *__restrict__ Y,
+                         double *x0) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nu) return;
+  const int i = last[g];
+  for (int d = 0; d < 3; d++) x0[3 * g + d] = (double)Y[(size_t)3 * i + d];
+}
+__global__ void k_fill_uniform(int nu, double *e) {  // e_d: unit traction along d
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= nu) return;
+  for (int d = 0; d < 3; d++)
+    for (int k = 0; k < 3; k++) e[(size_t)d * 3 * nu + 3 * g + k] = k == d ? 1. : 0.;
+}
+// x = c0 x + c_d on component d (the uniform modes), r = b - (c0 Lx + sum c_d L e_d)
+__global__ void k_combine_x0(int nu, double c0, double c1, double c2, double c3, double *x,
+                             const double *__restrict__ b, const double *__restrict__ V,
+                             size_t ld, double *r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= 3 * nu) return;
+  const int d = i % 3;
+  x[i] = c0 * x[i] + (d == 0 ? c1 : (d == 1 ? c2 : c3));
+  r[i] = b[i] - (c0 * V[i] + c1 * V[ld + i] + c2 * V[2 * ld + i] + c3 * V[3 * ld + i]);
+}
+// solve the k x k symmetric system G c = h by Gaussian elimination with partial
+// pivoting; directions whose pivot is below tol times the largest diagonal get c = 0
+static void small_solve(int k, std::vector<double> G, std::vector<double> h, std::vector<double> &c) {
+  double dmax = 0;
+  for (int i = 0; i < k; i++) dmax = std::max(dmax, fabs(G[i * k + i]));
+  std::vector<int> alive(k, 1);
+  for (int i = 0; i < k; i++)
+    if (!(fabs(G[i * k + i]) > 1e-12 * dmax)) alive[i] = 0;
+  std::vector<int> idx;
+  for (int i = 0; i < k; i++)
+    if (alive[i]) idx.push_back(i);
+  const int m = (int)idx.size();
+  std::vector<double> A(m * m), y(m);
+  for (int a = 0; a < m; a++) {
+    y[a] = h[idx[a]];
+    for (int b = 0; b < m; b++) A[a * m + b] = G[idx[a] * k + idx[b]];
+  }
+  for (int col = 0; col < m; col++) {
+    int piv = col;
+    for (int r = col + 1; r < m; r++)
+      if (fabs(A[r * m + col]) > fabs(A[piv * m + col])) piv = r;
+    if (!(fabs(A[piv * m + col]) > 1e-14 * dmax)) continue;
+    if (piv != col) {
+      for (int b = 0; b < m; b++) std::swap(A[col * m + b], A[piv * m + b]);
+      std::swap(y[col], y[piv]);
+    }
+    for (int r = col + 1; r < m; r++) {
+      const double f = A[r * m + col] / A[col * m + col];
+      for (int b = col; b < m; b++) A[r * m + b] -= f * A[col * m + b];
+      y[r] -= f * y[col];
+    }
+  }
+  std::vector<double> z(m, 0.);
+  for (int r = m - 1; r >= 0; r--) {
+    if (!(fabs(A[r * m + r]) > 1e-14 * dmax)) continue;
+    double s = y[r];
+    for (int b = r + 1; b < m; b++) s -= A[r * m + b] * z[b];
+    z[r] = s / A[r * m + r];
+  }
+  c.assign(k, 0.);
+  for (int a = 0; a < m; a++) c[idx[a]] = z[a];
+}
+
+__global__ void k_add3(int n, double *v, const double *__restrict__ a, double s0, double s1,
+                       double s2) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  v[i] += a[i] + s0;
+  v[i + n] += a[i + n] + s1;
+  v[i + 2 * (size_t)n] += a[i + 2 * (size_t)n] + s2;
+}
+
+extern "C" int rbc3d_noslip_wall_learned(rbc3d_ctx *c, rbc3d_mlp *m, double rtol, int maxit,
+                                         int *niter, double *rnorm, int *converged,
+                                         double *vres_max, int *fallback) {
+  if (!c || !m || !niter || !rnorm || !converged || !fallback) return RBC3D_EINVAL;
+  WallData &w = c->wall;
+  if (!w.nvert) return RBC3D_ESTATE;
+  if (!w.prepared) RBC_TRY(rbc_wall_prepare(c));
+  hipStream_t st = c->stream;
+  const int nv = w.nvert, nd = w.ndof, nu = nd / 3;
+  DBuf<double> f0, uc, rhs, x, r0, LV;
+  DBuf<float> X, Y;
+  RBC_TRY(f0.ensure((size_t)3 * nv));
+  RBC_TRY(uc.ensure((size_t)3 * nv));
+  RBC_TRY(rhs.ensure(nd + 1));
+  RBC_TRY(x.ensure(nd + 1));
+  RBC_TRY(r0.ensure(nd + 1));
+  RBC_TRY(X.ensure((size_t)9 * nv));
+  RBC_TRY(Y.ensure((size_t)3 * nv));
+  HIP_TRY(hipMemcpyAsync(f0.p, w.d_f.p, sizeof(double) * 3 * nv, hipMemcpyDeviceToDevice, st));
+  // residual velocity = cells' velocity + walls' velocity from f0 + background flow,
+  // the cells' part kept as the network's input
+  RBC_TRY(rbc_wall_cell_vel_dev(c, uc.p));
+  HIP_TRY(hipMemsetAsync(c->d_v.p, 0, sizeof(double) * 3 * nv, st));
+  const double c1 = 1. / (4. * RBC_PI);
+  RBC_TRY(rbc_add_int_on_walls_dev(c, c1, nv, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
+                                   c->d_v.p, true));
+  RBC_TRY(rbc_pme_spread(c, c1, 0., 0, 1));
+  RBC_TRY(rbc_pme_transform(c));
+  RBC_TRY(rbc_pme_gather(c, nv, c->d_tx.p, c->d_tact.p, c->d_tA.p, c->d_v.p));
+  k_add3<<<(nv + 255) / 256, 256, 0, st>>>(nv, c->d_v.p, uc.p, c->conf.vBkg[0], c->conf.vBkg[1],
+                                           c->conf.vBkg[2]);
+  k_neg_assemble<<<(nu + 255) / 256, 256, 0, st>>>(nu, nv, w.d_glb.p + nv, c->d_v.p, rhs.p);
+  // the network's prediction of the increment
+  k_mlp_features<<<(nv + 255) / 256, 256, 0, st>>>(nv, uc.p, f0.p, w.d_vnorm.p, X.p);
+  HIP_TRY(hipGetLastError());
+  RBC_TRY(rbc_mlp_forward(m, nv, X.p, Y.p, st));
+  k_mlp_x0<<<(nu + 255) / 256, 256, 0, st>>>(nu, w.d_glb.p + nv, Y.p, x.p);
+  HIP_TRY(hipGetLastError());
+  // least-residual combination of the prediction and the uniform tractions:
+  // V = [L p, L e_x, L e_y, L e_z], G = V^T V, h = V^T b, x0 = [p e] c
+  const size_t ld = (size_t)nd;
+  if (w.d_lmodes.n < 3 * ld || w.lmodes_prec != c->prec || !w.lmodes_ready) {
+    DBuf<double> e;
+    RBC_TRY(e.ensure(3 * ld));
+    RBC_TRY(w.d_lmodes.ensure(3 * ld));
+    k_fill_uniform<<<(nu + 255) / 256, 256, 0, st>>>(nu, e.p);
+    HIP_TRY(hipGetLastError());
+    for (int d = 0; d < 3; d++) RBC_TRY(rbc_wall_matmult_dev(c, e.p + d * ld, w.d_lmodes.p + d * ld));
+    HIP_TRY(hipStreamSynchronize(st));
+    w.lmodes_prec = c->prec;
+    w.lmodes_ready = true;
+  }
+  RBC_TRY(LV.ensure(4 * ld));
+  RBC_TRY(rbc_wall_matmult_dev(c, x.p, LV.p));
+  HIP_TRY(hipMemcpyAsync(LV.p + ld, w.d_lmodes.p, sizeof(double) * 3 * ld, hipMemcpyDeviceToDevice,
+                         st));
+  std::vector<double> G(16), h(4), cc;
+  for (int j = 0; j < 4; j++) {
+    double col[4];
+    RBC_TRY(dots(st, c->gmres, nd, 4, LV.p, ld, LV.p + j * ld, col));
+    for (int i = 0; i < 4; i++) G[i * 4 + j] = col[i];
+  }
+  {
+    double hb[4];
+    RBC_TRY(dots(st, c->gmres, nd, 4, LV.p, ld, rhs.p, hb));
+    for (int i = 0; i < 4; i++) h[i] = hb[i];
+  }
+  small_solve(4, G, h, cc);
+  k_combine_x0<<<(nd + 255) / 256, 256, 0, st>>>(nu, cc[0], cc[1], cc[2], cc[3], x.p, rhs.p, LV.p, ld,
+                                                 r0.p);
+  HIP_TRY(hipGetLastError());
+  double b2, r2;
+  RBC_TRY(dots(st, c->gmres, nd, 1, rhs.p, (size_t)nd, rhs.p, &b2));
+  RBC_TRY(dots(st, c->gmres, nd, 1, r0.p, (size_t)nd, r0.p, &r2));
+  auto mv = [c](const double *u, double *b) { return rbc_wall_matmult_dev(c, u, b); };
+  int reason = 0, its = 0;
+  *fallback = 0;
+  c->gmres.history.clear();
+  if (r2 < b2) {
+    RBC_TRY(rbc_gmres(c, nd, mv, rhs.p, x.p, true, rtol, 1e-50, 1e5, maxit, 30, &its, rnorm, &reason,
+                      &c->gmres.history));
+  }
+  *niter = its;
+  if (reason <= 0) {  // plain GMRES from zero
+    *fallback = 1;
+    int its2 = 0;
+    RBC_TRY(rbc_gmres(c, nd, mv, rhs.p, x.p, false, rtol, 1e-50, 1e5, maxit, 30, &its2, rnorm,
+                      &reason, &c->gmres.history));
+    *niter += its2;
+  }
+  *converged = reason > 0;
+  k_add_assembled<<<(nv + 255) / 256, 256, 0, st>>>(nv, w.d_glb.p, f0.p, x.p, w.d_f.p);
+  HIP_TRY(hipGetLastError());
+  if (vres_max) {
+    DBuf<double> tmpv;
+    RBC_TRY(rbc_wall_residual_dev(c));
+    RBC_TRY(tmpv.ensure(1));
+    k_absmax<<<1, 256, 0, st>>>(3 * nv, c->d_v.p, tmpv.p);
+    HIP_TRY(hipMemcpyAsync(vres_max, tmpv.p, sizeof(double), hipMemcpyDeviceToHost, st));
+    HIP_TRY(hipStreamSynchronize(st));
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  return RBC3D_OK;
+}
+
+extern "C" int rbc3d_gmres_history(rbc3d_ctx *c, double *h, int maxn, int *n) {
+  if (!c || !n || maxn < 0 || (maxn > 0 && !h)) return RBC3D_EINVAL;
+  const std::vector<double> &hv = c->gmres.history;
+  *n = (int)hv.size();
+  for (int i = 0; i < std::min(maxn, *n); i++) h[i] = hv[i];
   return RBC3D_OK;
 }
 

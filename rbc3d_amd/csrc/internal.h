@@ -89,6 +89,7 @@ struct GridType {
 // GMRES work space (Krylov basis, partial dot products), kept across solves
 struct GmresWork {
   DBuf<double> V, w, part, hd;
+  std::vector<double> history;  // residual estimates of the last solve
 };
 
 // Stage timing (HIP events on the context's stream, enabled on request)
@@ -128,7 +129,8 @@ struct WallData {
   int nvert = 0, nele = 0, id = 0, ndof = 0;
   std::vector<double> x;
   std::vector<int> e2v, v2v, glb;
-  DBuf<double> d_x, d_f, d_area, d_eps, d_cen;
+  std::vector<double> vnorm;  // area-weighted vertex normals (nvert,3)
+  DBuf<double> d_x, d_f, d_area, d_eps, d_cen, d_vnorm;
   DBuf<int> d_e2v, d_glb;
   // cell list of wall sources (triangle centroids)
   DBuf<int> d_bin_start, d_order;
@@ -138,6 +140,11 @@ struct WallData {
   DBuf<float> d_valf;  // split form: the 3x3 blocks in FP32
   long long nnz = 0;
   bool prepared = false, valf_ready = false;
+  // images of the three uniform tractions under the no-slip operator (learned warm
+  // start), for the precision mode they were computed in
+  DBuf<double> d_lmodes;
+  int lmodes_prec = -1;
+  bool lmodes_ready = false;
 };
 
 struct rbc3d_ctx {

@@ -503,7 +503,7 @@ extern "C" int rbc3d_set_wall(rbc3d_ctx *c, int nvert, int nele, const double *x
   for (int i = 0; i < nvert; i++) w.glb[i] = w.v2v[i] == 0 ? ++p : w.glb[w.v2v[i] - 1];
   w.ndof = 3 * p;
   // Wall_ComputeGeometry (common/ModWall.F90:118-145) and centroids
-  std::vector<double> area(nele), eps(nele), cen((size_t)3 * nele);
+  std::vector<double> area(nele), eps(nele), cen((size_t)3 * nele), vn((size_t)3 * p, 0.);
   for (int e = 0; e < nele; e++) {
     double xe[3][3];
     for (int l = 0; l < 3; l++)
@@ -519,12 +519,23 @@ extern "C" int rbc3d_set_wall(rbc3d_ctx *c, int nvert, int nele, const double *x
     double n = sqrt(cr[0] * cr[0] + cr[1] * cr[1] + cr[2] * cr[2]);
     area[e] = 0.5 * n;
     eps[e] = sqrt(area[e]);
+    // area-weighted vertex normals, merged over the duplicated seam vertices
+    for (int l = 0; l < 3; l++) {
+      const int gi = w.glb[e2v[e + (size_t)nele * l] - 1] - 1;
+      for (int d = 0; d < 3; d++) vn[3 * (size_t)gi + d] += cr[d];
+    }
     for (int d = 0; d < 3; d++) cen[e + (size_t)nele * d] = (1.0 / 3) * ((xe[0][d] + xe[1][d]) + xe[2][d]);
   }
   // dedup map for AssembleArray (common/ModNoSlip.F90:361-384): the last vertex with a
   // given global index supplies that unknown's value
   std::vector<int> last(p, -1);
   for (int i = 0; i < nvert; i++) last[w.glb[i] - 1] = i;
+  w.vnorm.assign((size_t)3 * nvert, 0.);
+  for (int i = 0; i < nvert; i++) {
+    const double *v = &vn[3 * (size_t)(w.glb[i] - 1)];
+    const double nn = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    for (int d = 0; d < 3; d++) w.vnorm[i + (size_t)nvert * d] = nn > 0 ? v[d] / nn : 0.;
+  }
   hipStream_t st = c->stream;
   RBC_TRY(w.d_x.ensure((size_t)3 * nvert));
   RBC_TRY(w.d_f.ensure((size_t)3 * nvert));
@@ -533,6 +544,8 @@ extern "C" int rbc3d_set_wall(rbc3d_ctx *c, int nvert, int nele, const double *x
   RBC_TRY(w.d_cen.ensure((size_t)3 * nele));
   RBC_TRY(w.d_e2v.ensure((size_t)3 * nele));
   RBC_TRY(w.d_glb.ensure(nvert + p + 1));
+  RBC_TRY(w.d_vnorm.ensure((size_t)3 * nvert));
+  HIP_TRY(hipMemcpy(w.d_vnorm.p, w.vnorm.data(), sizeof(double) * 3 * nvert, hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(w.d_x.p, x, sizeof(double) * 3 * nvert, hipMemcpyHostToDevice));
   HIP_TRY(hipMemset(w.d_f.p, 0, sizeof(double) * 3 * nvert));
   HIP_TRY(hipMemcpy(w.d_area.p, area.data(), sizeof(double) * nele, hipMemcpyHostToDevice));
@@ -550,6 +563,7 @@ extern "C" int rbc3d_set_wall(rbc3d_ctx *c, int nvert, int nele, const double *x
   bins.release();
   w.prepared = false;
   w.valf_ready = false;
+  w.lmodes_ready = false;
   HIP_TRY(hipStreamSynchronize(st));
   return RBC3D_OK;
 }
@@ -635,6 +649,7 @@ int rbc_wall_prepare(rbc3d_ctx *c) {
   w.nnz = 9LL * nb;
   w.prepared = true;
   w.valf_ready = false;
+  w.lmodes_ready = false;
   return RBC3D_OK;
 }
 
@@ -799,6 +814,44 @@ __global__ void k_add_const(double *v, int n, double s0, double s1, double s2) {
   v[i] += s0;
   v[i + n] += s1;
   v[i + 2 * (size_t)n] += s2;
+}
+
+// The cells' part of Compute_Wall_Residual_Vel (common/ModNoSlip.F90:153-193): the
+// cell sources' single- and double-layer velocity (real space and PME) at the wall
+// vertices, into d_out (nvert,3)
+int rbc_wall_cell_vel_dev(rbc3d_ctx *c, double *d_out) {
+  WallData &w = c->wall;
+  if (!w.nvert) return RBC3D_ESTATE;
+  hipStream_t st = c->stream;
+  int nv = w.nvert;
+  RBC_TRY(wall_targets_dev(c));
+  HIP_TRY(hipMemsetAsync(d_out, 0, sizeof(double) * 3 * nv, st));
+  if (!c->nrbc) return RBC3D_OK;
+  double c1 = 1. / (4. * RBC_PI), c2 = 1. / (4. * RBC_PI);
+  RBC_TRY(rbc_add_int_on_rbcs_dev(c, c1, c2, nv, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
+                                  d_out, false));
+  RBC_TRY(rbc_pme_init(c));
+  RBC_TRY(rbc_pme_spread(c, c1, c2, 1, 0));
+  RBC_TRY(rbc_pme_transform(c));
+  RBC_TRY(rbc_pme_gather(c, nv, c->d_tx.p, c->d_tact.p, c->d_tA.p, d_out));
+  return RBC3D_OK;
+}
+
+extern "C" int rbc3d_wall_cell_velocity(rbc3d_ctx *c, double *v) {
+  if (!c || !v) return RBC3D_EINVAL;
+  RBC_TRY(c->d_shbuf2.ensure((size_t)3 * c->wall.nvert + 1));
+  RBC_TRY(rbc_wall_cell_vel_dev(c, c->d_shbuf2.p));
+  HIP_TRY(hipMemcpyAsync(v, c->d_shbuf2.p, sizeof(double) * 3 * c->wall.nvert,
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RBC3D_OK;
+}
+
+extern "C" int rbc3d_wall_vertex_normals(rbc3d_ctx *c, double *n) {
+  if (!c || !n) return RBC3D_EINVAL;
+  if (!c->wall.nvert) return RBC3D_ESTATE;
+  std::copy(c->wall.vnorm.begin(), c->wall.vnorm.end(), n);
+  return RBC3D_OK;
 }
 
 // Compute_Wall_Residual_Vel (common/ModNoSlip.F90:153-193) into d_v (nvert,3)
