@@ -84,13 +84,23 @@ def cpu_baseline(w, sample_every=32, threads=None):
     return total, t, threads
 
 
-def timestep_bench(steps, warmup):
+MLP_C3 = os.path.join(ROOT, "rbc3d_amd", "data", "mlp_C3.npz")
+
+
+def timestep_bench(steps, warmup, maxit=4000):
     """One timestep of the 200-cell tube at viscosity ratio 1 (ModTimeInt.F90:716-808 +
     ModNoSlip.F90:45-150): Solve_RBC_Vel (right-hand side: cell single layer, walls,
-    PME; no cell GMRES at ratio 1; density g := v) then NoSlipWall (residual velocity,
-    wall-traction GMRES to 1e-8 relative residual, at most 60 iterations, x0 = 0, update,
-    monitoring residual).  The wall traction is reset to zero before every step, as at
-    the first step of a run, so every step does the same work."""
+    PME; no cell GMRES at ratio 1; density g := v), NoSlipWall (residual velocity,
+    wall-traction GMRES to 1e-8 relative residual, update), and Solve_RBC_Vel again with
+    the new wall traction (the velocity the step hands on).  The wall traction is reset
+    to zero before every step, as at the first step of a run, so every step does the
+    same work.
+
+    Forms: "plain" = FP64 throughout, GMRES from zero; "chosen" = the split form plus
+    the learned warm start (rbc3d_noslip_wall_learned with the network trained by
+    python -m rbc3d_amd.learned train), GMRES to the same 1e-8; "split_rtol1e-6" = the
+    split form from zero stopped at 1e-6, the tolerance whose velocity error stays
+    below 1e-6.  Each form's cell velocity is compared with the plain form's."""
     import ctypes as C
     import torch
     import rbc3d_amd
@@ -103,43 +113,69 @@ def timestep_bench(steps, warmup):
     nnz = C.c_longlong()
     rbc3d_amd._call("rbc3d_wall_self_nnz", g.h, C.byref(nnz))
     stream = torch.cuda.ExternalStream(g.stream())
-    ni, rn = C.c_int(), C.c_double()
-    total_ms, its = 0.0, []
-    res = None
-    for k in range(warmup + steps):
-        if k == warmup:
-            g.timing(-1)
-            g.timing(1)
-        g.set_wall_density(f0)
-        g.sync()
-        e0 = torch.cuda.Event(enable_timing=True)
-        e1 = torch.cuda.Event(enable_timing=True)
-        t0 = time.perf_counter()
-        e0.record(stream)
-        rbc3d_amd._call("rbc3d_solve_rbc_vel", g.h, None, C.byref(ni), C.byref(rn))
-        res = g.noslip_wall(1e-8, 60)
-        e1.record(stream)
-        g.sync()
-        wall = time.perf_counter() - t0
-        if k >= warmup:
-            total_ms += max(e0.elapsed_time(e1), 1e3 * wall)
-            its.append(res[0])
-    stages = g.stage_times()
-    g.timing(0)
+    mlp = None
+    if os.path.exists(MLP_C3):
+        from rbc3d_amd import learned
+        mlp = learned.load_model(MLP_C3)
+
+    def run(form):
+        prec = g.PREC_FP64 if form == "plain" else g.PREC_SPLIT
+        rtol = 1e-6 if form == "split_rtol1e-6" else 1e-8
+        g.set_precision(prec)
+        total_ms, its, res, v = 0.0, [], None, None
+        for k in range(warmup + steps):
+            if k == warmup:
+                g.timing(-1)
+                g.timing(1)
+            g.set_wall_density(f0)
+            g.sync()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            t0 = time.perf_counter()
+            e0.record(stream)
+            g.solve_rbc_vel()
+            if form == "chosen":
+                res = g.noslip_wall_learned(mlp, rtol, maxit)
+            else:
+                res = g.noslip_wall(rtol, maxit)
+            v, _, _ = g.solve_rbc_vel()
+            e1.record(stream)
+            g.sync()
+            wall = time.perf_counter() - t0
+            if k >= warmup:
+                total_ms += max(e0.elapsed_time(e1), 1e3 * wall)
+                its.append(res[0])
+        stages = g.stage_times()
+        g.timing(0)
+        g.set_precision(g.PREC_FP64)
+        out = {"s_per_timestep": round(total_ms / steps * 1e-3, 5), "rtol": rtol,
+               "gmres_iterations": its, "gmres_residual": res[1], "gmres_converged": res[2],
+               "wall_velocity_after": res[3],
+               "stages_ms_per_step": {k: round(v_[0] / steps, 4) for k, v_ in stages.items() if v_[1]}}
+        if form == "chosen":
+            out["fallback"] = res[4]
+        return out, v
+
     cells = w.cells
-    return {
+    out = {
         "workload": f"C3: {cells.ncell} biconcave cells (radius 1.722, nlat 13x26, 25% "
                     f"haematocrit) in an R=8, Lz=32 periodic tube, {nv} wall vertices "
                     f"({w.wall[3]} Tri3), box {w.Lb}, PME {w.Nb}, viscosity ratio 1",
-        "s_per_timestep": round(total_ms / steps * 1e-3, 5),
-        "steps": steps,
-        "gmres_iterations": its,
-        "gmres_residual": res[1],
-        "gmres_converged": res[2],
-        "wall_velocity_after": res[3],
-        "wall_self_matrix_nnz": int(nnz.value),
-        "stages_ms_per_step": {k: round(v[0] / steps, 4) for k, v in stages.items() if v[1]},
-    }
+        "steps": steps, "wall_self_matrix_nnz": int(nnz.value)}
+    plain, vp = run("plain")
+    out["plain"] = plain
+    forms = ["split_rtol1e-6"] + (["chosen"] if mlp is not None else [])
+    for form in forms:
+        r, v = run(form)
+        r["velocity_rel_err_vs_plain"] = float(np.abs(v - vp).max() / np.abs(vp).max())
+        r["speedup_vs_plain"] = round(plain["s_per_timestep"] / r["s_per_timestep"], 3)
+        out[form] = r
+    if mlp is None:
+        out["chosen"] = {"skipped": f"no trained network at {MLP_C3}"}
+    # the headline fields (plain FP64, the reference's arithmetic)
+    out["s_per_timestep"] = plain["s_per_timestep"]
+    out["gmres_iterations"] = plain["gmres_iterations"]
+    return out
 
 
 def c5_bench(steps, warmup):
@@ -268,6 +304,14 @@ def main():
     ms_reuse, stages_reuse = timed_matvecs(True, args.steps, 1)
     npairs, _ = sysg.pair_stats()
     nproj, nnear = sysg.near_stats()
+    b_plain = b.clone()
+    # the split form on the same vector: time and error against the plain result
+    sysg.set_precision(sysg.PREC_SPLIT)
+    ms_split, stages_split = timed_matvecs(False, args.steps, 1)
+    ms_split_reuse, _ = timed_matvecs(True, args.steps, 1)
+    torch.cuda.synchronize()
+    split_err = float((b - b_plain).abs().max() / b_plain.abs().max())
+    sysg.set_precision(sysg.PREC_FP64)
 
     # dominant kernel and its roofline: the real-space pair kernel is FP64-compute
     # bound; algorithmic work = 28 flop per in-cutoff (target, source) pair for the
@@ -306,6 +350,15 @@ def main():
         "dominant_stage": {"name": dom[0], "ms_per_step": round(dom[1][0] / args.steps, 4)},
         "stages_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in stages.items() if v[1]},
         "near_singular": {"projections": int(nproj), "corrections": int(nnear)},
+        "chosen": {
+            "what": "split form (FP32 pair sums with FP64 near pairs, FP32 PME, split-BF16 "
+                    "MFMA Legendre GEMMs; singular quadrature FP64), same matvec",
+            "ms_per_step": round(ms_split, 4),
+            "cell_matvecs_per_s": round(ncell / (ms_split * 1e-3), 3),
+            "geometry_reused_ms_per_step": round(ms_split_reuse, 4),
+            "rel_err_vs_plain": split_err,
+            "stages_ms_per_step": {k: round(v[0] / args.steps, 4)
+                                   for k, v in stages_split.items() if v[1]}},
         "geometry_reused": {
             "what": "matvecs 2..K of a GMRES solve at fixed geometry: neighbouring-cell "
                     "lists, target binning and closest-point projections from the first",

@@ -17,20 +17,40 @@
 #define G_T 256
 #define G_MAXV 64
 
-// partial dot products of w with V[0..k) (column stride ld), one row of partials per block
+// partial dot products of w with V[0..k) (column stride ld), one row of partials per
+// block: each thread accumulates eight dot products at a time over its elements, the
+// wavefronts reduce them with shuffles and the block adds its wavefronts in order
+// (one pass over w per eight vectors, two barriers per eight vectors)
 __global__ void __launch_bounds__(G_T) k_mdot(int n, int k, const double *__restrict__ V, size_t ld,
                                               const double *__restrict__ w, double *part) {
-  __shared__ double red[G_T];
-  for (int j = 0; j < k; j++) {
-    double s = 0;
-    for (int i = blockIdx.x * G_T + threadIdx.x; i < n; i += gridDim.x * G_T) s += V[j * ld + i] * w[i];
-    red[threadIdx.x] = s;
-    __syncthreads();
-    for (int o = G_T / 2; o > 0; o >>= 1) {
-      if (threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
-      __syncthreads();
+  __shared__ double red[8][G_T / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int j0 = 0; j0 < k; j0 += 8) {
+    const int nj = min(8, k - j0);
+    double s[8];
+#pragma unroll
+    for (int q = 0; q < 8; q++) s[q] = 0.;
+    for (int i = blockIdx.x * G_T + threadIdx.x; i < n; i += gridDim.x * G_T) {
+      const double wi = w[i];
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        if (q < nj) s[q] += V[(size_t)(j0 + q) * ld + i] * wi;
     }
-    if (threadIdx.x == 0) part[(size_t)j * gridDim.x + blockIdx.x] = red[0];
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      double v = s[q];
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+      s[q] = v;
+    }
+    if (lane == 0)
+#pragma unroll
+      for (int q = 0; q < 8; q++) red[q][wv] = s[q];
+    __syncthreads();
+    if (threadIdx.x < nj) {
+      double t = 0.;
+      for (int u = 0; u < G_T / 64; u++) t += red[threadIdx.x][u];
+      part[(size_t)(j0 + threadIdx.x) * gridDim.x + blockIdx.x] = t;
+    }
     __syncthreads();
   }
 }

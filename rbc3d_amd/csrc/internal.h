@@ -140,6 +140,12 @@ struct WallData {
   DBuf<float> d_valf;  // split form: the 3x3 blocks in FP32
   long long nnz = 0;
   bool prepared = false, valf_ready = false;
+  // the wall-only single-layer PME spread as a fixed sparse map (the centroids do not
+  // move): per grid point (CSR over all points) the source triangles and weights
+  DBuf<int> d_wsp_ptr, d_wsp_src;
+  DBuf<double> d_wsp_w, d_wsp_f;
+  DBuf<float> d_wsp_wf;
+  bool wsp_ready = false;
   // images of the three uniform tractions under the no-slip operator (learned warm
   // start), for the precision mode they were computed in
   DBuf<double> d_lmodes;
@@ -211,6 +217,7 @@ struct rbc3d_ctx {
   int tw_off[3];
   DBuf<int> d_pme_key, d_pme_order, d_pme_start;
   DBuf<double> d_pme_src;  // sorted spread records
+  DBuf<double> d_spread_tiles;  // per-brick tiles of the two-pass single-layer spread
   // SH scratch
   DBuf<double> d_shbuf, d_shbuf2;
   // wall

@@ -317,6 +317,289 @@ __global__ void __launch_bounds__(SPR_T) k_spread(SpreadArgs<T> a) {
   }
 }
 
+// Single-layer spread in two passes (used when only the three single-layer fields
+// are spread, P <= SPT_PMAX and every grid size is at least 3 SPR_B: the wall-only
+// spread of every no-slip GMRES iteration).  Pass 1: one workgroup per source
+// brick accumulates its own sources into an LDS tile of (SPR_B + P - 1)^3 points,
+// the tile of points their stencils reach, and stores it; every source is handled
+// once.  Pass 2: every grid point adds the tiles that reach it, in increasing brick
+// order.  Within pass 1 each wavefront owns a fixed set of tile z-planes and lanes
+// take a plane's P x P stencil points, so each tile point is updated by one lane in
+// source order: deterministic, no atomics.
+#define SPT_PMAX 8
+#define SPT_E (SPR_B + SPT_PMAX - 1)  // tile edge bound
+template <typename T>
+__global__ void __launch_bounds__(SPR_T) k_spread_tile(SpreadArgs<T> a, T *tiles) {
+  const int P = a.P, E = SPR_B + P - 1, E3 = E * E * E;
+  __shared__ T acc[3 * SPT_E * SPT_E * SPT_E];  // field f at f * E3
+  __shared__ T srec[SPR_TILE * (3 + 3 * SPT_PMAX + 3)];
+  const int bid = blockIdx.x;
+  const int s0 = a.brick_start[bid], s1 = a.brick_start[bid + 1];
+  if (s0 == s1) return;  // empty brick: no tile (pass 2 skips it)
+  const int bx = bid % a.nbx, by = (bid / a.nbx) % a.nby, bz = bid / (a.nbx * a.nby);
+  const int ib0 = bx * SPR_B, jb0 = by * SPR_B, kb0 = bz * SPR_B;
+  const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, nwv = SPR_T / 64;
+  for (int t = threadIdx.x; t < 3 * E3; t += SPR_T) acc[t] = (T)0;
+  const int rl = 3 + 3 * P + 3;
+  for (int base = s0; base < s1; base += SPR_TILE) {
+    const int cnt = min(SPR_TILE, s1 - base);
+    __syncthreads();
+    for (int t = threadIdx.x; t < cnt * rl; t += SPR_T) {
+      const int q = fd_div(t, a.frl), e = t - q * rl;
+      srec[t] = a.rec[(size_t)a.reclen * a.order[base + q] + e];
+    }
+    __syncthreads();
+    for (int q = 0; q < cnt; q++) {
+      const T *r = srec + q * rl;
+      // the source's origin relative to the brick (0..SPR_B-1 on each axis)
+      const int oi = (int)r[0] - ib0, oj = (int)r[1] - jb0, ok = (int)r[2] - kb0;
+      const T *fv = r + 3 + 3 * P;
+      for (int tz = wv; tz < E; tz += nwv) {  // this wavefront's tile planes
+        const int dk = tz - ok;
+        if (dk < 0 || dk >= P) continue;
+        const T wz = a.c1 * r[3 + 2 * P + dk];
+        for (int t = lane; t < P * P; t += 64) {
+          const int di = t % P, dj = t / P;
+          const int pt = (oi + di) + E * ((oj + dj) + E * tz);
+          const T w = r[3 + di] * r[3 + P + dj] * wz;
+#pragma unroll
+          for (int f = 0; f < 3; f++) acc[f * E3 + pt] += w * fv[f];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  T *out = tiles + (size_t)bid * 3 * E3;
+  for (int t = threadIdx.x; t < 3 * E3; t += SPR_T) out[t] = acc[t];
+}
+
+// pass 2: grid point g sums the tiles of the bricks b with (g - SPR_B b) mod N < E
+template <typename T>
+__global__ void k_spread_merge(SpreadArgs<T> a, const T *__restrict__ tiles) {
+  const size_t ng = (size_t)a.N1 * a.N2 * a.N3;
+  const size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (o >= ng) return;
+  const int P = a.P, E = SPR_B + P - 1, E3 = E * E * E;
+  const int g[3] = {(int)(o % a.N1), (int)((o / a.N1) % a.N2), (int)(o / ((size_t)a.N1 * a.N2))};
+  const int Ns[3] = {a.N1, a.N2, a.N3}, nbs[3] = {a.nbx, a.nby, a.nbz};
+  // candidate bricks per axis in increasing brick index, with the tile coordinate
+  int cb[3][3], cl[3][3], nc[3];
+  for (int ax = 0; ax < 3; ax++) {
+    nc[ax] = 0;
+    int b0 = g[ax] / SPR_B;
+    int cand[3] = {b0 - 2, b0 - 1, b0};
+    for (int k = 0; k < 3; k++) {
+      int b = cand[k];
+      b += b < 0 ? nbs[ax] : 0;
+      int l = g[ax] - SPR_B * b;
+      l += l < 0 ? Ns[ax] : 0;
+      if (l < E) {
+        cb[ax][nc[ax]] = b;
+        cl[ax][nc[ax]] = l;
+        nc[ax]++;
+      }
+    }
+    // keep increasing brick order (a wrapped candidate may come first)
+    for (int i = 1; i < nc[ax]; i++)
+      for (int j = i; j > 0 && cb[ax][j] < cb[ax][j - 1]; j--) {
+        int tb = cb[ax][j], tl = cl[ax][j];
+        cb[ax][j] = cb[ax][j - 1];
+        cl[ax][j] = cl[ax][j - 1];
+        cb[ax][j - 1] = tb;
+        cl[ax][j - 1] = tl;
+      }
+  }
+  T s0 = 0, s1 = 0, s2 = 0;
+  for (int kz = 0; kz < nc[2]; kz++)
+    for (int ky = 0; ky < nc[1]; ky++)
+      for (int kx = 0; kx < nc[0]; kx++) {
+        const int b = cb[0][kx] + a.nbx * (cb[1][ky] + a.nby * cb[2][kz]);
+        if (a.brick_start[b] == a.brick_start[b + 1]) continue;
+        const T *t = tiles + (size_t)b * 3 * E3 + cl[0][kx] + E * (cl[1][ky] + E * cl[2][kz]);
+        s0 += t[0];
+        s1 += t[E3];
+        s2 += t[2 * E3];
+      }
+  a.grid[o] = s0;
+  a.grid[o + ng] = s1;
+  a.grid[o + 2 * ng] = s2;
+}
+
+// Wall-only single-layer spread (every no-slip GMRES iteration) as a fixed sparse
+// map: the sources are the wall triangles' centroids, which do not move, so their
+// stencil origins and B-spline weights are computed once per wall and grid and kept
+// as a CSR matrix over the grid points (entries sorted by point, then by triangle and
+// stencil offset: the summation order at a point is fixed).  Each spread is then one
+// pass over the entries: grid[g] = c1 sum_k w_k fbar(src_k) area(src_k).
+__global__ void k_wsp_keys(int nele, const double *__restrict__ cen, double ih0, double ih1,
+                           double ih2, int N1, int N2, int N3, int P, int *key, int *val) {
+  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  const int P3 = P * P * P;
+  if (t >= (long long)nele * P3) return;
+  const int e = (int)(t / P3), s = (int)(t % P3);
+  const int di = s % P, dj = (s / P) % P, dk = s / (P * P);
+  double w[12];
+  const int i0 = bspline_d(cen[e] * ih0, P, w);
+  const int j0 = bspline_d(cen[e + (size_t)nele] * ih1, P, w);
+  const int k0 = bspline_d(cen[e + 2 * (size_t)nele] * ih2, P, w);
+  const int gi = imod_d(i0 + di, N1), gj = imod_d(j0 + dj, N2), gk = imod_d(k0 + dk, N3);
+  key[t] = gi + N1 * (gj + N2 * gk);
+  val[t] = (int)t;
+}
+__global__ void k_wsp_weights(long long ne, const int *__restrict__ sval, const double *__restrict__ cen,
+                              int nele, double ih0, double ih1, double ih2, int P, int *src,
+                              double *wgt, float *wgtf) {
+  const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k >= ne) return;
+  const int P3 = P * P * P, t = sval[k];
+  const int e = t / P3, s = t % P3;
+  const int di = s % P, dj = (s / P) % P, dk = s / (P * P);
+  double wx[12], wy[12], wz[12];
+  bspline_d(cen[e] * ih0, P, wx);
+  bspline_d(cen[e + (size_t)nele] * ih1, P, wy);
+  bspline_d(cen[e + 2 * (size_t)nele] * ih2, P, wz);
+  const double w = wx[di] * wy[dj] * wz[dk];
+  src[k] = e;
+  wgt[k] = w;
+  wgtf[k] = (float)w;
+}
+// row pointers: ptr[g] = first sorted entry with point >= g
+__global__ void k_wsp_ptr(long long ne, const int *__restrict__ skey, int ng, int *ptr) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g > ng) return;
+  long long lo = 0, hi = ne;
+  while (lo < hi) {
+    const long long mid = (lo + hi) >> 1;
+    if (skey[mid] < g)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  ptr[g] = (int)lo;
+}
+// per-triangle source values: mean vertex traction times area (PME_Distrib_Source
+// :108-130), f(3, nele)
+__global__ void k_wsp_src(int nele, const int *__restrict__ e2v, int nvert,
+                          const double *__restrict__ wf, const double *__restrict__ area,
+                          double *fsrc) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nele) return;
+  const int v0 = e2v[e] - 1, v1 = e2v[e + nele] - 1, v2 = e2v[e + 2 * (size_t)nele] - 1;
+  for (int d = 0; d < 3; d++) {
+    const double f = (1.0 / 3) * (wf[v0 + (size_t)nvert * d] + wf[v1 + (size_t)nvert * d] +
+                                  wf[v2 + (size_t)nvert * d]) * area[e];
+    fsrc[e + (size_t)nele * d] = f;
+  }
+}
+template <typename T, typename W>
+__global__ void k_wsp_spread(int ng, int nele, double c1, const int *__restrict__ ptr,
+                             const int *__restrict__ src, const W *__restrict__ wgt,
+                             const double *__restrict__ fsrc, T *grid) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= ng) return;
+  double s0 = 0, s1 = 0, s2 = 0;
+  const int k0 = ptr[g], k1 = ptr[g + 1];
+  // four entries per step: their index and weight loads, then their source gathers,
+  // are issued together (the row is a chain of dependent loads otherwise)
+  int k = k0;
+  for (; k + 4 <= k1; k += 4) {
+    int e[4];
+    double cw[4], f[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      e[u] = src[k + u];
+      cw[u] = c1 * (double)wgt[k + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int d = 0; d < 3; d++) f[u][d] = fsrc[e[u] + (size_t)nele * d];
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      s0 += cw[u] * f[u][0];
+      s1 += cw[u] * f[u][1];
+      s2 += cw[u] * f[u][2];
+    }
+  }
+  for (; k < k1; k++) {
+    const int e = src[k];
+    const double cw = c1 * (double)wgt[k];
+    s0 += cw * fsrc[e];
+    s1 += cw * fsrc[e + (size_t)nele];
+    s2 += cw * fsrc[e + 2 * (size_t)nele];
+  }
+  grid[g] = (T)s0;
+  grid[g + (size_t)ng] = (T)s1;
+  grid[g + 2 * (size_t)ng] = (T)s2;
+}
+
+static int wall_spread_setup(rbc3d_ctx *c) {
+  WallData &w = c->wall;
+  if (w.wsp_ready) return RBC3D_OK;
+  hipStream_t st = c->stream;
+  const int N1 = c->conf.Nb[0], N2 = c->conf.Nb[1], N3 = c->conf.Nb[2], P = c->conf.P;
+  const long long ne = (long long)w.nele * P * P * P;
+  const int ng = N1 * N2 * N3;
+  if (ne > 0x7fffffffLL) {
+    rbc_set_error("wall spread map too large");
+    return RBC3D_EINVAL;
+  }
+  const double ih0 = N1 / c->Lb[0], ih1 = N2 / c->Lb[1], ih2 = N3 / c->Lb[2];
+  DBuf<int> key, val, skey, sval;
+  RBC_TRY(key.ensure(ne));
+  RBC_TRY(val.ensure(ne));
+  RBC_TRY(skey.ensure(ne));
+  RBC_TRY(sval.ensure(ne));
+  k_wsp_keys<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(w.nele, w.d_cen.p, ih0, ih1, ih2, N1, N2,
+                                                          N3, P, key.p, val.p);
+  HIP_TRY(hipGetLastError());
+  int bits = 1;
+  while ((1LL << bits) < ng) bits++;
+  size_t tmp = 0;
+  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key.p, skey.p, val.p, sval.p, (int)ne, 0,
+                                             bits, st));
+  RBC_TRY(c->d_cub_tmp.ensure(tmp + 16));
+  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->d_cub_tmp.p, tmp, key.p, skey.p, val.p, sval.p,
+                                             (int)ne, 0, bits, st));
+  RBC_TRY(w.d_wsp_ptr.ensure(ng + 1));
+  RBC_TRY(w.d_wsp_src.ensure(ne));
+  RBC_TRY(w.d_wsp_w.ensure(ne));
+  RBC_TRY(w.d_wsp_wf.ensure(ne));
+  RBC_TRY(w.d_wsp_f.ensure(3 * (size_t)w.nele));
+  k_wsp_ptr<<<(ng + 1 + 255) / 256, 256, 0, st>>>(ne, skey.p, ng, w.d_wsp_ptr.p);
+  k_wsp_weights<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(ne, sval.p, w.d_cen.p, w.nele, ih0,
+                                                             ih1, ih2, P, w.d_wsp_src.p,
+                                                             w.d_wsp_w.p, w.d_wsp_wf.p);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
+  w.wsp_ready = true;
+  return RBC3D_OK;
+}
+
+template <typename T>
+static int wall_spread(rbc3d_ctx *c, double c1) {
+  WallData &w = c->wall;
+  RBC_TRY(wall_spread_setup(c));
+  hipStream_t st = c->stream;
+  const int ng = c->conf.Nb[0] * c->conf.Nb[1] * c->conf.Nb[2];
+  rbc_stage_begin(c, ST_SPREAD_PREP);
+  k_wsp_src<<<(w.nele + 255) / 256, 256, 0, st>>>(w.nele, w.d_e2v.p, w.nvert, w.d_f.p, w.d_area.p,
+                                                  w.d_wsp_f.p);
+  rbc_stage_end(c, ST_SPREAD_PREP);
+  rbc_stage_begin(c, ST_SPREAD);
+  if (sizeof(T) == 4)
+    k_wsp_spread<T, float><<<(ng + 255) / 256, 256, 0, st>>>(ng, w.nele, c1, w.d_wsp_ptr.p,
+                                                             w.d_wsp_src.p, w.d_wsp_wf.p,
+                                                             w.d_wsp_f.p, (T *)c->d_grid.p);
+  else
+    k_wsp_spread<T, double><<<(ng + 255) / 256, 256, 0, st>>>(ng, w.nele, c1, w.d_wsp_ptr.p,
+                                                              w.d_wsp_src.p, w.d_wsp_w.p,
+                                                              w.d_wsp_f.p, (T *)c->d_grid.p);
+  rbc_stage_end(c, ST_SPREAD);
+  HIP_TRY(hipGetLastError());
+  return RBC3D_OK;
+}
+
 // brick_start[b] = first sorted position with brick >= b, b = 0..nbr
 __global__ void k_brick_start(int n, const int *__restrict__ skey, int nbr, int *start) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
@@ -338,6 +621,8 @@ static int pme_spread_t(rbc3d_ctx *c, double c1, double c2, int cells, int walls
   int N1 = c->conf.Nb[0], N2 = c->conf.Nb[1], N3 = c->conf.Nb[2], P = c->conf.P;
   c->flag_sl = fabs(c1) > 1.e-10;
   c->flag_dl = fabs(c2) > 1.e-10;
+  if (!(cells && c->nrbc) && walls && c->wall.nvert && c->flag_sl && !c->flag_dl)
+    return wall_spread<T>(c, c1);
   int ncell = (cells && c->nrbc) ? c->np : 0;
   int nele = (walls && c->wall.nvert) ? c->wall.nele : 0;
   int ns = ncell + nele;
@@ -393,7 +678,18 @@ static int pme_spread_t(rbc3d_ctx *c, double c1, double c2, int cells, int walls
   }
   a.frl = make_fastdiv(3 + 3 * P + (c->flag_sl ? 3 : 0) + (c->flag_dl ? 9 : 0));
   rbc_stage_begin(c, ST_SPREAD);
-  if (c->flag_sl && c->flag_dl)
+  // (three or more bricks per axis keep pass 2's candidate bricks distinct)
+  const bool tile_path = c->flag_sl && !c->flag_dl && P <= SPT_PMAX && N1 >= 3 * SPR_B &&
+                         N2 >= 3 * SPR_B && N3 >= 3 * SPR_B;
+  if (tile_path) {
+    const size_t E = SPR_B + P - 1, tsz = 3 * E * E * E;
+    RBC_TRY(c->d_spread_tiles.ensure((tsz * nbr * sizeof(T) + 7) / 8));
+    T *tiles = (T *)c->d_spread_tiles.p;
+    a.frl = make_fastdiv(3 + 3 * P + 3);
+    k_spread_tile<T><<<nbr, SPR_T, 0, st>>>(a, tiles);
+    const size_t ng = (size_t)N1 * N2 * N3;
+    k_spread_merge<T><<<(unsigned)((ng + 255) / 256), 256, 0, st>>>(a, tiles);
+  } else if (c->flag_sl && c->flag_dl)
     k_spread<true, true, T><<<nbr, SPR_T, 0, st>>>(a);
   else if (c->flag_sl)
     k_spread<true, false, T><<<nbr, SPR_T, 0, st>>>(a);

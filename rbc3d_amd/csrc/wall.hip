@@ -355,7 +355,7 @@ __global__ void __launch_bounds__(128) k_wall_reduce(int nvert, const int *__res
                                                      const int *__restrict__ skey,
                                                      const int *__restrict__ sidx,
                                                      const double *__restrict__ eval, int *rowcnt,
-                                                     int *bcol, double *bval, int pass) {
+                                                     int *bcol, double *bval, size_t nbt, int pass) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nvert) return;
   long long s = 3LL * off[i], e = 3LL * off[i + 1];
@@ -366,13 +366,13 @@ __global__ void __launch_bounds__(128) k_wall_reduce(int nvert, const int *__res
     if (first) {
       if (pass) {
         bcol[out + n] = skey[k];
-        for (int q = 0; q < 9; q++) bval[9 * (out + n) + q] = 0.;
+        for (int q = 0; q < 9; q++) bval[q * nbt + out + n] = 0.;
       }
       n++;
     }
     if (pass) {
       const double *src = eval + 9 * (size_t)sidx[k];
-      for (int q = 0; q < 9; q++) bval[9 * (out + n - 1) + q] += src[q];
+      for (int q = 0; q < 9; q++) bval[q * nbt + out + n - 1] += src[q];
     }
   }
   if (!pass) rowcnt[i] = n;
@@ -380,8 +380,9 @@ __global__ void __launch_bounds__(128) k_wall_reduce(int nvert, const int *__res
 
 // v_i += c1 (L f)_i / Acoef_i over wall vertices (SingIntOnWall :136-172)
 // v_i += c1/A_i * sum_k B_k f_(col k): one wavefront per vertex row, lanes over the
-// row's 3x3 blocks (each lane reads its blocks' 9 values, which lie next to those of
-// the neighbouring lanes, and the 3 traction components of the block's column).
+// row's 3x3 blocks.  The blocks are stored as nine component arrays (bval[q][k]), so
+// each of a lane's nine value loads is one coalesced 64-lane access; the 3 traction
+// components of the block's column are gathered (the traction vector stays in L2).
 // V: the block scalar (double, or float in the split form); sums are FP64 either way.
 template <typename V>
 __global__ void __launch_bounds__(256) k_wall_spmv(int nvert, double c1, const int *__restrict__ rowptr,
@@ -391,10 +392,13 @@ __global__ void __launch_bounds__(256) k_wall_spmv(int nvert, double c1, const i
                                                    const double *__restrict__ tA, double *v) {
   const int i = (blockIdx.x * blockDim.x + threadIdx.x) >> 6, lane = threadIdx.x & 63;
   if (i >= nvert) return;
+  const size_t nbt = rowptr[nvert];
   double s0 = 0, s1 = 0, s2 = 0;
   for (int k = rowptr[i] + lane; k < rowptr[i + 1]; k += 64) {
     const int j = bcol[k];
-    const V *B = bval + 9 * (size_t)k;
+    V B[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) B[q] = bval[q * nbt + k];
     const double f0 = f[j], f1 = f[j + nvert], f2 = f[j + 2 * (size_t)nvert];
     s0 += B[0] * f0 + B[1] * f1 + B[2] * f2;
     s1 += B[3] * f0 + B[4] * f1 + B[5] * f2;
@@ -564,6 +568,7 @@ extern "C" int rbc3d_set_wall(rbc3d_ctx *c, int nvert, int nele, const double *x
   w.prepared = false;
   w.valf_ready = false;
   w.lmodes_ready = false;
+  w.wsp_ready = false;
   HIP_TRY(hipStreamSynchronize(st));
   return RBC3D_OK;
 }
@@ -627,7 +632,7 @@ int rbc_wall_prepare(rbc3d_ctx *c) {
                                                       0, bits, st));
   RBC_TRY(rowcnt.ensure(nv + 1));
   k_wall_reduce<<<(nv + 127) / 128, 128, 0, st>>>(nv, off.p, skey.p, sidx.p, eval.p, rowcnt.p,
-                                                  nullptr, nullptr, 0);
+                                                  nullptr, nullptr, 0, 0);
   HIP_TRY(hipMemsetAsync(rowcnt.p + nv, 0, sizeof(int), st));
   RBC_TRY(w.d_rowptr.ensure(nv + 1));
   HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, rowcnt.p, w.d_rowptr.p, nv + 1, st));
@@ -643,7 +648,7 @@ int rbc_wall_prepare(rbc3d_ctx *c) {
   RBC_TRY(w.d_col.ensure(nb + 1));
   RBC_TRY(w.d_val.ensure(9 * (size_t)nb + 9));
   k_wall_reduce<<<(nv + 127) / 128, 128, 0, st>>>(nv, off.p, skey.p, sidx.p, eval.p, w.d_rowptr.p,
-                                                  w.d_col.p, w.d_val.p, 1);
+                                                  w.d_col.p, w.d_val.p, (size_t)nb, 1);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st));
   w.nnz = 9LL * nb;

@@ -53,16 +53,32 @@ def features(g, normals=None):
     return np.ascontiguousarray(np.hstack([u, f0, nrm]), dtype=np.float32), f0
 
 
-def series(g, cells, nsteps, dt, rtol, maxit, log=None):
+def rhs_norm(g):
+    """|u_res| over the wall vertices: the norm of NoSlipWall's right-hand side (up to
+    the duplicated seam vertices)."""
+    return float(np.linalg.norm(g.wall_residual_vel()))
+
+
+def step_rtol(g, rtol, atol):
+    """The relative tolerance that stops GMRES at max(rtol |b|, atol)."""
+    if not atol:
+        return rtol
+    return max(rtol, atol / max(rhs_norm(g), 1e-300))
+
+
+def series(g, cells, nsteps, dt, rtol, maxit, log=None, atol=None):
     """Run a tube time series with the plain solver; yields (X, T, info) per step with
-    X the inputs (nvert, 9) and T = f - f0 (nvert, 3), both float32."""
+    X the inputs (nvert, 9) and T = f - f0 (nvert, 3), both float32.  GMRES stops at
+    max(rtol |b|, atol): in a series |b| is only the change since the last step, and a
+    tolerance relative to it alone asks for ever smaller absolute residuals."""
     nv = g.nvert
     normals = g.wall_vertex_normals()
     for step in range(nsteps):
         v, _, _ = g.solve_rbc_vel()
         X, f0 = features(g, normals)
+        rt = step_rtol(g, rtol, atol)
         t0 = time.perf_counter()
-        it, res, conv, _ = g.noslip_wall(rtol, maxit)
+        it, res, conv, _ = g.noslip_wall(rt, maxit)
         dtw = time.perf_counter() - t0
         f = g.wall_density().reshape(3, nv).T
         T = np.ascontiguousarray(f - f0, dtype=np.float32)
@@ -114,20 +130,22 @@ def train(mlp, Xd, Td, epochs=30, batch=65536, lr=2e-3, seed=0, log=None):
     return hist
 
 
-def evaluate(g, cells, mlp, nsteps, dt, rtol=1e-8, maxit=4000, log=None, split=False):
+def evaluate(g, cells, mlp, nsteps, dt, rtol=1e-8, maxit=4000, log=None, split=False,
+             atol=None):
     """From the same state at every step: plain NoSlipWall from zero (FP64) and the
     warm-started one (in the split form when `split`, the cells' velocity after it
-    too), both to rtol; the series continues from the plain solution.  Returns
-    per-step records (iterations, seconds, cell-velocity difference)."""
+    too), both to max(rtol |b|, atol); the series continues from the plain solution.
+    Returns per-step records (iterations, seconds, cell-velocity difference)."""
     nv = g.nvert
     out = []
     for step in range(nsteps):
         f0 = g.wall_density()
         v0, _, _ = g.solve_rbc_vel()
+        rtol_s = step_rtol(g, rtol, atol)
         g.set_wall_density(f0)
         g.sync()
         t0 = time.perf_counter()
-        ip, rp, cp, _ = g.noslip_wall(rtol, maxit)
+        ip, rp, cp, _ = g.noslip_wall(rtol_s, maxit)
         tp = time.perf_counter() - t0
         fp = g.wall_density()
         vp, _, _ = g.solve_rbc_vel()
@@ -136,11 +154,11 @@ def evaluate(g, cells, mlp, nsteps, dt, rtol=1e-8, maxit=4000, log=None, split=F
             g.set_precision(g.PREC_SPLIT)
         g.sync()
         t0 = time.perf_counter()
-        il, rl, cl, _, fb = g.noslip_wall_learned(mlp, rtol, maxit)
+        il, rl, cl, _, fb = g.noslip_wall_learned(mlp, rtol_s, maxit)
         tl = time.perf_counter() - t0
         vl, _, _ = g.solve_rbc_vel()
         g.set_precision(g.PREC_FP64)
-        rec = {"step": step, "plain_iters": ip, "plain_s": tp, "plain_conv": cp,
+        rec = {"step": step, "rtol": rtol_s, "plain_iters": ip, "plain_s": tp, "plain_conv": cp,
                "learned_iters": il, "learned_s": tl, "learned_conv": cl, "fallback": fb,
                "vel_diff": float(np.abs(vl - vp).max() / np.abs(vp).max())}
         out.append(rec)
@@ -202,6 +220,11 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=None, help="steps per seed (train 100, eval 8)")
     ap.add_argument("--dt", type=float, default=0.05)
     ap.add_argument("--rtol", type=float, default=None, help="train: 1e-6 targets; eval: 1e-8")
+    ap.add_argument("--atol-rel", type=float, default=None,
+                    help="absolute tolerance as a multiple of the first step's |b| "
+                         "(train: 1e-6, eval: 1e-8)")
+    ap.add_argument("--time-budget", type=float, default=600.0,
+                    help="train: stop generating data after this many seconds")
     ap.add_argument("--maxit", type=int, default=4000)
     ap.add_argument("--epochs", type=int, default=30)
     ap.add_argument("--batch", type=int, default=65536)
@@ -218,15 +241,21 @@ def main(argv=None):
         dev = torch.device("cuda:0")
         Xs, Ts = [], []
         t0 = time.perf_counter()
+        used = []
         for s in seeds:
+            if time.perf_counter() - t0 > a.time_budget:
+                say({"time_budget_reached": True, "seeds_used": used})
+                break
             w = configs.make_workload(a.config, seed=s)
             g = rbc3d_amd.from_workload(w)
             g.prepare_sing_int_on_wall()
-            for X, T, info in series(g, w.cells, steps, a.dt, rtol, a.maxit):
+            atol = (a.atol_rel or 1e-6) * rhs_norm(g)
+            for X, T, info in series(g, w.cells, steps, a.dt, rtol, a.maxit, atol=atol):
                 Xs.append(torch.from_numpy(X).to(dev))
                 Ts.append(torch.from_numpy(T).to(dev))
                 if info["step"] % 10 == 0 or info["step"] == steps - 1:
                     say({"seed": s, **info, "elapsed_s": round(time.perf_counter() - t0, 1)})
+            used.append(s)
             g.close()
         Xd, Td = torch.cat(Xs), torch.cat(Ts)
         del Xs, Ts
@@ -237,7 +266,7 @@ def main(argv=None):
         m.set_norm(mean, std, scale)
         hist = train(m, Xd, Td, a.epochs, a.batch, a.lr, log=say)
         save_model(a.model, m, mean, std, scale,
-                   {"config": a.config, "seeds": a.seeds or "101-120", "steps": steps,
+                   {"config": a.config, "seeds": used, "steps": steps,
                     "dt": a.dt, "rtol": rtol, "epochs": a.epochs, "samples": int(Xd.shape[0]),
                     "loss": hist[-1]})
         say({"saved": a.model, "samples": int(Xd.shape[0]), "loss_first": hist[0],
@@ -252,8 +281,9 @@ def main(argv=None):
         w = configs.make_workload(a.config, seed=s)
         g = rbc3d_amd.from_workload(w)
         g.prepare_sing_int_on_wall()
+        atol = (a.atol_rel or 1e-8) * rhs_norm(g)
         recs = evaluate(g, w.cells, m, steps, a.dt, rtol, a.maxit,
-                        log=lambda r, s=s: say({"seed": s, **r}), split=a.split)
+                        log=lambda r, s=s: say({"seed": s, **r}), split=a.split, atol=atol)
         allrec += recs
         g.close()
     ip = np.array([r["plain_iters"] for r in allrec], float)
