@@ -199,6 +199,11 @@ typedef struct rbc3d_mlp rbc3d_mlp;
 int rbc3d_mlp_create(rbc3d_mlp **out);
 int rbc3d_mlp_destroy(rbc3d_mlp *m);
 int rbc3d_mlp_nparams(void);
+/* The HIP stream the device-pointer entries (_dev) run on from now on (NULL names
+ * the null stream); until this is called they run on the network's own stream.  A
+ * caller whose device arrays are written on a stream of its own passes that stream,
+ * so the network's kernels are ordered after that work. */
+int rbc3d_mlp_set_stream(rbc3d_mlp *m, void *hip_stream);
 int rbc3d_mlp_set_params(rbc3d_mlp *m, const float *params);
 int rbc3d_mlp_get_params(rbc3d_mlp *m, float *params);
 int rbc3d_mlp_set_norm(rbc3d_mlp *m, const float *mean_in, const float *std_in,

@@ -55,6 +55,7 @@ _SIG = {
     "rbc3d_gmres_history": [C.c_void_p, dp, C.c_int, C.POINTER(C.c_int)],
     "rbc3d_mlp_create": [C.POINTER(C.c_void_p)],
     "rbc3d_mlp_destroy": [C.c_void_p],
+    "rbc3d_mlp_set_stream": [C.c_void_p, C.c_void_p],
     "rbc3d_mlp_set_params": [C.c_void_p, fp],
     "rbc3d_mlp_get_params": [C.c_void_p, fp],
     "rbc3d_mlp_set_norm": [C.c_void_p, fp, fp, fp],
@@ -478,6 +479,15 @@ class Mlp:
             self.close()
         except Exception:
             pass
+
+    def set_stream(self, stream):
+        """Run the device-pointer entries on `stream` (a HIP stream handle, e.g.
+        torch.cuda.current_stream().cuda_stream; 0 is the null stream)."""
+        _call("rbc3d_mlp_set_stream", self.h, C.c_void_p(stream))
+
+    def use_torch_stream(self):
+        import torch
+        self.set_stream(torch.cuda.current_stream().cuda_stream)
 
     def set_params(self, p):
         p = np.ascontiguousarray(p, dtype=np.float32)

@@ -49,7 +49,10 @@ enum { GA_1 = 0, GA_2 = GA_1 + 4 * 256, GA_3 = GA_2 + 16 * 256, GA_4 = GA_3 + 16
 #define MLP_TS 72  // LDS row stride (bf16) of the transpose tiles: 64 + 8
 
 struct rbc3d_mlp {
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;  // own stream, or `ext` once rbc3d_mlp_set_stream was
+  hipStream_t ext = nullptr;     // called (which may name the null stream)
+  bool use_ext = false;
+  hipStream_t st() const { return use_ext ? ext : stream; }
   DBuf<float> p, m1, m2, grad, part, norm;
   DBuf<unsigned short> wb;
   DBuf<double> lpart;
@@ -441,11 +444,19 @@ extern "C" int rbc3d_mlp_destroy(rbc3d_mlp *m) {
 
 extern "C" int rbc3d_mlp_nparams(void) { return MP_N; }
 
+extern "C" int rbc3d_mlp_set_stream(rbc3d_mlp *m, void *stream) {
+  if (!m) return RBC3D_EINVAL;
+  HIP_TRY(hipStreamSynchronize(m->st()));
+  m->ext = (hipStream_t)stream;
+  m->use_ext = true;
+  return RBC3D_OK;
+}
+
 extern "C" int rbc3d_mlp_set_params(rbc3d_mlp *m, const float *params) {
   if (!m || !params) return RBC3D_EINVAL;
   for (int i = 0; i < MP_N; i++)
     if (!std::isfinite(params[i])) return RBC3D_ENONFINITE;
-  HIP_TRY(hipStreamSynchronize(m->stream));
+  HIP_TRY(hipStreamSynchronize(m->st()));
   HIP_TRY(hipMemcpy(m->p.p, params, MP_N * 4, hipMemcpyHostToDevice));
   HIP_TRY(hipMemset(m->m1.p, 0, MP_N * 4));
   HIP_TRY(hipMemset(m->m2.p, 0, MP_N * 4));
@@ -458,7 +469,7 @@ extern "C" int rbc3d_mlp_set_params(rbc3d_mlp *m, const float *params) {
 
 extern "C" int rbc3d_mlp_get_params(rbc3d_mlp *m, float *params) {
   if (!m || !params) return RBC3D_EINVAL;
-  HIP_TRY(hipStreamSynchronize(m->stream));
+  HIP_TRY(hipStreamSynchronize(m->st()));
   HIP_TRY(hipMemcpy(params, m->p.p, MP_N * 4, hipMemcpyDeviceToHost));
   return RBC3D_OK;
 }
@@ -478,7 +489,7 @@ extern "C" int rbc3d_mlp_set_norm(rbc3d_mlp *m, const float *mean_in, const floa
     if (!(scale_out[i] > 0.f)) return RBC3D_EINVAL;
     nm[2 * MLP_NIN + i] = scale_out[i];
   }
-  HIP_TRY(hipStreamSynchronize(m->stream));
+  HIP_TRY(hipStreamSynchronize(m->st()));
   HIP_TRY(hipMemcpy(m->norm.p, nm.data(), nm.size() * 4, hipMemcpyHostToDevice));
   return RBC3D_OK;
 }
@@ -494,24 +505,25 @@ int rbc_mlp_forward(rbc3d_mlp *m, int n, const float *d_x, float *d_y, hipStream
 
 extern "C" int rbc3d_mlp_forward_dev(rbc3d_mlp *m, int n, const float *d_x, float *d_y) {
   if (!m || n < 0 || (n > 0 && (!d_x || !d_y))) return RBC3D_EINVAL;
-  RBC_TRY(rbc_mlp_forward(m, n, d_x, d_y, m->stream));
-  HIP_TRY(hipStreamSynchronize(m->stream));
+  RBC_TRY(rbc_mlp_forward(m, n, d_x, d_y, m->st()));
+  HIP_TRY(hipStreamSynchronize(m->st()));
   return RBC3D_OK;
 }
 
 static int mlp_grad(rbc3d_mlp *m, int n, const float *d_x, const float *d_t, double *loss) {
+  hipStream_t st = m->st();
   const int ntile = (n + 15) / 16;
   const int nwg = std::min(512, ntile);
   RBC_TRY(m->part.ensure((size_t)nwg * GA_N));
   RBC_TRY(m->lpart.ensure(nwg));
-  k_mlp_train<<<nwg, 64, 0, m->stream>>>(n, d_x, d_t, m->wb.p, m->p.p, m->norm.p, m->part.p,
+  k_mlp_train<<<nwg, 64, 0, m->st()>>>(n, d_x, d_t, m->wb.p, m->p.p, m->norm.p, m->part.p,
                                          m->lpart.p);
-  k_mlp_reduce<<<(MP_N + 255) / 256, 256, 0, m->stream>>>(nwg, m->part.p, m->grad.p);
+  k_mlp_reduce<<<(MP_N + 255) / 256, 256, 0, m->st()>>>(nwg, m->part.p, m->grad.p);
   HIP_TRY(hipGetLastError());
   if (loss) {
     std::vector<double> lp(nwg);
-    HIP_TRY(hipMemcpyAsync(lp.data(), m->lpart.p, nwg * 8, hipMemcpyDeviceToHost, m->stream));
-    HIP_TRY(hipStreamSynchronize(m->stream));
+    HIP_TRY(hipMemcpyAsync(lp.data(), m->lpart.p, nwg * 8, hipMemcpyDeviceToHost, m->st()));
+    HIP_TRY(hipStreamSynchronize(m->st()));
     double s = 0;
     #pragma unroll
     for (int i = 0; i < nwg; i++) s += lp[i];
@@ -525,8 +537,8 @@ extern "C" int rbc3d_mlp_grad_dev(rbc3d_mlp *m, int n, const float *d_x, const f
   if (!m || n <= 0 || !d_x || !d_t) return RBC3D_EINVAL;
   RBC_TRY(mlp_grad(m, n, d_x, d_t, loss));
   if (d_grad)
-    HIP_TRY(hipMemcpyAsync(d_grad, m->grad.p, MP_N * 4, hipMemcpyDeviceToDevice, m->stream));
-  HIP_TRY(hipStreamSynchronize(m->stream));
+    HIP_TRY(hipMemcpyAsync(d_grad, m->grad.p, MP_N * 4, hipMemcpyDeviceToDevice, m->st()));
+  HIP_TRY(hipStreamSynchronize(m->st()));
   return RBC3D_OK;
 }
 
@@ -538,9 +550,9 @@ extern "C" int rbc3d_mlp_adam_step_dev(rbc3d_mlp *m, int n, const float *d_x, co
   const float b1 = 0.9f, b2 = 0.999f;
   const float c1 = (float)(1.0 / (1.0 - std::pow((double)b1, m->t)));
   const float c2 = (float)(1.0 / (1.0 - std::pow((double)b2, m->t)));
-  k_mlp_adam<<<(MP_N + 255) / 256, 256, 0, m->stream>>>(m->p.p, m->m1.p, m->m2.p, m->grad.p, lr, b1,
+  k_mlp_adam<<<(MP_N + 255) / 256, 256, 0, m->st()>>>(m->p.p, m->m1.p, m->m2.p, m->grad.p, lr, b1,
                                                         b2, 1e-8f, c1, c2, m->wb.p);
   HIP_TRY(hipGetLastError());
-  if (!loss) HIP_TRY(hipStreamSynchronize(m->stream));
+  if (!loss) HIP_TRY(hipStreamSynchronize(m->st()));
   return RBC3D_OK;
 }

@@ -491,46 +491,74 @@ __global__ void k_wsp_src(int nele, const int *__restrict__ e2v, int nvert,
     fsrc[e + (size_t)nele * d] = f;
   }
 }
+// One wavefront per 64 consecutive grid points (rows): their entries are one
+// contiguous range, which the lanes take 64 at a time (coalesced index and weight
+// loads, 64 independent source gathers); a segmented inclusive scan over the lanes
+// (keyed by row; entries are sorted by row) sums each row's entries of the step, and
+// the last lane of each row's segment adds that partial to the row's LDS accumulator.
+// The additions happen in a fixed pattern: deterministic.
 template <typename T, typename W>
-__global__ void k_wsp_spread(int ng, int nele, double c1, const int *__restrict__ ptr,
-                             const int *__restrict__ src, const W *__restrict__ wgt,
-                             const double *__restrict__ fsrc, T *grid) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g >= ng) return;
-  double s0 = 0, s1 = 0, s2 = 0;
-  const int k0 = ptr[g], k1 = ptr[g + 1];
-  // four entries per step: their index and weight loads, then their source gathers,
-  // are issued together (the row is a chain of dependent loads otherwise)
-  int k = k0;
-  for (; k + 4 <= k1; k += 4) {
-    int e[4];
-    double cw[4], f[4][3];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      e[u] = src[k + u];
-      cw[u] = c1 * (double)wgt[k + u];
+__global__ void __launch_bounds__(256) k_wsp_spread(int ng, int nele, double c1,
+                                                    const int *__restrict__ ptr,
+                                                    const int *__restrict__ src,
+                                                    const W *__restrict__ wgt,
+                                                    const double *__restrict__ fsrc, T *grid) {
+  __shared__ int sptr[4][65];
+  __shared__ double sacc[4][3][64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int g0 = (blockIdx.x * 4 + wv) * 64;
+  const int nr = g0 < ng ? min(64, ng - g0) : 0;
+  if (nr > 0) {
+    sptr[wv][lane] = ptr[g0 + min(lane, nr)];
+    if (lane == 0) sptr[wv][64] = ptr[g0 + nr];
+  }
+  for (int d = 0; d < 3; d++) sacc[wv][d][lane] = 0.;
+  __syncthreads();
+  const int k0 = nr > 0 ? sptr[wv][0] : 0, k1 = nr > 0 ? sptr[wv][nr] : 0;
+  for (int kb = k0; kb < k1; kb += 64) {
+    const int k = kb + lane;
+    const bool live = k < k1;
+    int r = 0;
+    double v0 = 0., v1 = 0., v2 = 0.;
+    if (live) {
+      int lo = 0, hi = nr;  // row r: sptr[r] <= k < sptr[r + 1]
+      while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (sptr[wv][mid] <= k)
+          lo = mid;
+        else
+          hi = mid;
+      }
+      r = lo;
+      const int e = src[k];
+      const double cw = c1 * (double)wgt[k];
+      v0 = cw * fsrc[e];
+      v1 = cw * fsrc[e + (size_t)nele];
+      v2 = cw * fsrc[e + 2 * (size_t)nele];
     }
-#pragma unroll
-    for (int u = 0; u < 4; u++)
-#pragma unroll
-      for (int d = 0; d < 3; d++) f[u][d] = fsrc[e[u] + (size_t)nele * d];
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      s0 += cw[u] * f[u][0];
-      s1 += cw[u] * f[u][1];
-      s2 += cw[u] * f[u][2];
+    const int key = live ? r : 64 + lane;
+    for (int o = 1; o < 64; o <<= 1) {
+      const double u0 = __shfl_up(v0, o, 64), u1 = __shfl_up(v1, o, 64), u2 = __shfl_up(v2, o, 64);
+      const int kk = __shfl_up(key, o, 64);
+      if (lane >= o && kk == key) {
+        v0 += u0;
+        v1 += u1;
+        v2 += u2;
+      }
+    }
+    const int nk = __shfl_down(key, 1, 64);
+    if (live && (lane == 63 || nk != key)) {
+      sacc[wv][0][r] += v0;
+      sacc[wv][1][r] += v1;
+      sacc[wv][2][r] += v2;
     }
   }
-  for (; k < k1; k++) {
-    const int e = src[k];
-    const double cw = c1 * (double)wgt[k];
-    s0 += cw * fsrc[e];
-    s1 += cw * fsrc[e + (size_t)nele];
-    s2 += cw * fsrc[e + 2 * (size_t)nele];
+  __syncthreads();
+  if (lane < nr) {
+    grid[g0 + lane] = (T)sacc[wv][0][lane];
+    grid[g0 + lane + (size_t)ng] = (T)sacc[wv][1][lane];
+    grid[g0 + lane + 2 * (size_t)ng] = (T)sacc[wv][2][lane];
   }
-  grid[g] = (T)s0;
-  grid[g + (size_t)ng] = (T)s1;
-  grid[g + 2 * (size_t)ng] = (T)s2;
 }
 
 static int wall_spread_setup(rbc3d_ctx *c) {
@@ -587,14 +615,13 @@ static int wall_spread(rbc3d_ctx *c, double c1) {
                                                   w.d_wsp_f.p);
   rbc_stage_end(c, ST_SPREAD_PREP);
   rbc_stage_begin(c, ST_SPREAD);
+  const unsigned nblk = (unsigned)((ng + 255) / 256);  // 4 wavefronts of 64 rows each
   if (sizeof(T) == 4)
-    k_wsp_spread<T, float><<<(ng + 255) / 256, 256, 0, st>>>(ng, w.nele, c1, w.d_wsp_ptr.p,
-                                                             w.d_wsp_src.p, w.d_wsp_wf.p,
-                                                             w.d_wsp_f.p, (T *)c->d_grid.p);
+    k_wsp_spread<T, float><<<nblk, 256, 0, st>>>(ng, w.nele, c1, w.d_wsp_ptr.p, w.d_wsp_src.p,
+                                                 w.d_wsp_wf.p, w.d_wsp_f.p, (T *)c->d_grid.p);
   else
-    k_wsp_spread<T, double><<<(ng + 255) / 256, 256, 0, st>>>(ng, w.nele, c1, w.d_wsp_ptr.p,
-                                                              w.d_wsp_src.p, w.d_wsp_w.p,
-                                                              w.d_wsp_f.p, (T *)c->d_grid.p);
+    k_wsp_spread<T, double><<<nblk, 256, 0, st>>>(ng, w.nele, c1, w.d_wsp_ptr.p, w.d_wsp_src.p,
+                                                  w.d_wsp_w.p, w.d_wsp_f.p, (T *)c->d_grid.p);
   rbc_stage_end(c, ST_SPREAD);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;

@@ -33,7 +33,12 @@ module rbc3d_bind
        rbc3d_sh_anal_gau, rbc3d_sh_synth_gau, rbc3d_sh_synth_equ, rbc3d_glob_sph_trans, &
        rbc3d_calc_velocity_field, rbc3d_cell_dof, rbc3d_cell_matmult, rbc3d_cell_rhs, &
        rbc3d_wall_dof, rbc3d_wall_matmult, rbc3d_wall_residual_vel, rbc3d_get_wall_density, &
-       rbc3d_noslip_wall, rbc3d_solve_rbc_vel, rbc3d_pair_stats
+       rbc3d_noslip_wall, rbc3d_solve_rbc_vel, rbc3d_pair_stats, rbc3d_set_precision, &
+       rbc3d_mlp_create, rbc3d_mlp_destroy, rbc3d_mlp_set_params, rbc3d_mlp_set_norm, &
+       rbc3d_noslip_wall_learned
+
+  ! rbc3d_set_precision modes
+  integer(c_int), parameter, public :: RBC3D_PREC_FP64 = 0, RBC3D_PREC_SPLIT = 1
 
   interface
     integer(c_int) function rbc3d_create(conf, ctx) bind(C, name="rbc3d_create")
@@ -229,6 +234,41 @@ module rbc3d_bind
       import :: c_ptr, c_size_t
       type(c_ptr), value :: s
       integer(c_size_t) :: n
+    end function
+    ! precision of the hot path (no counterpart in the reference)
+    integer(c_int) function rbc3d_set_precision(ctx, mode) bind(C, name="rbc3d_set_precision")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: mode
+    end function
+    ! the learned warm start of NoSlipWall (no counterpart in the reference)
+    integer(c_int) function rbc3d_mlp_create(m) bind(C, name="rbc3d_mlp_create")
+      import :: c_int, c_ptr
+      type(c_ptr), intent(out) :: m
+    end function
+    integer(c_int) function rbc3d_mlp_destroy(m) bind(C, name="rbc3d_mlp_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+    end function
+    integer(c_int) function rbc3d_mlp_set_params(m, params) bind(C, name="rbc3d_mlp_set_params")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: m
+      real(c_float), intent(in) :: params(*)
+    end function
+    integer(c_int) function rbc3d_mlp_set_norm(m, mean_in, std_in, scale_out) &
+         bind(C, name="rbc3d_mlp_set_norm")
+      import :: c_int, c_ptr, c_float
+      type(c_ptr), value :: m
+      real(c_float), intent(in) :: mean_in(9), std_in(9), scale_out(3)
+    end function
+    integer(c_int) function rbc3d_noslip_wall_learned(ctx, m, rtol, maxit, niter, rnorm, &
+         converged, vres_max, fallback) bind(C, name="rbc3d_noslip_wall_learned")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx, m
+      real(c_double), value :: rtol
+      integer(c_int), value :: maxit
+      integer(c_int), intent(out) :: niter, converged, fallback
+      real(c_double), intent(out) :: rnorm, vres_max
     end function
   end interface
 

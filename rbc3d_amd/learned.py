@@ -106,6 +106,7 @@ def train(mlp, Xd, Td, epochs=30, batch=65536, lr=2e-3, seed=0, log=None):
     cosine learning-rate decay to lr/20.  Returns the loss after each epoch (on a fixed
     sample of up to 2^18 points)."""
     import torch
+    mlp.use_torch_stream()  # the batches are gathered on torch's stream
     N = Xd.shape[0]
     gen = torch.Generator(device=Xd.device).manual_seed(seed)
     nb = max(1, N // batch)

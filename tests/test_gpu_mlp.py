@@ -29,6 +29,7 @@ def _data(seed, n):
 
 def _net(R, seed=0):
     m = R.Mlp()
+    m.use_torch_stream()  # inputs are written on torch's stream
     p = M.init_params(np.random.default_rng(seed))
     p[M.NPARAMS - 3:] = [0.01, -0.02, 0.03]
     m.set_params(p)
