@@ -172,10 +172,39 @@ def timestep_bench(steps, warmup, maxit=4000):
         out[form] = r
     if mlp is None:
         out["chosen"] = {"skipped": f"no trained network at {MLP_C3}"}
+    else:
+        out["learned_series"] = learned_series_bench(mlp)
     # the headline fields (plain FP64, the reference's arithmetic)
     out["s_per_timestep"] = plain["s_per_timestep"]
     out["gmres_iterations"] = plain["gmres_iterations"]
     return out
+
+
+def learned_series_bench(mlp, seed=201, steps=4):
+    """The warm start where it is meant to work: consecutive steps of a tube series
+    (held-out placement, cells moving rigidly with their mean velocity, dt = 0.05).
+    From the same state at every step: plain FP64 NoSlipWall from zero against the
+    split form with the network's warm start, both to the same absolute residual
+    1e-8 |b_0| (|b_0|: the first step's right-hand side)."""
+    import rbc3d_amd
+    from rbc3d_amd import configs, learned
+    w = configs.make_workload("C3", seed=seed)
+    g = rbc3d_amd.from_workload(w)
+    g.prepare_sing_int_on_wall()
+    atol = 1e-8 * learned.rhs_norm(g)
+    recs = learned.evaluate(g, w.cells, mlp, steps, 0.05, 1e-8, 4000, split=True, atol=atol)
+    g.close()
+    ip = [r["plain_iters"] for r in recs]
+    il = [r["learned_iters"] for r in recs]
+    tp = sum(r["plain_s"] for r in recs)
+    tl = sum(r["learned_s"] for r in recs)
+    return {"workload": f"C3 series, placement seed {seed}, {steps} steps (step 0 starts from "
+                        "zero traction)",
+            "plain_iterations": ip, "chosen_iterations": il,
+            "plain_s_per_step": round(tp / steps, 5), "chosen_s_per_step": round(tl / steps, 5),
+            "speedup_vs_plain": round(tp / tl, 3),
+            "fallbacks": sum(int(r["fallback"]) for r in recs),
+            "velocity_rel_err_vs_plain_max": max(r["vel_diff"] for r in recs)}
 
 
 def c5_bench(steps, warmup):

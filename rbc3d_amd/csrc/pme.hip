@@ -960,7 +960,8 @@ static FftPlan<T> make_plan(rbc3d_ctx *c, int ax, int sign) {
 }
 
 // ~1024 complex elements per workgroup (2048 in FP32): 32 KiB of LDS for the two
-// buffers, so five workgroups fit a CU
+// buffers, so five workgroups fit a CU.  (A 48 KiB budget -- longer contiguous runs
+// in the strided passes, fewer workgroups -- measured slower on the tube's grid.)
 template <typename T>
 static int lines_per_block(int N) { return std::max(1, (int)(8192 / sizeof(T)) / N); }
 
