@@ -168,6 +168,13 @@ int rbc3d_set_precision(rbc3d_ctx *ctx, int mode);
 
 /* --- the two GMRES solves (PETSc KSPGMRES semantics: restart 30, classical
  *     Gram-Schmidt, no preconditioner, x += V y at restarts) ------------------ */
+/* Restart length of both solves (and of the learned warm start's), 1..255.  The
+ * reference's solves keep PETSc's default, 30 (KSPSetType(KSPGMRES) with no
+ * KSPGMRESSetRestart, common/ModVelSolver.F90:82, common/ModNoSlip.F90:79); this is the
+ * KSPGMRESSetRestart call a maintainer would add there.  The Krylov basis takes
+ * (restart + 1) x ndof doubles of HBM. */
+#define RBC3D_GMRES_MAX_RESTART 255
+int rbc3d_set_gmres_restart(rbc3d_ctx *ctx, int restart);
 /* NoSlipWall (common/ModNoSlip.F90:45-150): residual wall velocity, GMRES for the
  * traction increment (rtol relative to the initial residual, maxit), f = f0 + finc.
  * finc0: initial guess of the increment on the de-duplicated unknowns (NULL: zero,

@@ -41,6 +41,7 @@ def lib():
         "orc_set_cell_density": (C.c_int, [vp, C.c_int, dp]),
         "orc_set_wall": (C.c_int, [vp, C.c_int, C.c_int, dp, ip]),
         "orc_set_wall_density": (C.c_int, [vp, dp]),
+        "orc_set_gmres_restart": (C.c_int, [vp, C.c_int]),
         "orc_prepare_sing_int_on_wall": (C.c_int, [vp]),
         "orc_ntargets_rbc": (C.c_int, [vp]),
         "orc_ntargets_wall": (C.c_int, [vp]),
@@ -276,6 +277,9 @@ class Oracle:
         d = np.zeros(cap)
         n = L().orc_near_list(self.h, len(Ac), tx, ti, i, cells, il, jl, d, cap)
         return n, cells[:max(n, 0)], il[:max(n, 0)], jl[:max(n, 0)], d[:max(n, 0)]
+
+    def set_gmres_restart(self, m):
+        assert L().orc_set_gmres_restart(self.h, m) == 0
 
     def noslip_wall(self, rtol=1e-8, maxit=60, finc0=None):
         ni = C.c_int()

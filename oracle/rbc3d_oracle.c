@@ -107,6 +107,7 @@ struct orc_sys {
   double complex *ffC, *ttC, *vvC;
   double *bb, *qq;
   unsigned long long last_checksum;
+  int gmres_restart; /* KSPGMRESSetRestart; 30 is PETSc's default */
 };
 
 /* ======================================================================== */
@@ -789,6 +790,7 @@ static void hash_num_blocks(const orc_sys *s, int Nc[3], double iLbNc[3]) {
 orc_sys *orc_create(const orc_conf *conf) {
   orc_sys *s = calloc(1, sizeof(orc_sys));
   s->conf = *conf;
+  s->gmres_restart = 30;
   for (int d = 0; d < 3; d++) {
     s->Lb[d] = conf->Lb[d];
     s->iLb[d] = 1. / conf->Lb[d];
@@ -1736,6 +1738,13 @@ int orc_set_wall(orc_sys *s, int nvert, int nele, const double *x, const int *e2
   return 0;
 }
 
+/* GMRES restart length of both solves (PETSc -ksp_gmres_restart) */
+int orc_set_gmres_restart(orc_sys *s, int m) {
+  if (m < 1 || m > 255) return 1;
+  s->gmres_restart = m;
+  return 0;
+}
+
 int orc_set_wall_density(orc_sys *s, const double *f) {
   if (!s->nwall) return 1;
   memcpy(s->wall.f, f, sizeof(double) * 3 * s->wall.nvert);
@@ -2426,11 +2435,12 @@ int orc_wall_residual_vel(orc_sys *s, double *v) {
 }
 
 /* ======================================================================== */
-/* GMRES(30), PETSc KSPGMRES semantics as used by common/ModVelSolver.F90:74-90
+/* GMRES(m), PETSc KSPGMRES semantics as used by common/ModVelSolver.F90:74-90
  * and common/ModNoSlip.F90:70-87: classical Gram-Schmidt (no refinement), no
  * preconditioner, convergence when the Hessenberg residual estimate <=
  * max(rtol*||r0||, atol) (KSPConvergedDefault: ||r0|| is ||b|| when the initial
- * guess is nonzero), explicit residual at restarts, divergence at 1e5*||r0||. */
+ * guess is nonzero), explicit residual at restarts, divergence at 1e5*||r0||.
+ * m is PETSc's restart length (-ksp_gmres_restart), 30 unless set. */
 typedef int (*orc_matvec_fn)(orc_sys *, const double *, double *);
 
 static double dotp(int n, const double *a, const double *b) {
@@ -2441,9 +2451,12 @@ static double dotp(int n, const double *a, const double *b) {
 
 static int orc_gmres(orc_sys *s, orc_matvec_fn mv, int n, const double *b, double *x, int x0nz,
                      double rtol, int maxit, int *niter, double *rnorm) {
-  const int m = 30;
+  const int m = s->gmres_restart;
   double *V = malloc(sizeof(double) * (size_t)n * (m + 1));
-  double *H = calloc((size_t)(m + 1) * m, sizeof(double)), cs[30], sn[30], g[31], h[31];
+  double *H = calloc((size_t)(m + 1) * m, sizeof(double));
+  double *cs = malloc(sizeof(double) * m), *sn = malloc(sizeof(double) * m);
+  double *g = malloc(sizeof(double) * (m + 1)), *h = malloc(sizeof(double) * (m + 1));
+  double *y = malloc(sizeof(double) * m);
   int its = 0, reason = 0, first = 1;
   double ttol = 0, rnorm0 = 0, res = 0;
   if (!x0nz) memset(x, 0, sizeof(double) * n);
@@ -2519,7 +2532,6 @@ static int orc_gmres(orc_sys *s, orc_matvec_fn mv, int n, const double *b, doubl
         break;
       }
     }
-    double y[30];
     for (int i = it - 1; i >= 0; i--) {
       double t = g[i];
       for (int k = i + 1; k < it; k++) t -= H[(size_t)i * m + k] * y[k];
@@ -2534,6 +2546,11 @@ static int orc_gmres(orc_sys *s, orc_matvec_fn mv, int n, const double *b, doubl
   }
   free(V);
   free(H);
+  free(cs);
+  free(sn);
+  free(g);
+  free(h);
+  free(y);
   *niter = its;
   *rnorm = res;
   return reason > 0;

@@ -49,6 +49,8 @@ int orc_set_cell_density(orc_sys *s, int which, const double *dens);
 /* one wall (ModWall): x(nvert,3), e2v(nele,3) 1-based, f(nvert,3) */
 int orc_set_wall(orc_sys *s, int nvert, int nele, const double *x, const int *e2v);
 int orc_set_wall_density(orc_sys *s, const double *f);
+/* GMRES restart length of the NoSlipWall and Solve_RBC_Vel solves, 1..255 (30) */
+int orc_set_gmres_restart(orc_sys *s, int m);
 int orc_prepare_sing_int_on_wall(orc_sys *s);
 
 /* target-list helpers: the reference's tlist_rbc / tlist_wall */

@@ -52,6 +52,7 @@ _SIG = {
     "rbc3d_near_stats": [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
     "rbc3d_set_geometry_reuse": [C.c_void_p, C.c_int],
     "rbc3d_set_precision": [C.c_void_p, C.c_int],
+    "rbc3d_set_gmres_restart": [C.c_void_p, C.c_int],
     "rbc3d_gmres_history": [C.c_void_p, dp, C.c_int, C.POINTER(C.c_int)],
     "rbc3d_mlp_create": [C.POINTER(C.c_void_p)],
     "rbc3d_mlp_destroy": [C.c_void_p],
@@ -285,6 +286,10 @@ class System:
         """0: plain FP64 (default); 1: split form (FP32 pair sums and PME, split-BF16
         Legendre GEMMs, FP32 wall self-matrix; quadratures and GMRES in FP64)."""
         _call("rbc3d_set_precision", self.h, int(mode))
+
+    def set_gmres_restart(self, restart):
+        """GMRES restart length of both solves (30, PETSc's default, unless set)."""
+        _call("rbc3d_set_gmres_restart", self.h, int(restart))
 
     def near_stats(self):
         """(projections, near-singular corrections) of the last add_int_on_rbcs."""

@@ -770,6 +770,15 @@ extern "C" int rbc3d_set_precision(rbc3d_ctx *c, int mode) {
   return RBC3D_OK;
 }
 
+extern "C" int rbc3d_set_gmres_restart(rbc3d_ctx *c, int restart) {
+  if (!c || restart < 1 || restart > RBC3D_GMRES_MAX_RESTART) {
+    rbc_set_error("rbc3d_set_gmres_restart: restart must be in 1..255");
+    return RBC3D_EINVAL;
+  }
+  c->gmres_restart = restart;
+  return RBC3D_OK;
+}
+
 extern "C" int rbc3d_set_geometry_reuse(rbc3d_ctx *c, int enable) {
   if (!c) return RBC3D_EINVAL;
   c->geom_reuse = enable != 0;

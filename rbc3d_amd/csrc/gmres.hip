@@ -1,6 +1,6 @@
 // Restarted GMRES on the device with PETSc KSPGMRES semantics (the solver both
 // reference solves use: common/ModVelSolver.F90:74-90, common/ModNoSlip.F90:70-87):
-// restart 30, classical Gram-Schmidt without refinement, no preconditioner, the
+// restart 30 unless set, classical Gram-Schmidt without refinement, no preconditioner, the
 // residual estimate from the Givens-rotated Hessenberg system, convergence when the
 // estimate drops to max(rtol * ||r0||, atol) -- with ||r0|| = ||b|| when the initial
 // guess is nonzero, as KSPConvergedDefault does -- explicit residual b - A x at
@@ -15,7 +15,7 @@
 #include "internal.h"
 
 #define G_T 256
-#define G_MAXV 64
+#define G_MAXV (RBC3D_GMRES_MAX_RESTART + 1)
 
 // partial dot products of w with V[0..k) (column stride ld), one row of partials per
 // block: each thread accumulates eight dot products at a time over its elements, the
@@ -317,7 +317,7 @@ extern "C" int rbc3d_noslip_wall(rbc3d_ctx *c, double rtol, int maxit, const dou
   int reason = 0;
   auto mv = [c](const double *u, double *b) { return rbc_wall_matmult_dev(c, u, b); };
   c->gmres.history.clear();
-  RBC_TRY(rbc_gmres(c, nd, mv, rhs.p, x.p, x0nz, rtol, 1e-50, 1e5, maxit, 30, niter, rnorm, &reason,
+  RBC_TRY(rbc_gmres(c, nd, mv, rhs.p, x.p, x0nz, rtol, 1e-50, 1e5, maxit, c->gmres_restart, niter, rnorm, &reason,
                     &c->gmres.history));
   *converged = reason > 0;
   // f = f0 + finc (AssembleArray(finc, finc1D, -1))
@@ -517,14 +517,14 @@ extern "C" int rbc3d_noslip_wall_learned(rbc3d_ctx *c, rbc3d_mlp *m, double rtol
   *fallback = 0;
   c->gmres.history.clear();
   if (r2 < b2) {
-    RBC_TRY(rbc_gmres(c, nd, mv, rhs.p, x.p, true, rtol, 1e-50, 1e5, maxit, 30, &its, rnorm, &reason,
+    RBC_TRY(rbc_gmres(c, nd, mv, rhs.p, x.p, true, rtol, 1e-50, 1e5, maxit, c->gmres_restart, &its, rnorm, &reason,
                       &c->gmres.history));
   }
   *niter = its;
   if (reason <= 0) {  // plain GMRES from zero
     *fallback = 1;
     int its2 = 0;
-    RBC_TRY(rbc_gmres(c, nd, mv, rhs.p, x.p, false, rtol, 1e-50, 1e5, maxit, 30, &its2, rnorm,
+    RBC_TRY(rbc_gmres(c, nd, mv, rhs.p, x.p, false, rtol, 1e-50, 1e5, maxit, c->gmres_restart, &its2, rnorm,
                       &reason, &c->gmres.history));
     *niter += its2;
   }
@@ -588,7 +588,7 @@ extern "C" int rbc3d_solve_rbc_vel(rbc3d_ctx *c, double *v, int *niter, double *
   } else {
     int reason = 0;
     auto mv = [c](const double *u, double *b) { return rbc3d_cell_matmult_dev(c, u, b); };
-    RBC_TRY(rbc_gmres(c, dof, mv, rhs.p, c->d_cell_sol.p, true, 1e-11, 1e-50, 1e5, 200, 30, niter,
+    RBC_TRY(rbc_gmres(c, dof, mv, rhs.p, c->d_cell_sol.p, true, 1e-11, 1e-50, 1e5, 200, c->gmres_restart, niter,
                       rnorm, &reason, nullptr));
   }
   RBC_TRY(c->d_shbuf.ensure((size_t)3 * c->np + 1));

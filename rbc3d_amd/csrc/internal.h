@@ -159,6 +159,7 @@ struct rbc3d_ctx {
   double Acoef[8], Bcoef[8];
   hipStream_t stream = nullptr;
   int prec = RBC3D_PREC_FP64;  // rbc3d_set_precision
+  int gmres_restart = 30;      // rbc3d_set_gmres_restart (PETSc's default length)
   int ncu = 256;               // compute units of the device
   // tables
   DBuf<double> d_sl1, d_sl2, d_dl1, d_mtab;

@@ -35,7 +35,7 @@ module rbc3d_bind
        rbc3d_wall_dof, rbc3d_wall_matmult, rbc3d_wall_residual_vel, rbc3d_get_wall_density, &
        rbc3d_noslip_wall, rbc3d_solve_rbc_vel, rbc3d_pair_stats, rbc3d_set_precision, &
        rbc3d_mlp_create, rbc3d_mlp_destroy, rbc3d_mlp_set_params, rbc3d_mlp_set_norm, &
-       rbc3d_noslip_wall_learned
+       rbc3d_noslip_wall_learned, rbc3d_set_gmres_restart
 
   ! rbc3d_set_precision modes
   integer(c_int), parameter, public :: RBC3D_PREC_FP64 = 0, RBC3D_PREC_SPLIT = 1
@@ -240,6 +240,13 @@ module rbc3d_bind
       import :: c_int, c_ptr
       type(c_ptr), value :: ctx
       integer(c_int), value :: mode
+    end function
+    ! GMRES restart length of both solves (the reference keeps PETSc's 30)
+    integer(c_int) function rbc3d_set_gmres_restart(ctx, restart) &
+        bind(C, name="rbc3d_set_gmres_restart")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: restart
     end function
     ! the learned warm start of NoSlipWall (no counterpart in the reference)
     integer(c_int) function rbc3d_mlp_create(m) bind(C, name="rbc3d_mlp_create")

@@ -132,12 +132,13 @@ def train(mlp, Xd, Td, epochs=30, batch=65536, lr=2e-3, seed=0, log=None):
 
 
 def evaluate(g, cells, mlp, nsteps, dt, rtol=1e-8, maxit=4000, log=None, split=False,
-             atol=None):
-    """From the same state at every step: plain NoSlipWall from zero (FP64) and the
-    warm-started one (in the split form when `split`, the cells' velocity after it
-    too), both to max(rtol |b|, atol); the series continues from the plain solution.
-    Returns per-step records (iterations, seconds, cell-velocity difference)."""
-    nv = g.nvert
+             atol=None, restart=30, cold=False):
+    """From the same state at every step: plain NoSlipWall from zero (FP64, GMRES(30))
+    and the warm-started one (in the split form when `split`, the cells' velocity after
+    it too; GMRES(`restart`)), both to max(rtol |b|, atol); with `cold`, also the
+    warm-started arm's settings from zero, which isolates what the network adds.  The
+    series continues from the plain solution.  Returns per-step records (iterations,
+    seconds, cell-velocity difference)."""
     out = []
     for step in range(nsteps):
         f0 = g.wall_density()
@@ -150,18 +151,31 @@ def evaluate(g, cells, mlp, nsteps, dt, rtol=1e-8, maxit=4000, log=None, split=F
         tp = time.perf_counter() - t0
         fp = g.wall_density()
         vp, _, _ = g.solve_rbc_vel()
-        g.set_wall_density(f0)
+        rel = lambda v: float(np.abs(v - vp).max() / np.abs(vp).max())
+        rec = {"step": step, "rtol": rtol_s, "plain_iters": ip, "plain_s": tp, "plain_conv": cp}
+        g.set_gmres_restart(restart)
         if split:
             g.set_precision(g.PREC_SPLIT)
-        g.sync()
-        t0 = time.perf_counter()
-        il, rl, cl, _, fb = g.noslip_wall_learned(mlp, rtol_s, maxit)
-        tl = time.perf_counter() - t0
-        vl, _, _ = g.solve_rbc_vel()
-        g.set_precision(g.PREC_FP64)
-        rec = {"step": step, "rtol": rtol_s, "plain_iters": ip, "plain_s": tp, "plain_conv": cp,
-               "learned_iters": il, "learned_s": tl, "learned_conv": cl, "fallback": fb,
-               "vel_diff": float(np.abs(vl - vp).max() / np.abs(vp).max())}
+        try:
+            if cold:
+                g.set_wall_density(f0)
+                g.sync()
+                t0 = time.perf_counter()
+                ic, _, cc, _ = g.noslip_wall(rtol_s, maxit)
+                tc = time.perf_counter() - t0
+                vc, _, _ = g.solve_rbc_vel()
+                rec.update(cold_iters=ic, cold_s=tc, cold_conv=cc, cold_vel_diff=rel(vc))
+            g.set_wall_density(f0)
+            g.sync()
+            t0 = time.perf_counter()
+            il, rl, cl, _, fb = g.noslip_wall_learned(mlp, rtol_s, maxit)
+            tl = time.perf_counter() - t0
+            vl, _, _ = g.solve_rbc_vel()
+        finally:
+            g.set_precision(g.PREC_FP64)
+            g.set_gmres_restart(30)
+        rec.update(learned_iters=il, learned_s=tl, learned_conv=cl, fallback=fb,
+                   vel_diff=rel(vl))
         out.append(rec)
         if log:
             log(rec)
@@ -231,6 +245,10 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=65536)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--split", action="store_true", help="eval: the learned solve in the split form")
+    ap.add_argument("--restart", type=int, default=30,
+                    help="eval: GMRES restart length of the warm-started solve")
+    ap.add_argument("--cold", action="store_true",
+                    help="eval: also the warm-started solve's settings from zero")
     ap.add_argument("--model", default="mlp_model.npz")
     a = ap.parse_args(argv)
     say = lambda d: print(json.dumps(d), flush=True)
@@ -284,17 +302,22 @@ def main(argv=None):
         g.prepare_sing_int_on_wall()
         atol = (a.atol_rel or 1e-8) * rhs_norm(g)
         recs = evaluate(g, w.cells, m, steps, a.dt, rtol, a.maxit,
-                        log=lambda r, s=s: say({"seed": s, **r}), split=a.split, atol=atol)
+                        log=lambda r, s=s: say({"seed": s, **r}), split=a.split, atol=atol,
+                        restart=a.restart, cold=a.cold)
         allrec += recs
         g.close()
     ip = np.array([r["plain_iters"] for r in allrec], float)
     il = np.array([r["learned_iters"] for r in allrec], float)
     tp = np.array([r["plain_s"] for r in allrec])
     tl = np.array([r["learned_s"] for r in allrec])
-    say({"summary": True, "steps": len(allrec), "plain_iters_mean": ip.mean(),
-         "learned_iters_mean": il.mean(), "plain_s_mean": tp.mean(), "learned_s_mean": tl.mean(),
-         "fallbacks": int(sum(r["fallback"] for r in allrec)),
-         "vel_diff_max": max(r["vel_diff"] for r in allrec)})
+    summ = {"summary": True, "steps": len(allrec), "plain_iters_mean": ip.mean(),
+            "learned_iters_mean": il.mean(), "plain_s_mean": tp.mean(), "learned_s_mean": tl.mean(),
+            "fallbacks": int(sum(r["fallback"] for r in allrec)),
+            "vel_diff_max": max(r["vel_diff"] for r in allrec)}
+    if a.cold:
+        summ["cold_iters_mean"] = float(np.mean([r["cold_iters"] for r in allrec]))
+        summ["cold_s_mean"] = float(np.mean([r["cold_s"] for r in allrec]))
+    say(summ)
     return 0
 
 

@@ -71,6 +71,61 @@ def test_noslip_wall_gmres(tube):
     assert vg < 1e-3 and abs(vg - vo) <= 1e-8
 
 
+def test_noslip_wall_gmres_restart(tube):
+    """GMRES with a restart length other than PETSc's 30: 60 iterations at restart 20
+    cross two explicit restarts; the oracle runs the same length.  (Long cycles are
+    compared on the device alone, below: CGS without refinement amplifies the last-bit
+    differences of the two operators with the cycle length, 4e-5 in the residual
+    estimate after 45 iterations of one cycle.)"""
+    w, g, o = tube
+    f = np.zeros(3 * w.wall[2])
+    g.set_wall_density(f)
+    o.set_wall_density(f)
+    g.set_gmres_restart(20)
+    o.set_gmres_restart(20)
+    try:
+        ng, rg, cg, vg = g.noslip_wall(1e-8, 60)
+        no, ro, co, vo = o.noslip_wall(1e-8, 60)
+    finally:
+        g.set_gmres_restart(30)
+        o.set_gmres_restart(30)
+    assert ng == no == 60 and cg == co
+    assert abs(rg - ro) <= 1e-6 * ro
+    assert relerr(g.wall_density(), o.wall_density()) < 1e-10
+
+
+def test_gmres_restart_changes_only_the_restart_point(tube):
+    """Restarted GMRES: a longer restart repeats the shorter one's first cycle exactly
+    (same Krylov basis, same Givens estimates) and then departs from it; both solves
+    reach the tolerance and leave the same no-slip residual."""
+    import rbc3d_amd
+    w, g, o = tube
+    f = np.zeros(3 * w.wall[2])
+    hist, vres = {}, {}
+    for m in (30, 100):
+        g.set_wall_density(f)
+        g.set_gmres_restart(m)
+        try:
+            n, r, conv, vm = g.noslip_wall(1e-8, 1000)
+        finally:
+            g.set_gmres_restart(30)
+        assert conv
+        hist[m] = g.gmres_history()
+        vres[m] = vm
+        assert len(hist[m]) == n + 1
+    h30, h100 = hist[30], hist[100]
+    print(f"restart 30: {len(h30) - 1} iterations, restart 100: {len(h100) - 1}")
+    assert np.array_equal(h30[:31], h100[:31])
+    assert len(h100) < len(h30)
+    assert h100[-1] <= 1e-8 * h100[0] and h30[-1] <= 1e-8 * h30[0]
+    assert abs(vres[30] - vres[100]) < 1e-6
+    g.set_wall_density(o.wall_density())  # the state the oracle holds, for the tests below
+    with pytest.raises(rbc3d_amd.Rbc3dError):
+        g.set_gmres_restart(256)
+    with pytest.raises(rbc3d_amd.Rbc3dError):
+        g.set_gmres_restart(0)
+
+
 def test_cell_velocity_solve(tube):
     w, g, o = tube
     vg, ng, _ = g.solve_rbc_vel()
