@@ -483,6 +483,9 @@ extern "C" int rbc3d_create(const rbc3d_conf *conf, rbc3d_ctx **out) {
         hi[d] = R[d];
       }
     }
+    c->pair_img = 1;
+    for (int d = 0; d < 3; d++)
+      if (c->Nf[d] < 2 * R[d] + 3) c->pair_img = 0;
     // kept offsets grouped into x-rows: for each (b, q) the admissible a form one
     // contiguous range (the box distance grows with |a|), and the source bins of a
     // row are contiguous in the bin-sorted source order
@@ -566,6 +569,9 @@ extern "C" int rbc3d_destroy(rbc3d_ctx *c) {
   c->d_linpart.release();
   rbc_stage_flush(c);
   for (auto e : c->timer.pool) (void)hipEventDestroy(e);
+  for (auto e : c->gmres.ev)
+    if (e) (void)hipEventDestroy(e);
+  if (c->gmres.hpin) (void)hipHostFree(c->gmres.hpin);
   c->d_pairstat.release();
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;

@@ -17,41 +17,39 @@
 #define G_T 256
 #define G_MAXV (RBC3D_GMRES_MAX_RESTART + 1)
 
-// partial dot products of w with V[0..k) (column stride ld), one row of partials per
-// block: each thread accumulates eight dot products at a time over its elements, the
-// wavefronts reduce them with shuffles and the block adds its wavefronts in order
-// (one pass over w per eight vectors, two barriers per eight vectors)
+// partial dot products of w with V[0..k) (column stride ld): blockIdx.y takes eight
+// vectors, blockIdx.x a slice of the rows; each thread accumulates the eight dot
+// products over its elements, the wavefronts reduce them with shuffles and the block
+// adds its wavefronts in order into part[j][blockIdx.x]
 __global__ void __launch_bounds__(G_T) k_mdot(int n, int k, const double *__restrict__ V, size_t ld,
                                               const double *__restrict__ w, double *part) {
   __shared__ double red[8][G_T / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int j0 = 0; j0 < k; j0 += 8) {
-    const int nj = min(8, k - j0);
-    double s[8];
+  const int j0 = blockIdx.y * 8;
+  const int nj = min(8, k - j0);
+  double s[8];
 #pragma unroll
-    for (int q = 0; q < 8; q++) s[q] = 0.;
-    for (int i = blockIdx.x * G_T + threadIdx.x; i < n; i += gridDim.x * G_T) {
-      const double wi = w[i];
+  for (int q = 0; q < 8; q++) s[q] = 0.;
+  for (int i = blockIdx.x * G_T + threadIdx.x; i < n; i += gridDim.x * G_T) {
+    const double wi = w[i];
 #pragma unroll
-      for (int q = 0; q < 8; q++)
-        if (q < nj) s[q] += V[(size_t)(j0 + q) * ld + i] * wi;
-    }
+    for (int q = 0; q < 8; q++)
+      if (q < nj) s[q] += V[(size_t)(j0 + q) * ld + i] * wi;
+  }
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      double v = s[q];
-      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-      s[q] = v;
-    }
-    if (lane == 0)
+  for (int q = 0; q < 8; q++) {
+    double v = s[q];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    s[q] = v;
+  }
+  if (lane == 0)
 #pragma unroll
-      for (int q = 0; q < 8; q++) red[q][wv] = s[q];
-    __syncthreads();
-    if (threadIdx.x < nj) {
-      double t = 0.;
-      for (int u = 0; u < G_T / 64; u++) t += red[threadIdx.x][u];
-      part[(size_t)(j0 + threadIdx.x) * gridDim.x + blockIdx.x] = t;
-    }
-    __syncthreads();
+    for (int q = 0; q < 8; q++) red[q][wv] = s[q];
+  __syncthreads();
+  if (threadIdx.x < nj) {
+    double t = 0.;
+    for (int u = 0; u < G_T / 64; u++) t += red[threadIdx.x][u];
+    part[(size_t)(j0 + threadIdx.x) * gridDim.x + blockIdx.x] = t;
   }
 }
 
@@ -99,9 +97,11 @@ __global__ void k_sub(int n, const double *__restrict__ b, double *r) {  // r = 
 // out[j] = <V_j, w>, j < k, left on the device
 static int dots_dev(hipStream_t st, GmresWork &ws, int n, int k, const double *V, size_t ld,
                     const double *w, double *out) {
-  int nb = std::min(1024, (n + G_T - 1) / G_T);
+  // about 2048 workgroups over the row slices and the groups of eight vectors
+  const int groups = (k + 7) / 8;
+  const int nb = std::max(1, std::min((n + G_T - 1) / G_T, std::max(32, 2048 / groups)));
   RBC_TRY(ws.part.ensure((size_t)nb * G_MAXV + 1));
-  k_mdot<<<nb, G_T, 0, st>>>(n, k, V, ld, w, ws.part.p);
+  k_mdot<<<dim3(nb, groups), G_T, 0, st>>>(n, k, V, ld, w, ws.part.p);
   k_mdot_final<<<k, 64, 0, st>>>(k, nb, ws.part.p, out);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;
@@ -116,13 +116,15 @@ static int dots(hipStream_t st, GmresWork &ws, int n, int k, const double *V, si
   return RBC3D_OK;
 }
 
-// w /= sqrt(t2[0]) when sqrt(t2[0]) > bound (the host decides happy breakdown from the
-// same value afterwards)
-__global__ void k_scale_norm(int n, const double *__restrict__ t2, double bound, double *w) {
+// w /= sqrt(t2[0]) unless it is zero.  PETSc skips the scaling at a happy breakdown
+// (|w| <= 1e-30 |g|), which the host decides from the same value afterwards; the
+// vector is then never used (the cycle ends and the update takes the vectors before
+// it), so scaling it regardless changes no result.
+__global__ void k_scale_norm(int n, const double *__restrict__ t2, double *w) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double tt = sqrt(t2[0]);
-  if (tt > bound) w[i] = (1.0 / tt) * w[i];
+  if (tt > 0.) w[i] = (1.0 / tt) * w[i];
 }
 
 int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, double *)> &matvec,
@@ -135,14 +137,17 @@ int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, doubl
   size_t ld = (size_t)n + 64;
   RBC_TRY(ws.V.ensure(ld * (restart + 1)));
   RBC_TRY(ws.w.ensure(n + 1));
+  RBC_TRY(ws.hd.ensure(2 * G_MAXV + 2));
+  if (!ws.hpin) HIP_TRY(hipHostMalloc((void **)&ws.hpin, sizeof(double) * 3 * (G_MAXV + 2)));
+  for (auto &e : ws.ev)
+    if (!e) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   double *V = ws.V.p;
   const int nb = (n + 255) / 256;
   int its = 0;
   double ttol = 0, rnorm0 = 0, res = 0;
   *reason = 0;
   if (!x0_nonzero) HIP_TRY(hipMemsetAsync(d_x, 0, sizeof(double) * n, st));
-  std::vector<double> H((size_t)(restart + 1) * restart), cs(restart), sn(restart), g(restart + 1),
-      h(restart + 1);
+  std::vector<double> H((size_t)(restart + 1) * restart), cs(restart), sn(restart), g(restart + 1);
   bool first = true;
   while (true) {
     // r = b - A x  (KSPInitialResidual); with x = 0 this is b
@@ -178,27 +183,34 @@ int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, doubl
     g[0] = res;
     int it = 0;
     bool happy = false;
-    while (it < restart && its < maxit) {
-      double *vj = V + ld * it, *vn = V + ld * (it + 1);
+    // One Arnoldi step on the device: w = A V_j, classical Gram-Schmidt h = V^T w,
+    // w -= V h, |w|^2 and the normalisation, then (h, |w|^2) copied to a pinned slot
+    // and marked by that slot's event.  The step needs nothing from the host, so step
+    // j + 1 is queued before the host reads step j's column: the device never waits
+    // for the Givens update.  A step queued past the end of the cycle (convergence,
+    // breakdown) only writes V_{j+2} and the device scratch, which nothing reads.
+    auto enqueue = [&](int j) -> int {
+      double *vj = V + ld * j, *vn = V + ld * (j + 1);
       RBC_TRY(matvec(vj, vn));
-      // classical Gram-Schmidt: h = V^T w, w -= V h
-      // the projections h stay on the device for the update; |w|^2 follows in
-      // hd[G_MAXV], the normalisation runs on the device, and the host reads h and
-      // |w|^2 once per iteration
-      RBC_TRY(ws.hd.ensure(2 * G_MAXV + 2));
-      RBC_TRY(dots_dev(st, ws, n, it + 1, V, ld, vn, ws.hd.p));
-      k_maxpy<<<nb, 256, 0, st>>>(n, it + 1, V, ld, ws.hd.p, vn);
-      RBC_TRY(dots_dev(st, ws, n, 1, vn, ld, vn, ws.hd.p + G_MAXV));
-      // happy breakdown bound as in PETSc: haptol * |g(it)|
-      double hapbnd = fabs(g[it]) * 1.0e-30;
-      k_scale_norm<<<nb, 256, 0, st>>>(n, ws.hd.p + G_MAXV, hapbnd, vn);
+      RBC_TRY(dots_dev(st, ws, n, j + 1, V, ld, vn, ws.hd.p));
+      k_maxpy<<<nb, 256, 0, st>>>(n, j + 1, V, ld, ws.hd.p, vn);
+      RBC_TRY(dots_dev(st, ws, n, 1, vn, ld, vn, ws.hd.p + j + 1));
+      k_scale_norm<<<nb, 256, 0, st>>>(n, ws.hd.p + j + 1, vn);
       HIP_TRY(hipGetLastError());
-      HIP_TRY(hipMemcpyAsync(h.data(), ws.hd.p, sizeof(double) * (it + 1), hipMemcpyDeviceToHost, st));
-      double tt2;
-      HIP_TRY(hipMemcpyAsync(&tt2, ws.hd.p + G_MAXV, sizeof(double), hipMemcpyDeviceToHost, st));
-      HIP_TRY(hipStreamSynchronize(st));
-      double tt = sqrt(tt2);
-      for (int i = 0; i <= it; i++) H[(size_t)i * restart + it] = h[i];
+      HIP_TRY(hipMemcpyAsync(ws.hpin + (size_t)(j & 1) * (G_MAXV + 2), ws.hd.p,
+                             sizeof(double) * (j + 2), hipMemcpyDeviceToHost, st));
+      HIP_TRY(hipEventRecord(ws.ev[j & 1], st));
+      return RBC3D_OK;
+    };
+    if (its < maxit) RBC_TRY(enqueue(0));
+    while (it < restart && its < maxit) {
+      if (it + 1 < restart && its + 1 < maxit) RBC_TRY(enqueue(it + 1));
+      HIP_TRY(hipEventSynchronize(ws.ev[it & 1]));
+      const double *col = ws.hpin + (size_t)(it & 1) * (G_MAXV + 2);
+      // happy breakdown bound as in PETSc: haptol * |g(it)|
+      const double hapbnd = fabs(g[it]) * 1.0e-30;
+      const double tt = sqrt(col[it + 1]);
+      for (int i = 0; i <= it; i++) H[(size_t)i * restart + it] = col[i];
       H[(size_t)(it + 1) * restart + it] = tt;
       if (!(tt > hapbnd)) happy = true;
       // apply the previous rotations and form a new one (KSPGMRESUpdateHessenberg)
@@ -238,18 +250,19 @@ int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, doubl
         break;
       }
     }
-    // x += V y with H y = g (upper triangular)
-    std::vector<double> y(it, 0.);
+    // x += V y with H y = g (upper triangular); y goes through the pinned buffer
+    double *y = ws.hpin + 2 * (size_t)(G_MAXV + 2);
     for (int i = it - 1; i >= 0; i--) {
       double s = g[i];
       for (int k = i + 1; k < it; k++) s -= H[(size_t)i * restart + k] * y[k];
       y[i] = s / H[(size_t)i * restart + i];
     }
     if (it > 0) {
-      HIP_TRY(hipMemcpyAsync(ws.hd.p, y.data(), sizeof(double) * it, hipMemcpyHostToDevice, st));
-      HIP_TRY(hipStreamSynchronize(st));
+      HIP_TRY(hipMemcpyAsync(ws.hd.p, y, sizeof(double) * it, hipMemcpyHostToDevice, st));
       k_update<<<nb, 256, 0, st>>>(n, it, V, ld, ws.hd.p, d_x);
       HIP_TRY(hipGetLastError());
+      // the copy has read y before the host writes it again: the next cycle starts
+      // with a residual norm that synchronises the stream
     }
     if (*reason != 0 || its >= maxit) break;
   }

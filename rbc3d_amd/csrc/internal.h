@@ -90,6 +90,10 @@ struct GridType {
 struct GmresWork {
   DBuf<double> V, w, part, hd;
   std::vector<double> history;  // residual estimates of the last solve
+  // pinned host slots for the Hessenberg columns of two Arnoldi steps in flight and
+  // the update coefficients, with an event per column slot (created on first use)
+  double *hpin = nullptr;
+  hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
 // Stage timing (HIP events on the context's stream, enabled on request)
@@ -171,6 +175,9 @@ struct rbc3d_ctx {
   int nbins;
   // finer grid of the pair kernel (width >= rc/2) and its pruned source-bin offsets
   int Nf[3];
+  // every dimension has at least 2R+3 fine bins (R: bins within rc): the visited
+  // source bins then determine the periodic image of every in-cutoff pair
+  int pair_img = 0;
   double iLbNf[3];
   int nfbins;
   size_t noffs = 0;
