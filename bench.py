@@ -90,7 +90,7 @@ MLP_C3 = os.path.join(ROOT, "rbc3d_amd", "data", "mlp_C3.npz")
 CHOSEN_RESTART = 150
 
 
-def timestep_bench(steps, warmup, maxit=4000):
+def timestep_bench(steps, warmup, maxit=4000, forms=None):
     """One timestep of the 200-cell tube at viscosity ratio 1 (ModTimeInt.F90:716-808 +
     ModNoSlip.F90:45-150): Solve_RBC_Vel (right-hand side: cell single layer, walls,
     PME; no cell GMRES at ratio 1; density g := v), NoSlipWall (residual velocity,
@@ -172,7 +172,10 @@ def timestep_bench(steps, warmup, maxit=4000):
         "steps": steps, "wall_self_matrix_nnz": int(nnz.value)}
     plain, vp = run("plain")
     out["plain"] = plain
+    only = forms
     forms = ["split", "fp64_restart150", "chosen"] + (["chosen_learned"] if mlp is not None else [])
+    if only is not None:  # a subset, for profiling one form (the plain form always runs)
+        forms = [f for f in forms if f in only]
     for form in forms:
         r, v = run(form)
         r["velocity_rel_err_vs_plain"] = float(np.abs(v - vp).max() / np.abs(vp).max())
@@ -180,7 +183,7 @@ def timestep_bench(steps, warmup, maxit=4000):
         out[form] = r
     if mlp is None:
         out["chosen_learned"] = {"skipped": f"no trained network at {MLP_C3}"}
-    else:
+    elif only is None:
         out["learned_series"] = learned_series_bench(mlp)
     # the headline fields (plain FP64, the reference's arithmetic)
     out["s_per_timestep"] = plain["s_per_timestep"]
@@ -306,6 +309,9 @@ def main():
     ap.add_argument("--timestep-steps", type=int, default=3)
     ap.add_argument("--c5", action="store_true",
                     help="also time the 4,096-cell mixed-size matvec (C5)")
+    ap.add_argument("--timestep-forms", default=None,
+                    help="comma list of timestep forms beside plain (default: all, and the "
+                         "learned series)")
     ap.add_argument("--timestep-only", action="store_true",
                     help="only the tube timestep (prints its JSON object)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -318,7 +324,8 @@ def main():
         sys.exit(2)
 
     if args.timestep_only:
-        print(json.dumps(timestep_bench(args.timestep_steps, args.warmup)))
+        forms = args.timestep_forms.split(",") if args.timestep_forms else None
+        print(json.dumps(timestep_bench(args.timestep_steps, args.warmup, forms=forms)))
         return
 
     from rbc3d_amd import configs

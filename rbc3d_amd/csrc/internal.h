@@ -134,6 +134,7 @@ struct WallData {
   std::vector<double> x;
   std::vector<int> e2v, v2v, glb;
   std::vector<double> vnorm;  // area-weighted vertex normals (nvert,3)
+  double rmax = 0;            // largest triangle centroid-to-vertex distance
   DBuf<double> d_x, d_f, d_area, d_eps, d_cen, d_vnorm;
   DBuf<int> d_e2v, d_glb;
   // cell list of wall sources (triangle centroids)

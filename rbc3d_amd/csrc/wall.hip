@@ -239,7 +239,23 @@ struct WallArgs {
   const double *wx, *wf, *eps;  // vertices (nvert,3), traction (nvert,3), epsDist
   const int *e2v;               // (nele,3) 1-based
   const int *bin_start, *order; // centroid cell list (coarse bins)
+  const double *cen;            // triangle centroids (nele,3)
+  double far2;                  // (rc + largest centroid-vertex distance)^2, padded
 };
+
+// A triangle whose centroid (minimum image) is farther from xi than rc plus the
+// largest centroid-to-vertex distance of the mesh is farther than rc in every image:
+// this skips it before its vertices are loaded, and skips nothing the exact distance
+// test would keep.
+__device__ __forceinline__ bool tri_far(const WallArgs &a, const double xi[3], int e) {
+  double s2 = 0;
+  for (int d = 0; d < 3; d++) {
+    double t = a.cen[e + (size_t)a.nele * d] - xi[d];
+    t -= rint(t * a.iLb[d]) * a.Lb[d];
+    s2 += t * t;
+  }
+  return s2 > a.far2;
+}
 
 __device__ __forceinline__ void load_tri(const WallArgs &a, int e, double x[3][3], double f[3][3]) {
   for (int l = 0; l < 3; l++) {
@@ -284,6 +300,7 @@ __global__ void __launch_bounds__(128) k_wall_targets(WallArgs a, double c1, int
     int b = n1 + a.Nc[0] * (n2 + a.Nc[1] * n3);
     for (int k = a.bin_start[b]; k < a.bin_start[b + 1]; k++) {
       int e = a.order[k];
+      if (tri_far(a, xi, e)) continue;
       double x[3][3], f[3][3], dv[3], s0, t0;
       load_tri(a, e, x, f);
       shift_tri(a, xi, x);
@@ -309,6 +326,7 @@ __global__ void __launch_bounds__(128) k_wall_count(WallArgs a, int *cnt) {
     int b = n1 + a.Nc[0] * (n2 + a.Nc[1] * n3);
     for (int k = a.bin_start[b]; k < a.bin_start[b + 1]; k++) {
       int e = a.order[k];
+      if (tri_far(a, xi, e)) continue;
       double x[3][3], s0, t0;
       load_tri(a, e, x, nullptr);
       shift_tri(a, xi, x);
@@ -333,6 +351,7 @@ __global__ void __launch_bounds__(128) k_wall_fill(WallArgs a, const int *__rest
     int b = n1 + a.Nc[0] * (n2 + a.Nc[1] * n3);
     for (int k = a.bin_start[b]; k < a.bin_start[b + 1]; k++) {
       int e = a.order[k];
+      if (tri_far(a, xi, e)) continue;
       double x[3][3], f[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}}, s0, t0, rhs[3], lhs[3][3][3];
       load_tri(a, e, x, nullptr);
       shift_tri(a, xi, x);
@@ -446,6 +465,9 @@ static WallArgs wall_args(rbc3d_ctx *c) {
   a.e2v = w.d_e2v.p;
   a.bin_start = w.d_bin_start.p;
   a.order = w.d_order.p;
+  a.cen = w.d_cen.p;
+  const double far = (c->conf.rc + w.rmax) * (1 + 1e-9) + 1e-12;
+  a.far2 = far * far;
   return a;
 }
 
@@ -508,6 +530,7 @@ extern "C" int rbc3d_set_wall(rbc3d_ctx *c, int nvert, int nele, const double *x
   w.ndof = 3 * p;
   // Wall_ComputeGeometry (common/ModWall.F90:118-145) and centroids
   std::vector<double> area(nele), eps(nele), cen((size_t)3 * nele), vn((size_t)3 * p, 0.);
+  double rmax = 0;  // largest centroid-to-vertex distance
   for (int e = 0; e < nele; e++) {
     double xe[3][3];
     for (int l = 0; l < 3; l++)
@@ -529,7 +552,16 @@ extern "C" int rbc3d_set_wall(rbc3d_ctx *c, int nvert, int nele, const double *x
       for (int d = 0; d < 3; d++) vn[3 * (size_t)gi + d] += cr[d];
     }
     for (int d = 0; d < 3; d++) cen[e + (size_t)nele * d] = (1.0 / 3) * ((xe[0][d] + xe[1][d]) + xe[2][d]);
+    for (int l = 0; l < 3; l++) {
+      double r2 = 0;
+      for (int d = 0; d < 3; d++) {
+        const double t = xe[l][d] - cen[e + (size_t)nele * d];
+        r2 += t * t;
+      }
+      rmax = std::max(rmax, sqrt(r2));
+    }
   }
+  w.rmax = rmax;
   // dedup map for AssembleArray (common/ModNoSlip.F90:361-384): the last vertex with a
   // given global index supplies that unknown's value
   std::vector<int> last(p, -1);
