@@ -139,6 +139,8 @@ struct WallData {
   DBuf<int> d_e2v, d_glb;
   // cell list of wall sources (triangle centroids)
   DBuf<int> d_bin_start, d_order;
+  // targets of AddIntOnWalls sorted by the same coarse bins, and their 64-target chunks
+  DBuf<int> d_tg_bin, d_tg_order, d_tg_start, d_tg_chunk;
   // self matrix CSR (rows i + k nvert)
   DBuf<int> d_rowptr, d_col;
   DBuf<double> d_val;

@@ -313,6 +313,99 @@ __global__ void __launch_bounds__(128) k_wall_targets(WallArgs a, double c1, int
   for (int d = 0; d < 3; d++) v[ti + (size_t)nT * d] += acc[d];
 }
 
+// The same sums with the targets sorted by coarse bin: one wavefront takes up to 64
+// targets of one bin and streams the triangles of its 27 neighbouring bins through
+// LDS, 64 at a time, in the order k_wall_targets visits them, so every target adds the
+// same terms in the same order (bit-identical) while the triangle data arrive as
+// coalesced loads instead of one dependent chain per target.
+#define WT_T 64
+__global__ void k_wall_chunks(int nbins, const int *__restrict__ start, int *cnt) {
+  int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b > nbins) return;
+  cnt[b] = b < nbins ? (start[b + 1] - start[b] + WT_T - 1) / WT_T : 0;
+}
+
+__global__ void __launch_bounds__(WT_T) k_wall_targets_tiled(WallArgs a, double c1, int nT,
+                                                             const double *__restrict__ tx,
+                                                             const int *__restrict__ tindx,
+                                                             const int *__restrict__ tact,
+                                                             const double *__restrict__ tA,
+                                                             const int *__restrict__ torder,
+                                                             const int *__restrict__ tstart,
+                                                             const int *__restrict__ chunk_off,
+                                                             int nbins, double *v) {
+  __shared__ double sx[9][WT_T], sf[9][WT_T], scen[3][WT_T], seps[WT_T];
+  const int lane = threadIdx.x, blk = blockIdx.x;
+  if (blk >= chunk_off[nbins]) return;
+  int lo = 0, hi = nbins;  // chunk_off[lo] <= blk < chunk_off[hi]
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (chunk_off[mid] <= blk)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  const int bin = lo;
+  const int k = tstart[bin] + (blk - chunk_off[lo]) * WT_T + lane;
+  const bool have = k < tstart[bin + 1];
+  const int ti = have ? torder[k] : 0;
+  const bool act = have && tact[ti] && tindx[ti] != a.wall_id;
+  double xi[3] = {0, 0, 0};
+  if (have) {
+    xi[0] = tx[ti];
+    xi[1] = tx[ti + nT];
+    xi[2] = tx[ti + 2 * (size_t)nT];
+  }
+  const int i1 = bin % a.Nc[0], i2 = (bin / a.Nc[0]) % a.Nc[1], i3 = bin / (a.Nc[0] * a.Nc[1]);
+  double acc[3] = {0, 0, 0};
+  for (int nb = 0; nb < 27; nb++) {
+    int n1 = imod_d(i1 + nb / 9 - 1, a.Nc[0]), n2 = imod_d(i2 + (nb / 3) % 3 - 1, a.Nc[1]),
+        n3 = imod_d(i3 + nb % 3 - 1, a.Nc[2]);
+    int b = n1 + a.Nc[0] * (n2 + a.Nc[1] * n3);
+    const int s0b = a.bin_start[b], s1b = a.bin_start[b + 1];
+    for (int base = s0b; base < s1b; base += WT_T) {
+      const int cnt = min(WT_T, s1b - base);
+      __syncthreads();
+      if (lane < cnt) {
+        const int e = a.order[base + lane];
+        for (int l = 0; l < 3; l++) {
+          const int iv = a.e2v[e + (size_t)a.nele * l] - 1;
+          for (int d = 0; d < 3; d++) {
+            sx[3 * l + d][lane] = a.wx[iv + (size_t)a.nvert * d];
+            sf[3 * l + d][lane] = a.wf[iv + (size_t)a.nvert * d];
+          }
+        }
+        for (int d = 0; d < 3; d++) scen[d][lane] = a.cen[e + (size_t)a.nele * d];
+        seps[lane] = a.eps[e];
+      }
+      __syncthreads();
+      if (!act) continue;
+      for (int j = 0; j < cnt; j++) {
+        double s2 = 0;
+        for (int d = 0; d < 3; d++) {
+          double t = scen[d][j] - xi[d];
+          t -= rint(t * a.iLb[d]) * a.Lb[d];
+          s2 += t * t;
+        }
+        if (s2 > a.far2) continue;
+        double x[3][3], f[3][3], dv[3], s0, t0;
+        for (int l = 0; l < 3; l++)
+          for (int d = 0; d < 3; d++) {
+            x[l][d] = sx[3 * l + d][j];
+            f[l][d] = sf[3 * l + d][j];
+          }
+        shift_tri(a, xi, x);
+        double rr = min_dist_to_tri(xi, x, s0, t0);
+        if (rr > a.rc) continue;
+        tri_int(a.R, a.ew, x, f, xi, rr < seps[j], s0, t0, dv, nullptr);
+        for (int d = 0; d < 3; d++) acc[d] += c1 * dv[d] / tA[ti];
+      }
+    }
+  }
+  if (act)
+    for (int d = 0; d < 3; d++) v[ti + (size_t)nT * d] += acc[d];
+}
+
 // Self-matrix assembly (PrepareSingIntOnWall :181-308), pass 1: triangles per vertex
 __global__ void __launch_bounds__(128) k_wall_count(WallArgs a, int *cnt) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -726,8 +819,23 @@ int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double c1, int nT, const double *d_tx
                                                 w.d_f.p, d_A, d_v);
     }
   } else {
-    k_wall_targets<<<(nT + 127) / 128, 128, 0, st>>>(wall_args(c), c1, nT, d_tx, d_tindx, d_act, d_A,
-                                                     d_v);
+    // targets sorted by coarse bin, then one wavefront per 64 targets of a bin
+    const int nbins = c->nbins;
+    RBC_TRY(w.d_tg_bin.ensure(nT + 1));
+    RBC_TRY(w.d_tg_order.ensure(nT + 1));
+    RBC_TRY(w.d_tg_start.ensure(nbins + 1));
+    RBC_TRY(w.d_tg_chunk.ensure(2 * (size_t)(nbins + 1)));
+    RBC_TRY(rbc_hash_sort(c, nT, d_tx, nT, w.d_tg_bin.p, w.d_tg_order.p, w.d_tg_start.p));
+    int *ccnt = w.d_tg_chunk.p, *coff = ccnt + (nbins + 1);
+    k_wall_chunks<<<(nbins + 1 + 255) / 256, 256, 0, st>>>(nbins, w.d_tg_start.p, ccnt);
+    HIP_TRY(hipGetLastError());
+    size_t tmp = 0;
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, ccnt, coff, nbins + 1, st));
+    RBC_TRY(c->d_cub_tmp.ensure(tmp + 16));
+    HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->d_cub_tmp.p, tmp, ccnt, coff, nbins + 1, st));
+    const unsigned nblk = (unsigned)(nbins + (nT + WT_T - 1) / WT_T + 1);
+    k_wall_targets_tiled<<<nblk, WT_T, 0, st>>>(wall_args(c), c1, nT, d_tx, d_tindx, d_act, d_A,
+                                                w.d_tg_order.p, w.d_tg_start.p, coff, nbins, d_v);
   }
   rbc_stage_end(c, ST_WALL);
   HIP_TRY(hipGetLastError());
