@@ -673,11 +673,13 @@ __global__ void __launch_bounds__(PT *PW) __attribute__((amdgpu_waves_per_eu(6))
 // wavefront first tests its 64 sources of a tile against its target with the heads
 // alone and keeps the in-cutoff ones as two bit masks (far, near); it then evaluates
 // only those, so the loop that evaluates the Stokeslet / stresslet runs as many
-// times as the busiest lane has in-cutoff sources instead of once per candidate.  The Ewald tables (FP32) are held
-// in LDS; workgroups are persistent, stride over the target chunks, and each XCD
+// times as the busiest lane has in-cutoff sources instead of once per candidate.  The
+// Ewald tables (FP32) are held in LDS; a workgroup takes one target chunk (the
+// kernel also strides over chunks when launched with fewer workgroups), and each XCD
 // takes one contiguous range of chunks (neighbouring chunks share source tiles in
 // that XCD's L2).
 #define PS32 (64 * PW)  // sources per tile: 64 per wavefront
+#define RUN_MAX 128     // source runs per chunk: two per row of source bins
 #define RBC_SPLIT_NEAR 0.25  // pairs with r < RBC_SPLIT_NEAR rc run in FP64
 
 struct Pair32Args {
@@ -761,17 +763,19 @@ __global__ void __launch_bounds__(PT *PW) k_pair32(Pair32Args a) {
   // the wavefronts' partial sums reuse the position buffers once a chunk's tiles are done
   static_assert(sizeof(double) * (PW - 1) * 3 * PT <= sizeof(spos), "red in spos");
   double(*red)[3][PT] = reinterpret_cast<double(*)[3][PT]>(&spos[0][0]);
+  __shared__ int run_start[RUN_MAX], run_cnt[RUN_MAX], run_cum[RUN_MAX + 1];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  for (int t = threadIdx.x; t < NTB * (RBC_NTAB + 1); t += PT * PW) {
-    const int q = t / (RBC_NTAB + 1), e = t - q * (RBC_NTAB + 1);
-    const int g = SL ? q : 2;  // global table: sl1, sl2, dl1
-    tab[q][e] = a.tab[(size_t)g * (RBC_NTAB + 1) + e];
-  }
   const int nb = a.nfbins;
   const int nchunk = a.chunk_off[nb];
   const int xcd = (int)(blockIdx.x % RBC_NXCD), nper = (int)(gridDim.x / RBC_NXCD);
   const int per = (nchunk + RBC_NXCD - 1) / RBC_NXCD;
   const int c_lo = xcd * per, c_hi = min(nchunk, c_lo + per);
+  if (c_lo + (int)(blockIdx.x / RBC_NXCD) >= c_hi) return;  // no chunk for this workgroup
+  for (int t = threadIdx.x; t < NTB * (RBC_NTAB + 1); t += PT * PW) {
+    const int q = t / (RBC_NTAB + 1), e = t - q * (RBC_NTAB + 1);
+    const int g = SL ? q : 2;  // global table: sl1, sl2, dl1
+    tab[q][e] = a.tab[(size_t)g * (RBC_NTAB + 1) + e];
+  }
   const int Nf0 = a.Nf[0], Nf1 = a.Nf[1];
   const size_t np = a.np;
   const double tsd = (double)RBC_NTAB / a.rcd;
@@ -826,34 +830,76 @@ __global__ void __launch_bounds__(PT *PW) k_pair32(Pair32Args a) {
     }
     float acc0 = 0, acc1 = 0, acc2 = 0;
     double accd0 = 0, accd1 = 0, accd2 = 0;
-    for (int ri = 0; ri < a.nrows; ri++) {
-      const unsigned code = (unsigned)a.rows[ri];
-      const int alo = (int)(code & 255) - 128, ahi = (int)((code >> 8) & 255) - 128;
-      const int o2 = (int)((code >> 16) & 255) - 128, o3 = (int)(code >> 24) - 128;
-      const int rowbase = Nf0 * (imod_d(b2 + o2, Nf1) + Nf1 * imod_d(b3 + o3, a.Nf[2]));
-      int rlo[2], rhi[2], nrun = 1;
-      if (ahi - alo + 1 >= Nf0) {
-        rlo[0] = 0;
-        rhi[0] = Nf0 - 1;
-      } else {
-        const int x0 = b1 + alo, x1 = b1 + ahi;
-        if (x0 < 0) {
-          rlo[0] = x0 + Nf0, rhi[0] = Nf0 - 1, rlo[1] = 0, rhi[1] = x1, nrun = 2;
-        } else if (x1 >= Nf0) {
-          rlo[0] = x0, rhi[0] = Nf0 - 1, rlo[1] = 0, rhi[1] = x1 - Nf0, nrun = 2;
+    // The chunk's source runs (the x-row ranges of the sorted sources, in the order the
+    // FP64 kernel visits them) are packed back to back into full tiles: every source
+    // is stored relative to the bin centre, so one tile may hold several runs, and
+    // sparse bins do not leave most of a tile (and three of the four wavefronts) idle.
+    __syncthreads();  // the previous chunk is done with the run table
+    {
+      const int t = threadIdx.x;
+      int st = 0, n = 0;
+      if (t < 2 * a.nrows) {
+        const int ri = t >> 1, ru = t & 1;
+        const unsigned code = (unsigned)a.rows[ri];
+        const int alo = (int)(code & 255) - 128, ahi = (int)((code >> 8) & 255) - 128;
+        const int o2 = (int)((code >> 16) & 255) - 128, o3 = (int)(code >> 24) - 128;
+        const int rowbase = Nf0 * (imod_d(b2 + o2, Nf1) + Nf1 * imod_d(b3 + o3, a.Nf[2]));
+        int rlo[2], rhi[2], nrun = 1;
+        if (ahi - alo + 1 >= Nf0) {
+          rlo[0] = 0;
+          rhi[0] = Nf0 - 1;
         } else {
-          rlo[0] = x0, rhi[0] = x1;
+          const int x0 = b1 + alo, x1 = b1 + ahi;
+          if (x0 < 0) {
+            rlo[0] = x0 + Nf0, rhi[0] = Nf0 - 1, rlo[1] = 0, rhi[1] = x1, nrun = 2;
+          } else if (x1 >= Nf0) {
+            rlo[0] = x0, rhi[0] = Nf0 - 1, rlo[1] = 0, rhi[1] = x1 - Nf0, nrun = 2;
+          } else {
+            rlo[0] = x0, rhi[0] = x1;
+          }
+        }
+        if (ru < nrun) {
+          st = a.bin_start[rowbase + rlo[ru]];
+          n = a.bin_start[rowbase + rhi[ru] + 1] - st;
         }
       }
-      for (int ru = 0; ru < nrun; ru++) {
-        const int s0 = a.bin_start[rowbase + rlo[ru]], s1 = a.bin_start[rowbase + rhi[ru] + 1];
-        for (int base = s0; base < s1; base += PS32) {
-          const int cnt = min(PS32, s1 - base);
+      if (t < RUN_MAX) {
+        run_start[t] = st;
+        run_cnt[t] = n;
+      }
+    }
+    __syncthreads();
+    if (wv == 0) {  // exclusive prefix of the run lengths, two runs per lane
+      const int c0 = run_cnt[2 * lane], c1 = run_cnt[2 * lane + 1];
+      int incl = c0 + c1;
+      for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += u;
+      }
+      run_cum[2 * lane] = incl - c0 - c1;
+      run_cum[2 * lane + 1] = incl - c1;
+      if (lane == 63) run_cum[RUN_MAX] = incl;
+    }
+    __syncthreads();
+    const int total = run_cum[RUN_MAX];
+    {
+      {
+        for (int base = 0; base < total; base += PS32) {
+          const int cnt = min(PS32, total - base);
           __syncthreads();
           {
             const int t = threadIdx.x;  // one source per thread (PS32 = threads)
             if (t < cnt) {
-              const int q = base + t;
+              const int g = base + t;
+              int rl = 0, rh = RUN_MAX;  // the run holding g: run_cum[rl] <= g < run_cum[rl + 1]
+              while (rh - rl > 1) {
+                const int mid = (rl + rh) >> 1;
+                if (run_cum[mid] <= g)
+                  rl = mid;
+                else
+                  rh = mid;
+              }
+              const int q = run_start[rl] + (g - run_cum[rl]);
               double r0 = a.rec[q] - ctr0, r1 = a.rec[q + np] - ctr1, r2 = a.rec[q + 2 * np] - ctr2;
               r0 -= rint(r0 * a.iLb[0]) * a.Lb[0];
               r1 -= rint(r1 * a.iLb[1]) * a.Lb[1];
@@ -1702,8 +1748,7 @@ static int exclusive_scan(rbc3d_ctx *c, const int *in, int *out, int n) {
   return RBC3D_OK;
 }
 
-// launch the split-form pair kernel with the arguments of the FP64 one: persistent
-// workgroups, as many per CU as the LDS (tables + tiles) allows
+// launch the split-form pair kernel with the arguments of the FP64 one
 static int launch_pair32(rbc3d_ctx *c, const PairArgs &pa, int nblk_max,
                          const float *const *maskf) {
   Pair32Args a;
@@ -1751,13 +1796,17 @@ static int launch_pair32(rbc3d_ctx *c, const PairArgs &pa, int nblk_max,
   a.tact = pa.tact;
   a.torder = pa.torder;
   a.tbin_start = pa.tbin_start;
+  if (2 * pa.nrows > RUN_MAX) {
+    rbc_set_error("split pair kernel: more source-bin rows than its run table holds");
+    return RBC3D_EINVAL;
+  }
   a.chunk_off = pa.chunk_off;
   a.vpair = pa.vpair;
   a.stat = pa.stat;
   const bool sl = pa.c1 != 0, dl = pa.c2 != 0;
-  const int per_cu = sl && dl ? 1 : (sl ? 2 : 3);
-  int grid = std::min(nblk_max, c->ncu * per_cu);
-  grid = (grid + RBC_NXCD - 1) / RBC_NXCD * RBC_NXCD;
+  // one workgroup per target chunk at most (the hardware balances chunks of uneven
+  // cost); workgroups without a chunk leave before loading the tables
+  int grid = (int)xcd_grid(nblk_max);
   hipStream_t st = c->stream;
   if (wrap) {
     if (sl && dl)
