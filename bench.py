@@ -261,7 +261,10 @@ def learned_series_bench(mlp, seed=201, steps=4):
 def c5_bench(steps, warmup):
     """Cell-BIE matvec on the large mixed batch: 3,584 biconcave discs (nlat 13) and 512
     spheres of radius 1.4 (nlat 17) at 30% volume fraction, PME 256^3, P=6, viscosity
-    ratio 5; two grid types in one launch of every kernel."""
+    ratio 5; two grid types in one launch of every kernel.  Plain FP64 and the split
+    form on the same vector (error against FP64), each with its stage breakdown (one
+    extra matvec with the stage events), the Legendre GEMMs' rate against the matrix
+    cores' peak and the PME stages' algorithmic HBM rate."""
     import torch
     import rbc3d_amd
     from rbc3d_amd import configs
@@ -274,27 +277,66 @@ def c5_bench(steps, warmup):
                     generator=torch.Generator(device=dev).manual_seed(5))
     b = torch.empty(dof, dtype=torch.float64, device=dev)
     stream = torch.cuda.ExternalStream(g.stream())
-    for _ in range(warmup):
+    cells = w.cells
+    # algorithmic work: Legendre stage of the two global SH transforms (synthesis of the
+    # density, analysis of the velocity), 2 nlat nlat0 (nlat0 + 1) flop per cell and
+    # component each; PME bytes for the double layer (n_f = 9 spread fields)
+    nl = np.asarray(cells.nlat, np.float64)
+    nl0 = np.asarray(cells.nlat0, np.float64)
+    f_leg = 2 * 3 * float(np.sum(2 * nl * nl0 * (nl0 + 1)))
+    nb3 = float(np.prod(w.Nb))
+    ns = float(cells.npoint)
+    b_pme = 8 * (13 * ns + 7 * ns) + 8 * nb3 * (3 * 9 + 9 + 3 + 6 + 3)
+
+    def run(prec):
+        g.set_precision(prec)
+        for _ in range(warmup):
+            g.cell_matmult_dev(u.data_ptr(), b.data_ptr())
+        g.sync()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(steps):
+            g.cell_matmult_dev(u.data_ptr(), b.data_ptr())
+        e1.record(stream)
+        g.sync()
+        torch.cuda.synchronize()
+        ms = max(e0.elapsed_time(e1), 1e3 * (time.perf_counter() - t0)) / steps
+        g.timing(-1)
+        g.timing(1)
         g.cell_matmult_dev(u.data_ptr(), b.data_ptr())
-    g.sync()
-    e0 = torch.cuda.Event(enable_timing=True)
-    e1 = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    e0.record(stream)
-    for _ in range(steps):
-        g.cell_matmult_dev(u.data_ptr(), b.data_ptr())
-    e1.record(stream)
-    g.sync()
-    torch.cuda.synchronize()
-    ms = max(e0.elapsed_time(e1), 1e3 * (time.perf_counter() - t0)) / steps
+        g.sync()
+        st = {k: v[0] for k, v in g.stage_times().items() if v[1]}
+        g.timing(0)
+        out = b.clone()
+        g.set_precision(g.PREC_FP64)
+        leg_ms = st.get("sh_glob", 0.0)
+        pme_ms = sum(st.get(k, 0.0) for k in ("spread_prep", "spread", "fft", "green", "gather"))
+        rec = {"ms_per_step": round(ms, 4), "cell_matvecs_per_s": round(cells.ncell / (ms * 1e-3), 3),
+               "stages_ms": {k: round(v, 4) for k, v in st.items()},
+               # Legendre flop over the whole SH-transform stage (it also holds the
+               # longitude DFTs and the packing), so the rate is a lower bound
+               "sh_legendre": {"flop": f_leg, "sh_stage_ms": round(leg_ms, 4),
+                               "tflops": round(f_leg / (leg_ms * 1e-3) / 1e12, 3) if leg_ms else None},
+               "pme": {"algorithmic_bytes": b_pme, "ms": round(pme_ms, 4),
+                       "gb_per_s": round(b_pme / (pme_ms * 1e-3) / 1e9, 1) if pme_ms else None,
+                       "hbm_peak_gb_per_s": 8000}}
+        return rec, out
+
+    plain, b64 = run(g.PREC_FP64)
+    split, b32 = run(g.PREC_SPLIT)
+    plain["sh_legendre"]["peak_tflops"] = PEAK_FP64_TFLOPS  # FP64 MFMA
+    split["sh_legendre"]["peak_tflops"] = 2500.0 / 3  # three BF16 products per FP32 product
+    split["rel_err_vs_plain"] = float((b32 - b64).abs().max() / b64.abs().max())
     npairs, _ = g.pair_stats()
-    return {"workload": f"C5: {w.cells.ncell} cells (3584 discs nlat 13, 512 spheres R 1.4 "
+    return {"workload": f"C5: {cells.ncell} cells (3584 discs nlat 13, 512 spheres R 1.4 "
                         f"nlat 17), box {w.Lb[0]:.3f}^3 (30% volume fraction), PME {w.Nb[0]}^3, "
                         f"P={w.P}, lambda=5, one cell-BIE matvec",
-            "nodes": w.cells.npoint, "ms_per_step": round(ms, 4),
-            "cell_matvecs_per_s": round(w.cells.ncell / (ms * 1e-3), 3),
-            "pairs_per_matvec": int(npairs), "steps": steps}
+            "nodes": cells.npoint, "pairs_per_matvec": int(npairs), "steps": steps,
+            "ms_per_step": plain["ms_per_step"], "cell_matvecs_per_s": plain["cell_matvecs_per_s"],
+            "plain": plain, "split": split}
 
 
 def main():
