@@ -549,7 +549,7 @@ extern "C" int rbc3d_destroy(rbc3d_ctx *c) {
   for (DBuf<int> *b : {&c->d_node_cell, &c->d_node_ll, &c->d_src_bin, &c->d_src_order,
                        &c->d_bin_start, &c->d_tmp_i, &c->d_tindx, &c->d_tact, &c->d_tbin,
                        &c->d_torder, &c->d_tbin_start, &c->d_chunk_off, &c->d_nbr_cnt,
-                       &c->d_nbr_cell, &c->d_nbr_node, &c->d_work, &c->d_work_cnt, &c->d_flag,
+                       &c->d_nbr_cell, &c->d_nbr_node, &c->d_cell_cand, &c->d_work, &c->d_work_cnt, &c->d_flag,
                        &c->d_cellbin_start, &c->d_cellbin_order, &c->d_pme_key, &c->d_pme_order,
                        &c->d_pme_start, &c->wall.d_e2v, &c->wall.d_glb, &c->wall.d_bin_start,
                        &c->wall.d_order, &c->wall.d_rowptr, &c->wall.d_col})

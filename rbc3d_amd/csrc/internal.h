@@ -202,6 +202,7 @@ struct rbc3d_ctx {
   DBuf<double> d_tx, d_tA, d_v, d_vacc;
   DBuf<int> d_tindx, d_tact, d_tbin, d_torder, d_tbin_start, d_chunk_off;
   DBuf<int> d_nbr_cnt, d_nbr_cell, d_nbr_node;
+  DBuf<int> d_cell_cand;  // per cell: candidate neighbouring cells, then their counts
   DBuf<double> d_nbr_dist;
   DBuf<int> d_work, d_work_cnt;
   DBuf<double> d_dvitem, d_scratch;

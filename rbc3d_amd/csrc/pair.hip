@@ -1074,7 +1074,40 @@ struct NbrArgs {
   int *cnt, *slot_cell, *slot_node;
   double *slot_dist;
   int *flag;
+  const int *cand, *ccnt;  // per cell: cells within reach of any of its nodes (k_cell_cands)
 };
+
+// Candidate neighbouring cells per cell, in increasing index: cell j is a candidate of
+// cell i when the minimum-image distance of their bounding-sphere centres does not
+// exceed r_i + r_j + rc (padded).  A target within r_i of c_i that passes k_nbr's
+// sphere test for cell j (|x - c_j| <= rc + r_j) has |c_i - c_j| <= r_i + rc + r_j, so
+// the list holds every cell the full scan would keep for it.  -1: more than
+// RBC_CAND_MAX candidates (the cell's targets scan every cell).
+#define RBC_CAND_MAX 128
+__global__ void __launch_bounds__(NBR_T) k_cell_cands(NbrArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int ci = (blockIdx.x * NBR_T + (int)threadIdx.x) >> 6;
+  if (ci >= a.nrbc) return;
+  const double ci3[3] = {a.sph[4 * ci], a.sph[4 * ci + 1], a.sph[4 * ci + 2]};
+  const double ri = a.sph[4 * ci + 3];
+  int *cand = const_cast<int *>(a.cand) + (size_t)ci * RBC_CAND_MAX;
+  int n = 0;
+  for (int cb = 0; cb < a.nrbc; cb += 64) {
+    const int cj = cb + lane;
+    bool pass = false;
+    if (cj < a.nrbc && cj != ci) {
+      const double cj3[3] = {a.sph[4 * cj], a.sph[4 * cj + 1], a.sph[4 * cj + 2]};
+      double dd[3];
+      const double r = ref_min_image_dist(ci3, cj3, a.Lb, a.iLb, dd);
+      pass = !(r > (a.rc + ri + a.sph[4 * cj + 3]) * (1 + 1e-9) + 1e-9 * a.rc);
+    }
+    const unsigned long long m = __ballot(pass);
+    const int pos = n + __popcll(m & ((1ull << lane) - 1ull));
+    if (pass && pos < RBC_CAND_MAX) cand[pos] = cj;
+    n += __popcll(m);
+  }
+  if (lane == 0) const_cast<int *>(a.ccnt)[ci] = n <= RBC_CAND_MAX ? n : -1;
+}
 
 __device__ __forceinline__ bool nbr_better(double r1, int k1, int n1, double r2, int k2, int n2) {
   if (n1 < 0) return false;
@@ -1106,10 +1139,24 @@ __global__ void __launch_bounds__(NBR_T) k_nbr(NbrArgs a) {
   int n = 0;
   bool ovf = false;
   const size_t np = a.np;
-  for (int cbase = 0; cbase < a.nrbc; cbase += 64) {
-    int cj = cbase + lane;
+  // the cells to test: the candidate list of the target's own cell when the target
+  // lies within that cell's bounding sphere, else every cell
+  const int *clist = nullptr;
+  int ncl = a.nrbc;
+  if (surf_i >= 1 && surf_i <= a.nrbc && a.ccnt[surf_i - 1] >= 0) {
+    const int ci = surf_i - 1;
+    const double c3[3] = {a.sph[4 * ci], a.sph[4 * ci + 1], a.sph[4 * ci + 2]};
+    double dd[3];
+    if (!(ref_min_image_dist(xi, c3, a.Lb, a.iLb, dd) > a.sph[4 * ci + 3])) {
+      clist = a.cand + (size_t)ci * RBC_CAND_MAX;
+      ncl = a.ccnt[ci];
+    }
+  }
+  for (int cbase = 0; cbase < ncl; cbase += 64) {
+    const int idx = cbase + lane;
+    const int cj = idx < ncl ? (clist ? clist[idx] : idx) : -1;
     bool pass = false;
-    if (cj < a.nrbc && cj != surf_i - 1) {
+    if (cj >= 0 && cj != surf_i - 1) {
       double cen[3] = {a.sph[4 * cj], a.sph[4 * cj + 1], a.sph[4 * cj + 2]}, dd[3];
       double rcen = ref_min_image_dist(xi, cen, a.Lb, a.iLb, dd);
       pass = !(rcen > a.rc + a.sph[4 * cj + 3] + 1e-9 * a.rc);
@@ -1118,7 +1165,7 @@ __global__ void __launch_bounds__(NBR_T) k_nbr(NbrArgs a) {
     while (m) {
       int kk = __ffsll((long long)m) - 1;
       m &= m - 1;
-      int c2 = cbase + kk;
+      int c2 = __shfl(cj, kk, 64);
       const CellRec cr = a.cells[c2];
       int npt = cr.nlat * cr.nlon, nch = (npt + 63) / 64;
       double best = 0;
@@ -1944,7 +1991,11 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   na.slot_node = c->d_nbr_node.p;
   na.slot_dist = c->d_nbr_dist.p;
   na.flag = c->d_flag.p;
+  RBC_TRY(c->d_cell_cand.ensure((size_t)(RBC_CAND_MAX + 1) * c->nrbc + 1));
+  na.cand = c->d_cell_cand.p;
+  na.ccnt = c->d_cell_cand.p + (size_t)RBC_CAND_MAX * c->nrbc;
   rbc_stage_begin(c, ST_NBR);
+  k_cell_cands<<<(unsigned)(((size_t)c->nrbc * 64 + NBR_T - 1) / NBR_T), NBR_T, 0, st>>>(na);
   k_nbr<<<(unsigned)(((size_t)nT * 64 + NBR_T - 1) / NBR_T), NBR_T, 0, st>>>(na);
   rbc_stage_end(c, ST_NBR);
   HIP_TRY(hipGetLastError());
