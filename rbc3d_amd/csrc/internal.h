@@ -407,6 +407,11 @@ __host__ __device__ __forceinline__ unsigned fd_div(unsigned n, const FastDiv &f
   unsigned t = (unsigned)(((unsigned long long)n * f.m) >> 32);  // multiply-high
   return (unsigned)(((unsigned long long)t + n) >> f.s);
 }
+// the same for n < 2^31 in 32-bit arithmetic: the multiply-high is below n, so the sum
+// does not overflow
+__device__ __forceinline__ unsigned fd_div32(unsigned n, const FastDiv &f) {
+  return (__umulhi(n, f.m) + n) >> f.s;
+}
 
 // Spline_Interp (common/ModSpline.F90:150-191) on a cell's spline block.
 // blk points at variable v0; nvar consecutive variables are interpolated.

@@ -1203,6 +1203,7 @@ __global__ void __launch_bounds__(NBR_T) k_nbr(NbrArgs a) {
           }
         }
       }
+      if (__ballot(bnode >= 0) == 0ull) continue;  // no node of this cell within rc
       for (int o = 32; o > 0; o >>= 1) {
         double ob = __shfl_xor(best, o, 64);
         int ok = __shfl_xor(bkey, o, 64), on = __shfl_xor(bnode, o, 64);

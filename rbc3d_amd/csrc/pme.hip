@@ -107,6 +107,10 @@ int rbc_pme_init(rbc3d_ctx *c) {
   }
   int H = N1 / 2 + 1;
   size_t ng = (size_t)N1 * N2 * N3, nc = (size_t)H * N2 * N3;
+  if (12 * ng >= (1ull << 31) || 15 * nc >= (1ull << 31)) {
+    rbc_set_error("PME grid too large for the FFT kernels' 32-bit offsets");
+    return RBC3D_EINVAL;
+  }
   RBC_TRY(c->d_grid.ensure(12 * ng));
   RBC_TRY(c->d_cgrid.ensure(2 * 15 * nc));
   RBC_TRY(c->d_vgrid.ensure(3 * ng));
@@ -774,10 +778,10 @@ __device__ __forceinline__ void fft_pass(const typename CxT<T>::t *in, typename 
   const int NR = fNR.d;
   const int tws = NR / Ns;  // twiddle stride N / (Ns R)
   for (int t = threadIdx.x; t < TL * NR; t += blockDim.x) {
-    const int l = fd_div(t, fNR), j = t - l * NR;
-    const C *src = in + (size_t)l * N;
-    C *dst = out + (size_t)l * N;
-    const int jq = fd_div(j, fNs), k = j - jq * Ns;
+    const int l = fd_div32(t, fNR), j = t - l * NR;
+    const C *src = in + l * N;
+    C *dst = out + l * N;
+    const int jq = fd_div32(j, fNs), k = j - jq * Ns;
     C v[R];
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -844,10 +848,11 @@ __device__ typename CxT<T>::t *fft_lds(typename CxT<T>::t *buf, typename CxT<T>:
   return in;
 }
 
-// complex lines: base(l) = (l % A)*sa + (l / A)*sb, element stride es
+// complex lines: base(l) = (l % A)*sa + (l / A)*sb, element stride es (32-bit: the
+// host checks that every offset fits)
 struct LineMap {
   FastDiv A;  // lines per outer index
-  long long sa, sb, es;
+  int sa, sb, es;
   int nlines;
 };
 
@@ -862,20 +867,20 @@ __global__ void __launch_bounds__(FFT_T) k_fft_c2c(typename CxT<T>::t *data, Fft
   const int N = pl.N;
   // load: lines fastest (adjacent lines are adjacent in memory when sa = 1)
   for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
-    const int e = fd_div(t, fTL), l = t - e * TL, L = l0 + l;
+    const int e = fd_div32(t, fTL), l = t - e * TL, L = l0 + l;
     if (L >= lm.nlines) continue;
-    const int Lq = fd_div(L, lm.A), Lr = L - Lq * (int)lm.A.d;
-    long long o = (long long)Lr * lm.sa + (long long)Lq * lm.sb + (long long)e * lm.es;
-    buf[(size_t)l * N + e] = data[o];
+    const int Lq = fd_div32(L, lm.A), Lr = L - Lq * (int)lm.A.d;
+    const int o = Lr * lm.sa + Lq * lm.sb + e * lm.es;
+    buf[l * N + e] = data[o];
   }
   __syncthreads();
   C *res = fft_lds<T>(buf, tmp, TL, pl);
   for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
-    const int e = fd_div(t, fTL), l = t - e * TL, L = l0 + l;
+    const int e = fd_div32(t, fTL), l = t - e * TL, L = l0 + l;
     if (L >= lm.nlines) continue;
-    const int Lq = fd_div(L, lm.A), Lr = L - Lq * (int)lm.A.d;
-    long long o = (long long)Lr * lm.sa + (long long)Lq * lm.sb + (long long)e * lm.es;
-    data[o] = res[(size_t)l * N + e];
+    const int Lq = fd_div32(L, lm.A), Lr = L - Lq * (int)lm.A.d;
+    const int o = Lr * lm.sa + Lq * lm.sb + e * lm.es;
+    data[o] = res[l * N + e];
   }
 }
 
@@ -891,15 +896,15 @@ __global__ void __launch_bounds__(FFT_T) k_fft_r2c(const T *in, typename CxT<T>:
   const int N = pl.N, H = N / 2 + 1, r0 = blockIdx.x * TL;
   // the block's rows are contiguous in memory: element t of the block is row t / N
   for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
-    if (r0 + fd_div(t, fN) >= nrows) continue;
-    buf[t] = cpx<C>(in[(size_t)r0 * N + t], 0.);
+    if (r0 + (int)fd_div32(t, fN) >= nrows) continue;
+    buf[t] = cpx<C>(in[r0 * N + t], 0.);
   }
   __syncthreads();
   C *res = fft_lds<T>(buf, tmp, TL, pl);
   for (int t = threadIdx.x; t < TL * H; t += blockDim.x) {
-    const int l = fd_div(t, fH), e = t - l * H, R = r0 + l;
+    const int l = fd_div32(t, fH), e = t - l * H, R = r0 + l;
     if (R >= nrows) continue;
-    out[(size_t)R * H + e] = res[(size_t)l * N + e];
+    out[R * H + e] = res[l * N + e];
   }
 }
 
@@ -914,17 +919,17 @@ __global__ void __launch_bounds__(FFT_T) k_fft_c2r(const typename CxT<T>::t *in,
   C *buf = lds, *tmp = lds + (size_t)TL * pl.N;
   const int N = pl.N, H = N / 2 + 1, r0 = blockIdx.x * TL;
   for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
-    const int l = fd_div(t, fN), e = t - l * N, R = r0 + l;
+    const int l = fd_div32(t, fN), e = t - l * N, R = r0 + l;
     if (R >= nrows) continue;
     C v;
     if (e == 0) {
-      v = cpx<C>(in[(size_t)R * H].x, 0.);
+      v = cpx<C>(in[R * H].x, 0.);
     } else if (2 * e == N) {
-      v = cpx<C>(in[(size_t)R * H + e].x, 0.);
+      v = cpx<C>(in[R * H + e].x, 0.);
     } else if (e < H) {
-      v = in[(size_t)R * H + e];
+      v = in[R * H + e];
     } else {
-      C w = in[(size_t)R * H + (N - e)];
+      C w = in[R * H + (N - e)];
       v = cpx<C>(w.x, -w.y);
     }
     buf[t] = v;
@@ -932,8 +937,8 @@ __global__ void __launch_bounds__(FFT_T) k_fft_c2r(const typename CxT<T>::t *in,
   __syncthreads();
   C *res = fft_lds<T>(buf, tmp, TL, pl);
   for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
-    if (r0 + fd_div(t, fN) >= nrows) continue;
-    out[(size_t)r0 * N + t] = res[t].x;
+    if (r0 + (int)fd_div32(t, fN) >= nrows) continue;
+    out[r0 * N + t] = res[t].x;
   }
 }
 
@@ -970,17 +975,21 @@ static int fft_c2c(rbc3d_ctx *c, typename CxT<T>::t *data, int ax, int sign, int
   int N1 = c->conf.Nb[0], N2 = c->conf.Nb[1], N3 = c->conf.Nb[2], H = N1 / 2 + 1;
   FftPlan<T> pl = make_plan<T>(c, ax, sign);
   LineMap lm;
+  if ((long long)nf * H * N2 * N3 >= (1LL << 31)) {
+    rbc_set_error("PME grid too large for 32-bit FFT offsets");
+    return RBC3D_EINVAL;
+  }
   if (ax == 1) {  // lines over (kx, kz, f), stride H
     lm.A = make_fastdiv(H);
     lm.sa = 1;
-    lm.sb = (long long)H * N2;
+    lm.sb = H * N2;
     lm.es = H;
     lm.nlines = H * N3 * nf;
   } else {  // ax == 2: lines over (kx, ky, f), stride H*N2
     lm.A = make_fastdiv(H * N2);
     lm.sa = 1;
-    lm.sb = (long long)H * N2 * N3;
-    lm.es = (long long)H * N2;
+    lm.sb = H * N2 * N3;
+    lm.es = H * N2;
     lm.nlines = H * N2 * nf;
   }
   int TL = lines_per_block<T>(pl.N);
