@@ -313,29 +313,26 @@ __global__ void __launch_bounds__(128) k_wall_targets(WallArgs a, double c1, int
   for (int d = 0; d < 3; d++) v[ti + (size_t)nT * d] += acc[d];
 }
 
-// The same sums with the targets sorted by coarse bin: one wavefront takes up to 64
-// targets of one bin and streams the triangles of its 27 neighbouring bins through
-// LDS, 64 at a time, in the order k_wall_targets visits them, so every target adds the
-// same terms in the same order (bit-identical) while the triangle data arrive as
-// coalesced loads instead of one dependent chain per target.
+// The same sums with the targets sorted by coarse bin: a workgroup takes up to 64
+// targets of one bin; its four wavefronts hold the same targets and take every fourth
+// of the 27 neighbouring bins, streaming that bin's triangles through their own LDS
+// tile, 64 at a time, in the order k_wall_targets visits them.  The four partial sums
+// are added in wavefront order (deterministic; the order of the additions differs from
+// k_wall_targets' single running sum by rounding only).
 #define WT_T 64
+#define WT_W 4
 __global__ void k_wall_chunks(int nbins, const int *__restrict__ start, int *cnt) {
   int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b > nbins) return;
   cnt[b] = b < nbins ? (start[b + 1] - start[b] + WT_T - 1) / WT_T : 0;
 }
 
-__global__ void __launch_bounds__(WT_T) k_wall_targets_tiled(WallArgs a, double c1, int nT,
-                                                             const double *__restrict__ tx,
-                                                             const int *__restrict__ tindx,
-                                                             const int *__restrict__ tact,
-                                                             const double *__restrict__ tA,
-                                                             const int *__restrict__ torder,
-                                                             const int *__restrict__ tstart,
-                                                             const int *__restrict__ chunk_off,
-                                                             int nbins, double *v) {
-  __shared__ double sx[9][WT_T], sf[9][WT_T], scen[3][WT_T], seps[WT_T];
-  const int lane = threadIdx.x, blk = blockIdx.x;
+__global__ void __launch_bounds__(WT_T *WT_W) k_wall_targets_tiled(
+    WallArgs a, double c1, int nT, const double *__restrict__ tx, const int *__restrict__ tindx,
+    const int *__restrict__ tact, const double *__restrict__ tA, const int *__restrict__ torder,
+    const int *__restrict__ tstart, const int *__restrict__ chunk_off, int nbins, double *v) {
+  __shared__ double sx[WT_W][9][WT_T], sf[WT_W][9][WT_T], scen[WT_W][3][WT_T], seps[WT_W][WT_T];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, blk = blockIdx.x;
   if (blk >= chunk_off[nbins]) return;
   int lo = 0, hi = nbins;  // chunk_off[lo] <= blk < chunk_off[hi]
   while (hi - lo > 1) {
@@ -358,32 +355,37 @@ __global__ void __launch_bounds__(WT_T) k_wall_targets_tiled(WallArgs a, double 
   }
   const int i1 = bin % a.Nc[0], i2 = (bin / a.Nc[0]) % a.Nc[1], i3 = bin / (a.Nc[0] * a.Nc[1]);
   double acc[3] = {0, 0, 0};
-  for (int nb = 0; nb < 27; nb++) {
+  // each wavefront walks its own bins with its own tile: only wavefront-local
+  // synchronisation (the LDS writes of one wavefront are read by the same wavefront)
+  for (int nb = wv; nb < 27; nb += WT_W) {
     int n1 = imod_d(i1 + nb / 9 - 1, a.Nc[0]), n2 = imod_d(i2 + (nb / 3) % 3 - 1, a.Nc[1]),
         n3 = imod_d(i3 + nb % 3 - 1, a.Nc[2]);
     int b = n1 + a.Nc[0] * (n2 + a.Nc[1] * n3);
     const int s0b = a.bin_start[b], s1b = a.bin_start[b + 1];
     for (int base = s0b; base < s1b; base += WT_T) {
       const int cnt = min(WT_T, s1b - base);
-      __syncthreads();
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
       if (lane < cnt) {
         const int e = a.order[base + lane];
         for (int l = 0; l < 3; l++) {
           const int iv = a.e2v[e + (size_t)a.nele * l] - 1;
           for (int d = 0; d < 3; d++) {
-            sx[3 * l + d][lane] = a.wx[iv + (size_t)a.nvert * d];
-            sf[3 * l + d][lane] = a.wf[iv + (size_t)a.nvert * d];
+            sx[wv][3 * l + d][lane] = a.wx[iv + (size_t)a.nvert * d];
+            sf[wv][3 * l + d][lane] = a.wf[iv + (size_t)a.nvert * d];
           }
         }
-        for (int d = 0; d < 3; d++) scen[d][lane] = a.cen[e + (size_t)a.nele * d];
-        seps[lane] = a.eps[e];
+        for (int d = 0; d < 3; d++) scen[wv][d][lane] = a.cen[e + (size_t)a.nele * d];
+        seps[wv][lane] = a.eps[e];
       }
-      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
       if (!act) continue;
       for (int j = 0; j < cnt; j++) {
         double s2 = 0;
         for (int d = 0; d < 3; d++) {
-          double t = scen[d][j] - xi[d];
+          double t = scen[wv][d][j] - xi[d];
           t -= rint(t * a.iLb[d]) * a.Lb[d];
           s2 += t * t;
         }
@@ -391,19 +393,29 @@ __global__ void __launch_bounds__(WT_T) k_wall_targets_tiled(WallArgs a, double 
         double x[3][3], f[3][3], dv[3], s0, t0;
         for (int l = 0; l < 3; l++)
           for (int d = 0; d < 3; d++) {
-            x[l][d] = sx[3 * l + d][j];
-            f[l][d] = sf[3 * l + d][j];
+            x[l][d] = sx[wv][3 * l + d][j];
+            f[l][d] = sf[wv][3 * l + d][j];
           }
         shift_tri(a, xi, x);
         double rr = min_dist_to_tri(xi, x, s0, t0);
         if (rr > a.rc) continue;
-        tri_int(a.R, a.ew, x, f, xi, rr < seps[j], s0, t0, dv, nullptr);
+        tri_int(a.R, a.ew, x, f, xi, rr < seps[wv][j], s0, t0, dv, nullptr);
         for (int d = 0; d < 3; d++) acc[d] += c1 * dv[d] / tA[ti];
       }
     }
   }
-  if (act)
+  // partial sums of wavefronts 1..3 through LDS (reusing wavefront 0's tile), added by
+  // wavefront 0 in order
+  __syncthreads();
+  double(*red)[3][WT_T] = reinterpret_cast<double(*)[3][WT_T]>(&sx[0][0][0]);
+  if (wv > 0)
+    for (int d = 0; d < 3; d++) red[wv - 1][d][lane] = acc[d];
+  __syncthreads();
+  if (wv == 0 && act) {
+    for (int w = 0; w < WT_W - 1; w++)
+      for (int d = 0; d < 3; d++) acc[d] += red[w][d][lane];
     for (int d = 0; d < 3; d++) v[ti + (size_t)nT * d] += acc[d];
+  }
 }
 
 // Self-matrix assembly (PrepareSingIntOnWall :181-308), pass 1: triangles per vertex
@@ -834,7 +846,7 @@ int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double c1, int nT, const double *d_tx
     RBC_TRY(c->d_cub_tmp.ensure(tmp + 16));
     HIP_TRY(hipcub::DeviceScan::ExclusiveSum(c->d_cub_tmp.p, tmp, ccnt, coff, nbins + 1, st));
     const unsigned nblk = (unsigned)(nbins + (nT + WT_T - 1) / WT_T + 1);
-    k_wall_targets_tiled<<<nblk, WT_T, 0, st>>>(wall_args(c), c1, nT, d_tx, d_tindx, d_act, d_A,
+    k_wall_targets_tiled<<<nblk, WT_T * WT_W, 0, st>>>(wall_args(c), c1, nT, d_tx, d_tindx, d_act, d_A,
                                                 w.d_tg_order.p, w.d_tg_start.p, coff, nbins, d_v);
   }
   rbc_stage_end(c, ST_WALL);
