@@ -152,6 +152,7 @@ struct WallData {
   DBuf<int> d_wsp_ptr, d_wsp_src;
   DBuf<double> d_wsp_w, d_wsp_f;
   DBuf<float> d_wsp_wf;
+  DBuf<unsigned char> d_wsp_row;  // per entry: its point's index within its group of 64
   bool wsp_ready = false;
   // images of the three uniform tractions under the no-slip operator (learned warm
   // start), for the precision mode they were computed in

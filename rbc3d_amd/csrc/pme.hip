@@ -450,11 +450,13 @@ __global__ void k_wsp_keys(int nele, const double *__restrict__ cen, double ih0,
   key[t] = gi + N1 * (gj + N2 * gk);
   val[t] = (int)t;
 }
-__global__ void k_wsp_weights(long long ne, const int *__restrict__ sval, const double *__restrict__ cen,
-                              int nele, double ih0, double ih1, double ih2, int P, int *src,
-                              double *wgt, float *wgtf) {
+__global__ void k_wsp_weights(long long ne, const int *__restrict__ skey, const int *__restrict__ sval,
+                              const double *__restrict__ cen, int nele, double ih0, double ih1,
+                              double ih2, int P, int *src, double *wgt, float *wgtf,
+                              unsigned char *row64) {
   const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
   if (k >= ne) return;
+  row64[k] = (unsigned char)(skey[k] & 63);  // the entry's row within its group of 64
   const int P3 = P * P * P, t = sval[k];
   const int e = t / P3, s = t % P3;
   const int di = s % P, dj = (s / P) % P, dk = s / (P * P);
@@ -506,34 +508,22 @@ __global__ void __launch_bounds__(256) k_wsp_spread(int ng, int nele, double c1,
                                                     const int *__restrict__ ptr,
                                                     const int *__restrict__ src,
                                                     const W *__restrict__ wgt,
+                                                    const unsigned char *__restrict__ row64,
                                                     const double *__restrict__ fsrc, T *grid) {
-  __shared__ int sptr[4][65];
   __shared__ double sacc[4][3][64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int g0 = (blockIdx.x * 4 + wv) * 64;
   const int nr = g0 < ng ? min(64, ng - g0) : 0;
-  if (nr > 0) {
-    sptr[wv][lane] = ptr[g0 + min(lane, nr)];
-    if (lane == 0) sptr[wv][64] = ptr[g0 + nr];
-  }
   for (int d = 0; d < 3; d++) sacc[wv][d][lane] = 0.;
   __syncthreads();
-  const int k0 = nr > 0 ? sptr[wv][0] : 0, k1 = nr > 0 ? sptr[wv][nr] : 0;
+  const int k0 = nr > 0 ? ptr[g0] : 0, k1 = nr > 0 ? ptr[g0 + nr] : 0;
   for (int kb = k0; kb < k1; kb += 64) {
     const int k = kb + lane;
     const bool live = k < k1;
     int r = 0;
     double v0 = 0., v1 = 0., v2 = 0.;
     if (live) {
-      int lo = 0, hi = nr;  // row r: sptr[r] <= k < sptr[r + 1]
-      while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (sptr[wv][mid] <= k)
-          lo = mid;
-        else
-          hi = mid;
-      }
-      r = lo;
+      r = row64[k];  // the row within this wavefront's 64 (entries are sorted by row)
       const int e = src[k];
       const double cw = c1 * (double)wgt[k];
       v0 = cw * fsrc[e];
@@ -599,9 +589,11 @@ static int wall_spread_setup(rbc3d_ctx *c) {
   RBC_TRY(w.d_wsp_wf.ensure(ne));
   RBC_TRY(w.d_wsp_f.ensure(3 * (size_t)w.nele));
   k_wsp_ptr<<<(ng + 1 + 255) / 256, 256, 0, st>>>(ne, skey.p, ng, w.d_wsp_ptr.p);
-  k_wsp_weights<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(ne, sval.p, w.d_cen.p, w.nele, ih0,
-                                                             ih1, ih2, P, w.d_wsp_src.p,
-                                                             w.d_wsp_w.p, w.d_wsp_wf.p);
+  RBC_TRY(w.d_wsp_row.ensure(ne));
+  k_wsp_weights<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(ne, skey.p, sval.p, w.d_cen.p, w.nele,
+                                                             ih0, ih1, ih2, P, w.d_wsp_src.p,
+                                                             w.d_wsp_w.p, w.d_wsp_wf.p,
+                                                             w.d_wsp_row.p);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(st));
   w.wsp_ready = true;
@@ -622,10 +614,12 @@ static int wall_spread(rbc3d_ctx *c, double c1) {
   const unsigned nblk = (unsigned)((ng + 255) / 256);  // 4 wavefronts of 64 rows each
   if (sizeof(T) == 4)
     k_wsp_spread<T, float><<<nblk, 256, 0, st>>>(ng, w.nele, c1, w.d_wsp_ptr.p, w.d_wsp_src.p,
-                                                 w.d_wsp_wf.p, w.d_wsp_f.p, (T *)c->d_grid.p);
+                                                 w.d_wsp_wf.p, w.d_wsp_row.p, w.d_wsp_f.p,
+                                                 (T *)c->d_grid.p);
   else
     k_wsp_spread<T, double><<<nblk, 256, 0, st>>>(ng, w.nele, c1, w.d_wsp_ptr.p, w.d_wsp_src.p,
-                                                  w.d_wsp_w.p, w.d_wsp_f.p, (T *)c->d_grid.p);
+                                                  w.d_wsp_w.p, w.d_wsp_row.p, w.d_wsp_f.p,
+                                                  (T *)c->d_grid.p);
   rbc_stage_end(c, ST_SPREAD);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;
