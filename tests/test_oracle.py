@@ -495,3 +495,28 @@ def test_gmres_matches_dense_solve(orc):
     v, niter, rn = o2.solve_rbc_vel()
     assert niter > 0
     assert relerr(o2._sol, x) < 1e-9
+
+
+def test_gmres_restart_length(orc):
+    """The oracle's GMRES restart length (KSPGMRESSetRestart; PETSc's 30 unless set): any
+    length reaches the dense solution; two lengths longer than the iteration count take
+    identical steps; a short length restarts and needs more iterations."""
+    from rbc3d_amd import configs
+    w = configs.make_small(ncell=2, nlat=5, Nb=(16, 16, 16), seed=3)
+    o = orc.from_workload(w)
+    rhs = o.cell_rhs()
+    n = o.cell_dof()
+    Amat = np.array([o.cell_matmult(np.eye(n)[j]) for j in range(n)]).T
+    x = np.linalg.solve(Amat, rhs)
+    res = {}
+    for m in (3, 30, 200, 255):
+        om = orc.from_workload(w)
+        om.set_gmres_restart(m)
+        v, niter, rn = om.solve_rbc_vel()
+        res[m] = (niter, om._sol.copy())
+        assert relerr(om._sol, x) < 1e-9, m
+    assert res[200][0] < 200 and res[200][0] == res[255][0]
+    assert np.array_equal(res[200][1], res[255][1])
+    assert res[3][0] > res[200][0]
+    with pytest.raises(AssertionError):
+        orc.from_workload(w).set_gmres_restart(256)
