@@ -798,6 +798,37 @@ __device__ __forceinline__ void fft_pass(const typename CxT<T>::t *in, typename 
       dst[base + Ns] = cpx<C>(a1.x + jb1.x, a1.y + jb1.y);
       dst[base + 2 * Ns] = cpx<C>(a0.x - b0.x, a0.y - b0.y);
       dst[base + 3 * Ns] = cpx<C>(a1.x - jb1.x, a1.y - jb1.y);
+    } else if constexpr (R == 3) {
+      // y_q = sum_r v_r w^(qr), w = exp(sign 2 pi i / 3): y_1,2 = m +- i sign (sqrt3/2) d
+      typedef decltype(C().x) Re;
+      const Re hs = (Re)pl.sign * (Re)0.86602540378443864676;
+      const C t = cpx<C>(v[1].x + v[2].x, v[1].y + v[2].y);
+      const C d = cpx<C>(v[1].x - v[2].x, v[1].y - v[2].y);
+      const C m = cpx<C>(v[0].x - (Re)0.5 * t.x, v[0].y - (Re)0.5 * t.y);
+      const Re ux = -hs * d.y, uy = hs * d.x;
+      dst[base] = cpx<C>(v[0].x + t.x, v[0].y + t.y);
+      dst[base + Ns] = cpx<C>(m.x + ux, m.y + uy);
+      dst[base + 2 * Ns] = cpx<C>(m.x - ux, m.y - uy);
+    } else if constexpr (R == 5) {
+      // w = exp(sign 2 pi i / 5); with a_k = v_k + v_(5-k), b_k = v_k - v_(5-k):
+      // y_1,4 = v_0 + c1 a_1 + c2 a_2 +- i sign (s1 b_1 + s2 b_2)
+      // y_2,3 = v_0 + c2 a_1 + c1 a_2 +- i sign (s2 b_1 - s1 b_2)
+      typedef decltype(C().x) Re;
+      const Re c1 = (Re)0.30901699437494742410, c2 = (Re)-0.80901699437494742410;
+      const Re sg = (Re)pl.sign;
+      const Re s1 = sg * (Re)0.95105651629515357212, s2 = sg * (Re)0.58778525229247312917;
+      const C a1 = cpx<C>(v[1].x + v[4].x, v[1].y + v[4].y), b1 = cpx<C>(v[1].x - v[4].x, v[1].y - v[4].y);
+      const C a2 = cpx<C>(v[2].x + v[3].x, v[2].y + v[3].y), b2 = cpx<C>(v[2].x - v[3].x, v[2].y - v[3].y);
+      const C m1 = cpx<C>(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
+      const C m2 = cpx<C>(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
+      // i (s b) = (-(s b).y, (s b).x)
+      const Re u1x = -(s1 * b1.y + s2 * b2.y), u1y = s1 * b1.x + s2 * b2.x;
+      const Re u2x = -(s2 * b1.y - s1 * b2.y), u2y = s2 * b1.x - s1 * b2.x;
+      dst[base] = cpx<C>(v[0].x + a1.x + a2.x, v[0].y + a1.y + a2.y);
+      dst[base + Ns] = cpx<C>(m1.x + u1x, m1.y + u1y);
+      dst[base + 4 * Ns] = cpx<C>(m1.x - u1x, m1.y - u1y);
+      dst[base + 2 * Ns] = cpx<C>(m2.x + u2x, m2.y + u2y);
+      dst[base + 3 * Ns] = cpx<C>(m2.x - u2x, m2.y - u2y);
     } else {
 #pragma unroll
       for (int q = 0; q < R; q++) {
