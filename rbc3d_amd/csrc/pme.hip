@@ -767,7 +767,8 @@ __device__ __forceinline__ C cpx(decltype(C().x) x, decltype(C().x) y) {
 template <int R, typename T>
 __device__ __forceinline__ void fft_pass(const typename CxT<T>::t *in, typename CxT<T>::t *out,
                                          int TL, int N, int Ns, const FastDiv &fNR,
-                                         const FastDiv &fNs, const FftPlan<T> &pl) {
+                                         const FastDiv &fNs, const FftPlan<T> &pl,
+                                         const typename CxT<T>::t *tw) {
   typedef typename CxT<T>::t C;
   const int NR = fNR.d;
   const int tws = NR / Ns;  // twiddle stride N / (Ns R)
@@ -780,7 +781,7 @@ __device__ __forceinline__ void fft_pass(const typename CxT<T>::t *in, typename 
 #pragma unroll
     for (int r = 0; r < R; r++) {
       C x = src[j + r * NR];
-      if (r && k) x = cmul(x, twid(pl.tw, r * k * tws, pl.sign));
+      if (r && k) x = cmul(x, twid(tw, r * k * tws, pl.sign));
       v[r] = x;
     }
     const int base = jq * Ns * R + k;
@@ -835,7 +836,7 @@ __device__ __forceinline__ void fft_pass(const typename CxT<T>::t *in, typename 
         C sum = v[0];
 #pragma unroll
         for (int r = 1; r < R; r++) {
-          C w = twid(pl.tw, ((q * r) % R) * NR, pl.sign);
+          C w = twid(tw, ((q * r) % R) * NR, pl.sign);
           C p = cmul(v[r], w);
           sum.x += p.x;
           sum.y += p.y;
@@ -850,20 +851,20 @@ __device__ __forceinline__ void fft_pass(const typename CxT<T>::t *in, typename 
 // Returns pointer to the buffer holding the result.
 template <typename T>
 __device__ typename CxT<T>::t *fft_lds(typename CxT<T>::t *buf, typename CxT<T>::t *tmp, int TL,
-                                       const FftPlan<T> &pl) {
+                                       const FftPlan<T> &pl, const typename CxT<T>::t *tw) {
   typedef typename CxT<T>::t C;
   int N = pl.N, Ns = 1;
   C *in = buf, *out = tmp;
   for (int s = 0; s < pl.nrad; s++) {
     const int R = pl.rad[s];
     if (R == 2)
-      fft_pass<2, T>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl);
+      fft_pass<2, T>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl, tw);
     else if (R == 3)
-      fft_pass<3, T>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl);
+      fft_pass<3, T>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl, tw);
     else if (R == 4)
-      fft_pass<4, T>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl);
+      fft_pass<4, T>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl, tw);
     else
-      fft_pass<5, T>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl);
+      fft_pass<5, T>(in, out, TL, N, Ns, pl.fNR[s], pl.fNs[s], pl, tw);
     __syncthreads();
     C *t2 = in;
     in = out;
@@ -887,7 +888,10 @@ __global__ void __launch_bounds__(FFT_T) k_fft_c2c(typename CxT<T>::t *data, Fft
   typedef typename CxT<T>::t C;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   C *lds = reinterpret_cast<C *>(lds_raw);
-  C *buf = lds, *tmp = lds + (size_t)TL * pl.N;
+  C *buf = lds, *tmp = lds + (size_t)TL * pl.N, *stw = tmp + (size_t)TL * pl.N;
+  // the axis' twiddle table in LDS (read by every pass; the barrier after the load
+  // phase below makes it visible)
+  for (int m = threadIdx.x; m < pl.N; m += blockDim.x) stw[m] = pl.tw[m];
   const int l0 = blockIdx.x * TL;
   const int N = pl.N;
   // load: lines fastest (adjacent lines are adjacent in memory when sa = 1)
@@ -899,7 +903,7 @@ __global__ void __launch_bounds__(FFT_T) k_fft_c2c(typename CxT<T>::t *data, Fft
     buf[l * N + e] = data[o];
   }
   __syncthreads();
-  C *res = fft_lds<T>(buf, tmp, TL, pl);
+  C *res = fft_lds<T>(buf, tmp, TL, pl, stw);
   for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
     const int e = fd_div32(t, fTL), l = t - e * TL, L = l0 + l;
     if (L >= lm.nlines) continue;
@@ -917,7 +921,10 @@ __global__ void __launch_bounds__(FFT_T) k_fft_r2c(const T *in, typename CxT<T>:
   typedef typename CxT<T>::t C;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   C *lds = reinterpret_cast<C *>(lds_raw);
-  C *buf = lds, *tmp = lds + (size_t)TL * pl.N;
+  C *buf = lds, *tmp = lds + (size_t)TL * pl.N, *stw = tmp + (size_t)TL * pl.N;
+  // the axis' twiddle table in LDS (read by every pass; the barrier after the load
+  // phase below makes it visible)
+  for (int m = threadIdx.x; m < pl.N; m += blockDim.x) stw[m] = pl.tw[m];
   const int N = pl.N, H = N / 2 + 1, r0 = blockIdx.x * TL;
   // the block's rows are contiguous in memory: element t of the block is row t / N
   for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
@@ -925,7 +932,7 @@ __global__ void __launch_bounds__(FFT_T) k_fft_r2c(const T *in, typename CxT<T>:
     buf[t] = cpx<C>(in[r0 * N + t], 0.);
   }
   __syncthreads();
-  C *res = fft_lds<T>(buf, tmp, TL, pl);
+  C *res = fft_lds<T>(buf, tmp, TL, pl, stw);
   for (int t = threadIdx.x; t < TL * H; t += blockDim.x) {
     const int l = fd_div32(t, fH), e = t - l * H, R = r0 + l;
     if (R >= nrows) continue;
@@ -941,7 +948,10 @@ __global__ void __launch_bounds__(FFT_T) k_fft_c2r(const typename CxT<T>::t *in,
   typedef typename CxT<T>::t C;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   C *lds = reinterpret_cast<C *>(lds_raw);
-  C *buf = lds, *tmp = lds + (size_t)TL * pl.N;
+  C *buf = lds, *tmp = lds + (size_t)TL * pl.N, *stw = tmp + (size_t)TL * pl.N;
+  // the axis' twiddle table in LDS (read by every pass; the barrier after the load
+  // phase below makes it visible)
+  for (int m = threadIdx.x; m < pl.N; m += blockDim.x) stw[m] = pl.tw[m];
   const int N = pl.N, H = N / 2 + 1, r0 = blockIdx.x * TL;
   for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
     const int l = fd_div32(t, fN), e = t - l * N, R = r0 + l;
@@ -960,7 +970,7 @@ __global__ void __launch_bounds__(FFT_T) k_fft_c2r(const typename CxT<T>::t *in,
     buf[t] = v;
   }
   __syncthreads();
-  C *res = fft_lds<T>(buf, tmp, TL, pl);
+  C *res = fft_lds<T>(buf, tmp, TL, pl, stw);
   for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
     if (r0 + (int)fd_div32(t, fN) >= nrows) continue;
     out[r0 * N + t] = res[t].x;
@@ -1018,7 +1028,7 @@ static int fft_c2c(rbc3d_ctx *c, typename CxT<T>::t *data, int ax, int sign, int
     lm.nlines = H * N2 * nf;
   }
   int TL = lines_per_block<T>(pl.N);
-  size_t lds = 2 * sizeof(typename CxT<T>::t) * TL * pl.N;
+  size_t lds = sizeof(typename CxT<T>::t) * (2 * (size_t)TL * pl.N + pl.N);
   k_fft_c2c<T><<<(lm.nlines + TL - 1) / TL, FFT_T, lds, c->stream>>>(data, pl, lm, TL,
                                                                        make_fastdiv(TL));
   HIP_TRY(hipGetLastError());
@@ -1126,7 +1136,7 @@ static int pme_transform_t(rbc3d_ctx *c) {
   if (nf > 0) {
     FftPlan<T> px = make_plan<T>(c, 0, -1);
     int TL = lines_per_block<T>(N1), nrows = N2 * N3 * nf;
-    size_t lds = 2 * sizeof(CT) * TL * N1;
+    size_t lds = sizeof(CT) * (2 * (size_t)TL * N1 + N1);
     k_fft_r2c<T><<<(nrows + TL - 1) / TL, FFT_T, lds, st>>>((const T *)c->d_grid.p + ng * f0,
                                                             C + nc * f0, px, nrows, TL,
                                                             make_fastdiv(N1), make_fastdiv(H));
@@ -1156,7 +1166,7 @@ static int pme_transform_t(rbc3d_ctx *c) {
   RBC_TRY(fft_c2c<T>(c, C + nc * 12, 1, +1, 3));
   FftPlan<T> pxi = make_plan<T>(c, 0, +1);
   int TL = lines_per_block<T>(N1), nrows = N2 * N3 * 3;
-  size_t lds = 2 * sizeof(CT) * TL * N1;
+  size_t lds = sizeof(CT) * (2 * (size_t)TL * N1 + N1);
   k_fft_c2r<T><<<(nrows + TL - 1) / TL, FFT_T, lds, st>>>(C + nc * 12, (T *)c->d_vgrid.p, pxi,
                                                            nrows, TL, make_fastdiv(N1));
   rbc_stage_end(c, ST_FFT);
