@@ -1605,23 +1605,41 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   const double eps = 1.e-10;
   double h_phi = RBC_TWO_PI / nn, ih_phi = 1.0 / h_phi, cosr0 = cos(radPat);
   double s0 = 0, s1 = 0, s2 = 0;
+  // the longitude range of each latitude row inside the patch depends on the row only:
+  // lane i computes row i's (jmin, jmax) once (0 rows: outside the latitude band), and
+  // the node loop reads it from the wavefront's slot in LDS
+  __shared__ int rowj[256 / 64][64][2];
+  const int wsl = (int)(threadIdx.x >> 6);
+  if (lane < nl) {
+    const int i = lane;
+    int jmin = 1, jmax = 0;
+    if (!(th[i] <= th0 - radPat) && !(th[i] >= th0 + radPat)) {
+      double dth = th[i] - th0;
+      double dphi = 1. - (cos(dth) - cosr0) / ((sin(th[i]) + eps) * (sin(th0) + eps));
+      dphi = fmax(-1., fmin(1., dphi));
+      dphi = acos(dphi);
+      if (dphi > RBC_PI - eps) {
+        jmin = 1;
+        jmax = nn;
+      } else {
+        jmin = (int)ceil((phi0 - dphi - ph[0]) * ih_phi) + 1;
+        jmax = (int)floor((phi0 + dphi - ph[0]) * ih_phi) + 1;
+      }
+    } else {
+      jmin = 1;
+      jmax = -(1 << 29);  // no node of this row
+    }
+    rowj[wsl][i][0] = jmin;
+    rowj[wsl][i][1] = jmax;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
   // first row that fails "ths >= th0 + r0" ends the scan (ths ascending)
   for (int q = lane; q < nl * nn; q += 64) {
     int i = q % nl, jl = q / nl;
-    if (th[i] <= th0 - radPat) continue;
-    if (th[i] >= th0 + radPat) continue;
-    double dth = th[i] - th0;
-    double dphi = 1. - (cos(dth) - cosr0) / ((sin(th[i]) + eps) * (sin(th0) + eps));
-    dphi = fmax(-1., fmin(1., dphi));
-    dphi = acos(dphi);
-    int jmin, jmax;
-    if (dphi > RBC_PI - eps) {
-      jmin = 1;
-      jmax = nn;
-    } else {
-      jmin = (int)ceil((phi0 - dphi - ph[0]) * ih_phi) + 1;
-      jmax = (int)floor((phi0 + dphi - ph[0]) * ih_phi) + 1;
-    }
+    const int jmin = rowj[wsl][i][0], jmax = rowj[wsl][i][1];
+    if (jmax < -(1 << 28)) continue;  // outside the latitude band
     // multiplicity of node jl: #{j in [jmin,jmax] : (j-1) mod nn == jl}
     int lo = jmin - 1 - jl, hi = jmax - 1 - jl;  // need k*nn in [lo, hi]
     if (hi < lo) continue;
