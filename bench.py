@@ -10,7 +10,10 @@ function, gather), + g, SH analysis.
 Prints one JSON line.  `value` is cell-matvecs per second (cells / seconds per
 matvec).  `roofline` describes the dominant kernel measured with HIP events on the
 library's stream; `cpu_baseline` times the CPU restatement of the same algorithm
-(oracle/) on this host on a bounded sample.
+(oracle/) on this host on a bounded sample.  Beside it: `chosen` (the split-precision
+form of the same matvec and its error), `timestep` (the 200-cell tube timestep, GMRES
+seconds per timestep in the plain and chosen forms, with its own CPU baseline and the
+learned warm start's series) and, with --c5, the 4,096-cell mixed batch.
 """
 from __future__ import annotations
 
