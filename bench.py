@@ -89,7 +89,7 @@ def cpu_baseline(w, sample_every=32, threads=None):
 
 MLP_C3 = os.path.join(ROOT, "rbc3d_amd", "data", "mlp_C3.npz")
 # GMRES restart length of the chosen form: 150 and longer take the same iterations on
-# the tube (133 to 1e-8 against 911 at PETSc's default 30)
+# the tube (133 to 1e-8 against 910 at PETSc's default 30)
 CHOSEN_RESTART = 150
 
 
@@ -261,6 +261,32 @@ def learned_series_bench(mlp, seed=201, steps=4):
             "velocity_rel_err_vs_plain_max": max(r["vel_diff"] for r in recs)}
 
 
+def stage_rates(w, st, split):
+    """Rates of the matrix-core and bandwidth stages of one cell matvec from its stage
+    times (ms): the Legendre GEMMs of the two global SH transforms (synthesis of the
+    density, analysis of the velocity; 2 nlat nlat0 (nlat0 + 1) flop per cell and
+    component each) against the matrix cores' peak -- over the whole SH-transform
+    stage, which also holds the longitude DFTs and the packing, so a lower bound -- and
+    the PME stages' algorithmic bytes for the double layer (n_f = 9 spread fields:
+    8 (13 N_s + 7 N_t) + 8 Nb^3 (3 n_f + n_f + 3 + 6 + 3)) against HBM."""
+    cells = w.cells
+    nl = np.asarray(cells.nlat, np.float64)
+    nl0 = np.asarray(cells.nlat0, np.float64)
+    f_leg = 2 * 3 * float(np.sum(2 * nl * nl0 * (nl0 + 1)))
+    nb3 = float(np.prod(w.Nb))
+    ns = float(cells.npoint)
+    b_pme = 8 * (13 * ns + 7 * ns) + 8 * nb3 * (3 * 9 + 9 + 3 + 6 + 3)
+    leg_ms = st.get("sh_glob", 0.0)
+    pme_ms = sum(st.get(k, 0.0) for k in ("spread_prep", "spread", "fft", "green", "gather"))
+    return {"sh_legendre": {"flop": f_leg, "sh_stage_ms": round(leg_ms, 4),
+                            "tflops": round(f_leg / (leg_ms * 1e-3) / 1e12, 3) if leg_ms else None,
+                            # split: three BF16 products per FP32 product
+                            "peak_tflops": round(2500.0 / 3, 1) if split else PEAK_FP64_TFLOPS},
+            "pme": {"algorithmic_bytes": b_pme, "ms": round(pme_ms, 4),
+                    "gb_per_s": round(b_pme / (pme_ms * 1e-3) / 1e9, 1) if pme_ms else None,
+                    "hbm_peak_gb_per_s": PEAK_HBM_GBS}}
+
+
 def c5_bench(steps, warmup):
     """Cell-BIE matvec on the large mixed batch: 3,584 biconcave discs (nlat 13) and 512
     spheres of radius 1.4 (nlat 17) at 30% volume fraction, PME 256^3, P=6, viscosity
@@ -281,15 +307,6 @@ def c5_bench(steps, warmup):
     b = torch.empty(dof, dtype=torch.float64, device=dev)
     stream = torch.cuda.ExternalStream(g.stream())
     cells = w.cells
-    # algorithmic work: Legendre stage of the two global SH transforms (synthesis of the
-    # density, analysis of the velocity), 2 nlat nlat0 (nlat0 + 1) flop per cell and
-    # component each; PME bytes for the double layer (n_f = 9 spread fields)
-    nl = np.asarray(cells.nlat, np.float64)
-    nl0 = np.asarray(cells.nlat0, np.float64)
-    f_leg = 2 * 3 * float(np.sum(2 * nl * nl0 * (nl0 + 1)))
-    nb3 = float(np.prod(w.Nb))
-    ns = float(cells.npoint)
-    b_pme = 8 * (13 * ns + 7 * ns) + 8 * nb3 * (3 * 9 + 9 + 3 + 6 + 3)
 
     def run(prec):
         g.set_precision(prec)
@@ -315,23 +332,13 @@ def c5_bench(steps, warmup):
         g.timing(0)
         out = b.clone()
         g.set_precision(g.PREC_FP64)
-        leg_ms = st.get("sh_glob", 0.0)
-        pme_ms = sum(st.get(k, 0.0) for k in ("spread_prep", "spread", "fft", "green", "gather"))
         rec = {"ms_per_step": round(ms, 4), "cell_matvecs_per_s": round(cells.ncell / (ms * 1e-3), 3),
-               "stages_ms": {k: round(v, 4) for k, v in st.items()},
-               # Legendre flop over the whole SH-transform stage (it also holds the
-               # longitude DFTs and the packing), so the rate is a lower bound
-               "sh_legendre": {"flop": f_leg, "sh_stage_ms": round(leg_ms, 4),
-                               "tflops": round(f_leg / (leg_ms * 1e-3) / 1e12, 3) if leg_ms else None},
-               "pme": {"algorithmic_bytes": b_pme, "ms": round(pme_ms, 4),
-                       "gb_per_s": round(b_pme / (pme_ms * 1e-3) / 1e9, 1) if pme_ms else None,
-                       "hbm_peak_gb_per_s": 8000}}
+               "stages_ms": {k: round(v, 4) for k, v in st.items()}}
+        rec.update(stage_rates(w, st, prec == g.PREC_SPLIT))
         return rec, out
 
     plain, b64 = run(g.PREC_FP64)
     split, b32 = run(g.PREC_SPLIT)
-    plain["sh_legendre"]["peak_tflops"] = PEAK_FP64_TFLOPS  # FP64 MFMA
-    split["sh_legendre"]["peak_tflops"] = 2500.0 / 3  # three BF16 products per FP32 product
     split["rel_err_vs_plain"] = float((b32 - b64).abs().max() / b64.abs().max())
     npairs, _ = g.pair_stats()
     return {"workload": f"C5: {cells.ncell} cells (3584 discs nlat 13, 512 spheres R 1.4 "
@@ -479,6 +486,7 @@ def main():
         "dominant_stage": {"name": dom[0], "ms_per_step": round(dom[1][0] / args.steps, 4)},
         "stages_ms_per_step": {k: round(v[0] / args.steps, 4) for k, v in stages.items() if v[1]},
         "near_singular": {"projections": int(nproj), "corrections": int(nnear)},
+        "rates": stage_rates(w, {k: v[0] / args.steps for k, v in stages.items() if v[1]}, False),
         "chosen": {
             "what": "split form (FP32 pair sums with FP64 near pairs, FP32 PME, split-BF16 "
                     "MFMA Legendre GEMMs; singular quadrature FP64), same matvec",
@@ -486,6 +494,8 @@ def main():
             "cell_matvecs_per_s": round(ncell / (ms_split * 1e-3), 3),
             "geometry_reused_ms_per_step": round(ms_split_reuse, 4),
             "rel_err_vs_plain": split_err,
+            "rates": stage_rates(w, {k: v[0] / args.steps for k, v in stages_split.items() if v[1]},
+                                 True),
             "stages_ms_per_step": {k: round(v[0] / args.steps, 4)
                                    for k, v in stages_split.items() if v[1]}},
         "geometry_reused": {
