@@ -1243,9 +1243,10 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 // PolarPatch_Build for a single local point (common/ModPolarPatch.F90:99-148)
-__device__ __forceinline__ void patch_point(double s0, double c0, double phi0, double thL,
-                                            double phL, double &thG, double &phG) {
-  double st = sin(thL), ct = cos(thL), sp = sin(phL), cp = cos(phL);
+// with the local point's sines and cosines given (st, ct of thL; sp, cp of phL)
+__device__ __forceinline__ void patch_point_sc(double s0, double c0, double phi0, double st,
+                                               double ct, double sp, double cp, double &thG,
+                                               double &phG) {
   double x0 = st * cp, x1 = st * sp, x2 = ct;
   double y0 = c0 * x0 + 0. * x1 + s0 * x2;
   double y1 = 0. * x0 + 1. * x1 + 0. * x2;
@@ -1254,6 +1255,10 @@ __device__ __forceinline__ void patch_point(double s0, double c0, double phi0, d
   double ph = atan2(y1, y0) + phi0;
   thG = acos(y2);
   phG = ph - floor(ph * (1. / RBC_TWO_PI)) * RBC_TWO_PI;
+}
+__device__ __forceinline__ void patch_point(double s0, double c0, double phi0, double thL,
+                                            double phL, double &thG, double &phG) {
+  patch_point_sc(s0, c0, phi0, sin(thL), cos(thL), sin(phL), cos(phL), thG, phG);
 }
 
 // PolarPatch_Map (common/ModPolarPatch.F90:218-242)
@@ -1544,10 +1549,23 @@ __device__ void readd_wave(const SingArgs &a, const CellRec &cr, const double *b
   size_t vs = (size_t)m * n * 4;
   double s0 = sin(th0), c0 = cos(th0);
   double a0 = 0, a1 = 0, a2 = 0;
+  // sin and cos of the nazm azimuths, once per wavefront (lane j < nazm: azimuth j, the
+  // same expression as the point loop's), and of the lane's own radial angle
+  __shared__ double azsc[256 / 64][nazm][2];
+  const int wsl = (int)(threadIdx.x >> 6);
+  if (lane < nazm) {
+    const double phL = lane * RBC_TWO_PI / nazm;
+    azsc[wsl][lane][0] = sin(phL);
+    azsc[wsl][lane][1] = cos(phL);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  const double stl = sin(thl), ctl = cos(thl);
   for (int p = lane; p < nrad * nazm; p += 64) {
     int iazm = p / nrad;
-    double phL = iazm * RBC_TWO_PI / nazm, thj, phj;
-    patch_point(s0, c0, phi0, thl, phL, thj, phj);
+    double thj, phj;
+    patch_point_sc(s0, c0, phi0, stl, ctl, azsc[wsl][iazm][0], azsc[wsl][iazm][1], thj, phj);
     double xj[3], xx[3];
     spline_interp(blk + vs * SPL_X, m, n, thj, phj, 3, xj);
     for (int d = 0; d < 3; d++) xx[d] = xj[d] - xi[d];
