@@ -508,6 +508,14 @@ def main():
     }
     if not args.no_timestep:
         out["timestep"] = timestep_bench(args.timestep_steps, 1)
+        # the metric's second half: GMRES seconds per tube timestep, chosen vs plain
+        ts = out["timestep"]
+        ch, pl = ts.get("chosen", {}), ts.get("plain", {})
+        out["gmres_s_per_timestep"] = {
+            "plain_fp64_gmres30": pl.get("s_per_timestep"),
+            "chosen_split_gmres150": ch.get("s_per_timestep"),
+            "speedup": ch.get("speedup_vs_plain"),
+            "velocity_rel_err": ch.get("velocity_rel_err_vs_plain")}
     if args.c5:
         out["C5"] = c5_bench(3, 1)
     if not args.no_cpu:
