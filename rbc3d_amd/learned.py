@@ -1,14 +1,14 @@
 """The learned warm start of NoSlipWall (common/ModNoSlip.F90:45-150): training data
 from the plain solver on tube time series, training of the per-wall-vertex network
 with the library's own BF16 kernels (rbc3d_mlp_*), and its evaluation against plain
-GMRES (the fallback and the judge).
+GMRES (which is both the fallback and the measure of success).
 
 One step of a series, at viscosity ratio 1 (the tube configuration): Solve_RBC_Vel
 gives the cells' velocity; the network's inputs are read at the wall vertices (the
 cells' velocity there, the current traction f0, the vertex normal); NoSlipWall with
 the plain solver gives the converged traction f, and f - f0 is the training target;
 then every cell moves rigidly with the mean velocity of its nodes for dt.  Membrane
-mechanics are outside this build, so cells keep their shape and force density: the
+mechanics are not part of this library, so cells keep their shape and force density: the
 series exercises how the wall traction follows cells that move along the tube.
 
 The network itself (9 -> 64 -> 64 -> 64 -> 3, GELU, BF16 weights, FP32

@@ -1,7 +1,7 @@
 """The learned NoSlipWall warm start on the GPU: the network's BF16 forward and
 backward kernels against the CPU oracle (oracle/mlp_oracle.py, rounding at the same
 BF16 points, FP64 accumulation), an Adam step, training on the device, and the
-warm-started NoSlipWall with plain GMRES as fallback and judge."""
+warm-started NoSlipWall with plain GMRES as the fallback and the measure of success."""
 import numpy as np
 import pytest
 
@@ -128,7 +128,7 @@ def tube():
     return w, g
 
 
-def test_learned_noslip_fallback_and_judge(tube):
+def test_learned_noslip_fallback(tube):
     """A network whose prediction is garbage (outputs of order 1e32) cannot give a
     useful initial guess: the warm-started solve stalls, plain GMRES from zero takes
     over, and the result equals the plain solve's."""
