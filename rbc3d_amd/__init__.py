@@ -109,6 +109,13 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError(f"{LIB_PATH} is missing: build it with rbc3d_amd/build.py "
                               "(there is no CPU fallback)")
+        # PyTorch carries its own HIP runtime; loading it first lets this library bind to
+        # the same one (the other order leaves torch without a usable device in this
+        # process).  The library itself does not need torch.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, args in _SIG.items():
             fn = getattr(L, name)
