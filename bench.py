@@ -25,6 +25,11 @@ import time
 
 import numpy as np
 
+try:  # before the library, so both share torch's HIP runtime
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 

@@ -968,6 +968,13 @@ int orc_set_cells(orc_sys *s, int nrbc, const int *nlat0, const int *nlat, const
                   const double *x, const double *a3, const double *detj) {
   free_cells(s);
   s->nrbc = nrbc;
+  /* surface ids: cells 1..nrbc, then the wall (GlobData_Init common/ModData.F90:40-49),
+   * whichever was set first */
+  if (s->nwall) {
+    orc_wall *w = &s->wall;
+    w->id = nrbc + 1;
+    for (int e = 0; e < s->slw.nPoint; e++) s->slw.indx[e] = w->id;
+  }
   s->rbcs = calloc(nrbc, sizeof(orc_rbc));
   size_t off = 0;
   int np = 0;

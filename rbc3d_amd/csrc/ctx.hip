@@ -583,6 +583,9 @@ extern "C" int rbc3d_set_cells(rbc3d_ctx *c, int nrbc, const int *nlat0, const i
                                const double *detj) {
   if (!c || nrbc < 0 || (nrbc > 0 && (!nlat0 || !nlat || !celltype || !x || !a3 || !detj)))
     return RBC3D_EINVAL;
+  int prev_dof = 0;
+  for (const CellRec &r : c->cells) prev_dof += 3 * r.nlat0 * r.nlat0;
+  const int prev_nrbc = c->nrbc;
   c->nrbc = nrbc;
   c->cells.resize(nrbc);
   int np = 0, coff = 0, choff = 0;
@@ -624,6 +627,12 @@ extern "C" int rbc3d_set_cells(rbc3d_ctx *c, int nrbc, const int *nlat0, const i
   c->np = np;
   c->nchunks = choff;
   c->geom_epoch++;
+  // the previous cell-BIE solution is the next solve's initial guess only for the same
+  // set of unknowns
+  if (nrbc != prev_nrbc || coff != prev_dof) c->cell_sol_init = false;
+  // surface ids follow the cells: the wall is nrbc + 1 (GlobData_Init,
+  // common/ModData.F90:40-49), whichever of cells and wall was set first
+  c->wall.id = nrbc + 1;
   c->spl_total = spl;
   hipStream_t st = c->stream;
   RBC_TRY(upload(c->d_cells, c->cells, st));

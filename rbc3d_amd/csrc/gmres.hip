@@ -136,7 +136,6 @@ int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, doubl
   GmresWork &ws = c->gmres;
   size_t ld = (size_t)n + 64;
   RBC_TRY(ws.V.ensure(ld * (restart + 1)));
-  RBC_TRY(ws.w.ensure(n + 1));
   RBC_TRY(ws.hd.ensure(2 * G_MAXV + 2));
   if (!ws.hpin) HIP_TRY(hipHostMalloc((void **)&ws.hpin, sizeof(double) * 3 * (G_MAXV + 2)));
   for (auto &e : ws.ev)
@@ -174,6 +173,10 @@ int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, doubl
       if (history) history->push_back(res);
     }
     first = false;
+    if (!std::isfinite(res)) {  // KSP_DIVERGED_NANORINF
+      *reason = -4;
+      break;
+    }
     if (res == 0.0 || res <= ttol) {
       *reason = 1;
       break;
@@ -210,6 +213,14 @@ int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, doubl
       // happy breakdown bound as in PETSc: haptol * |g(it)|
       const double hapbnd = fabs(g[it]) * 1.0e-30;
       const double tt = sqrt(col[it + 1]);
+      // a non-finite Hessenberg column ends the solve before it reaches x
+      // (KSP_DIVERGED_NANORINF): x keeps the last finite update
+      bool finite = std::isfinite(tt);
+      for (int i = 0; i <= it; i++) finite = finite && std::isfinite(col[i]);
+      if (!finite) {
+        *reason = -4;
+        break;
+      }
       for (int i = 0; i <= it; i++) H[(size_t)i * restart + it] = col[i];
       H[(size_t)(it + 1) * restart + it] = tt;
       if (!(tt > hapbnd)) happy = true;
@@ -234,6 +245,10 @@ int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, doubl
         g[it + 1] = 0.;
       }
       res = fabs(g[it + 1]);
+      if (!std::isfinite(res)) {
+        *reason = -4;
+        break;
+      }
       it++;
       its++;
       if (history) history->push_back(res);
@@ -250,6 +265,7 @@ int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, doubl
         break;
       }
     }
+    if (*reason == -4) break;
     // x += V y with H y = g (upper triangular); y goes through the pinned buffer
     double *y = ws.hpin + 2 * (size_t)(G_MAXV + 2);
     for (int i = it - 1; i >= 0; i--) {
@@ -270,10 +286,7 @@ int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, doubl
   *niter = its;
   *rnorm = res;
   if (*reason == 0) *reason = -3;  // KSP_DIVERGED_ITS
-  ws.V.release();
-  ws.w.release();
-  ws.part.release();
-  ws.hd.release();
+  // the Krylov basis and the scratch stay allocated for the next solve
   return RBC3D_OK;
 }
 
@@ -308,6 +321,21 @@ __global__ void k_absmax(int n, const double *__restrict__ v, double *out) {
   if (threadIdx.x == 0) *out = red[0];
 }
 
+// Status of a solve from the GMRES stop reason: the outputs are written either way
+// (the reference prints the residual and goes on, common/ModNoSlip.F90:120-126); a
+// caller that keeps the reference's behaviour ignores RBC3D_ENOCONV.
+static int gmres_status(int reason, const char *who) {
+  if (reason > 0) return RBC3D_OK;
+  if (reason == -4) {
+    rbc_set_error(std::string(who) + ": non-finite residual in GMRES (the input or the operator "
+                  "holds NaN or Inf)");
+    return RBC3D_ENONFINITE;
+  }
+  rbc_set_error(std::string(who) + (reason == -1 ? ": GMRES diverged"
+                                                 : ": GMRES reached maxit before the tolerance"));
+  return RBC3D_ENOCONV;
+}
+
 // NoSlipWall (common/ModNoSlip.F90:45-150)
 extern "C" int rbc3d_noslip_wall(rbc3d_ctx *c, double rtol, int maxit, const double *finc0,
                                  int *niter, double *rnorm, int *converged, double *vres_max) {
@@ -316,7 +344,7 @@ extern "C" int rbc3d_noslip_wall(rbc3d_ctx *c, double rtol, int maxit, const dou
   if (!w.nvert) return RBC3D_ESTATE;
   hipStream_t st = c->stream;
   int nv = w.nvert, nd = w.ndof, nu = nd / 3;
-  DBuf<double> f0, rhs, x, tmpv;
+  DBuf<double> &f0 = c->gmres.f0, &rhs = c->gmres.rhs, &x = c->gmres.x, &tmpv = c->gmres.tmp;
   RBC_TRY(f0.ensure((size_t)3 * nv));
   RBC_TRY(rhs.ensure(nd + 1));
   RBC_TRY(x.ensure(nd + 1));
@@ -343,11 +371,7 @@ extern "C" int rbc3d_noslip_wall(rbc3d_ctx *c, double rtol, int maxit, const dou
     HIP_TRY(hipMemcpyAsync(vres_max, tmpv.p, sizeof(double), hipMemcpyDeviceToHost, st));
   }
   HIP_TRY(hipStreamSynchronize(st));
-  f0.release();
-  rhs.release();
-  x.release();
-  tmpv.release();
-  return RBC3D_OK;
+  return gmres_status(reason, "rbc3d_noslip_wall");
 }
 
 // ---------------------------------------------------------------------------
@@ -361,6 +385,7 @@ extern "C" int rbc3d_noslip_wall(rbc3d_ctx *c, double rtol, int maxit, const dou
 // residual at full strength.  Plain GMRES from zero is the fallback when that
 // residual is not below |b| or the warm-started solve does not converge.
 int rbc_mlp_forward(rbc3d_mlp *m, int n, const float *d_x, float *d_y, hipStream_t st);
+int rbc_mlp_wait(rbc3d_mlp *m, hipStream_t st);
 int rbc_wall_cell_vel_dev(rbc3d_ctx *c, double *d_out);
 int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double c1, int nT, const double *d_tx, const int *d_tindx,
                              const int *d_act, const double *d_A, double *d_v, bool wall_targets);
@@ -460,8 +485,9 @@ extern "C" int rbc3d_noslip_wall_learned(rbc3d_ctx *c, rbc3d_mlp *m, double rtol
   if (!w.prepared) RBC_TRY(rbc_wall_prepare(c));
   hipStream_t st = c->stream;
   const int nv = w.nvert, nd = w.ndof, nu = nd / 3;
-  DBuf<double> f0, uc, rhs, x, r0, LV;
-  DBuf<float> X, Y;
+  GmresWork &gw = c->gmres;
+  DBuf<double> &f0 = gw.f0, &uc = gw.uc, &rhs = gw.rhs, &x = gw.x, &r0 = gw.r0, &LV = gw.LV;
+  DBuf<float> &X = gw.X, &Y = gw.Y;
   RBC_TRY(f0.ensure((size_t)3 * nv));
   RBC_TRY(uc.ensure((size_t)3 * nv));
   RBC_TRY(rhs.ensure(nd + 1));
@@ -486,6 +512,9 @@ extern "C" int rbc3d_noslip_wall_learned(rbc3d_ctx *c, rbc3d_mlp *m, double rtol
   // the network's prediction of the increment
   k_mlp_features<<<(nv + 255) / 256, 256, 0, st>>>(nv, uc.p, f0.p, w.d_vnorm.p, X.p);
   HIP_TRY(hipGetLastError());
+  // the network's parameters may still be written by work queued on its own stream
+  // (an Adam step): order the forward after it
+  RBC_TRY(rbc_mlp_wait(m, st));
   RBC_TRY(rbc_mlp_forward(m, nv, X.p, Y.p, st));
   k_mlp_x0<<<(nu + 255) / 256, 256, 0, st>>>(nu, w.d_glb.p + nv, Y.p, x.p);
   HIP_TRY(hipGetLastError());
@@ -545,15 +574,14 @@ extern "C" int rbc3d_noslip_wall_learned(rbc3d_ctx *c, rbc3d_mlp *m, double rtol
   k_add_assembled<<<(nv + 255) / 256, 256, 0, st>>>(nv, w.d_glb.p, f0.p, x.p, w.d_f.p);
   HIP_TRY(hipGetLastError());
   if (vres_max) {
-    DBuf<double> tmpv;
+    DBuf<double> &tmpv = gw.tmp;
     RBC_TRY(rbc_wall_residual_dev(c));
     RBC_TRY(tmpv.ensure(1));
     k_absmax<<<1, 256, 0, st>>>(3 * nv, c->d_v.p, tmpv.p);
     HIP_TRY(hipMemcpyAsync(vres_max, tmpv.p, sizeof(double), hipMemcpyDeviceToHost, st));
-    HIP_TRY(hipStreamSynchronize(st));
   }
   HIP_TRY(hipStreamSynchronize(st));
-  return RBC3D_OK;
+  return gmres_status(reason, "rbc3d_noslip_wall_learned");
 }
 
 extern "C" int rbc3d_gmres_history(rbc3d_ctx *c, double *h, int maxn, int *n) {
@@ -584,8 +612,9 @@ extern "C" int rbc3d_solve_rbc_vel(rbc3d_ctx *c, double *v, int *niter, double *
   hipStream_t st = c->stream;
   int dof;
   RBC_TRY(rbc3d_cell_dof(c, &dof));
-  DBuf<double> rhs;
+  DBuf<double> &rhs = c->gmres.rhs;
   RBC_TRY(rhs.ensure(dof + 1));
+  if (c->d_cell_sol.n < (size_t)dof + 1) c->cell_sol_init = false;  // a new buffer
   RBC_TRY(c->d_cell_sol.ensure(dof + 1));
   if (!c->cell_sol_init) {
     HIP_TRY(hipMemsetAsync(c->d_cell_sol.p, 0, sizeof(double) * dof, st));
@@ -596,10 +625,10 @@ extern "C" int rbc3d_solve_rbc_vel(rbc3d_ctx *c, double *v, int *niter, double *
   for (int l = 0; l < c->conf.ntypes; l++) all_one = all_one && c->conf.viscRat[l] == 1.0;
   *niter = 0;
   *rnorm = 0;
+  int reason = 1;
   if (all_one) {
     HIP_TRY(hipMemcpyAsync(c->d_cell_sol.p, rhs.p, sizeof(double) * dof, hipMemcpyDeviceToDevice, st));
   } else {
-    int reason = 0;
     auto mv = [c](const double *u, double *b) { return rbc3d_cell_matmult_dev(c, u, b); };
     RBC_TRY(rbc_gmres(c, dof, mv, rhs.p, c->d_cell_sol.p, true, 1e-11, 1e-50, 1e5, 200, c->gmres_restart, niter,
                       rnorm, &reason, nullptr));
@@ -609,6 +638,5 @@ extern "C" int rbc3d_solve_rbc_vel(rbc3d_ctx *c, double *v, int *niter, double *
   RBC_TRY(rbc_set_g_nodes(c, c->d_shbuf.p));
   if (v) HIP_TRY(hipMemcpyAsync(v, c->d_shbuf.p, sizeof(double) * 3 * c->np, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
-  rhs.release();
-  return RBC3D_OK;
+  return gmres_status(reason, "rbc3d_solve_rbc_vel");
 }

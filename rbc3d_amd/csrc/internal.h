@@ -89,6 +89,10 @@ struct GridType {
 // GMRES work space (Krylov basis, partial dot products), kept across solves
 struct GmresWork {
   DBuf<double> V, w, part, hd;
+  // vectors of the solves around the GMRES (previous traction, right-hand side,
+  // solution, the learned warm start's inputs and residual images), kept across calls
+  DBuf<double> f0, rhs, x, r0, uc, LV, tmp;
+  DBuf<float> X, Y;
   std::vector<double> history;  // residual estimates of the last solve
   // pinned host slots for the Hessenberg columns of two Arnoldi steps in flight and
   // the update coefficients, with an event per column slot (created on first use)

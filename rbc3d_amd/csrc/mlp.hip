@@ -56,6 +56,7 @@ struct rbc3d_mlp {
   DBuf<float> p, m1, m2, grad, part, norm;
   DBuf<unsigned short> wb;
   DBuf<double> lpart;
+  hipEvent_t ev = nullptr;  // marks the last work queued on st() (rbc_mlp_wait)
   int t = 0;
 };
 
@@ -437,6 +438,7 @@ extern "C" int rbc3d_mlp_create(rbc3d_mlp **out) {
 extern "C" int rbc3d_mlp_destroy(rbc3d_mlp *m) {
   if (!m) return RBC3D_OK;
   if (m->stream) (void)hipStreamSynchronize(m->stream);
+  if (m->ev) (void)hipEventDestroy(m->ev);
   if (m->stream) (void)hipStreamDestroy(m->stream);
   delete m;
   return RBC3D_OK;
@@ -553,6 +555,18 @@ extern "C" int rbc3d_mlp_adam_step_dev(rbc3d_mlp *m, int n, const float *d_x, co
   k_mlp_adam<<<(MP_N + 255) / 256, 256, 0, m->st()>>>(m->p.p, m->m1.p, m->m2.p, m->grad.p, lr, b1,
                                                         b2, 1e-8f, c1, c2, m->wb.p);
   HIP_TRY(hipGetLastError());
-  if (!loss) HIP_TRY(hipStreamSynchronize(m->st()));
+  // the update is complete when the call returns, so a caller may read or use the
+  // parameters from any stream
+  HIP_TRY(hipStreamSynchronize(m->st()));
+  return RBC3D_OK;
+}
+
+// Make stream st wait for the work queued on the network's stream (an event, no host
+// wait): a forward on another stream then reads the parameters after every update.
+int rbc_mlp_wait(rbc3d_mlp *m, hipStream_t st) {
+  if (m->st() == st) return RBC3D_OK;
+  if (!m->ev) HIP_TRY(hipEventCreateWithFlags(&m->ev, hipEventDisableTiming));
+  HIP_TRY(hipEventRecord(m->ev, m->st()));
+  HIP_TRY(hipStreamWaitEvent(st, m->ev, 0));
   return RBC3D_OK;
 }

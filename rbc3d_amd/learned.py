@@ -22,6 +22,11 @@ from dataclasses import replace
 
 import numpy as np
 
+try:  # before the library, so both share torch's HIP runtime
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 
 def cell_offsets(cells):
     n = (cells.nlat * 2 * cells.nlat).astype(np.int64)
