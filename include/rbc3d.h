@@ -77,6 +77,22 @@ int rbc3d_wall_self_nnz(rbc3d_ctx *ctx, long long *nnz);
 int rbc3d_hashtable_build(const int Nc[3], const double iLbNc[3], int nPoint, const double *x,
                           int ldx, int *hoc, int *next);
 
+/* No counterpart in the reference: rbc3d_hashtable_build for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same result, bit for bit, on device pointers: d_x(ldx,3), d_hoc
+ * ((Nc1+2)(Nc2+2)(Nc3+2)) and d_next(nPoint) in HBM; queued asynchronously on `stream`
+ * (a hipStream_t; NULL = the null stream), results valid once the stream reaches them.
+ * Working storage is one device array of RBC3D_NOALLOC_SCRATCH_BYTES per GPU, held until
+ * the process ends: nPoint <= RBC3D_NOALLOC_MAX_POINTS and Nc1*Nc2*Nc3 <=
+ * RBC3D_NOALLOC_MAX_BINS, else RBC3D_EINVAL.  Every call on one GPU shares that array,
+ * so calls that overlap on one GPU (two streams, two threads) race. */
+#define RBC3D_NOALLOC_MAX_POINTS (1 << 20)
+#define RBC3D_NOALLOC_MAX_BINS (1 << 18)
+#define RBC3D_NOALLOC_SCRATCH_BYTES (40u << 20)
+int rbc3d_hashtable_build_noalloc(const int Nc[3], const double iLbNc[3], int nPoint,
+                                  const double *d_x, int ldx, int *d_hoc, int *d_next,
+                                  void *stream);
+
 /* --- hot path: real space ------------------------------------------------- */
 /* AddIntOnRbcs(c1,c2,tlist,v) (common/ModIntOnRbcs.F90:25-158): pair sums over the
  * cell list, same-surface masking, self-patch singular quadrature, near-singular

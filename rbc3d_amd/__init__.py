@@ -46,6 +46,8 @@ _SIG = {
     "rbc3d_prepare_sing_int_on_wall": [C.c_void_p],
     "rbc3d_wall_self_nnz": [C.c_void_p, C.POINTER(C.c_longlong)],
     "rbc3d_hashtable_build": [ip, dp, C.c_int, dp, C.c_int, ip, ip],
+    "rbc3d_hashtable_build_noalloc": [ip, dp, C.c_int, C.c_void_p, C.c_int, C.c_void_p,
+                                      C.c_void_p, C.c_void_p],
     "rbc3d_add_int_on_rbcs": [C.c_void_p, C.c_double, C.c_double, C.c_int, dp, ip, ip, dp, dp],
     "rbc3d_add_int_on_walls": [C.c_void_p, C.c_double, C.c_int, dp, ip, ip, dp, dp],
     "rbc3d_pair_stats": [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_ulonglong)],
@@ -169,6 +171,14 @@ def hashtable_build(Nc, iLbNc, x, nPoint=None):
     nxt = np.zeros(max(n, 1), np.int32)
     _call("rbc3d_hashtable_build", Nc, _f64(iLbNc), n, x, ldx, hoc, nxt)
     return hoc, nxt[:n]
+
+
+def hashtable_build_noalloc(Nc, iLbNc, nPoint, d_x, ldx, d_hoc, d_next, stream=0):
+    """HashTable_Build on device pointers without allocating (rbc3d_hashtable_build_noalloc):
+    d_x (ldx, 3) FP64, d_hoc ((Nc1+2)(Nc2+2)(Nc3+2)) and d_next (nPoint) int32, all raw
+    device addresses; queued on ``stream`` (a hipStream_t handle, 0 = null stream)."""
+    _call("rbc3d_hashtable_build_noalloc", _i32(Nc), _f64(iLbNc), int(nPoint), d_x, int(ldx),
+          d_hoc, d_next, stream)
 
 
 def sh_anal_gau(nlat, nlon, g, nt=1, ida=None, jda=None):

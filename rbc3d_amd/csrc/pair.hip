@@ -16,6 +16,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 
 #include "internal.h"
 
@@ -61,28 +62,59 @@ __global__ void k_bin_start(int n, const int *__restrict__ skey, int nbins, int 
   bin_start[b] = lo;
 }
 
-// Sort n points by bin (stable: ascending index inside a bin).  d_bin receives the
-// bin of each point in ORIGINAL order, d_order the sorted point indices.
+static int radix_bits(int nbins) {
+  int bits = 1;
+  while ((1 << bits) < nbins) bits++;
+  return bits;
+}
+
+// Bytes of radix-sort working storage for n (bin, index) pairs (a host-side query).
+static int hash_sort_tmp_bytes(int n, int nbins, hipStream_t st, size_t *bytes) {
+  *bytes = 0;
+  if (n <= 0) return RBC3D_OK;
+  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, *bytes, (const int *)nullptr, (int *)nullptr,
+                                             (const int *)nullptr, (int *)nullptr, n, 0,
+                                             radix_bits(nbins), st));
+  return RBC3D_OK;
+}
+
+// Sort n points by bin (stable: ascending index inside a bin) into caller-provided
+// buffers, asynchronously on st: d_bin receives the bin of each point in ORIGINAL order,
+// d_order the sorted point indices, d_bin_start[0..nbins] the first sorted position of
+// each bin.  keys_out and idx_in hold n ints each; tmp holds tmp_bytes (at least
+// hash_sort_tmp_bytes).  Allocates nothing.
+static int hash_sort_bufs(hipStream_t st, const HashGeom &hg, int nbins, int n, const double *d_x,
+                          int ldx, int *d_bin, int *d_order, int *d_bin_start, int *keys_out,
+                          int *idx_in, void *tmp, size_t tmp_bytes) {
+  if (n > 0) {
+    k_bins<<<(n + 255) / 256, 256, 0, st>>>(n, d_x, ldx, hg, d_bin, idx_in);
+    HIP_TRY(hipGetLastError());
+    size_t need = 0;
+    RBC_TRY(hash_sort_tmp_bytes(n, nbins, st, &need));
+    if (need > tmp_bytes) {
+      rbc_set_error("cell-list sort: working storage too small");
+      return RBC3D_EINVAL;
+    }
+    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, need, d_bin, keys_out, idx_in, d_order, n, 0,
+                                               radix_bits(nbins), st));
+  }
+  k_bin_start<<<(nbins + 1 + 255) / 256, 256, 0, st>>>(n, keys_out, nbins, d_bin_start);
+  HIP_TRY(hipGetLastError());
+  return RBC3D_OK;
+}
+
+// As hash_sort_bufs with the working buffers allocated here, then synchronised.
 static int hash_sort_grid(rbc3d_ctx *c, const HashGeom &hg, int nbins, int n, const double *d_x,
                           int ldx, int *d_bin, int *d_order, int *d_bin_start) {
   hipStream_t st = c->stream;
   DBuf<int> keys_out, idx_in;
   RBC_TRY(keys_out.ensure(n + 1));
   RBC_TRY(idx_in.ensure(n + 1));
-  if (n > 0) {
-    k_bins<<<(n + 255) / 256, 256, 0, st>>>(n, d_x, ldx, hg, d_bin, idx_in.p);
-    HIP_TRY(hipGetLastError());
-    int bits = 1;
-    while ((1 << bits) < nbins) bits++;
-    size_t tmp = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, d_bin, keys_out.p, idx_in.p, d_order, n,
-                                               0, bits, st));
-    RBC_TRY(c->d_cub_tmp.ensure(tmp + 16));
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->d_cub_tmp.p, tmp, d_bin, keys_out.p, idx_in.p,
-                                               d_order, n, 0, bits, st));
-  }
-  k_bin_start<<<(nbins + 1 + 255) / 256, 256, 0, st>>>(n, keys_out.p, nbins, d_bin_start);
-  HIP_TRY(hipGetLastError());
+  size_t tmp = 0;
+  RBC_TRY(hash_sort_tmp_bytes(n, nbins, st, &tmp));
+  RBC_TRY(c->d_cub_tmp.ensure(tmp + 16));
+  RBC_TRY(hash_sort_bufs(st, hg, nbins, n, d_x, ldx, d_bin, d_order, d_bin_start, keys_out.p,
+                         idx_in.p, c->d_cub_tmp.p, tmp + 16));
   HIP_TRY(hipStreamSynchronize(st));
   keys_out.release();
   idx_in.release();
@@ -198,6 +230,98 @@ extern "C" int rbc3d_hashtable_build(const int Nc[3], const double iLbNc[3], int
   (void)hipStreamDestroy(tmp.stream);
   tmp.stream = nullptr;
   return r;
+}
+
+// ---------------------------------------------------------------------------
+// HashTable_Build without allocation: every working buffer is carved from `scratch`, a
+// file-scope device array that each GPU holds for the life of the process.  Host code
+// never passes the array's name to a kernel (on the host it is only a stand-in symbol):
+// its device address is looked up once per device with hipGetSymbolAddress and kept.
+
+static __device__ __align__(256) unsigned char scratch[RBC3D_NOALLOC_SCRATCH_BYTES];
+
+static std::mutex g_scratch_mu;
+static unsigned char *g_scratch_ptr[64];
+
+// The calling thread's current device's copy of `scratch`.
+static int scratch_address(unsigned char **out) {
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) {
+    rbc_set_error("no-allocation scratch: device index out of range");
+    return RBC3D_EINVAL;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    HIP_TRY(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) {
+      rbc_set_error("no-allocation scratch: no device address");
+      return RBC3D_EHIP;
+    }
+    g_scratch_ptr[dev] = static_cast<unsigned char *>(p);
+  }
+  *out = g_scratch_ptr[dev];
+  return RBC3D_OK;
+}
+
+static size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+extern "C" int rbc3d_hashtable_build_noalloc(const int Nc[3], const double iLbNc[3], int nPoint,
+                                             const double *d_x, int ldx, int *d_hoc, int *d_next,
+                                             void *stream) {
+  if (!Nc || !iLbNc || nPoint < 0 || (nPoint > 0 && (!d_x || ldx < nPoint || !d_next)) || !d_hoc) {
+    rbc_set_error("rbc3d_hashtable_build_noalloc: invalid arguments");
+    return RBC3D_EINVAL;
+  }
+  for (int d = 0; d < 3; d++)
+    if (Nc[d] < 1) {
+      rbc_set_error("rbc3d_hashtable_build_noalloc: Nc must be >= 1");
+      return RBC3D_EINVAL;
+    }
+  long long nbins_ll = (long long)Nc[0] * Nc[1] * Nc[2];
+  if (nPoint > RBC3D_NOALLOC_MAX_POINTS || nbins_ll > RBC3D_NOALLOC_MAX_BINS) {
+    rbc_set_error("rbc3d_hashtable_build_noalloc: more than " +
+                  std::to_string(RBC3D_NOALLOC_MAX_POINTS) + " points or " +
+                  std::to_string(RBC3D_NOALLOC_MAX_BINS) + " bins (the scratch maximum)");
+    return RBC3D_EINVAL;
+  }
+  int nbins = (int)nbins_ll;
+  hipStream_t st = static_cast<hipStream_t>(stream);
+  unsigned char *base = nullptr;
+  RBC_TRY(scratch_address(&base));
+  // carve: bin, order, keys_out, idx_in (n each), bin_start (nbins + 1), sort storage
+  size_t nb = align256(sizeof(int) * ((size_t)nPoint + 1));
+  size_t off = 0;
+  int *bin = reinterpret_cast<int *>(base + off);
+  off += nb;
+  int *order = reinterpret_cast<int *>(base + off);
+  off += nb;
+  int *keys_out = reinterpret_cast<int *>(base + off);
+  off += nb;
+  int *idx_in = reinterpret_cast<int *>(base + off);
+  off += nb;
+  int *bin_start = reinterpret_cast<int *>(base + off);
+  off += align256(sizeof(int) * ((size_t)nbins + 1));
+  size_t need = 0;
+  RBC_TRY(hash_sort_tmp_bytes(nPoint, nbins, st, &need));
+  if (off + need > (size_t)RBC3D_NOALLOC_SCRATCH_BYTES) {
+    rbc_set_error("rbc3d_hashtable_build_noalloc: scratch too small for this size");
+    return RBC3D_EINVAL;
+  }
+  HashGeom hg;
+  for (int d = 0; d < 3; d++) {
+    hg.Nc[d] = Nc[d];
+    hg.iLbNc[d] = iLbNc[d];
+  }
+  RBC_TRY(hash_sort_bufs(st, hg, nbins, nPoint, d_x, ldx, bin, order, bin_start, keys_out, idx_in,
+                         base + off, (size_t)RBC3D_NOALLOC_SCRATCH_BYTES - off));
+  int nh = (Nc[0] + 2) * (Nc[1] + 2) * (Nc[2] + 2);
+  k_fill_i<<<(nh + 255) / 256, 256, 0, st>>>(d_hoc, nh, -1);
+  HIP_TRY(hipGetLastError());
+  k_hoc_next<<<(nbins + 255) / 256, 256, 0, st>>>(nbins, hg, bin_start, order, d_hoc, d_next);
+  HIP_TRY(hipGetLastError());
+  return RBC3D_OK;
 }
 
 // ---------------------------------------------------------------------------
