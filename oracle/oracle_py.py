@@ -92,6 +92,13 @@ def lib():
         "orc_get_wall_density": (C.c_int, [vp, dp]),
         "orc_solve_rbc_vel": (C.c_int, [vp, dp, dp, C.POINTER(C.c_int), C.POINTER(C.c_double)]),
         "orc_near_list": (C.c_int, [vp, C.c_int, dp, ip, C.c_int, ip, ip, ip, dp, C.c_int]),
+        "orc_set_terms": (C.c_int, [vp, C.c_int]),
+        "orc_spline_eval": (C.c_int, [vp, C.c_int, C.c_int, C.c_double, C.c_double, dp]),
+        "orc_find_projection": (C.c_int, [vp, C.c_int, dp, C.POINTER(C.c_double),
+                                          C.POINTER(C.c_double), dp]),
+        "orc_spline_interp_raw": (None, [C.c_int, C.c_int, C.c_int, dp, dp, dp, dp, C.c_double,
+                                         C.c_double, dp]),
+        "orc_cell_mesh_size": (C.c_double, [vp, C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -308,6 +315,27 @@ class Oracle:
 
     def pair_checksum(self):
         return L().orc_pair_checksum(self.h)
+
+    TERM_PAIR, TERM_SELF, TERM_NEAR, TERM_LINEAR, TERM_ALL = 1, 2, 4, 8, 15
+
+    def set_terms(self, terms):
+        """Which parts of AddIntOnRbcs run (a test hook; all by default)."""
+        assert L().orc_set_terms(self.h, terms) == 0
+
+    def spline_eval(self, irbc, which, th, phi, nvar):
+        out = np.zeros(nvar)
+        assert L().orc_spline_eval(self.h, irbc, which, th, phi, out) == 0
+        return out
+
+    def find_projection(self, irbc, xtar, th0, phi0):
+        t, p = C.c_double(th0), C.c_double(phi0)
+        x0 = np.zeros(3)
+        assert L().orc_find_projection(self.h, irbc, np.ascontiguousarray(xtar, np.float64),
+                                       C.byref(t), C.byref(p), x0) == 0
+        return t.value, p.value, x0
+
+    def mesh_size(self, irbc):
+        return L().orc_cell_mesh_size(self.h, irbc)
 
     def ewald_sl(self, r):
         A, B = C.c_double(), C.c_double()

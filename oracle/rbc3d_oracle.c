@@ -108,6 +108,7 @@ struct orc_sys {
   double *bb, *qq;
   unsigned long long last_checksum;
   int gmres_restart; /* KSPGMRESSetRestart; 30 is PETSc's default */
+  int terms;         /* AddIntOnRbcs parts evaluated (ORC_TERM_*), all by default */
 };
 
 /* ======================================================================== */
@@ -791,6 +792,7 @@ orc_sys *orc_create(const orc_conf *conf) {
   orc_sys *s = calloc(1, sizeof(orc_sys));
   s->conf = *conf;
   s->gmres_restart = 30;
+  s->terms = ORC_TERM_ALL;
   for (int d = 0; d < 3; d++) {
     s->Lb[d] = conf->Lb[d];
     s->iLb[d] = 1. / conf->Lb[d];
@@ -1087,6 +1089,50 @@ int orc_cell_spline(const orc_sys *s, int irbc, int which, double *u, double *u1
   memcpy(u2, sp->u2, n * sizeof(double));
   memcpy(u12, sp->u12, n * sizeof(double));
   return (int)n;
+}
+
+/* Test hooks: the pieces of the path on their own, so each can be pinned. */
+int orc_set_terms(orc_sys *s, int terms) {
+  if (terms & ~ORC_TERM_ALL) return 1;
+  s->terms = terms;
+  return 0;
+}
+static const orc_spline *cell_spline_of(const orc_rbc *r, int which) {
+  return which == 0 ? &r->sx : which == 1 ? &r->sa3 : which == 2 ? &r->sdetj
+         : which == 3 ? &r->sF : &r->sG;
+}
+static void spline_interp(const orc_spline *sp, double x, double y, double *f);
+static void spline_find_projection(const orc_spline *sp, const double *xtar, double *th0,
+                                   double *phi0, double *x0);
+int orc_spline_eval(const orc_sys *s, int irbc, int which, double th, double phi, double *out) {
+  if (irbc < 0 || irbc >= s->nrbc || which < 0 || which > 4) return 1;
+  spline_interp(cell_spline_of(&s->rbcs[irbc], which), th, phi, out);
+  return 0;
+}
+int orc_find_projection(const orc_sys *s, int irbc, const double *xtar, double *th0,
+                        double *phi0, double *x0) {
+  if (irbc < 0 || irbc >= s->nrbc) return 1;
+  spline_find_projection(&s->rbcs[irbc].sx, xtar, th0, phi0, x0);
+  return 0;
+}
+void orc_spline_interp_raw(int m, int n, int nvar, const double *u, const double *u1,
+                           const double *u2, const double *u12, double x, double y, double *f) {
+  orc_spline sp;
+  sp.m = m;
+  sp.n = n;
+  sp.nvar = nvar;
+  sp.hx = TWO_PI / m;
+  sp.hy = TWO_PI / n;
+  sp.ihx = 1. / sp.hx;
+  sp.ihy = 1. / sp.hy;
+  sp.u = (double *)u;
+  sp.u1 = (double *)u1;
+  sp.u2 = (double *)u2;
+  sp.u12 = (double *)u12;
+  spline_interp(&sp, x, y, f);
+}
+double orc_cell_mesh_size(const orc_sys *s, int irbc) {
+  return (irbc < 0 || irbc >= s->nrbc) ? -1. : s->rbcs[irbc].meshSize;
 }
 
 /* ======================================================================== */
@@ -1387,17 +1433,18 @@ int orc_add_int_on_rbcs(orc_sys *s, double c1, double c2, int nT, const double *
     long long np_i = 0;
     unsigned long long cs_i = 0;
     pair_loop(s, c1, c2, xi, surfId_i, th_i, phi_i, Acoef[i], acc, &L, &np_i, &cs_i, i, &ovf);
+    if (!(s->terms & ORC_TERM_PAIR)) acc[0] = acc[1] = acc[2] = 0.;
     npairs += np_i;
     csum += cs_i;
     double dv[3];
     /* singular integration on the target's own surface */
-    if (rbc_i) {
+    if (rbc_i && (s->terms & ORC_TERM_SELF)) {
       double c2Mod = c2 * s->Bcoef[rbc_i->celltype - 1];
       rbc_sing_int(s, c1, c2Mod, rbc_i, tindx[i + nT] - 1, tindx[i + 2 * (size_t)nT] - 1, dv);
       for (int d = 0; d < 3; d++) acc[d] += dv[d] / Acoef[i];
     }
     /* near-singular integration on neighbouring cells */
-    for (int p = 0; p < L.n; p++) {
+    for (int p = 0; p < ((s->terms & ORC_TERM_NEAR) ? L.n : 0); p++) {
       const orc_rbc *rj = &s->rbcs[L.id[p] - 1];
       int ilat0 = L.ilat[p] - 1, ilon0 = L.ilon[p] - 1;
       double th0 = rj->th[ilat0], phi0 = rj->phi[ilon0], x0[3], xt[3];
@@ -1419,7 +1466,7 @@ int orc_add_int_on_rbcs(orc_sys *s, double c1, double c2, int nT, const double *
   s->last_checksum = csum;
   if (ovf) return 2;
   /* AddLinearInt :162-201 */
-  if (c2 != 0) {
+  if (c2 != 0 && (s->terms & ORC_TERM_LINEAR)) {
     double xv[3] = {0, 0, 0};
     for (int ir = 0; ir < s->nrbc; ir++) {
       const orc_rbc *r = &s->rbcs[ir];

@@ -117,6 +117,24 @@ int orc_solve_rbc_vel(orc_sys *s, double *sol, double *v, int *niter, double *rn
 int orc_near_list(orc_sys *s, int nT, const double *tx, const int *tindx, int i, int *cells,
                   int *ilat, int *ilon, double *dist, int cap);
 
+/* test hooks: the parts of AddIntOnRbcs evaluated (all by default), a cell's surface
+ * splines (which: 0 x, 1 a3, 2 detJ, 3 f detJ, 4 g detJ) at (th, phi), the closest-point
+ * projection onto cell irbc (0-based) starting from (th0, phi0), the bicubic Hermite
+ * interpolation of given tables on the doubly periodic [0,2pi)^2 grid (m x n), and a
+ * cell's mesh size */
+#define ORC_TERM_PAIR 1
+#define ORC_TERM_SELF 2
+#define ORC_TERM_NEAR 4
+#define ORC_TERM_LINEAR 8
+#define ORC_TERM_ALL 15
+int orc_set_terms(orc_sys *s, int terms);
+int orc_spline_eval(const orc_sys *s, int irbc, int which, double th, double phi, double *out);
+int orc_find_projection(const orc_sys *s, int irbc, const double *xtar, double *th0,
+                        double *phi0, double *x0);
+void orc_spline_interp_raw(int m, int n, int nvar, const double *u, const double *u1,
+                           const double *u2, const double *u12, double x, double y, double *f);
+double orc_cell_mesh_size(const orc_sys *s, int irbc);
+
 #ifdef __cplusplus
 }
 #endif
