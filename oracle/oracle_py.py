@@ -57,6 +57,7 @@ def lib():
         "orc_glob_sph_trans": (C.c_int, [vp, dp, dp, C.c_int]),
         "orc_cell_dof": (C.c_int, [vp]),
         "orc_cell_matmult": (C.c_int, [vp, dp, dp]),
+        "orc_cell_matmult_sample": (C.c_int, [vp, dp, dp, ip]),
         "orc_cell_rhs": (C.c_int, [vp, dp]),
         "orc_wall_dof": (C.c_int, [vp]),
         "orc_wall_matmult": (C.c_int, [vp, dp, dp]),
@@ -154,6 +155,8 @@ class Oracle:
                               cells.detj)
         assert r == 0
         self.np = L().orc_ntargets_rbc(self.h)
+        self._nlat = np.asarray(cells.nlat, np.int32).copy()
+        self._nlat0 = np.asarray(cells.nlat0, np.int32).copy()
 
     def set_density(self, which, dens):
         assert L().orc_set_cell_density(self.h, which, np.ascontiguousarray(dens)) == 0
@@ -237,6 +240,22 @@ class Oracle:
         if r:
             raise RuntimeError(f"orc_cell_matmult status {r}")
         return b
+
+    def cell_matmult_sample(self, u, cells):
+        """MyMatMult evaluated on the nodes of ``cells`` (0-based) only: b is valid on
+        those cells' coefficient blocks."""
+        act = np.zeros(len(self._nlat), np.int32)
+        act[np.asarray(cells)] = 1
+        b = np.zeros(self.cell_dof())
+        r = L().orc_cell_matmult_sample(self.h, np.ascontiguousarray(u), b, act)
+        if r:
+            raise RuntimeError(f"orc_cell_matmult_sample status {r}")
+        return b
+
+    def cell_blocks(self, cells):
+        """Indices of the Glob_Sph_Trans unknowns (3 nlat0^2 per cell) of ``cells``."""
+        off = np.concatenate([[0], np.cumsum(3 * self._nlat0.astype(np.int64) ** 2)])
+        return np.concatenate([np.arange(off[c], off[c + 1]) for c in cells])
 
     def cell_rhs(self):
         b = np.zeros(self.cell_dof())

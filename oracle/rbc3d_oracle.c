@@ -1011,16 +1011,20 @@ int orc_set_cells(orc_sys *s, int nrbc, const int *nlat0, const int *nlat, const
     for (int ilat = 0; ilat < nl; ilat++)
       for (int ilon = 0; ilon < nn; ilon++) r->area += r->detj[ilat + nl * ilon] * r->w[ilat];
     r->meshSize = sqrt(r->area / (nl * nl));
-    /* splines of x, a3, detJ: Rbc_BuildSurfaceSource(xFlag) */
     spline_create(&r->sx, 2 * nl, nn, 3);
     spline_create(&r->sa3, 2 * nl, nn, 3);
     spline_create(&r->sdetj, 2 * nl, nn, 1);
     spline_create(&r->sF, 2 * nl, nn, 3);
     spline_create(&r->sG, 2 * nl, nn, 3);
+    r->patch = find_or_make_patch(s, r);
+  }
+  /* splines of x, a3, detJ: Rbc_BuildSurfaceSource(xFlag), independent per cell */
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int i = 0; i < nrbc; i++) {
+    orc_rbc *r = &s->rbcs[i];
     build_surface_spline(r, 3, r->x, &r->sx);
     build_surface_spline(r, 3, r->a3, &r->sa3);
     build_surface_spline(r, 1, r->detj, &r->sdetj);
-    r->patch = find_or_make_patch(s, r);
   }
   /* SourceList_Create + SourceList_UpdateCoord common/ModSourceList.F90:27-151 */
   int Nc[3];
@@ -1051,13 +1055,12 @@ int orc_set_cells(orc_sys *s, int nrbc, const int *nlat0, const int *nlat, const
 }
 
 int orc_set_cell_density(orc_sys *s, int which, const double *dens) {
-  size_t off = 0;
+#pragma omp parallel for schedule(dynamic, 4)
   for (int i = 0; i < s->nrbc; i++) {
     orc_rbc *r = &s->rbcs[i];
     int nl = r->nlat, nn = r->nlon, npt = nl * nn;
     double *dst = which == 1 ? r->f : r->g;
-    memcpy(dst, dens + 3 * off, sizeof(double) * 3 * npt);
-    off += npt;
+    memcpy(dst, dens + 3 * (size_t)r->off, sizeof(double) * 3 * npt);
     double *fd = malloc(sizeof(double) * 3 * npt);
     for (int k = 0; k < 3; k++)
       for (int q = 0; q < npt; q++) fd[q + npt * k] = dst[q + npt * k] * r->detj[q];
@@ -1979,18 +1982,46 @@ int orc_add_int_on_walls(orc_sys *s, double c1, int nT, const double *tx, const 
 
 /* ======================================================================== */
 /* FFT used by the PME restatement (unnormalised, e^{sign 2 pi i jk/N}).     */
-static void fft1(double complex *a, int n, int sign, double complex *tmp) {
-  if (n == 1) return;
-  int r = 0;
+/* Mixed-radix decimation-in-time DFT of one line (smallest factor first, direct DFT for
+ * a prime length), unnormalised, exp(sign 2 pi i j k / n).  The twiddles of every length
+ * in the factor chain are tabulated once per plan, each entry the same expression the
+ * transform would otherwise evaluate per term. */
+static int fft_factor(int n) {
   for (int q = 2; q * q <= n; q++)
-    if (n % q == 0) {
-      r = q;
-      break;
-    }
+    if (n % q == 0) return q;
+  return 0;
+}
+typedef struct {
+  int nlev;
+  int n[40];
+  double complex *tw[40];
+} fft_plan;
+static void fft_plan_make(fft_plan *p, int n, int sign) {
+  p->nlev = 0;
+  int m = n;
+  for (;;) {
+    int li = p->nlev++;
+    p->n[li] = m;
+    p->tw[li] = malloc(sizeof(double complex) * (m > 0 ? m : 1));
+    for (int a = 0; a < m; a++) p->tw[li][a] = cexp(sign * 2 * PI_ * I * (double)a / m);
+    int r = m > 1 ? fft_factor(m) : 0;
+    if (!r) break;
+    m /= r;
+  }
+}
+static void fft_plan_free(fft_plan *p) {
+  for (int i = 0; i < p->nlev; i++) free(p->tw[i]);
+  p->nlev = 0;
+}
+static void fft1(double complex *a, const fft_plan *p, int li, double complex *tmp) {
+  int n = p->n[li];
+  if (n == 1) return;
+  const double complex *tw = p->tw[li];
+  int r = fft_factor(n);
   if (!r) { /* prime length: direct DFT */
     for (int k = 0; k < n; k++) {
       double complex sm = 0;
-      for (int j = 0; j < n; j++) sm += a[j] * cexp(sign * 2 * PI_ * I * (double)(((long)j * k) % n) / n);
+      for (int j = 0; j < n; j++) sm += a[j] * tw[((long)j * k) % n];
       tmp[k] = sm;
     }
     memcpy(a, tmp, sizeof(double complex) * n);
@@ -2001,11 +2032,10 @@ static void fft1(double complex *a, int n, int sign, double complex *tmp) {
   double complex *sub = malloc(sizeof(double complex) * n);
   for (int q = 0; q < r; q++)
     for (int j = 0; j < m; j++) sub[q * m + j] = a[q + r * j];
-  for (int q = 0; q < r; q++) fft1(sub + q * m, m, sign, tmp);
+  for (int q = 0; q < r; q++) fft1(sub + q * m, p, li + 1, tmp);
   for (int k = 0; k < n; k++) {
     double complex sm = 0;
-    for (int q = 0; q < r; q++)
-      sm += sub[q * m + (k % m)] * cexp(sign * 2 * PI_ * I * (double)(((long)q * k) % n) / n);
+    for (int q = 0; q < r; q++) sm += sub[q * m + (k % m)] * tw[((long)q * k) % n];
     a[k] = sm;
   }
   free(sub);
@@ -2013,31 +2043,45 @@ static void fft1(double complex *a, int n, int sign, double complex *tmp) {
 
 /* Forward 3-D transform in the reference's PFFTW convention (Get_That,
  * common/ModPFFTW.F90:148-167): x r2c (sign -1), y c2c (sign -1), z c2c (sign +1).
- * out_cplx: interleaved complex (0:N1/2, 0:N2-1, 0:N3-1), x fastest. */
+ * out_cplx: interleaved complex (0:N1/2, 0:N2-1, 0:N3-1), x fastest.  Lines are
+ * independent and run on the OpenMP threads. */
 void orc_fft3d_forward(int N1, int N2, int N3, const double *in, double *out_cplx) {
   int H = N1 / 2 + 1, L = N1 > N2 ? (N1 > N3 ? N1 : N3) : (N2 > N3 ? N2 : N3);
   double complex *out = (double complex *)out_cplx;
-  double complex *line = malloc(sizeof(double complex) * L), *tmp = malloc(sizeof(double complex) * L);
-  for (int k = 0; k < N3; k++)
-    for (int j = 0; j < N2; j++) {
-      for (int i = 0; i < N1; i++) line[i] = in[i + (size_t)N1 * (j + (size_t)N2 * k)];
-      fft1(line, N1, -1, tmp);
-      for (int i = 0; i < H; i++) out[i + (size_t)H * (j + (size_t)N2 * k)] = line[i];
-    }
-  for (int k = 0; k < N3; k++)
-    for (int i = 0; i < H; i++) {
-      for (int j = 0; j < N2; j++) line[j] = out[i + (size_t)H * (j + (size_t)N2 * k)];
-      fft1(line, N2, -1, tmp);
-      for (int j = 0; j < N2; j++) out[i + (size_t)H * (j + (size_t)N2 * k)] = line[j];
-    }
-  for (int j = 0; j < N2; j++)
-    for (int i = 0; i < H; i++) {
-      for (int k = 0; k < N3; k++) line[k] = out[i + (size_t)H * (j + (size_t)N2 * k)];
-      fft1(line, N3, +1, tmp);
-      for (int k = 0; k < N3; k++) out[i + (size_t)H * (j + (size_t)N2 * k)] = line[k];
-    }
-  free(line);
-  free(tmp);
+  fft_plan p1, p2, p3;
+  fft_plan_make(&p1, N1, -1);
+  fft_plan_make(&p2, N2, -1);
+  fft_plan_make(&p3, N3, +1);
+#pragma omp parallel
+  {
+    double complex *line = malloc(sizeof(double complex) * L), *tmp = malloc(sizeof(double complex) * L);
+#pragma omp for collapse(2) schedule(static)
+    for (int k = 0; k < N3; k++)
+      for (int j = 0; j < N2; j++) {
+        for (int i = 0; i < N1; i++) line[i] = in[i + (size_t)N1 * (j + (size_t)N2 * k)];
+        fft1(line, &p1, 0, tmp);
+        for (int i = 0; i < H; i++) out[i + (size_t)H * (j + (size_t)N2 * k)] = line[i];
+      }
+#pragma omp for collapse(2) schedule(static)
+    for (int k = 0; k < N3; k++)
+      for (int i = 0; i < H; i++) {
+        for (int j = 0; j < N2; j++) line[j] = out[i + (size_t)H * (j + (size_t)N2 * k)];
+        fft1(line, &p2, 0, tmp);
+        for (int j = 0; j < N2; j++) out[i + (size_t)H * (j + (size_t)N2 * k)] = line[j];
+      }
+#pragma omp for collapse(2) schedule(static)
+    for (int j = 0; j < N2; j++)
+      for (int i = 0; i < H; i++) {
+        for (int k = 0; k < N3; k++) line[k] = out[i + (size_t)H * (j + (size_t)N2 * k)];
+        fft1(line, &p3, 0, tmp);
+        for (int k = 0; k < N3; k++) out[i + (size_t)H * (j + (size_t)N2 * k)] = line[k];
+      }
+    free(line);
+    free(tmp);
+  }
+  fft_plan_free(&p1);
+  fft_plan_free(&p2);
+  fft_plan_free(&p3);
 }
 
 /* Inverse (Get_R, common/ModPFFTW.F90:170-185): z c2c (sign -1), y c2c (+1),
@@ -2046,31 +2090,44 @@ void orc_fft3d_inverse(int N1, int N2, int N3, const double *in_cplx, double *ou
   int H = N1 / 2 + 1, L = N1 > N2 ? (N1 > N3 ? N1 : N3) : (N2 > N3 ? N2 : N3);
   double complex *w = malloc(sizeof(double complex) * (size_t)H * N2 * N3);
   memcpy(w, in_cplx, sizeof(double complex) * (size_t)H * N2 * N3);
-  double complex *line = malloc(sizeof(double complex) * L), *tmp = malloc(sizeof(double complex) * L);
-  for (int j = 0; j < N2; j++)
-    for (int i = 0; i < H; i++) {
-      for (int k = 0; k < N3; k++) line[k] = w[i + (size_t)H * (j + (size_t)N2 * k)];
-      fft1(line, N3, -1, tmp);
-      for (int k = 0; k < N3; k++) w[i + (size_t)H * (j + (size_t)N2 * k)] = line[k];
-    }
-  for (int k = 0; k < N3; k++)
-    for (int i = 0; i < H; i++) {
-      for (int j = 0; j < N2; j++) line[j] = w[i + (size_t)H * (j + (size_t)N2 * k)];
-      fft1(line, N2, +1, tmp);
-      for (int j = 0; j < N2; j++) w[i + (size_t)H * (j + (size_t)N2 * k)] = line[j];
-    }
-  for (int k = 0; k < N3; k++)
-    for (int j = 0; j < N2; j++) {
-      for (int i = 0; i < H; i++) line[i] = w[i + (size_t)H * (j + (size_t)N2 * k)];
-      line[0] = creal(line[0]);
-      if (N1 % 2 == 0) line[N1 / 2] = creal(line[N1 / 2]);
-      for (int i = H; i < N1; i++) line[i] = conj(line[N1 - i]);
-      fft1(line, N1, +1, tmp);
-      for (int i = 0; i < N1; i++) out[i + (size_t)N1 * (j + (size_t)N2 * k)] = creal(line[i]);
-    }
+  fft_plan p1, p2, p3;
+  fft_plan_make(&p1, N1, +1);
+  fft_plan_make(&p2, N2, +1);
+  fft_plan_make(&p3, N3, -1);
+#pragma omp parallel
+  {
+    double complex *line = malloc(sizeof(double complex) * L), *tmp = malloc(sizeof(double complex) * L);
+#pragma omp for collapse(2) schedule(static)
+    for (int j = 0; j < N2; j++)
+      for (int i = 0; i < H; i++) {
+        for (int k = 0; k < N3; k++) line[k] = w[i + (size_t)H * (j + (size_t)N2 * k)];
+        fft1(line, &p3, 0, tmp);
+        for (int k = 0; k < N3; k++) w[i + (size_t)H * (j + (size_t)N2 * k)] = line[k];
+      }
+#pragma omp for collapse(2) schedule(static)
+    for (int k = 0; k < N3; k++)
+      for (int i = 0; i < H; i++) {
+        for (int j = 0; j < N2; j++) line[j] = w[i + (size_t)H * (j + (size_t)N2 * k)];
+        fft1(line, &p2, 0, tmp);
+        for (int j = 0; j < N2; j++) w[i + (size_t)H * (j + (size_t)N2 * k)] = line[j];
+      }
+#pragma omp for collapse(2) schedule(static)
+    for (int k = 0; k < N3; k++)
+      for (int j = 0; j < N2; j++) {
+        for (int i = 0; i < H; i++) line[i] = w[i + (size_t)H * (j + (size_t)N2 * k)];
+        line[0] = creal(line[0]);
+        if (N1 % 2 == 0) line[N1 / 2] = creal(line[N1 / 2]);
+        for (int i = H; i < N1; i++) line[i] = conj(line[N1 - i]);
+        fft1(line, &p1, 0, tmp);
+        for (int i = 0; i < N1; i++) out[i + (size_t)N1 * (j + (size_t)N2 * k)] = creal(line[i]);
+      }
+    free(line);
+    free(tmp);
+  }
   free(w);
-  free(line);
-  free(tmp);
+  fft_plan_free(&p1);
+  fft_plan_free(&p2);
+  fft_plan_free(&p3);
 }
 
 /* ======================================================================== */
@@ -2127,7 +2184,7 @@ static void pme_init(orc_sys *s) {
 
 /* Distrib_Source :405-443 */
 static void distrib_source(orc_sys *s, const double *x, double c1, double c2, const double *f,
-                           const double *t) {
+                           const double *t, int klo, int khi) {
   int N1 = s->conf.Nb[0], N2 = s->conf.Nb[1], N3 = s->conf.Nb[2], P = s->conf.P;
   size_t ng = (size_t)N1 * N2 * N3;
   double wx[16], wy[16], wz[16];
@@ -2137,6 +2194,7 @@ static void distrib_source(orc_sys *s, const double *x, double c1, double c2, co
   orc_bspline(x[2] * (N3 / s->Lb[2]), P, &kmin, wz);
   for (int k0 = 0; k0 < P; k0++) {
     int k = imod(kmin + k0, N3);
+    if (k < klo || k >= khi) continue;
     for (int j0 = 0; j0 < P; j0++)
       for (int i0 = 0; i0 < P; i0++) {
         int j = imod(jmin + j0, N2), i = imod(imin + i0, N1);
@@ -2159,6 +2217,12 @@ int orc_pme_distrib_source(orc_sys *s, double c1, double c2, int cells, int wall
   s->flag_dl = fabs(c2) > 1.e-10;
   memset(s->ff, 0, sizeof(double) * 3 * ng);
   memset(s->tt, 0, sizeof(double) * 9 * ng);
+  /* each thread owns a slab of z planes and adds, in source order, every contribution
+   * that lands there: the same sums in the same order as one thread */
+#pragma omp parallel
+  {
+  int nth = omp_get_num_threads(), ith = omp_get_thread_num();
+  int klo = (int)((long)N3 * ith / nth), khi = (int)((long)N3 * (ith + 1) / nth);
   if (cells && s->nrbc) {
     const orc_slist *l = &s->slr;
     int np = l->nPoint;
@@ -2177,7 +2241,7 @@ int orc_pme_distrib_source(orc_sys *s, double c1, double c2, int cells, int wall
         for (int ii = 0; ii < 3; ii++)
           for (int jj = 0; jj < 3; jj++) t[ii + 3 * jj] = g[ii] * a3[jj];
       }
-      distrib_source(s, x, c1, c2, f, t);
+      distrib_source(s, x, c1, c2, f, t, klo, khi);
     }
   }
   if (walls && s->nwall) {
@@ -2190,8 +2254,9 @@ int orc_pme_distrib_source(orc_sys *s, double c1, double c2, int cells, int wall
       if (s->flag_sl)
         for (int d = 0; d < 3; d++)
           f[d] = (1.0 / 3) * (fe[3 * d] + fe[1 + 3 * d] + fe[2 + 3 * d]) * w->area[e];
-      distrib_source(s, x, c1, c2, f, t);
+      distrib_source(s, x, c1, c2, f, t, klo, khi);
     }
+  }
   }
   return 0;
 }
@@ -2356,8 +2421,18 @@ int orc_glob_sph_trans(orc_sys *s, double *v, double *c, int direction) {
   return 0;
 }
 
-/* MyMatMult: common/ModVelSolver.F90:523-601 (no deflation) */
+/* MyMatMult: common/ModVelSolver.F90:523-601 (no deflation).  cell_active (or NULL for
+ * every cell) limits the velocity evaluation to the nodes of the listed cells, so b is
+ * the reference's only on those cells' coefficient blocks (a bounded CPU sample of a
+ * large system; the sources, the PME grids and the linear term are always complete). */
+static int cell_matmult_masked(orc_sys *s, const double *u, double *b, const int *cell_active);
 int orc_cell_matmult(orc_sys *s, const double *u, double *b) {
+  return cell_matmult_masked(s, u, b, NULL);
+}
+int orc_cell_matmult_sample(orc_sys *s, const double *u, double *b, const int *cell_active) {
+  return cell_matmult_masked(s, u, b, cell_active);
+}
+static int cell_matmult_masked(orc_sys *s, const double *u, double *b, const int *cell_active) {
   int nP = s->slr.nPoint;
   double *g = calloc((size_t)3 * nP, sizeof(double)), *v = calloc((size_t)3 * nP, sizeof(double));
   double *uc = malloc(sizeof(double) * orc_cell_dof(s));
@@ -2378,7 +2453,7 @@ int orc_cell_matmult(orc_sys *s, const double *u, double *b) {
   double *tx = malloc(sizeof(double) * 3 * nP), *Ac = malloc(sizeof(double) * nP);
   int *ti = malloc(sizeof(int) * 3 * nP), *act = malloc(sizeof(int) * nP);
   orc_tlist_rbc(s, tx, ti, Ac);
-  for (int i = 0; i < nP; i++) act[i] = 1;
+  for (int i = 0; i < nP; i++) act[i] = cell_active ? (cell_active[ti[i] - 1] != 0) : 1;
   int rc = orc_add_int_on_rbcs(s, c1, c2, nP, tx, ti, act, Ac, v, NULL);
   orc_pme_distrib_source(s, c1, c2, 1, 0);
   orc_pme_transform(s);

@@ -74,6 +74,9 @@ int orc_calc_velocity_field(orc_sys *s, int nT, const double *tx, const int *tin
 int orc_glob_sph_trans(orc_sys *s, double *v, double *c, int direction);
 int orc_cell_dof(const orc_sys *s);
 int orc_cell_matmult(orc_sys *s, const double *u, double *b);
+/* the same on the nodes of the cells with cell_active[irbc] != 0 only (other cells'
+ * coefficient blocks of b are not the reference's) */
+int orc_cell_matmult_sample(orc_sys *s, const double *u, double *b, const int *cell_active);
 int orc_cell_rhs(orc_sys *s, double *rhs);
 int orc_wall_dof(const orc_sys *s);
 int orc_wall_matmult(orc_sys *s, const double *u, double *b);

@@ -76,10 +76,21 @@ def pbar_table(nlat: int, th: np.ndarray, mmax: int) -> np.ndarray:
     return P
 
 
+_PBAR_CACHE = {}
+
+
+def _pbar_cached(nmax, th, mmax):
+    key = (nmax, mmax, th.tobytes())
+    P = _PBAR_CACHE.get(key)
+    if P is None:
+        P = _PBAR_CACHE[key] = pbar_table(nmax, th, mmax)
+    return P
+
+
 def sh_synth(a: np.ndarray, b: np.ndarray, th: np.ndarray, nlon: int) -> np.ndarray:
     """g(i, j) = sum_n .5 a[0,n] Pbar(0,n) + sum_{m>=1,n>=m} Pbar (a cos m phi - b sin m phi)."""
     mmax, nmax = a.shape
-    P = pbar_table(nmax, th, mmax)
+    P = _pbar_cached(nmax, th, mmax)
     phi = 2 * np.pi * np.arange(nlon) / nlon
     g = np.zeros((len(th), nlon))
     for m in range(mmax):
@@ -215,9 +226,14 @@ def place_cells_gap(rng, shapes, L, gap, max_tries=2_000_000, cyl=None):
     return np.asarray(cs), Qs
 
 
+_COLAT_CACHE = {}
+
+
 def random_density(rng, nlat, nlat0):
     """One surface field (nlat, nlon, 3) from SH coefficients ~ N(0,1)(l+1)^-2, l < nlat0."""
-    th, _ = gauss_colat(nlat)
+    if nlat not in _COLAT_CACHE:
+        _COLAT_CACHE[nlat] = gauss_colat(nlat)
+    th, _ = _COLAT_CACHE[nlat]
     nlon = 2 * nlat
     out = np.zeros((nlat, nlon, 3))
     n = np.arange(nlat0)

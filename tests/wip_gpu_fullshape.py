@@ -1,0 +1,183 @@
+"""GPU parity and properties at the benchmark shapes themselves, not pieces of them:
+C2 (512 discs at nlat 17, PME 128^3), C5 (3,584 discs at nlat 13 + 512 spheres at
+nlat 17, PME 256^3), the C3 tube (200 cells, 20,608-vertex wall, PME 96x96x180) and the
+committed C4 network.  The oracle evaluates the matvec on the nodes of a sample of cells
+(the sources, PME grids and linear term stay complete), which keeps its CPU time bounded;
+the GPU computes everything and is compared on the sample.
+
+Bars as elsewhere: pair sets bit-exact, FP64 velocities and coefficients within 1e-12
+(max-norm relative); the chosen form within 1e-6 of the plain form."""
+import numpy as np
+import pytest
+
+from tests.conftest import relerr
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module")
+def R():
+    import rbc3d_amd
+    rbc3d_amd.lib()
+    return rbc3d_amd
+
+
+def _matvec_and_pairs_on_sample(R, orc, name, cells):
+    from rbc3d_amd import configs
+    w = configs.make_workload(name, seed=1)
+    print(f"{name}: {w.cells.ncell} cells, {w.cells.npoint} nodes, PME {w.Nb}", flush=True)
+    g = R.from_workload(w)
+    o = orc.from_workload(w)
+    try:
+        u = np.random.default_rng(0).standard_normal(o.cell_dof())
+        bg = g.cell_matmult(u)
+        bo = o.cell_matmult_sample(u, cells)
+        idx = o.cell_blocks(cells)
+        e_b = relerr(bg[idx], bo[idx])
+        # AddIntOnRbcs (SL + DL with every correction) on the sample's nodes; MyMatMult
+        # replaced g, so both sides get the workload's densities back first
+        g.set_cell_density(1, w.cells.f)
+        g.set_cell_density(2, w.cells.g)
+        o.set_density(1, w.cells.f)
+        o.set_density(2, w.cells.g)
+        tx, ti, act, A = o.tlist_rbc()
+        act = np.isin(ti[:len(A)] - 1, cells).astype(np.int32)
+        c = 1 / (4 * np.pi)
+        vg = g.add_int_on_rbcs(c, c, (tx, ti, act, A)).reshape(3, -1)[:, act == 1]
+        vo, npo = o.add_int_on_rbcs(c, c, (tx, ti, act, A))
+        vo = vo.reshape(3, -1)[:, act == 1]
+        npg, csg = g.pair_stats()
+        e_v = relerr(vg, vo)
+        print(f"{name}: matvec on {len(cells)} cells {e_b:.2e}; AddIntOnRbcs on "
+              f"{int(act.sum())} targets {e_v:.2e}, {npg} pairs", flush=True)
+        assert npg == npo and csg == o.pair_checksum()
+        assert e_b < TOL and e_v < TOL
+    finally:
+        g.close()
+
+
+def test_c2_cell_matvec_full_shape(R, orc):
+    """C2 at its own shape: the full cell matvec, compared on 8 cells."""
+    cells = np.random.default_rng(1).choice(512, 8, replace=False)
+    _matvec_and_pairs_on_sample(R, orc, "C2", cells)
+
+
+def test_c5_cell_matvec_full_shape(R, orc):
+    """C5 at its own shape (two grid types, PME 256^3), compared on 5 discs and 5 spheres."""
+    rng = np.random.default_rng(2)
+    cells = np.concatenate([rng.choice(3584, 5, replace=False), 3584 + rng.choice(512, 5, replace=False)])
+    _matvec_and_pairs_on_sample(R, orc, "C5", cells)
+
+
+@pytest.fixture(scope="module")
+def c3(R):
+    from rbc3d_amd import configs
+    w = configs.make_workload("C3", seed=1)
+    g = R.from_workload(w)
+    g.prepare_sing_int_on_wall()
+    yield w, g
+    g.close()
+
+
+def _unique_wall_rows(w):
+    """The tube's vertices without the duplicated seam ring (the ring at z = Lz, which
+    repeats the ring at z = 0): the unknowns of the no-slip system."""
+    z = w.wall[0].reshape(3, -1)[2]
+    return np.nonzero(z < w.Lb[2] - 1e-9)[0]
+
+
+def test_c3_wall_operator_full_shape(c3, orc):
+    """The no-slip operator (58M-entry self-matrix SpMV + wall-only PME) and the
+    residual wall velocity of the 200 cells, on the full tube, against the oracle."""
+    import ctypes as C
+
+    import rbc3d_amd
+    w, g = c3
+    o = orc.from_workload(w)
+    o.prepare_wall()
+    nnz = C.c_longlong()
+    rbc3d_amd._call("rbc3d_wall_self_nnz", g.h, C.byref(nnz))
+    rp, col, val = o.wall_csr()
+    assert nnz.value == len(val)
+    u = np.random.default_rng(8).standard_normal(o.wall_dof())
+    e_op = relerr(g.wall_matmult(u), o.wall_matmult(u))
+    f = np.random.default_rng(4).standard_normal(3 * w.wall[2]) * 0.1
+    g.set_wall_density(f)
+    o.set_wall_density(f)
+    e_res = relerr(g.wall_residual_vel(), o.wall_residual_vel())
+    print(f"C3: self-matrix {nnz.value} entries; operator {e_op:.2e}, residual velocity "
+          f"{e_res:.2e}", flush=True)
+    assert e_op < TOL and e_res < TOL
+
+
+def test_c3_chosen_timestep_full_shape(c3):
+    """One tube timestep from zero wall traction, plain (FP64, GMRES(30)) and chosen
+    (split form, GMRES(150)), both to 1e-8.  After each solve the residual is recomputed
+    from scratch (the wall velocity the new traction leaves, against the velocity before
+    the solve), not taken from GMRES's estimate: <= 1e-8 for the operator each form
+    solved with; the chosen solution's residual under the FP64 operator and its cell
+    velocity against the plain form's stay within the split form's 1e-6."""
+    w, g = c3
+    rows = _unique_wall_rows(w)
+    nv = w.wall[2]
+
+    def true_res(before):
+        r = g.wall_residual_vel().reshape(3, nv)[:, rows]
+        return float(np.linalg.norm(r) / np.linalg.norm(before))
+
+    out = {}
+    for form in ("plain", "chosen"):
+        g.set_wall_density(np.zeros(3 * nv))
+        if form == "chosen":
+            g.set_precision(g.PREC_SPLIT)
+            g.set_gmres_restart(150)
+        try:
+            g.solve_rbc_vel()
+            b = g.wall_residual_vel().reshape(3, nv)[:, rows]
+            it, est, conv, _ = g.noslip_wall(1e-8, 4000)
+            res = true_res(b)
+            v, _, _ = g.solve_rbc_vel()
+            g.set_precision(g.PREC_FP64)
+            res64 = true_res(b)
+        finally:
+            g.set_precision(g.PREC_FP64)
+            g.set_gmres_restart(30)
+        out[form] = (v, it, conv, res, res64)
+        print(f"C3 {form}: {it} iterations, residual recomputed {res:.2e} (own operator), "
+              f"{res64:.2e} (FP64 operator)", flush=True)
+    vp, ip, cp, rp, _ = out["plain"]
+    vc, ic, cc, rc, rc64 = out["chosen"]
+    e_v = relerr(vc, vp)
+    print(f"C3 chosen vs plain: cell velocity {e_v:.2e}, {ic} vs {ip} iterations", flush=True)
+    assert cp and cc and ic < ip
+    assert rp <= 1.05e-8 and rc <= 1.05e-8
+    assert rc64 <= 1e-6 and e_v <= 1e-6
+
+
+def test_c4_committed_network_held_out_seed(R):
+    """The committed network (rbc3d_amd/data/mlp_C3.npz) on a held-out placement (seed
+    201), one step of the C4 protocol from zero traction: the warm-started solve in the
+    chosen form converges to 1e-8 (with plain GMRES as the fallback), and its cell
+    velocity matches the plain FP64 solve's within 1e-6.  The iteration counts are
+    printed; the gain itself is measured by the benchmark."""
+    import os
+
+    from rbc3d_amd import configs, learned
+    path = os.path.join(os.path.dirname(R.__file__), "data", "mlp_C3.npz")
+    m = learned.load_model(path)
+    w = configs.make_workload("C3", seed=201)
+    g = R.from_workload(w)
+    try:
+        g.prepare_sing_int_on_wall()
+        recs = learned.evaluate(g, w.cells, m, 1, 0.05, rtol=1e-8, maxit=4000, split=True,
+                                restart=150, cold=True)
+        r = recs[0]
+        print(f"C4 seed 201: plain {r['plain_iters']}, chosen from zero {r['cold_iters']}, "
+              f"chosen + network {r['learned_iters']} iterations (fallback {r['fallback']}), "
+              f"velocity {r['vel_diff']:.2e}", flush=True)
+        assert r["plain_conv"] and r["learned_conv"] and r["cold_conv"]
+        assert r["vel_diff"] <= 1e-6
+    finally:
+        g.close()
+        m.close()
