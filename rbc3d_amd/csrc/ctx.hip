@@ -170,16 +170,19 @@ static int make_gtype(rbc3d_ctx *c, int nlat, GridType **out) {
   RBC_TRY(upload(g->d_phi, g->phi, st));
   RBC_TRY(upload(g->d_w, g->w, st));
   // SH tables
-  std::vector<double> P, pw((size_t)g->mmax * nl * nl), pg((size_t)g->mmax * nl * nl);
+  std::vector<double> P, pw((size_t)g->mmax * nl * nl), pg((size_t)g->mmax * nl * nl),
+      pws((size_t)g->mmax * nl * nl);
   pbar(nl, g->mmax, g->th, P);
   for (int m = 0; m < g->mmax; m++)
     for (int n = 0; n < nl; n++)
       for (int i = 0; i < nl; i++) {
         size_t o = ((size_t)m * nl + n) * nl + i;
         pw[o] = (2.0 / nn) * wg[i] * P[o];
+        pws[o] = P[o] * wg[i];
         pg[o] = (m == 0 ? 0.5 : 1.0) * P[o];
       }
   RBC_TRY(upload(g->d_pw, pw, st));
+  RBC_TRY(upload(g->d_pws, pws, st));
   RBC_TRY(upload(g->d_pg, pg, st));
   int ne = nl + 1, mmax_e = std::min(ne, nn / 2 + 1);
   std::vector<double> the(ne), Pe, pe((size_t)mmax_e * ne * ne);
@@ -532,7 +535,7 @@ extern "C" int rbc3d_destroy(rbc3d_ctx *c) {
   if (!c) return RBC3D_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   for (auto *g : c->gtypes) {
-    for (DBuf<double> *b : {&g->d_th, &g->d_phi, &g->d_w, &g->d_pw, &g->d_pg, &g->d_pe, &g->d_cs,
+    for (DBuf<double> *b : {&g->d_th, &g->d_phi, &g->d_w, &g->d_pw, &g->d_pws, &g->d_pg, &g->d_pe, &g->d_cs,
                             &g->d_Dm, &g->d_Dn, &g->d_thG, &g->d_phiG, &g->d_pwt, &g->d_mask,
                             &g->d_proj})
       b->release();

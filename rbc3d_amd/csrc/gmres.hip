@@ -127,10 +127,10 @@ __global__ void k_scale_norm(int n, const double *__restrict__ t2, double *w) {
   if (tt > 0.) w[i] = (1.0 / tt) * w[i];
 }
 
-int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, double *)> &matvec,
-              const double *d_b, double *d_x, bool x0_nonzero, double rtol, double atol,
-              double dtol, int maxit, int restart, int *niter, double *rnorm, int *reason,
-              std::vector<double> *history) {
+int rbc_gmres(rbc3d_ctx *c, int n, const MatvecFn &matvec, const double *d_b, double *d_x,
+              bool x0_nonzero, double rtol, double atol, double dtol, int maxit, int restart,
+              int *niter, double *rnorm, int *reason, std::vector<double> *history,
+              const MatvecFn *matvec_lo, double lo_below) {
   hipStream_t st = c->stream;
   if (restart < 1 || restart > G_MAXV - 1) return RBC3D_EINVAL;
   GmresWork &ws = c->gmres;
@@ -192,9 +192,17 @@ int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, doubl
     // j + 1 is queued before the host reads step j's column: the device never waits
     // for the Givens update.  A step queued past the end of the cycle (convergence,
     // breakdown) only writes V_{j+2} and the device scratch, which nothing reads.
+    // With a second, cheaper operator (an inexact one), an Arnoldi step uses it once the
+    // latest residual estimate is below lo_below times the initial residual: the error a
+    // step's product may carry grows as the residual falls (relaxed inexact GMRES), so the
+    // first steps take the exact operator and the rest the cheap one.  Residuals at
+    // restarts always use the exact operator.
     auto enqueue = [&](int j) -> int {
       double *vj = V + ld * j, *vn = V + ld * (j + 1);
-      RBC_TRY(matvec(vj, vn));
+      if (matvec_lo && res <= lo_below * rnorm0)
+        RBC_TRY((*matvec_lo)(vj, vn));
+      else
+        RBC_TRY(matvec(vj, vn));
       RBC_TRY(dots_dev(st, ws, n, j + 1, V, ld, vn, ws.hd.p));
       k_maxpy<<<nb, 256, 0, st>>>(n, j + 1, V, ld, ws.hd.p, vn);
       RBC_TRY(dots_dev(st, ws, n, 1, vn, ld, vn, ws.hd.p + j + 1));
@@ -336,6 +344,126 @@ static int gmres_status(int reason, const char *who) {
   return RBC3D_ENOCONV;
 }
 
+__global__ void k_axpy1(int n, const double *__restrict__ d, double *x) {  // x += d
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += d[i];
+}
+
+// Runs the enclosed operator applications in the given precision form, then restores.
+struct PrecScope {
+  rbc3d_ctx *c;
+  int saved;
+  PrecScope(rbc3d_ctx *ctx, int mode) : c(ctx), saved(ctx->prec) { c->prec = mode; }
+  ~PrecScope() { c->prec = saved; }
+};
+
+#define RBC_IR_MAX_OUTER 12
+#ifndef RBC_IR_INNER_MIN
+#define RBC_IR_INNER_MIN 5e-9
+#endif
+#ifndef RBC_IR_SWITCH
+#define RBC_IR_SWITCH 1e-2
+#endif
+
+// The wall solve of both NoSlipWall entries (x += the solution of L x = b, from x when
+// x0nz).  FP64 form: restarted GMRES on the FP64 operator, as the reference.  Split form:
+// the FP32 operator is exact only to ~1e-7 relative, so GMRES on it alone drives its own
+// residual estimate below 1e-8 while the residual of the FP64 system stalls near 1e-7.
+// Here one GMRES builds its Krylov space with the FP64 operator while the residual
+// estimate is above 1e-2 |r| and with the split operator below (relaxed inexact GMRES,
+// Simoncini & Szyld, SIAM J. Sci. Comput. 25 (2003) 454; Bouras & Fraysse, SIAM J.
+// Matrix Anal. Appl. 26 (2005) 660: the operator error a step may carry grows as the
+// residual falls), so the split operator's ~1e-7 error enters only in steps whose
+// contribution to the solution is already small.  The FP64 residual r = b - L_64 x is
+// then computed; while it is above rtol |b|, one more such solve of L d = r (to
+// 0.9 rtol |b| / |r|) refines x += d (iterative refinement, Carson & Higham, SIAM J. Sci.
+// Comput. 40 (2018) A817).  The reported residual is the FP64 one; the iteration count
+// sums the solves.
+static int wall_gmres(rbc3d_ctx *c, int nd, const double *d_b, double *d_x, bool x0nz,
+                      double rtol, int maxit, int *niter, double *rnorm, int *reason) {
+  MatvecFn mv = [c](const double *u, double *b) { return rbc_wall_matmult_dev(c, u, b); };
+  GmresWork &gw = c->gmres;
+  if (c->prec != RBC3D_PREC_SPLIT)
+    return rbc_gmres(c, nd, mv, d_b, d_x, x0nz, rtol, 1e-50, 1e5, maxit, c->gmres_restart, niter,
+                     rnorm, reason, &gw.history);
+  MatvecFn mv64 = [c](const double *u, double *b) {
+    PrecScope fp64(c, RBC3D_PREC_FP64);
+    return rbc_wall_matmult_dev(c, u, b);
+  };
+  hipStream_t st = c->stream;
+  RBC_TRY(gw.ir_r.ensure(nd + 1));
+  RBC_TRY(gw.ir_d.ensure(nd + 1));
+  const int nb = (nd + 255) / 256;
+  if (!x0nz) HIP_TRY(hipMemsetAsync(d_x, 0, sizeof(double) * nd, st));
+  double b2;
+  RBC_TRY(dots(st, gw, nd, 1, d_b, (size_t)nd, d_b, &b2));
+  const double ttol = rtol * sqrt(b2);
+  int its = 0;
+  double res = 0, prev = 0;
+  *reason = 0;
+  for (int k = 0;; k++) {
+    if (k == 0 && !x0nz) {
+      HIP_TRY(hipMemcpyAsync(gw.ir_r.p, d_b, sizeof(double) * nd, hipMemcpyDeviceToDevice, st));
+    } else {
+      {
+        PrecScope fp64(c, RBC3D_PREC_FP64);
+        RBC_TRY(rbc_wall_matmult_dev(c, d_x, gw.ir_r.p));
+      }
+      k_sub<<<nb, 256, 0, st>>>(nd, d_b, gw.ir_r.p);
+      HIP_TRY(hipGetLastError());
+    }
+    double r2;
+    RBC_TRY(dots(st, gw, nd, 1, gw.ir_r.p, (size_t)nd, gw.ir_r.p, &r2));
+    res = sqrt(r2);
+    if (!std::isfinite(res)) {
+      *reason = -4;
+      break;
+    }
+    if (res <= ttol) {
+      *reason = 1;
+      break;
+    }
+    // out of iterations, or refinement no longer contracts (the split operator is too
+    // inexact for this system): not converged
+    if (its >= maxit || k >= RBC_IR_MAX_OUTER || (k > 0 && !(res < 0.9 * prev))) {
+      *reason = -3;
+      break;
+    }
+    prev = res;
+    const double tau = std::min(0.5, std::max(0.9 * ttol / res, RBC_IR_INNER_MIN));
+    int in_its = 0, in_reason = 0;
+    double in_res = 0;
+    RBC_TRY(rbc_gmres(c, nd, mv64, gw.ir_r.p, gw.ir_d.p, false, tau, 1e-50, 1e5, maxit - its,
+                      c->gmres_restart, &in_its, &in_res, &in_reason, &gw.history, &mv,
+                      RBC_IR_SWITCH));
+    its += in_its;
+    if (in_reason == -4) {
+      *reason = -4;
+      break;
+    }
+    k_axpy1<<<nb, 256, 0, st>>>(nd, gw.ir_d.p, d_x);
+    HIP_TRY(hipGetLastError());
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  *niter = its;
+  *rnorm = res;
+  return RBC3D_OK;
+}
+
+// The right-hand side of NoSlipWall, -(residual velocity) on the unique vertices, from
+// FP64 sums in either form (the split form refines against it).
+static int wall_rhs_fp64(rbc3d_ctx *c, double *rhs) {
+  WallData &w = c->wall;
+  const int nv = w.nvert, nu = w.ndof / 3;
+  {
+    PrecScope fp64(c, RBC3D_PREC_FP64);
+    RBC_TRY(rbc_wall_residual_dev(c));
+  }
+  k_neg_assemble<<<(nu + 255) / 256, 256, 0, c->stream>>>(nu, nv, w.d_glb.p + nv, c->d_v.p, rhs);
+  HIP_TRY(hipGetLastError());
+  return RBC3D_OK;
+}
+
 // NoSlipWall (common/ModNoSlip.F90:45-150)
 extern "C" int rbc3d_noslip_wall(rbc3d_ctx *c, double rtol, int maxit, const double *finc0,
                                  int *niter, double *rnorm, int *converged, double *vres_max) {
@@ -350,16 +478,13 @@ extern "C" int rbc3d_noslip_wall(rbc3d_ctx *c, double rtol, int maxit, const dou
   RBC_TRY(x.ensure(nd + 1));
   HIP_TRY(hipMemcpyAsync(f0.p, w.d_f.p, sizeof(double) * 3 * nv, hipMemcpyDeviceToDevice, st));
   // rhs = -(residual velocity), assembled on the unique vertices
-  RBC_TRY(rbc_wall_residual_dev(c));
-  k_neg_assemble<<<(nu + 255) / 256, 256, 0, st>>>(nu, nv, w.d_glb.p + nv, c->d_v.p, rhs.p);
-  HIP_TRY(hipGetLastError());
+  RBC_TRY(wall_rhs_fp64(c, rhs.p));
+  (void)nu;
   bool x0nz = finc0 != nullptr;
   if (x0nz) HIP_TRY(hipMemcpyAsync(x.p, finc0, sizeof(double) * nd, hipMemcpyHostToDevice, st));
   int reason = 0;
-  auto mv = [c](const double *u, double *b) { return rbc_wall_matmult_dev(c, u, b); };
   c->gmres.history.clear();
-  RBC_TRY(rbc_gmres(c, nd, mv, rhs.p, x.p, x0nz, rtol, 1e-50, 1e5, maxit, c->gmres_restart, niter, rnorm, &reason,
-                    &c->gmres.history));
+  RBC_TRY(wall_gmres(c, nd, rhs.p, x.p, x0nz, rtol, maxit, niter, rnorm, &reason));
   *converged = reason > 0;
   // f = f0 + finc (AssembleArray(finc, finc1D, -1))
   k_add_assembled<<<(nv + 255) / 256, 256, 0, st>>>(nv, w.d_glb.p, f0.p, x.p, w.d_f.p);
@@ -509,6 +634,9 @@ extern "C" int rbc3d_noslip_wall_learned(rbc3d_ctx *c, rbc3d_mlp *m, double rtol
   k_add3<<<(nv + 255) / 256, 256, 0, st>>>(nv, c->d_v.p, uc.p, c->conf.vBkg[0], c->conf.vBkg[1],
                                            c->conf.vBkg[2]);
   k_neg_assemble<<<(nu + 255) / 256, 256, 0, st>>>(nu, nv, w.d_glb.p + nv, c->d_v.p, rhs.p);
+  // the split form refines against the FP64 right-hand side; the features above keep the
+  // cells' velocity of the current form
+  if (c->prec == RBC3D_PREC_SPLIT) RBC_TRY(wall_rhs_fp64(c, rhs.p));
   // the network's prediction of the increment
   k_mlp_features<<<(nv + 255) / 256, 256, 0, st>>>(nv, uc.p, f0.p, w.d_vnorm.p, X.p);
   HIP_TRY(hipGetLastError());
@@ -554,20 +682,15 @@ extern "C" int rbc3d_noslip_wall_learned(rbc3d_ctx *c, rbc3d_mlp *m, double rtol
   double b2, r2;
   RBC_TRY(dots(st, c->gmres, nd, 1, rhs.p, (size_t)nd, rhs.p, &b2));
   RBC_TRY(dots(st, c->gmres, nd, 1, r0.p, (size_t)nd, r0.p, &r2));
-  auto mv = [c](const double *u, double *b) { return rbc_wall_matmult_dev(c, u, b); };
   int reason = 0, its = 0;
   *fallback = 0;
   c->gmres.history.clear();
-  if (r2 < b2) {
-    RBC_TRY(rbc_gmres(c, nd, mv, rhs.p, x.p, true, rtol, 1e-50, 1e5, maxit, c->gmres_restart, &its, rnorm, &reason,
-                      &c->gmres.history));
-  }
+  if (r2 < b2) RBC_TRY(wall_gmres(c, nd, rhs.p, x.p, true, rtol, maxit, &its, rnorm, &reason));
   *niter = its;
   if (reason <= 0) {  // plain GMRES from zero
     *fallback = 1;
     int its2 = 0;
-    RBC_TRY(rbc_gmres(c, nd, mv, rhs.p, x.p, false, rtol, 1e-50, 1e5, maxit, c->gmres_restart, &its2, rnorm,
-                      &reason, &c->gmres.history));
+    RBC_TRY(wall_gmres(c, nd, rhs.p, x.p, false, rtol, maxit, &its2, rnorm, &reason));
     *niter += its2;
   }
   *converged = reason > 0;

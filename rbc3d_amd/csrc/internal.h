@@ -73,6 +73,7 @@ struct GridType {
   std::vector<double> th, phi, w;  // w includes 2 pi / nlon (RBC_Create)
   DBuf<double> d_th, d_phi, d_w;
   DBuf<double> d_pw;     // analysis  [m][n][i] = (2/nlon) wgauss_i Pbar(m,n,th_i)
+  DBuf<double> d_pws;    // analysis of the spline build [m][n][i] = Pbar(m,n,th_i) wgauss_i
   DBuf<double> d_pg;     // synthesis [m][n][i] = Pbar(m,n,th_i) (x 1/2 for m = 0)
   DBuf<double> d_pe;     // equi synthesis, nlat+1 rows incl. poles, [m][n][ie]
   DBuf<double> d_cs;     // cos(m phi_j) [m][j] then sin(m phi_j) [m][j], m <= nlat
@@ -92,6 +93,7 @@ struct GmresWork {
   // vectors of the solves around the GMRES (previous traction, right-hand side,
   // solution, the learned warm start's inputs and residual images), kept across calls
   DBuf<double> f0, rhs, x, r0, uc, LV, tmp;
+  DBuf<double> ir_r, ir_d;  // iterative refinement of the split form: FP64 residual, correction
   DBuf<float> X, Y;
   std::vector<double> history;  // residual estimates of the last solve
   // pinned host slots for the Hessenberg columns of two Arnoldi steps in flight and
@@ -485,9 +487,13 @@ int rbc_wall_setup(rbc3d_ctx *c);
 int rbc_wall_prepare(rbc3d_ctx *c);
 int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double c1, int nT, const double *d_tx, const int *d_tindx,
                              const int *d_act, const double *d_A, double *d_v, bool wall_targets);
-int rbc_gmres(rbc3d_ctx *c, int n, const std::function<int(const double *, double *)> &matvec,
-              const double *d_b, double *d_x, bool x0_nonzero, double rtol, double atol,
-              double dtol, int maxit, int restart, int *niter, double *rnorm, int *reason,
-              std::vector<double> *history);
+typedef std::function<int(const double *, double *)> MatvecFn;
+// Restarted GMRES (PETSc KSPGMRES semantics).  matvec_lo (optional): a cheaper, inexact
+// operator that Arnoldi steps use once the residual estimate is below lo_below times
+// the initial residual.
+int rbc_gmres(rbc3d_ctx *c, int n, const MatvecFn &matvec, const double *d_b, double *d_x,
+              bool x0_nonzero, double rtol, double atol, double dtol, int maxit, int restart,
+              int *niter, double *rnorm, int *reason, std::vector<double> *history,
+              const MatvecFn *matvec_lo = nullptr, double lo_below = 0.0);
 int rbc_hash_sort(rbc3d_ctx *c, int n, const double *d_x, int ldx, int *d_bin, int *d_order,
                   int *d_bin_start);

@@ -14,9 +14,16 @@
 
 #define SH_THREADS 256
 
+// The per-cell transforms below sum in the reference's order (longitude DFT, then
+// (2/nlon) sum_i (Pbar w_i) C_i, then the synthesis sums over n and m) with FMA
+// contraction off, so the surface splines built from them equal the CPU restatement's
+// bit for bit: the double-layer self patch subtracts spline positions of magnitude ~L
+// that lie ~0.01 apart, and one ulp there moves a velocity by ~1e-12.
+
 // C[m][i] = sum_j f[i + ld*j] cos(m phi_j),  S[m][i] = sum_j f sin(m phi_j)
 __device__ void lon_dft(const double *f, int ld, int nl, int nn, int mmax, const double *cs,
                         int cs_ld_m, double *C, double *S) {
+#pragma clang fp contract(off)
   const double *cosT = cs, *sinT = cs + (size_t)cs_ld_m * nn;
   for (int t = threadIdx.x; t < mmax * nl; t += blockDim.x) {
     int m = t / nl, i = t % nl;
@@ -31,9 +38,11 @@ __device__ void lon_dft(const double *f, int ld, int nl, int nn, int mmax, const
   }
 }
 
-// a[m + lda*n] = sum_i pw[m][n][i] C[m][i];  b = -sum_i pw S   (n in [m, nmax))
+// a[m + lda*n] = scale sum_i pw[m][n][i] C[m][i];  b = -scale sum_i pw S  (n in [m, nmax)),
+// pw = Pbar(m,n,th_i) w_i and scale = 2/nlon
 __device__ void leg_anal(const double *C, const double *S, const double *pw, int nl, int mmax,
-                         int nmax, double *a, double *b, int lda) {
+                         int nmax, double *a, double *b, int lda, double scale) {
+#pragma clang fp contract(off)
   for (int t = threadIdx.x; t < mmax * nmax; t += blockDim.x) {
     int m = t / nmax, n = t % nmax;
     double sa = 0, sb = 0;
@@ -44,14 +53,15 @@ __device__ void leg_anal(const double *C, const double *S, const double *pw, int
         sb += p[i] * S[m * nl + i];
       }
     }
-    a[m + lda * n] = sa;
-    b[m + lda * n] = -sb;
+    a[m + lda * n] = scale * sa;
+    b[m + lda * n] = -scale * sb;
   }
 }
 
 // G[m][i] = sum_{n=m}^{nmax-1} pt[m][n][i] a[m + lda n]  (pt includes the 1/2 at m = 0)
 __device__ void leg_synth(const double *a, const double *b, int lda, const double *pt, int nlt,
                           int nrow, int mmax, int nmax, double *Ga, double *Gb) {
+#pragma clang fp contract(off)
   for (int t = threadIdx.x; t < mmax * nrow; t += blockDim.x) {
     int m = t / nrow, i = t % nrow;
     double sa = 0, sb = 0;
@@ -68,6 +78,7 @@ __device__ void leg_synth(const double *a, const double *b, int lda, const doubl
 // g[i + ld*j] = sum_m Ga[m][i] cos(m phi_j) - Gb[m][i] sin(m phi_j)
 __device__ void lon_synth(const double *Ga, const double *Gb, int nrow, int nn, int mmax,
                           const double *cs, int cs_ld_m, double *g, int ld) {
+#pragma clang fp contract(off)
   const double *cosT = cs, *sinT = cs + (size_t)cs_ld_m * nn;
   for (int t = threadIdx.x; t < nrow * nn; t += blockDim.x) {
     int i = t % nrow, j = t / nrow;
@@ -85,6 +96,7 @@ __global__ void __launch_bounds__(SH_THREADS) k_build_splines(
     const double *__restrict__ Dn, const double *__restrict__ x, const double *__restrict__ a3,
     const double *__restrict__ detj, const double *__restrict__ fd, int np, int which,
     double *__restrict__ spl, const int *__restrict__ cell_list) {
+#pragma clang fp contract(off)
   extern __shared__ double sm[];
   int ci = cell_list[cell0 + blockIdx.x];
   const CellRec cr = cells[ci];
@@ -129,7 +141,7 @@ __global__ void __launch_bounds__(SH_THREADS) k_build_splines(
     lon_dft(r1, nl, nl, nn, mmax, cs, nl + 1, r2, r2 + mmax * nl);
     __syncthreads();
     // a, b (n < nlat0: ShFilter) in r1, lda = n0
-    leg_anal(r2, r2 + mmax * nl, pw, nl, min(mmax, n0), n0, r1, r1 + n0 * n0, n0);
+    leg_anal(r2, r2 + mmax * nl, pw, nl, min(mmax, n0), n0, r1, r1 + n0 * n0, n0, 2.0 / nn);
     __syncthreads();
     // equi-spaced synthesis on nl+1 rows: Ga, Gb in r2
     int mme = min(n0, min(ne, nn / 2 + 1));
@@ -214,7 +226,7 @@ int rbc_sh_build_splines(rbc3d_ctx *c, int which) {
     const double *fd = which == 1 ? c->d_f.p : which == 2 ? c->d_g.p : nullptr;
     rbc_stage_begin(c, ST_SPLINE);
     hipLaunchKernelGGL(k_build_splines, dim3(n), dim3(SH_THREADS), lds, c->stream, c->d_cells.p,
-                       base, (int)gt, g->d_pw.p, g->d_pe.p, g->d_cs.p, g->d_Dm.p, g->d_Dn.p,
+                       base, (int)gt, g->d_pws.p, g->d_pe.p, g->d_cs.p, g->d_Dm.p, g->d_Dn.p,
                        c->d_x.p, c->d_a3.p, c->d_detj.p, fd, c->np, which, c->d_spl.p,
                        c->d_tmp_i.p);
     rbc_stage_end(c, ST_SPLINE);
@@ -576,7 +588,7 @@ __global__ void __launch_bounds__(SH_THREADS) k_anal_batch(int nl, int nn, int m
   __syncthreads();
   lon_dft(f, nl, nl, nn, mmax, cs, mmax, C, C + mmax * nl);
   __syncthreads();
-  leg_anal(C, C + mmax * nl, pw, nl, mmax, nl, ab, ab + mmax * nl, mmax);
+  leg_anal(C, C + mmax * nl, pw, nl, mmax, nl, ab, ab + mmax * nl, mmax, 2.0 / nn);
   __syncthreads();
   for (int t = threadIdx.x; t < mmax * nl; t += blockDim.x) {
     int m = t % mmax, n = t / mmax;
@@ -623,7 +635,7 @@ static int batch_tables(int nl, int nn, int mmax, bool equi, bool anal, double *
     for (int n = 0; n < nl; n++)
       for (int i = 0; i < nl; i++) {
         size_t o = ((size_t)m * nl + n) * nl + i;
-        P[o] = anal ? (2.0 / nn) * w[i] * P[o] : (m == 0 ? 0.5 : 1.0) * P[o];
+        P[o] = anal ? P[o] * w[i] : (m == 0 ? 0.5 : 1.0) * P[o];
       }
   std::vector<double> cs((size_t)2 * mmax * nn);
   for (int m = 0; m < mmax; m++)

@@ -71,8 +71,11 @@ static TriRules make_rules() {
   return t;
 }
 
-// MinDistToTri (common/ModIntOnWalls.F90:480-577); x[l][d]
+// MinDistToTri (common/ModIntOnWalls.F90:480-577); x[l][d].  Evaluated without FMA
+// contraction, operation for operation as the reference, so the in-cutoff test (and with
+// it the self-matrix sparsity pattern) is the reference's bit for bit.
 __device__ double min_dist_to_tri(const double xt[3], const double x[3][3], double &s0, double &t0) {
+#pragma clang fp contract(off)
   double x12[3], x13[3], x1T[3];
   for (int d = 0; d < 3; d++) {
     x12[d] = x[1][d] - x[0][d];
@@ -275,6 +278,7 @@ __device__ __forceinline__ void hash_bins(const WallArgs &a, const double xi[3],
 
 // translate the triangle next to xi (the shift of vertex 1, :103-108)
 __device__ __forceinline__ void shift_tri(const WallArgs &a, const double xi[3], double x[3][3]) {
+#pragma clang fp contract(off)
   for (int d = 0; d < 3; d++) {
     double sh = round((xi[d] - x[0][d]) * a.iLb[d]) * a.Lb[d];
     for (int l = 0; l < 3; l++) x[l][d] += sh;
