@@ -132,6 +132,11 @@ int rbc3d_sh_synth_gau(int nlat, int nlon, int nt, double *g, int idg, int jdg, 
                        const double *b, int ida, int jda);
 int rbc3d_sh_synth_equ(int nlat, int nlon, int nt, double *g, int idg, int jdg, const double *a,
                        const double *b, int ida, int jda);
+/* ShGradGau (common/ModSphpk.F90:377-417, SPHEREPACK gradgs plus the sin(th) factor the
+ * routine multiplies back): v(idv,jdv,nt) = dg/dth and w = dg/dphi on the Gauss grid of
+ * the scalar field whose shags coefficients are a, b. */
+int rbc3d_sh_grad_gau(int nlat, int nlon, int nt, double *v, double *w, int idv, int jdv,
+                      const double *a, const double *b, int ida, int jda);
 /* Glob_Sph_Trans(v,c,direction) (common/ModVelSolver.F90:641-719); direction
  * 1 = PHYS_TO_FOUR, -1 = FOUR_TO_PHYS.  v(nPoint,3), c(3*sum nlat0^2). */
 int rbc3d_glob_sph_trans(rbc3d_ctx *ctx, double *v, double *c, int direction);

@@ -78,6 +78,8 @@ def lib():
         "orc_sh_synth_equ": (C.c_int, [C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, dp,
                                        C.c_int, C.c_int]),
         "orc_fft_diff": (None, [C.c_int, C.c_int, C.c_int, dp, dp, dp]),
+        "orc_sh_grad_gau": (C.c_int, [C.c_int, C.c_int, C.c_int, dp, dp, C.c_int, C.c_int, dp, dp,
+                                      C.c_int, C.c_int]),
         "orc_min_dist_to_tri": (C.c_double, [dp, dp, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
         "orc_tri_int_regular": (None, [vp, dp, dp, dp, dp]),
         "orc_tri_int_duffy": (None, [vp, dp, dp, dp, C.c_double, C.c_double, dp]),

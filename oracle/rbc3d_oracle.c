@@ -428,6 +428,54 @@ int orc_sh_synth_equ(int nlat, int nlon, int nt, double *g, int idg, int jdg, co
   return r;
 }
 
+/* ShGradGau common/ModSphpk.F90:377-417 (gradgs, then w times sin th): v = dg/dth,
+ * w = dg/dphi on the Gauss grid, from the shags coefficients of g.  dPbar/dth from
+ * (1 - x^2) dP_n^m/dx = (n + m) P_{n-1}^m - n x P_n^m in the normalised functions. */
+int orc_sh_grad_gau(int nlat, int nlon, int nt, double *v, double *w, int idv, int jdv,
+                    const double *a, const double *b, int ida, int jda) {
+  int mmax = nlat < nlon / 2 + 1 ? nlat : nlon / 2 + 1;
+  if (mmax > ida) mmax = ida;
+  int nmax = nlat < jda ? nlat : jda;
+  if (idv < nlat || jdv < nlon) return 1;
+  double *th = malloc(sizeof(double) * nlat), *wt = malloc(sizeof(double) * nlat);
+  orc_gaqd(nlat, th, wt);
+  double *pb = malloc(sizeof(double) * nlat * nlat * mmax);
+  orc_pbar_table(nlat, th, mmax, pb);
+  for (int k = 0; k < nt; k++) {
+    double *vk = v + (size_t)k * idv * jdv, *wk = w + (size_t)k * idv * jdv;
+    const double *ak = a + (size_t)k * ida * jda, *bk = b + (size_t)k * ida * jda;
+    for (int i = 0; i < nlat; i++) {
+      double ct = cos(th[i]), st = sin(th[i]);
+      for (int j = 0; j < nlon; j++) {
+        double ph = TWO_PI * j / nlon, sv = 0, sw = 0;
+        for (int m = 0; m < mmax; m++) {
+          double da = 0, db = 0, pa = 0, pbb = 0, h = m == 0 ? 0.5 : 1.0;
+          for (int n = m; n < nmax; n++) {
+            double p = pb[i + nlat * (n + nlat * m)];
+            double dp = n * ct * p;
+            if (n > m)
+              dp -= sqrt((2.0 * n + 1) * ((double)n * n - (double)m * m) / (2.0 * n - 1)) *
+                    pb[i + nlat * ((n - 1) + nlat * m)];
+            dp = dp / st;
+            da += dp * ak[m + ida * n];
+            db += dp * bk[m + ida * n];
+            pa += p * ak[m + ida * n];
+            pbb += p * bk[m + ida * n];
+          }
+          sv += h * (da * cos(m * ph) - db * sin(m * ph));
+          sw += h * m * (-pa * sin(m * ph) - pbb * cos(m * ph));
+        }
+        vk[i + idv * j] = sv;
+        wk[i + idv * j] = sw;
+      }
+    }
+  }
+  free(th);
+  free(wt);
+  free(pb);
+  return 0;
+}
+
 /* ======================================================================== */
 /* FFT_Diff: common/ModFFT.F90:25-93.  r2c -> multiply by i*k/m (k<m/2,      */
 /* Nyquist zeroed) -> c2r, unnormalised FFTW.  Written as the equivalent real */

@@ -99,6 +99,9 @@ int orc_sh_synth_gau(int nlat, int nlon, int nt, double *g, int idg, int jdg, co
                      const double *b, int ida, int jda);
 int orc_sh_synth_equ(int nlat, int nlon, int nt, double *g, int idg, int jdg, const double *a,
                      const double *b, int ida, int jda);
+/* ShGradGau: v = dg/dth, w = dg/dphi on the Gauss grid from the shags coefficients */
+int orc_sh_grad_gau(int nlat, int nlon, int nt, double *v, double *w, int idv, int jdv,
+                    const double *a, const double *b, int ida, int jda);
 void orc_fft_diff(int m, int n, int nvar, const double *u, double *u1, double *u2);
 double orc_min_dist_to_tri(const double *xtar, const double *x /* x(3,3) */, double *s0, double *t0);
 void orc_tri_int_regular(const orc_sys *s, const double *x, const double *f, const double *xtar,

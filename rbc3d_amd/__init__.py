@@ -79,6 +79,8 @@ _SIG = {
     "rbc3d_sh_anal_gau": [C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, dp, C.c_int, C.c_int],
     "rbc3d_sh_synth_gau": [C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, dp, C.c_int, C.c_int],
     "rbc3d_sh_synth_equ": [C.c_int, C.c_int, C.c_int, dp, C.c_int, C.c_int, dp, dp, C.c_int, C.c_int],
+    "rbc3d_sh_grad_gau": [C.c_int, C.c_int, C.c_int, dp, dp, C.c_int, C.c_int, dp, dp, C.c_int,
+                          C.c_int],
     "rbc3d_glob_sph_trans": [C.c_void_p, dp, dp, C.c_int],
     "rbc3d_cell_spline": [C.c_void_p, C.c_int, C.c_int, dp, dp, dp, dp],
     "rbc3d_calc_velocity_field": [C.c_void_p, C.c_int, dp, ip, ip, dp, dp],
@@ -205,6 +207,17 @@ def sh_synth_equ(nlat, nlon, a, b, nt=1, ida=None, jda=None):
     g = np.zeros(nlat * nlon * nt)
     _call("rbc3d_sh_synth_equ", nlat, nlon, nt, g, nlat, nlon, _f64(a), _f64(b), ida, jda)
     return g
+
+
+def sh_grad_gau(nlat, nlon, a, b, nt=1, ida=None, jda=None):
+    """ShGradGau (common/ModSphpk.F90:377-417): coefficients a, b -> v = dg/dth, w = dg/dphi
+    on the Gauss grid, (nlat, nlon, nt) Fortran-flat."""
+    ida = ida or nlat
+    jda = jda or nlat
+    v = np.zeros(nlat * nlon * nt)
+    w = np.zeros(nlat * nlon * nt)
+    _call("rbc3d_sh_grad_gau", nlat, nlon, nt, v, w, nlat, nlon, _f64(a), _f64(b), ida, jda)
+    return v, w
 
 
 class System:

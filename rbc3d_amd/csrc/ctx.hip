@@ -905,6 +905,30 @@ extern "C" int rbc3d_sh_synth_equ(int nlat, int nlon, int nt, double *g, int idg
   return sh_host(2, nlat, nlon, nt, g, idg, jdg, (double *)a, (double *)b, ida, jda);
 }
 
+int rbc_sh_grad_batch(int nlat, int nlon, int nt, double *d_v, double *d_w, int idv, int jdv,
+                      const double *d_a, const double *d_b, int ida, int jda, hipStream_t st);
+extern "C" int rbc3d_sh_grad_gau(int nlat, int nlon, int nt, double *v, double *w, int idv,
+                                 int jdv, const double *a, const double *b, int ida, int jda) {
+  if (nlat < 2 || nlon < 2 || nt < 1 || idv < nlat || jdv < nlon || !v || !w || !a || !b)
+    return RBC3D_EINVAL;
+  int mmax = std::min(nlat, nlon / 2 + 1);
+  if (ida < mmax || jda < nlat) return RBC3D_EINVAL;
+  size_t ng = (size_t)idv * jdv * nt, na = (size_t)ida * jda * nt;
+  DBuf<double> bv, bw, ba, bb;  // released on every return path
+  RBC_TRY(bv.ensure(ng));
+  RBC_TRY(bw.ensure(ng));
+  RBC_TRY(ba.ensure(na));
+  RBC_TRY(bb.ensure(na));
+  HIP_TRY(hipMemcpy(bv.p, v, ng * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(bw.p, w, ng * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(ba.p, a, na * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(bb.p, b, na * 8, hipMemcpyHostToDevice));
+  RBC_TRY(rbc_sh_grad_batch(nlat, nlon, nt, bv.p, bw.p, idv, jdv, ba.p, bb.p, ida, jda, 0));
+  HIP_TRY(hipMemcpy(v, bv.p, ng * 8, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(w, bw.p, ng * 8, hipMemcpyDeviceToHost));
+  return RBC3D_OK;
+}
+
 extern "C" int rbc3d_cell_dof(rbc3d_ctx *c, int *dof) {
   if (!c || !dof) return RBC3D_EINVAL;
   int s = 0;

@@ -9,6 +9,7 @@
 // as small dense contractions with precomputed tables.
 #include <algorithm>
 #include <cmath>
+#include <vector>
 
 #include "internal.h"
 
@@ -622,6 +623,51 @@ __global__ void __launch_bounds__(SH_THREADS) k_synth_batch(int nl, int nn, int 
   }
 }
 
+// ShGradGau: v = dg/dth = sum dPbar/dth (a cos m phi - b sin m phi), w = dg/dphi =
+// sum m Pbar (-a sin m phi - b cos m phi), from a, b (n < nmax) on the Gauss grid.
+// pd holds dPbar/dth and pt Pbar, both with the 1/2 of m = 0.
+__global__ void __launch_bounds__(SH_THREADS) k_grad_batch(int nl, int nn, int mmax, int nmax,
+                                                         const double *__restrict__ pd,
+                                                         const double *__restrict__ pt,
+                                                         const double *__restrict__ cs, double *v,
+                                                         double *w, int idv, int jdv,
+                                                         const double *a, const double *b,
+                                                         int ida, int jda) {
+  extern __shared__ double sm[];
+  int k = blockIdx.x;
+  double *ab = sm, *G = ab + 2 * mmax * nmax, *gg = G + 2 * mmax * nl;
+  for (int t = threadIdx.x; t < mmax * nmax; t += blockDim.x) {
+    int m = t % mmax, n = t / mmax;
+    ab[t] = n >= m ? a[(size_t)k * ida * jda + m + (size_t)ida * n] : 0.;
+    ab[mmax * nmax + t] = n >= m ? b[(size_t)k * ida * jda + m + (size_t)ida * n] : 0.;
+  }
+  __syncthreads();
+  leg_synth(ab, ab + mmax * nmax, mmax, pd, nl, nl, mmax, nmax, G, G + mmax * nl);
+  __syncthreads();
+  lon_synth(G, G + mmax * nl, nl, nn, mmax, cs, mmax, gg, nl);
+  __syncthreads();
+  for (int t = threadIdx.x; t < nl * nn; t += blockDim.x) {
+    int i = t % nl, j = t / nl;
+    v[(size_t)k * idv * jdv + i + (size_t)idv * j] = gg[t];
+  }
+  __syncthreads();
+  leg_synth(ab, ab + mmax * nmax, mmax, pt, nl, nl, mmax, nmax, G, G + mmax * nl);
+  __syncthreads();
+  // d/dphi of (Ga cos m phi - Gb sin m phi) is (-m Gb) cos m phi - (m Ga) sin m phi
+  for (int t = threadIdx.x; t < mmax * nl; t += blockDim.x) {
+    const double mm = (double)(t / nl), ga = G[t], gb = G[mmax * nl + t];
+    G[t] = -mm * gb;
+    G[mmax * nl + t] = mm * ga;
+  }
+  __syncthreads();
+  lon_synth(G, G + mmax * nl, nl, nn, mmax, cs, mmax, gg, nl);
+  __syncthreads();
+  for (int t = threadIdx.x; t < nl * nn; t += blockDim.x) {
+    int i = t % nl, j = t / nl;
+    w[(size_t)k * idv * jdv + i + (size_t)idv * j] = gg[t];
+  }
+}
+
 static int batch_tables(int nl, int nn, int mmax, bool equi, bool anal, double **d_p, double **d_cs) {
   std::vector<double> th, w, P;
   if (equi) {
@@ -664,6 +710,53 @@ int rbc_sh_anal_batch(int nlat, int nlon, int nt, const double *d_g, int idg, in
   HIP_TRY(hipStreamSynchronize(st));
   (void)hipFree(p);
   (void)hipFree(cs);
+  return RBC3D_OK;
+}
+
+// dPbar/dth on the Gauss colatitudes, [m][n][i] with the 1/2 of m = 0, from
+// (1 - x^2) dP_n^m/dx = (n + m) P_{n-1}^m - n x P_n^m in the normalised functions:
+// dPbar_n^m/dth = (n cos th Pbar_n^m - sqrt((2n+1)(n^2-m^2)/(2n-1)) Pbar_{n-1}^m) / sin th
+int rbc_sh_grad_batch(int nlat, int nlon, int nt, double *d_v, double *d_w, int idv, int jdv,
+                      const double *d_a, const double *d_b, int ida, int jda, hipStream_t st) {
+  int mmax = std::min(std::min(nlat, nlon / 2 + 1), ida);
+  int nmax = std::min(nlat, jda);
+  std::vector<double> th, wt, P;
+  host_gauss(nlat, th, wt);
+  host_pbar(nlat, mmax, th, P);
+  std::vector<double> pd(P.size()), pt(P.size());
+  for (int m = 0; m < mmax; m++)
+    for (int n = 0; n < nlat; n++)
+      for (int i = 0; i < nlat; i++) {
+        size_t o = ((size_t)m * nlat + n) * nlat + i;
+        double d = (double)n * cos(th[i]) * P[o];
+        if (n > m) {
+          const double nn = n, mm = m;
+          d -= sqrt((2 * nn + 1) * (nn * nn - mm * mm) / (2 * nn - 1)) * P[o - nlat];
+        }
+        const double h = m == 0 ? 0.5 : 1.0;
+        pd[o] = h * (d / sin(th[i]));
+        pt[o] = h * P[o];
+      }
+  std::vector<double> cs((size_t)2 * mmax * nlon);
+  for (int m = 0; m < mmax; m++)
+    for (int j = 0; j < nlon; j++) {
+      double ph = RBC_TWO_PI * j / nlon;
+      cs[(size_t)m * nlon + j] = cos(m * ph);
+      cs[(size_t)mmax * nlon + m * nlon + j] = sin(m * ph);
+    }
+  size_t lds = sizeof(double) * (2 * (size_t)mmax * nmax + 2 * (size_t)mmax * nlat + (size_t)nlat * nlon);
+  if (lds > 160 * 1024) return RBC3D_EINVAL;
+  DBuf<double> dpd, dpt, dcs;
+  RBC_TRY(dpd.ensure(pd.size()));
+  RBC_TRY(dpt.ensure(pt.size()));
+  RBC_TRY(dcs.ensure(cs.size()));
+  HIP_TRY(hipMemcpy(dpd.p, pd.data(), pd.size() * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dpt.p, pt.data(), pt.size() * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dcs.p, cs.data(), cs.size() * 8, hipMemcpyHostToDevice));
+  hipLaunchKernelGGL(k_grad_batch, dim3(nt), dim3(SH_THREADS), lds, st, nlat, nlon, mmax, nmax,
+                     dpd.p, dpt.p, dcs.p, d_v, d_w, idv, jdv, d_a, d_b, ida, jda);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(st));
   return RBC3D_OK;
 }
 

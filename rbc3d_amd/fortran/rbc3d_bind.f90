@@ -30,7 +30,8 @@ module rbc3d_bind
        rbc3d_set_wall, rbc3d_set_wall_density, rbc3d_prepare_sing_int_on_wall, &
        rbc3d_hashtable_build, rbc3d_add_int_on_rbcs, rbc3d_add_int_on_walls, &
        rbc3d_pme_distrib_source, rbc3d_pme_transform, rbc3d_pme_add_interp_vel, &
-       rbc3d_sh_anal_gau, rbc3d_sh_synth_gau, rbc3d_sh_synth_equ, rbc3d_glob_sph_trans, &
+       rbc3d_sh_anal_gau, rbc3d_sh_synth_gau, rbc3d_sh_synth_equ, rbc3d_sh_grad_gau, &
+       rbc3d_glob_sph_trans, &
        rbc3d_calc_velocity_field, rbc3d_cell_dof, rbc3d_cell_matmult, rbc3d_cell_rhs, &
        rbc3d_wall_dof, rbc3d_wall_matmult, rbc3d_wall_residual_vel, rbc3d_get_wall_density, &
        rbc3d_noslip_wall, rbc3d_solve_rbc_vel, rbc3d_pair_stats, rbc3d_set_precision, &
@@ -158,6 +159,13 @@ module rbc3d_bind
       import :: c_int, c_double
       integer(c_int), value :: nlat, nlon, nt, idg, jdg, ida, jda
       real(c_double), intent(out) :: g(*)
+      real(c_double), intent(in) :: a(*), b(*)
+    end function
+    integer(c_int) function rbc3d_sh_grad_gau(nlat, nlon, nt, v, w, idv, jdv, a, b, ida, jda) &
+         bind(C, name="rbc3d_sh_grad_gau")
+      import :: c_int, c_double
+      integer(c_int), value :: nlat, nlon, nt, idv, jdv, ida, jda
+      real(c_double), intent(out) :: v(*), w(*)
       real(c_double), intent(in) :: a(*), b(*)
     end function
     integer(c_int) function rbc3d_glob_sph_trans(ctx, v, c, direction) &

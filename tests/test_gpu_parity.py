@@ -78,6 +78,25 @@ def test_sh_transforms(R, orc, nlat):
     assert relerr(a2[mask], af[mask]) < 1e-12
 
 
+@pytest.mark.parametrize("nlat", [13, 17])
+def test_sh_gradient(R, orc, nlat):
+    """ShGradGau on the GPU against the oracle (three fields at once)."""
+    rng = np.random.default_rng(nlat + 7)
+    nlon, nt = 2 * nlat, 3
+    a = np.zeros((nlat, nlat, nt))
+    b = np.zeros((nlat, nlat, nt))
+    for k in range(nt):
+        a[:, :, k] = np.triu(rng.standard_normal((nlat, nlat)))
+        b[:, :, k] = np.triu(rng.standard_normal((nlat, nlat)))
+    af = np.ascontiguousarray(a.reshape(-1, order="F"))
+    bf = np.ascontiguousarray(b.reshape(-1, order="F"))
+    vg, wg = R.sh_grad_gau(nlat, nlon, af, bf, nt=nt)
+    vo = np.zeros(nlat * nlon * nt)
+    wo = np.zeros(nlat * nlon * nt)
+    orc.L().orc_sh_grad_gau(nlat, nlon, nt, vo, wo, nlat, nlon, af, bf, nlat, nlat)
+    assert relerr(vg, vo) < TOL and relerr(wg, wo) < TOL
+
+
 def test_surface_splines(c1pair):
     """The surface spline tables (values and the three derivative tables of x, a3,
     detJ, f detJ, g detJ) are the oracle's bit for bit: the device sums in the same order

@@ -90,6 +90,45 @@ def test_sh_roundtrip_and_synthesis_formula(L, nlat):
     assert np.abs(b2[iu] - b[iu]).max() < 1e-13
 
 
+@pytest.mark.parametrize("nlat", [13, 17])
+def test_sh_gradient_vs_finite_differences(L, nlat):
+    """ShGradGau (gradgs plus the sin th factor): v = dg/dth and w = dg/dphi of the
+    band-limited field.  Reference: the same coefficients synthesised with scipy's lpmv
+    (normalised, no Condon-Shortley phase) at th +- h and phi +- h, central differences."""
+    nlon = 2 * nlat
+    rng = np.random.default_rng(nlat + 1)
+    a, b = _coeffs(rng, nlat)
+    th = np.zeros(nlat)
+    wg = np.zeros(nlat)
+    L.orc_gaqd(nlat, th, wg)
+    ph = 2 * np.pi * np.arange(nlon) / nlon
+    af = np.ascontiguousarray(a.reshape(-1, order="F"))
+    bf = np.ascontiguousarray(b.reshape(-1, order="F"))
+    v = np.zeros(nlat * nlon)
+    w = np.zeros(nlat * nlon)
+    assert L.orc_sh_grad_gau(nlat, nlon, 1, v, w, nlat, nlon, af, bf, nlat, nlat) == 0
+    v = v.reshape((nlat, nlon), order="F")
+    w = w.reshape((nlat, nlon), order="F")
+
+    def synth(t, p):
+        T, Pp = np.meshgrid(t, p, indexing="ij")
+        g = np.zeros_like(T)
+        for m in range(nlat):
+            for n in range(m, nlat):
+                nb = math.sqrt((2 * n + 1) / 2 * factorial(n - m) / factorial(n + m)) * (-1) ** m
+                P = nb * lpmv(m, n, np.cos(T))
+                h = 0.5 if m == 0 else 1.0
+                g += h * P * (a[m, n] * np.cos(m * Pp) - b[m, n] * np.sin(m * Pp))
+        return g
+
+    h = 1e-5
+    v_fd = (synth(th + h, ph) - synth(th - h, ph)) / (2 * h)
+    w_fd = (synth(th, ph + h) - synth(th, ph - h)) / (2 * h)
+    scale = max(np.abs(v_fd).max(), np.abs(w_fd).max())
+    assert np.abs(v - v_fd).max() < 1e-7 * scale
+    assert np.abs(w - w_fd).max() < 1e-7 * scale
+
+
 def test_sh_equi_synthesis_poles(L):
     rng = np.random.default_rng(3)
     nlat, nlon = 14, 26
