@@ -13,7 +13,8 @@ library's stream; `cpu_baseline` times the CPU restatement of the same algorithm
 (oracle/) on this host on a bounded sample.  Beside it: `chosen` (the split-precision
 form of the same matvec and its error), `timestep` (the 200-cell tube timestep, GMRES
 seconds per timestep in the plain and chosen forms, with its own CPU baseline and the
-learned warm start's series) and, with --c5, the 4,096-cell mixed batch.
+learned warm start's series, and the factors each layer of the chosen form contributes)
+and the 4,096-cell mixed batch (C5).
 """
 from __future__ import annotations
 
@@ -42,11 +43,33 @@ K_PAIR_C2_TRAFFIC_BYTES = int((6029.4 + 37.4) * 1e6)
 
 
 def host_threads():
+    """The host threads this process may use: OMP_NUM_THREADS when set (the GPU box sets
+    it to its per-GPU CPU share), else the CPUs in the affinity mask."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
     try:
-        n = len(os.sched_getaffinity(0))
+        return max(1, len(os.sched_getaffinity(0)))
     except AttributeError:
-        n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+        return os.cpu_count() or 1
+
+
+def host_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": aff,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS")}
 
 
 def cpu_baseline(w, sample_every=32, threads=None):
@@ -195,27 +218,47 @@ def timestep_bench(steps, warmup, maxit=4000, forms=None):
         out["chosen_learned"] = {"skipped": f"no trained network at {MLP_C3}"}
     elif only is None:
         out["learned_series"] = learned_series_bench(mlp)
+    # what each layer of the chosen form contributes, as ratios of seconds per timestep
+    def sec(f):
+        r = out.get(f, {})
+        return r.get("s_per_timestep") if isinstance(r, dict) else None
+    fac = {}
+    if sec("fp64_restart150"):
+        fac["restart_150_vs_30_fp64"] = round(plain["s_per_timestep"] / sec("fp64_restart150"), 3)
+    if sec("fp64_restart150") and sec("chosen"):
+        fac["precision_split_vs_fp64_at_150"] = round(sec("fp64_restart150") / sec("chosen"), 3)
+    if sec("split"):
+        fac["precision_split_vs_fp64_at_30"] = round(plain["s_per_timestep"] / sec("split"), 3)
+    if sec("chosen") and sec("chosen_learned"):
+        fac["learned_warm_start_vs_chosen"] = round(sec("chosen") / sec("chosen_learned"), 3)
+    if sec("chosen"):
+        fac["chosen_vs_plain"] = round(plain["s_per_timestep"] / sec("chosen"), 3)
+    out["factors"] = fac
     # the headline fields (plain FP64, the reference's arithmetic)
     out["s_per_timestep"] = plain["s_per_timestep"]
     out["gmres_iterations"] = plain["gmres_iterations"]
     return out
 
 
-def timestep_cpu_baseline(niter, threads=None):
+def timestep_cpu_baseline(niter, threads=None, o=None):
     """The oracle's tube timestep on this host, from a bounded sample: one Solve_RBC_Vel
     and two applications of the no-slip operator timed, the timestep estimated as two
     Solve_RBC_Vel plus `niter` (the plain GMRES(30) count) operator applications; the
-    wall self-matrix assembly is reported beside it, not counted (it is done once)."""
+    wall self-matrix assembly is reported beside it, not counted (it is done once, here
+    on every host thread).  Pass `o` to reuse a prepared oracle."""
     from oracle import oracle_py
     from rbc3d_amd import configs
     threads = threads or host_threads()
+    t_prep = None
+    if o is None:
+        oracle_py.Oracle.threads(host_threads())
+        w = configs.make_workload("C3", seed=1)
+        o = oracle_py.from_workload(w)
+        t0 = time.perf_counter()
+        o.prepare_wall()
+        t_prep = time.perf_counter() - t0
     oracle_py.Oracle.threads(threads)
-    w = configs.make_workload("C3", seed=1)
-    o = oracle_py.from_workload(w)
-    t0 = time.perf_counter()
-    o.prepare_wall()
-    t_prep = time.perf_counter() - t0
-    o.set_wall_density(np.zeros(3 * w.wall[2]))
+    o.set_wall_density(np.zeros(3 * oracle_py.L().orc_ntargets_wall(o.h)))
     t0 = time.perf_counter()
     o.solve_rbc_vel()
     t_rhs = time.perf_counter() - t0
@@ -230,7 +273,7 @@ def timestep_cpu_baseline(niter, threads=None):
                       f"two no-slip operator applications timed; timestep = 2 x Solve_RBC_Vel "
                       f"+ {niter} operator applications (the plain GMRES(30) count)",
             "solve_rbc_vel_s": round(t_rhs, 3), "wall_matvec_s": round(t_mv, 3),
-            "wall_self_matrix_assembly_s": round(t_prep, 3)}
+            "wall_self_matrix_assembly_s": None if t_prep is None else round(t_prep, 3)}, o
 
 
 def learned_series_bench(mlp, seed=201, steps=4):
@@ -366,8 +409,8 @@ def main():
     ap.add_argument("--no-timestep", action="store_true",
                     help="skip the tube timestep (GMRES seconds per timestep)")
     ap.add_argument("--timestep-steps", type=int, default=3)
-    ap.add_argument("--c5", action="store_true",
-                    help="also time the 4,096-cell mixed-size matvec (C5)")
+    ap.add_argument("--no-c5", action="store_true",
+                    help="skip the 4,096-cell mixed-size matvec (C5)")
     ap.add_argument("--timestep-forms", default=None,
                     help="comma list of timestep forms beside plain (default: all, and the "
                          "learned series)")
@@ -461,8 +504,9 @@ def main():
     pair_ms, pair_n = stages["pair"]
     pair_avg_ms = pair_ms / max(pair_n, 1)
     pair_tflops = 28.0 * npairs / (pair_avg_ms * 1e-3) / 1e12
-    roofline = {"kernel": "k_pair (double-layer real-space pair sums)", "bound": "mfma",
-                "compute": "fp64 VALU (same 78.6 TFLOP/s peak as fp64 matrix)",
+    roofline = {"kernel": "k_pair (double-layer real-space pair sums)", "bound": "fp64-valu",
+                "compute": "FP64 vector ALU (no matrix shape; MI355X FP64 vector peak 78.6 "
+                           "TFLOP/s)",
                 "achieved": round(pair_tflops, 3), "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": round(pair_tflops / PEAK_FP64_TFLOPS, 4),
                 "traffic": K_PAIR_C2_TRAFFIC_BYTES if args.config == "C2" else None,
@@ -521,23 +565,41 @@ def main():
             "chosen_split_gmres150": ch.get("s_per_timestep"),
             "speedup": ch.get("speedup_vs_plain"),
             "velocity_rel_err": ch.get("velocity_rel_err_vs_plain")}
-    if args.c5:
+    if not args.no_c5:
         out["C5"] = c5_bench(3, 1)
     if not args.no_cpu:
-        tot, parts, thr = cpu_baseline(w, args.sample_every)
+        thr = host_threads()
+        tot, parts, thr = cpu_baseline(w, args.sample_every, thr)
+        one_every = 8 * args.sample_every
+        tot1, parts1, _ = cpu_baseline(w, one_every, 1)
         out["cpu_baseline"] = {"value": round(ncell / tot, 4), "unit": "cell-matvecs/s",
                                "cores": thr, "kind": "port",
                                "sample": f"oracle (C restatement, OpenMP {thr} threads): SH, "
                                f"splines and PME in full; real-space stage on every "
                                f"{args.sample_every}th target, scaled by {args.sample_every}",
                                "s_per_matvec": round(tot, 3),
-                               "parts_s": {k: round(v, 3) for k, v in parts.items()}}
+                               "parts_s": {k: round(v, 3) for k, v in parts.items()},
+                               "host": host_info(),
+                               "one_thread": {
+                                   "value": round(ncell / tot1, 5), "cores": 1,
+                                   "s_per_matvec": round(tot1, 3),
+                                   "sample": f"the same on 1 thread (one MPI rank's share), "
+                                             f"real-space stage on every {one_every}th target, "
+                                             f"scaled by {one_every}",
+                                   "parts_s": {k: round(v, 3) for k, v in parts1.items()}},
+                               "gpu_speedup": round(out["value"] / (ncell / tot), 1),
+                               "gpu_speedup_vs_one_thread": round(out["value"] / (ncell / tot1), 1)}
         if "timestep" in out:
             ts = out["timestep"]
-            cb = timestep_cpu_baseline(int(np.mean(ts["plain"]["gmres_iterations"])), thr)
+            niter = int(np.mean(ts["plain"]["gmres_iterations"]))
+            cb, o3 = timestep_cpu_baseline(niter, thr)
+            cb1, _ = timestep_cpu_baseline(niter, 1, o3)
+            cb["one_thread"] = cb1
+            cb["host"] = host_info()
             ts["cpu_baseline"] = cb
-            ts["plain"]["speedup_vs_cpu"] = round(cb["value"] / ts["plain"]["s_per_timestep"], 1)
-            ts["chosen"]["speedup_vs_cpu"] = round(cb["value"] / ts["chosen"]["s_per_timestep"], 1)
+            for f in ("plain", "chosen"):
+                ts[f]["speedup_vs_cpu"] = round(cb["value"] / ts[f]["s_per_timestep"], 1)
+                ts[f]["speedup_vs_cpu_one_thread"] = round(cb1["value"] / ts[f]["s_per_timestep"], 1)
     print(json.dumps(out))
 
 

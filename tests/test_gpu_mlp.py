@@ -50,7 +50,8 @@ def test_forward_matches_oracle(dev):
     yo = M.predict(p, mean, std, scale, X)
     err = np.abs(yg - yo).max() / np.abs(yo).max()
     print(f"mlp forward vs oracle: max rel err {err:.2e}")
-    assert err < 1e-2
+    # measured 2.8e-4 (BF16 activations rounded at the same points, FP32 vs FP64 sums)
+    assert err < 3e-3
     assert np.median(np.abs(yg - yo)) / np.abs(yo).max() < 1e-4
 
 
@@ -69,13 +70,14 @@ def test_gradient_matches_oracle(dev):
     rel = np.linalg.norm(gg - go) / np.linalg.norm(go)
     print(f"mlp gradient vs oracle: rel L2 err {rel:.2e}, loss {loss:.6e} vs {lo:.6e}")
     assert abs(loss - lo) <= 1e-4 * lo
-    assert rel < 1e-2
+    # measured 9.3e-6
+    assert rel < 1e-4
     # every parameter block, each against its own scale (catches a misplaced block)
     o = 0
     for name, shp in M.SHAPES:
         k = int(np.prod(shp))
         a, b = gg[o:o + k], go[o:o + k]
-        assert np.linalg.norm(a - b) <= 2e-2 * np.linalg.norm(b) + 1e-12, name
+        assert np.linalg.norm(a - b) <= 1e-3 * np.linalg.norm(b) + 1e-12, name
         o += k
 
 
