@@ -512,8 +512,7 @@ extern "C" int rbc3d_noslip_wall(rbc3d_ctx *c, double rtol, int maxit, const dou
 int rbc_mlp_forward(rbc3d_mlp *m, int n, const float *d_x, float *d_y, hipStream_t st);
 int rbc_mlp_wait(rbc3d_mlp *m, hipStream_t st);
 int rbc_wall_cell_vel_dev(rbc3d_ctx *c, double *d_out);
-int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double c1, int nT, const double *d_tx, const int *d_tindx,
-                             const int *d_act, const double *d_A, double *d_v, bool wall_targets);
+
 
 __global__ void k_mlp_features(int nv, const double *__restrict__ uc, const double *__restrict__ f0,
                                const double *__restrict__ vn, float *X) {
@@ -630,7 +629,7 @@ extern "C" int rbc3d_noslip_wall_learned(rbc3d_ctx *c, rbc3d_mlp *m, double rtol
                                    c->d_v.p, true));
   RBC_TRY(rbc_pme_spread(c, c1, 0., 0, 1));
   RBC_TRY(rbc_pme_transform(c));
-  RBC_TRY(rbc_pme_gather(c, nv, c->d_tx.p, c->d_tact.p, c->d_tA.p, c->d_v.p));
+  RBC_TRY(rbc_pme_gather_wall(c, c->d_tA.p, c->d_v.p));
   k_add3<<<(nv + 255) / 256, 256, 0, st>>>(nv, c->d_v.p, uc.p, c->conf.vBkg[0], c->conf.vBkg[1],
                                            c->conf.vBkg[2]);
   k_neg_assemble<<<(nu + 255) / 256, 256, 0, st>>>(nu, nv, w.d_glb.p + nv, c->d_v.p, rhs.p);

@@ -149,17 +149,20 @@ struct WallData {
   DBuf<int> d_tg_bin, d_tg_order, d_tg_start, d_tg_chunk;
   // self matrix CSR (rows i + k nvert)
   DBuf<int> d_rowptr, d_col;
-  DBuf<double> d_val;
-  DBuf<float> d_valf;  // split form: the 3x3 blocks in FP32
+  DBuf<double> d_val;   // symmetric 3x3 blocks: B00, B11, B22, B01, B02, B12 arrays
+  DBuf<float> d_valf;  // split form: the same in FP32
   long long nnz = 0;
   bool prepared = false, valf_ready = false;
-  // the wall-only single-layer PME spread as a fixed sparse map (the centroids do not
-  // move): per grid point (CSR over all points) the source triangles and weights
-  DBuf<int> d_wsp_ptr, d_wsp_src;
-  DBuf<double> d_wsp_w, d_wsp_f;
-  DBuf<float> d_wsp_wf;
-  DBuf<unsigned char> d_wsp_row;  // per entry: its point's index within its group of 64
+  // wall-only spread: per 4^3 brick the sources reaching it and their brick-local
+  // stencil origins (CSR), per source its B-spline weights (fixed for a wall and
+  // grid), and the per-source values of the current spread
+  DBuf<int> d_wsb_start, d_wsb_src, d_wsb_rel;
+  DBuf<double> d_wsb_w, d_wsb_f;
   bool wsp_ready = false;
+  // the gather at the vertices: per vertex its stencil origin and B-spline weights
+  DBuf<int> d_wg_org;
+  DBuf<double> d_wg_w;
+  bool wgt_ready = false;
   // images of the three uniform tractions under the no-slip operator (learned warm
   // start), for the precision mode they were computed in
   DBuf<double> d_lmodes;
@@ -172,9 +175,14 @@ struct rbc3d_ctx {
   double Lb[3], iLb[3];
   double Acoef[8], Bcoef[8];
   hipStream_t stream = nullptr;
+  // second stream (default priority) on which the no-slip operator's self-matrix SpMV
+  // overlaps the PME chain of the main stream; the fork and join events
+  hipStream_t side = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int prec = RBC3D_PREC_FP64;  // rbc3d_set_precision
   int gmres_restart = 30;      // rbc3d_set_gmres_restart (PETSc's default length)
   int ncu = 256;               // compute units of the device
+  int lds_max = 65536;         // LDS bytes one workgroup may allocate on the device
   // tables
   DBuf<double> d_sl1, d_sl2, d_dl1, d_mtab;
   DBuf<float> d_tabf;  // split form: sl1, sl2, dl1 in FP32, (RBC_NTAB + 1) each
@@ -462,8 +470,9 @@ __device__ __forceinline__ double dist_on_sphere(double th0, double phi0, double
   return acos(d);
 }
 
-void rbc_stage_begin(rbc3d_ctx *c, int stage);
-void rbc_stage_end(rbc3d_ctx *c, int stage);
+// stage timers: events recorded on stream st (the context's stream when null)
+void rbc_stage_begin(rbc3d_ctx *c, int stage, hipStream_t st = nullptr);
+void rbc_stage_end(rbc3d_ctx *c, int stage, hipStream_t st = nullptr);
 void rbc_stage_flush(rbc3d_ctx *c);
 
 // Kernel launchers (defined in the .hip files)
@@ -481,12 +490,16 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
 int rbc_pme_init(rbc3d_ctx *c);
 int rbc_pme_spread(rbc3d_ctx *c, double c1, double c2, int cells, int walls);
 int rbc_pme_transform(rbc3d_ctx *c);
+// the gather at the wall's vertices (targets as set by the wall target list: all active)
+int rbc_pme_gather_wall(rbc3d_ctx *c, const double *d_A, double *d_v);
 int rbc_pme_gather(rbc3d_ctx *c, int nT, const double *d_tx, const int *d_act, const double *d_A,
                    double *d_v);
 int rbc_wall_setup(rbc3d_ctx *c);
 int rbc_wall_prepare(rbc3d_ctx *c);
+// (on: the stream to run on, the context's stream when null)
 int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double c1, int nT, const double *d_tx, const int *d_tindx,
-                             const int *d_act, const double *d_A, double *d_v, bool wall_targets);
+                             const int *d_act, const double *d_A, double *d_v, bool wall_targets,
+                             hipStream_t on = nullptr);
 typedef std::function<int(const double *, double *)> MatvecFn;
 // Restarted GMRES (PETSc KSPGMRES semantics).  matvec_lo (optional): a cheaper, inexact
 // operator that Arnoldi steps use once the residual estimate is below lo_below times

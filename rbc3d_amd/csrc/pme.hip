@@ -23,6 +23,12 @@
 #define SPR_B 8        // grid brick edge
 #define SPR_T 256
 #define FFT_T 256
+#ifndef RBC_XY_LDS
+#define RBC_XY_LDS (80 * 1024)
+#endif
+#ifndef RBC_ZG_LDS
+#define RBC_ZG_LDS (40 * 1024)
+#endif
 #define FFT_ELEMS 2048 // complex elements per FFT workgroup (lines x length)
 
 template <typename T>
@@ -37,24 +43,10 @@ struct CxT<float> {
 };
 
 // ---------------------------------------------------------------------------
-// B-spline weights: BsplineFunc (common/ModBasicMath.F90:392-417)
-__device__ __forceinline__ int bspline_d(double xc, int P, double *w) {
+// B-spline weights: BsplineFunc (common/ModBasicMath.F90:392-417), on the host (tables,
+// the wall spread's fixed weights) and the device (sources that move)
+__host__ __device__ inline int bspline_d(double xc, int P, double *w) {
   double u[12];
-  int imin = (int)floor(xc) - (P - 1);
-  u[0] = imin - (xc - P);
-  for (int j = 1; j < P; j++) u[j] = u[j - 1] + 1;
-  w[0] = 1.;
-  for (int j = 1; j < P; j++) w[j] = 0.;
-  for (int pp = 2; pp <= P; pp++) {
-    for (int j = pp; j >= 2; j--)
-      w[j - 1] = u[j - 1] / (pp - 1.) * w[j - 1] + (pp - u[j - 1]) / (pp - 1.) * w[j - 2];
-    w[0] = u[0] / (pp - 1.) * w[0];
-  }
-  return imin;
-}
-
-static int host_bspline(double xc, int P, double *w) {
-  double u[16];
   int imin = (int)floor(xc) - (P - 1);
   u[0] = imin - (xc - P);
   for (int j = 1; j < P; j++) u[j] = u[j - 1] + 1;
@@ -116,7 +108,7 @@ int rbc_pme_init(rbc3d_ctx *c) {
   RBC_TRY(c->d_vgrid.ensure(3 * ng));
   // B-factor per axis (PME_Init :309-336); bb = ((1*bx)*by)*bz, bb(0,0,0) = 0
   double MP[16];
-  host_bspline(P + 2.220446049250313e-16, P, MP);
+  bspline_d(P + 2.220446049250313e-16, P, MP);
   std::vector<double> fac((size_t)H + N2 + N3);
   int Ns[3] = {N1, N2, N3}, cnt[3] = {H, N2, N3}, off = 0;
   for (int ax = 0; ax < 3; ax++) {
@@ -429,173 +421,190 @@ __global__ void k_spread_merge(SpreadArgs<T> a, const T *__restrict__ tiles) {
   a.grid[o + 2 * ng] = s2;
 }
 
-// Wall-only single-layer spread (every no-slip GMRES iteration) as a fixed sparse
-// map: the sources are the wall triangles' centroids, which do not move, so their
-// stencil origins and B-spline weights are computed once per wall and grid and kept
-// as a CSR matrix over the grid points (entries sorted by point, then by triangle and
-// stencil offset: the summation order at a point is fixed).  Each spread is then one
-// pass over the entries: grid[g] = c1 sum_k w_k fbar(src_k) area(src_k).
-__global__ void k_wsp_keys(int nele, const double *__restrict__ cen, double ih0, double ih1,
-                           double ih2, int N1, int N2, int N3, int P, int *key, int *val) {
-  const long long t = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  const int P3 = P * P * P;
-  if (t >= (long long)nele * P3) return;
-  const int e = (int)(t / P3), s = (int)(t % P3);
-  const int di = s % P, dj = (s / P) % P, dk = s / (P * P);
-  double w[12];
-  const int i0 = bspline_d(cen[e] * ih0, P, w);
-  const int j0 = bspline_d(cen[e + (size_t)nele] * ih1, P, w);
-  const int k0 = bspline_d(cen[e + 2 * (size_t)nele] * ih2, P, w);
-  const int gi = imod_d(i0 + di, N1), gj = imod_d(j0 + dj, N2), gk = imod_d(k0 + dk, N3);
-  key[t] = gi + N1 * (gj + N2 * gk);
-  val[t] = (int)t;
-}
-__global__ void k_wsp_weights(long long ne, const int *__restrict__ skey, const int *__restrict__ sval,
-                              const double *__restrict__ cen, int nele, double ih0, double ih1,
-                              double ih2, int P, int *src, double *wgt, float *wgtf,
-                              unsigned char *row64) {
-  const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-  if (k >= ne) return;
-  row64[k] = (unsigned char)(skey[k] & 63);  // the entry's row within its group of 64
-  const int P3 = P * P * P, t = sval[k];
-  const int e = t / P3, s = t % P3;
-  const int di = s % P, dj = (s / P) % P, dk = s / (P * P);
-  double wx[12], wy[12], wz[12];
-  bspline_d(cen[e] * ih0, P, wx);
-  bspline_d(cen[e + (size_t)nele] * ih1, P, wy);
-  bspline_d(cen[e + 2 * (size_t)nele] * ih2, P, wz);
-  const double w = wx[di] * wy[dj] * wz[dk];
-  src[k] = e;
-  wgt[k] = w;
-  wgtf[k] = (float)w;
-}
-// row pointers: ptr[g] = first sorted entry with point >= g
-__global__ void k_wsp_ptr(long long ne, const int *__restrict__ skey, int ng, int *ptr) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
-  if (g > ng) return;
-  long long lo = 0, hi = ne;
-  while (lo < hi) {
-    const long long mid = (lo + hi) >> 1;
-    if (skey[mid] < g)
-      lo = mid + 1;
-    else
-      hi = mid;
-  }
-  ptr[g] = (int)lo;
-}
+// Wall-only single-layer spread (every no-slip GMRES iteration).  The sources are the
+// wall triangles' centroids, which do not move: once per wall and grid, each source's
+// stencil origin and B-spline weights are computed, and each 4^3 brick of the grid gets
+// the list of sources whose stencil reaches it, in ascending source order, with the
+// stencil origin relative to the brick.  A spread is one wavefront per brick, a lane
+// per grid point.  Per round of 64 sources each lane stages one source's weights on
+// the brick's 4 x 4 (x, y) points (wx wy, zero off the stencil), on its 4 z planes, and
+// its value; then every lane walks the round's sources in order and adds
+// c1 ((wx wy) wz) f to its point.  Every point adds its sources in ascending order with
+// the reference's expressions (PME_Distrib_Source :108-130, Distrib_Source :405-443)
+// and no FMA contraction: the grid is the CPU restatement's bit for bit (a point off a
+// source's stencil adds an exact zero).  Bricks without sources write zeros.
+#define WSB 4          // brick edge in grid points (a brick is 64 points)
+#define WSB_CH 64      // sources staged per round
+#define WSB_PMAX 8
+template <typename T>
+struct WspArgs {
+  int N1, N2, N3, P, nbx, nby, nbz, nele, nvert, nbr;
+  double c1;
+  const int *bstart, *bsrc, *brel;  // per brick: its sources, their origins (packed)
+  const double *wts;                // per source 3 P weights
+  const int *e2v;
+  const double *wf, *area;
+  double *fs;  // per-source values f(nele, 3)
+  T *grid;
+};
+
 // per-triangle source values: mean vertex traction times area (PME_Distrib_Source
-// :108-130), f(3, nele)
-__global__ void k_wsp_src(int nele, const int *__restrict__ e2v, int nvert,
-                          const double *__restrict__ wf, const double *__restrict__ area,
-                          double *fsrc) {
+// :108-130), fs(nele, 3)
+template <typename T>
+__global__ void k_wsp_src(WspArgs<T> a) {
+#pragma clang fp contract(off)
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nele) return;
-  const int v0 = e2v[e] - 1, v1 = e2v[e + nele] - 1, v2 = e2v[e + 2 * (size_t)nele] - 1;
+  if (e >= a.nele) return;
+  const int v0 = a.e2v[e] - 1, v1 = a.e2v[e + a.nele] - 1, v2 = a.e2v[e + 2 * (size_t)a.nele] - 1;
   for (int d = 0; d < 3; d++) {
-    const double f = (1.0 / 3) * (wf[v0 + (size_t)nvert * d] + wf[v1 + (size_t)nvert * d] +
-                                  wf[v2 + (size_t)nvert * d]) * area[e];
-    fsrc[e + (size_t)nele * d] = f;
+    const size_t o = (size_t)a.nvert * d;
+    a.fs[e + (size_t)a.nele * d] = (1.0 / 3) * (a.wf[v0 + o] + a.wf[v1 + o] + a.wf[v2 + o]) * a.area[e];
   }
 }
-// One wavefront per 64 consecutive grid points (rows): their entries are one
-// contiguous range, which the lanes take 64 at a time (coalesced index and weight
-// loads, 64 independent source gathers); a segmented inclusive scan over the lanes
-// (keyed by row; entries are sorted by row) sums each row's entries of the step, and
-// the last lane of each row's segment adds that partial to the row's LDS accumulator.
-// The additions happen in a fixed pattern: deterministic.
-template <typename T, typename W>
-__global__ void __launch_bounds__(256) k_wsp_spread(int ng, int nele, double c1,
-                                                    const int *__restrict__ ptr,
-                                                    const int *__restrict__ src,
-                                                    const W *__restrict__ wgt,
-                                                    const unsigned char *__restrict__ row64,
-                                                    const double *__restrict__ fsrc, T *grid) {
-  __shared__ double sacc[4][3][64];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int g0 = (blockIdx.x * 4 + wv) * 64;
-  const int nr = g0 < ng ? min(64, ng - g0) : 0;
-  for (int d = 0; d < 3; d++) sacc[wv][d][lane] = 0.;
-  __syncthreads();
-  const int k0 = nr > 0 ? ptr[g0] : 0, k1 = nr > 0 ? ptr[g0 + nr] : 0;
-  for (int kb = k0; kb < k1; kb += 64) {
-    const int k = kb + lane;
-    const bool live = k < k1;
-    int r = 0;
-    double v0 = 0., v1 = 0., v2 = 0.;
-    if (live) {
-      r = row64[k];  // the row within this wavefront's 64 (entries are sorted by row)
-      const int e = src[k];
-      const double cw = c1 * (double)wgt[k];
-      v0 = cw * fsrc[e];
-      v1 = cw * fsrc[e + (size_t)nele];
-      v2 = cw * fsrc[e + 2 * (size_t)nele];
-    }
-    const int key = live ? r : 64 + lane;
-    for (int o = 1; o < 64; o <<= 1) {
-      const double u0 = __shfl_up(v0, o, 64), u1 = __shfl_up(v1, o, 64), u2 = __shfl_up(v2, o, 64);
-      const int kk = __shfl_up(key, o, 64);
-      if (lane >= o && kk == key) {
-        v0 += u0;
-        v1 += u1;
-        v2 += u2;
-      }
-    }
-    const int nk = __shfl_down(key, 1, 64);
-    if (live && (lane == 63 || nk != key)) {
-      sacc[wv][0][r] += v0;
-      sacc[wv][1][r] += v1;
-      sacc[wv][2][r] += v2;
+
+// one source staged for a brick: wx wy on the 16 (x, y) points, wz on the 4 z planes,
+// the value
+struct WspStage {
+  double xy[16], z[4], f[3];
+};
+template <typename T>
+__device__ __forceinline__ void wsp_fetch(WspStage &r, const WspArgs<T> &a, int k) {
+#pragma clang fp contract(off)
+  const int e = a.bsrc[k], rp = a.brel[k], P = a.P;
+  const double *w = a.wts + (size_t)3 * P * e;
+  double wd[3][WSB];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const int rel = ((rp >> (8 * d)) & 255) - 16;  // brick-local stencil start
+#pragma unroll
+    for (int l = 0; l < WSB; l++) {
+      const int q = l - rel;
+      wd[d][l] = (q >= 0 && q < P) ? w[d * P + q] : 0.;
     }
   }
-  __syncthreads();
-  if (lane < nr) {
-    grid[g0 + lane] = (T)sacc[wv][0][lane];
-    grid[g0 + lane + (size_t)ng] = (T)sacc[wv][1][lane];
-    grid[g0 + lane + 2 * (size_t)ng] = (T)sacc[wv][2][lane];
+#pragma unroll
+  for (int ly = 0; ly < WSB; ly++)
+#pragma unroll
+    for (int lx = 0; lx < WSB; lx++) r.xy[lx + WSB * ly] = wd[0][lx] * wd[1][ly];
+#pragma unroll
+  for (int l = 0; l < WSB; l++) r.z[l] = wd[2][l];
+#pragma unroll
+  for (int d = 0; d < 3; d++) r.f[d] = a.fs[e + (size_t)a.nele * d];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(64) k_wsp_brick(WspArgs<T> a) {
+#pragma clang fp contract(off)
+  __shared__ double sxy[WSB_CH][18];  // rows padded: conflict-free 16-byte stores
+  __shared__ double4 sz[WSB_CH];
+  __shared__ double4 sf[WSB_CH];
+  const int lane = threadIdx.x, b = blockIdx.x;
+  const int bx = b % a.nbx, by = (b / a.nbx) % a.nby, bz = b / (a.nbx * a.nby);
+  const int lxy = lane & 15, lz = lane >> 4;
+  const int gi = bx * WSB + (lane & 3), gj = by * WSB + ((lane >> 2) & 3), gk = bz * WSB + lz;
+  double acc0 = 0., acc1 = 0., acc2 = 0.;
+  const int s0 = a.bstart[b], s1 = a.bstart[b + 1];
+  WspStage r;
+  if (s0 + lane < s1) wsp_fetch(r, a, s0 + lane);
+  for (int base = s0; base < s1; base += WSB_CH) {
+    const int cnt = min(WSB_CH, s1 - base);
+    __syncthreads();
+    if (lane < cnt) {
+#pragma unroll
+      for (int q = 0; q < 16; q++) sxy[lane][q] = r.xy[q];
+      sz[lane] = make_double4(r.z[0], r.z[1], r.z[2], r.z[3]);
+      sf[lane] = make_double4(r.f[0], r.f[1], r.f[2], 0.);
+    }
+    __syncthreads();
+    // the next round's sources load while this round is summed
+    if (base + WSB_CH + lane < s1) wsp_fetch(r, a, base + WSB_CH + lane);
+    const double *zrow = reinterpret_cast<const double *>(sz) + lz;
+#pragma unroll 8
+    for (int t = 0; t < cnt; t++) {
+      const double wxyz = sxy[t][lxy] * zrow[4 * t];
+      const double cw = a.c1 * wxyz;
+      const double4 f = sf[t];
+      acc0 = acc0 + cw * f.x;
+      acc1 = acc1 + cw * f.y;
+      acc2 = acc2 + cw * f.z;
+    }
   }
+  if (gi < a.N1 && gj < a.N2 && gk < a.N3) {
+    const size_t ng = (size_t)a.N1 * a.N2 * a.N3;
+    const size_t g = gi + (size_t)a.N1 * (gj + (size_t)a.N2 * gk);
+    a.grid[g] = (T)acc0;
+    a.grid[g + ng] = (T)acc1;
+    a.grid[g + 2 * ng] = (T)acc2;
+  }
+}
+
+// whether the wall-only spread above applies: B-spline order <= 8 and every grid
+// dimension at least P + 4 (a stencil then reaches a brick through one image only);
+// else the general spread runs
+static bool wall_spread_ok(const rbc3d_ctx *c) {
+  const int P = c->conf.P;
+  return P <= WSB_PMAX && c->conf.Nb[0] >= P + WSB && c->conf.Nb[1] >= P + WSB &&
+         c->conf.Nb[2] >= P + WSB;
 }
 
 static int wall_spread_setup(rbc3d_ctx *c) {
   WallData &w = c->wall;
   if (w.wsp_ready) return RBC3D_OK;
-  hipStream_t st = c->stream;
   const int N1 = c->conf.Nb[0], N2 = c->conf.Nb[1], N3 = c->conf.Nb[2], P = c->conf.P;
-  const long long ne = (long long)w.nele * P * P * P;
-  const int ng = N1 * N2 * N3;
-  if (ne > 0x7fffffffLL) {
-    rbc_set_error("wall spread map too large");
-    return RBC3D_EINVAL;
+  const int nele = w.nele;
+  const int nbx = (N1 + WSB - 1) / WSB, nby = (N2 + WSB - 1) / WSB, nbz = (N3 + WSB - 1) / WSB;
+  const double ih[3] = {N1 / c->Lb[0], N2 / c->Lb[1], N3 / c->Lb[2]};
+  const int Nd[3] = {N1, N2, N3};
+  std::vector<double> cen((size_t)3 * nele);
+  HIP_TRY(hipMemcpy(cen.data(), w.d_cen.p, sizeof(double) * 3 * nele, hipMemcpyDeviceToHost));
+  std::vector<double> wts((size_t)3 * P * nele);
+  // per brick: (source, brick-local stencil origin + 16 per axis, packed in bytes)
+  std::vector<std::vector<std::pair<int, int>>> lists((size_t)nbx * nby * nbz);
+  for (int e = 0; e < nele; e++) {
+    int bl[3][WSB_PMAX], rl[3][WSB_PMAX], nb[3];  // distinct bricks per axis, local origin
+    for (int d = 0; d < 3; d++) {
+      const int o = bspline_d(cen[e + (size_t)nele * d] * ih[d], P, &wts[(size_t)3 * P * e + d * P]);
+      nb[d] = 0;
+      for (int q = 0; q < P; q++) {
+        const int br = (((o + q) % Nd[d]) + Nd[d]) % Nd[d] / WSB;
+        int u = 0;
+        while (u < nb[d] && bl[d][u] != br) u++;
+        if (u == nb[d]) {
+          // the stencil start relative to the brick, in [-(P - 1), WSB - 1]
+          int rel = (((o - br * WSB) % Nd[d]) + Nd[d]) % Nd[d];
+          if (rel > WSB - 1) rel -= Nd[d];
+          bl[d][nb[d]] = br;
+          rl[d][nb[d]++] = rel + 16;
+        }
+      }
+    }
+    for (int u = 0; u < nb[0]; u++)
+      for (int v = 0; v < nb[1]; v++)
+        for (int x = 0; x < nb[2]; x++)
+          lists[bl[0][u] + (size_t)nbx * (bl[1][v] + (size_t)nby * bl[2][x])].push_back(
+              {e, rl[0][u] | (rl[1][v] << 8) | (rl[2][x] << 16)});
   }
-  const double ih0 = N1 / c->Lb[0], ih1 = N2 / c->Lb[1], ih2 = N3 / c->Lb[2];
-  DBuf<int> key, val, skey, sval;
-  RBC_TRY(key.ensure(ne));
-  RBC_TRY(val.ensure(ne));
-  RBC_TRY(skey.ensure(ne));
-  RBC_TRY(sval.ensure(ne));
-  k_wsp_keys<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(w.nele, w.d_cen.p, ih0, ih1, ih2, N1, N2,
-                                                          N3, P, key.p, val.p);
-  HIP_TRY(hipGetLastError());
-  int bits = 1;
-  while ((1LL << bits) < ng) bits++;
-  size_t tmp = 0;
-  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key.p, skey.p, val.p, sval.p, (int)ne, 0,
-                                             bits, st));
-  RBC_TRY(c->d_cub_tmp.ensure(tmp + 16));
-  HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->d_cub_tmp.p, tmp, key.p, skey.p, val.p, sval.p,
-                                             (int)ne, 0, bits, st));
-  RBC_TRY(w.d_wsp_ptr.ensure(ng + 1));
-  RBC_TRY(w.d_wsp_src.ensure(ne));
-  RBC_TRY(w.d_wsp_w.ensure(ne));
-  RBC_TRY(w.d_wsp_wf.ensure(ne));
-  RBC_TRY(w.d_wsp_f.ensure(3 * (size_t)w.nele));
-  k_wsp_ptr<<<(ng + 1 + 255) / 256, 256, 0, st>>>(ne, skey.p, ng, w.d_wsp_ptr.p);
-  RBC_TRY(w.d_wsp_row.ensure(ne));
-  k_wsp_weights<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(ne, skey.p, sval.p, w.d_cen.p, w.nele,
-                                                             ih0, ih1, ih2, P, w.d_wsp_src.p,
-                                                             w.d_wsp_w.p, w.d_wsp_wf.p,
-                                                             w.d_wsp_row.p);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(st));
+  std::vector<int> bstart(lists.size() + 1, 0), bsrc, brel;
+  for (size_t b = 0; b < lists.size(); b++) {
+    bstart[b + 1] = bstart[b] + (int)lists[b].size();
+    for (auto &pr : lists[b]) {
+      bsrc.push_back(pr.first);
+      brel.push_back(pr.second);
+    }
+  }
+  if (bsrc.empty()) {
+    bsrc.push_back(0);
+    brel.push_back(0);
+  }
+  RBC_TRY(w.d_wsb_start.ensure(bstart.size()));
+  RBC_TRY(w.d_wsb_src.ensure(bsrc.size()));
+  RBC_TRY(w.d_wsb_rel.ensure(brel.size()));
+  RBC_TRY(w.d_wsb_w.ensure(wts.size()));
+  RBC_TRY(w.d_wsb_f.ensure(3 * (size_t)nele));
+  HIP_TRY(hipMemcpy(w.d_wsb_start.p, bstart.data(), bstart.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(w.d_wsb_src.p, bsrc.data(), bsrc.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(w.d_wsb_rel.p, brel.data(), brel.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(w.d_wsb_w.p, wts.data(), wts.size() * sizeof(double), hipMemcpyHostToDevice));
   w.wsp_ready = true;
   return RBC3D_OK;
 }
@@ -604,22 +613,31 @@ template <typename T>
 static int wall_spread(rbc3d_ctx *c, double c1) {
   WallData &w = c->wall;
   RBC_TRY(wall_spread_setup(c));
-  hipStream_t st = c->stream;
-  const int ng = c->conf.Nb[0] * c->conf.Nb[1] * c->conf.Nb[2];
-  rbc_stage_begin(c, ST_SPREAD_PREP);
-  k_wsp_src<<<(w.nele + 255) / 256, 256, 0, st>>>(w.nele, w.d_e2v.p, w.nvert, w.d_f.p, w.d_area.p,
-                                                  w.d_wsp_f.p);
-  rbc_stage_end(c, ST_SPREAD_PREP);
+  const int N1 = c->conf.Nb[0], N2 = c->conf.Nb[1], N3 = c->conf.Nb[2];
+  WspArgs<T> a;
+  a.N1 = N1;
+  a.N2 = N2;
+  a.N3 = N3;
+  a.P = c->conf.P;
+  a.nbx = (N1 + WSB - 1) / WSB;
+  a.nby = (N2 + WSB - 1) / WSB;
+  a.nbz = (N3 + WSB - 1) / WSB;
+  a.nbr = a.nbx * a.nby * a.nbz;
+  a.nele = w.nele;
+  a.nvert = w.nvert;
+  a.c1 = c1;
+  a.bstart = w.d_wsb_start.p;
+  a.bsrc = w.d_wsb_src.p;
+  a.brel = w.d_wsb_rel.p;
+  a.wts = w.d_wsb_w.p;
+  a.e2v = w.d_e2v.p;
+  a.wf = w.d_f.p;
+  a.area = w.d_area.p;
+  a.fs = w.d_wsb_f.p;
+  a.grid = (T *)c->d_grid.p;
   rbc_stage_begin(c, ST_SPREAD);
-  const unsigned nblk = (unsigned)((ng + 255) / 256);  // 4 wavefronts of 64 rows each
-  if (sizeof(T) == 4)
-    k_wsp_spread<T, float><<<nblk, 256, 0, st>>>(ng, w.nele, c1, w.d_wsp_ptr.p, w.d_wsp_src.p,
-                                                 w.d_wsp_wf.p, w.d_wsp_row.p, w.d_wsp_f.p,
-                                                 (T *)c->d_grid.p);
-  else
-    k_wsp_spread<T, double><<<nblk, 256, 0, st>>>(ng, w.nele, c1, w.d_wsp_ptr.p, w.d_wsp_src.p,
-                                                  w.d_wsp_w.p, w.d_wsp_row.p, w.d_wsp_f.p,
-                                                  (T *)c->d_grid.p);
+  k_wsp_src<T><<<(w.nele + 255) / 256, 256, 0, c->stream>>>(a);
+  k_wsp_brick<T><<<a.nbr, 64, 0, c->stream>>>(a);
   rbc_stage_end(c, ST_SPREAD);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;
@@ -646,7 +664,8 @@ static int pme_spread_t(rbc3d_ctx *c, double c1, double c2, int cells, int walls
   int N1 = c->conf.Nb[0], N2 = c->conf.Nb[1], N3 = c->conf.Nb[2], P = c->conf.P;
   c->flag_sl = fabs(c1) > 1.e-10;
   c->flag_dl = fabs(c2) > 1.e-10;
-  if (!(cells && c->nrbc) && walls && c->wall.nvert && c->flag_sl && !c->flag_dl)
+  if (!(cells && c->nrbc) && walls && c->wall.nvert && c->flag_sl && !c->flag_dl &&
+      wall_spread_ok(c))
     return wall_spread<T>(c, c1);
   int ncell = (cells && c->nrbc) ? c->np : 0;
   int nele = (walls && c->wall.nvert) ? c->wall.nele : 0;
@@ -891,8 +910,12 @@ __global__ void __launch_bounds__(FFT_T) k_fft_c2c(typename CxT<T>::t *data, Fft
   C *buf = lds, *tmp = lds + (size_t)TL * pl.N, *stw = tmp + (size_t)TL * pl.N;
   // the axis' twiddle table in LDS (read by every pass; the barrier after the load
   // phase below makes it visible)
+  // consecutive blocks of lines share 128-byte lines of the grid: they run on one XCD
+  // (workgroups go to the 8 XCDs round-robin; the grid is a multiple of 8 workgroups)
+  const int blk = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
+  if (blk * TL >= lm.nlines) return;
   for (int m = threadIdx.x; m < pl.N; m += blockDim.x) stw[m] = pl.tw[m];
-  const int l0 = blockIdx.x * TL;
+  const int l0 = blk * TL;
   const int N = pl.N;
   // load: lines fastest (adjacent lines are adjacent in memory when sa = 1)
   for (int t = threadIdx.x; t < TL * N; t += blockDim.x) {
@@ -1029,10 +1052,304 @@ static int fft_c2c(rbc3d_ctx *c, typename CxT<T>::t *data, int ax, int sign, int
   }
   int TL = lines_per_block<T>(pl.N);
   size_t lds = sizeof(typename CxT<T>::t) * (2 * (size_t)TL * pl.N + pl.N);
-  k_fft_c2c<T><<<(lm.nlines + TL - 1) / TL, FFT_T, lds, c->stream>>>(data, pl, lm, TL,
-                                                                       make_fastdiv(TL));
+  const unsigned nblk = (unsigned)(((lm.nlines + TL - 1) / TL + 7) / 8 * 8);
+  k_fft_c2c<T><<<nblk, FFT_T, lds, c->stream>>>(data, pl, lm, TL, make_fastdiv(TL));
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;
+}
+
+// ---------------------------------------------------------------------------
+// x-y planes in one pass.  A workgroup takes one z plane of one field: the real plane
+// is loaded once, transformed along x (two real rows as one complex row) and along y
+// in LDS, and stored as the half-complex plane; the inverse does the reverse.  The
+// grid is read and written once per direction instead of twice.  Each Stockham pass
+// is register-staged: every thread loads the inputs of its butterflies, the
+// workgroup synchronizes, every thread writes its outputs into the same buffer, so
+// one plane buffer suffices (N2/2 rows of stride N1 + 2: a row pair's unpacked
+// half-spectra fit in place).
+
+// radix-R butterfly of v (inputs already twiddled) into natural-order outputs, the
+// same arithmetic as the passes above
+template <int R, typename C>
+__device__ __forceinline__ void bfly(C (&v)[R], int sign, const C *tw, int NR) {
+  typedef decltype(C().x) Re;
+  if constexpr (R == 2) {
+    const C a = v[0], b = v[1];
+    v[0] = cpx<C>(a.x + b.x, a.y + b.y);
+    v[1] = cpx<C>(a.x - b.x, a.y - b.y);
+  } else if constexpr (R == 4) {
+    C a0 = cpx<C>(v[0].x + v[2].x, v[0].y + v[2].y);
+    C a1 = cpx<C>(v[0].x - v[2].x, v[0].y - v[2].y);
+    C b0 = cpx<C>(v[1].x + v[3].x, v[1].y + v[3].y);
+    C b1 = cpx<C>(v[1].x - v[3].x, v[1].y - v[3].y);
+    C jb1 = sign < 0 ? cpx<C>(b1.y, -b1.x) : cpx<C>(-b1.y, b1.x);
+    v[0] = cpx<C>(a0.x + b0.x, a0.y + b0.y);
+    v[1] = cpx<C>(a1.x + jb1.x, a1.y + jb1.y);
+    v[2] = cpx<C>(a0.x - b0.x, a0.y - b0.y);
+    v[3] = cpx<C>(a1.x - jb1.x, a1.y - jb1.y);
+  } else if constexpr (R == 3) {
+    const Re hs = (Re)sign * (Re)0.86602540378443864676;
+    const C t = cpx<C>(v[1].x + v[2].x, v[1].y + v[2].y);
+    const C d = cpx<C>(v[1].x - v[2].x, v[1].y - v[2].y);
+    const C m = cpx<C>(v[0].x - (Re)0.5 * t.x, v[0].y - (Re)0.5 * t.y);
+    const Re ux = -hs * d.y, uy = hs * d.x;
+    const C y0 = cpx<C>(v[0].x + t.x, v[0].y + t.y);
+    v[1] = cpx<C>(m.x + ux, m.y + uy);
+    v[2] = cpx<C>(m.x - ux, m.y - uy);
+    v[0] = y0;
+  } else if constexpr (R == 5) {
+    const Re c1 = (Re)0.30901699437494742410, c2 = (Re)-0.80901699437494742410;
+    const Re sg = (Re)sign;
+    const Re s1 = sg * (Re)0.95105651629515357212, s2 = sg * (Re)0.58778525229247312917;
+    const C a1 = cpx<C>(v[1].x + v[4].x, v[1].y + v[4].y), b1 = cpx<C>(v[1].x - v[4].x, v[1].y - v[4].y);
+    const C a2 = cpx<C>(v[2].x + v[3].x, v[2].y + v[3].y), b2 = cpx<C>(v[2].x - v[3].x, v[2].y - v[3].y);
+    const C m1 = cpx<C>(v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y);
+    const C m2 = cpx<C>(v[0].x + c2 * a1.x + c1 * a2.x, v[0].y + c2 * a1.y + c1 * a2.y);
+    const Re u1x = -(s1 * b1.y + s2 * b2.y), u1y = s1 * b1.x + s2 * b2.x;
+    const Re u2x = -(s2 * b1.y - s1 * b2.y), u2y = s2 * b1.x - s1 * b2.x;
+    const C y0 = cpx<C>(v[0].x + a1.x + a2.x, v[0].y + a1.y + a2.y);
+    v[1] = cpx<C>(m1.x + u1x, m1.y + u1y);
+    v[4] = cpx<C>(m1.x - u1x, m1.y - u1y);
+    v[2] = cpx<C>(m2.x + u2x, m2.y + u2y);
+    v[3] = cpx<C>(m2.x - u2x, m2.y - u2y);
+    v[0] = y0;
+  }
+}
+
+// one register-staged Stockham pass over L lines of length N in buf (element e of
+// line l at l ls + e es).  LFAST: butterfly b is (line b % L, index b / L), consecutive
+// threads on consecutive lines (lines adjacent in LDS); else (line b / NR, index
+// b % NR), consecutive threads along one line (elements adjacent in LDS).
+template <int R, int MAXE, bool LFAST, typename T>
+__device__ __forceinline__ void rs_pass(typename CxT<T>::t *buf, int L, const FastDiv &fL, int N,
+                                        int ls, int es, int Ns, const FastDiv &fNs,
+                                        const FastDiv &fNR, int sign,
+                                        const typename CxT<T>::t *tw) {
+  typedef typename CxT<T>::t C;
+  constexpr int MB = (MAXE + R - 1) / R;
+  const int NR = N / R, nb = L * NR, tws = NR / Ns;
+  C v[MB][R];
+  int o[MB];
+#pragma unroll
+  for (int i = 0; i < MB; i++) {
+    const int b = threadIdx.x + i * blockDim.x;
+    o[i] = -1;
+    if (b < nb) {
+      int j, line;
+      if constexpr (LFAST) {
+        j = fd_div32(b, fL);
+        line = b - j * L;
+      } else {
+        line = fd_div32(b, fNR);
+        j = b - line * NR;
+      }
+      const int jq = fd_div32(j, fNs), k = j - jq * Ns;
+      const C *src = buf + line * ls;
+#pragma unroll
+      for (int r = 0; r < R; r++) {
+        C x = src[(j + r * NR) * es];
+        if (r && k) x = cmul(x, twid(tw, r * k * tws, sign));
+        v[i][r] = x;
+      }
+      o[i] = line * ls + (jq * Ns * R + k) * es;
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < MB; i++) {
+    if (o[i] >= 0) {
+      bfly<R>(v[i], sign, tw, NR);
+#pragma unroll
+      for (int q = 0; q < R; q++) buf[o[i] + q * Ns * es] = v[i][q];
+    }
+  }
+  __syncthreads();
+}
+
+template <int MAXE, bool LFAST, typename T>
+__device__ __forceinline__ void rs_fft(typename CxT<T>::t *buf, int L, const FastDiv &fL, int ls,
+                                       int es, const FftPlan<T> &pl, const typename CxT<T>::t *tw) {
+  int Ns = 1;
+  for (int s = 0; s < pl.nrad; s++) {
+    const int R = pl.rad[s];
+    if (R == 2)
+      rs_pass<2, MAXE, LFAST, T>(buf, L, fL, pl.N, ls, es, Ns, pl.fNs[s], pl.fNR[s], pl.sign, tw);
+    else if (R == 3)
+      rs_pass<3, MAXE, LFAST, T>(buf, L, fL, pl.N, ls, es, Ns, pl.fNs[s], pl.fNR[s], pl.sign, tw);
+    else if (R == 4)
+      rs_pass<4, MAXE, LFAST, T>(buf, L, fL, pl.N, ls, es, Ns, pl.fNs[s], pl.fNR[s], pl.sign, tw);
+    else
+      rs_pass<5, MAXE, LFAST, T>(buf, L, fL, pl.N, ls, es, Ns, pl.fNs[s], pl.fNR[s], pl.sign, tw);
+    Ns *= R;
+  }
+}
+
+#define XY_T 512    // threads per plane workgroup
+#define XY_MAXE 12  // complex elements a thread holds per pass
+#define XY_MAXP 10  // row-pair items a thread holds in the (un)packing steps
+#ifndef XY_XLF
+#define XY_XLF false  // x passes: consecutive threads along a row
+#endif
+struct XyArgs {
+  int N1, N2, N3, H, S, nf;
+  size_t ng, nc;
+  FastDiv fL1, fL2, fN1, fH;  // N2 / 2 (x lines), H (y lines), N1, H
+};
+
+// forward: real fields f0.. of the spread grid -> half-complex planes (x then y)
+template <typename T>
+__global__ void __launch_bounds__(XY_T) k_fft_xy_fwd(XyArgs a, const T *__restrict__ in,
+                                                     typename CxT<T>::t *out, FftPlan<T> px,
+                                                     FftPlan<T> py) {
+  typedef typename CxT<T>::t C;
+  typedef decltype(C().x) Re;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  C *buf = reinterpret_cast<C *>(lds_raw);
+  const int N1 = a.N1, N2 = a.N2, H = a.H, S = a.S, M = N2 / 2;
+  C *twx = buf + (size_t)M * S, *twy = twx + N1;
+  const int z = blockIdx.x % a.N3, f = blockIdx.x / a.N3;
+  for (int m = threadIdx.x; m < N1; m += blockDim.x) twx[m] = px.tw[m];
+  for (int m = threadIdx.x; m < N2; m += blockDim.x) twy[m] = py.tw[m];
+  // rows 2m and 2m + 1 as the real and imaginary parts of complex row m
+  const T *pin = in + a.ng * f + (size_t)N1 * N2 * z;
+  for (int t = threadIdx.x; t < M * N1; t += blockDim.x) {
+    const int m = fd_div32(t, a.fN1), n = t - m * N1;
+    buf[m * S + n] = cpx<C>(pin[2 * m * N1 + n], pin[(2 * m + 1) * N1 + n]);
+  }
+  __syncthreads();
+  rs_fft<XY_MAXE, XY_XLF, T>(buf, M, a.fL1, S, 1, px, twx);
+  // unpack: A[k] = (Z[k] + conj Z[N1-k]) / 2 into row 2m, B[k] = (Z[k] - conj Z[N1-k]) / 2i
+  // into row 2m + 1, k = 0..H-1 (rows of H complex: row y at y H)
+  {
+    C za[XY_MAXP], zb[XY_MAXP];
+    int it[XY_MAXP];
+#pragma unroll
+    for (int i = 0; i < XY_MAXP; i++) {
+      const int t = threadIdx.x + i * blockDim.x;
+      it[i] = -1;
+      if (t < M * H) {
+        const int m = fd_div32(t, a.fH), k = t - m * H;
+        za[i] = buf[m * S + k];
+        zb[i] = buf[m * S + (k == 0 ? 0 : N1 - k)];
+        it[i] = t;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < XY_MAXP; i++) {
+      if (it[i] >= 0) {
+        const int m = fd_div32(it[i], a.fH), k = it[i] - m * H;
+        const C p = za[i], q = zb[i];
+        buf[m * S + k] = cpx<C>((Re)0.5 * (p.x + q.x), (Re)0.5 * (p.y - q.y));
+        buf[m * S + H + k] = cpx<C>((Re)0.5 * (p.y + q.y), (Re)-0.5 * (p.x - q.x));
+      }
+    }
+    __syncthreads();
+  }
+  rs_fft<XY_MAXE, true, T>(buf, H, a.fL2, 1, H, py, twy);
+  C *pout = out + a.nc * f + (size_t)H * N2 * z;
+  for (int t = threadIdx.x; t < H * N2; t += blockDim.x) pout[t] = buf[t];
+}
+
+// inverse: half-complex planes of fields 12..14 -> real velocity planes (y then x)
+template <typename T>
+__global__ void __launch_bounds__(XY_T) k_fft_xy_inv(XyArgs a, const typename CxT<T>::t *in,
+                                                     T *__restrict__ out, FftPlan<T> px,
+                                                     FftPlan<T> py) {
+  typedef typename CxT<T>::t C;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  C *buf = reinterpret_cast<C *>(lds_raw);
+  const int N1 = a.N1, N2 = a.N2, H = a.H, S = a.S, M = N2 / 2;
+  C *twx = buf + (size_t)M * S, *twy = twx + N1;
+  const int z = blockIdx.x % a.N3, f = blockIdx.x / a.N3;
+  for (int m = threadIdx.x; m < N1; m += blockDim.x) twx[m] = px.tw[m];
+  for (int m = threadIdx.x; m < N2; m += blockDim.x) twy[m] = py.tw[m];
+  const C *pin = in + a.nc * f + (size_t)H * N2 * z;
+  for (int t = threadIdx.x; t < H * N2; t += blockDim.x) buf[t] = pin[t];
+  __syncthreads();
+  rs_fft<XY_MAXE, true, T>(buf, H, a.fL2, 1, H, py, twy);
+  // pack: Z[k] = A[k] + i B[k] over k = 0..N1-1 with A, B extended by Hermitian
+  // symmetry (the imaginary parts of the zero and Nyquist frequencies dropped: the
+  // halfcomplex convention of FFTW's c2r)
+  {
+    C za[XY_MAXP], zb[XY_MAXP];
+    int it[XY_MAXP];
+#pragma unroll
+    for (int i = 0; i < XY_MAXP; i++) {
+      const int t = threadIdx.x + i * blockDim.x;
+      it[i] = -1;
+      if (t < M * N1) {
+        const int m = fd_div32(t, a.fN1), k = t - m * N1;
+        const int kk = k < H ? k : N1 - k;
+        C A = buf[m * S + kk], B = buf[m * S + H + kk];
+        if (kk == 0 || 2 * kk == N1) {
+          A.y = 0;
+          B.y = 0;
+        }
+        if (k >= H) {
+          A.y = -A.y;
+          B.y = -B.y;
+        }
+        za[i] = A;
+        zb[i] = B;
+        it[i] = t;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < XY_MAXP; i++) {
+      if (it[i] >= 0) {
+        const int m = fd_div32(it[i], a.fN1), k = it[i] - m * N1;
+        buf[m * S + k] = cpx<C>(za[i].x - zb[i].y, za[i].y + zb[i].x);
+      }
+    }
+    __syncthreads();
+  }
+  rs_fft<XY_MAXE, XY_XLF, T>(buf, M, a.fL1, S, 1, px, twx);
+  T *pout = out + a.ng * f + (size_t)N1 * N2 * z;
+  for (int t = threadIdx.x; t < M * N1; t += blockDim.x) {
+    const int m = fd_div32(t, a.fN1), n = t - m * N1;
+    const C v = buf[m * S + n];
+    pout[2 * m * N1 + n] = v.x;
+    pout[(2 * m + 1) * N1 + n] = v.y;
+  }
+}
+
+// whether the plane kernels apply: even N1 and N2, every pass and (un)packing step
+// within the per-thread register budget, the plane within RBC_XY_LDS bytes of LDS
+template <typename T>
+static bool xy_plane_ok(rbc3d_ctx *c, size_t *lds) {
+  const int N1 = c->conf.Nb[0], N2 = c->conf.Nb[1], H = N1 / 2 + 1, M = N2 / 2;
+  if (N1 % 2 || N2 % 2) return false;
+  *lds = sizeof(typename CxT<T>::t) * ((size_t)M * (2 * H) + N1 + N2);
+  if (*lds > std::min<size_t>(RBC_XY_LDS, (size_t)c->lds_max)) return false;
+  auto fits = [](int L, int N, int R) { return (L * (N / R) + XY_T - 1) / XY_T <= (XY_MAXE + R - 1) / R; };
+  std::vector<int> r;
+  factor_radices(N1, r);
+  for (int R : r)
+    if (!fits(M, N1, R)) return false;
+  factor_radices(N2, r);
+  for (int R : r)
+    if (!fits(H, N2, R)) return false;
+  return (M * H + XY_T - 1) / XY_T <= XY_MAXP && (M * N1 + XY_T - 1) / XY_T <= XY_MAXP;
+}
+
+template <typename T>
+static XyArgs xy_args(rbc3d_ctx *c, int nf) {
+  XyArgs a;
+  a.N1 = c->conf.Nb[0];
+  a.N2 = c->conf.Nb[1];
+  a.N3 = c->conf.Nb[2];
+  a.H = a.N1 / 2 + 1;
+  a.S = 2 * a.H;
+  a.nf = nf;
+  a.ng = (size_t)a.N1 * a.N2 * a.N3;
+  a.nc = (size_t)a.H * a.N2 * a.N3;
+  a.fL1 = make_fastdiv(a.N2 / 2);
+  a.fL2 = make_fastdiv(a.H);
+  a.fN1 = make_fastdiv(a.N1);
+  a.fH = make_fastdiv(a.H);
+  return a;
 }
 
 // ---------------------------------------------------------------------------
@@ -1046,7 +1363,68 @@ struct GreenArgs {
   typename CxT<T>::t *C;    // 15 complex fields
 };
 
-// The k-dependent factors are formed in FP64 for either grid scalar T.
+// Green's function at one wave vector (i, j, k): vc = G(q) applied to the spread fields
+// (F(f) gives field f's value there, fields 0..2 SL, 3..11 DL), times the B-factor;
+// zero at q = 0.  The k-dependent factors are formed in FP64 for either grid scalar.
+template <typename T, typename F>
+__device__ __forceinline__ void green_at(const GreenArgs<T> &a, int i, int j, int k, F field,
+                                         double2 vc[3]) {
+  vc[0] = vc[1] = vc[2] = make_double2(0., 0.);
+  if (i == 0 && j == 0 && k == 0) return;
+  double q[3];
+  q[0] = i * a.iLb[0];
+  q[1] = (j < a.N2 / 2) ? j * a.iLb[1] : (j - a.N2) * a.iLb[1];
+  double q3 = (k < a.N3 / 2) ? k * a.iLb[2] : (k - a.N3) * a.iLb[2];
+  q[2] = -q3;
+  double spa = sqrt(RBC_PI * a.alpha), qt[3];
+  for (int d = 0; d < 3; d++) qt[d] = spa * q[d];
+  double q2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
+  double q2t = RBC_PI * a.alpha * q2, e = exp(-q2t);
+  double phi0 = e / q2t, phi1 = (e + phi0) / q2t;
+  if (a.flag_sl) {
+    double2 f[3];
+    for (int d = 0; d < 3; d++) f[d] = field(d);
+    double2 qf = make_double2(qt[0] * f[0].x + qt[1] * f[1].x + qt[2] * f[2].x,
+                              qt[0] * f[0].y + qt[1] * f[1].y + qt[2] * f[2].y);
+    double s = 2 * a.alpha / a.vol * phi1;
+    for (int d = 0; d < 3; d++) {
+      vc[d].x += s * (q2t * f[d].x - qt[d] * qf.x);
+      vc[d].y += s * (q2t * f[d].y - qt[d] * qf.y);
+    }
+  }
+  if (a.flag_dl) {
+    double2 Tm[3][3];
+    for (int ii = 0; ii < 3; ii++)
+      for (int jj = 0; jj < 3; jj++) Tm[ii][jj] = field(3 + ii + 3 * jj);
+    double2 tr = make_double2(Tm[0][0].x + Tm[1][1].x + Tm[2][2].x, Tm[0][0].y + Tm[1][1].y + Tm[2][2].y);
+    double2 qT[3], Tq[3];
+    for (int jj = 0; jj < 3; jj++)
+      qT[jj] = make_double2(q[0] * Tm[0][jj].x + q[1] * Tm[1][jj].x + q[2] * Tm[2][jj].x,
+                            q[0] * Tm[0][jj].y + q[1] * Tm[1][jj].y + q[2] * Tm[2][jj].y);
+    for (int ii = 0; ii < 3; ii++)
+      Tq[ii] = make_double2(Tm[ii][0].x * q[0] + Tm[ii][1].x * q[1] + Tm[ii][2].x * q[2],
+                            Tm[ii][0].y * q[0] + Tm[ii][1].y * q[1] + Tm[ii][2].y * q[2]);
+    double2 qTq = make_double2(q[0] * Tq[0].x + q[1] * Tq[1].x + q[2] * Tq[2].x,
+                               q[0] * Tq[0].y + q[1] * Tq[1].y + q[2] * Tq[2].y);
+    double s0 = 4 * RBC_PI * a.alpha / a.vol * phi0;
+    double s1 = 8 * RBC_PI * RBC_PI * a.alpha * a.alpha / a.vol * phi1;
+    for (int d = 0; d < 3; d++) {
+      // u = q tr + qT + Tq;  dv = i s0 u - i s1 qTq q;  vc -= dv
+      double ux = q[d] * tr.x + qT[d].x + Tq[d].x, uy = q[d] * tr.y + qT[d].y + Tq[d].y;
+      double wx = qTq.x * q[d], wy = qTq.y * q[d];
+      // i*s0*u = (-s0*uy, s0*ux); i*s1*w = (-s1*wy, s1*wx)
+      double dvx = -s0 * uy - (-s1 * wy), dvy = s0 * ux - s1 * wx;
+      vc[d].x -= dvx;
+      vc[d].y -= dvy;
+    }
+  }
+  double b = ((1.0 * a.bb[i]) * a.bb[a.H + j]) * a.bb[a.H + a.N2 + k];
+  for (int d = 0; d < 3; d++) {
+    vc[d].x *= b;
+    vc[d].y *= b;
+  }
+}
+
 template <typename T>
 __global__ void k_green(GreenArgs<T> a) {
   typedef typename CxT<T>::t CT;
@@ -1054,72 +1432,80 @@ __global__ void k_green(GreenArgs<T> a) {
   size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (o >= nc) return;
   int i = (int)(o % a.H), j = (int)((o / a.H) % a.N2), k = (int)(o / ((size_t)a.H * a.N2));
-  double2 vc[3] = {{0, 0}, {0, 0}, {0, 0}};
-  if (o != 0) {
-    double q[3];
-    q[0] = i * a.iLb[0];
-    q[1] = (j < a.N2 / 2) ? j * a.iLb[1] : (j - a.N2) * a.iLb[1];
-    double q3 = (k < a.N3 / 2) ? k * a.iLb[2] : (k - a.N3) * a.iLb[2];
-    q[2] = -q3;
-    double spa = sqrt(RBC_PI * a.alpha), qt[3];
-    for (int d = 0; d < 3; d++) qt[d] = spa * q[d];
-    double q2 = q[0] * q[0] + q[1] * q[1] + q[2] * q[2];
-    double q2t = RBC_PI * a.alpha * q2, e = exp(-q2t);
-    double phi0 = e / q2t, phi1 = (e + phi0) / q2t;
-    if (a.flag_sl) {
-      double2 f[3];
-      for (int d = 0; d < 3; d++) {
-        const CT v = a.C[o + nc * d];
-        f[d] = make_double2(v.x, v.y);
-      }
-      double2 qf = make_double2(qt[0] * f[0].x + qt[1] * f[1].x + qt[2] * f[2].x,
-                                qt[0] * f[0].y + qt[1] * f[1].y + qt[2] * f[2].y);
-      double s = 2 * a.alpha / a.vol * phi1;
-      for (int d = 0; d < 3; d++) {
-        vc[d].x += s * (q2t * f[d].x - qt[d] * qf.x);
-        vc[d].y += s * (q2t * f[d].y - qt[d] * qf.y);
-      }
-    }
-    if (a.flag_dl) {
-      double2 Tm[3][3];
-      for (int ii = 0; ii < 3; ii++)
-        for (int jj = 0; jj < 3; jj++) {
-          const CT v = a.C[o + nc * (3 + ii + 3 * jj)];
-          Tm[ii][jj] = make_double2(v.x, v.y);
-        }
-      double2 tr = make_double2(Tm[0][0].x + Tm[1][1].x + Tm[2][2].x, Tm[0][0].y + Tm[1][1].y + Tm[2][2].y);
-      double2 qT[3], Tq[3];
-      for (int jj = 0; jj < 3; jj++)
-        qT[jj] = make_double2(q[0] * Tm[0][jj].x + q[1] * Tm[1][jj].x + q[2] * Tm[2][jj].x,
-                              q[0] * Tm[0][jj].y + q[1] * Tm[1][jj].y + q[2] * Tm[2][jj].y);
-      for (int ii = 0; ii < 3; ii++)
-        Tq[ii] = make_double2(Tm[ii][0].x * q[0] + Tm[ii][1].x * q[1] + Tm[ii][2].x * q[2],
-                              Tm[ii][0].y * q[0] + Tm[ii][1].y * q[1] + Tm[ii][2].y * q[2]);
-      double2 qTq = make_double2(q[0] * Tq[0].x + q[1] * Tq[1].x + q[2] * Tq[2].x,
-                                 q[0] * Tq[0].y + q[1] * Tq[1].y + q[2] * Tq[2].y);
-      double s0 = 4 * RBC_PI * a.alpha / a.vol * phi0;
-      double s1 = 8 * RBC_PI * RBC_PI * a.alpha * a.alpha / a.vol * phi1;
-      for (int d = 0; d < 3; d++) {
-        // u = q tr + qT + Tq;  dv = i s0 u - i s1 qTq q;  vc -= dv
-        double ux = q[d] * tr.x + qT[d].x + Tq[d].x, uy = q[d] * tr.y + qT[d].y + Tq[d].y;
-        double wx = qTq.x * q[d], wy = qTq.y * q[d];
-        // i*s0*u = (-s0*uy, s0*ux); i*s1*w = (-s1*wy, s1*wx)
-        double dvx = -s0 * uy - (-s1 * wy), dvy = s0 * ux - s1 * wx;
-        vc[d].x -= dvx;
-        vc[d].y -= dvy;
-      }
-    }
-    double b = ((1.0 * a.bb[i]) * a.bb[a.H + j]) * a.bb[a.H + a.N2 + k];
-    for (int d = 0; d < 3; d++) {
-      vc[d].x *= b;
-      vc[d].y *= b;
-    }
-  }
+  double2 vc[3];
+  green_at(a, i, j, k,
+           [&](int f) {
+             const CT v = a.C[o + nc * f];
+             return make_double2(v.x, v.y);
+           },
+           vc);
   for (int d = 0; d < 3; d++) {
     CT v;
     v.x = (decltype(v.x))vc[d].x;
     v.y = (decltype(v.y))vc[d].y;
     a.C[o + nc * (12 + d)] = v;
+  }
+}
+
+// The z-stage of PME_Transform in one pass: for TL adjacent kx at one ky, the z-lines of
+// the nf spread fields (f0..f0+nf-1) are loaded, transformed forward (sign +1), taken
+// through Green's function and the B-factor into the three velocity fields, transformed
+// back (sign -1) and stored as fields 12..14: the grid is read and written once instead
+// of twice, with Green's multiply in between.  Per line the same Stockham passes as the
+// separate kernels, so the results are theirs bit for bit.
+//
+// Workgroups are dispatched to the 8 XCDs round-robin; the logical block order puts
+// consecutive column blocks (which share 128-byte lines of the grid) on one XCD's L2:
+// workgroup b takes block (b % 8) * (gridDim.x / 8) + b / 8 (the grid is a multiple of 8
+// workgroups; those past the last block return at once).
+template <typename T>
+__global__ void __launch_bounds__(FFT_T) k_fft_zgreen(GreenArgs<T> g, FftPlan<T> pf,
+                                                      FftPlan<T> pi, int f0, int nf, int TL) {
+  typedef typename CxT<T>::t C;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  C *lds = reinterpret_cast<C *>(lds_raw);
+  const int N3 = g.N3, H = g.H;
+  const size_t nc = (size_t)H * g.N2 * N3;
+  const int ncb = (H + TL - 1) / TL;  // column blocks per ky
+  const int blk = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
+  if (blk >= ncb * g.N2) return;
+  C *buf = lds, *tmp = lds + (size_t)nf * TL * N3, *stw = tmp + (size_t)nf * TL * N3;
+  for (int m = threadIdx.x; m < N3; m += blockDim.x) stw[m] = pf.tw[m];
+  const int j = blk / ncb, i0 = (blk % ncb) * TL;
+  const int tl = min(TL, H - i0);
+  // load: line (f, l) at (f * TL + l) * N3, kx fastest in memory
+  for (int t = threadIdx.x; t < nf * TL * N3; t += blockDim.x) {
+    const int l = t % TL, r = t / TL, z = r % N3, f = r / N3;
+    C v = cpx<C>(0., 0.);
+    if (l < tl) v = g.C[(i0 + l) + (size_t)H * (j + (size_t)g.N2 * z) + nc * (f0 + f)];
+    buf[(f * TL + l) * N3 + z] = v;
+  }
+  __syncthreads();
+  C *res = fft_lds<T>(buf, tmp, nf * TL, pf, stw);
+  C *out = res == buf ? tmp : buf;
+  for (int t = threadIdx.x; t < TL * N3; t += blockDim.x) {
+    const int l = t / N3, z = t % N3;
+    double2 vc[3] = {make_double2(0., 0.), make_double2(0., 0.), make_double2(0., 0.)};
+    if (l < tl)
+      green_at(g, i0 + l, j, z,
+               [&](int f) {
+                 const C v = res[((f - f0) * TL + l) * N3 + z];
+                 return make_double2(v.x, v.y);
+               },
+               vc);
+    for (int d = 0; d < 3; d++) {
+      C v;
+      v.x = (decltype(v.x))vc[d].x;
+      v.y = (decltype(v.y))vc[d].y;
+      out[(d * TL + l) * N3 + z] = v;
+    }
+  }
+  __syncthreads();
+  // the inverse z transform's twiddles (the same table, conjugated by the plan's sign)
+  C *res2 = fft_lds<T>(out, res, 3 * TL, pi, stw);
+  for (int t = threadIdx.x; t < 3 * TL * N3; t += blockDim.x) {
+    const int l = t % TL, r = t / TL, z = r % N3, d = r / N3;
+    if (l < tl) g.C[(i0 + l) + (size_t)H * (j + (size_t)g.N2 * z) + nc * (12 + d)] = res2[(d * TL + l) * N3 + z];
   }
 }
 
@@ -1133,7 +1519,15 @@ static int pme_transform_t(rbc3d_ctx *c) {
   int f0 = c->flag_sl ? 0 : 3, f1 = c->flag_dl ? 12 : 3;
   int nf = f1 - f0;
   rbc_stage_begin(c, ST_FFT);
-  if (nf > 0) {
+  size_t xy_lds = 0;
+  const bool xy = xy_plane_ok<T>(c, &xy_lds);
+  if (nf > 0 && xy) {
+    // x and y in one pass per plane
+    k_fft_xy_fwd<T><<<(unsigned)(nf * N3), XY_T, xy_lds, st>>>(
+        xy_args<T>(c, nf), (const T *)c->d_grid.p + ng * f0, C + nc * f0, make_plan<T>(c, 0, -1),
+        make_plan<T>(c, 1, -1));
+    HIP_TRY(hipGetLastError());
+  } else if (nf > 0) {
     FftPlan<T> px = make_plan<T>(c, 0, -1);
     int TL = lines_per_block<T>(N1), nrows = N2 * N3 * nf;
     size_t lds = sizeof(CT) * (2 * (size_t)TL * N1 + N1);
@@ -1142,8 +1536,17 @@ static int pme_transform_t(rbc3d_ctx *c) {
                                                             make_fastdiv(N1), make_fastdiv(H));
     HIP_TRY(hipGetLastError());
     RBC_TRY(fft_c2c<T>(c, C + nc * f0, 1, -1, nf));
-    RBC_TRY(fft_c2c<T>(c, C + nc * f0, 2, +1, nf));
   }
+  // the z stage: forward transform, Green's function and inverse transform in one pass
+  // when two or more adjacent columns of every field fit in RBC_ZG_LDS bytes of LDS
+  // (four workgroups a CU), else in three passes
+  const size_t zcol = 2 * (size_t)nf * N3 * sizeof(CT);
+  const size_t zlds = std::min<size_t>(RBC_ZG_LDS, (size_t)c->lds_max);
+  const int TLz = nf > 0 && zlds > N3 * sizeof(CT)
+                      ? (int)std::min<size_t>(8, (zlds - N3 * sizeof(CT)) / zcol)
+                      : 0;
+  const bool fused = nf > 0 && TLz >= 2;
+  if (nf > 0 && !fused) RBC_TRY(fft_c2c<T>(c, C + nc * f0, 2, +1, nf));
   rbc_stage_end(c, ST_FFT);
   GreenArgs<T> g;
   g.N1 = N1;
@@ -1157,18 +1560,38 @@ static int pme_transform_t(rbc3d_ctx *c) {
   g.flag_dl = c->flag_dl;
   g.bb = c->d_bb.p;
   g.C = C;
-  rbc_stage_begin(c, ST_GREEN);
-  k_green<T><<<(unsigned)((nc + 255) / 256), 256, 0, st>>>(g);
-  rbc_stage_end(c, ST_GREEN);
-  HIP_TRY(hipGetLastError());
+  if (fused) {
+    if ((long long)15 * nc >= (1LL << 31)) {
+      rbc_set_error("PME grid too large for 32-bit FFT offsets");
+      return RBC3D_EINVAL;
+    }
+    FftPlan<T> pzf = make_plan<T>(c, 2, +1), pzi = make_plan<T>(c, 2, -1);
+    const size_t lds = sizeof(CT) * (2 * (size_t)nf * TLz * N3 + N3);
+    const unsigned nblk = (unsigned)((((H + TLz - 1) / TLz) * N2 + 7) / 8 * 8);
+    rbc_stage_begin(c, ST_GREEN);
+    k_fft_zgreen<T><<<nblk, FFT_T, lds, st>>>(g, pzf, pzi, f0, nf, TLz);
+    rbc_stage_end(c, ST_GREEN);
+    HIP_TRY(hipGetLastError());
+  } else {
+    rbc_stage_begin(c, ST_GREEN);
+    k_green<T><<<(unsigned)((nc + 255) / 256), 256, 0, st>>>(g);
+    rbc_stage_end(c, ST_GREEN);
+    HIP_TRY(hipGetLastError());
+  }
   rbc_stage_begin(c, ST_FFT);
-  RBC_TRY(fft_c2c<T>(c, C + nc * 12, 2, -1, 3));
-  RBC_TRY(fft_c2c<T>(c, C + nc * 12, 1, +1, 3));
-  FftPlan<T> pxi = make_plan<T>(c, 0, +1);
-  int TL = lines_per_block<T>(N1), nrows = N2 * N3 * 3;
-  size_t lds = sizeof(CT) * (2 * (size_t)TL * N1 + N1);
-  k_fft_c2r<T><<<(nrows + TL - 1) / TL, FFT_T, lds, st>>>(C + nc * 12, (T *)c->d_vgrid.p, pxi,
-                                                           nrows, TL, make_fastdiv(N1));
+  if (!fused) RBC_TRY(fft_c2c<T>(c, C + nc * 12, 2, -1, 3));
+  if (xy) {
+    k_fft_xy_inv<T><<<(unsigned)(3 * N3), XY_T, xy_lds, st>>>(
+        xy_args<T>(c, 3), C + nc * 12, (T *)c->d_vgrid.p, make_plan<T>(c, 0, +1),
+        make_plan<T>(c, 1, +1));
+  } else {
+    RBC_TRY(fft_c2c<T>(c, C + nc * 12, 1, +1, 3));
+    FftPlan<T> pxi = make_plan<T>(c, 0, +1);
+    int TL = lines_per_block<T>(N1), nrows = N2 * N3 * 3;
+    size_t lds = sizeof(CT) * (2 * (size_t)TL * N1 + N1);
+    k_fft_c2r<T><<<(nrows + TL - 1) / TL, FFT_T, lds, st>>>(C + nc * 12, (T *)c->d_vgrid.p, pxi,
+                                                             nrows, TL, make_fastdiv(N1));
+  }
   rbc_stage_end(c, ST_FFT);
   HIP_TRY(hipGetLastError());
   c->pme_grid_prec = c->prec;
@@ -1259,6 +1682,106 @@ static int pme_gather_t(rbc3d_ctx *c, int nT, const double *d_tx, const int *d_a
     k_gather<6, T><<<nblk, 256, 0, c->stream>>>(g);
   else
     k_gather<0, T><<<nblk, 256, 0, c->stream>>>(g);
+  rbc_stage_end(c, ST_GATHER);
+  HIP_TRY(hipGetLastError());
+  return RBC3D_OK;
+}
+
+// The gather at the wall's vertices (the targets of the no-slip operator and of the
+// residual wall velocity): the vertices do not move, so each one's stencil origin and
+// B-spline weights are computed once per wall and grid.  GATHER_WL lanes per vertex
+// share its P x P (y, z) stencil rows, combined with shuffles; the division by the
+// target's A closes the sum as in the general gather.
+#define GATHER_WL 16
+template <int PC, typename T>
+__global__ void __launch_bounds__(256) k_gather_wall(int nT, int N1, int N2, int N3, int P_,
+                                                      const int *__restrict__ org,
+                                                      const double *__restrict__ wts,
+                                                      const T *__restrict__ vv,
+                                                      const double *__restrict__ tA, double *v) {
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int ti = gid / GATHER_WL, sub = gid % GATHER_WL;
+  const bool live = ti < nT;
+  const int P = PC ? PC : P_;
+  double v0 = 0, v1 = 0, v2 = 0;
+  if (live) {
+    const int imin = org[3 * ti], jmin = org[3 * ti + 1], kmin = org[3 * ti + 2];
+    const double *W = wts + (size_t)3 * P * ti;
+    const size_t ng = (size_t)N1 * N2 * N3;
+    for (int r = sub; r < P * P; r += GATHER_WL) {
+      const int j0 = r % P, k0 = r / P;
+      const int k = imod_d(kmin + k0, N3), j = imod_d(jmin + j0, N2);
+      const double wjk = W[P + j0] * W[2 * P + k0];
+      const size_t row = (size_t)N1 * (j + (size_t)N2 * k);
+#pragma unroll
+      for (int i0 = 0; i0 < (PC ? PC : 16); i0++) {
+        if (!PC && i0 >= P) break;
+        const int i = imod_d(imin + i0, N1);
+        const double w = W[i0] * wjk;
+        const size_t o = row + i;
+        v0 += w * vv[o];
+        v1 += w * vv[o + ng];
+        v2 += w * vv[o + 2 * ng];
+      }
+    }
+  }
+  for (int o = GATHER_WL / 2; o > 0; o >>= 1) {
+    v0 += __shfl_xor(v0, o, GATHER_WL);
+    v1 += __shfl_xor(v1, o, GATHER_WL);
+    v2 += __shfl_xor(v2, o, GATHER_WL);
+  }
+  if (live && sub == 0) {
+    const double A = tA[ti];
+    v[ti] += v0 / A;
+    v[ti + (size_t)nT] += v1 / A;
+    v[ti + 2 * (size_t)nT] += v2 / A;
+  }
+}
+
+static int wall_gather_setup(rbc3d_ctx *c) {
+  WallData &w = c->wall;
+  if (w.wgt_ready) return RBC3D_OK;
+  const int nv = w.nvert, P = c->conf.P;
+  if (P > 16) {
+    rbc_set_error("B-spline order above 16");
+    return RBC3D_EINVAL;
+  }
+  std::vector<double> x((size_t)3 * nv), wts((size_t)3 * P * nv);
+  std::vector<int> org((size_t)3 * nv);
+  HIP_TRY(hipMemcpy(x.data(), w.d_x.p, sizeof(double) * 3 * nv, hipMemcpyDeviceToHost));
+  for (int i = 0; i < nv; i++)
+    for (int d = 0; d < 3; d++)
+      org[3 * i + d] = bspline_d(x[i + (size_t)nv * d] * (c->conf.Nb[d] / c->Lb[d]), P,
+                                 &wts[(size_t)3 * P * i + d * P]);
+  RBC_TRY(w.d_wg_org.ensure(org.size()));
+  RBC_TRY(w.d_wg_w.ensure(wts.size()));
+  HIP_TRY(hipMemcpy(w.d_wg_org.p, org.data(), org.size() * sizeof(int), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(w.d_wg_w.p, wts.data(), wts.size() * sizeof(double), hipMemcpyHostToDevice));
+  w.wgt_ready = true;
+  return RBC3D_OK;
+}
+
+int rbc_pme_gather_wall(rbc3d_ctx *c, const double *d_A, double *d_v) {
+  WallData &w = c->wall;
+  const int nT = w.nvert;
+  if (nT <= 0) return RBC3D_OK;
+  RBC_TRY(wall_gather_setup(c));
+  const int N1 = c->conf.Nb[0], N2 = c->conf.Nb[1], N3 = c->conf.Nb[2], P = c->conf.P;
+  const unsigned nblk = (unsigned)(((size_t)nT * GATHER_WL + 255) / 256);
+  rbc_stage_begin(c, ST_GATHER);
+  if (c->pme_grid_prec == RBC3D_PREC_SPLIT) {
+    const float *vv = (const float *)c->d_vgrid.p;
+    if (P == 6)
+      k_gather_wall<6, float><<<nblk, 256, 0, c->stream>>>(nT, N1, N2, N3, P, w.d_wg_org.p, w.d_wg_w.p, vv, d_A, d_v);
+    else
+      k_gather_wall<0, float><<<nblk, 256, 0, c->stream>>>(nT, N1, N2, N3, P, w.d_wg_org.p, w.d_wg_w.p, vv, d_A, d_v);
+  } else {
+    const double *vv = (const double *)c->d_vgrid.p;
+    if (P == 6)
+      k_gather_wall<6, double><<<nblk, 256, 0, c->stream>>>(nT, N1, N2, N3, P, w.d_wg_org.p, w.d_wg_w.p, vv, d_A, d_v);
+    else
+      k_gather_wall<0, double><<<nblk, 256, 0, c->stream>>>(nT, N1, N2, N3, P, w.d_wg_org.p, w.d_wg_w.p, vv, d_A, d_v);
+  }
   rbc_stage_end(c, ST_GATHER);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;

@@ -506,11 +506,27 @@ __global__ void __launch_bounds__(128) k_wall_reduce(int nvert, const int *__res
   if (!pass) rowcnt[i] = n;
 }
 
+// The 3x3 blocks of the self-matrix are symmetric: each is a Stokeslet-type tensor
+// EA x x^T + EB I integrated against a scalar hat function (their two off-diagonal
+// copies differ only by rounding, 3e-16 relative).  After assembly every block is
+// kept as its symmetric part in six component arrays (B00, B11, B22, B01, B02, B12):
+// 6 values + 1 column index per block instead of 9 + 1.
+__global__ void k_wall_sym6(size_t nb, const double *__restrict__ b9, double *b6) {
+  const size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (k >= nb) return;
+  b6[k] = b9[k];
+  b6[nb + k] = b9[4 * nb + k];
+  b6[2 * nb + k] = b9[8 * nb + k];
+  b6[3 * nb + k] = 0.5 * (b9[nb + k] + b9[3 * nb + k]);
+  b6[4 * nb + k] = 0.5 * (b9[2 * nb + k] + b9[6 * nb + k]);
+  b6[5 * nb + k] = 0.5 * (b9[5 * nb + k] + b9[7 * nb + k]);
+}
+
 // v_i += c1 (L f)_i / Acoef_i over wall vertices (SingIntOnWall :136-172)
 // v_i += c1/A_i * sum_k B_k f_(col k): one wavefront per vertex row, lanes over the
-// row's 3x3 blocks.  The blocks are stored as nine component arrays (bval[q][k]), so
-// each of a lane's nine value loads is one coalesced 64-lane access; the 3 traction
-// components of the block's column are gathered (the traction vector stays in L2).
+// row's blocks.  The six component arrays make each of a lane's value loads one
+// coalesced 64-lane access; the 3 traction components of the block's column are
+// gathered (the traction vector stays in L2).
 // V: the block scalar (double, or float in the split form); sums are FP64 either way.
 template <typename V>
 __global__ void __launch_bounds__(256) k_wall_spmv(int nvert, double c1, const int *__restrict__ rowptr,
@@ -522,16 +538,26 @@ __global__ void __launch_bounds__(256) k_wall_spmv(int nvert, double c1, const i
   if (i >= nvert) return;
   const size_t nbt = rowptr[nvert];
   double s0 = 0, s1 = 0, s2 = 0;
-  for (int k = rowptr[i] + lane; k < rowptr[i + 1]; k += 64) {
+  auto block = [&](int k) {
     const int j = bcol[k];
-    V B[9];
-#pragma unroll
-    for (int q = 0; q < 9; q++) B[q] = bval[q * nbt + k];
+    const double b00 = bval[k], b11 = bval[nbt + k], b22 = bval[2 * nbt + k];
+    const double b01 = bval[3 * nbt + k], b02 = bval[4 * nbt + k], b12 = bval[5 * nbt + k];
     const double f0 = f[j], f1 = f[j + nvert], f2 = f[j + 2 * (size_t)nvert];
-    s0 += B[0] * f0 + B[1] * f1 + B[2] * f2;
-    s1 += B[3] * f0 + B[4] * f1 + B[5] * f2;
-    s2 += B[6] * f0 + B[7] * f1 + B[8] * f2;
+    s0 += b00 * f0 + b01 * f1 + b02 * f2;
+    s1 += b01 * f0 + b11 * f1 + b12 * f2;
+    s2 += b02 * f0 + b12 * f1 + b22 * f2;
+  };
+  // four blocks per lane per step: their column, value and source loads are issued
+  // together (the source gathers depend on the column loads)
+  int k = rowptr[i] + lane;
+  const int kend = rowptr[i + 1];
+  for (; k + 192 < kend; k += 256) {
+    block(k);
+    block(k + 64);
+    block(k + 128);
+    block(k + 192);
   }
+  for (; k < kend; k += 64) block(k);
   for (int o = 32; o > 0; o >>= 1) {
     s0 += __shfl_xor(s0, o, 64);
     s1 += __shfl_xor(s1, o, 64);
@@ -710,6 +736,7 @@ extern "C" int rbc3d_set_wall(rbc3d_ctx *c, int nvert, int nele, const double *x
   w.valf_ready = false;
   w.lmodes_ready = false;
   w.wsp_ready = false;
+  w.wgt_ready = false;
   HIP_TRY(hipStreamSynchronize(st));
   return RBC3D_OK;
 }
@@ -787,11 +814,17 @@ int rbc_wall_prepare(rbc3d_ctx *c) {
     return RBC3D_EINVAL;
   }
   RBC_TRY(w.d_col.ensure(nb + 1));
-  RBC_TRY(w.d_val.ensure(9 * (size_t)nb + 9));
-  k_wall_reduce<<<(nv + 127) / 128, 128, 0, st>>>(nv, off.p, skey.p, sidx.p, eval.p, w.d_rowptr.p,
-                                                  w.d_col.p, w.d_val.p, (size_t)nb, 1);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(st));
+  {
+    DBuf<double> b9;
+    RBC_TRY(b9.ensure(9 * (size_t)nb + 9));
+    k_wall_reduce<<<(nv + 127) / 128, 128, 0, st>>>(nv, off.p, skey.p, sidx.p, eval.p, w.d_rowptr.p,
+                                                    w.d_col.p, b9.p, (size_t)nb, 1);
+    HIP_TRY(hipGetLastError());
+    RBC_TRY(w.d_val.ensure(6 * (size_t)nb + 6));
+    k_wall_sym6<<<(unsigned)(((size_t)nb + 255) / 256), 256, 0, st>>>((size_t)nb, b9.p, w.d_val.p);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(st));
+  }
   w.nnz = 9LL * nb;
   w.prepared = true;
   w.valf_ready = false;
@@ -811,20 +844,21 @@ extern "C" int rbc3d_wall_self_nnz(rbc3d_ctx *c, long long *nnz) {
 }
 
 int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double c1, int nT, const double *d_tx, const int *d_tindx,
-                             const int *d_act, const double *d_A, double *d_v, bool wall_targets) {
+                             const int *d_act, const double *d_A, double *d_v, bool wall_targets,
+                             hipStream_t on) {
   WallData &w = c->wall;
   if (!w.nvert || nT == 0) return RBC3D_OK;
-  hipStream_t st = c->stream;
-  rbc_stage_begin(c, ST_WALL);
+  hipStream_t st = on ? on : c->stream;
+  rbc_stage_begin(c, ST_WALL, st);
   if (wall_targets) {
     // wall vertices: the precomputed self-interaction; with one wall that is all
     if (!w.prepared) RBC_TRY(rbc_wall_prepare(c));
     const unsigned nblk = (unsigned)(((size_t)w.nvert * 64 + 255) / 256);
     if (c->prec == RBC3D_PREC_SPLIT) {
       if (!w.valf_ready) {
-        const size_t nv9 = (size_t)w.nnz;  // 9 values per 3x3 block
-        RBC_TRY(w.d_valf.ensure(nv9 + 1));
-        k_to_float<<<(unsigned)((nv9 + 255) / 256), 256, 0, st>>>(w.d_val.p, nv9, w.d_valf.p);
+        const size_t nv6 = (size_t)w.nnz / 9 * 6;  // 6 values per symmetric 3x3 block
+        RBC_TRY(w.d_valf.ensure(nv6 + 1));
+        k_to_float<<<(unsigned)((nv6 + 255) / 256), 256, 0, st>>>(w.d_val.p, nv6, w.d_valf.p);
         HIP_TRY(hipGetLastError());
         w.valf_ready = true;
       }
@@ -853,7 +887,7 @@ int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double c1, int nT, const double *d_tx
     k_wall_targets_tiled<<<nblk, WT_T * WT_W, 0, st>>>(wall_args(c), c1, nT, d_tx, d_tindx, d_act, d_A,
                                                 w.d_tg_order.p, w.d_tg_start.p, coff, nbins, d_v);
   }
-  rbc_stage_end(c, ST_WALL);
+  rbc_stage_end(c, ST_WALL, st);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;
 }
@@ -938,12 +972,19 @@ int rbc_wall_matmult_dev(rbc3d_ctx *c, const double *d_u, double *d_b) {
   k_assemble_in<<<(nv + 255) / 256, 256, 0, st>>>(nv, w.d_glb.p, d_u, w.d_f.p);
   HIP_TRY(hipMemsetAsync(c->d_v.p, 0, sizeof(double) * 3 * nv, st));
   double c1 = 1. / (4. * RBC_PI);
-  RBC_TRY(rbc_add_int_on_walls_dev(c, c1, nv, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
-                                   c->d_v.p, true));
   RBC_TRY(rbc_pme_init(c));
+  // the self-matrix SpMV (into v) runs on the side stream beside the spread and the
+  // transforms (which read the same traction and write only the grids); the gather
+  // adds the PME part after it, so v sums in the same order as one stream would
+  HIP_TRY(hipEventRecord(c->ev_fork, st));
+  HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
+  RBC_TRY(rbc_add_int_on_walls_dev(c, c1, nv, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
+                                   c->d_v.p, true, c->side));
+  HIP_TRY(hipEventRecord(c->ev_join, c->side));
   RBC_TRY(rbc_pme_spread(c, c1, 0., 0, 1));
   RBC_TRY(rbc_pme_transform(c));
-  RBC_TRY(rbc_pme_gather(c, nv, c->d_tx.p, c->d_tact.p, c->d_tA.p, c->d_v.p));
+  HIP_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
+  RBC_TRY(rbc_pme_gather_wall(c, c->d_tA.p, c->d_v.p));
   k_assemble_out<<<(nu + 255) / 256, 256, 0, st>>>(nu, nv, w.d_glb.p + nv, c->d_v.p, d_b);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;
@@ -994,7 +1035,7 @@ int rbc_wall_cell_vel_dev(rbc3d_ctx *c, double *d_out) {
   RBC_TRY(rbc_pme_init(c));
   RBC_TRY(rbc_pme_spread(c, c1, c2, 1, 0));
   RBC_TRY(rbc_pme_transform(c));
-  RBC_TRY(rbc_pme_gather(c, nv, c->d_tx.p, c->d_tact.p, c->d_tA.p, d_out));
+  RBC_TRY(rbc_pme_gather_wall(c, c->d_tA.p, d_out));
   return RBC3D_OK;
 }
 
@@ -1033,7 +1074,7 @@ int rbc_wall_residual_dev(rbc3d_ctx *c) {
   RBC_TRY(rbc_pme_init(c));
   RBC_TRY(rbc_pme_spread(c, c1, c2, 1, 1));
   RBC_TRY(rbc_pme_transform(c));
-  RBC_TRY(rbc_pme_gather(c, nv, c->d_tx.p, c->d_tact.p, c->d_tA.p, c->d_v.p));
+  RBC_TRY(rbc_pme_gather_wall(c, c->d_tA.p, c->d_v.p));
   k_add_const<<<(nv + 255) / 256, 256, 0, st>>>(c->d_v.p, nv, c->conf.vBkg[0], c->conf.vBkg[1],
                                                 c->conf.vBkg[2]);
   HIP_TRY(hipGetLastError());

@@ -38,6 +38,22 @@ def test_wall_self_matrix(tube):
     assert nnz.value == len(val)
 
 
+def test_wall_only_spread_grid(tube):
+    """The wall-only single-layer spread (every no-slip GMRES iteration): per-brick
+    owner-computes over the fixed centroids, each point summing its triangles in
+    ascending order with the reference's expressions, gives the oracle's grid bit for
+    bit."""
+    w, g, o = tube
+    c = 1 / (4 * np.pi)
+    g.pme_distrib_source(c, 0.0, cells=False, walls=True)
+    L = __import__("oracle").oracle_py.L()
+    L.orc_pme_distrib_source(o.h, c, 0.0, 0, 1)
+    gg = g.pme_grid(0)
+    og = o.pme_grid(0)[:len(gg)]
+    assert np.abs(og).max() > 0
+    assert np.array_equal(gg, og), np.abs(gg - og).max()
+
+
 def test_add_int_on_walls_cell_targets(tube):
     w, g, o = tube
     tl = o.tlist_rbc()
