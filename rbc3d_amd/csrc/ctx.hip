@@ -325,20 +325,20 @@ static hipEvent_t ev_get(StageTimer &t) {
   (void)hipEventCreate(&e);
   return e;
 }
-void rbc_stage_begin(rbc3d_ctx *c, int stage, hipStream_t st) {
+void rbc_stage_begin(rbc3d_ctx *c, int stage) {
   if (!c->timer.on) return;
   StageTimer::Rec r;
   r.stage = stage;
   r.a = ev_get(c->timer);
   r.b = ev_get(c->timer);
-  (void)hipEventRecord(r.a, st ? st : c->stream);
+  (void)hipEventRecord(r.a, c->stream);
   c->timer.pending.push_back(r);
 }
-void rbc_stage_end(rbc3d_ctx *c, int stage, hipStream_t st) {
+void rbc_stage_end(rbc3d_ctx *c, int stage) {
   if (!c->timer.on) return;
   for (auto it = c->timer.pending.rbegin(); it != c->timer.pending.rend(); ++it)
     if (it->stage == stage) {
-      (void)hipEventRecord(it->b, st ? st : c->stream);
+      (void)hipEventRecord(it->b, c->stream);
       return;
     }
 }
@@ -420,15 +420,8 @@ extern "C" int rbc3d_create(const rbc3d_conf *conf, rbc3d_ctx **out) {
       c->Bcoef[l] = -1.;
     }
   }
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess) {
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     rbc_set_error("rbc3d_create: stream");
-    if (c->ev_join) (void)hipEventDestroy(c->ev_join);
-    if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
-    if (c->side) (void)hipStreamDestroy(c->side);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return RBC3D_EHIP;
   }
@@ -586,11 +579,8 @@ extern "C" int rbc3d_destroy(rbc3d_ctx *c) {
   for (auto e : c->gmres.ev)
     if (e) (void)hipEventDestroy(e);
   if (c->gmres.hpin) (void)hipHostFree(c->gmres.hpin);
+  if (c->gmres.hcoh) (void)hipHostFree(c->gmres.hcoh);
   c->d_pairstat.release();
-  if (c->side) (void)hipStreamSynchronize(c->side);
-  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
-  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
-  if (c->side) (void)hipStreamDestroy(c->side);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
   return RBC3D_OK;

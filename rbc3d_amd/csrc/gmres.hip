@@ -53,6 +53,67 @@ __global__ void __launch_bounds__(G_T) k_mdot(int n, int k, const double *__rest
   }
 }
 
+// The partial sums of vector j over nb blocks, by one wavefront: lanes stride the
+// blocks in order, then a shuffle tree (the order of k_mdot_final)
+__device__ __forceinline__ double sum_parts(const double *part, int nb, int lane) {
+  double s = 0;
+  for (int b = lane; b < nb; b += 64) s += part[b];
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return s;
+}
+
+// w -= sum_j h_j V_j (j < k), with the block's partial sum of the new |w|^2 in
+// part[blockIdx.x] (wavefront shuffle trees, then the block's wavefronts in order)
+__global__ void __launch_bounds__(G_T) k_maxpy_part(int n, int k, const double *__restrict__ V,
+                                                    size_t ld, const double *__restrict__ h,
+                                                    double *w, double *part) {
+  __shared__ double red[G_T / 64];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  double q = 0.;
+  if (i < n) {
+    double s = w[i];
+    for (int j = 0; j < k; j++) s -= h[j] * V[j * ld + i];
+    w[i] = s;
+    q = s * s;
+  }
+  for (int o = 32; o > 0; o >>= 1) q += __shfl_xor(q, o, 64);
+  if (lane == 0) red[wv] = q;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.;
+    for (int u = 0; u < G_T / 64; u++) t += red[u];
+    part[blockIdx.x] = t;
+  }
+}
+
+// t2 = sum of the nb partials (every block forms the same sum, in the same order) and
+// w /= sqrt(t2) unless it is zero; block 0 stores t2.  PETSc skips the scaling at a
+// happy breakdown (|w| <= 1e-30 |g|), which the host decides from the same value
+// afterwards; the vector is then never used, so scaling it regardless changes nothing.
+// Block 0 also copies the Hessenberg column (h_0..h_{k-1} from hd, then t2) to the
+// host's coherent pinned slot col, which the host reads once the launch is done.
+__global__ void __launch_bounds__(G_T) k_scale_norm_sum(int n, int nb, const double *__restrict__ part,
+                                                        double *t2, double *w, int k,
+                                                        const double *__restrict__ hd, double *col) {
+  __shared__ double tot;
+  if (threadIdx.x < 64) {
+    const double t = sum_parts(part, nb, threadIdx.x);
+    if (threadIdx.x == 0) {
+      tot = t;
+      if (blockIdx.x == 0) *t2 = t;
+    }
+  }
+  __syncthreads();
+  if (blockIdx.x == 0) {
+    for (int q = threadIdx.x; q < k; q += blockDim.x) col[q] = hd[q];
+    if (threadIdx.x == 0) col[k] = tot;
+    __threadfence_system();
+  }
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double tt = sqrt(tot);
+  if (i < n && tt > 0.) w[i] = (1.0 / tt) * w[i];
+}
+
 // out[j] = sum over the nb partial sums of vector j: one wavefront per vector
 __global__ void k_mdot_final(int k, int nb, const double *__restrict__ part, double *out) {
   const int j = blockIdx.x, lane = threadIdx.x;
@@ -137,7 +198,12 @@ int rbc_gmres(rbc3d_ctx *c, int n, const MatvecFn &matvec, const double *d_b, do
   size_t ld = (size_t)n + 64;
   RBC_TRY(ws.V.ensure(ld * (restart + 1)));
   RBC_TRY(ws.hd.ensure(2 * G_MAXV + 2));
+  RBC_TRY(ws.part2.ensure((size_t)(n + G_T - 1) / G_T + 1));
   if (!ws.hpin) HIP_TRY(hipHostMalloc((void **)&ws.hpin, sizeof(double) * 3 * (G_MAXV + 2)));
+  if (!ws.hcoh) {
+    HIP_TRY(hipHostMalloc((void **)&ws.hcoh, sizeof(double) * 2 * (G_MAXV + 2), hipHostMallocCoherent));
+    HIP_TRY(hipHostGetDevicePointer((void **)&ws.hcoh_dev, ws.hcoh, 0));
+  }
   for (auto &e : ws.ev)
     if (!e) HIP_TRY(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   double *V = ws.V.p;
@@ -203,13 +269,12 @@ int rbc_gmres(rbc3d_ctx *c, int n, const MatvecFn &matvec, const double *d_b, do
         RBC_TRY((*matvec_lo)(vj, vn));
       else
         RBC_TRY(matvec(vj, vn));
+      // h = V^T w; w -= V h with the partial sums of |w|^2; |w|^2 and w / |w|
       RBC_TRY(dots_dev(st, ws, n, j + 1, V, ld, vn, ws.hd.p));
-      k_maxpy<<<nb, 256, 0, st>>>(n, j + 1, V, ld, ws.hd.p, vn);
-      RBC_TRY(dots_dev(st, ws, n, 1, vn, ld, vn, ws.hd.p + j + 1));
-      k_scale_norm<<<nb, 256, 0, st>>>(n, ws.hd.p + j + 1, vn);
+      k_maxpy_part<<<nb, G_T, 0, st>>>(n, j + 1, V, ld, ws.hd.p, vn, ws.part2.p);
+      k_scale_norm_sum<<<nb, G_T, 0, st>>>(n, nb, ws.part2.p, ws.hd.p + j + 1, vn, j + 1, ws.hd.p,
+                                           ws.hcoh_dev + (size_t)(j & 1) * (G_MAXV + 2));
       HIP_TRY(hipGetLastError());
-      HIP_TRY(hipMemcpyAsync(ws.hpin + (size_t)(j & 1) * (G_MAXV + 2), ws.hd.p,
-                             sizeof(double) * (j + 2), hipMemcpyDeviceToHost, st));
       HIP_TRY(hipEventRecord(ws.ev[j & 1], st));
       return RBC3D_OK;
     };
@@ -217,7 +282,7 @@ int rbc_gmres(rbc3d_ctx *c, int n, const MatvecFn &matvec, const double *d_b, do
     while (it < restart && its < maxit) {
       if (it + 1 < restart && its + 1 < maxit) RBC_TRY(enqueue(it + 1));
       HIP_TRY(hipEventSynchronize(ws.ev[it & 1]));
-      const double *col = ws.hpin + (size_t)(it & 1) * (G_MAXV + 2);
+      const volatile double *col = ws.hcoh + (size_t)(it & 1) * (G_MAXV + 2);
       // happy breakdown bound as in PETSc: haptol * |g(it)|
       const double hapbnd = fabs(g[it]) * 1.0e-30;
       const double tt = sqrt(col[it + 1]);

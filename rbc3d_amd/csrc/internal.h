@@ -89,7 +89,7 @@ struct GridType {
 
 // GMRES work space (Krylov basis, partial dot products), kept across solves
 struct GmresWork {
-  DBuf<double> V, w, part, hd;
+  DBuf<double> V, w, part, part2, hd;
   // vectors of the solves around the GMRES (previous traction, right-hand side,
   // solution, the learned warm start's inputs and residual images), kept across calls
   DBuf<double> f0, rhs, x, r0, uc, LV, tmp;
@@ -99,6 +99,9 @@ struct GmresWork {
   // pinned host slots for the Hessenberg columns of two Arnoldi steps in flight and
   // the update coefficients, with an event per column slot (created on first use)
   double *hpin = nullptr;
+  // coherent pinned slots the Arnoldi step's last kernel writes its Hessenberg column
+  // to (host and device addresses)
+  double *hcoh = nullptr, *hcoh_dev = nullptr;
   hipEvent_t ev[2] = {nullptr, nullptr};
 };
 
@@ -162,7 +165,13 @@ struct WallData {
   // the gather at the vertices: per vertex its stencil origin and B-spline weights
   DBuf<int> d_wg_org;
   DBuf<double> d_wg_w;
+  DBuf<float> d_wg_wf;  // the same weights in FP32 (the split form's gather)
   bool wgt_ready = false;
+  // the operator on the de-duplicated vector: self-matrix columns and triangle
+  // vertices as unique indices, the targets' A (2 at every vertex)
+  DBuf<int> d_colu, d_e2u;
+  DBuf<double> d_A2;
+  bool op_ready = false;
   // images of the three uniform tractions under the no-slip operator (learned warm
   // start), for the precision mode they were computed in
   DBuf<double> d_lmodes;
@@ -175,10 +184,6 @@ struct rbc3d_ctx {
   double Lb[3], iLb[3];
   double Acoef[8], Bcoef[8];
   hipStream_t stream = nullptr;
-  // second stream (default priority) on which the no-slip operator's self-matrix SpMV
-  // overlaps the PME chain of the main stream; the fork and join events
-  hipStream_t side = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   int prec = RBC3D_PREC_FP64;  // rbc3d_set_precision
   int gmres_restart = 30;      // rbc3d_set_gmres_restart (PETSc's default length)
   int ncu = 256;               // compute units of the device
@@ -470,9 +475,8 @@ __device__ __forceinline__ double dist_on_sphere(double th0, double phi0, double
   return acos(d);
 }
 
-// stage timers: events recorded on stream st (the context's stream when null)
-void rbc_stage_begin(rbc3d_ctx *c, int stage, hipStream_t st = nullptr);
-void rbc_stage_end(rbc3d_ctx *c, int stage, hipStream_t st = nullptr);
+void rbc_stage_begin(rbc3d_ctx *c, int stage);
+void rbc_stage_end(rbc3d_ctx *c, int stage);
 void rbc_stage_flush(rbc3d_ctx *c);
 
 // Kernel launchers (defined in the .hip files)
@@ -492,14 +496,21 @@ int rbc_pme_spread(rbc3d_ctx *c, double c1, double c2, int cells, int walls);
 int rbc_pme_transform(rbc3d_ctx *c);
 // the gather at the wall's vertices (targets as set by the wall target list: all active)
 int rbc_pme_gather_wall(rbc3d_ctx *c, const double *d_A, double *d_v);
+// the same for the de-duplicated output: b(3 k + d) = v(i, d) + pme(i, d) / A_i for the
+// vertex i = last[k] of each unique index k < nu
+int rbc_pme_gather_wall_out(rbc3d_ctx *c, const double *d_A, const double *d_v, const int *d_last,
+                            int nu, double *d_b);
+// the wall-only single-layer spread with the sources' tractions read from the
+// de-duplicated vector u through the triangles' unique vertex indices e2u(nele, 3)
+int rbc_pme_spread_wall_u(rbc3d_ctx *c, double c1, const double *d_u, const int *d_e2u);
+// whether the wall-only spread kernels apply to this grid
+bool rbc_wall_spread_applies(const rbc3d_ctx *c);
 int rbc_pme_gather(rbc3d_ctx *c, int nT, const double *d_tx, const int *d_act, const double *d_A,
                    double *d_v);
 int rbc_wall_setup(rbc3d_ctx *c);
 int rbc_wall_prepare(rbc3d_ctx *c);
-// (on: the stream to run on, the context's stream when null)
 int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double c1, int nT, const double *d_tx, const int *d_tindx,
-                             const int *d_act, const double *d_A, double *d_v, bool wall_targets,
-                             hipStream_t on = nullptr);
+                             const int *d_act, const double *d_A, double *d_v, bool wall_targets);
 typedef std::function<int(const double *, double *)> MatvecFn;
 // Restarted GMRES (PETSc KSPGMRES semantics).  matvec_lo (optional): a cheaper, inexact
 // operator that Arnoldi steps use once the residual estimate is below lo_below times

@@ -19,10 +19,14 @@
 #include <complex>
 
 #include "internal.h"
+#include "fft_fixed.h"
 
 #define SPR_B 8        // grid brick edge
 #define SPR_T 256
 #define FFT_T 256
+#ifndef RBC_FX_ON
+#define RBC_FX_ON true  // the compile-time-size kernels where an instance exists
+#endif
 #ifndef RBC_XY_LDS
 #define RBC_XY_LDS (80 * 1024)
 #endif
@@ -432,7 +436,8 @@ __global__ void k_spread_merge(SpreadArgs<T> a, const T *__restrict__ tiles) {
 // c1 ((wx wy) wz) f to its point.  Every point adds its sources in ascending order with
 // the reference's expressions (PME_Distrib_Source :108-130, Distrib_Source :405-443)
 // and no FMA contraction: the grid is the CPU restatement's bit for bit (a point off a
-// source's stencil adds an exact zero).  Bricks without sources write zeros.
+// source's stencil adds an exact zero).  Bricks without sources write zeros.  The split
+// form stages and sums in FP32, the scalar of its grid.
 #define WSB 4          // brick edge in grid points (a brick is 64 points)
 #define WSB_CH 64      // sources staged per round
 #define WSB_PMAX 8
@@ -444,6 +449,8 @@ struct WspArgs {
   const double *wts;                // per source 3 P weights
   const int *e2v;
   const double *wf, *area;
+  const double *u;    // when set: tractions from the de-duplicated vector u(3, nunique)
+  const int *e2u;     //   through the triangles' unique vertex indices e2u(nele, 3)
   double *fs;  // per-source values f(nele, 3)
   T *grid;
 };
@@ -455,6 +462,13 @@ __global__ void k_wsp_src(WspArgs<T> a) {
 #pragma clang fp contract(off)
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= a.nele) return;
+  if (a.u) {
+    const int g0 = a.e2u[e], g1 = a.e2u[e + a.nele], g2 = a.e2u[e + 2 * (size_t)a.nele];
+    for (int d = 0; d < 3; d++)
+      a.fs[e + (size_t)a.nele * d] =
+          (1.0 / 3) * (a.u[3 * g0 + d] + a.u[3 * g1 + d] + a.u[3 * g2 + d]) * a.area[e];
+    return;
+  }
   const int v0 = a.e2v[e] - 1, v1 = a.e2v[e + a.nele] - 1, v2 = a.e2v[e + 2 * (size_t)a.nele] - 1;
   for (int d = 0; d < 3; d++) {
     const size_t o = (size_t)a.nvert * d;
@@ -463,12 +477,13 @@ __global__ void k_wsp_src(WspArgs<T> a) {
 }
 
 // one source staged for a brick: wx wy on the 16 (x, y) points, wz on the 4 z planes,
-// the value
+// the value; in the grid's scalar A (the split form stages and sums in FP32)
+template <typename A>
 struct WspStage {
-  double xy[16], z[4], f[3];
+  A xy[16], z[4], f[3];
 };
 template <typename T>
-__device__ __forceinline__ void wsp_fetch(WspStage &r, const WspArgs<T> &a, int k) {
+__device__ __forceinline__ void wsp_fetch(WspStage<T> &r, const WspArgs<T> &a, int k) {
 #pragma clang fp contract(off)
   const int e = a.bsrc[k], rp = a.brel[k], P = a.P;
   const double *w = a.wts + (size_t)3 * P * e;
@@ -485,26 +500,39 @@ __device__ __forceinline__ void wsp_fetch(WspStage &r, const WspArgs<T> &a, int 
 #pragma unroll
   for (int ly = 0; ly < WSB; ly++)
 #pragma unroll
-    for (int lx = 0; lx < WSB; lx++) r.xy[lx + WSB * ly] = wd[0][lx] * wd[1][ly];
+    for (int lx = 0; lx < WSB; lx++) r.xy[lx + WSB * ly] = (T)(wd[0][lx] * wd[1][ly]);
 #pragma unroll
-  for (int l = 0; l < WSB; l++) r.z[l] = wd[2][l];
+  for (int l = 0; l < WSB; l++) r.z[l] = (T)wd[2][l];
 #pragma unroll
-  for (int d = 0; d < 3; d++) r.f[d] = a.fs[e + (size_t)a.nele * d];
+  for (int d = 0; d < 3; d++) r.f[d] = (T)a.fs[e + (size_t)a.nele * d];
 }
+
+template <typename T>
+struct Vec4T;
+template <>
+struct Vec4T<double> {
+  typedef double4 t;
+};
+template <>
+struct Vec4T<float> {
+  typedef float4 t;
+};
 
 template <typename T>
 __global__ void __launch_bounds__(64) k_wsp_brick(WspArgs<T> a) {
 #pragma clang fp contract(off)
-  __shared__ double sxy[WSB_CH][18];  // rows padded: conflict-free 16-byte stores
-  __shared__ double4 sz[WSB_CH];
-  __shared__ double4 sf[WSB_CH];
+  typedef typename Vec4T<T>::t V4;
+  __shared__ T sxy[WSB_CH][16 + 16 / sizeof(T)];  // rows padded: conflict-free 16-byte stores
+  __shared__ V4 sz[WSB_CH];
+  __shared__ V4 sf[WSB_CH];
   const int lane = threadIdx.x, b = blockIdx.x;
   const int bx = b % a.nbx, by = (b / a.nbx) % a.nby, bz = b / (a.nbx * a.nby);
   const int lxy = lane & 15, lz = lane >> 4;
   const int gi = bx * WSB + (lane & 3), gj = by * WSB + ((lane >> 2) & 3), gk = bz * WSB + lz;
-  double acc0 = 0., acc1 = 0., acc2 = 0.;
+  T acc0 = 0., acc1 = 0., acc2 = 0.;
+  const T c1 = (T)a.c1;
   const int s0 = a.bstart[b], s1 = a.bstart[b + 1];
-  WspStage r;
+  WspStage<T> r;
   if (s0 + lane < s1) wsp_fetch(r, a, s0 + lane);
   for (int base = s0; base < s1; base += WSB_CH) {
     const int cnt = min(WSB_CH, s1 - base);
@@ -512,18 +540,18 @@ __global__ void __launch_bounds__(64) k_wsp_brick(WspArgs<T> a) {
     if (lane < cnt) {
 #pragma unroll
       for (int q = 0; q < 16; q++) sxy[lane][q] = r.xy[q];
-      sz[lane] = make_double4(r.z[0], r.z[1], r.z[2], r.z[3]);
-      sf[lane] = make_double4(r.f[0], r.f[1], r.f[2], 0.);
+      sz[lane] = V4{r.z[0], r.z[1], r.z[2], r.z[3]};
+      sf[lane] = V4{r.f[0], r.f[1], r.f[2], (T)0};
     }
     __syncthreads();
     // the next round's sources load while this round is summed
     if (base + WSB_CH + lane < s1) wsp_fetch(r, a, base + WSB_CH + lane);
-    const double *zrow = reinterpret_cast<const double *>(sz) + lz;
+    const T *zrow = reinterpret_cast<const T *>(sz) + lz;
 #pragma unroll 8
     for (int t = 0; t < cnt; t++) {
-      const double wxyz = sxy[t][lxy] * zrow[4 * t];
-      const double cw = a.c1 * wxyz;
-      const double4 f = sf[t];
+      const T wxyz = sxy[t][lxy] * zrow[4 * t];
+      const T cw = c1 * wxyz;
+      const V4 f = sf[t];
       acc0 = acc0 + cw * f.x;
       acc1 = acc1 + cw * f.y;
       acc2 = acc2 + cw * f.z;
@@ -532,9 +560,9 @@ __global__ void __launch_bounds__(64) k_wsp_brick(WspArgs<T> a) {
   if (gi < a.N1 && gj < a.N2 && gk < a.N3) {
     const size_t ng = (size_t)a.N1 * a.N2 * a.N3;
     const size_t g = gi + (size_t)a.N1 * (gj + (size_t)a.N2 * gk);
-    a.grid[g] = (T)acc0;
-    a.grid[g + ng] = (T)acc1;
-    a.grid[g + 2 * ng] = (T)acc2;
+    a.grid[g] = acc0;
+    a.grid[g + ng] = acc1;
+    a.grid[g + 2 * ng] = acc2;
   }
 }
 
@@ -610,7 +638,8 @@ static int wall_spread_setup(rbc3d_ctx *c) {
 }
 
 template <typename T>
-static int wall_spread(rbc3d_ctx *c, double c1) {
+static int wall_spread(rbc3d_ctx *c, double c1, const double *d_u = nullptr,
+                       const int *d_e2u = nullptr) {
   WallData &w = c->wall;
   RBC_TRY(wall_spread_setup(c));
   const int N1 = c->conf.Nb[0], N2 = c->conf.Nb[1], N3 = c->conf.Nb[2];
@@ -633,6 +662,8 @@ static int wall_spread(rbc3d_ctx *c, double c1) {
   a.e2v = w.d_e2v.p;
   a.wf = w.d_f.p;
   a.area = w.d_area.p;
+  a.u = d_u;
+  a.e2u = d_e2u;
   a.fs = w.d_wsb_f.p;
   a.grid = (T *)c->d_grid.p;
   rbc_stage_begin(c, ST_SPREAD);
@@ -742,6 +773,16 @@ static int pme_spread_t(rbc3d_ctx *c, double c1, double c2, int cells, int walls
   rbc_stage_end(c, ST_SPREAD);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;
+}
+
+bool rbc_wall_spread_applies(const rbc3d_ctx *c) { return c->wall.nvert > 0 && wall_spread_ok(c); }
+
+int rbc_pme_spread_wall_u(rbc3d_ctx *c, double c1, const double *d_u, const int *d_e2u) {
+  if (!rbc_wall_spread_applies(c)) return RBC3D_EINVAL;
+  c->flag_sl = fabs(c1) > 1.e-10;
+  c->flag_dl = false;
+  return c->prec == RBC3D_PREC_SPLIT ? wall_spread<float>(c, c1, d_u, d_e2u)
+                                     : wall_spread<double>(c, c1, d_u, d_e2u);
 }
 
 int rbc_pme_spread(rbc3d_ctx *c, double c1, double c2, int cells, int walls) {
@@ -1509,6 +1550,244 @@ __global__ void __launch_bounds__(FFT_T) k_fft_zgreen(GreenArgs<T> g, FftPlan<T>
   }
 }
 
+// ---------------------------------------------------------------------------
+// The transform at compile-time grid sizes (the tube's 96 x 96 x 180 and its reduced
+// 32 x 32 x 60): x-y plane kernels and the z + Green kernel on the two-pass line FFTs
+// of fft_fixed.h.  Same data layout and results (to rounding) as the general kernels.
+#ifndef FX_T
+#define FX_T 512  // threads of a plane workgroup
+#endif
+#ifndef FZ_T
+#define FZ_T 256  // threads of a z + Green workgroup
+#endif
+#ifndef FZ_TL180
+#define FZ_TL180 4  // columns per workgroup at N3 = 180 (H = 49; measured best of 2..7)
+#endif
+
+template <int N1, int N2>
+struct PlaneGeom {
+  static constexpr int H = N1 / 2 + 1, M = N2 / 2;
+  static constexpr int S = N1 + 1;  // row-pair stride of the x stage (odd: LDS banks)
+  static constexpr int P = H;       // row stride of the y stage
+  static constexpr int BUF = (M * S > N2 * P ? M * S : N2 * P);
+  static constexpr int LDS_ELEMS = BUF + N1 + N2;
+};
+
+template <typename T, int N1, int N2>
+__global__ void __launch_bounds__(FX_T) k_xy_fwd_fx(const T *__restrict__ in, typename CxT<T>::t *out,
+                                                    int N3, size_t ng, size_t nc,
+                                                    const typename CxT<T>::t *gtwx,
+                                                    const typename CxT<T>::t *gtwy) {
+  typedef typename CxT<T>::t C;
+  typedef decltype(C().x) Re;
+  typedef PlaneGeom<N1, N2> G;
+  constexpr int H = G::H, M = G::M, S = G::S, P = G::P;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  C *buf = reinterpret_cast<C *>(lds_raw), *twx = buf + G::BUF, *twy = twx + N1;
+  const int z = blockIdx.x % N3, f = blockIdx.x / N3;
+  for (int m = threadIdx.x; m < N1; m += FX_T) twx[m] = gtwx[m];
+  for (int m = threadIdx.x; m < N2; m += FX_T) twy[m] = gtwy[m];
+  const T *pin = in + ng * f + (size_t)N1 * N2 * z;
+  for (int t = threadIdx.x; t < M * N1; t += FX_T) {
+    const int m = t / N1, n = t % N1;
+    buf[m * S + n] = cpx<C>(pin[2 * m * N1 + n], pin[(2 * m + 1) * N1 + n]);
+  }
+  __syncthreads();
+  fft_fixed<N1, M, S, 1, true, -1, FX_T>(buf, twx);
+  // unpack row pair m into rows 2m and 2m + 1 (row y at y P): A[k] = (Z[k] + conj
+  // Z[N1-k]) / 2, B[k] = (Z[k] - conj Z[N1-k]) / 2i, k < H
+  {
+    constexpr int PER = (M * H + FX_T - 1) / FX_T;
+    C za[PER], zb[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int t = threadIdx.x + i * FX_T;
+      if (t < M * H) {
+        const int m = t / H, k = t % H;
+        za[i] = buf[m * S + k];
+        zb[i] = buf[m * S + (k == 0 ? 0 : N1 - k)];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int t = threadIdx.x + i * FX_T;
+      if (t < M * H) {
+        const int m = t / H, k = t % H;
+        const C p = za[i], q = zb[i];
+        buf[2 * m * P + k] = cpx<C>((Re)0.5 * (p.x + q.x), (Re)0.5 * (p.y - q.y));
+        buf[(2 * m + 1) * P + k] = cpx<C>((Re)0.5 * (p.y + q.y), (Re)-0.5 * (p.x - q.x));
+      }
+    }
+    __syncthreads();
+  }
+  fft_fixed<N2, H, 1, P, true, -1, FX_T>(buf, twy);
+  C *pout = out + nc * f + (size_t)H * N2 * z;
+  for (int t = threadIdx.x; t < H * N2; t += FX_T) pout[t] = buf[(t / H) * P + t % H];
+}
+
+template <typename T, int N1, int N2>
+__global__ void __launch_bounds__(FX_T) k_xy_inv_fx(const typename CxT<T>::t *in, T *__restrict__ out,
+                                                    int N3, size_t ng, size_t nc,
+                                                    const typename CxT<T>::t *gtwx,
+                                                    const typename CxT<T>::t *gtwy) {
+  typedef typename CxT<T>::t C;
+  typedef PlaneGeom<N1, N2> G;
+  constexpr int H = G::H, M = G::M, S = G::S, P = G::P;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  C *buf = reinterpret_cast<C *>(lds_raw), *twx = buf + G::BUF, *twy = twx + N1;
+  const int z = blockIdx.x % N3, f = blockIdx.x / N3;
+  for (int m = threadIdx.x; m < N1; m += FX_T) twx[m] = gtwx[m];
+  for (int m = threadIdx.x; m < N2; m += FX_T) twy[m] = gtwy[m];
+  const C *pin = in + nc * f + (size_t)H * N2 * z;
+  for (int t = threadIdx.x; t < H * N2; t += FX_T) buf[(t / H) * P + t % H] = pin[t];
+  __syncthreads();
+  fft_fixed<N2, H, 1, P, true, +1, FX_T>(buf, twy);
+  // pack: Z[k] = A[k] + i B[k], k < N1, A and B (rows 2m, 2m + 1) extended by Hermitian
+  // symmetry, the imaginary parts of the zero and Nyquist frequencies dropped (FFTW's
+  // c2r convention)
+  {
+    constexpr int PER = (M * N1 + FX_T - 1) / FX_T;
+    C za[PER], zb[PER];
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int t = threadIdx.x + i * FX_T;
+      if (t < M * N1) {
+        const int m = t / N1, k = t % N1;
+        const int kk = k < H ? k : N1 - k;
+        C A = buf[2 * m * P + kk], B = buf[(2 * m + 1) * P + kk];
+        if (kk == 0 || 2 * kk == N1) {
+          A.y = 0;
+          B.y = 0;
+        }
+        if (k >= H) {
+          A.y = -A.y;
+          B.y = -B.y;
+        }
+        za[i] = A;
+        zb[i] = B;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < PER; i++) {
+      const int t = threadIdx.x + i * FX_T;
+      if (t < M * N1) {
+        const int m = t / N1, k = t % N1;
+        buf[m * S + k] = cpx<C>(za[i].x - zb[i].y, za[i].y + zb[i].x);
+      }
+    }
+    __syncthreads();
+  }
+  fft_fixed<N1, M, S, 1, true, +1, FX_T>(buf, twx);
+  T *pout = out + ng * f + (size_t)N1 * N2 * z;
+  for (int t = threadIdx.x; t < M * N1; t += FX_T) {
+    const int m = t / N1, n = t % N1;
+    const C v = buf[m * S + n];
+    pout[2 * m * N1 + n] = v.x;
+    pout[(2 * m + 1) * N1 + n] = v.y;
+  }
+}
+
+// z stage of the wall-only transform (fields 0..2 in, 12..14 out): TL adjacent kx at one
+// ky per workgroup, forward z (sign +1), Green's function, inverse z, as k_fft_zgreen
+template <typename T, int N3, int TL>
+__global__ void __launch_bounds__(FZ_T) k_zg_fx(GreenArgs<T> g, const typename CxT<T>::t *gtw) {
+  typedef typename CxT<T>::t C;
+  constexpr int LS = N3 + 1;  // line stride (odd: LDS banks)
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  C *buf = reinterpret_cast<C *>(lds_raw), *tw = buf + 3 * TL * LS;
+  const int H = g.H;
+  const size_t nc = (size_t)H * g.N2 * N3;
+  const int ncb = (H + TL - 1) / TL;
+  const int blk = (int)(blockIdx.x % 8) * (int)(gridDim.x / 8) + (int)(blockIdx.x / 8);
+  if (blk >= ncb * g.N2) return;
+  const int j = blk / ncb, i0 = (blk % ncb) * TL, tl = min(TL, H - i0);
+  for (int m = threadIdx.x; m < N3; m += FZ_T) tw[m] = gtw[m];
+  for (int t = threadIdx.x; t < 3 * TL * N3; t += FZ_T) {
+    const int l = t % TL, r = t / TL, z = r % N3, f = r / N3;
+    C v = cpx<C>(0., 0.);
+    if (l < tl) v = g.C[(i0 + l) + (size_t)H * (j + (size_t)g.N2 * z) + nc * f];
+    buf[(f * TL + l) * LS + z] = v;
+  }
+  __syncthreads();
+  fft_fixed<N3, 3 * TL, LS, 1, true, +1, FZ_T>(buf, tw);
+  // Green's function point by point: each point's three outputs replace its three inputs
+  for (int t = threadIdx.x; t < TL * N3; t += FZ_T) {
+    const int l = t % TL, z = t / TL;
+    double2 vc[3] = {make_double2(0., 0.), make_double2(0., 0.), make_double2(0., 0.)};
+    if (l < tl)
+      green_at(g, i0 + l, j, z,
+               [&](int f) {
+                 const C v = buf[(f * TL + l) * LS + z];
+                 return make_double2(v.x, v.y);
+               },
+               vc);
+    for (int d = 0; d < 3; d++) {
+      C v;
+      v.x = (decltype(v.x))vc[d].x;
+      v.y = (decltype(v.y))vc[d].y;
+      buf[(d * TL + l) * LS + z] = v;
+    }
+  }
+  __syncthreads();
+  fft_fixed<N3, 3 * TL, LS, 1, true, -1, FZ_T>(buf, tw);
+  for (int t = threadIdx.x; t < 3 * TL * N3; t += FZ_T) {
+    const int l = t % TL, r = t / TL, z = r % N3, d = r / N3;
+    if (l < tl) g.C[(i0 + l) + (size_t)H * (j + (size_t)g.N2 * z) + nc * (12 + d)] = buf[(d * TL + l) * LS + z];
+  }
+}
+
+template <typename T, int N1, int N2>
+static size_t fx_plane_lds() {
+  return sizeof(typename CxT<T>::t) * PlaneGeom<N1, N2>::LDS_ELEMS;
+}
+
+// the compile-time grids: x-y planes for any fields, the z stage with Green for the
+// wall-only case (three single-layer fields); returns false where no instance exists
+template <typename T>
+static bool fx_xy(rbc3d_ctx *c, bool forward, int nf, const void *in, void *out) {
+  typedef typename CxT<T>::t CT;
+  const int N1 = c->conf.Nb[0], N2 = c->conf.Nb[1], N3 = c->conf.Nb[2];
+  const size_t ng = (size_t)N1 * N2 * N3, nc = (size_t)(N1 / 2 + 1) * N2 * N3;
+  const CT *twx = make_plan<T>(c, 0, -1).tw, *twy = make_plan<T>(c, 1, -1).tw;
+  hipStream_t st = c->stream;
+  const unsigned nb = (unsigned)(nf * N3);
+#define RBC_FX_XY(A, B)                                                                          \
+  if (N1 == A && N2 == B && fx_plane_lds<T, A, B>() <= (size_t)c->lds_max) {                    \
+    if (forward)                                                                                 \
+      k_xy_fwd_fx<T, A, B><<<nb, FX_T, fx_plane_lds<T, A, B>(), st>>>((const T *)in, (CT *)out, \
+                                                                      N3, ng, nc, twx, twy);     \
+    else                                                                                         \
+      k_xy_inv_fx<T, A, B><<<nb, FX_T, fx_plane_lds<T, A, B>(), st>>>((const CT *)in, (T *)out, \
+                                                                      N3, ng, nc, twx, twy);     \
+    return true;                                                                                 \
+  }
+  RBC_FX_XY(96, 96)
+  RBC_FX_XY(32, 32)
+#undef RBC_FX_XY
+  return false;
+}
+
+template <typename T>
+static bool fx_zgreen(rbc3d_ctx *c, const GreenArgs<T> *g) {  // g null: only whether it applies
+  typedef typename CxT<T>::t CT;
+  const int N3 = c->conf.Nb[2], H = c->conf.Nb[0] / 2 + 1, N2 = c->conf.Nb[1];
+  const CT *tw = make_plan<T>(c, 2, -1).tw;
+#define RBC_FX_Z(NZ, HH, TLC)                                                                    \
+  if (N3 == NZ && H == HH) {                                                                     \
+    const size_t lds = sizeof(CT) * (3 * (size_t)TLC * (NZ + 1) + NZ);                           \
+    if (lds > (size_t)c->lds_max) return false;                                                  \
+    const unsigned nblk = (unsigned)((((HH + TLC - 1) / TLC) * N2 + 7) / 8 * 8);                \
+    if (g) k_zg_fx<T, NZ, TLC><<<nblk, FZ_T, lds, c->stream>>>(*g, tw);                          \
+    return true;                                                                                 \
+  }
+  RBC_FX_Z(180, 49, FZ_TL180)
+  RBC_FX_Z(60, 17, 17)
+#undef RBC_FX_Z
+  return false;
+}
+
 template <typename T>
 static int pme_transform_t(rbc3d_ctx *c) {
   typedef typename CxT<T>::t CT;
@@ -1521,7 +1800,12 @@ static int pme_transform_t(rbc3d_ctx *c) {
   rbc_stage_begin(c, ST_FFT);
   size_t xy_lds = 0;
   const bool xy = xy_plane_ok<T>(c, &xy_lds);
-  if (nf > 0 && xy) {
+  bool xy_fixed_fwd = false;
+  if (nf > 0 && RBC_FX_ON)
+    xy_fixed_fwd = fx_xy<T>(c, true, nf, (const T *)c->d_grid.p + ng * f0, C + nc * f0);
+  if (xy_fixed_fwd) {
+    HIP_TRY(hipGetLastError());
+  } else if (nf > 0 && xy) {
     // x and y in one pass per plane
     k_fft_xy_fwd<T><<<(unsigned)(nf * N3), XY_T, xy_lds, st>>>(
         xy_args<T>(c, nf), (const T *)c->d_grid.p + ng * f0, C + nc * f0, make_plan<T>(c, 0, -1),
@@ -1545,8 +1829,9 @@ static int pme_transform_t(rbc3d_ctx *c) {
   const int TLz = nf > 0 && zlds > N3 * sizeof(CT)
                       ? (int)std::min<size_t>(8, (zlds - N3 * sizeof(CT)) / zcol)
                       : 0;
-  const bool fused = nf > 0 && TLz >= 2;
-  if (nf > 0 && !fused) RBC_TRY(fft_c2c<T>(c, C + nc * f0, 2, +1, nf));
+  const bool z_fixed = RBC_FX_ON && nf == 3 && f0 == 0 && fx_zgreen<T>(c, nullptr);
+  const bool fused = !z_fixed && nf > 0 && TLz >= 2;
+  if (nf > 0 && !fused && !z_fixed) RBC_TRY(fft_c2c<T>(c, C + nc * f0, 2, +1, nf));
   rbc_stage_end(c, ST_FFT);
   GreenArgs<T> g;
   g.N1 = N1;
@@ -1560,7 +1845,16 @@ static int pme_transform_t(rbc3d_ctx *c) {
   g.flag_dl = c->flag_dl;
   g.bb = c->d_bb.p;
   g.C = C;
-  if (fused) {
+  if (z_fixed) {
+    if ((long long)15 * nc >= (1LL << 31)) {
+      rbc_set_error("PME grid too large for 32-bit FFT offsets");
+      return RBC3D_EINVAL;
+    }
+    rbc_stage_begin(c, ST_GREEN);
+    fx_zgreen<T>(c, &g);
+    rbc_stage_end(c, ST_GREEN);
+    HIP_TRY(hipGetLastError());
+  } else if (fused) {
     if ((long long)15 * nc >= (1LL << 31)) {
       rbc_set_error("PME grid too large for 32-bit FFT offsets");
       return RBC3D_EINVAL;
@@ -1579,8 +1873,9 @@ static int pme_transform_t(rbc3d_ctx *c) {
     HIP_TRY(hipGetLastError());
   }
   rbc_stage_begin(c, ST_FFT);
-  if (!fused) RBC_TRY(fft_c2c<T>(c, C + nc * 12, 2, -1, 3));
-  if (xy) {
+  if (!fused && !z_fixed) RBC_TRY(fft_c2c<T>(c, C + nc * 12, 2, -1, 3));
+  if (RBC_FX_ON && fx_xy<T>(c, false, 3, C + nc * 12, c->d_vgrid.p)) {
+  } else if (xy) {
     k_fft_xy_inv<T><<<(unsigned)(3 * N3), XY_T, xy_lds, st>>>(
         xy_args<T>(c, 3), C + nc * 12, (T *)c->d_vgrid.p, make_plan<T>(c, 0, +1),
         make_plan<T>(c, 1, +1));
@@ -1690,40 +1985,65 @@ static int pme_gather_t(rbc3d_ctx *c, int nT, const double *d_tx, const int *d_a
 // The gather at the wall's vertices (the targets of the no-slip operator and of the
 // residual wall velocity): the vertices do not move, so each one's stencil origin and
 // B-spline weights are computed once per wall and grid.  GATHER_WL lanes per vertex
-// share its P x P (y, z) stencil rows, combined with shuffles; the division by the
-// target's A closes the sum as in the general gather.
-#define GATHER_WL 16
-template <int PC, typename T>
+// share its P^3 stencil points, x fastest (a load instruction then touches a few rows
+// rather than a row per lane), combined with shuffles; the division by the target's A
+// closes the sum as in the general gather.
+#ifndef GATHER_WL
+#define GATHER_WL 32
+#endif
+#ifndef GATHER_PT
+#define GATHER_PT 1  // lane per stencil point (measured faster than lane per row)
+#endif
+// OUT: the targets are the vertices last[k], k < nout, and the result goes to
+// b(3 k + d) = vin(i, d) + sum / A_i; else v(i, d) += sum / A_i for every vertex i < nT
+template <int PC, typename T, bool OUT = false>
 __global__ void __launch_bounds__(256) k_gather_wall(int nT, int N1, int N2, int N3, int P_,
                                                       const int *__restrict__ org,
-                                                      const double *__restrict__ wts,
+                                                      const T *__restrict__ wts,
                                                       const T *__restrict__ vv,
-                                                      const double *__restrict__ tA, double *v) {
+                                                      const double *__restrict__ tA, double *v,
+                                                      const double *__restrict__ vin = nullptr,
+                                                      const int *__restrict__ last = nullptr,
+                                                      int nout = 0) {
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
-  const int ti = gid / GATHER_WL, sub = gid % GATHER_WL;
-  const bool live = ti < nT;
+  const int tk = gid / GATHER_WL, sub = gid % GATHER_WL;
+  const bool live = OUT ? tk < nout : tk < nT;
+  const int ti = OUT ? (live ? last[tk] : 0) : tk;
   const int P = PC ? PC : P_;
-  double v0 = 0, v1 = 0, v2 = 0;
+  T v0 = 0, v1 = 0, v2 = 0;  // the split form sums in FP32, the scalar of its grid
   if (live) {
     const int imin = org[3 * ti], jmin = org[3 * ti + 1], kmin = org[3 * ti + 2];
-    const double *W = wts + (size_t)3 * P * ti;
+    const T *W = wts + (size_t)3 * P * ti;
     const size_t ng = (size_t)N1 * N2 * N3;
+#if GATHER_PT
+    // lane per stencil point, x fastest: one load instruction covers a few rows
+    for (int p = sub; p < P * P * P; p += GATHER_WL) {
+      const int i0 = p % P, r = p / P, j0 = r % P, k0 = r / P;
+      const int k = imod_d(kmin + k0, N3), j = imod_d(jmin + j0, N2), i = imod_d(imin + i0, N1);
+      const T w = W[i0] * (W[P + j0] * W[2 * P + k0]);
+      const size_t o = i + (size_t)N1 * (j + (size_t)N2 * k);
+      v0 += w * vv[o];
+      v1 += w * vv[o + ng];
+      v2 += w * vv[o + 2 * ng];
+    }
+#else
     for (int r = sub; r < P * P; r += GATHER_WL) {
       const int j0 = r % P, k0 = r / P;
       const int k = imod_d(kmin + k0, N3), j = imod_d(jmin + j0, N2);
-      const double wjk = W[P + j0] * W[2 * P + k0];
+      const T wjk = W[P + j0] * W[2 * P + k0];
       const size_t row = (size_t)N1 * (j + (size_t)N2 * k);
 #pragma unroll
       for (int i0 = 0; i0 < (PC ? PC : 16); i0++) {
         if (!PC && i0 >= P) break;
         const int i = imod_d(imin + i0, N1);
-        const double w = W[i0] * wjk;
+        const T w = W[i0] * wjk;
         const size_t o = row + i;
         v0 += w * vv[o];
         v1 += w * vv[o + ng];
         v2 += w * vv[o + 2 * ng];
       }
     }
+#endif
   }
   for (int o = GATHER_WL / 2; o > 0; o >>= 1) {
     v0 += __shfl_xor(v0, o, GATHER_WL);
@@ -1732,9 +2052,16 @@ __global__ void __launch_bounds__(256) k_gather_wall(int nT, int N1, int N2, int
   }
   if (live && sub == 0) {
     const double A = tA[ti];
-    v[ti] += v0 / A;
-    v[ti + (size_t)nT] += v1 / A;
-    v[ti + 2 * (size_t)nT] += v2 / A;
+    const double u0 = v0, u1 = v1, u2 = v2;
+    if constexpr (OUT) {
+      v[3 * (size_t)tk] = vin[ti] + u0 / A;
+      v[3 * (size_t)tk + 1] = vin[ti + (size_t)nT] + u1 / A;
+      v[3 * (size_t)tk + 2] = vin[ti + 2 * (size_t)nT] + u2 / A;
+    } else {
+      v[ti] += u0 / A;
+      v[ti + (size_t)nT] += u1 / A;
+      v[ti + 2 * (size_t)nT] += u2 / A;
+    }
   }
 }
 
@@ -1753,10 +2080,13 @@ static int wall_gather_setup(rbc3d_ctx *c) {
     for (int d = 0; d < 3; d++)
       org[3 * i + d] = bspline_d(x[i + (size_t)nv * d] * (c->conf.Nb[d] / c->Lb[d]), P,
                                  &wts[(size_t)3 * P * i + d * P]);
+  std::vector<float> wtsf(wts.begin(), wts.end());
   RBC_TRY(w.d_wg_org.ensure(org.size()));
   RBC_TRY(w.d_wg_w.ensure(wts.size()));
+  RBC_TRY(w.d_wg_wf.ensure(wts.size()));
   HIP_TRY(hipMemcpy(w.d_wg_org.p, org.data(), org.size() * sizeof(int), hipMemcpyHostToDevice));
   HIP_TRY(hipMemcpy(w.d_wg_w.p, wts.data(), wts.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(w.d_wg_wf.p, wtsf.data(), wtsf.size() * sizeof(float), hipMemcpyHostToDevice));
   w.wgt_ready = true;
   return RBC3D_OK;
 }
@@ -1772,15 +2102,42 @@ int rbc_pme_gather_wall(rbc3d_ctx *c, const double *d_A, double *d_v) {
   if (c->pme_grid_prec == RBC3D_PREC_SPLIT) {
     const float *vv = (const float *)c->d_vgrid.p;
     if (P == 6)
-      k_gather_wall<6, float><<<nblk, 256, 0, c->stream>>>(nT, N1, N2, N3, P, w.d_wg_org.p, w.d_wg_w.p, vv, d_A, d_v);
+      k_gather_wall<6, float><<<nblk, 256, 0, c->stream>>>(nT, N1, N2, N3, P, w.d_wg_org.p, w.d_wg_wf.p, vv, d_A, d_v);
     else
-      k_gather_wall<0, float><<<nblk, 256, 0, c->stream>>>(nT, N1, N2, N3, P, w.d_wg_org.p, w.d_wg_w.p, vv, d_A, d_v);
+      k_gather_wall<0, float><<<nblk, 256, 0, c->stream>>>(nT, N1, N2, N3, P, w.d_wg_org.p, w.d_wg_wf.p, vv, d_A, d_v);
   } else {
     const double *vv = (const double *)c->d_vgrid.p;
     if (P == 6)
       k_gather_wall<6, double><<<nblk, 256, 0, c->stream>>>(nT, N1, N2, N3, P, w.d_wg_org.p, w.d_wg_w.p, vv, d_A, d_v);
     else
       k_gather_wall<0, double><<<nblk, 256, 0, c->stream>>>(nT, N1, N2, N3, P, w.d_wg_org.p, w.d_wg_w.p, vv, d_A, d_v);
+  }
+  rbc_stage_end(c, ST_GATHER);
+  HIP_TRY(hipGetLastError());
+  return RBC3D_OK;
+}
+
+int rbc_pme_gather_wall_out(rbc3d_ctx *c, const double *d_A, const double *d_v, const int *d_last,
+                            int nu, double *d_b) {
+  WallData &w = c->wall;
+  const int nT = w.nvert;
+  if (nT <= 0 || nu <= 0) return RBC3D_OK;
+  RBC_TRY(wall_gather_setup(c));
+  const int N1 = c->conf.Nb[0], N2 = c->conf.Nb[1], N3 = c->conf.Nb[2], P = c->conf.P;
+  const unsigned nblk = (unsigned)(((size_t)nu * GATHER_WL + 255) / 256);
+  rbc_stage_begin(c, ST_GATHER);
+  if (c->pme_grid_prec == RBC3D_PREC_SPLIT) {
+    const float *vv = (const float *)c->d_vgrid.p;
+    if (P == 6)
+      k_gather_wall<6, float, true><<<nblk, 256, 0, c->stream>>>(nT, N1, N2, N3, P, w.d_wg_org.p, w.d_wg_wf.p, vv, d_A, d_b, d_v, d_last, nu);
+    else
+      k_gather_wall<0, float, true><<<nblk, 256, 0, c->stream>>>(nT, N1, N2, N3, P, w.d_wg_org.p, w.d_wg_wf.p, vv, d_A, d_b, d_v, d_last, nu);
+  } else {
+    const double *vv = (const double *)c->d_vgrid.p;
+    if (P == 6)
+      k_gather_wall<6, double, true><<<nblk, 256, 0, c->stream>>>(nT, N1, N2, N3, P, w.d_wg_org.p, w.d_wg_w.p, vv, d_A, d_b, d_v, d_last, nu);
+    else
+      k_gather_wall<0, double, true><<<nblk, 256, 0, c->stream>>>(nT, N1, N2, N3, P, w.d_wg_org.p, w.d_wg_w.p, vv, d_A, d_b, d_v, d_last, nu);
   }
   rbc_stage_end(c, ST_GATHER);
   HIP_TRY(hipGetLastError());

@@ -528,7 +528,14 @@ __global__ void k_wall_sym6(size_t nb, const double *__restrict__ b9, double *b6
 // coalesced 64-lane access; the 3 traction components of the block's column are
 // gathered (the traction vector stays in L2).
 // V: the block scalar (double, or float in the split form); sums are FP64 either way.
-template <typename V>
+#ifndef RBC_SPMV_NT
+#define RBC_SPMV_NT 1  // the streamed matrix read with nontemporal loads (measured faster)
+#endif
+// One wavefront per target row of the self-matrix: v(i) (+)= c1 / A_i sum_j B_ij f_j.
+// FROM_U: the sources are read from the de-duplicated vector u (3 per unique vertex,
+// col holding unique indices); else from the vertex array f(nvert, 3).  ASSIGN: v is
+// written rather than accumulated (the same values: the sum starts from zero).
+template <typename V, bool FROM_U = false, bool ASSIGN = false>
 __global__ void __launch_bounds__(256) k_wall_spmv(int nvert, double c1, const int *__restrict__ rowptr,
                                                    const int *__restrict__ bcol,
                                                    const V *__restrict__ bval,
@@ -539,10 +546,27 @@ __global__ void __launch_bounds__(256) k_wall_spmv(int nvert, double c1, const i
   const size_t nbt = rowptr[nvert];
   double s0 = 0, s1 = 0, s2 = 0;
   auto block = [&](int k) {
+#if RBC_SPMV_NT
+    const int j = __builtin_nontemporal_load(bcol + k);
+    const double b00 = __builtin_nontemporal_load(bval + k), b11 = __builtin_nontemporal_load(bval + nbt + k),
+                 b22 = __builtin_nontemporal_load(bval + 2 * nbt + k);
+    const double b01 = __builtin_nontemporal_load(bval + 3 * nbt + k), b02 = __builtin_nontemporal_load(bval + 4 * nbt + k),
+                 b12 = __builtin_nontemporal_load(bval + 5 * nbt + k);
+#else
     const int j = bcol[k];
     const double b00 = bval[k], b11 = bval[nbt + k], b22 = bval[2 * nbt + k];
     const double b01 = bval[3 * nbt + k], b02 = bval[4 * nbt + k], b12 = bval[5 * nbt + k];
-    const double f0 = f[j], f1 = f[j + nvert], f2 = f[j + 2 * (size_t)nvert];
+#endif
+    double f0, f1, f2;
+    if constexpr (FROM_U) {
+      f0 = f[3 * j];
+      f1 = f[3 * j + 1];
+      f2 = f[3 * j + 2];
+    } else {
+      f0 = f[j];
+      f1 = f[j + nvert];
+      f2 = f[j + 2 * (size_t)nvert];
+    }
     s0 += b00 * f0 + b01 * f1 + b02 * f2;
     s1 += b01 * f0 + b11 * f1 + b12 * f2;
     s2 += b02 * f0 + b12 * f1 + b22 * f2;
@@ -565,9 +589,15 @@ __global__ void __launch_bounds__(256) k_wall_spmv(int nvert, double c1, const i
   }
   if (lane == 0) {
     const double A = tA[i];
-    v[i] += c1 * s0 / A;
-    v[i + nvert] += c1 * s1 / A;
-    v[i + 2 * (size_t)nvert] += c1 * s2 / A;
+    if constexpr (ASSIGN) {
+      v[i] = c1 * s0 / A;
+      v[i + nvert] = c1 * s1 / A;
+      v[i + 2 * (size_t)nvert] = c1 * s2 / A;
+    } else {
+      v[i] += c1 * s0 / A;
+      v[i + nvert] += c1 * s1 / A;
+      v[i + 2 * (size_t)nvert] += c1 * s2 / A;
+    }
   }
 }
 
@@ -737,6 +767,7 @@ extern "C" int rbc3d_set_wall(rbc3d_ctx *c, int nvert, int nele, const double *x
   w.lmodes_ready = false;
   w.wsp_ready = false;
   w.wgt_ready = false;
+  w.op_ready = false;
   HIP_TRY(hipStreamSynchronize(st));
   return RBC3D_OK;
 }
@@ -844,12 +875,11 @@ extern "C" int rbc3d_wall_self_nnz(rbc3d_ctx *c, long long *nnz) {
 }
 
 int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double c1, int nT, const double *d_tx, const int *d_tindx,
-                             const int *d_act, const double *d_A, double *d_v, bool wall_targets,
-                             hipStream_t on) {
+                             const int *d_act, const double *d_A, double *d_v, bool wall_targets) {
   WallData &w = c->wall;
   if (!w.nvert || nT == 0) return RBC3D_OK;
-  hipStream_t st = on ? on : c->stream;
-  rbc_stage_begin(c, ST_WALL, st);
+  hipStream_t st = c->stream;
+  rbc_stage_begin(c, ST_WALL);
   if (wall_targets) {
     // wall vertices: the precomputed self-interaction; with one wall that is all
     if (!w.prepared) RBC_TRY(rbc_wall_prepare(c));
@@ -887,7 +917,7 @@ int rbc_add_int_on_walls_dev(rbc3d_ctx *c, double c1, int nT, const double *d_tx
     k_wall_targets_tiled<<<nblk, WT_T * WT_W, 0, st>>>(wall_args(c), c1, nT, d_tx, d_tindx, d_act, d_A,
                                                 w.d_tg_order.p, w.d_tg_start.p, coff, nbins, d_v);
   }
-  rbc_stage_end(c, ST_WALL, st);
+  rbc_stage_end(c, ST_WALL);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;
 }
@@ -962,28 +992,87 @@ static int wall_targets_dev(rbc3d_ctx *c) {
   return RBC3D_OK;
 }
 
+__global__ void k_col_unique(long long nb, const int *__restrict__ col, const int *__restrict__ glb,
+                             int *colu) {
+  const long long k = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+  if (k < nb) colu[k] = glb[col[k]] - 1;
+}
+__global__ void k_e2v_unique(int nele, const int *__restrict__ e2v, const int *__restrict__ glb,
+                             int *e2u) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < 3 * nele) e2u[k] = glb[e2v[k] - 1] - 1;
+}
+__global__ void k_fill2(int n, double *a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 2.;
+}
+
+// The operator on the de-duplicated vector without assembling it into the vertex array:
+// the self-matrix columns and the triangles' vertices are mapped to unique indices once,
+// the SpMV and the spread read u directly, and the gather writes b for the last vertex
+// of each unique index (AssembleArray's choice) as v_self + v_pme / A, the same sums as
+// assembling, accumulating and de-duplicating in turn.  (The SpMV on a second stream
+// beside the PME chain measured slower than in turn: both are bound by memory.)
+static int wall_matmult_fused(rbc3d_ctx *c, const double *d_u, double *d_b) {
+  WallData &w = c->wall;
+  hipStream_t st = c->stream;
+  const int nv = w.nvert, nu = w.ndof / 3;
+  if (!w.op_ready) {
+    const long long nb = w.nnz / 9;
+    RBC_TRY(w.d_colu.ensure(nb + 1));
+    RBC_TRY(w.d_e2u.ensure(3 * (size_t)w.nele + 1));
+    RBC_TRY(w.d_A2.ensure(nv + 1));
+    k_col_unique<<<(unsigned)((nb + 255) / 256), 256, 0, st>>>(nb, w.d_col.p, w.d_glb.p, w.d_colu.p);
+    k_e2v_unique<<<(3 * w.nele + 255) / 256, 256, 0, st>>>(w.nele, w.d_e2v.p, w.d_glb.p, w.d_e2u.p);
+    k_fill2<<<(nv + 255) / 256, 256, 0, st>>>(nv, w.d_A2.p);
+    HIP_TRY(hipGetLastError());
+    w.op_ready = true;
+  }
+  RBC_TRY(c->d_v.ensure((size_t)3 * nv + 1));
+  const double c1 = 1. / (4. * RBC_PI);
+  RBC_TRY(rbc_pme_init(c));
+  {
+    hipStream_t ss = st;
+    rbc_stage_begin(c, ST_WALL);
+    const unsigned nblk = (unsigned)(((size_t)nv * 64 + 255) / 256);
+    if (c->prec == RBC3D_PREC_SPLIT) {
+      if (!w.valf_ready) {
+        const size_t nv6 = (size_t)w.nnz / 9 * 6;
+        RBC_TRY(w.d_valf.ensure(nv6 + 1));
+        k_to_float<<<(unsigned)((nv6 + 255) / 256), 256, 0, ss>>>(w.d_val.p, nv6, w.d_valf.p);
+        w.valf_ready = true;
+      }
+      k_wall_spmv<float, true, true><<<nblk, 256, 0, ss>>>(nv, c1, w.d_rowptr.p, w.d_colu.p,
+                                                           w.d_valf.p, d_u, w.d_A2.p, c->d_v.p);
+    } else {
+      k_wall_spmv<double, true, true><<<nblk, 256, 0, ss>>>(nv, c1, w.d_rowptr.p, w.d_colu.p,
+                                                            w.d_val.p, d_u, w.d_A2.p, c->d_v.p);
+    }
+    HIP_TRY(hipGetLastError());
+    rbc_stage_end(c, ST_WALL);
+  }
+  RBC_TRY(rbc_pme_spread_wall_u(c, c1, d_u, w.d_e2u.p));
+  RBC_TRY(rbc_pme_transform(c));
+  RBC_TRY(rbc_pme_gather_wall_out(c, w.d_A2.p, c->d_v.p, w.d_glb.p + nv, nu, d_b));
+  return RBC3D_OK;
+}
+
 int rbc_wall_matmult_dev(rbc3d_ctx *c, const double *d_u, double *d_b) {
   WallData &w = c->wall;
   if (!w.nvert) return RBC3D_ESTATE;
   if (!w.prepared) RBC_TRY(rbc_wall_prepare(c));
+  if (rbc_wall_spread_applies(c)) return wall_matmult_fused(c, d_u, d_b);
   hipStream_t st = c->stream;
   int nv = w.nvert, nu = w.ndof / 3;
   RBC_TRY(wall_targets_dev(c));
   k_assemble_in<<<(nv + 255) / 256, 256, 0, st>>>(nv, w.d_glb.p, d_u, w.d_f.p);
   HIP_TRY(hipMemsetAsync(c->d_v.p, 0, sizeof(double) * 3 * nv, st));
   double c1 = 1. / (4. * RBC_PI);
-  RBC_TRY(rbc_pme_init(c));
-  // the self-matrix SpMV (into v) runs on the side stream beside the spread and the
-  // transforms (which read the same traction and write only the grids); the gather
-  // adds the PME part after it, so v sums in the same order as one stream would
-  HIP_TRY(hipEventRecord(c->ev_fork, st));
-  HIP_TRY(hipStreamWaitEvent(c->side, c->ev_fork, 0));
   RBC_TRY(rbc_add_int_on_walls_dev(c, c1, nv, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
-                                   c->d_v.p, true, c->side));
-  HIP_TRY(hipEventRecord(c->ev_join, c->side));
+                                   c->d_v.p, true));
+  RBC_TRY(rbc_pme_init(c));
   RBC_TRY(rbc_pme_spread(c, c1, 0., 0, 1));
   RBC_TRY(rbc_pme_transform(c));
-  HIP_TRY(hipStreamWaitEvent(st, c->ev_join, 0));
   RBC_TRY(rbc_pme_gather_wall(c, c->d_tA.p, c->d_v.p));
   k_assemble_out<<<(nu + 255) / 256, 256, 0, st>>>(nu, nv, w.d_glb.p + nv, c->d_v.p, d_b);
   HIP_TRY(hipGetLastError());
