@@ -42,6 +42,12 @@ PEAK_HBM_GBS = 8000.0
 K_PAIR_C2_TRAFFIC_BYTES = int((6029.4 + 37.4) * 1e6)
 
 
+def progress(msg):
+    """One line on stderr when a block of the benchmark ends (the JSON line stays the
+    only output on stdout)."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
 def host_threads():
     """The host threads this process may use: OMP_NUM_THREADS when set (the GPU box sets
     it to its per-GPU CPU share), else the CPUs in the affinity mask."""
@@ -204,6 +210,7 @@ def timestep_bench(steps, warmup, maxit=4000, forms=None):
                     f"({w.wall[3]} Tri3), box {w.Lb}, PME {w.Nb}, viscosity ratio 1",
         "steps": steps, "wall_self_matrix_nnz": int(nnz.value)}
     plain, vp = run("plain")
+    progress(f"tube timestep plain: {plain['s_per_timestep']} s")
     out["plain"] = plain
     only = forms
     forms = ["split", "fp64_restart150", "chosen"] + (["chosen_learned"] if mlp is not None else [])
@@ -211,6 +218,7 @@ def timestep_bench(steps, warmup, maxit=4000, forms=None):
         forms = [f for f in forms if f in only]
     for form in forms:
         r, v = run(form)
+        progress(f"tube timestep {form}: {r['s_per_timestep']} s")
         r["velocity_rel_err_vs_plain"] = float(np.abs(v - vp).max() / np.abs(vp).max())
         r["speedup_vs_plain"] = round(plain["s_per_timestep"] / r["s_per_timestep"], 3)
         out[form] = r
@@ -483,6 +491,7 @@ def main():
     # the headline: every matvec does all of its work, geometry included (as the
     # reference's MyMatMult does)
     ms_per_step, stages = timed_matvecs(False, args.steps, args.warmup)
+    progress(f"{args.config} plain matvec: {ms_per_step:.3f} ms")
     # within one GMRES solve the geometry is fixed: neighbour lists and projections
     # computed by the first matvec are reused by the others
     ms_reuse, stages_reuse = timed_matvecs(True, args.steps, 1)
@@ -495,6 +504,7 @@ def main():
     ms_split_reuse, _ = timed_matvecs(True, args.steps, 1)
     torch.cuda.synchronize()
     split_err = float((b - b_plain).abs().max() / b_plain.abs().max())
+    progress(f"{args.config} reused {ms_reuse:.3f} ms, split {ms_split:.3f} ms (err {split_err:.2e})")
     sysg.set_precision(sysg.PREC_FP64)
 
     # dominant kernel and its roofline: the real-space pair kernel is FP64-compute
@@ -557,6 +567,7 @@ def main():
     }
     if not args.no_timestep:
         out["timestep"] = timestep_bench(args.timestep_steps, 1)
+        progress("tube timestep done")
         # the metric's second half: GMRES seconds per tube timestep, chosen vs plain
         ts = out["timestep"]
         ch, pl = ts.get("chosen", {}), ts.get("plain", {})
@@ -567,11 +578,14 @@ def main():
             "velocity_rel_err": ch.get("velocity_rel_err_vs_plain")}
     if not args.no_c5:
         out["C5"] = c5_bench(3, 1)
+        progress(f"C5 matvec: {out['C5']['ms_per_step']:.3f} ms")
     if not args.no_cpu:
         thr = host_threads()
         tot, parts, thr = cpu_baseline(w, args.sample_every, thr)
         one_every = 8 * args.sample_every
+        progress(f"CPU baseline ({thr} threads): {tot:.2f} s per matvec")
         tot1, parts1, _ = cpu_baseline(w, one_every, 1)
+        progress(f"CPU baseline (1 thread): {tot1:.2f} s per matvec")
         out["cpu_baseline"] = {"value": round(ncell / tot, 4), "unit": "cell-matvecs/s",
                                "cores": thr, "kind": "port",
                                "sample": f"oracle (C restatement, OpenMP {thr} threads): SH, "
@@ -593,7 +607,9 @@ def main():
             ts = out["timestep"]
             niter = int(np.mean(ts["plain"]["gmres_iterations"]))
             cb, o3 = timestep_cpu_baseline(niter, thr)
+            progress(f"tube CPU baseline ({thr} threads): {cb['value']} s per timestep")
             cb1, _ = timestep_cpu_baseline(niter, 1, o3)
+            progress(f"tube CPU baseline (1 thread): {cb1['value']} s per timestep")
             cb["one_thread"] = cb1
             cb["host"] = host_info()
             ts["cpu_baseline"] = cb
