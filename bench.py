@@ -346,7 +346,8 @@ def stage_rates(w, st, split):
 def c5_bench(steps, warmup):
     """Cell-BIE matvec on the large mixed batch: 3,584 biconcave discs (nlat 13) and 512
     spheres of radius 1.4 (nlat 17) at 30% volume fraction, PME 256^3, P=6, viscosity
-    ratio 5; two grid types in one launch of every kernel.  Plain FP64 and the split
+    ratio 5; two grid types in one launch of every kernel, and the matvec at fixed
+    geometry replayed from a recorded hipGraph in both forms.  Plain FP64 and the split
     form on the same vector (error against FP64), each with its stage breakdown (one
     extra matvec with the stage events), the Legendre GEMMs' rate against the matrix
     cores' peak and the PME stages' algorithmic HBM rate."""
@@ -397,12 +398,49 @@ def c5_bench(steps, warmup):
     split, b32 = run(g.PREC_SPLIT)
     split["rel_err_vs_plain"] = float((b32 - b64).abs().max() / b64.abs().max())
     npairs, _ = g.pair_stats()
+
+    def run_graph(prec):
+        """The matvec as a GMRES solve applies it: geometry reused, replayed from one
+        recorded hipGraph."""
+        g.set_precision(prec)
+        g.set_geometry_reuse(True)
+        g.set_graph(True)
+        for _ in range(max(warmup, 2)):  # the first records the graph
+            g.cell_matmult_dev(u.data_ptr(), b.data_ptr())
+        g.sync()
+        assert g.graph_ready()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e0.record(stream)
+        for _ in range(steps):
+            g.cell_matmult_dev(u.data_ptr(), b.data_ptr())
+        e1.record(stream)
+        g.sync()
+        torch.cuda.synchronize()
+        ms = max(e0.elapsed_time(e1), 1e3 * (time.perf_counter() - t0)) / steps
+        out = b.clone()
+        g.set_graph(False)
+        g.set_geometry_reuse(False)
+        g.set_precision(g.PREC_FP64)
+        return {"ms_per_step": round(ms, 4),
+                "cell_matvecs_per_s": round(cells.ncell / (ms * 1e-3), 3)}, out
+
+    graph64, bg64 = run_graph(g.PREC_FP64)
+    graph32, bg32 = run_graph(g.PREC_SPLIT)
+    graph64["equal_to_eager"] = bool(torch.equal(bg64, b64))
+    graph32["rel_err_vs_plain"] = float((bg32 - b64).abs().max() / b64.abs().max())
     return {"workload": f"C5: {cells.ncell} cells (3584 discs nlat 13, 512 spheres R 1.4 "
                         f"nlat 17), box {w.Lb[0]:.3f}^3 (30% volume fraction), PME {w.Nb[0]}^3, "
                         f"P={w.P}, lambda=5, one cell-BIE matvec",
             "nodes": cells.npoint, "pairs_per_matvec": int(npairs), "steps": steps,
             "ms_per_step": plain["ms_per_step"], "cell_matvecs_per_s": plain["cell_matvecs_per_s"],
-            "plain": plain, "split": split}
+            "plain": plain, "split": split,
+            "graph": {"what": "geometry reused (the matvecs of one GMRES solve), each stage one "
+                              "launch over both grid types, the whole matvec replayed from "
+                              "one recorded hipGraph",
+                      "fp64": graph64, "split": graph32}}
 
 
 def main():

@@ -175,6 +175,16 @@ int rbc3d_cell_rhs_dev(rbc3d_ctx *ctx, double *d_rhs);
  * are identical either way.  enable: 1 (default) or 0. */
 int rbc3d_set_geometry_reuse(rbc3d_ctx *ctx, int enable);
 
+/* Graph form of the cell BIE matvec (no counterpart in the reference): with on = 1 and
+ * geometry reuse on, rbc3d_cell_matmult_dev records the matvec at the current geometry
+ * once into a hipGraph (the first call of a geometry runs eagerly, then records) and
+ * replays it in later calls, one launch per matvec instead of one per kernel.  A new
+ * geometry (rbc3d_set_cells), precision form or grown work buffer records a new graph.
+ * Results are identical to the eager matvec.  on = 0 (default) frees the graph. */
+int rbc3d_set_graph(rbc3d_ctx *ctx, int on);
+/* 1 when a recorded cell-matvec graph is ready to replay, else 0 */
+int rbc3d_graph_ready(rbc3d_ctx *ctx, int *ready);
+
 /* Precision of the hot path (no counterpart in the reference, which is FP64
  * throughout).  RBC3D_PREC_FP64, the default, is the plain form: every stage in FP64,
  * the reference's arithmetic.  RBC3D_PREC_SPLIT is the split form: real-space pair

@@ -53,6 +53,8 @@ _SIG = {
     "rbc3d_pair_stats": [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_ulonglong)],
     "rbc3d_near_stats": [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
     "rbc3d_set_geometry_reuse": [C.c_void_p, C.c_int],
+    "rbc3d_set_graph": [C.c_void_p, C.c_int],
+    "rbc3d_graph_ready": [C.c_void_p, C.POINTER(C.c_int)],
     "rbc3d_set_precision": [C.c_void_p, C.c_int],
     "rbc3d_set_gmres_restart": [C.c_void_p, C.c_int],
     "rbc3d_gmres_history": [C.c_void_p, dp, C.c_int, C.POINTER(C.c_int)],
@@ -317,6 +319,15 @@ class System:
 
     def set_geometry_reuse(self, enable):
         _call("rbc3d_set_geometry_reuse", self.h, int(bool(enable)))
+
+    def set_graph(self, on):
+        """Replay the cell matvec from a recorded hipGraph at fixed geometry."""
+        _call("rbc3d_set_graph", self.h, int(bool(on)))
+
+    def graph_ready(self):
+        r = C.c_int()
+        _call("rbc3d_graph_ready", self.h, C.byref(r))
+        return bool(r.value)
 
     PREC_FP64 = 0
     PREC_SPLIT = 1

@@ -325,8 +325,10 @@ static hipEvent_t ev_get(StageTimer &t) {
   (void)hipEventCreate(&e);
   return e;
 }
+unsigned long long g_dbuf_allocs = 0;
+
 void rbc_stage_begin(rbc3d_ctx *c, int stage) {
-  if (!c->timer.on) return;
+  if (!c->timer.on || c->capturing) return;
   StageTimer::Rec r;
   r.stage = stage;
   r.a = ev_get(c->timer);
@@ -335,7 +337,7 @@ void rbc_stage_begin(rbc3d_ctx *c, int stage) {
   c->timer.pending.push_back(r);
 }
 void rbc_stage_end(rbc3d_ctx *c, int stage) {
-  if (!c->timer.on) return;
+  if (!c->timer.on || c->capturing) return;
   for (auto it = c->timer.pending.rbegin(); it != c->timer.pending.rend(); ++it)
     if (it->stage == stage) {
       (void)hipEventRecord(it->b, c->stream);
@@ -535,9 +537,14 @@ static void rel(DBuf<T> &b) {
   b.release();
 }
 
+static void drop_graph(rbc3d_ctx *c);
+
 extern "C" int rbc3d_destroy(rbc3d_ctx *c) {
   if (!c) return RBC3D_OK;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
+  c->d_gin.release();
+  c->d_gout.release();
   for (auto *g : c->gtypes) {
     for (DBuf<double> *b : {&g->d_th, &g->d_phi, &g->d_w, &g->d_pw, &g->d_pws, &g->d_pg, &g->d_pe, &g->d_cs,
                             &g->d_Dm, &g->d_Dn, &g->d_thG, &g->d_phiG, &g->d_pwt, &g->d_mask,
@@ -779,6 +786,14 @@ extern "C" int rbc3d_add_int_on_rbcs(rbc3d_ctx *c, double c1, double c2, int nT,
 
 extern "C" int rbc3d_pair_stats(rbc3d_ctx *c, long long *npairs, unsigned long long *checksum) {
   if (!c) return RBC3D_EINVAL;
+  // the statistics of the last AddIntOnRbcs stay on the device until asked for
+  if (c->d_pairstat.p) {
+    unsigned long long hs[2];
+    HIP_TRY(hipMemcpyAsync(hs, c->d_pairstat.p, sizeof(hs), hipMemcpyDeviceToHost, c->stream));
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->last_npairs = (long long)hs[0];
+    c->last_checksum = hs[1];
+  }
   if (npairs) *npairs = c->last_npairs;
   if (checksum) *checksum = c->last_checksum;
   return RBC3D_OK;
@@ -1015,9 +1030,14 @@ static int ensure_rbc_targets(rbc3d_ctx *c) {
   return RBC3D_OK;
 }
 
-extern "C" int rbc3d_cell_matmult_dev(rbc3d_ctx *c, const double *d_u, double *d_b) {
-  if (!c || !d_u || !d_b) return RBC3D_EINVAL;
-  if (c->nrbc == 0) return RBC3D_ESTATE;
+__global__ void k_zero(double *v, size_t n) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) v[i] = 0.;
+}
+
+// MyMatMult (common/ModVelSolver.F90:523-601) on device vectors; queues work only (the
+// host reads nothing back), so a stream capture can record it
+static int cell_matmult_body(rbc3d_ctx *c, const double *d_u, double *d_b) {
   int np = c->np;
   hipStream_t st = c->stream;
   RBC_TRY(c->d_shbuf.ensure((size_t)3 * np + 1));
@@ -1026,7 +1046,7 @@ extern "C" int rbc3d_cell_matmult_dev(rbc3d_ctx *c, const double *d_u, double *d
   RBC_TRY(rbc_sh_glob(c, d_u, c->d_shbuf.p, -1));
   RBC_TRY(rbc_set_g_nodes(c, c->d_shbuf.p));
   RBC_TRY(ensure_rbc_targets(c));
-  HIP_TRY(hipMemsetAsync(c->d_v.p, 0, sizeof(double) * 3 * np, st));
+  k_zero<<<(unsigned)((3 * (size_t)np + 255) / 256), 256, 0, st>>>(c->d_v.p, 3 * (size_t)np);
   double c1 = 0., c2 = -1. / (4. * RBC_PI);
   RBC_TRY(rbc_add_int_on_rbcs_dev(c, c1, c2, np, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
                                   c->d_v.p, true));
@@ -1039,6 +1059,88 @@ extern "C" int rbc3d_cell_matmult_dev(rbc3d_ctx *c, const double *d_u, double *d
   HIP_TRY(hipGetLastError());
   RBC_TRY(rbc_sh_glob(c, c->d_v.p, d_b, 1));
   return RBC3D_OK;
+}
+
+static void drop_graph(rbc3d_ctx *c) {
+  if (c->mv_exec) (void)hipGraphExecDestroy(c->mv_exec);
+  if (c->mv_graph) (void)hipGraphDestroy(c->mv_graph);
+  c->mv_exec = nullptr;
+  c->mv_graph = nullptr;
+  c->mv_epoch = -1;
+}
+
+extern "C" int rbc3d_set_graph(rbc3d_ctx *c, int on) {
+  if (!c) return RBC3D_EINVAL;
+  c->use_graph = on != 0;
+  if (!c->use_graph) {
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    drop_graph(c);
+  }
+  return RBC3D_OK;
+}
+
+// The graph form of the cell matvec (geometry reuse on): the matvec at fixed geometry is
+// a fixed sequence of launches on fixed buffers, recorded once into a hipGraph on the
+// context's own input and output vectors and replayed per call (u copied in, b out).
+// A matvec whose geometry epoch, precision form or buffer allocations differ from the
+// recorded one runs eagerly (which brings every buffer to size and fills the geometry
+// caches) and records a new graph.
+static int cell_matmult_graph(rbc3d_ctx *c, const double *d_u, double *d_b) {
+  hipStream_t st = c->stream;
+  int dof;
+  RBC_TRY(rbc3d_cell_dof(c, &dof));
+  RBC_TRY(c->d_gin.ensure(dof + 1));
+  RBC_TRY(c->d_gout.ensure(dof + 1));
+  const size_t bytes = sizeof(double) * (size_t)dof;
+  const bool valid = c->mv_exec && c->mv_epoch == c->geom_epoch && c->mv_prec == c->prec &&
+                     c->mv_allocs == g_dbuf_allocs && c->cache_epoch == c->geom_epoch;
+  HIP_TRY(hipMemcpyAsync(c->d_gin.p, d_u, bytes, hipMemcpyDeviceToDevice, st));
+  if (valid) {
+    HIP_TRY(hipGraphLaunch(c->mv_exec, st));
+    HIP_TRY(hipMemcpyAsync(d_b, c->d_gout.p, bytes, hipMemcpyDeviceToDevice, st));
+    return RBC3D_OK;
+  }
+  HIP_TRY(hipStreamSynchronize(st));
+  drop_graph(c);
+  RBC_TRY(cell_matmult_body(c, c->d_gin.p, c->d_gout.p));  // this call's result
+  HIP_TRY(hipMemcpyAsync(d_b, c->d_gout.p, bytes, hipMemcpyDeviceToDevice, st));
+  RBC_TRY(rbc3d_sync(c));
+  rbc_stage_flush(c);
+  if (c->cache_epoch != c->geom_epoch) return RBC3D_OK;  // nothing kept to replay against
+  const unsigned long long allocs = g_dbuf_allocs;
+  c->capturing = true;
+  HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  const int r = cell_matmult_body(c, c->d_gin.p, c->d_gout.p);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &g);
+  c->capturing = false;
+  if (r != RBC3D_OK || e != hipSuccess || !g || g_dbuf_allocs != allocs) {
+    if (g) (void)hipGraphDestroy(g);
+    if (r != RBC3D_OK) return r;
+    rbc_set_error(std::string("cell matvec graph capture failed: ") +
+                  (e != hipSuccess ? hipGetErrorString(e) : "a buffer was allocated while recording"));
+    return RBC3D_EHIP;
+  }
+  c->mv_graph = g;
+  HIP_TRY(hipGraphInstantiate(&c->mv_exec, g, nullptr, nullptr, 0));
+  c->mv_epoch = c->geom_epoch;
+  c->mv_prec = c->prec;
+  c->mv_allocs = allocs;
+  return RBC3D_OK;
+}
+
+extern "C" int rbc3d_graph_ready(rbc3d_ctx *c, int *ready) {
+  if (!c || !ready) return RBC3D_EINVAL;
+  *ready = c->mv_exec && c->mv_epoch == c->geom_epoch && c->mv_prec == c->prec &&
+           c->mv_allocs == g_dbuf_allocs;
+  return RBC3D_OK;
+}
+
+extern "C" int rbc3d_cell_matmult_dev(rbc3d_ctx *c, const double *d_u, double *d_b) {
+  if (!c || !d_u || !d_b) return RBC3D_EINVAL;
+  if (c->nrbc == 0) return RBC3D_ESTATE;
+  if (c->use_graph && c->geom_reuse) return cell_matmult_graph(c, d_u, d_b);
+  return cell_matmult_body(c, d_u, d_b);
 }
 
 extern "C" int rbc3d_cell_matmult(rbc3d_ctx *c, const double *u, double *b) {

@@ -39,6 +39,10 @@ void rbc_set_error(const std::string &msg);
     if (_r != RBC3D_OK) return _r; \
   } while (0)
 
+// Count of device-buffer (re)allocations in the process: a captured graph holds buffer
+// addresses, so it is valid only while this count is unchanged.
+extern unsigned long long g_dbuf_allocs;
+
 // Device buffer with explicit lifetime (hipMalloc / hipFree); grows on demand.
 template <typename T>
 struct DBuf {
@@ -56,6 +60,7 @@ struct DBuf {
     if (count == 0) return RBC3D_OK;
     HIP_TRY(hipMalloc(&p, count * sizeof(T)));
     n = count;
+    g_dbuf_allocs++;
     return RBC3D_OK;
   }
   void release() {
@@ -273,6 +278,19 @@ struct rbc3d_ctx {
   GmresWork gmres;
   DBuf<double> d_linpart;
   unsigned long long last_checksum = 0;
+  // the PME spread's sort of the cell sources by brick depends on the geometry only:
+  // kept for the geometry epoch it was made at (with geometry reuse on)
+  int spread_sort_epoch = -1, spread_sort_ns = 0;
+  // hipGraph of the cell matvec at fixed geometry (rbc3d_set_graph): captured on the
+  // matvec's own input and output buffers, valid for one geometry epoch, precision
+  // form and buffer-allocation count
+  bool use_graph = false;
+  bool capturing = false;  // a capture is being recorded: no host synchronisation
+  hipGraph_t mv_graph = nullptr;
+  hipGraphExec_t mv_exec = nullptr;
+  int mv_epoch = -1, mv_prec = -1;
+  unsigned long long mv_allocs = 0;
+  DBuf<double> d_gin, d_gout;
 };
 
 // ---------------------------------------------------------------------------

@@ -169,6 +169,10 @@ __global__ void k_hoc_next(int nbins, HashGeom hg, const int *__restrict__ bin_s
       for (int a1 = 0; a1 < npos[0]; a1++) hoc[pos[0][a1] + n1 * (pos[1][a2] + n2 * pos[2][a3])] = head;
 }
 
+__global__ void k_fill_u64(unsigned long long *p, int n, unsigned long long v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
 __global__ void k_fill_i(int *p, int n, int v) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -2065,8 +2069,9 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   }
   RBC_TRY(c->d_vacc.ensure((size_t)3 * nT + 1));
   RBC_TRY(c->d_pairstat.ensure(2));
-  HIP_TRY(hipMemsetAsync(c->d_pairstat.p, 0, 2 * sizeof(unsigned long long), st));
-  HIP_TRY(hipMemsetAsync(c->d_flag.p, 0, 8 * sizeof(int), st));
+  // cleared by kernels, not memsets: this path is recorded into graphs
+  k_fill_u64<<<1, 64, 0, st>>>(c->d_pairstat.p, 2, 0ull);
+  k_fill_i<<<1, 64, 0, st>>>(c->d_flag.p, 8, 0);
   PairArgs pa;
   for (int d = 0; d < 3; d++) {
     pa.Lb[d] = c->Lb[d];
@@ -2184,7 +2189,7 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   // d_work holds the sorted-position -> node map; the near-singular worklist is separate
   DBuf<int> &wl = c->d_worklist;
   RBC_TRY(wl.ensure(2 * (size_t)nitems + 2));
-  HIP_TRY(hipMemsetAsync(c->d_work_cnt.p, 0, sizeof(int), st));
+  k_fill_i<<<1, 64, 0, st>>>(c->d_work_cnt.p, 1, 0);
   SingArgs sa;
   for (int d = 0; d < 3; d++) {
     sa.Lb[d] = c->Lb[d];
@@ -2309,11 +2314,7 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   k_combine<<<(nT + 255) / 256, 256, 0, st>>>(ca);
   rbc_stage_end(c, ST_COMBINE);
   HIP_TRY(hipGetLastError());
-  unsigned long long hs[2];
-  HIP_TRY(hipMemcpyAsync(hs, c->d_pairstat.p, sizeof(hs), hipMemcpyDeviceToHost, st));
-  HIP_TRY(hipStreamSynchronize(st));
-  c->last_npairs = (long long)hs[0];
-  c->last_checksum = hs[1];
+  // the pair statistics stay on the device (rbc3d_pair_stats reads them)
   if (cacheable) {
     c->cache_epoch = c->geom_epoch;
     c->cache_nT = nT;

@@ -719,16 +719,27 @@ static int pme_spread_t(rbc3d_ctx *c, double c1, double c2, int cells, int walls
         c->flag_dl, nbx, nby, nbz, (T *)c->d_pme_src.p, reclen, key, idx);
     rbc_stage_end(c, ST_SPREAD_PREP);
     HIP_TRY(hipGetLastError());
-    int bits = 1;
-    while ((1 << bits) < nbr) bits++;
-    size_t tmp = 0;
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key, key2, idx, order, ns, 0, bits, st));
-    RBC_TRY(c->d_cub_tmp.ensure(tmp + 16));
-    HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->d_cub_tmp.p, tmp, key, key2, idx, order, ns, 0,
-                                               bits, st));
   }
-  k_brick_start<<<(nbr + 1 + 255) / 256, 256, 0, st>>>(ns, key2, nbr, c->d_pme_start.p);
-  HIP_TRY(hipGetLastError());
+  // The sources' order by brick depends on the positions only: with geometry reuse, the
+  // cell sources' sort (order, brick starts) is kept for the geometry epoch it was made
+  // at.  Every call that sorts replaces it, so the kept one is the last sort made.
+  const bool keep = cells && !walls && ncell > 0 && c->geom_reuse;
+  const bool sorted = keep && c->spread_sort_epoch == c->geom_epoch && c->spread_sort_ns == ns;
+  if (!sorted) {
+    if (ns > 0) {
+      int bits = 1;
+      while ((1 << bits) < nbr) bits++;
+      size_t tmp = 0;
+      HIP_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp, key, key2, idx, order, ns, 0, bits, st));
+      RBC_TRY(c->d_cub_tmp.ensure(tmp + 16));
+      HIP_TRY(hipcub::DeviceRadixSort::SortPairs(c->d_cub_tmp.p, tmp, key, key2, idx, order, ns, 0,
+                                                 bits, st));
+    }
+    k_brick_start<<<(nbr + 1 + 255) / 256, 256, 0, st>>>(ns, key2, nbr, c->d_pme_start.p);
+    HIP_TRY(hipGetLastError());
+  }
+  c->spread_sort_epoch = keep ? c->geom_epoch : -1;
+  c->spread_sort_ns = ns;
   SpreadArgs<T> a;
   a.N1 = N1;
   a.N2 = N2;

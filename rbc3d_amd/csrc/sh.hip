@@ -91,17 +91,29 @@ __device__ void lon_synth(const double *Ga, const double *Gb, int nrow, int nn, 
 
 // ---------------------------------------------------------------------------
 // Surface splines of a cell: which = 0 -> x, a3, detJ; 1 -> f detJ; 2 -> g detJ
+// The grid-type tables of one launch over cells of up to SH_MAXSEG grid types
+#define SH_MAXSEG 8
+struct SplTabs {
+  int ncell;  // cells of the launch (cell_list entries from cell0)
+  int gt[SH_MAXSEG];
+  const double *pw[SH_MAXSEG], *pe[SH_MAXSEG], *cs[SH_MAXSEG], *Dm[SH_MAXSEG], *Dn[SH_MAXSEG];
+};
+
+// One persistent launch over the cells of every grid type: each workgroup takes cells
+// in turn (cells of different sizes side by side) with its type's tables.
 __global__ void __launch_bounds__(SH_THREADS) k_build_splines(
-    const CellRec *__restrict__ cells, int cell0, int gtype, const double *__restrict__ pw,
-    const double *__restrict__ pe, const double *__restrict__ cs, const double *__restrict__ Dm,
-    const double *__restrict__ Dn, const double *__restrict__ x, const double *__restrict__ a3,
-    const double *__restrict__ detj, const double *__restrict__ fd, int np, int which,
-    double *__restrict__ spl, const int *__restrict__ cell_list) {
+    const CellRec *__restrict__ cells, int cell0, SplTabs T, const double *__restrict__ x,
+    const double *__restrict__ a3, const double *__restrict__ detj, const double *__restrict__ fd,
+    int np, int which, double *__restrict__ spl, const int *__restrict__ cell_list) {
 #pragma clang fp contract(off)
   extern __shared__ double sm[];
-  int ci = cell_list[cell0 + blockIdx.x];
+  for (int w = blockIdx.x; w < T.ncell; w += gridDim.x) {
+  __syncthreads();  // the previous cell is done with the LDS buffers
+  int ci = cell_list[cell0 + w];
   const CellRec cr = cells[ci];
-  if (cr.gtype != gtype) return;
+  int k = 0;
+  while (k < SH_MAXSEG - 1 && T.gt[k] != cr.gtype) k++;
+  const double *pw = T.pw[k], *pe = T.pe[k], *cs = T.cs[k], *Dm = T.Dm[k], *Dn = T.Dn[k];
   int nl = cr.nlat, nn = cr.nlon, n0 = cr.nlat0, ne = nl + 1, m2 = 2 * nl;
   int mmax = min(nl, nn / 2 + 1);
   int R1 = ne * nn, R2 = 2 * ne * nl;
@@ -183,6 +195,7 @@ __global__ void __launch_bounds__(SH_THREADS) k_build_splines(
     }
     __syncthreads();
   }
+  }
 }
 
 static size_t spline_lds(int nl) {
@@ -214,25 +227,47 @@ static int upload_ids(rbc3d_ctx *c) {
   return RBC3D_OK;
 }
 
+// workgroups of a persistent launch over ncell cells
+static unsigned persistent_grid(const rbc3d_ctx *c, int nwork, int per_cu) {
+  return (unsigned)std::max(1, std::min(nwork, per_cu * c->ncu));
+}
+
 int rbc_sh_build_splines(rbc3d_ctx *c, int which) {
   if (c->nrbc == 0) return RBC3D_OK;
   if (which == 0) RBC_TRY(upload_ids(c));
   std::vector<int> ids;
+  const double *fd = which == 1 ? c->d_f.p : which == 2 ? c->d_g.p : nullptr;
+  // the grid types in groups of SH_MAXSEG, one launch per group (one launch in all for
+  // up to SH_MAXSEG distinct grids); cell_list holds the cells type by type
   int base = 0;
-  for (size_t gt = 0; gt < c->gtypes.size(); gt++) {
-    int n = cells_of_type(c, (int)gt, ids);
-    if (!n) continue;
-    GridType *g = c->gtypes[gt];
-    size_t lds = spline_lds(g->nlat);
-    const double *fd = which == 1 ? c->d_f.p : which == 2 ? c->d_g.p : nullptr;
+  for (size_t g0 = 0; g0 < c->gtypes.size(); g0 += SH_MAXSEG) {
+    SplTabs T{};
+    int nseg = 0, ncell = 0;
+    size_t lds = 0;
+    for (size_t gt = g0; gt < std::min(c->gtypes.size(), g0 + SH_MAXSEG); gt++) {
+      const int n = cells_of_type(c, (int)gt, ids);
+      if (!n) continue;
+      GridType *g = c->gtypes[gt];
+      T.gt[nseg] = (int)gt;
+      T.pw[nseg] = g->d_pws.p;
+      T.pe[nseg] = g->d_pe.p;
+      T.cs[nseg] = g->d_cs.p;
+      T.Dm[nseg] = g->d_Dm.p;
+      T.Dn[nseg] = g->d_Dn.p;
+      lds = std::max(lds, spline_lds(g->nlat));
+      nseg++;
+      ncell += n;
+    }
+    if (!ncell) continue;
+    for (int k = nseg; k < SH_MAXSEG; k++) T.gt[k] = -1;
+    T.ncell = ncell;
     rbc_stage_begin(c, ST_SPLINE);
-    hipLaunchKernelGGL(k_build_splines, dim3(n), dim3(SH_THREADS), lds, c->stream, c->d_cells.p,
-                       base, (int)gt, g->d_pws.p, g->d_pe.p, g->d_cs.p, g->d_Dm.p, g->d_Dn.p,
-                       c->d_x.p, c->d_a3.p, c->d_detj.p, fd, c->np, which, c->d_spl.p,
-                       c->d_tmp_i.p);
+    hipLaunchKernelGGL(k_build_splines, dim3(persistent_grid(c, ncell, 4)), dim3(SH_THREADS), lds,
+                       c->stream, c->d_cells.p, base, T, c->d_x.p, c->d_a3.p, c->d_detj.p, fd, c->np,
+                       which, c->d_spl.p, c->d_tmp_i.p);
     rbc_stage_end(c, ST_SPLINE);
     HIP_TRY(hipGetLastError());
-    base += n;
+    base += ncell;
   }
   return RBC3D_OK;
 }
@@ -259,26 +294,55 @@ __device__ __forceinline__ int pack_b(int m, int n, int n0) {
 // ld = ncolp (6 * ncell rounded up to 16).
 typedef double rbc_d4 __attribute__((ext_vector_type(4)));
 
+// One launch of each stage covers the cells of every grid type: a plan holds, per grid
+// type (segment), its sizes, tables and the offset of its operands in the staging
+// buffer; cell-wise kernels take cells (workgroups) and the GEMMs take 16 x 16 output
+// tiles (wavefronts) from one range over all segments, persistently.
+struct ShSeg {
+  int nl, nn, mmax, ncell, cell0, ncolp, RT, CT, tile0;
+  long long xoff, mat;  // X at xoff, Y at xoff + mat (doubles)
+  const double *cs, *P;          // longitude table; Legendre table of the direction
+  const unsigned short *P3;      // its three BF16 terms (split form)
+};
+struct ShPlan {
+  int nseg, ncell, ntile;
+  ShSeg s[SH_MAXSEG];
+};
+__device__ __forceinline__ int seg_of_cell(const ShPlan &P, int w) {
+  int k = 0;
+  while (k + 1 < P.nseg && w >= P.s[k + 1].cell0) k++;
+  return k;
+}
+__device__ __forceinline__ int seg_of_tile(const ShPlan &P, int w) {
+  int k = 0;
+  while (k + 1 < P.nseg && w >= P.s[k + 1].tile0) k++;
+  return k;
+}
+
 // X_m(i, col) = sum_j v(i, j) cos(m phi_j) / sin(m phi_j) for the 3 components
 __global__ void __launch_bounds__(SH_THREADS) k_sh_lon_anal(
-    const CellRec *__restrict__ cells, const int *__restrict__ cell_list, int cell0, int mmax,
-    const double *__restrict__ cs, const double *__restrict__ v, int np, int ncolp, double *X) {
-  const int c = blockIdx.x;
-  const CellRec cr = cells[cell_list[cell0 + c]];
-  const int nl = cr.nlat, nn = cr.nlon;
-  const double *cosT = cs, *sinT = cs + (size_t)(nl + 1) * nn;
-  for (int t = threadIdx.x; t < 3 * mmax * nl; t += blockDim.x) {
-    const int d = t / (mmax * nl), r = t - d * mmax * nl, m = r / nl, i = r - m * nl;
-    const double *f = v + cr.off + (size_t)np * d;
-    double a = 0, b = 0;
-    for (int j = 0; j < nn; j++) {
-      const double x = f[i + nl * j];
-      a += x * cosT[m * nn + j];
-      b += x * sinT[m * nn + j];
+    const CellRec *__restrict__ cells, const int *__restrict__ cell_list, int cell0, ShPlan P,
+    const double *__restrict__ v, int np, double *Xb) {
+  for (int w = blockIdx.x; w < P.ncell; w += gridDim.x) {
+    const ShSeg &S = P.s[seg_of_cell(P, w)];
+    const int c = w - S.cell0, mmax = S.mmax, ncolp = S.ncolp;
+    const CellRec cr = cells[cell_list[cell0 + w]];
+    const int nl = cr.nlat, nn = cr.nlon;
+    const double *cosT = S.cs, *sinT = S.cs + (size_t)(nl + 1) * nn;
+    double *X = Xb + S.xoff;
+    for (int t = threadIdx.x; t < 3 * mmax * nl; t += blockDim.x) {
+      const int d = t / (mmax * nl), r = t - d * mmax * nl, m = r / nl, i = r - m * nl;
+      const double *f = v + cr.off + (size_t)np * d;
+      double a = 0, b = 0;
+      for (int j = 0; j < nn; j++) {
+        const double x = f[i + nl * j];
+        a += x * cosT[m * nn + j];
+        b += x * sinT[m * nn + j];
+      }
+      double *Xm = X + ((size_t)m * nl + i) * ncolp + 6 * c + 2 * d;
+      Xm[0] = a;
+      Xm[1] = b;
     }
-    double *Xm = X + ((size_t)m * nl + i) * ncolp + 6 * c + 2 * d;
-    Xm[0] = a;
-    Xm[1] = b;
   }
 }
 
@@ -287,29 +351,31 @@ __global__ void __launch_bounds__(SH_THREADS) k_sh_lon_anal(
 // B[l / 16][l % 16] and holds D[4 i + l / 16][l % 16], i < 4).
 // TRANS_A: A(row, k) = P[m][k][row] (synthesis) instead of P[m][row][k] (analysis).
 template <bool TRANS_A>
-__global__ void __launch_bounds__(256) k_sh_leg_mfma(int nl, int mmax, int ncolp, int RT, int CT,
-                                                     const double *__restrict__ P,
-                                                     const double *__restrict__ X, double *Y) {
-  const int wid = (int)((blockIdx.x * 256 + threadIdx.x) >> 6), lane = threadIdx.x & 63;
-  if (wid >= mmax * RT * CT) return;
-  const int ct = wid % CT, rt = (wid / CT) % RT, m = wid / (CT * RT);
-  const int row = rt * 16 + (lane & 15), kq = lane >> 4, col = ct * 16 + (lane & 15);
-  const double *Pm = P + (size_t)m * nl * nl;
-  const double *Xm = X + (size_t)m * nl * ncolp;
-  rbc_d4 acc = {0., 0., 0., 0.};
-  for (int k0 = 0; k0 < nl; k0 += 4) {
-    const int k = k0 + kq;
-    double a = 0., b = 0.;
-    if (k < nl) {
-      if (row < nl) a = TRANS_A ? Pm[(size_t)k * nl + row] : Pm[(size_t)row * nl + k];
-      b = Xm[(size_t)k * ncolp + col];
+__global__ void __launch_bounds__(256) k_sh_leg_mfma(ShPlan Pl, double *XY) {
+  const int lane = threadIdx.x & 63;
+  const int nwave = (int)(gridDim.x * 4);
+  for (int wid = (int)((blockIdx.x * 256 + threadIdx.x) >> 6); wid < Pl.ntile; wid += nwave) {
+    const ShSeg &S = Pl.s[seg_of_tile(Pl, wid)];
+    const int nl = S.nl, ncolp = S.ncolp, RT = S.RT, CT = S.CT, tw = wid - S.tile0;
+    const int ct = tw % CT, rt = (tw / CT) % RT, m = tw / (CT * RT);
+    const int row = rt * 16 + (lane & 15), kq = lane >> 4, col = ct * 16 + (lane & 15);
+    const double *Pm = S.P + (size_t)m * nl * nl;
+    const double *Xm = XY + S.xoff + (size_t)m * nl * ncolp;
+    rbc_d4 acc = {0., 0., 0., 0.};
+    for (int k0 = 0; k0 < nl; k0 += 4) {
+      const int k = k0 + kq;
+      double a = 0., b = 0.;
+      if (k < nl) {
+        if (row < nl) a = TRANS_A ? Pm[(size_t)k * nl + row] : Pm[(size_t)row * nl + k];
+        b = Xm[(size_t)k * ncolp + col];
+      }
+      acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
     }
-    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
-  }
-  double *Ym = Y + (size_t)m * nl * ncolp;
-  for (int i = 0; i < 4; i++) {
-    const int r = rt * 16 + 4 * i + (lane >> 4);
-    if (r < nl) Ym[(size_t)r * ncolp + col] = acc[i];
+    double *Ym = XY + S.xoff + S.mat + (size_t)m * nl * ncolp;
+    for (int i = 0; i < 4; i++) {
+      const int r = rt * 16 + 4 * i + (lane >> 4);
+      if (r < nl) Ym[(size_t)r * ncolp + col] = acc[i];
+    }
   }
 }
 
@@ -347,51 +413,53 @@ __host__ __device__ __forceinline__ void bf16_split3(double x, unsigned short &h
 }
 
 template <bool TRANS_A>
-__global__ void __launch_bounds__(256) k_sh_leg_bf16x3(int nl, int mmax, int ncolp, int RT, int CT,
-                                                       const unsigned short *__restrict__ P3,
-                                                       const double *__restrict__ X, double *Y) {
-  const int wid = (int)((blockIdx.x * 256 + threadIdx.x) >> 6), lane = threadIdx.x & 63;
-  if (wid >= mmax * RT * CT) return;
-  const int ct = wid % CT, rt = (wid / CT) % RT, m = wid / (CT * RT);
-  const int row = rt * 16 + (lane & 15), kq = lane >> 4, col = ct * 16 + (lane & 15);
-  const size_t psz = (size_t)mmax * nl * nl;
-  const unsigned short *Pm = P3 + (size_t)m * nl * nl;
-  const double *Xm = X + (size_t)m * nl * ncolp;
-  rbc_f4 hi = {0.f, 0.f, 0.f, 0.f}, lo = {0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < nl; k0 += 16) {
-    rbc_s4 a0, a1, a2, b0, b1, b2;
+__global__ void __launch_bounds__(256) k_sh_leg_bf16x3(ShPlan Pl, double *XY) {
+  const int lane = threadIdx.x & 63;
+  const int nwave = (int)(gridDim.x * 4);
+  for (int wid = (int)((blockIdx.x * 256 + threadIdx.x) >> 6); wid < Pl.ntile; wid += nwave) {
+    const ShSeg &S = Pl.s[seg_of_tile(Pl, wid)];
+    const int nl = S.nl, mmax = S.mmax, ncolp = S.ncolp, RT = S.RT, CT = S.CT, tw = wid - S.tile0;
+    const int ct = tw % CT, rt = (tw / CT) % RT, m = tw / (CT * RT);
+    const int row = rt * 16 + (lane & 15), kq = lane >> 4, col = ct * 16 + (lane & 15);
+    const size_t psz = (size_t)mmax * nl * nl;
+    const unsigned short *Pm = S.P3 + (size_t)m * nl * nl;
+    const double *Xm = XY + S.xoff + (size_t)m * nl * ncolp;
+    rbc_f4 hi = {0.f, 0.f, 0.f, 0.f}, lo = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 0; k0 < nl; k0 += 16) {
+      rbc_s4 a0, a1, a2, b0, b1, b2;
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-      const int k = k0 + 4 * kq + j;
-      unsigned short p0 = 0, p1 = 0, p2 = 0, x0 = 0, x1 = 0, x2 = 0;
-      if (k < nl) {
-        if (row < nl) {
-          const size_t o = TRANS_A ? (size_t)k * nl + row : (size_t)row * nl + k;
-          p0 = Pm[o];
-          p1 = Pm[psz + o];
-          p2 = Pm[2 * psz + o];
+      for (int j = 0; j < 4; j++) {
+        const int k = k0 + 4 * kq + j;
+        unsigned short p0 = 0, p1 = 0, p2 = 0, x0 = 0, x1 = 0, x2 = 0;
+        if (k < nl) {
+          if (row < nl) {
+            const size_t o = TRANS_A ? (size_t)k * nl + row : (size_t)row * nl + k;
+            p0 = Pm[o];
+            p1 = Pm[psz + o];
+            p2 = Pm[2 * psz + o];
+          }
+          bf16_split3(Xm[(size_t)k * ncolp + col], x0, x1, x2);
         }
-        bf16_split3(Xm[(size_t)k * ncolp + col], x0, x1, x2);
+        a0[j] = (short)p0;
+        a1[j] = (short)p1;
+        a2[j] = (short)p2;
+        b0[j] = (short)x0;
+        b1[j] = (short)x1;
+        b2[j] = (short)x2;
       }
-      a0[j] = (short)p0;
-      a1[j] = (short)p1;
-      a2[j] = (short)p2;
-      b0[j] = (short)x0;
-      b1[j] = (short)x1;
-      b2[j] = (short)x2;
+      hi = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b0, hi, 0, 0, 0);
+      lo = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b2, lo, 0, 0, 0);
+      lo = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1, lo, 0, 0, 0);
+      lo = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2, b0, lo, 0, 0, 0);
+      lo = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b1, lo, 0, 0, 0);
+      lo = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b0, lo, 0, 0, 0);
     }
-    hi = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b0, hi, 0, 0, 0);
-    lo = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b2, lo, 0, 0, 0);
-    lo = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b1, lo, 0, 0, 0);
-    lo = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a2, b0, lo, 0, 0, 0);
-    lo = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a0, b1, lo, 0, 0, 0);
-    lo = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a1, b0, lo, 0, 0, 0);
-  }
-  double *Ym = Y + (size_t)m * nl * ncolp;
+    double *Ym = XY + S.xoff + S.mat + (size_t)m * nl * ncolp;
 #pragma unroll
-  for (int i = 0; i < 4; i++) {
-    const int r = rt * 16 + 4 * (lane >> 4) + i;
-    if (r < nl) Ym[(size_t)r * ncolp + col] = (double)hi[i] + (double)lo[i];
+    for (int i = 0; i < 4; i++) {
+      const int r = rt * 16 + 4 * (lane >> 4) + i;
+      if (r < nl) Ym[(size_t)r * ncolp + col] = (double)hi[i] + (double)lo[i];
+    }
   }
 }
 
@@ -414,104 +482,136 @@ static int split_tables(GridType *g) {
 
 // coefficients of Y into the Glob_Sph_Trans vector: a = Y(cos), b = -Y(sin)
 __global__ void k_sh_pack(const CellRec *__restrict__ cells, const int *__restrict__ cell_list,
-                          int cell0, int nl, int mmax, int ncolp, const double *__restrict__ Y,
-                          double *cf) {
-  const int c = blockIdx.x;
-  const CellRec cr = cells[cell_list[cell0 + c]];
-  const int n0 = cr.nlat0, mm = min(mmax, n0);
-  for (int t = threadIdx.x; t < 3 * n0 * n0; t += blockDim.x) {
-    const int d = t / (n0 * n0), r = t - d * n0 * n0, m = r % n0, n = r / n0;
-    if (m >= mm || n < m) continue;
-    const double *y = Y + ((size_t)m * nl + n) * ncolp + 6 * c + 2 * d;
-    cf[cr.coff + 3 * pack_a(m, n, n0) + d] = y[0];
-    if (m >= 1) cf[cr.coff + 3 * pack_b(m, n, n0) + d] = -y[1];
+                          int cell0, ShPlan P, const double *__restrict__ XY, double *cf) {
+  for (int w = blockIdx.x; w < P.ncell; w += gridDim.x) {
+    const ShSeg &S = P.s[seg_of_cell(P, w)];
+    const int c = w - S.cell0, nl = S.nl, ncolp = S.ncolp;
+    const double *Y = XY + S.xoff + S.mat;
+    const CellRec cr = cells[cell_list[cell0 + w]];
+    const int n0 = cr.nlat0, mm = min(S.mmax, n0);
+    for (int t = threadIdx.x; t < 3 * n0 * n0; t += blockDim.x) {
+      const int d = t / (n0 * n0), r = t - d * n0 * n0, m = r % n0, n = r / n0;
+      if (m >= mm || n < m) continue;
+      const double *y = Y + ((size_t)m * nl + n) * ncolp + 6 * c + 2 * d;
+      cf[cr.coff + 3 * pack_a(m, n, n0) + d] = y[0];
+      if (m >= 1) cf[cr.coff + 3 * pack_b(m, n, n0) + d] = -y[1];
+    }
   }
 }
 
 // Z_m(n, col) from the Glob_Sph_Trans vector (zero outside m <= n < nlat0, m < mm)
 __global__ void k_sh_unpack(const CellRec *__restrict__ cells, const int *__restrict__ cell_list,
-                            int cell0, int nl, int mmax, int ncolp, const double *__restrict__ cf,
-                            double *Z) {
-  const int c = blockIdx.x;
-  const CellRec cr = cells[cell_list[cell0 + c]];
-  const int n0 = cr.nlat0, mm = min(n0, mmax);
-  for (int t = threadIdx.x; t < 3 * mmax * nl; t += blockDim.x) {
-    const int d = t / (mmax * nl), r = t - d * mmax * nl, m = r / nl, n = r - m * nl;
-    double a = 0., b = 0.;
-    if (m < mm && n >= m && n < n0) {
-      a = cf[cr.coff + 3 * pack_a(m, n, n0) + d];
-      if (m >= 1) b = cf[cr.coff + 3 * pack_b(m, n, n0) + d];
+                            int cell0, ShPlan P, const double *__restrict__ cf, double *XY) {
+  for (int w = blockIdx.x; w < P.ncell; w += gridDim.x) {
+    const ShSeg &S = P.s[seg_of_cell(P, w)];
+    const int c = w - S.cell0, nl = S.nl, mmax = S.mmax, ncolp = S.ncolp;
+    double *Z = XY + S.xoff;
+    const CellRec cr = cells[cell_list[cell0 + w]];
+    const int n0 = cr.nlat0, mm = min(n0, mmax);
+    for (int t = threadIdx.x; t < 3 * mmax * nl; t += blockDim.x) {
+      const int d = t / (mmax * nl), r = t - d * mmax * nl, m = r / nl, n = r - m * nl;
+      double a = 0., b = 0.;
+      if (m < mm && n >= m && n < n0) {
+        a = cf[cr.coff + 3 * pack_a(m, n, n0) + d];
+        if (m >= 1) b = cf[cr.coff + 3 * pack_b(m, n, n0) + d];
+      }
+      double *z = Z + ((size_t)m * nl + n) * ncolp + 6 * c + 2 * d;
+      z[0] = a;
+      z[1] = b;
     }
-    double *z = Z + ((size_t)m * nl + n) * ncolp + 6 * c + 2 * d;
-    z[0] = a;
-    z[1] = b;
   }
 }
 
 // v(i, j) = sum_m G_m(i, cos) cos(m phi_j) - G_m(i, sin) sin(m phi_j)
 __global__ void __launch_bounds__(SH_THREADS) k_sh_lon_synth(
-    const CellRec *__restrict__ cells, const int *__restrict__ cell_list, int cell0, int mmax,
-    const double *__restrict__ cs, const double *__restrict__ G, int ncolp, int np, double *v) {
-  const int c = blockIdx.x;
-  const CellRec cr = cells[cell_list[cell0 + c]];
-  const int nl = cr.nlat, nn = cr.nlon, mm = min(cr.nlat0, mmax);
-  const double *cosT = cs, *sinT = cs + (size_t)(nl + 1) * nn;
-  for (int t = threadIdx.x; t < 3 * nl * nn; t += blockDim.x) {
-    const int d = t / (nl * nn), q = t - d * nl * nn, i = q % nl, j = q / nl;
-    double x = 0;
-    for (int m = 0; m < mm; m++) {
-      const double *g = G + ((size_t)m * nl + i) * ncolp + 6 * c + 2 * d;
-      x += g[0] * cosT[m * nn + j] - g[1] * sinT[m * nn + j];
+    const CellRec *__restrict__ cells, const int *__restrict__ cell_list, int cell0, ShPlan P,
+    const double *__restrict__ XY, int np, double *v) {
+  for (int w = blockIdx.x; w < P.ncell; w += gridDim.x) {
+    const ShSeg &S = P.s[seg_of_cell(P, w)];
+    const int c = w - S.cell0, ncolp = S.ncolp;
+    const double *G = XY + S.xoff + S.mat;
+    const CellRec cr = cells[cell_list[cell0 + w]];
+    const int nl = cr.nlat, nn = cr.nlon, mm = min(cr.nlat0, S.mmax);
+    const double *cosT = S.cs, *sinT = S.cs + (size_t)(nl + 1) * nn;
+    for (int t = threadIdx.x; t < 3 * nl * nn; t += blockDim.x) {
+      const int d = t / (nl * nn), q = t - d * nl * nn, i = q % nl, j = q / nl;
+      double x = 0;
+      for (int m = 0; m < mm; m++) {
+        const double *g = G + ((size_t)m * nl + i) * ncolp + 6 * c + 2 * d;
+        x += g[0] * cosT[m * nn + j] - g[1] * sinT[m * nn + j];
+      }
+      v[cr.off + q + (size_t)np * d] = x;
     }
-    v[cr.off + q + (size_t)np * d] = x;
   }
 }
 
+// Glob_Sph_Trans (common/ModVelSolver.F90:641-719): direction 1 analysis (node values
+// -> packed coefficients), -1 synthesis.  Each stage is one persistent launch over
+// the cells of every grid type (up to SH_MAXSEG distinct grids per launch).
 int rbc_sh_glob(rbc3d_ctx *c, const double *d_in, double *d_out, int direction) {
   std::vector<int> ids;
+  const bool split = c->prec == RBC3D_PREC_SPLIT;
   int base = 0;
-  for (size_t gt = 0; gt < c->gtypes.size(); gt++) {
-    int n = cells_of_type(c, (int)gt, ids);
-    if (!n) continue;
-    GridType *g = c->gtypes[gt];
-    const int nl = g->nlat, nn = g->nlon, mmax = g->mmax;
-    const int ncolp = (6 * n + 15) / 16 * 16, RT = (nl + 15) / 16, CT = ncolp / 16;
-    const size_t mat = (size_t)mmax * nl * ncolp;
-    RBC_TRY(c->d_shx.ensure(2 * mat));
-    double *X = c->d_shx.p, *Y = X + mat;
-    const unsigned gblk = (unsigned)(((size_t)mmax * RT * CT * 64 + 255) / 256);
-    const bool split = c->prec == RBC3D_PREC_SPLIT;
-    if (split && !g->d_pw3.p) {  // once per grid type
-      HIP_TRY(hipStreamSynchronize(c->stream));
-      RBC_TRY(split_tables(g));
+  for (size_t g0 = 0; g0 < c->gtypes.size(); g0 += SH_MAXSEG) {
+    ShPlan P{};
+    long long off = 0;
+    for (size_t gt = g0; gt < std::min(c->gtypes.size(), g0 + SH_MAXSEG); gt++) {
+      const int n = cells_of_type(c, (int)gt, ids);
+      if (!n) continue;
+      GridType *g = c->gtypes[gt];
+      if (split && !g->d_pw3.p) {  // once per grid type
+        HIP_TRY(hipStreamSynchronize(c->stream));
+        RBC_TRY(split_tables(g));
+      }
+      ShSeg &S = P.s[P.nseg];
+      S.nl = g->nlat;
+      S.nn = g->nlon;
+      S.mmax = g->mmax;
+      S.ncell = n;
+      S.cell0 = P.ncell;
+      S.ncolp = (6 * n + 15) / 16 * 16;
+      S.RT = (S.nl + 15) / 16;
+      S.CT = S.ncolp / 16;
+      S.tile0 = P.ntile;
+      S.mat = (long long)S.mmax * S.nl * S.ncolp;
+      S.xoff = off;
+      S.cs = g->d_cs.p;
+      S.P = direction == 1 ? g->d_pw.p : g->d_pg.p;
+      S.P3 = direction == 1 ? g->d_pw3.p : g->d_pg3.p;
+      off += 2 * S.mat;
+      P.ncell += n;
+      P.ntile += S.mmax * S.RT * S.CT;
+      P.nseg++;
     }
+    if (!P.ncell) continue;
+    RBC_TRY(c->d_shx.ensure((size_t)off));
+    double *XY = c->d_shx.p;
+    const unsigned gc = persistent_grid(c, P.ncell, 4);
+    const unsigned gl = persistent_grid(c, (P.ntile + 3) / 4, 8);
     rbc_stage_begin(c, ST_SH_GLOB);
     if (direction == 1) {
-      k_sh_lon_anal<<<n, SH_THREADS, 0, c->stream>>>(c->d_cells.p, c->d_tmp_i.p, base, mmax,
-                                                       g->d_cs.p, d_in, c->np, ncolp, X);
+      k_sh_lon_anal<<<gc, SH_THREADS, 0, c->stream>>>(c->d_cells.p, c->d_tmp_i.p, base, P, d_in,
+                                                     c->np, XY);
       if (split)
-        k_sh_leg_bf16x3<false><<<gblk, 256, 0, c->stream>>>(nl, mmax, ncolp, RT, CT, g->d_pw3.p, X, Y);
+        k_sh_leg_bf16x3<false><<<gl, 256, 0, c->stream>>>(P, XY);
       else
-        k_sh_leg_mfma<false><<<gblk, 256, 0, c->stream>>>(nl, mmax, ncolp, RT, CT, g->d_pw.p, X, Y);
-      k_sh_pack<<<n, 256, 0, c->stream>>>(c->d_cells.p, c->d_tmp_i.p, base, nl, mmax, ncolp, Y,
-                                          d_out);
+        k_sh_leg_mfma<false><<<gl, 256, 0, c->stream>>>(P, XY);
+      k_sh_pack<<<gc, 256, 0, c->stream>>>(c->d_cells.p, c->d_tmp_i.p, base, P, XY, d_out);
     } else {
-      k_sh_unpack<<<n, 256, 0, c->stream>>>(c->d_cells.p, c->d_tmp_i.p, base, nl, mmax, ncolp, d_in,
-                                            X);
+      k_sh_unpack<<<gc, 256, 0, c->stream>>>(c->d_cells.p, c->d_tmp_i.p, base, P, d_in, XY);
       if (split)
-        k_sh_leg_bf16x3<true><<<gblk, 256, 0, c->stream>>>(nl, mmax, ncolp, RT, CT, g->d_pg3.p, X, Y);
+        k_sh_leg_bf16x3<true><<<gl, 256, 0, c->stream>>>(P, XY);
       else
-        k_sh_leg_mfma<true><<<gblk, 256, 0, c->stream>>>(nl, mmax, ncolp, RT, CT, g->d_pg.p, X, Y);
-      k_sh_lon_synth<<<n, SH_THREADS, 0, c->stream>>>(c->d_cells.p, c->d_tmp_i.p, base, mmax,
-                                                      g->d_cs.p, Y, ncolp, c->np, d_out);
+        k_sh_leg_mfma<true><<<gl, 256, 0, c->stream>>>(P, XY);
+      k_sh_lon_synth<<<gc, SH_THREADS, 0, c->stream>>>(c->d_cells.p, c->d_tmp_i.p, base, P, XY,
+                                                      c->np, d_out);
     }
     rbc_stage_end(c, ST_SH_GLOB);
     HIP_TRY(hipGetLastError());
-    base += n;
+    base += P.ncell;
   }
   return RBC3D_OK;
 }
-
 
 // ---------------------------------------------------------------------------
 // Standalone ShAnalGau / ShSynthGau / ShSynthEqu on nt fields with the reference's
