@@ -137,6 +137,36 @@ int rbc3d_sh_synth_equ(int nlat, int nlon, int nt, double *g, int idg, int jdg, 
  * the scalar field whose shags coefficients are a, b. */
 int rbc3d_sh_grad_gau(int nlat, int nlon, int nt, double *v, double *w, int idv, int jdv,
                       const double *a, const double *b, int ida, int jda);
+/* --- next to the hot path: per-cell geometry and membrane mechanics ---------------
+ * Batched over ncell cells, each on its Gauss grid nlat(i) x 2 nlat(i) (2 <= nlat <= 28),
+ * arrays of the reference's cell type concatenated over the cells in cell order:
+ * x, a1, a2, a3, f (nlat,nlon,3); a, a_rcp, b (nlat,nlon,2,2); detj (nlat,nlon);
+ * xc (3,ncell); area, vol, mesh_size (ncell). */
+/* RBC_ComputeGeometry (common/ModRbc.F90:404-509): from x, the tangents a1 = dx/dth,
+ * a2 = dx/dphi, unit normal a3, metric a and its inverse a_rcp, curvature tensor b,
+ * detJ = sqrt(det a)/sin th, centroid, area, volume and meshSize. */
+int rbc3d_rbc_compute_geometry(int ncell, const int *nlat, const double *x, double *a1,
+                               double *a2, double *a3, double *a, double *a_rcp, double *b,
+                               double *detj, double *xc, double *area, double *vol,
+                               double *mesh_size);
+/* Shell_ResForce (common/ModRbc.F90:815-900) with Shell_ElasStrs (Skalak law, :908-944)
+ * and Shell_Bend (:951-969): the membrane's residual force f (Cartesian) of each cell from
+ * its geometry (as rbc3d_rbc_compute_geometry returns it) against the reference cell of
+ * its type, rbcRefs(celltype): per type k (1-based celltype) nlat_ref(k), its a_rcp and b
+ * concatenated over the types, and the moduli ES(k), ED(k), EB(k).  Compute_Rbc_Vel
+ * (common/ModTimeInt.F90:746-753) uses -f as the surface force density. */
+int rbc3d_shell_res_force(int ncell, const int *nlat, const int *celltype, const double *a1,
+                          const double *a2, const double *a3, const double *a,
+                          const double *a_rcp, const double *b, const double *detj,
+                          const double *area, int ntypes, const int *nlat_ref,
+                          const double *a_rcp_ref, const double *b_ref, const double *ES,
+                          const double *ED, const double *EB, double *f);
+/* RBC_SphProject (common/ModRbc.F90:659-686): each cell's nvar fields u(nlat,nlon,nvar)
+ * (concatenated over cells) replaced by ShSynthGau(ShFilter(ShAnalGau(u))), modes
+ * n >= nlat0(i) dropped; with nvar = 3 on the coordinates this is FilterRbcs
+ * (common/ModTimeInt.F90:908-934). */
+int rbc3d_rbc_sph_project(int ncell, const int *nlat, const int *nlat0, int nvar, double *u);
+
 /* Glob_Sph_Trans(v,c,direction) (common/ModVelSolver.F90:641-719); direction
  * 1 = PHYS_TO_FOUR, -1 = FOUR_TO_PHYS.  v(nPoint,3), c(3*sum nlat0^2). */
 int rbc3d_glob_sph_trans(rbc3d_ctx *ctx, double *v, double *c, int direction);

@@ -102,6 +102,12 @@ def lib():
         "orc_spline_interp_raw": (None, [C.c_int, C.c_int, C.c_int, dp, dp, dp, dp, C.c_double,
                                          C.c_double, dp]),
         "orc_cell_mesh_size": (C.c_double, [vp, C.c_int]),
+        "orc_rbc_geometry": (C.c_int, [C.c_int, dp, dp, dp, dp, dp, dp, dp, dp, dp,
+                                       C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]),
+        "orc_shell_res_force": (C.c_int, [C.c_int, dp, dp, dp, dp, dp, dp, dp, C.c_double, dp, dp,
+                                          C.c_double, C.c_double, C.c_double, dp]),
+        "orc_rbc_sph_project": (C.c_int, [C.c_int, C.c_int, C.c_int, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -118,6 +124,48 @@ def L():
     if _L is None:
         _L = lib()
     return _L
+
+
+GEO_FIELDS = ("a1", "a2", "a3", "a", "a_rcp", "b", "detj")
+
+
+def rbc_geometry(x):
+    """RBC_ComputeGeometry of one cell: x(nlat, 2 nlat, 3) in the reference's
+    (column-major) layout as a flat array; returns the fields (flat, column-major)
+    and xc, area, vol, meshSize."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    npt = x.size // 3
+    nlat = int(round((npt / 2) ** 0.5))
+    assert 2 * nlat * nlat == npt
+    out = {k: np.zeros(n * npt) for k, n in (("a1", 3), ("a2", 3), ("a3", 3), ("a", 4),
+                                              ("a_rcp", 4), ("b", 4), ("detj", 1))}
+    xc = np.zeros(3)
+    area, vol, mesh = C.c_double(), C.c_double(), C.c_double()
+    r = L().orc_rbc_geometry(nlat, x, out["a1"], out["a2"], out["a3"], out["a"], out["a_rcp"],
+                             out["b"], out["detj"], xc, C.byref(area), C.byref(vol),
+                             C.byref(mesh))
+    assert r == 0
+    out.update(xc=xc, area=area.value, vol=vol.value, mesh=mesh.value, nlat=nlat)
+    return out
+
+
+def shell_res_force(geo, ref, ES, ED, EB):
+    """Shell_ResForce of one cell with geometry `geo` against the reference cell `ref`
+    (both from rbc_geometry); f(nlat, nlon, 3) flat, Cartesian."""
+    f = np.zeros_like(geo["a1"])
+    r = L().orc_shell_res_force(geo["nlat"], geo["a1"], geo["a2"], geo["a3"], geo["a"],
+                                geo["a_rcp"], geo["b"], geo["detj"], geo["area"], ref["a_rcp"],
+                                ref["b"], ES, ED, EB, f)
+    assert r == 0
+    return f
+
+
+def rbc_sph_project(u, nlat, nlat0):
+    """RBC_SphProject of nvar fields u(nlat, 2 nlat, nvar) (flat): returns the projection."""
+    u = np.array(u, dtype=np.float64).ravel()
+    nvar = u.size // (2 * nlat * nlat)
+    assert L().orc_rbc_sph_project(nlat, nlat0, nvar, u) == 0
+    return u
 
 
 def make_conf(Lb, alpha, rc, Nb, P, viscRat, vBkg=(0, 0, 0)):

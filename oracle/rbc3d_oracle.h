@@ -141,6 +141,19 @@ void orc_spline_interp_raw(int m, int n, int nvar, const double *u, const double
                            const double *u2, const double *u12, double x, double y, double *f);
 double orc_cell_mesh_size(const orc_sys *s, int irbc);
 
+/* Per-cell surface geometry and membrane mechanics (mech_oracle.c), one cell on its
+ * nlat x 2 nlat Gauss grid in the reference's layouts: RBC_ComputeGeometry
+ * (common/ModRbc.F90:404-509), Shell_ResForce with Shell_ElasStrs and Shell_Bend
+ * (:815-969), RBC_SphProject (:659-686). */
+int orc_rbc_geometry(int nlat, const double *x, double *a1, double *a2, double *a3, double *a,
+                     double *a_rcp, double *b, double *detj, double *xc, double *area,
+                     double *vol, double *mesh);
+int orc_shell_res_force(int nlat, const double *a1, const double *a2, const double *a3,
+                        const double *a, const double *a_rcp, const double *b, const double *detj,
+                        double area, const double *a_rcp_ref, const double *b_ref, double ES,
+                        double ED, double EB, double *f);
+int orc_rbc_sph_project(int nlat, int nlat0, int nvar, double *u);
+
 #ifdef __cplusplus
 }
 #endif

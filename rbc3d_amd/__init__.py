@@ -84,6 +84,10 @@ _SIG = {
     "rbc3d_sh_grad_gau": [C.c_int, C.c_int, C.c_int, dp, dp, C.c_int, C.c_int, dp, dp, C.c_int,
                           C.c_int],
     "rbc3d_glob_sph_trans": [C.c_void_p, dp, dp, C.c_int],
+    "rbc3d_rbc_compute_geometry": [C.c_int, ip, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp, dp],
+    "rbc3d_shell_res_force": [C.c_int, ip, ip, dp, dp, dp, dp, dp, dp, dp, dp, C.c_int, ip, dp, dp,
+                              dp, dp, dp, dp],
+    "rbc3d_rbc_sph_project": [C.c_int, ip, ip, C.c_int, dp],
     "rbc3d_cell_spline": [C.c_void_p, C.c_int, C.c_int, dp, dp, dp, dp],
     "rbc3d_calc_velocity_field": [C.c_void_p, C.c_int, dp, ip, ip, dp, dp],
     "rbc3d_cell_dof": [C.c_void_p, C.POINTER(C.c_int)],
@@ -220,6 +224,49 @@ def sh_grad_gau(nlat, nlon, a, b, nt=1, ida=None, jda=None):
     w = np.zeros(nlat * nlon * nt)
     _call("rbc3d_sh_grad_gau", nlat, nlon, nt, v, w, nlat, nlon, _f64(a), _f64(b), ida, jda)
     return v, w
+
+
+GEO_FIELDS = (("a1", 3), ("a2", 3), ("a3", 3), ("a", 4), ("a_rcp", 4), ("b", 4), ("detj", 1))
+
+
+def rbc_compute_geometry(nlat, x):
+    """RBC_ComputeGeometry (common/ModRbc.F90:404-509) of a batch of cells: nlat per cell,
+    x the cells' (nlat, 2 nlat, 3) arrays concatenated (Fortran-flat).  Returns a dict of
+    the concatenated fields (a1, a2, a3, a, a_rcp, b, detj) and per-cell xc (ncell, 3),
+    area, vol, mesh."""
+    nlat = np.ascontiguousarray(nlat, dtype=np.int32)
+    ncell = len(nlat)
+    npt = int(np.sum(2 * nlat.astype(np.int64) ** 2))
+    out = {k: np.zeros(n * npt) for k, n in GEO_FIELDS}
+    xc = np.zeros(3 * ncell)
+    area, vol, mesh = np.zeros(ncell), np.zeros(ncell), np.zeros(ncell)
+    _call("rbc3d_rbc_compute_geometry", ncell, nlat, _f64(x), out["a1"], out["a2"], out["a3"],
+          out["a"], out["a_rcp"], out["b"], out["detj"], xc, area, vol, mesh)
+    out.update(xc=xc.reshape(ncell, 3), area=area, vol=vol, mesh=mesh, nlat=nlat)
+    return out
+
+
+def shell_res_force(geo, celltype, ref_nlat, ref_a_rcp, ref_b, ES, ED, EB):
+    """Shell_ResForce (common/ModRbc.F90:815-900) of a batch of cells with geometry `geo`
+    (rbc_compute_geometry) against one reference cell per type (its nlat, a_rcp, b,
+    concatenated over the types) and per-type moduli; returns f (Cartesian,
+    concatenated (nlat, nlon, 3) arrays)."""
+    nlat = np.ascontiguousarray(geo["nlat"], dtype=np.int32)
+    f = np.zeros_like(geo["a1"])
+    ref_nlat = np.ascontiguousarray(ref_nlat, dtype=np.int32)
+    _call("rbc3d_shell_res_force", len(nlat), nlat, np.ascontiguousarray(celltype, dtype=np.int32),
+          geo["a1"], geo["a2"], geo["a3"], geo["a"], geo["a_rcp"], geo["b"], geo["detj"],
+          _f64(geo["area"]), len(ref_nlat), ref_nlat, _f64(ref_a_rcp), _f64(ref_b), _f64(ES),
+          _f64(ED), _f64(EB), f)
+    return f
+
+
+def rbc_sph_project(nlat, nlat0, u, nvar):
+    """RBC_SphProject (common/ModRbc.F90:659-686) of nvar fields per cell, concatenated."""
+    u = np.array(u, dtype=np.float64).ravel()
+    _call("rbc3d_rbc_sph_project", len(nlat), np.ascontiguousarray(nlat, dtype=np.int32),
+          np.ascontiguousarray(nlat0, dtype=np.int32), int(nvar), u)
+    return u
 
 
 class System:

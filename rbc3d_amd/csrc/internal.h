@@ -497,6 +497,11 @@ void rbc_stage_begin(rbc3d_ctx *c, int stage);
 void rbc_stage_end(rbc3d_ctx *c, int stage);
 void rbc_stage_flush(rbc3d_ctx *c);
 
+// Gauss colatitudes and weights (gaqd), normalised associated Legendre table
+// [m][n][i] on the given colatitudes (sh.hip)
+void host_gauss(int n, std::vector<double> &th, std::vector<double> &w);
+void host_pbar(int nl, int mmax, const std::vector<double> &th, std::vector<double> &out);
+
 // Kernel launchers (defined in the .hip files)
 int rbc_sh_build_splines(rbc3d_ctx *c, int which);
 int rbc_sh_glob(rbc3d_ctx *c, const double *d_in, double *d_out, int direction);

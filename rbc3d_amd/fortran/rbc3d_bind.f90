@@ -36,7 +36,8 @@ module rbc3d_bind
        rbc3d_wall_dof, rbc3d_wall_matmult, rbc3d_wall_residual_vel, rbc3d_get_wall_density, &
        rbc3d_noslip_wall, rbc3d_solve_rbc_vel, rbc3d_pair_stats, rbc3d_set_precision, &
        rbc3d_mlp_create, rbc3d_mlp_destroy, rbc3d_mlp_set_params, rbc3d_mlp_set_norm, &
-       rbc3d_noslip_wall_learned, rbc3d_set_gmres_restart
+       rbc3d_noslip_wall_learned, rbc3d_set_gmres_restart, &
+       rbc3d_rbc_compute_geometry, rbc3d_shell_res_force, rbc3d_rbc_sph_project
 
   ! rbc3d_set_precision modes
   integer(c_int), parameter, public :: RBC3D_PREC_FP64 = 0, RBC3D_PREC_SPLIT = 1
@@ -242,6 +243,34 @@ module rbc3d_bind
       import :: c_ptr, c_size_t
       type(c_ptr), value :: s
       integer(c_size_t) :: n
+    end function
+    ! per-cell geometry and membrane mechanics (RBC_ComputeGeometry, Shell_ResForce,
+    ! RBC_SphProject of common/ModRbc.F90), batched over cells
+    integer(c_int) function rbc3d_rbc_compute_geometry(ncell, nlat, x, a1, a2, a3, a, a_rcp, &
+         b, detj, xc, area, vol, mesh_size) bind(C, name="rbc3d_rbc_compute_geometry")
+      import :: c_int, c_double
+      integer(c_int), value :: ncell
+      integer(c_int), intent(in) :: nlat(*)
+      real(c_double), intent(in) :: x(*)
+      real(c_double), intent(out) :: a1(*), a2(*), a3(*), a(*), a_rcp(*), b(*), detj(*)
+      real(c_double), intent(out) :: xc(*), area(*), vol(*), mesh_size(*)
+    end function
+    integer(c_int) function rbc3d_shell_res_force(ncell, nlat, celltype, a1, a2, a3, a, a_rcp, &
+         b, detj, area, ntypes, nlat_ref, a_rcp_ref, b_ref, es, ed, eb, f) &
+         bind(C, name="rbc3d_shell_res_force")
+      import :: c_int, c_double
+      integer(c_int), value :: ncell, ntypes
+      integer(c_int), intent(in) :: nlat(*), celltype(*), nlat_ref(*)
+      real(c_double), intent(in) :: a1(*), a2(*), a3(*), a(*), a_rcp(*), b(*), detj(*), area(*)
+      real(c_double), intent(in) :: a_rcp_ref(*), b_ref(*), es(*), ed(*), eb(*)
+      real(c_double), intent(out) :: f(*)
+    end function
+    integer(c_int) function rbc3d_rbc_sph_project(ncell, nlat, nlat0, nvar, u) &
+         bind(C, name="rbc3d_rbc_sph_project")
+      import :: c_int, c_double
+      integer(c_int), value :: ncell, nvar
+      integer(c_int), intent(in) :: nlat(*), nlat0(*)
+      real(c_double), intent(inout) :: u(*)
     end function
     ! precision of the hot path (no counterpart in the reference)
     integer(c_int) function rbc3d_set_precision(ctx, mode) bind(C, name="rbc3d_set_precision")
