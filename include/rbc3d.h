@@ -205,6 +205,16 @@ int rbc3d_cell_rhs_dev(rbc3d_ctx *ctx, double *d_rhs);
  * are identical either way.  enable: 1 (default) or 0. */
 int rbc3d_set_geometry_reuse(rbc3d_ctx *ctx, int enable);
 
+/* Deflation of the cell BIE (the reference's Deflate switch, common/ModVelSolver.F90:
+ * 103-106, 122-125 and 144-449; Pozrikidis): rbc3d_solve_rbc_vel builds each cell's six
+ * solid-body modes after the right-hand side (DeflateGetSBModes), the right-hand side and
+ * the matvec carry the deflation terms (DeflateRHS, DeflateMatMult) and the solution is
+ * recovered from the deflated one (DeflateRecoverSol).  Each cell is addressed at its own
+ * node offset (the reference assumes one grid for all cells).  on: 1 or 0 (default). */
+int rbc3d_set_deflation(rbc3d_ctx *ctx, int on);
+/* the solid-body modes of the last deflated solve, modes[(m * 3 + d) * nPoint + node] */
+int rbc3d_sb_modes(rbc3d_ctx *ctx, double *modes);
+
 /* Graph form of the cell BIE matvec (no counterpart in the reference): with on = 1 and
  * geometry reuse on, rbc3d_cell_matmult_dev records the matvec at the current geometry
  * once into a hipGraph (the first call of a geometry runs eagerly, then records) and

@@ -108,6 +108,8 @@ def lib():
         "orc_shell_res_force": (C.c_int, [C.c_int, dp, dp, dp, dp, dp, dp, dp, C.c_double, dp, dp,
                                           C.c_double, C.c_double, C.c_double, dp]),
         "orc_rbc_sph_project": (C.c_int, [C.c_int, C.c_int, C.c_int, dp]),
+        "orc_set_deflation": (C.c_int, [vp, C.c_int]),
+        "orc_sb_modes": (C.c_int, [vp, dp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)
@@ -356,6 +358,14 @@ class Oracle:
 
     def set_gmres_restart(self, m):
         assert L().orc_set_gmres_restart(self.h, m) == 0
+
+    def set_deflation(self, on):
+        assert L().orc_set_deflation(self.h, int(bool(on))) == 0
+
+    def sb_modes(self):
+        m = np.zeros(18 * self.np)
+        assert L().orc_sb_modes(self.h, m) == 0
+        return m.reshape(6, 3, self.np)
 
     def noslip_wall(self, rtol=1e-8, maxit=60, finc0=None):
         ni = C.c_int()

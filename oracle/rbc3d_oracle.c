@@ -109,6 +109,8 @@ struct orc_sys {
   unsigned long long last_checksum;
   int gmres_restart; /* KSPGMRESSetRestart; 30 is PETSc's default */
   int terms;         /* AddIntOnRbcs parts evaluated (ORC_TERM_*), all by default */
+  int deflate;       /* the reference's Deflate switch (common/ModVelSolver.F90:103-106) */
+  double *sbq;       /* solid-body modes per node: sbq[(m * 3 + d) * nP + node], m < 6 */
 };
 
 /* ======================================================================== */
@@ -946,6 +948,7 @@ static void free_wall(orc_sys *s) {
 
 void orc_destroy(orc_sys *s) {
   if (!s) return;
+  free(s->sbq);
   free_cells(s);
   free_wall(s);
   free(s->ff);
@@ -2474,6 +2477,7 @@ int orc_glob_sph_trans(orc_sys *s, double *v, double *c, int direction) {
  * the reference's only on those cells' coefficient blocks (a bounded CPU sample of a
  * large system; the sources, the PME grids and the linear term are always complete). */
 static int cell_matmult_masked(orc_sys *s, const double *u, double *b, const int *cell_active);
+static void deflate_matmult(orc_sys *s, double *v);
 int orc_cell_matmult(orc_sys *s, const double *u, double *b) {
   return cell_matmult_masked(s, u, b, NULL);
 }
@@ -2507,6 +2511,7 @@ static int cell_matmult_masked(orc_sys *s, const double *u, double *b, const int
   orc_pme_transform(s);
   orc_pme_add_interp_vel(s, nP, tx, act, Ac, v);
   for (size_t q = 0; q < 3 * (size_t)nP; q++) v[q] += g[q];
+  if (s->deflate) deflate_matmult(s, v);
   orc_glob_sph_trans(s, v, b, 1);
   free(g);
   free(v);
@@ -2516,6 +2521,174 @@ static int cell_matmult_masked(orc_sys *s, const double *u, double *b, const int
   free(ti);
   free(act);
   return rc;
+}
+
+/* ======================================================================== */
+/* Deflation of the cell BIE (Pozrikidis), common/ModVelSolver.F90:144-449.  Node
+ * vectors v are (nP,3) component-major over the concatenated cells; the reference
+ * addresses cell irbc at (irbc-1)*nlat*nlon with the first cell's grid, which is each
+ * cell's own offset when every cell has the same grid (the only case it supports);
+ * here each cell's own offset is used. */
+int orc_set_deflation(orc_sys *s, int on) {
+  s->deflate = on != 0;
+  return 0;
+}
+int orc_sb_modes(const orc_sys *s, double *modes) {
+  if (!s->sbq) return 1;
+  memcpy(modes, s->sbq, sizeof(double) * 18 * (size_t)s->slr.nPoint);
+  return 0;
+}
+
+/* surface integral of sum_d a_d b_d over cell r, the reference's loop order (outer index
+ * first argument of the loop nest: ilat_outer = 1 for do ilat / do ilon) */
+static double cell_dot(const orc_rbc *r, size_t nP, const double *a, size_t lda,
+                       const double *b, size_t ldb, int ilat_outer) {
+  int nl = r->nlat, nn = r->nlon;
+  double fac = 0.;
+  if (ilat_outer) {
+    for (int ilat = 0; ilat < nl; ilat++)
+      for (int ilon = 0; ilon < nn; ilon++) {
+        const size_t q = (size_t)ilat + (size_t)nl * ilon;
+        const double ds = r->detj[q] * r->w[ilat];
+        double t = 0;
+        for (int d = 0; d < 3; d++) t += a[q + lda * d] * b[q + ldb * d];
+        fac = fac + t * ds;
+      }
+  } else {
+    for (int ilon = 0; ilon < nn; ilon++)
+      for (int ilat = 0; ilat < nl; ilat++) {
+        const size_t q = (size_t)ilat + (size_t)nl * ilon;
+        const double ds = r->detj[q] * r->w[ilat];
+        double t = 0;
+        for (int d = 0; d < 3; d++) t += a[q + lda * d] * b[q + ldb * d];
+        fac = fac + t * ds;
+      }
+  }
+  (void)nP;
+  return fac;
+}
+
+/* DeflateGetSBModes :354-449: per cell the 6 rigid-body velocity fields (translations,
+ * rotations about the centroid), Gram-Schmidt orthonormalised in the surface inner
+ * product.  The centroid is RBC_ComputeGeometry's (common/ModRbc.F90:478-491). */
+static void deflate_get_sb_modes(orc_sys *s) {
+  const size_t nP = s->slr.nPoint;
+  free(s->sbq);
+  s->sbq = calloc(18 * nP, sizeof(double));
+  for (int ir = 0; ir < s->nrbc; ir++) {
+    const orc_rbc *r = &s->rbcs[ir];
+    const int nl = r->nlat, nn = r->nlon, npt = nl * nn;
+    double xc[3] = {0, 0, 0}, ar = 0;
+    for (int ilat = 0; ilat < nl; ilat++)
+      for (int ilon = 0; ilon < nn; ilon++) {
+        const int q = ilat + nl * ilon;
+        const double ds = r->detj[q] * r->w[ilat];
+        ar += ds;
+        for (int d = 0; d < 3; d++) xc[d] += ds * r->x[q + npt * d];
+      }
+    for (int d = 0; d < 3; d++) xc[d] = xc[d] / ar;
+    double *pp = malloc(sizeof(double) * 18 * npt), *qq = malloc(sizeof(double) * 18 * npt);
+    for (int q = 0; q < npt; q++) {
+      double X[3];
+      for (int d = 0; d < 3; d++) X[d] = r->x[q + npt * d] - xc[d];
+      const double v[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, -X[2], X[1]},
+                              {X[2], 0, -X[0]}, {-X[1], X[0], 0}};
+      for (int m = 0; m < 6; m++)
+        for (int d = 0; d < 3; d++) pp[q + (size_t)npt * (d + 3 * m)] = v[m][d];
+    }
+    for (int m = 0; m < 6; m++) {
+      double *qm = qq + (size_t)npt * 3 * m;
+      const double *pm = pp + (size_t)npt * 3 * m;
+      memcpy(qm, pm, sizeof(double) * 3 * npt);
+      for (int n = 1; n < m + 1; n++) {
+        /* Proj(pp_m, qq_{n-1}) :603-621: fn / fd * qq_{n-1} */
+        const double *qn = qq + (size_t)npt * 3 * (n - 1);
+        const double fn = cell_dot(r, nP, pm, npt, qn, npt, 1);
+        const double fd = cell_dot(r, nP, qn, npt, qn, npt, 1);
+        for (int k = 0; k < 3 * npt; k++) qm[k] = qm[k] - fn / fd * qn[k];
+      }
+    }
+    for (int m = 0; m < 6; m++) {
+      const double *qm = qq + (size_t)npt * 3 * m;
+      const double fac = cell_dot(r, nP, qm, npt, qm, npt, 1);
+      for (int d = 0; d < 3; d++)
+        for (int q = 0; q < npt; q++)
+          s->sbq[(size_t)(m * 3 + d) * nP + r->off + q] = qm[q + (size_t)npt * d] / sqrt(fac);
+    }
+    free(pp);
+    free(qq);
+  }
+}
+
+/* DeflateRHS :219-274: FF += alpha/(1-alpha) a3 <FF, a3> / area, alpha = Bcoef/Acoef */
+static void deflate_rhs(orc_sys *s, double *v) {
+  const size_t nP = s->slr.nPoint;
+  for (int ir = 0; ir < s->nrbc; ir++) {
+    const orc_rbc *r = &s->rbcs[ir];
+    const int npt = r->nlat * r->nlon;
+    const double alpha = s->Bcoef[r->celltype - 1] / s->Acoef[r->celltype - 1];
+    double *F = v + r->off;
+    const double fac = cell_dot(r, nP, F, nP, r->a3, npt, 0);
+    for (int d = 0; d < 3; d++)
+      for (int q = 0; q < npt; q++)
+        F[q + nP * d] = F[q + nP * d] + alpha / (1. - alpha) * r->a3[q + (size_t)npt * d] * fac / r->area;
+  }
+}
+
+/* DeflateMatMult :281-349: vv += alpha a3 <g, a3> / area - alpha sum_m qq_m <g, qq_m> */
+static void deflate_matmult(orc_sys *s, double *v) {
+  const size_t nP = s->slr.nPoint;
+  for (int ir = 0; ir < s->nrbc; ir++) {
+    const orc_rbc *r = &s->rbcs[ir];
+    const int npt = r->nlat * r->nlon;
+    const double alpha = s->Bcoef[r->celltype - 1] / s->Acoef[r->celltype - 1];
+    double *V = v + r->off;
+    double fac = cell_dot(r, nP, r->g, npt, r->a3, npt, 0);
+    for (int d = 0; d < 3; d++)
+      for (int q = 0; q < npt; q++)
+        V[q + nP * d] = V[q + nP * d] + alpha * r->a3[q + (size_t)npt * d] * fac / r->area;
+    for (int m = 0; m < 6; m++) {
+      const double *qm = s->sbq + (size_t)m * 3 * nP + r->off;
+      fac = cell_dot(r, nP, r->g, npt, qm, nP, 0);
+      for (int d = 0; d < 3; d++)
+        for (int q = 0; q < npt; q++)
+          V[q + nP * d] = V[q + nP * d] - alpha * qm[q + nP * d] * fac;
+    }
+  }
+}
+
+/* DeflateRecoverSol :144-214 on the node velocities v (nP,3); rigid cells (viscRat <= 0)
+ * keep their solution as the double-layer potential g */
+static void deflate_recover(orc_sys *s, double *v) {
+  const size_t nP = s->slr.nPoint;
+  for (int ir = 0; ir < s->nrbc; ir++) {
+    orc_rbc *r = &s->rbcs[ir];
+    const int npt = r->nlat * r->nlon;
+    const double alpha = s->Bcoef[r->celltype - 1] / s->Acoef[r->celltype - 1];
+    double *vv = malloc(sizeof(double) * 3 * npt), *vw = malloc(sizeof(double) * 3 * npt);
+    for (int d = 0; d < 3; d++)
+      for (int q = 0; q < npt; q++) vv[q + (size_t)npt * d] = v[r->off + q + nP * d];
+    double afac;
+    if (s->conf.viscRat[r->celltype - 1] > 0) {
+      memcpy(vw, vv, sizeof(double) * 3 * npt);
+      afac = alpha / (alpha + 1.);
+    } else {
+      memset(vw, 0, sizeof(double) * 3 * npt);
+      memcpy(r->g, vv, sizeof(double) * 3 * npt);
+      afac = -1. / 2.;
+    }
+    for (int m = 0; m < 6; m++) {
+      const double *qm = s->sbq + (size_t)m * 3 * nP + r->off;
+      const double fac = cell_dot(r, nP, vv, npt, qm, nP, 0);
+      for (int d = 0; d < 3; d++)
+        for (int q = 0; q < npt; q++)
+          vw[q + (size_t)npt * d] = vw[q + (size_t)npt * d] - afac * qm[q + nP * d] * fac;
+    }
+    for (int d = 0; d < 3; d++)
+      for (int q = 0; q < npt; q++) v[r->off + q + nP * d] = vw[q + (size_t)npt * d];
+    free(vv);
+    free(vw);
+  }
 }
 
 /* Compute_Rhs: common/ModVelSolver.F90:455-515 (c1 = 1/4pi, c2 = 0) */
@@ -2534,6 +2707,7 @@ int orc_cell_rhs(orc_sys *s, double *rhs) {
   orc_pme_add_interp_vel(s, nP, tx, act, Ac, v);
   for (int d = 0; d < 3; d++)
     for (int i = 0; i < nP; i++) v[i + (size_t)nP * d] += 2. * s->conf.vBkg[d] / Ac[i];
+  if (s->deflate) deflate_rhs(s, v);
   orc_glob_sph_trans(s, v, rhs, 1);
   free(v);
   free(tx);
@@ -2775,6 +2949,7 @@ int orc_solve_rbc_vel(orc_sys *s, double *sol, double *v, int *niter, double *rn
   int dof = orc_cell_dof(s), nP = s->slr.nPoint;
   double *rhs = malloc(sizeof(double) * dof);
   orc_cell_rhs(s, rhs);
+  if (s->deflate) deflate_get_sb_modes(s);
   int all_one = 1;
   for (int l = 0; l < s->conf.ntypes; l++) all_one = all_one && s->conf.viscRat[l] == 1.0;
   *niter = 0;
@@ -2786,13 +2961,19 @@ int orc_solve_rbc_vel(orc_sys *s, double *sol, double *v, int *niter, double *rn
   double *c = malloc(sizeof(double) * dof);
   memcpy(c, sol, sizeof(double) * dof);
   orc_glob_sph_trans(s, v, c, -1);
-  /* g := v */
+  if (s->deflate) deflate_recover(s, v);
+  /* g := v for cells of finite viscosity (common/ModTimeInt.F90:780-784); a rigid cell
+   * keeps the potential the recovery stored */
   double *gc = malloc(sizeof(double) * 3 * nP);
   size_t off = 0;
   for (int ir = 0; ir < s->nrbc; ir++) {
-    int nvert = s->rbcs[ir].nlat * s->rbcs[ir].nlon;
+    const orc_rbc *r = &s->rbcs[ir];
+    int nvert = r->nlat * r->nlon;
+    const int finite = s->conf.viscRat[r->celltype - 1] > 0;
     for (int d = 0; d < 3; d++)
-      for (int q = 0; q < nvert; q++) gc[3 * off + q + (size_t)nvert * d] = v[off + q + (size_t)nP * d];
+      for (int q = 0; q < nvert; q++)
+        gc[3 * off + q + (size_t)nvert * d] =
+            finite ? v[off + q + (size_t)nP * d] : r->g[q + (size_t)nvert * d];
     off += nvert;
   }
   orc_set_cell_density(s, 2, gc);

@@ -153,6 +153,12 @@ int orc_shell_res_force(int nlat, const double *a1, const double *a2, const doub
                         double area, const double *a_rcp_ref, const double *b_ref, double ES,
                         double ED, double EB, double *f);
 int orc_rbc_sph_project(int nlat, int nlat0, int nvar, double *u);
+/* the reference's Deflate switch of the cell BIE (common/ModVelSolver.F90:103-106,
+ * 144-449): DeflateGetSBModes in Solve_RBC_Vel, DeflateRHS in Compute_Rhs,
+ * DeflateMatMult in MyMatMult, DeflateRecoverSol after the solve */
+int orc_set_deflation(orc_sys *s, int on);
+/* the solid-body modes of the last deflated solve: modes[(m * 3 + d) * nP + node] */
+int orc_sb_modes(const orc_sys *s, double *modes);
 
 #ifdef __cplusplus
 }

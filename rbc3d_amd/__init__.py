@@ -54,6 +54,8 @@ _SIG = {
     "rbc3d_near_stats": [C.c_void_p, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)],
     "rbc3d_set_geometry_reuse": [C.c_void_p, C.c_int],
     "rbc3d_set_graph": [C.c_void_p, C.c_int],
+    "rbc3d_set_deflation": [C.c_void_p, C.c_int],
+    "rbc3d_sb_modes": [C.c_void_p, dp],
     "rbc3d_graph_ready": [C.c_void_p, C.POINTER(C.c_int)],
     "rbc3d_set_precision": [C.c_void_p, C.c_int],
     "rbc3d_set_gmres_restart": [C.c_void_p, C.c_int],
@@ -366,6 +368,16 @@ class System:
 
     def set_geometry_reuse(self, enable):
         _call("rbc3d_set_geometry_reuse", self.h, int(bool(enable)))
+
+    def set_deflation(self, on):
+        """The reference's Deflate switch of the cell BIE (Pozrikidis' deflation of the
+        solid-body and normal modes)."""
+        _call("rbc3d_set_deflation", self.h, int(bool(on)))
+
+    def sb_modes(self):
+        m = np.zeros(18 * self.np)
+        _call("rbc3d_sb_modes", self.h, m)
+        return m.reshape(6, 3, self.np)
 
     def set_graph(self, on):
         """Replay the cell matvec from a recorded hipGraph at fixed geometry."""

@@ -640,6 +640,7 @@ extern "C" int rbc3d_set_cells(rbc3d_ctx *c, int nrbc, const int *nlat0, const i
     spl += (long long)SPL_NVAR * 2 * g->nlat * g->nlon * 4;
   }
   c->np = np;
+  c->sbq_ready = false;
   c->nchunks = choff;
   c->geom_epoch++;
   // the previous cell-BIE solution is the next solve's initial guess only for the same
@@ -1057,6 +1058,7 @@ static int cell_matmult_body(rbc3d_ctx *c, const double *d_u, double *d_b) {
   size_t n3 = (size_t)3 * np;
   k_add<<<(unsigned)((n3 + 255) / 256), 256, 0, st>>>(c->d_v.p, c->d_shbuf.p, n3);
   HIP_TRY(hipGetLastError());
+  RBC_TRY(rbc_defl_matmult(c, c->d_shbuf.p, c->d_v.p));
   RBC_TRY(rbc_sh_glob(c, c->d_v.p, d_b, 1));
   return RBC3D_OK;
 }
@@ -1176,6 +1178,7 @@ extern "C" int rbc3d_cell_rhs_dev(rbc3d_ctx *c, double *d_rhs) {
   RBC_TRY(rbc_pme_gather(c, np, c->d_tx.p, c->d_tact.p, c->d_tA.p, c->d_v.p));
   k_axpy_bkg<<<(np + 255) / 256, 256, 0, st>>>(c->d_v.p, np, c->d_tA.p, 2. * c->conf.vBkg[0],
                                                 2. * c->conf.vBkg[1], 2. * c->conf.vBkg[2], 1);
+  RBC_TRY(rbc_defl_rhs(c, c->d_v.p));
   RBC_TRY(rbc_sh_glob(c, c->d_v.p, d_rhs, 1));
   return RBC3D_OK;
 }

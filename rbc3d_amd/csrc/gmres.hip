@@ -8,6 +8,7 @@
 //
 // Vectors stay in HBM; the (restart+1) x restart Hessenberg matrix and the
 // rotations live on the host (a few hundred doubles).
+#include <algorithm>
 #include <cmath>
 #include <functional>
 #include <vector>
@@ -793,6 +794,220 @@ extern "C" int rbc3d_cell_rhs_dev(rbc3d_ctx *c, double *d_rhs);
 extern "C" int rbc3d_cell_matmult_dev(rbc3d_ctx *c, const double *d_u, double *d_b);
 int rbc_set_g_nodes(rbc3d_ctx *c, const double *d_gnodes);
 
+// ---------------------------------------------------------------------------
+// Deflation of the cell BIE (common/ModVelSolver.F90:144-449, the reference's Deflate
+// switch): per cell the six solid-body modes (DeflateGetSBModes), the right-hand side
+// and matvec corrections (DeflateRHS, DeflateMatMult) and the recovery of the velocity
+// (DeflateRecoverSol).  One workgroup per cell; the surface integrals sum(. * ds) are
+// workgroup reductions over the cell's nodes (ds = detJ w, the d_ds array).  Node
+// vectors are [3][np] over the concatenated cells, each cell at its own offset.
+#define DF_T 256
+
+__device__ __forceinline__ double df_block_sum(double v, double *red) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  __syncthreads();
+  if (lane == 0) red[wv] = v;
+  __syncthreads();
+  double t = 0;
+  for (int k = 0; k < DF_T / 64; k++) t += red[k];
+  return t;
+}
+
+// <a, b> over cell cr: sum_q sum_d a[q + lda d] b[q + ldb d] ds[q]
+__device__ double df_dot(const CellRec &cr, const double *a, size_t lda, const double *b,
+                         size_t ldb, const double *ds, double *red) {
+  const int npt = cr.nlat * cr.nlon;
+  double s = 0;
+  for (int q = threadIdx.x; q < npt; q += DF_T) {
+    double t = 0;
+    for (int d = 0; d < 3; d++) t += a[q + lda * d] * b[q + ldb * d];
+    s += t * ds[q];
+  }
+  return df_block_sum(s, red);
+}
+
+// DeflateGetSBModes (:354-449): translations and rotations about the centroid,
+// Gram-Schmidt orthonormalised (Proj, :603-621), into sbq[(m * 3 + d) * np + node]
+__global__ void __launch_bounds__(DF_T) k_defl_modes(int nrbc, const CellRec *__restrict__ cells,
+                                                     const double *__restrict__ x,
+                                                     const double *__restrict__ ds, int np,
+                                                     double *sbq) {
+  __shared__ double red[DF_T / 64];
+  const size_t NP = np;
+  for (int c = blockIdx.x; c < nrbc; c += gridDim.x) {
+    const CellRec cr = cells[c];
+    const int npt = cr.nlat * cr.nlon;
+    const double *X = x + cr.off, *S = ds + cr.off;
+    double ar = 0, c3[3] = {0, 0, 0};
+    for (int q = threadIdx.x; q < npt; q += DF_T) {
+      ar += S[q];
+      for (int d = 0; d < 3; d++) c3[d] += S[q] * X[q + NP * d];
+    }
+    ar = df_block_sum(ar, red);
+    for (int d = 0; d < 3; d++) c3[d] = df_block_sum(c3[d], red) / ar;
+    double *Q = sbq + cr.off;  // mode m, component d at Q + (m * 3 + d) * NP
+    for (int q = threadIdx.x; q < npt; q += DF_T) {
+      const double X0 = X[q] - c3[0], X1 = X[q + NP] - c3[1], X2 = X[q + 2 * NP] - c3[2];
+      const double v[6][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, -X2, X1}, {X2, 0, -X0}, {-X1, X0, 0}};
+      for (int m = 0; m < 6; m++)
+        for (int d = 0; d < 3; d++) Q[q + NP * (3 * m + d)] = v[m][d];
+    }
+    __syncthreads();
+    // qq_m = pp_m - sum_{n < m} <pp_m, qq_n> / <qq_n, qq_n> qq_n (pp_m is Q_m before the
+    // subtractions, kept in step by subtracting the projections all at the end)
+    for (int m = 1; m < 6; m++) {
+      double coef[6];
+      for (int n = 0; n < m; n++) {
+        const double fn = df_dot(cr, Q + NP * 3 * m, NP, Q + NP * 3 * n, NP, S, red);
+        const double fd = df_dot(cr, Q + NP * 3 * n, NP, Q + NP * 3 * n, NP, S, red);
+        coef[n] = fn / fd;
+      }
+      __syncthreads();
+      for (int q = threadIdx.x; q < npt; q += DF_T)
+        for (int d = 0; d < 3; d++) {
+          double t = Q[q + NP * (3 * m + d)];
+          for (int n = 0; n < m; n++) t = t - coef[n] * Q[q + NP * (3 * n + d)];
+          Q[q + NP * (3 * m + d)] = t;
+        }
+      __syncthreads();
+    }
+    double nrm[6];
+    for (int m = 0; m < 6; m++) nrm[m] = sqrt(df_dot(cr, Q + NP * 3 * m, NP, Q + NP * 3 * m, NP, S, red));
+    __syncthreads();
+    for (int q = threadIdx.x; q < npt; q += DF_T)
+      for (int m = 0; m < 6; m++)
+        for (int d = 0; d < 3; d++) Q[q + NP * (3 * m + d)] = Q[q + NP * (3 * m + d)] / nrm[m];
+    __syncthreads();
+  }
+}
+
+// DeflateRHS (:219-274): v += alpha/(1 - alpha) a3 <v, a3> / area, alpha = Bcoef/Acoef
+__global__ void __launch_bounds__(DF_T) k_defl_rhs(int nrbc, const CellRec *__restrict__ cells,
+                                                   const double *__restrict__ a3,
+                                                   const double *__restrict__ ds, int np, double *v) {
+  __shared__ double red[DF_T / 64];
+  const size_t NP = np;
+  for (int c = blockIdx.x; c < nrbc; c += gridDim.x) {
+    const CellRec cr = cells[c];
+    const int npt = cr.nlat * cr.nlon;
+    const double alpha = cr.Bcoef / cr.Acoef;
+    double *V = v + cr.off;
+    const double *A = a3 + cr.off;
+    const double fac = df_dot(cr, V, NP, A, NP, ds + cr.off, red);
+    __syncthreads();
+    for (int q = threadIdx.x; q < npt; q += DF_T)
+      for (int d = 0; d < 3; d++)
+        V[q + NP * d] = V[q + NP * d] + alpha / (1. - alpha) * A[q + NP * d] * fac / cr.area;
+    __syncthreads();
+  }
+}
+
+// DeflateMatMult (:281-349): v += alpha a3 <g, a3> / area - alpha sum_m qq_m <g, qq_m>
+__global__ void __launch_bounds__(DF_T) k_defl_matmult(int nrbc, const CellRec *__restrict__ cells,
+                                                       const double *__restrict__ a3,
+                                                       const double *__restrict__ ds,
+                                                       const double *__restrict__ sbq,
+                                                       const double *__restrict__ g, int np,
+                                                       double *v) {
+  __shared__ double red[DF_T / 64];
+  const size_t NP = np;
+  for (int c = blockIdx.x; c < nrbc; c += gridDim.x) {
+    const CellRec cr = cells[c];
+    const int npt = cr.nlat * cr.nlon;
+    const double alpha = cr.Bcoef / cr.Acoef;
+    const double *S = ds + cr.off, *A = a3 + cr.off, *G = g + cr.off, *Q = sbq + cr.off;
+    const double f0 = df_dot(cr, G, NP, A, NP, S, red);
+    double fm[6];
+    for (int m = 0; m < 6; m++) fm[m] = df_dot(cr, G, NP, Q + NP * 3 * m, NP, S, red);
+    __syncthreads();
+    double *V = v + cr.off;
+    for (int q = threadIdx.x; q < npt; q += DF_T)
+      for (int d = 0; d < 3; d++) {
+        double t = V[q + NP * d] + alpha * A[q + NP * d] * f0 / cr.area;
+        for (int m = 0; m < 6; m++) t = t - alpha * Q[q + NP * (3 * m + d)] * fm[m];
+        V[q + NP * d] = t;
+      }
+    __syncthreads();
+  }
+}
+
+// DeflateRecoverSol (:144-214) in place on the node velocities v; a rigid cell
+// (viscRat <= 0) keeps its solution before the recovery as the double-layer potential g
+__global__ void __launch_bounds__(DF_T) k_defl_recover(int nrbc, const CellRec *__restrict__ cells,
+                                                       const double *__restrict__ visc,
+                                                       const double *__restrict__ ds,
+                                                       const double *__restrict__ sbq, int np,
+                                                       double *v, double *g) {
+  __shared__ double red[DF_T / 64];
+  const size_t NP = np;
+  for (int c = blockIdx.x; c < nrbc; c += gridDim.x) {
+    const CellRec cr = cells[c];
+    const int npt = cr.nlat * cr.nlon;
+    const double alpha = cr.Bcoef / cr.Acoef;
+    const bool finite = visc[cr.celltype - 1] > 0;
+    const double afac = finite ? alpha / (alpha + 1.) : -1. / 2.;
+    double *V = v + cr.off;
+    const double *S = ds + cr.off, *Q = sbq + cr.off;
+    double fm[6];
+    for (int m = 0; m < 6; m++) fm[m] = df_dot(cr, V, NP, Q + NP * 3 * m, NP, S, red);
+    __syncthreads();
+    for (int q = threadIdx.x; q < npt; q += DF_T)
+      for (int d = 0; d < 3; d++) {
+        const double vv = V[q + NP * d];
+        if (!finite) g[cr.off + q + NP * d] = vv;
+        double t = finite ? vv : 0.;
+        for (int m = 0; m < 6; m++) t = t - afac * Q[q + NP * (3 * m + d)] * fm[m];
+        V[q + NP * d] = t;
+      }
+    __syncthreads();
+  }
+}
+
+// g := v on the nodes of cells of finite viscosity (common/ModTimeInt.F90:780-784)
+__global__ void k_g_from_v_finite(int np, const int *__restrict__ node_cell,
+                                  const CellRec *__restrict__ cells, const double *__restrict__ visc,
+                                  const double *__restrict__ v, double *g) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  if (!(visc[cells[node_cell[p]].celltype - 1] > 0)) return;
+  for (int d = 0; d < 3; d++) g[p + (size_t)np * d] = v[p + (size_t)np * d];
+}
+
+int rbc_defl_rhs(rbc3d_ctx *c, double *d_v) {
+  if (!c->deflate || !c->nrbc) return RBC3D_OK;
+  k_defl_rhs<<<std::min(c->nrbc, 4096), DF_T, 0, c->stream>>>(c->nrbc, c->d_cells.p, c->d_a3.p,
+                                                               c->d_ds.p, c->np, d_v);
+  HIP_TRY(hipGetLastError());
+  return RBC3D_OK;
+}
+int rbc_defl_matmult(rbc3d_ctx *c, const double *d_g, double *d_v) {
+  if (!c->deflate || !c->nrbc) return RBC3D_OK;
+  if (!c->sbq_ready) {
+    rbc_set_error("deflated matvec before the solid-body modes (rbc3d_solve_rbc_vel builds them)");
+    return RBC3D_ESTATE;
+  }
+  k_defl_matmult<<<std::min(c->nrbc, 4096), DF_T, 0, c->stream>>>(
+      c->nrbc, c->d_cells.p, c->d_a3.p, c->d_ds.p, c->d_sbq.p, d_g, c->np, d_v);
+  HIP_TRY(hipGetLastError());
+  return RBC3D_OK;
+}
+
+extern "C" int rbc3d_set_deflation(rbc3d_ctx *c, int on) {
+  if (!c) return RBC3D_EINVAL;
+  c->deflate = on != 0;
+  return RBC3D_OK;
+}
+
+extern "C" int rbc3d_sb_modes(rbc3d_ctx *c, double *modes) {
+  if (!c || !modes) return RBC3D_EINVAL;
+  if (!c->sbq_ready) return RBC3D_ESTATE;
+  HIP_TRY(hipMemcpyAsync(modes, c->d_sbq.p, sizeof(double) * 18 * (size_t)c->np,
+                         hipMemcpyDeviceToHost, c->stream));
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RBC3D_OK;
+}
+
 extern "C" int rbc3d_solve_rbc_vel(rbc3d_ctx *c, double *v, int *niter, double *rnorm) {
   if (!c || !niter || !rnorm) return RBC3D_EINVAL;
   if (c->nrbc == 0) return RBC3D_ESTATE;
@@ -808,6 +1023,13 @@ extern "C" int rbc3d_solve_rbc_vel(rbc3d_ctx *c, double *v, int *niter, double *
     c->cell_sol_init = true;
   }
   RBC_TRY(rbc3d_cell_rhs_dev(c, rhs.p));
+  if (c->deflate) {  // DeflateGetSBModes after Compute_Rhs (common/ModVelSolver.F90:98-101)
+    RBC_TRY(c->d_sbq.ensure(18 * (size_t)c->np + 1));
+    k_defl_modes<<<std::min(c->nrbc, 4096), DF_T, 0, st>>>(c->nrbc, c->d_cells.p, c->d_x.p,
+                                                           c->d_ds.p, c->np, c->d_sbq.p);
+    HIP_TRY(hipGetLastError());
+    c->sbq_ready = true;
+  }
   bool all_one = true;
   for (int l = 0; l < c->conf.ntypes; l++) all_one = all_one && c->conf.viscRat[l] == 1.0;
   *niter = 0;
@@ -822,7 +1044,27 @@ extern "C" int rbc3d_solve_rbc_vel(rbc3d_ctx *c, double *v, int *niter, double *
   }
   RBC_TRY(c->d_shbuf.ensure((size_t)3 * c->np + 1));
   RBC_TRY(rbc_sh_glob(c, c->d_cell_sol.p, c->d_shbuf.p, -1));
-  RBC_TRY(rbc_set_g_nodes(c, c->d_shbuf.p));
+  bool all_finite = true;
+  for (int l = 0; l < c->conf.ntypes; l++) all_finite = all_finite && c->conf.viscRat[l] > 0;
+  if (c->deflate || !all_finite) {
+    RBC_TRY(c->d_visc.ensure(8));
+    HIP_TRY(hipMemcpyAsync(c->d_visc.p, c->conf.viscRat, sizeof(double) * c->conf.ntypes,
+                           hipMemcpyHostToDevice, st));
+  }
+  if (c->deflate) {  // DeflateRecoverSol (common/ModVelSolver.F90:122-125)
+    k_defl_recover<<<std::min(c->nrbc, 4096), DF_T, 0, st>>>(c->nrbc, c->d_cells.p, c->d_visc.p,
+                                                             c->d_ds.p, c->d_sbq.p, c->np,
+                                                             c->d_shbuf.p, c->d_g.p);
+    HIP_TRY(hipGetLastError());
+  }
+  if (all_finite) {
+    RBC_TRY(rbc_set_g_nodes(c, c->d_shbuf.p));
+  } else {  // a rigid cell keeps its potential (the recovery's, or the last one set)
+    k_g_from_v_finite<<<(c->np + 255) / 256, 256, 0, st>>>(c->np, c->d_node_cell.p, c->d_cells.p,
+                                                           c->d_visc.p, c->d_shbuf.p, c->d_g.p);
+    HIP_TRY(hipGetLastError());
+    RBC_TRY(rbc_update_density(c, 2));
+  }
   if (v) HIP_TRY(hipMemcpyAsync(v, c->d_shbuf.p, sizeof(double) * 3 * c->np, hipMemcpyDeviceToHost, st));
   HIP_TRY(hipStreamSynchronize(st));
   return gmres_status(reason, "rbc3d_solve_rbc_vel");

@@ -291,6 +291,10 @@ struct rbc3d_ctx {
   int mv_epoch = -1, mv_prec = -1;
   unsigned long long mv_allocs = 0;
   DBuf<double> d_gin, d_gout;
+  // deflation of the cell BIE (rbc3d_set_deflation): the solid-body modes of the last
+  // Solve_RBC_Vel, [6][3][np]
+  bool deflate = false, sbq_ready = false;
+  DBuf<double> d_sbq, d_visc;
 };
 
 // ---------------------------------------------------------------------------
@@ -511,6 +515,9 @@ int rbc_sh_synth_batch(int nlat, int nlon, int nt, double *d_g, int idg, int jdg
                        const double *d_b, int ida, int jda, int equi, hipStream_t st);
 int rbc_build_source_list(rbc3d_ctx *c);
 int rbc_update_density(rbc3d_ctx *c, int which);
+// deflation corrections (gmres.hip); no-ops unless rbc3d_set_deflation is on
+int rbc_defl_rhs(rbc3d_ctx *c, double *d_v);
+int rbc_defl_matmult(rbc3d_ctx *c, const double *d_g, double *d_v);
 int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const double *d_tx,
                             const int *d_tindx, const int *d_act, const double *d_A, double *d_v,
                             bool cacheable);
