@@ -167,6 +167,45 @@ int rbc3d_shell_res_force(int ncell, const int *nlat, const int *celltype, const
  * (common/ModTimeInt.F90:908-934). */
 int rbc3d_rbc_sph_project(int ncell, const int *nlat, const int *nlat0, int nvar, double *u);
 
+/* --- time stepping of the context's cells (common/ModTimeInt.F90) ---------------
+ * The cells' coordinates stay on the device between steps; rbc3d_set_cells gives the
+ * initial state, rbc3d_get_rbc_coords reads it back in the same per-cell layout. */
+/* rbcRefs (common/ModTimeInt.F90:63-76, common/ModRbc.F90 RBC_ComputeGeometry): one
+ * reference cell per cell type k (1..ntypes), its coordinates x(nlat(k), 2 nlat(k), 3)
+ * concatenated over types, and the membrane moduli ES(k), ED(k), EB(k).  A cell's grid
+ * must be that of its type's reference cell. */
+int rbc3d_set_ref_cells(rbc3d_ctx *ctx, int ntypes, const int *nlat, const double *x,
+                        const double *ES, const double *ED, const double *EB);
+/* Compute_Rbc_Vel (common/ModTimeInt.F90:716-808): RBC_ComputeGeometry and the surface
+ * sources of every cell, the force density -Shell_ResForce projected by RBC_SphProject,
+ * Solve_RBC_Vel.  The node velocities v(3, np) (as rbc3d_solve_rbc_vel returns them) stay
+ * on the device for rbc3d_move_rbcs; v may be NULL.  The reference's intra-cell
+ * repulsion (AddIntraCellRepulsionForce, :756) is not applied. */
+int rbc3d_compute_rbc_vel(rbc3d_ctx *ctx, double *v, int *niter, double *rnorm);
+/* rbc%x = rbc%x + Ts*rbc%v for every cell (common/ModTimeInt.F90:131-136), with the
+ * velocities of the last rbc3d_compute_rbc_vel. */
+int rbc3d_move_rbcs(rbc3d_ctx *ctx, double Ts);
+/* ReboxRbcs (common/ModTimeInt.F90:1006-1020): each cell shifted by whole periods so the
+ * centre of its coordinate range lies in [0, Lb). */
+int rbc3d_rebox_rbcs(rbc3d_ctx *ctx);
+/* VolConstrainRbcs (common/ModTimeInt.F90:938-963): x += fac a3 per cell, fac =
+ * (vol_ref - vol)/area rounded to single precision (the reference's default REAL) and
+ * limited to eps_dist/20 in magnitude. */
+int rbc3d_vol_constrain_rbcs(rbc3d_ctx *ctx, double eps_dist);
+/* FilterRbcs (common/ModTimeInt.F90:908-934): every cell's coordinates projected onto
+ * spherical-harmonic modes below nlat0. */
+int rbc3d_filter_rbcs(rbc3d_ctx *ctx);
+/* The cells' coordinates, x(nlat, nlon, 3) per cell, concatenated (rbc3d_set_cells' layout). */
+int rbc3d_get_rbc_coords(rbc3d_ctx *ctx, double *x);
+/* TimeInt_Euler (common/ModTimeInt.F90:109-178), nsteps steps of Ts: Compute_Rbc_Vel,
+ * NoSlipWall when a wall is set (rtol, maxit as rbc3d_noslip_wall), x += Ts v,
+ * ReboxRbcs, four rounds of VolConstrainRbcs + FilterRbcs (the reference's
+ * InterCellRepulsion between them is not applied); *time += Ts per step.  its_cell,
+ * its_wall (length nsteps, optional): the GMRES iterations of each step's two solves. */
+int rbc3d_timeint_euler(rbc3d_ctx *ctx, double Ts, int nsteps, double eps_dist,
+                        double noslip_rtol, int noslip_maxit, double *time, int *its_cell,
+                        int *its_wall);
+
 /* Glob_Sph_Trans(v,c,direction) (common/ModVelSolver.F90:641-719); direction
  * 1 = PHYS_TO_FOUR, -1 = FOUR_TO_PHYS.  v(nPoint,3), c(3*sum nlat0^2). */
 int rbc3d_glob_sph_trans(rbc3d_ctx *ctx, double *v, double *c, int direction);

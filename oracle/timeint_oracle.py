@@ -1,0 +1,95 @@
+"""TimeInt_Euler (common/ModTimeInt.F90:109-178) restated over the oracle's pieces.
+
+TEST INFRASTRUCTURE ONLY: the checker of the device time stepping; the product path never
+imports this module.
+
+One step, in the reference's order:
+  Compute_Rbc_Vel (:716-808)  RBC_ComputeGeometry per cell (orc_rbc_geometry), the surface
+                              sources (orc_set_cells), f = -Shell_ResForce projected by
+                              RBC_SphProject, Solve_RBC_Vel (orc_solve_rbc_vel; its
+                              previous solution is the next initial guess)
+  NoSlipWall (ModNoSlip:45)   when the oracle holds a wall
+  rbc%x += Ts rbc%v (:131-136)
+  ReboxRbcs (:1006-1020)
+  4 x {VolConstrainRbcs (:938-963), FilterRbcs (:908-934)}; the reference's
+  InterCellRepulsion between them is not part of this restatement.
+"""
+from __future__ import annotations
+
+import dataclasses
+
+import numpy as np
+
+from oracle import oracle_py as O
+
+
+def split_cells(v, nlat, ncomp):
+    out, o = [], 0
+    for nl in nlat:
+        n = 2 * int(nl) * int(nl) * ncomp
+        out.append(np.array(v[o:o + n], dtype=np.float64))
+        o += n
+    return out
+
+
+def rebox(x, Lb):
+    """ReboxRbcs of one cell x(nlat, nlon, 3) (flat, column-major)."""
+    X = x.reshape(3, -1)
+    for d in range(3):
+        xc = 0.5 * (X[d].max() + X[d].min())
+        X[d] = X[d] - np.floor(xc * (1.0 / Lb[d])) * Lb[d]
+    return X.ravel()
+
+
+def vol_constrain(x, ref_vol, eps_dist):
+    """VolConstrainRbcs of one cell: fac is the reference's default (single precision)
+    REAL, fac = SIGN(MIN(ABS(fac), epsDist/20.), fac)."""
+    g = O.rbc_geometry(x)
+    fac = np.float32((ref_vol - g["vol"]) / g["area"])
+    fac = np.float32(np.copysign(min(abs(float(fac)), eps_dist / 20.0), float(fac)))
+    return x + float(fac) * g["a3"]
+
+
+def cell_force(x, nlat0, ref, ES, ED, EB):
+    """rbc%f = -Shell_ResForce(rbc, rbcRef), then RBC_SphProject(rbc, 3, rbc%f)."""
+    g = O.rbc_geometry(x)
+    return g, -O.rbc_sph_project(O.shell_res_force(g, ref, ES, ED, EB), g["nlat"], nlat0)
+
+
+def timeint_euler(o, cells, refs, ES, ED, EB, Ts, nsteps, eps_dist, Lb, noslip=None):
+    """nsteps Euler steps of the cells (a configs.Cells) on the oracle `o` (built with
+    the same configuration); refs[k]: rbc_geometry of cell type k+1's reference cell.
+    noslip: None, or (rtol, maxit) of NoSlipWall.  Returns the final coordinates in the
+    cells' layout and the per-step (cell, wall) GMRES iterations."""
+    nlat, nlat0, ct = cells.nlat, cells.nlat0, cells.celltype
+    xs = split_cells(cells.x, nlat, 3)
+    its = []
+    for _ in range(nsteps):
+        geos, fs = [], []
+        for c, x in enumerate(xs):
+            k = ct[c] - 1
+            g, f = cell_force(x, int(nlat0[c]), refs[k], ES[k], ED[k], EB[k])
+            geos.append(g)
+            fs.append(f)
+        cur = dataclasses.replace(cells, x=np.concatenate(xs),
+                                  a3=np.concatenate([g["a3"] for g in geos]),
+                                  detj=np.concatenate([g["detj"] for g in geos]))
+        o.set_cells(cur)
+        o.set_density(1, np.concatenate(fs))
+        v, ni, _ = o.solve_rbc_vel()
+        nw = 0
+        if noslip is not None:
+            nw = o.noslip_wall(*noslip)[0]
+        its.append((ni, nw))
+        V = v.reshape(3, -1)
+        off = 0
+        for c, x in enumerate(xs):
+            npt = 2 * int(nlat[c]) ** 2
+            X = x.reshape(3, npt) + Ts * V[:, off:off + npt]
+            xs[c] = rebox(X.ravel(), Lb)
+            off += npt
+        for _ in range(4):
+            for c in range(len(xs)):
+                xs[c] = vol_constrain(xs[c], refs[ct[c] - 1]["vol"], eps_dist)
+                xs[c] = O.rbc_sph_project(xs[c], int(nlat[c]), int(nlat0[c]))
+    return np.concatenate(xs), its

@@ -106,6 +106,15 @@ _SIG = {
     "rbc3d_noslip_wall": [C.c_void_p, C.c_double, C.c_int, C.c_void_p, C.POINTER(C.c_int),
                           C.POINTER(C.c_double), C.POINTER(C.c_int), C.POINTER(C.c_double)],
     "rbc3d_solve_rbc_vel": [C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)],
+    "rbc3d_set_ref_cells": [C.c_void_p, C.c_int, ip, dp, dp, dp, dp],
+    "rbc3d_compute_rbc_vel": [C.c_void_p, C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double)],
+    "rbc3d_move_rbcs": [C.c_void_p, C.c_double],
+    "rbc3d_rebox_rbcs": [C.c_void_p],
+    "rbc3d_vol_constrain_rbcs": [C.c_void_p, C.c_double],
+    "rbc3d_filter_rbcs": [C.c_void_p],
+    "rbc3d_get_rbc_coords": [C.c_void_p, dp],
+    "rbc3d_timeint_euler": [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int,
+                            C.POINTER(C.c_double), C.c_void_p, C.c_void_p],
     "rbc3d_timing": [C.c_void_p, C.c_int],
     "rbc3d_stage_time": [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_longlong),
                          C.POINTER(C.c_char_p)],
@@ -547,6 +556,48 @@ class System:
         rn = C.c_double()
         _call("rbc3d_solve_rbc_vel", self.h, v.ctypes.data_as(C.c_void_p), C.byref(ni), C.byref(rn))
         return v, ni.value, rn.value
+
+    # --- time stepping (common/ModTimeInt.F90) ----------------------------------
+    def set_ref_cells(self, nlat, x, ES, ED, EB):
+        """rbcRefs: one reference cell per type, x(nlat, 2 nlat, 3) concatenated."""
+        _call("rbc3d_set_ref_cells", self.h, len(nlat), _i32(nlat), _f64(x), _f64(ES), _f64(ED),
+              _f64(EB))
+
+    def compute_rbc_vel(self):
+        """Compute_Rbc_Vel: returns (v(3, np) flat, niter, residual)."""
+        v = np.zeros(3 * self.np)
+        ni = C.c_int()
+        rn = C.c_double()
+        _call_solve("rbc3d_compute_rbc_vel", self.h, v.ctypes.data_as(C.c_void_p), C.byref(ni),
+                    C.byref(rn))
+        return v, ni.value, rn.value
+
+    def move_rbcs(self, Ts):
+        _call("rbc3d_move_rbcs", self.h, Ts)
+
+    def rebox_rbcs(self):
+        _call("rbc3d_rebox_rbcs", self.h)
+
+    def vol_constrain_rbcs(self, eps_dist):
+        _call("rbc3d_vol_constrain_rbcs", self.h, eps_dist)
+
+    def filter_rbcs(self):
+        _call("rbc3d_filter_rbcs", self.h)
+
+    def rbc_coords(self):
+        """The cells' coordinates in rbc3d_set_cells' per-cell layout."""
+        x = np.zeros(3 * self.np)
+        _call("rbc3d_get_rbc_coords", self.h, x)
+        return x
+
+    def timeint_euler(self, Ts, nsteps, eps_dist, noslip_rtol=1e-8, noslip_maxit=60, time=0.0):
+        """TimeInt_Euler for nsteps: returns (time, cell iterations, wall iterations)."""
+        t = C.c_double(time)
+        ic = np.zeros(max(nsteps, 1), np.int32)
+        iw = np.zeros(max(nsteps, 1), np.int32)
+        _call("rbc3d_timeint_euler", self.h, Ts, nsteps, eps_dist, noslip_rtol, noslip_maxit,
+              C.byref(t), ic.ctypes.data_as(C.c_void_p), iw.ctypes.data_as(C.c_void_p))
+        return t.value, ic[:nsteps], iw[:nsteps]
 
     def stream(self):
         s = C.c_void_p()

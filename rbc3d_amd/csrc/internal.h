@@ -295,6 +295,20 @@ struct rbc3d_ctx {
   // Solve_RBC_Vel, [6][3][np]
   bool deflate = false, sbq_ready = false;
   DBuf<double> d_sbq, d_visc;
+  // time stepping on the device (timeint.hip, mech.hip): reference cells per type
+  // (rbcRefs: a_rcp, b of their grid, volume), membrane moduli, and the cells' geometry
+  // in per-cell blocks (x, a1, a2, a3, a, a_rcp, b, detJ, ds; xc, area, vol, mesh)
+  int nref = 0;
+  std::vector<int> ref_nlat, ref_off;
+  std::vector<double> ref_vol, ES, ED, EB;
+  DBuf<double> d_ref_ar, d_ref_b;
+  DBuf<double> d_mx, d_ma1, d_ma2, d_ma3, d_ma, d_mar, d_mb, d_mdetj, d_mscr, d_mf;
+  DBuf<double> d_mxc, d_marea, d_mvol, d_mmesh, d_mmod;
+  DBuf<int> d_moff, d_mnlat, d_mnlat0, d_mref;
+  DBuf<double> d_vel;  // the cells' node velocities of the last Compute_Rbc_Vel, [3][np]
+  bool vel_ready = false;
+  int mech_epoch = -1;  // geom_epoch the per-cell tables above were built for
+  bool rbc_geom_ready = false;  // the block geometry above is that of d_x
 };
 
 // ---------------------------------------------------------------------------
@@ -515,6 +529,12 @@ int rbc_sh_synth_batch(int nlat, int nlon, int nt, double *d_g, int idg, int jdg
                        const double *d_b, int ida, int jda, int equi, hipStream_t st);
 int rbc_build_source_list(rbc3d_ctx *c);
 int rbc_update_density(rbc3d_ctx *c, int which);
+// per-cell mechanics on the context's cells (mech.hip): RBC_ComputeGeometry of every
+// cell from d_x (refreshing a3, detJ, dS, the cell records, splines and source lists),
+// and the surface force density -Shell_ResForce, projected (RBC_SphProject), into d_f
+int rbc_cells_geometry(rbc3d_ctx *c, bool sources);
+int rbc_cells_force(rbc3d_ctx *c);
+int rbc_cells_project_x(rbc3d_ctx *c);
 // deflation corrections (gmres.hip); no-ops unless rbc3d_set_deflation is on
 int rbc_defl_rhs(rbc3d_ctx *c, double *d_v);
 int rbc_defl_matmult(rbc3d_ctx *c, const double *d_g, double *d_v);

@@ -698,3 +698,249 @@ extern "C" int rbc3d_rbc_sph_project(int ncell, const int *nlat, const int *nlat
   HIP_TRY(hipMemcpy(u, d_u, n * 8, hipMemcpyDeviceToHost));
   return RBC3D_OK;
 }
+
+// ---------------------------------------------------------------------------
+// the context's cells: geometry refresh, membrane force and coordinate filter on device
+
+// [3][NP] node arrays (the context's layout) <-> per-cell (nlat, nlon, 3) blocks
+__global__ void k_nodes_to_blocks(int np, const int *__restrict__ node_cell,
+                                  const CellRec *__restrict__ cells, const double *__restrict__ src,
+                                  int ncomp, double *dst) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const CellRec cr = cells[node_cell[p]];
+  const int npt = cr.nlat * cr.nlon, q = p - cr.off;
+  for (int d = 0; d < ncomp; d++) dst[(size_t)ncomp * cr.off + q + (size_t)npt * d] = src[p + (size_t)np * d];
+}
+__global__ void k_blocks_to_nodes(int np, const int *__restrict__ node_cell,
+                                  const CellRec *__restrict__ cells, const double *__restrict__ src,
+                                  int ncomp, double scale, double *dst) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const CellRec cr = cells[node_cell[p]];
+  const int npt = cr.nlat * cr.nlon, q = p - cr.off;
+  for (int d = 0; d < ncomp; d++)
+    dst[p + (size_t)np * d] = scale * src[(size_t)ncomp * cr.off + q + (size_t)npt * d];
+}
+// dS = detJ w at every node (the set_cells product) and the cell records' area and meshSize
+__global__ void k_ds_nodes(int np, const int *__restrict__ node_cell, const int *__restrict__ node_ll,
+                           const CellRec *__restrict__ cells, MechGrids G,
+                           const double *__restrict__ detj, double *ds) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= np) return;
+  const CellRec cr = cells[node_cell[p]];
+  const MechGrid &T = grid_of(G, cr.nlat);
+  ds[p] = detj[p] * T.w[node_ll[p] & 0xffff];
+}
+__global__ void k_cellrec_area(int nrbc, CellRec *cells, const double *__restrict__ area,
+                               const double *__restrict__ mesh) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nrbc) return;
+  cells[i].area = area[i];
+  cells[i].meshSize = mesh[i];
+}
+
+// per-cell tables of the context's cells (offsets, grids, reference cells), rebuilt when
+// the cells are set again
+static int mech_tables(rbc3d_ctx *c, CellBatch *cb) {
+  std::vector<int> nl(c->nrbc), nl0(c->nrbc), ref(c->nrbc, 0);
+  for (int i = 0; i < c->nrbc; i++) {
+    nl[i] = c->cells[i].nlat;
+    nl0[i] = c->cells[i].nlat0;
+  }
+  RBC_TRY(cell_batch(c->nrbc, nl.data(), cb));
+  if (c->mech_epoch == c->geom_epoch) return RBC3D_OK;
+  for (int i = 0; i < c->nrbc; i++)
+    if (cb->off[i] != c->cells[i].off) {
+      rbc_set_error("cell mechanics: node offsets differ from the cell records");
+      return RBC3D_ESTATE;
+    }
+  hipStream_t st = c->stream;
+  const size_t np = cb->np;
+  RBC_TRY(c->d_moff.ensure(c->nrbc));
+  RBC_TRY(c->d_mnlat.ensure(c->nrbc));
+  RBC_TRY(c->d_mnlat0.ensure(c->nrbc));
+  HIP_TRY(hipMemcpyAsync(c->d_moff.p, cb->off.data(), sizeof(int) * c->nrbc, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(c->d_mnlat.p, nl.data(), sizeof(int) * c->nrbc, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(c->d_mnlat0.p, nl0.data(), sizeof(int) * c->nrbc, hipMemcpyHostToDevice, st));
+  for (DBuf<double> *b : {&c->d_mx, &c->d_ma1, &c->d_ma2, &c->d_ma3, &c->d_mf})
+    RBC_TRY(b->ensure(3 * np + 1));
+  for (DBuf<double> *b : {&c->d_ma, &c->d_mar, &c->d_mb}) RBC_TRY(b->ensure(4 * np + 1));
+  RBC_TRY(c->d_mdetj.ensure(np + 1));
+  RBC_TRY(c->d_mscr.ensure(MECH_SCR * np + 1));
+  RBC_TRY(c->d_mxc.ensure(3 * (size_t)c->nrbc + 1));
+  for (DBuf<double> *b : {&c->d_marea, &c->d_mvol, &c->d_mmesh}) RBC_TRY(b->ensure(c->nrbc + 1));
+  RBC_TRY(c->d_vel.ensure(3 * np + 1));
+  HIP_TRY(hipStreamSynchronize(st));
+  c->mech_epoch = c->geom_epoch;
+  return RBC3D_OK;
+}
+
+// RBC_ComputeGeometry of every cell from d_x: the block geometry, a3, detJ, dS and the
+// cell records' area and meshSize.  With sources, also what Compute_Rbc_Vel does before
+// its forces (common/ModTimeInt.F90:737-745): Rbc_BuildSurfaceSource(xFlag),
+// SourceList_UpdateCoord, TargetList_Update -- the x/a3/detJ splines and the source and
+// cell lists, on the device, under a new geometry epoch.
+int rbc_cells_geometry(rbc3d_ctx *c, bool sources) {
+  if (!c->nrbc) return RBC3D_OK;
+  CellBatch cb;
+  RBC_TRY(mech_tables(c, &cb));
+  hipStream_t st = c->stream;
+  const int np = c->np, nb = (np + 255) / 256;
+  k_nodes_to_blocks<<<nb, 256, 0, st>>>(np, c->d_node_cell.p, c->d_cells.p, c->d_x.p, 3, c->d_mx.p);
+  GeoOut o;
+  o.x = c->d_mx.p;
+  o.a1 = c->d_ma1.p;
+  o.a2 = c->d_ma2.p;
+  o.a3 = c->d_ma3.p;
+  o.a = c->d_ma.p;
+  o.ar = c->d_mar.p;
+  o.b = c->d_mb.p;
+  o.detj = c->d_mdetj.p;
+  o.xc = c->d_mxc.p;
+  o.area = c->d_marea.p;
+  o.vol = c->d_mvol.p;
+  o.mesh = c->d_mmesh.p;
+  o.scr = c->d_mscr.p;
+  RBC_TRY(rbc_rbc_geometry_dev(st, c->nrbc, c->d_moff.p, c->d_mnlat.p, cb.G, cb.lds, o));
+  k_blocks_to_nodes<<<nb, 256, 0, st>>>(np, c->d_node_cell.p, c->d_cells.p, c->d_ma3.p, 3, 1.0, c->d_a3.p);
+  k_blocks_to_nodes<<<nb, 256, 0, st>>>(np, c->d_node_cell.p, c->d_cells.p, c->d_mdetj.p, 1, 1.0,
+                                        c->d_detj.p);
+  k_ds_nodes<<<nb, 256, 0, st>>>(np, c->d_node_cell.p, c->d_node_ll.p, c->d_cells.p, cb.G, c->d_detj.p,
+                                 c->d_ds.p);
+  k_cellrec_area<<<(c->nrbc + 255) / 256, 256, 0, st>>>(c->nrbc, c->d_cells.p, c->d_marea.p,
+                                                        c->d_mmesh.p);
+  HIP_TRY(hipGetLastError());
+  std::vector<double> ar(c->nrbc), me(c->nrbc);
+  HIP_TRY(hipMemcpyAsync(ar.data(), c->d_marea.p, sizeof(double) * c->nrbc, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipMemcpyAsync(me.data(), c->d_mmesh.p, sizeof(double) * c->nrbc, hipMemcpyDeviceToHost, st));
+  HIP_TRY(hipStreamSynchronize(st));
+  for (int i = 0; i < c->nrbc; i++) {
+    if (!std::isfinite(ar[i])) {
+      rbc_set_error("cell geometry: non-finite area (the cell surface degenerated)");
+      return RBC3D_ENONFINITE;
+    }
+    c->cells[i].area = ar[i];
+    c->cells[i].meshSize = me[i];
+  }
+  if (!sources) {
+    c->rbc_geom_ready = false;  // the splines and source list are not those of d_x
+    return RBC3D_OK;
+  }
+  RBC_TRY(rbc_sh_build_splines(c, 0));
+  RBC_TRY(rbc_build_source_list(c));
+  c->geom_epoch++;
+  c->mech_epoch = c->geom_epoch;  // same cells: the tables stay valid
+  c->rbc_geom_ready = true;
+  HIP_TRY(hipStreamSynchronize(st));
+  return RBC3D_OK;
+}
+
+// rbc%f = -Shell_ResForce(rbc, rbcRef), RBC_SphProject(rbc, 3, rbc%f) and the f spline and
+// source densities (common/ModTimeInt.F90:747-769, without the intra-cell repulsion)
+int rbc_cells_force(rbc3d_ctx *c) {
+  if (!c->nrbc) return RBC3D_OK;
+  if (!c->rbc_geom_ready) {
+    rbc_set_error("cell force before the cell geometry");
+    return RBC3D_ESTATE;
+  }
+  if (c->nref == 0) {
+    rbc_set_error("cell force: no reference cells (rbc3d_set_ref_cells)");
+    return RBC3D_ESTATE;
+  }
+  CellBatch cb;
+  RBC_TRY(mech_tables(c, &cb));
+  hipStream_t st = c->stream;
+  const int np = c->np;
+  std::vector<int> ref(c->nrbc);
+  std::vector<double> mod(3 * (size_t)c->nrbc);
+  for (int i = 0; i < c->nrbc; i++) {
+    const int k = c->cells[i].celltype - 1;
+    if (k >= c->nref || c->ref_nlat[k] != c->cells[i].nlat) {
+      rbc_set_error("cell force: cell " + std::to_string(i) + " has no reference cell of its grid");
+      return RBC3D_EINVAL;
+    }
+    ref[i] = c->ref_off[k];
+    mod[i] = c->ES[k];
+    mod[c->nrbc + i] = c->ED[k];
+    mod[2 * c->nrbc + i] = c->EB[k];
+  }
+  RBC_TRY(c->d_mref.ensure(c->nrbc));
+  RBC_TRY(c->d_mmod.ensure(3 * (size_t)c->nrbc));
+  HIP_TRY(hipMemcpyAsync(c->d_mref.p, ref.data(), sizeof(int) * c->nrbc, hipMemcpyHostToDevice, st));
+  HIP_TRY(hipMemcpyAsync(c->d_mmod.p, mod.data(), sizeof(double) * mod.size(), hipMemcpyHostToDevice, st));
+  ForceArgs o;
+  o.off = c->d_moff.p;
+  o.nlat = c->d_mnlat.p;
+  o.ref_of = c->d_mref.p;
+  o.a1 = c->d_ma1.p;
+  o.a2 = c->d_ma2.p;
+  o.a3 = c->d_ma3.p;
+  o.a = c->d_ma.p;
+  o.ar = c->d_mar.p;
+  o.b = c->d_mb.p;
+  o.detj = c->d_mdetj.p;
+  o.area = c->d_marea.p;
+  o.ar_ref = c->d_ref_ar.p;
+  o.b_ref = c->d_ref_b.p;
+  o.ES = c->d_mmod.p;
+  o.ED = c->d_mmod.p + c->nrbc;
+  o.EB = c->d_mmod.p + 2 * c->nrbc;
+  o.f = c->d_mf.p;
+  o.scr = c->d_mscr.p;
+  hipLaunchKernelGGL(k_shell_res_force, dim3(mech_blocks(c->nrbc)), dim3(MECH_T), cb.lds, st, c->nrbc,
+                     cb.G, o);
+  hipLaunchKernelGGL(k_rbc_sph_project, dim3(mech_blocks(c->nrbc)), dim3(MECH_T), cb.lds, st, c->nrbc,
+                     c->d_moff.p, c->d_mnlat.p, c->d_mnlat0.p, 3, cb.G, c->d_mf.p);
+  const int nb = (np + 255) / 256;
+  k_blocks_to_nodes<<<nb, 256, 0, st>>>(np, c->d_node_cell.p, c->d_cells.p, c->d_mf.p, 3, -1.0, c->d_f.p);
+  HIP_TRY(hipGetLastError());
+  RBC_TRY(rbc_update_density(c, 1));
+  return RBC3D_OK;
+}
+
+// FilterRbcs (common/ModTimeInt.F90:908-934): RBC_SphProject of every cell's coordinates
+int rbc_cells_project_x(rbc3d_ctx *c) {
+  if (!c->nrbc) return RBC3D_OK;
+  CellBatch cb;
+  RBC_TRY(mech_tables(c, &cb));
+  hipStream_t st = c->stream;
+  const int np = c->np, nb = (np + 255) / 256;
+  k_nodes_to_blocks<<<nb, 256, 0, st>>>(np, c->d_node_cell.p, c->d_cells.p, c->d_x.p, 3, c->d_mx.p);
+  hipLaunchKernelGGL(k_rbc_sph_project, dim3(mech_blocks(c->nrbc)), dim3(MECH_T), cb.lds, st, c->nrbc,
+                     c->d_moff.p, c->d_mnlat.p, c->d_mnlat0.p, 3, cb.G, c->d_mx.p);
+  k_blocks_to_nodes<<<nb, 256, 0, st>>>(np, c->d_node_cell.p, c->d_cells.p, c->d_mx.p, 3, 1.0, c->d_x.p);
+  HIP_TRY(hipGetLastError());
+  c->rbc_geom_ready = false;
+  return RBC3D_OK;
+}
+
+// rbcRefs (common/ModTimeInt.F90:63-76): one reference cell per cell type, its geometry
+// from x_ref, and the membrane moduli of the type
+extern "C" int rbc3d_set_ref_cells(rbc3d_ctx *c, int ntypes, const int *nlat, const double *x,
+                                   const double *ES, const double *ED, const double *EB) {
+  if (!c || ntypes < 1 || ntypes > 8 || !nlat || !x || !ES || !ED || !EB) return RBC3D_EINVAL;
+  size_t np = 0;
+  std::vector<int> off(ntypes);
+  for (int k = 0; k < ntypes; k++) {
+    off[k] = (int)np;
+    np += 2 * (size_t)nlat[k] * nlat[k];
+  }
+  std::vector<double> a1(3 * np), a2(3 * np), a3(3 * np), a(4 * np), ar(4 * np), b(4 * np), dj(np),
+      xc(3 * (size_t)ntypes), area(ntypes), vol(ntypes), mesh(ntypes);
+  RBC_TRY(rbc3d_rbc_compute_geometry(ntypes, nlat, x, a1.data(), a2.data(), a3.data(), a.data(),
+                                     ar.data(), b.data(), dj.data(), xc.data(), area.data(),
+                                     vol.data(), mesh.data()));
+  RBC_TRY(c->d_ref_ar.ensure(4 * np));
+  RBC_TRY(c->d_ref_b.ensure(4 * np));
+  HIP_TRY(hipMemcpy(c->d_ref_ar.p, ar.data(), ar.size() * 8, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(c->d_ref_b.p, b.data(), b.size() * 8, hipMemcpyHostToDevice));
+  c->nref = ntypes;
+  c->ref_nlat.assign(nlat, nlat + ntypes);
+  c->ref_off = off;
+  c->ref_vol = vol;
+  c->ES.assign(ES, ES + ntypes);
+  c->ED.assign(ED, ED + ntypes);
+  c->EB.assign(EB, EB + ntypes);
+  return RBC3D_OK;
+}

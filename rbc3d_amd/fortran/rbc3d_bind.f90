@@ -37,7 +37,9 @@ module rbc3d_bind
        rbc3d_noslip_wall, rbc3d_solve_rbc_vel, rbc3d_pair_stats, rbc3d_set_precision, &
        rbc3d_mlp_create, rbc3d_mlp_destroy, rbc3d_mlp_set_params, rbc3d_mlp_set_norm, &
        rbc3d_noslip_wall_learned, rbc3d_set_gmres_restart, &
-       rbc3d_rbc_compute_geometry, rbc3d_shell_res_force, rbc3d_rbc_sph_project
+       rbc3d_rbc_compute_geometry, rbc3d_shell_res_force, rbc3d_rbc_sph_project, &
+       rbc3d_set_ref_cells, rbc3d_compute_rbc_vel, rbc3d_move_rbcs, rbc3d_rebox_rbcs, &
+       rbc3d_vol_constrain_rbcs, rbc3d_filter_rbcs, rbc3d_get_rbc_coords, rbc3d_timeint_euler
 
   ! rbc3d_set_precision modes
   integer(c_int), parameter, public :: RBC3D_PREC_FP64 = 0, RBC3D_PREC_SPLIT = 1
@@ -238,6 +240,53 @@ module rbc3d_bind
       real(c_double), intent(out) :: v(*)
       integer(c_int), intent(out) :: niter
       real(c_double), intent(out) :: rnorm
+    end function
+    integer(c_int) function rbc3d_set_ref_cells(ctx, ntypes, nlat, x, ES, ED, EB) &
+        bind(C, name="rbc3d_set_ref_cells")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: ntypes
+      integer(c_int), intent(in) :: nlat(*)
+      real(c_double), intent(in) :: x(*), ES(*), ED(*), EB(*)
+    end function
+    integer(c_int) function rbc3d_compute_rbc_vel(ctx, v, niter, rnorm) bind(C, name="rbc3d_compute_rbc_vel")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: v(*)
+      integer(c_int), intent(out) :: niter
+      real(c_double), intent(out) :: rnorm
+    end function
+    integer(c_int) function rbc3d_move_rbcs(ctx, Ts) bind(C, name="rbc3d_move_rbcs")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: Ts
+    end function
+    integer(c_int) function rbc3d_rebox_rbcs(ctx) bind(C, name="rbc3d_rebox_rbcs")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function rbc3d_vol_constrain_rbcs(ctx, eps_dist) bind(C, name="rbc3d_vol_constrain_rbcs")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: eps_dist
+    end function
+    integer(c_int) function rbc3d_filter_rbcs(ctx) bind(C, name="rbc3d_filter_rbcs")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: ctx
+    end function
+    integer(c_int) function rbc3d_get_rbc_coords(ctx, x) bind(C, name="rbc3d_get_rbc_coords")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: x(*)
+    end function
+    integer(c_int) function rbc3d_timeint_euler(ctx, Ts, nsteps, eps_dist, noslip_rtol, noslip_maxit, &
+        time, its_cell, its_wall) bind(C, name="rbc3d_timeint_euler")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: Ts, eps_dist, noslip_rtol
+      integer(c_int), value :: nsteps, noslip_maxit
+      real(c_double), intent(inout) :: time
+      integer(c_int), intent(out) :: its_cell(*), its_wall(*)
     end function
     function c_strlen(s) bind(C, name="strlen") result(n)
       import :: c_ptr, c_size_t

@@ -1,0 +1,117 @@
+"""GPU parity of the device time stepping (rbc3d_amd/csrc/timeint.hip, mech.hip) against
+the oracle's restatement of TimeInt_Euler (oracle/timeint_oracle.py): deformed cells
+relaxing over three Euler steps at viscosity ratio 5 (GMRES in every step), and one step
+of a tube with cells, the background flow and NoSlipWall.
+
+Bar: 1e-9 in the coordinates (cell size 1).  Velocities agree to the GMRES tolerance
+(1e-11 relative); the volume constraint rounds its factor to single precision, as the
+reference's `real :: fac` does, so a difference of 1e-12 in a cell volume can move one
+factor by one single-precision ulp (~6e-8 of |fac| <= epsDist/20): a few 1e-11 in x."""
+import numpy as np
+import pytest
+
+from oracle import oracle_py as O
+from oracle import timeint_oracle as T
+from rbc3d_amd import configs
+from tests.test_oracle_timeint import NLAT, deformed_cells, ref_cell, workload
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-9
+ES, ED, EB = 1.0, 20.0, 0.03
+
+
+def test_euler_steps_match_the_oracle():
+    import rbc3d_amd as R
+    rng = np.random.default_rng(7)
+    rots = [configs.random_rotation(rng) for _ in range(2)]
+    cells = deformed_cells([(2.5, 2.5, 2.5), (5.5, 5.6, 5.4)], seed=7, rots=rots)
+    w = workload(cells, 5.0)
+    ref = ref_cell()
+    Ts, nsteps, eps = 0.02, 3, 0.05
+    g = R.from_workload(w)
+    g.set_ref_cells([NLAT], ref_x(), [ES], [ED], [EB])
+    t, ic, iw = g.timeint_euler(Ts, nsteps, eps)
+    xg = g.rbc_coords()
+    xo, its = T.timeint_euler(O.from_workload(w), cells, [ref], [ES], [ED], [EB], Ts, nsteps, eps,
+                              w.Lb)
+    disp = np.abs(xo - cells.x).max()
+    err = np.abs(xg - xo).max()
+    print(f"3 steps: moved {disp:.3e}, GPU vs oracle {err:.2e}; iterations {list(ic)} vs "
+          f"{[i for i, _ in its]}", flush=True)
+    assert disp > 1e-3
+    assert err < TOL
+    assert t == pytest.approx(nsteps * Ts, abs=1e-15)
+    assert all(abs(a - b[0]) <= 1 for a, b in zip(ic, its))
+
+
+def ref_x():
+    from tests.test_mech_oracle import flat, grid, shape_fn
+    from tests.test_oracle_timeint import AXES
+    th, phi, _ = grid(NLAT)
+    return flat(shape_fn(0, 0.0, axes=AXES)(*np.meshgrid(th, phi, indexing="ij")))
+
+
+def test_pieces_match_the_oracle():
+    """rbc3d_compute_rbc_vel, rbc3d_move_rbcs, rbc3d_rebox_rbcs (cells moved out of the
+    box first), rbc3d_vol_constrain_rbcs and rbc3d_filter_rbcs one at a time."""
+    import rbc3d_amd as R
+    cells = deformed_cells([(2.5, 2.5, 2.5), (5.5, 5.6, 5.4)], seed=11)
+    w = workload(cells, 5.0)
+    ref = ref_cell()
+    g = R.from_workload(w)
+    g.set_ref_cells([NLAT], ref_x(), [ES], [ED], [EB])
+    v, ni, _ = g.compute_rbc_vel()
+    o = O.from_workload(w)
+    fs = [T.cell_force(x, NLAT, ref, ES, ED, EB)[1] for x in T.split_cells(cells.x, cells.nlat, 3)]
+    o.set_density(1, np.concatenate(fs))
+    vo, no, _ = o.solve_rbc_vel()
+    e_v = np.abs(v - vo).max() / np.abs(vo).max()
+    g.move_rbcs(0.5)
+    xs = T.split_cells(cells.x, cells.nlat, 3)
+    V = vo.reshape(3, -1)
+    npt = 2 * NLAT * NLAT
+    xs = [(x.reshape(3, -1) + 0.5 * V[:, c * npt:(c + 1) * npt]).ravel() for c, x in enumerate(xs)]
+    e_m = np.abs(g.rbc_coords() - np.concatenate(xs)).max()
+    g.rebox_rbcs()
+    g.vol_constrain_rbcs(0.05)
+    g.filter_rbcs()
+    xs = [T.rebox(x, w.Lb) for x in xs]
+    xs = [O.rbc_sph_project(T.vol_constrain(x, ref["vol"], 0.05), NLAT, NLAT) for x in xs]
+    e_x = np.abs(g.rbc_coords() - np.concatenate(xs)).max()
+    print(f"velocity {e_v:.2e} ({ni} vs {no} iterations), move {e_m:.2e}, "
+          f"rebox + volume + filter {e_x:.2e}", flush=True)
+    assert e_v < 1e-9 and e_m < TOL and e_x < TOL
+
+
+def test_rebox_is_exact():
+    import rbc3d_amd as R
+    cells = deformed_cells([(2.5, 2.5, 2.5), (5.5, 5.6, 5.4)], seed=12)
+    X = cells.x.reshape(2, 3, -1).copy()
+    X[0] += np.array([8.0, -16.0, 24.0])[:, None]
+    X[1] += np.array([0.0, 0.0, -8.0])[:, None]
+    moved = X.ravel()
+    import dataclasses
+    g = R.from_workload(workload(dataclasses.replace(cells, x=moved), 5.0))
+    g.rebox_rbcs()
+    want = np.concatenate([T.rebox(x, (8.0, 8.0, 8.0)) for x in T.split_cells(moved, cells.nlat, 3)])
+    assert np.array_equal(g.rbc_coords(), want)
+
+
+def test_tube_step_with_noslip_wall():
+    import rbc3d_amd as R
+    w = configs.make_workload("C3s", seed=2)
+    ref = ref_cell()
+    Ts, eps = 0.02, 0.05
+    # the discs relax toward the ellipsoid reference shape while the flow carries them
+    refx = ref_x()
+    g = R.from_workload(w)
+    g.set_ref_cells([NLAT], refx, [ES], [ED], [EB])
+    t, ic, iw = g.timeint_euler(Ts, 1, eps, noslip_rtol=1e-7, noslip_maxit=200)
+    o = O.from_workload(w)
+    xo, its = T.timeint_euler(o, w.cells, [ref], [ES], [ED], [EB], Ts, 1, eps, w.Lb,
+                              noslip=(1e-7, 200))
+    err = np.abs(g.rbc_coords() - xo).max()
+    e_f = np.abs(g.wall_density() - o.wall_density()).max() / np.abs(o.wall_density()).max()
+    print(f"tube step: coordinates {err:.2e}, wall traction {e_f:.2e}, iterations "
+          f"{ic[0]}/{iw[0]} vs {its[0]}", flush=True)
+    assert err < TOL and e_f < 1e-9 and iw[0] == its[0][1]
