@@ -177,10 +177,10 @@ int rbc3d_rbc_sph_project(int ncell, const int *nlat, const int *nlat0, int nvar
 int rbc3d_set_ref_cells(rbc3d_ctx *ctx, int ntypes, const int *nlat, const double *x,
                         const double *ES, const double *ED, const double *EB);
 /* Compute_Rbc_Vel (common/ModTimeInt.F90:716-808): RBC_ComputeGeometry and the surface
- * sources of every cell, the force density -Shell_ResForce projected by RBC_SphProject,
- * Solve_RBC_Vel.  The node velocities v(3, np) (as rbc3d_solve_rbc_vel returns them) stay
- * on the device for rbc3d_move_rbcs; v may be NULL.  The reference's intra-cell
- * repulsion (AddIntraCellRepulsionForce, :756) is not applied. */
+ * sources of every cell, the force density -Shell_ResForce (plus the intra-cell repulsion
+ * when rbc3d_set_repulsion turned it on) projected by RBC_SphProject, Solve_RBC_Vel.  The
+ * node velocities v(3, np) (as rbc3d_solve_rbc_vel returns them) stay on the device for
+ * rbc3d_move_rbcs; v may be NULL. */
 int rbc3d_compute_rbc_vel(rbc3d_ctx *ctx, double *v, int *niter, double *rnorm);
 /* rbc%x = rbc%x + Ts*rbc%v for every cell (common/ModTimeInt.F90:131-136), with the
  * velocities of the last rbc3d_compute_rbc_vel. */
@@ -189,19 +189,34 @@ int rbc3d_move_rbcs(rbc3d_ctx *ctx, double Ts);
  * centre of its coordinate range lies in [0, Lb). */
 int rbc3d_rebox_rbcs(rbc3d_ctx *ctx);
 /* VolConstrainRbcs (common/ModTimeInt.F90:938-963): x += fac a3 per cell, fac =
- * (vol_ref - vol)/area rounded to single precision (the reference's default REAL) and
- * limited to eps_dist/20 in magnitude. */
+ * (vol_ref - vol)/area limited to eps_dist/20 in magnitude (double precision: the
+ * reference's default REAL under its -freal-4-real-8 build). */
 int rbc3d_vol_constrain_rbcs(rbc3d_ctx *ctx, double eps_dist);
 /* FilterRbcs (common/ModTimeInt.F90:908-934): every cell's coordinates projected onto
  * spherical-harmonic modes below nlat0. */
 int rbc3d_filter_rbcs(rbc3d_ctx *ctx);
 /* The cells' coordinates, x(nlat, nlon, 3) per cell, concatenated (rbc3d_set_cells' layout). */
 int rbc3d_get_rbc_coords(rbc3d_ctx *ctx, double *x);
+/* The repulsion (common/ModRepulsion.F90, parameters ForceCoef, viscRatThresh and
+ * rigidsep of common/ModConf.F90:52-59).  When on, rbc3d_compute_rbc_vel adds
+ * AddIntraCellRepulsionForce (:126-266) to the membrane force density before its
+ * projection, and rbc3d_timeint_euler runs InterCellRepulsion after every
+ * VolConstrainRbcs.  Off by default. */
+int rbc3d_set_repulsion(rbc3d_ctx *ctx, int on, double force_coef, double visc_rat_thresh,
+                        int rigidsep);
+/* AddIntraCellRepulsionForce on the cells' current coordinates: df(nlat, nlon, 3) per cell,
+ * concatenated (may be NULL), and the number of nodes that receive a force. */
+int rbc3d_intra_cell_repulsion(rbc3d_ctx *ctx, double *df, int *npoints);
+/* InterCellRepulsion (common/ModRepulsion.F90:270-402, with Closest_Neighbor_Cell :480-546
+ * and Closest_Neighbor_Wall :556-613) at epsDist = eps_dist: the cells' coordinates are
+ * moved in place.  npoints: nodes displaced; dist_min: the smallest node-surface
+ * separation below 2 eps_dist (infinity when none).  Optional outputs. */
+int rbc3d_inter_cell_repulsion(rbc3d_ctx *ctx, double eps_dist, int *npoints, double *dist_min);
 /* TimeInt_Euler (common/ModTimeInt.F90:109-178), nsteps steps of Ts: Compute_Rbc_Vel,
  * NoSlipWall when a wall is set (rtol, maxit as rbc3d_noslip_wall), x += Ts v,
- * ReboxRbcs, four rounds of VolConstrainRbcs + FilterRbcs (the reference's
- * InterCellRepulsion between them is not applied); *time += Ts per step.  its_cell,
- * its_wall (length nsteps, optional): the GMRES iterations of each step's two solves. */
+ * ReboxRbcs, four rounds of VolConstrainRbcs, InterCellRepulsion (when the repulsion is
+ * on, at eps_dist) and FilterRbcs; *time += Ts per step.  its_cell, its_wall (length
+ * nsteps, optional): the GMRES iterations of each step's two solves. */
 int rbc3d_timeint_euler(rbc3d_ctx *ctx, double Ts, int nsteps, double eps_dist,
                         double noslip_rtol, int noslip_maxit, double *time, int *its_cell,
                         int *its_wall);

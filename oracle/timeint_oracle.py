@@ -11,8 +11,8 @@ One step, in the reference's order:
   NoSlipWall (ModNoSlip:45)   when the oracle holds a wall
   rbc%x += Ts rbc%v (:131-136)
   ReboxRbcs (:1006-1020)
-  4 x {VolConstrainRbcs (:938-963), FilterRbcs (:908-934)}; the reference's
-  InterCellRepulsion between them is not part of this restatement.
+  4 x {VolConstrainRbcs (:938-963), InterCellRepulsion (common/ModRepulsion.F90:270-402,
+  when the repulsion is on), FilterRbcs (:908-934)}
 """
 from __future__ import annotations
 
@@ -21,6 +21,7 @@ import dataclasses
 import numpy as np
 
 from oracle import oracle_py as O
+from oracle import repulsion_oracle as RP
 
 
 def split_cells(v, nlat, ncomp):
@@ -42,25 +43,35 @@ def rebox(x, Lb):
 
 
 def vol_constrain(x, ref_vol, eps_dist):
-    """VolConstrainRbcs of one cell: fac is the reference's default (single precision)
-    REAL, fac = SIGN(MIN(ABS(fac), epsDist/20.), fac)."""
+    """VolConstrainRbcs of one cell: fac = SIGN(MIN(ABS(fac), epsDist/20.), fac).  fac is a
+    default REAL in the reference, double precision under its -freal-4-real-8 build
+    (CMakeLists.txt:61, Makefile.in:49)."""
     g = O.rbc_geometry(x)
-    fac = np.float32((ref_vol - g["vol"]) / g["area"])
-    fac = np.float32(np.copysign(min(abs(float(fac)), eps_dist / 20.0), float(fac)))
-    return x + float(fac) * g["a3"]
+    fac = (ref_vol - g["vol"]) / g["area"]
+    fac = float(np.copysign(min(abs(fac), eps_dist / 20.0), fac))
+    return x + fac * g["a3"]
 
 
-def cell_force(x, nlat0, ref, ES, ED, EB):
-    """rbc%f = -Shell_ResForce(rbc, rbcRef), then RBC_SphProject(rbc, 3, rbc%f)."""
+def cell_force(x, nlat0, ref, ES, ED, EB, force_coef=0.0):
+    """rbc%f = -Shell_ResForce(rbc, rbcRef), AddIntraCellRepulsionForce (force_coef != 0),
+    then RBC_SphProject(rbc, 3, rbc%f)."""
     g = O.rbc_geometry(x)
-    return g, -O.rbc_sph_project(O.shell_res_force(g, ref, ES, ED, EB), g["nlat"], nlat0)
+    f = O.shell_res_force(g, ref, ES, ED, EB)
+    if force_coef == 0:
+        return g, -O.rbc_sph_project(f, g["nlat"], nlat0)
+    df, _ = RP.intra_force(x, g["a3"], g["area"], g["nlat"], force_coef)
+    return g, O.rbc_sph_project(-f + df, g["nlat"], nlat0)
 
 
-def timeint_euler(o, cells, refs, ES, ED, EB, Ts, nsteps, eps_dist, Lb, noslip=None):
+def timeint_euler(o, cells, refs, ES, ED, EB, Ts, nsteps, eps_dist, Lb, noslip=None,
+                  repulsion=None, viscRat=None, wall=None):
     """nsteps Euler steps of the cells (a configs.Cells) on the oracle `o` (built with
     the same configuration); refs[k]: rbc_geometry of cell type k+1's reference cell.
-    noslip: None, or (rtol, maxit) of NoSlipWall.  Returns the final coordinates in the
-    cells' layout and the per-step (cell, wall) GMRES iterations."""
+    noslip: None, or (rtol, maxit) of NoSlipWall.  repulsion: None, or (ForceCoef,
+    viscRatThresh, rigidsep), with viscRat (per type) and wall (the oracle's wall tuple,
+    or None) for InterCellRepulsion.  Returns the final coordinates in the cells' layout
+    and the per-step (cell, wall) GMRES iterations."""
+    coef = repulsion[0] if repulsion is not None else 0.0
     nlat, nlat0, ct = cells.nlat, cells.nlat0, cells.celltype
     xs = split_cells(cells.x, nlat, 3)
     its = []
@@ -68,7 +79,7 @@ def timeint_euler(o, cells, refs, ES, ED, EB, Ts, nsteps, eps_dist, Lb, noslip=N
         geos, fs = [], []
         for c, x in enumerate(xs):
             k = ct[c] - 1
-            g, f = cell_force(x, int(nlat0[c]), refs[k], ES[k], ED[k], EB[k])
+            g, f = cell_force(x, int(nlat0[c]), refs[k], ES[k], ED[k], EB[k], coef)
             geos.append(g)
             fs.append(f)
         cur = dataclasses.replace(cells, x=np.concatenate(xs),
@@ -91,5 +102,9 @@ def timeint_euler(o, cells, refs, ES, ED, EB, Ts, nsteps, eps_dist, Lb, noslip=N
         for _ in range(4):
             for c in range(len(xs)):
                 xs[c] = vol_constrain(xs[c], refs[ct[c] - 1]["vol"], eps_dist)
+            if repulsion is not None:
+                xs, _, _ = RP.inter_repulsion(o, xs, nlat, ct, viscRat, eps_dist, Lb, repulsion[1],
+                                              repulsion[2], wall=wall)
+            for c in range(len(xs)):
                 xs[c] = O.rbc_sph_project(xs[c], int(nlat[c]), int(nlat0[c]))
     return np.concatenate(xs), its

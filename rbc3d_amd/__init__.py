@@ -113,6 +113,10 @@ _SIG = {
     "rbc3d_vol_constrain_rbcs": [C.c_void_p, C.c_double],
     "rbc3d_filter_rbcs": [C.c_void_p],
     "rbc3d_get_rbc_coords": [C.c_void_p, dp],
+    "rbc3d_set_repulsion": [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int],
+    "rbc3d_intra_cell_repulsion": [C.c_void_p, dp, C.POINTER(C.c_int)],
+    "rbc3d_inter_cell_repulsion": [C.c_void_p, C.c_double, C.POINTER(C.c_int),
+                                   C.POINTER(C.c_double)],
     "rbc3d_timeint_euler": [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_double, C.c_int,
                             C.POINTER(C.c_double), C.c_void_p, C.c_void_p],
     "rbc3d_timing": [C.c_void_p, C.c_int],
@@ -583,6 +587,25 @@ class System:
 
     def filter_rbcs(self):
         _call("rbc3d_filter_rbcs", self.h)
+
+    def set_repulsion(self, on, force_coef=0.0, visc_rat_thresh=0.0, rigidsep=False):
+        _call("rbc3d_set_repulsion", self.h, int(bool(on)), force_coef, visc_rat_thresh,
+              int(bool(rigidsep)))
+
+    def intra_cell_repulsion(self):
+        """AddIntraCellRepulsionForce on the current coordinates: (df per-cell layout,
+        number of nodes with a force)."""
+        df = np.zeros(3 * self.np)
+        n = C.c_int()
+        _call("rbc3d_intra_cell_repulsion", self.h, df, C.byref(n))
+        return df, n.value
+
+    def inter_cell_repulsion(self, eps_dist):
+        """InterCellRepulsion in place: (nodes moved, smallest separation below 2 eps)."""
+        n = C.c_int()
+        d = C.c_double()
+        _call("rbc3d_inter_cell_repulsion", self.h, eps_dist, C.byref(n), C.byref(d))
+        return n.value, d.value
 
     def rbc_coords(self):
         """The cells' coordinates in rbc3d_set_cells' per-cell layout."""

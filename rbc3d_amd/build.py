@@ -14,7 +14,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "librbc3d_gpu.so")
 OBJ = os.path.join(HERE, "_obj")
-SOURCES = ["ctx.hip", "sh.hip", "pair.hip", "pme.hip", "wall.hip", "gmres.hip", "mlp.hip", "mech.hip", "timeint.hip"]
+SOURCES = ["ctx.hip", "sh.hip", "pair.hip", "pme.hip", "wall.hip", "gmres.hip", "mlp.hip", "mech.hip", "timeint.hip", "repulsion.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-fPIC", "-std=c++17",
          "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -309,6 +309,15 @@ struct rbc3d_ctx {
   bool vel_ready = false;
   int mech_epoch = -1;  // geom_epoch the per-cell tables above were built for
   bool rbc_geom_ready = false;  // the block geometry above is that of d_x
+  // repulsion (rbc3d_set_repulsion, repulsion.hip): ForceCoef of the intra-cell force,
+  // viscRatThresh and rigidsep of the inter-cell displacement; per-cell bounding spheres,
+  // candidate neighbour cells and wall triangles (capacities grown on overflow), the
+  // displacement of every node, and (moved count, smallest separation)
+  bool rep_on = false, rigidsep = false;
+  double force_coef = 0, visc_rat_thresh = 0;
+  int rep_cap_c = 32, rep_cap_e = 256;
+  DBuf<double> d_rep_sph, d_rep_dx, d_rep_stat;
+  DBuf<int> d_rep_cnt, d_rep_cand_c, d_rep_cand_e;
 };
 
 // ---------------------------------------------------------------------------
@@ -528,6 +537,16 @@ int rbc_sh_anal_batch(int nlat, int nlon, int nt, const double *d_g, int idg, in
 int rbc_sh_synth_batch(int nlat, int nlon, int nt, double *d_g, int idg, int jdg, const double *d_a,
                        const double *d_b, int ida, int jda, int equi, hipStream_t st);
 int rbc_build_source_list(rbc3d_ctx *c);
+// device arrays of per-grid-type table pointers, indexed by CellRec::gtype (pair.hip):
+// same-surface mask, grid colatitudes and longitudes, patch points and weights, projection
+// tables (patch directions and least-squares weights)
+struct TypePtrs {
+  const double *const *mask, *const *th, *const *phi, *const *thG, *const *phiG, *const *pwt,
+      *const *proj;
+  const float *const *maskf;
+  const int *nrad, *nazm;
+};
+int rbc_type_ptrs(rbc3d_ctx *c, TypePtrs *tp);
 int rbc_update_density(rbc3d_ctx *c, int which);
 // per-cell mechanics on the context's cells (mech.hip): RBC_ComputeGeometry of every
 // cell from d_x (refreshing a3, detJ, dS, the cell records, splines and source lists),
@@ -535,6 +554,10 @@ int rbc_update_density(rbc3d_ctx *c, int which);
 int rbc_cells_geometry(rbc3d_ctx *c, bool sources);
 int rbc_cells_force(rbc3d_ctx *c);
 int rbc_cells_project_x(rbc3d_ctx *c);
+// AddIntraCellRepulsionForce on the block geometry (repulsion.hip): mode 0 writes df to
+// out, mode 1 turns the block force out = f into -f + df; count (optional, device) is
+// increased by the number of nodes that received a force
+int rbc_intra_repulsion(rbc3d_ctx *c, int mode, double *out, int *count);
 // deflation corrections (gmres.hip); no-ops unless rbc3d_set_deflation is on
 int rbc_defl_rhs(rbc3d_ctx *c, double *d_v);
 int rbc_defl_matmult(rbc3d_ctx *c, const double *d_g, double *d_v);

@@ -836,8 +836,9 @@ int rbc_cells_geometry(rbc3d_ctx *c, bool sources) {
   return RBC3D_OK;
 }
 
-// rbc%f = -Shell_ResForce(rbc, rbcRef), RBC_SphProject(rbc, 3, rbc%f) and the f spline and
-// source densities (common/ModTimeInt.F90:747-769, without the intra-cell repulsion)
+// rbc%f = -Shell_ResForce(rbc, rbcRef), AddIntraCellRepulsionForce (when the repulsion is
+// on), RBC_SphProject(rbc, 3, rbc%f) and the f spline and source densities
+// (common/ModTimeInt.F90:747-769)
 int rbc_cells_force(rbc3d_ctx *c) {
   if (!c->nrbc) return RBC3D_OK;
   if (!c->rbc_geom_ready) {
@@ -890,10 +891,15 @@ int rbc_cells_force(rbc3d_ctx *c) {
   o.scr = c->d_mscr.p;
   hipLaunchKernelGGL(k_shell_res_force, dim3(mech_blocks(c->nrbc)), dim3(MECH_T), cb.lds, st, c->nrbc,
                      cb.G, o);
+  // rbc%f = -rbc%f, AddIntraCellRepulsionForce, then the projection; without the
+  // repulsion force the sign is applied after the (linear) projection instead
+  const bool intra = c->rep_on && c->force_coef != 0;
+  if (intra) RBC_TRY(rbc_intra_repulsion(c, 1, c->d_mf.p, nullptr));
   hipLaunchKernelGGL(k_rbc_sph_project, dim3(mech_blocks(c->nrbc)), dim3(MECH_T), cb.lds, st, c->nrbc,
                      c->d_moff.p, c->d_mnlat.p, c->d_mnlat0.p, 3, cb.G, c->d_mf.p);
   const int nb = (np + 255) / 256;
-  k_blocks_to_nodes<<<nb, 256, 0, st>>>(np, c->d_node_cell.p, c->d_cells.p, c->d_mf.p, 3, -1.0, c->d_f.p);
+  k_blocks_to_nodes<<<nb, 256, 0, st>>>(np, c->d_node_cell.p, c->d_cells.p, c->d_mf.p, 3,
+                                        intra ? 1.0 : -1.0, c->d_f.p);
   HIP_TRY(hipGetLastError());
   RBC_TRY(rbc_update_density(c, 1));
   return RBC3D_OK;

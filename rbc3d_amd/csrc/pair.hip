@@ -19,6 +19,7 @@
 #include <mutex>
 
 #include "internal.h"
+#include "surf_core.h"
 
 #define NBR_T 256
 
@@ -1389,23 +1390,6 @@ __device__ __forceinline__ void patch_point(double s0, double c0, double phi0, d
   patch_point_sc(s0, c0, phi0, sin(thL), cos(thL), sin(phL), cos(phL), thG, phG);
 }
 
-// PolarPatch_Map (common/ModPolarPatch.F90:218-242)
-__device__ void patch_map(double th0, double phi0, double dth, double dphi, double &th, double &phi) {
-  double ct0 = cos(th0), st0 = sin(th0), cp0 = cos(phi0), sp0 = sin(phi0);
-  double s1[3] = {ct0 * cp0, ct0 * sp0, -st0};
-  double s2[3] = {-sp0, cp0, 0.};
-  double x0[3] = {st0 * cp0, st0 * sp0, ct0};
-  double xl[3] = {sin(dth) * cos(dphi), sin(dth) * sin(dphi), cos(dth)};
-  double x[3];
-  for (int d = 0; d < 3; d++) x[d] = xl[0] * s1[d] + xl[1] * s2[d] + xl[2] * x0[d];
-  double nrm = sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
-  for (int d = 0; d < 3; d++) x[d] = x[d] / nrm;
-  x[2] = fmax(-1., fmin(1., x[2]));
-  th = acos(x[2]);
-  phi = atan2(x[1], x[0]);
-  if (phi < 0) phi += RBC_TWO_PI;
-}
-
 // GauLeg (common/ModQuadRule.F90:135-208), node k (ascending) and its weight: each root's Newton iteration is
 // independent of the others, so one lane computes one node, as GauLeg does for it.
 __device__ void gauleg_node(double x1, double x2, int n, int k, double &x, double &w) {
@@ -1567,60 +1551,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) k
   const size_t vs = (size_t)m * n * 4;
   const double *sxp = blk + vs * SPL_X;
   const double *pj = a.proj[gt];
-  // Spline_FindProjection (common/ModSpline.F90:203-269)
-  spline_interp(sxp, m, n, th0, phi0, 3, x0);
-#pragma nounroll
-  for (int iter = 0; iter < 3; iter++) {
-    const double s0 = sin(th0), c0 = cos(th0);
-    double xx[3];
-    for (int d = 0; d < 3; d++) xx[d] = x0[d] - xt[d];
-    const double d20 = xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2];
-    const double *W = pj + RBC_PJ_W + iter * 17 * 6;
-    double c1 = W[1] * d20, c2 = W[2] * d20, c3 = W[3] * d20, c4 = W[4] * d20, c5 = W[5] * d20;
-#pragma nounroll
-    for (int k = 0; k < 16; k++) {
-      // PolarPatch_Build (common/ModPolarPatch.F90:99-148) for patch point k
-      const double *dr = pj + RBC_PJ_DIR + (iter * 16 + k) * 3;
-      double y0, y1, y2;
-      {
-#pragma clang fp contract(off)
-        y0 = c0 * dr[0] + 0. * dr[1] + s0 * dr[2];
-        y1 = 0. * dr[0] + 1. * dr[1] + 0. * dr[2];
-        y2 = -s0 * dr[0] + 0. * dr[1] + c0 * dr[2];
-      }
-      y2 = fmax(-1.0, fmin(1.0, y2));
-      double ph = atan2(y1, y0) + phi0;
-      const double thP = acos(y2);
-      const double phP = ph - floor(ph * (1. / RBC_TWO_PI)) * RBC_TWO_PI;
-      spline_interp(sxp, m, n, thP, phP, 3, xx);
-      for (int d = 0; d < 3; d++) xx[d] -= xt[d];
-      const double fk = xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2];
-      const double *w = W + (k + 1) * 6;
-      c1 += w[1] * fk;
-      c2 += w[2] * fk;
-      c3 += w[3] * fk;
-      c4 += w[4] * fk;
-      c5 += w[5] * fk;
-    }
-    // Min_Quad_2D (common/ModBasicMath.F90:296-328)
-    double l11 = 2. * c3, l12 = c4, l21 = c4, l22 = 2. * c5;
-    double det = l11 * l22 - l12 * l21, xm0 = 0., xm1 = 0.;
-    if (det > 0) {
-      double idet = 1. / det, r1 = -c1, r2 = -c2;
-      xm0 = idet * (l22 * r1 - l12 * r2);
-      xm1 = idet * (-l21 * r1 + l11 * r2);
-    }
-    double thMinL = sqrt(xm0 * xm0 + xm1 * xm1), phiMinL = atan2(xm1, xm0);
-    double thMin, phiMin, xMin[3];
-    patch_map(th0, phi0, thMinL, phiMinL, thMin, phiMin);
-    spline_interp(sxp, m, n, thMin, phiMin, 3, xMin);
-    for (int d = 0; d < 3; d++) xx[d] = xMin[d] - xt[d];
-    double d2Min = xx[0] * xx[0] + xx[1] * xx[1] + xx[2] * xx[2];
-    if (d2Min > d20) break;
-    th0 = thMin;
-    phi0 = phiMin;
-    for (int d = 0; d < 3; d++) x0[d] = xMin[d];
-  }
+  spline_find_projection(sxp, m, n, pj, xt, th0, phi0, x0);
   double a30[3];
   spline_interp(blk + vs * SPL_A3, m, n, th0, phi0, 3, a30);
   double dist = a30[0] * (xt[0] - x0[0]) + a30[1] * (xt[1] - x0[1]) + a30[2] * (xt[2] - x0[2]);
@@ -1905,13 +1836,7 @@ __global__ void k_combine(CombineArgs a) {
 // ---------------------------------------------------------------------------
 // table pointer arrays per grid type, owned by the context (rebuilt when a grid
 // type is added)
-struct TypePtrs {
-  const double *const *mask, *const *th, *const *phi, *const *thG, *const *phiG, *const *pwt,
-      *const *proj;
-  const float *const *maskf;
-  const int *nrad, *nazm;
-};
-static int type_ptrs(rbc3d_ctx *c, TypePtrs *tp) {
+int rbc_type_ptrs(rbc3d_ctx *c, TypePtrs *tp) {
   size_t nt = c->gtypes.size();
   if (c->tp_ntypes != nt) {
     std::vector<const double *> ptr(7 * nt);
@@ -2050,7 +1975,7 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   hipStream_t st = c->stream;
   int nb = c->nfbins;
   TypePtrs tpv, *tp = &tpv;
-  RBC_TRY(type_ptrs(c, tp));
+  RBC_TRY(rbc_type_ptrs(c, tp));
   cacheable = cacheable && c->geom_reuse;
   const bool hit = cacheable && c->cache_epoch == c->geom_epoch && c->cache_nT == nT;
   c->cache_epoch = -1;  // valid again only when this call completes as cacheable

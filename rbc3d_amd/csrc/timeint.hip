@@ -52,14 +52,14 @@ __global__ void __launch_bounds__(256) k_rebox(int nrbc, const CellRec *__restri
 }
 
 // VolConstrainRbcs (common/ModTimeInt.F90:938-963): x += fac a3 with
-// fac = (vol_ref - vol) / area held in a default (single precision) REAL, as the
-// reference declares it, and limited to epsDist/20
+// fac = (vol_ref - vol) / area limited to epsDist/20 in magnitude.  The reference declares
+// fac a default REAL and is built with -freal-4-real-8: it is double precision.
 __global__ void k_vol_constrain(int np, const int *__restrict__ node_cell,
-                                const float *__restrict__ fac, const double *__restrict__ a3,
+                                const double *__restrict__ fac, const double *__restrict__ a3,
                                 double *x) {
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= np) return;
-  const double f = (double)fac[node_cell[p]];
+  const double f = fac[node_cell[p]];
   for (int d = 0; d < 3; d++) x[p + (size_t)np * d] = x[p + (size_t)np * d] + f * a3[p + (size_t)np * d];
 }
 
@@ -72,9 +72,9 @@ static int require_cells(rbc3d_ctx *c) {
   return RBC3D_OK;
 }
 
-// Compute_Rbc_Vel (common/ModTimeInt.F90:716-808) without the intra-cell repulsion: the
-// cells' geometry, -Shell_ResForce projected as the force density, Solve_RBC_Vel; the
-// node velocities stay on the device for rbc3d_move_rbcs (and go to v when given)
+// Compute_Rbc_Vel (common/ModTimeInt.F90:716-808): the cells' geometry, -Shell_ResForce
+// plus the intra-cell repulsion (when on) projected as the force density, Solve_RBC_Vel;
+// the node velocities stay on the device for rbc3d_move_rbcs (and go to v when given)
 extern "C" int rbc3d_compute_rbc_vel(rbc3d_ctx *c, double *v, int *niter, double *rnorm) {
   RBC_TRY(require_cells(c));
   if (!niter || !rnorm) return RBC3D_EINVAL;
@@ -125,20 +125,19 @@ extern "C" int rbc3d_vol_constrain_rbcs(rbc3d_ctx *c, double eps_dist) {
   std::vector<double> vol(c->nrbc), area(c->nrbc);
   HIP_TRY(hipMemcpy(vol.data(), c->d_mvol.p, sizeof(double) * c->nrbc, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(area.data(), c->d_marea.p, sizeof(double) * c->nrbc, hipMemcpyDeviceToHost));
-  std::vector<float> fac(c->nrbc);
+  std::vector<double> fac(c->nrbc);
   for (int i = 0; i < c->nrbc; i++) {
     const int k = c->cells[i].celltype - 1;
     if (k >= c->nref) {
       rbc_set_error("rbc3d_vol_constrain_rbcs: no reference cell for cell type " + std::to_string(k + 1));
       return RBC3D_EINVAL;
     }
-    float f = (float)((c->ref_vol[k] - vol[i]) / area[i]);
-    f = (float)std::copysign(std::fmin(std::fabs((double)f), eps_dist / 20.), (double)f);
-    fac[i] = f;
+    const double f = (c->ref_vol[k] - vol[i]) / area[i];
+    fac[i] = std::copysign(std::fmin(std::fabs(f), eps_dist / 20.), f);
   }
-  DBuf<float> d_fac;
+  DBuf<double> d_fac;
   RBC_TRY(d_fac.ensure(c->nrbc));
-  HIP_TRY(hipMemcpy(d_fac.p, fac.data(), sizeof(float) * c->nrbc, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(d_fac.p, fac.data(), sizeof(double) * c->nrbc, hipMemcpyHostToDevice));
   k_vol_constrain<<<(c->np + 255) / 256, 256, 0, c->stream>>>(c->np, c->d_node_cell.p, d_fac.p,
                                                                c->d_a3.p, c->d_x.p);
   HIP_TRY(hipGetLastError());
@@ -171,9 +170,9 @@ extern "C" int rbc3d_get_rbc_coords(rbc3d_ctx *c, double *x) {
 
 // TimeInt_Euler (common/ModTimeInt.F90:109-178) for nsteps steps: Compute_Rbc_Vel,
 // NoSlipWall (when a wall is set), x += Ts v, ReboxRbcs, then four rounds of
-// VolConstrainRbcs and FilterRbcs (the reference's InterCellRepulsion between them is
-// not part of this library).  time advances by Ts per step; per step the cell and wall
-// GMRES iteration counts go to its_cell / its_wall when given (length nsteps).
+// VolConstrainRbcs, InterCellRepulsion (when the repulsion is on) and FilterRbcs.  time
+// advances by Ts per step; per step the cell and wall GMRES iteration counts go to
+// its_cell / its_wall when given (length nsteps).
 extern "C" int rbc3d_timeint_euler(rbc3d_ctx *c, double Ts, int nsteps, double eps_dist,
                                    double noslip_rtol, int noslip_maxit, double *time,
                                    int *its_cell, int *its_wall) {
@@ -191,6 +190,7 @@ extern "C" int rbc3d_timeint_euler(rbc3d_ctx *c, double Ts, int nsteps, double e
     RBC_TRY(rbc3d_rebox_rbcs(c));
     for (int k = 0; k < 4; k++) {
       RBC_TRY(rbc3d_vol_constrain_rbcs(c, eps_dist));
+      if (c->rep_on) RBC_TRY(rbc3d_inter_cell_repulsion(c, eps_dist, nullptr, nullptr));
       RBC_TRY(rbc3d_filter_rbcs(c));
     }
     *time += Ts;

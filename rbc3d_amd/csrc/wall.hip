@@ -14,6 +14,7 @@
 #include <cmath>
 
 #include "internal.h"
+#include "surf_core.h"
 
 struct TriRules {
   double r7[7][2], w7[7];  // gqTri7 (common/ModQuadRule.F90:70-94)
@@ -69,68 +70,6 @@ static TriRules make_rules() {
   t.w7[6] = 9. / 80.;
   gauleg_h(0., 1., 4, t.rd, t.wd);
   return t;
-}
-
-// MinDistToTri (common/ModIntOnWalls.F90:480-577); x[l][d].  Evaluated without FMA
-// contraction, operation for operation as the reference, so the in-cutoff test (and with
-// it the self-matrix sparsity pattern) is the reference's bit for bit.
-__device__ double min_dist_to_tri(const double xt[3], const double x[3][3], double &s0, double &t0) {
-#pragma clang fp contract(off)
-  double x12[3], x13[3], x1T[3];
-  for (int d = 0; d < 3; d++) {
-    x12[d] = x[1][d] - x[0][d];
-    x13[d] = x[2][d] - x[0][d];
-    x1T[d] = x[0][d] - xt[d];
-  }
-  double a = x12[0] * x12[0] + x12[1] * x12[1] + x12[2] * x12[2];
-  double b = x12[0] * x13[0] + x12[1] * x13[1] + x12[2] * x13[2];
-  double c = x13[0] * x13[0] + x13[1] * x13[1] + x13[2] * x13[2];
-  double det = a * c - b * b, invDet = 1. / det;
-  double d = x12[0] * x1T[0] + x12[1] * x1T[1] + x12[2] * x1T[2];
-  double e = x13[0] * x1T[0] + x13[1] * x1T[1] + x13[2] * x1T[2];
-  double f = x1T[0] * x1T[0] + x1T[1] * x1T[1] + x1T[2] * x1T[2];
-  double s = b * e - c * d, t = b * d - a * e;
-  int reg;
-  if (s + t <= det) {
-    if (s < 0)
-      reg = t < 0 ? 4 : 3;
-    else if (t < 0)
-      reg = 5;
-    else
-      reg = 0;
-  } else {
-    if (s < 0)
-      reg = 2;
-    else if (t < 0)
-      reg = 6;
-    else
-      reg = 1;
-  }
-  if (reg == 2)
-    reg = (-(c + e) < 0) ? 3 : 1;
-  else if (reg == 4)
-    reg = (d < 0) ? 5 : 3;
-  else if (reg == 6)
-    reg = (b + e - a - d < 0) ? 1 : 5;
-  if (reg == 0) {
-    s = invDet * s;
-    t = invDet * t;
-  } else if (reg == 1) {
-    s = (c + e - b - d) / (a - 2 * b + c);
-    s = fmin(1., fmax(0., s));
-    t = 1. - s;
-  } else if (reg == 3) {
-    s = 0.;
-    t = -e / c;
-    t = fmin(1., fmax(0., t));
-  } else if (reg == 5) {
-    t = 0.;
-    s = -d / a;
-    s = fmin(1., fmax(0., s));
-  }
-  s0 = s;
-  t0 = t;
-  return sqrt(a * s * s + 2 * b * s * t + c * t * t + 2 * d * s + 2 * e * t + f);
 }
 
 struct EwT {

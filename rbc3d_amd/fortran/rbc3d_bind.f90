@@ -39,7 +39,8 @@ module rbc3d_bind
        rbc3d_noslip_wall_learned, rbc3d_set_gmres_restart, &
        rbc3d_rbc_compute_geometry, rbc3d_shell_res_force, rbc3d_rbc_sph_project, &
        rbc3d_set_ref_cells, rbc3d_compute_rbc_vel, rbc3d_move_rbcs, rbc3d_rebox_rbcs, &
-       rbc3d_vol_constrain_rbcs, rbc3d_filter_rbcs, rbc3d_get_rbc_coords, rbc3d_timeint_euler
+       rbc3d_vol_constrain_rbcs, rbc3d_filter_rbcs, rbc3d_get_rbc_coords, rbc3d_timeint_euler, &
+       rbc3d_set_repulsion, rbc3d_intra_cell_repulsion, rbc3d_inter_cell_repulsion
 
   ! rbc3d_set_precision modes
   integer(c_int), parameter, public :: RBC3D_PREC_FP64 = 0, RBC3D_PREC_SPLIT = 1
@@ -287,6 +288,28 @@ module rbc3d_bind
       integer(c_int), value :: nsteps, noslip_maxit
       real(c_double), intent(inout) :: time
       integer(c_int), intent(out) :: its_cell(*), its_wall(*)
+    end function
+    integer(c_int) function rbc3d_set_repulsion(ctx, on, force_coef, visc_rat_thresh, rigidsep) &
+        bind(C, name="rbc3d_set_repulsion")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      integer(c_int), value :: on, rigidsep
+      real(c_double), value :: force_coef, visc_rat_thresh
+    end function
+    integer(c_int) function rbc3d_intra_cell_repulsion(ctx, df, npoints) &
+        bind(C, name="rbc3d_intra_cell_repulsion")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), intent(out) :: df(*)
+      integer(c_int), intent(out) :: npoints
+    end function
+    integer(c_int) function rbc3d_inter_cell_repulsion(ctx, eps_dist, npoints, dist_min) &
+        bind(C, name="rbc3d_inter_cell_repulsion")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: eps_dist
+      integer(c_int), intent(out) :: npoints
+      real(c_double), intent(out) :: dist_min
     end function
     function c_strlen(s) bind(C, name="strlen") result(n)
       import :: c_ptr, c_size_t

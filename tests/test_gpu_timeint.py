@@ -3,10 +3,8 @@ the oracle's restatement of TimeInt_Euler (oracle/timeint_oracle.py): deformed c
 relaxing over three Euler steps at viscosity ratio 5 (GMRES in every step), and one step
 of a tube with cells, the background flow and NoSlipWall.
 
-Bar: 1e-9 in the coordinates (cell size 1).  Velocities agree to the GMRES tolerance
-(1e-11 relative); the volume constraint rounds its factor to single precision, as the
-reference's `real :: fac` does, so a difference of 1e-12 in a cell volume can move one
-factor by one single-precision ulp (~6e-8 of |fac| <= epsDist/20): a few 1e-11 in x."""
+Bar: 1e-12 in the coordinates (cell size 1): the same arithmetic up to summation order,
+with velocities from GMRES solves that take the same iterations."""
 import numpy as np
 import pytest
 
@@ -16,7 +14,7 @@ from rbc3d_amd import configs
 from tests.test_oracle_timeint import NLAT, deformed_cells, ref_cell, workload
 
 pytestmark = pytest.mark.gpu
-TOL = 1e-9
+TOL = 1e-12
 ES, ED, EB = 1.0, 20.0, 0.03
 
 

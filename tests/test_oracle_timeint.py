@@ -9,7 +9,7 @@ common/ModTimeInt.F90:109-178):
 - a cell of its reference shape, rotated and moved, is at rest (frame indifference).
 - ReboxRbcs moves a cell shifted by whole periods back, by whole periods.
 - VolConstrainRbcs moves the volume toward the reference volume, to second order, and
-  by exactly epsDist/20 (single precision) along the normals when the limit applies.
+  by exactly epsDist/20 along the normals when the limit applies.
 """
 import dataclasses
 
@@ -134,8 +134,8 @@ def test_vol_constrain():
     print(f"volume error {v0:.3e} -> {v1:.3e}")
     # first order removed: what is left is the curvature term, ~ H fac ~ 4e-3 of it
     assert v0 > 0 and abs(v1) < 1e-2 * abs(v0)
-    # the limit: |fac| = epsDist/20 in single precision, along -a3 for an inflated cell
+    # the limit: |fac| = epsDist/20, along -a3 for an inflated cell
     g = O.rbc_geometry(x)
     eps = 0.01
     d = (T.vol_constrain(x, ref["vol"], eps) - x).reshape(3, -1)
-    assert np.allclose(d, -float(np.float32(eps / 20)) * g["a3"].reshape(3, -1), rtol=0, atol=1e-14)
+    assert np.allclose(d, -(eps / 20) * g["a3"].reshape(3, -1), rtol=0, atol=1e-14)
