@@ -25,7 +25,8 @@ enum {
   RBC3D_ENBR = 3,        /* more than 32 neighbouring cells within rc of a target */
   RBC3D_ESTATE = 4,      /* call out of order (e.g. no cells set) */
   RBC3D_ENONFINITE = 5,  /* non-finite input */
-  RBC3D_ENOCONV = 6      /* GMRES did not reach the tolerance within maxit */
+  RBC3D_ENOCONV = 6,     /* GMRES did not reach the tolerance within maxit */
+  RBC3D_EIO = 7          /* a file is missing, unreadable or not of the expected format */
 };
 
 typedef struct rbc3d_ctx rbc3d_ctx;
@@ -371,6 +372,40 @@ int rbc3d_stage_time(rbc3d_ctx *ctx, int stage, double *ms, long long *count, co
 /* --- timing hooks (HIP events on the context's stream) --------------------- */
 int rbc3d_stream(rbc3d_ctx *ctx, void **hip_stream);
 int rbc3d_sync(rbc3d_ctx *ctx);
+
+/* --- files (common/ModIO.F90), host only ---------------------------------------- */
+/* ReadWallMesh (:547-605): an Exodus II Tri3 wall mesh (netCDF, classic, 64-bit-offset or
+ * 64-bit-data encoding).  rbc3d_wall_mesh_info gives num_nodes and num_elem; then
+ * rbc3d_read_wall_mesh fills x(nvert,3) from coordx/coordy/coordz and e2v(nele,3),
+ * 1-based, from connect1 (what rbc3d_set_wall takes).  A mesh that is not Tri3, has more
+ * than one element block or names nodes outside 1..num_nodes is RBC3D_EIO. */
+int rbc3d_wall_mesh_info(const char *fn, int *nvert, int *nele);
+int rbc3d_read_wall_mesh(const char *fn, double *x, int *e2v);
+
+/* The restart file of WriteRestart (:742-786) and ReadRestart (:792-921), byte for byte:
+ * Fortran sequential unformatted records (a 4-byte length before and after each), 4-byte
+ * integers, 8-byte reals.  Records: Lb(3); lt; time; vBkg(3); nrbc; per cell (nlat0,
+ * nlon0), (nlat, nlon), celltype, starting_area, x(nlat,nlon,3); nwall; per wall (nvert,
+ * nele), x(nvert,3), f(nvert,3), e2v(nele,3).  Arrays are concatenated over cells and over
+ * walls.  npoint, nvert, nele of the info are totals (sum of nlat*nlon, of nvert, of nele). */
+typedef struct {
+  double Lb[3];
+  int lt; /* time step (the reader's Nt0) */
+  double time;
+  double vBkg[3];
+  int nrbc, nwall;
+  long long npoint, nvert, nele;
+} rbc3d_restart_info;
+int rbc3d_restart_write(const char *fn, const rbc3d_restart_info *info, const int *nlat0,
+                        const int *nlon0, const int *nlat, const int *nlon, const int *celltype,
+                        const double *starting_area, const double *x, const int *nvert,
+                        const int *nele, const double *wx, const double *wf, const int *e2v);
+/* The header and counts, to size the arrays of rbc3d_restart_read; that call checks the
+ * file still has the counts given in *info, then fills *info and the arrays. */
+int rbc3d_restart_info_read(const char *fn, rbc3d_restart_info *info);
+int rbc3d_restart_read(const char *fn, rbc3d_restart_info *info, int *nlat0, int *nlon0, int *nlat,
+                       int *nlon, int *celltype, double *starting_area, double *x, int *nvert,
+                       int *nele, double *wx, double *wf, int *e2v);
 
 #ifdef __cplusplus
 }

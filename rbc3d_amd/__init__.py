@@ -21,7 +21,7 @@ fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
 ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "HIP error", 3: "neighbour-cell list overflow (>32)",
-          4: "call out of order", 5: "non-finite input", 6: "GMRES did not converge"}
+          4: "call out of order", 5: "non-finite input", 6: "GMRES did not converge", 7: "file missing, unreadable or malformed"}
 
 
 class Rbc3dError(RuntimeError):
@@ -114,6 +114,15 @@ _SIG = {
     "rbc3d_filter_rbcs": [C.c_void_p],
     "rbc3d_get_rbc_coords": [C.c_void_p, dp],
     "rbc3d_set_repulsion": [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int],
+    "rbc3d_wall_mesh_info": [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)],
+    "rbc3d_read_wall_mesh": [C.c_char_p, dp, ip],
+    "rbc3d_restart_write": [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                            C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
+    "rbc3d_restart_info_read": [C.c_char_p, C.c_void_p],
+    "rbc3d_restart_read": [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                           C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p],
     "rbc3d_intra_cell_repulsion": [C.c_void_p, dp, C.POINTER(C.c_int)],
     "rbc3d_inter_cell_repulsion": [C.c_void_p, C.c_double, C.POINTER(C.c_int),
                                    C.POINTER(C.c_double)],
@@ -282,6 +291,84 @@ def rbc_sph_project(nlat, nlat0, u, nvar):
     _call("rbc3d_rbc_sph_project", len(nlat), np.ascontiguousarray(nlat, dtype=np.int32),
           np.ascontiguousarray(nlat0, dtype=np.int32), int(nvar), u)
     return u
+
+
+def read_wall_mesh(fn):
+    """ReadWallMesh (common/ModIO.F90:547-605): an Exodus II Tri3 mesh.  Returns
+    (x(nvert,3) Fortran-flat, e2v(nele,3) 1-based Fortran-flat, nvert, nele), the
+    arguments of System.set_wall."""
+    fb = os.fsencode(fn)
+    nv, ne = C.c_int(), C.c_int()
+    _call("rbc3d_wall_mesh_info", fb, C.byref(nv), C.byref(ne))
+    x = np.zeros(3 * nv.value)
+    e2v = np.zeros(3 * ne.value, np.int32)
+    _call("rbc3d_read_wall_mesh", fb, x, e2v)
+    return x, e2v, nv.value, ne.value
+
+
+class RestartInfo(C.Structure):
+    _fields_ = [("Lb", C.c_double * 3), ("lt", C.c_int), ("time", C.c_double),
+                ("vBkg", C.c_double * 3), ("nrbc", C.c_int), ("nwall", C.c_int),
+                ("npoint", C.c_longlong), ("nvert", C.c_longlong), ("nele", C.c_longlong)]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def write_restart(fn, Lb, lt, time, vBkg, cells=None, starting_area=None, walls=()):
+    """WriteRestart (common/ModIO.F90:742-786).  cells: dict with nlat0, nlon0, nlat, nlon,
+    celltype and x (concatenated x(nlat,nlon,3) per cell); starting_area per cell;
+    walls: sequence of (x(nvert,3), f(nvert,3), e2v(nele,3) 1-based, nvert, nele)."""
+    info = RestartInfo()
+    info.Lb[:] = [float(v) for v in Lb]
+    info.lt = int(lt)
+    info.time = float(time)
+    info.vBkg[:] = [float(v) for v in vBkg]
+    cell_args = [None] * 7
+    if cells is not None and len(cells["nlat"]):
+        info.nrbc = len(cells["nlat"])
+        cell_args = [_i32(cells["nlat0"]), _i32(cells["nlon0"]), _i32(cells["nlat"]),
+                     _i32(cells["nlon"]), _i32(cells["celltype"]), _f64(starting_area),
+                     _f64(cells["x"])]
+        info.npoint = int((cell_args[2].astype(np.int64) * cell_args[3]).sum())
+        assert cell_args[6].size == 3 * info.npoint and cell_args[5].size == info.nrbc
+    wall_args = [None] * 5
+    if walls:
+        info.nwall = len(walls)
+        wall_args = [_i32([w[3] for w in walls]), _i32([w[4] for w in walls]),
+                     _f64(np.concatenate([np.ravel(w[0]) for w in walls])),
+                     _f64(np.concatenate([np.ravel(w[1]) for w in walls])),
+                     _i32(np.concatenate([np.ravel(w[2]) for w in walls]))]
+    keep = cell_args + wall_args
+    _call("rbc3d_restart_write", os.fsencode(fn), C.byref(info), *[_ptr(a) for a in keep])
+
+
+def read_restart(fn):
+    """ReadRestart (common/ModIO.F90:792-921): a dict with Lb, lt, time, vBkg, the cells'
+    nlat0, nlon0, nlat, nlon, celltype, starting_area, x and the walls as (x, f, e2v,
+    nvert, nele) tuples."""
+    fb = os.fsencode(fn)
+    info = RestartInfo()
+    _call("rbc3d_restart_info_read", fb, C.byref(info))
+    nc, nw = info.nrbc, info.nwall
+    ci = {k: np.zeros(nc, np.int32) for k in ("nlat0", "nlon0", "nlat", "nlon", "celltype")}
+    area = np.zeros(nc)
+    x = np.zeros(3 * info.npoint)
+    nvert, nele = np.zeros(nw, np.int32), np.zeros(nw, np.int32)
+    wx, wf = np.zeros(3 * info.nvert), np.zeros(3 * info.nvert)
+    e2v = np.zeros(3 * info.nele, np.int32)
+    _call("rbc3d_restart_read", fb, C.byref(info), _ptr(ci["nlat0"]), _ptr(ci["nlon0"]),
+          _ptr(ci["nlat"]), _ptr(ci["nlon"]), _ptr(ci["celltype"]), _ptr(area), _ptr(x),
+          _ptr(nvert), _ptr(nele), _ptr(wx), _ptr(wf), _ptr(e2v))
+    walls, vo, eo = [], 0, 0
+    for k in range(nw):
+        nv, ne = int(nvert[k]), int(nele[k])
+        walls.append((wx[3 * vo:3 * (vo + nv)], wf[3 * vo:3 * (vo + nv)], e2v[3 * eo:3 * (eo + ne)], nv, ne))
+        vo += nv
+        eo += ne
+    return dict(Lb=tuple(info.Lb), lt=info.lt, time=info.time, vBkg=tuple(info.vBkg),
+                starting_area=area, x=x, walls=walls, **ci)
 
 
 class System:

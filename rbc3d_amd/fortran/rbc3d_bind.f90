@@ -40,7 +40,8 @@ module rbc3d_bind
        rbc3d_rbc_compute_geometry, rbc3d_shell_res_force, rbc3d_rbc_sph_project, &
        rbc3d_set_ref_cells, rbc3d_compute_rbc_vel, rbc3d_move_rbcs, rbc3d_rebox_rbcs, &
        rbc3d_vol_constrain_rbcs, rbc3d_filter_rbcs, rbc3d_get_rbc_coords, rbc3d_timeint_euler, &
-       rbc3d_set_repulsion, rbc3d_intra_cell_repulsion, rbc3d_inter_cell_repulsion
+       rbc3d_set_repulsion, rbc3d_intra_cell_repulsion, rbc3d_inter_cell_repulsion, &
+       rbc3d_wall_mesh_info, rbc3d_read_wall_mesh
 
   ! rbc3d_set_precision modes
   integer(c_int), parameter, public :: RBC3D_PREC_FP64 = 0, RBC3D_PREC_SPLIT = 1
@@ -310,6 +311,18 @@ module rbc3d_bind
       real(c_double), value :: eps_dist
       integer(c_int), intent(out) :: npoints
       real(c_double), intent(out) :: dist_min
+    end function
+    ! file names are NUL-terminated: pass trim(fn)//c_null_char
+    integer(c_int) function rbc3d_wall_mesh_info(fn, nvert, nele) bind(C, name="rbc3d_wall_mesh_info")
+      import :: c_int, c_char
+      character(kind=c_char), intent(in) :: fn(*)
+      integer(c_int), intent(out) :: nvert, nele
+    end function
+    integer(c_int) function rbc3d_read_wall_mesh(fn, x, e2v) bind(C, name="rbc3d_read_wall_mesh")
+      import :: c_int, c_char, c_double
+      character(kind=c_char), intent(in) :: fn(*)
+      real(c_double), intent(out) :: x(*)
+      integer(c_int), intent(out) :: e2v(*)
     end function
     function c_strlen(s) bind(C, name="strlen") result(n)
       import :: c_ptr, c_size_t
