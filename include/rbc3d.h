@@ -221,6 +221,18 @@ int rbc3d_inter_cell_repulsion(rbc3d_ctx *ctx, double eps_dist, int *npoints, do
 int rbc3d_timeint_euler(rbc3d_ctx *ctx, double Ts, int nsteps, double eps_dist,
                         double noslip_rtol, int noslip_maxit, double *time, int *its_cell,
                         int *its_wall);
+/* TimeInt_RK2(alpha) (common/ModTimeInt.F90:183-271), nsteps steps: k1 = Compute_Rbc_Vel +
+ * NoSlipWall, the cells moved by alpha Ts k1 for k2 and moved back, x += Ts (b1 k1 + b2 k2)
+ * with b2 = 1/(2 alpha), b1 = 1 - b2, ReboxRbcs.  its_cell, its_wall (optional): 2 nsteps
+ * counts, the two evaluations of every step. */
+int rbc3d_timeint_rk2(rbc3d_ctx *ctx, double alpha, double Ts, int nsteps, double noslip_rtol,
+                      int noslip_maxit, double *time, int *its_cell, int *its_wall);
+/* TimeInt_AB2 (common/ModTimeInt.F90:275-341): one Euler step from Compute_Rbc_Vel alone
+ * (no NoSlipWall or rebox, time not advanced, as the reference), then nsteps - 1 steps of
+ * x += Ts (3 v - v_prev)/2 after Compute_Rbc_Vel + NoSlipWall, ReboxRbcs, time + Ts.
+ * its_cell, its_wall (optional): nsteps counts. */
+int rbc3d_timeint_ab2(rbc3d_ctx *ctx, double Ts, int nsteps, double noslip_rtol, int noslip_maxit,
+                      double *time, int *its_cell, int *its_wall);
 
 /* Glob_Sph_Trans(v,c,direction) (common/ModVelSolver.F90:641-719); direction
  * 1 = PHYS_TO_FOUR, -1 = FOUR_TO_PHYS.  v(nPoint,3), c(3*sum nlat0^2). */

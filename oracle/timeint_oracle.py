@@ -13,6 +13,10 @@ One step, in the reference's order:
   ReboxRbcs (:1006-1020)
   4 x {VolConstrainRbcs (:938-963), InterCellRepulsion (common/ModRepulsion.F90:270-402,
   when the repulsion is on), FilterRbcs (:908-934)}
+
+TimeInt_RK2 (:183-271) and TimeInt_AB2 (:275-341) use the same velocity evaluation
+(Compute_Rbc_Vel, then NoSlipWall) and ReboxRbcs, without the volume constraint, the
+repulsion and the filter.
 """
 from __future__ import annotations
 
@@ -107,4 +111,79 @@ def timeint_euler(o, cells, refs, ES, ED, EB, Ts, nsteps, eps_dist, Lb, noslip=N
                                               repulsion[2], wall=wall)
             for c in range(len(xs)):
                 xs[c] = O.rbc_sph_project(xs[c], int(nlat[c]), int(nlat0[c]))
+    return np.concatenate(xs), its
+
+
+def velocity(o, cells, xs, refs, ES, ED, EB, noslip=None):
+    """Compute_Rbc_Vel (+ NoSlipWall): the node velocities (3, np) of the cells at xs, and
+    the (cell, wall) GMRES iterations."""
+    nlat0, ct = cells.nlat0, cells.celltype
+    geos, fs = [], []
+    for c, x in enumerate(xs):
+        k = ct[c] - 1
+        g, f = cell_force(x, int(nlat0[c]), refs[k], ES[k], ED[k], EB[k])
+        geos.append(g)
+        fs.append(f)
+    cur = dataclasses.replace(cells, x=np.concatenate(xs),
+                              a3=np.concatenate([g["a3"] for g in geos]),
+                              detj=np.concatenate([g["detj"] for g in geos]))
+    o.set_cells(cur)
+    o.set_density(1, np.concatenate(fs))
+    v, ni, _ = o.solve_rbc_vel()
+    nw = o.noslip_wall(*noslip)[0] if noslip is not None else 0
+    return v.reshape(3, -1), (ni, nw)
+
+
+def _add(xs, nlat, V, fn):
+    out, off = [], 0
+    for c, x in enumerate(xs):
+        npt = 2 * int(nlat[c]) ** 2
+        out.append(fn(x.reshape(3, npt), V[:, off:off + npt] if V is not None else None).ravel())
+        off += npt
+    return out
+
+
+def timeint_rk2(o, cells, refs, ES, ED, EB, alpha, Ts, nsteps, Lb, noslip=None):
+    """TimeInt_RK2(alpha): returns the final coordinates and the iterations of every
+    evaluation."""
+    c2, b2 = alpha, 1 / (2 * alpha)
+    b1 = 1 - b2
+    xs = split_cells(cells.x, cells.nlat, 3)
+    its = []
+    for _ in range(nsteps):
+        k1, it = velocity(o, cells, xs, refs, ES, ED, EB, noslip)
+        its.append(it)
+        xs = _add(xs, cells.nlat, k1, lambda X, K: X + c2 * Ts * K)
+        k2, it = velocity(o, cells, xs, refs, ES, ED, EB, noslip)
+        its.append(it)
+        xs = _add(xs, cells.nlat, k1, lambda X, K: X - c2 * Ts * K)
+        off = 0
+        out = []
+        for c, x in enumerate(xs):
+            npt = 2 * int(cells.nlat[c]) ** 2
+            K1, K2 = k1[:, off:off + npt], k2[:, off:off + npt]
+            out.append(rebox((x.reshape(3, npt) + Ts * (b1 * K1 + b2 * K2)).ravel(), Lb))
+            off += npt
+        xs = out
+    return np.concatenate(xs), its
+
+
+def timeint_ab2(o, cells, refs, ES, ED, EB, Ts, nsteps, Lb, noslip=None):
+    """TimeInt_AB2: an Euler step from Compute_Rbc_Vel alone, then nsteps - 1 AB2 steps."""
+    xs = split_cells(cells.x, cells.nlat, 3)
+    v, it = velocity(o, cells, xs, refs, ES, ED, EB)
+    its = [it]
+    xs = _add(xs, cells.nlat, v, lambda X, V: X + Ts * V)
+    for _ in range(1, nsteps):
+        v1 = v
+        v, it = velocity(o, cells, xs, refs, ES, ED, EB, noslip)
+        its.append(it)
+        off = 0
+        out = []
+        for c, x in enumerate(xs):
+            npt = 2 * int(cells.nlat[c]) ** 2
+            V, V1 = v[:, off:off + npt], v1[:, off:off + npt]
+            out.append(rebox((x.reshape(3, npt) + Ts * (3 * V - V1) / 2).ravel(), Lb))
+            off += npt
+        xs = out
     return np.concatenate(xs), its

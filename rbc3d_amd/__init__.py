@@ -114,6 +114,10 @@ _SIG = {
     "rbc3d_filter_rbcs": [C.c_void_p],
     "rbc3d_get_rbc_coords": [C.c_void_p, dp],
     "rbc3d_set_repulsion": [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int],
+    "rbc3d_timeint_rk2": [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int,
+                          C.POINTER(C.c_double), C.c_void_p, C.c_void_p],
+    "rbc3d_timeint_ab2": [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_int,
+                          C.POINTER(C.c_double), C.c_void_p, C.c_void_p],
     "rbc3d_wall_mesh_info": [C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_int)],
     "rbc3d_read_wall_mesh": [C.c_char_p, dp, ip],
     "rbc3d_restart_write": [C.c_char_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
@@ -707,6 +711,25 @@ class System:
         iw = np.zeros(max(nsteps, 1), np.int32)
         _call("rbc3d_timeint_euler", self.h, Ts, nsteps, eps_dist, noslip_rtol, noslip_maxit,
               C.byref(t), ic.ctypes.data_as(C.c_void_p), iw.ctypes.data_as(C.c_void_p))
+        return t.value, ic[:nsteps], iw[:nsteps]
+
+    def timeint_rk2(self, alpha, Ts, nsteps, noslip_rtol=1e-8, noslip_maxit=60, time=0.0):
+        """TimeInt_RK2(alpha): returns (time, cell iterations, wall iterations), two
+        evaluations per step."""
+        t = C.c_double(time)
+        ic = np.zeros(max(2 * nsteps, 1), np.int32)
+        iw = np.zeros(max(2 * nsteps, 1), np.int32)
+        _call("rbc3d_timeint_rk2", self.h, alpha, Ts, nsteps, noslip_rtol, noslip_maxit,
+              C.byref(t), ic.ctypes.data_as(C.c_void_p), iw.ctypes.data_as(C.c_void_p))
+        return t.value, ic[:2 * nsteps], iw[:2 * nsteps]
+
+    def timeint_ab2(self, Ts, nsteps, noslip_rtol=1e-8, noslip_maxit=60, time=0.0):
+        """TimeInt_AB2 (nsteps position updates, the first an Euler step)."""
+        t = C.c_double(time)
+        ic = np.zeros(max(nsteps, 1), np.int32)
+        iw = np.zeros(max(nsteps, 1), np.int32)
+        _call("rbc3d_timeint_ab2", self.h, Ts, nsteps, noslip_rtol, noslip_maxit, C.byref(t),
+              ic.ctypes.data_as(C.c_void_p), iw.ctypes.data_as(C.c_void_p))
         return t.value, ic[:nsteps], iw[:nsteps]
 
     def stream(self):

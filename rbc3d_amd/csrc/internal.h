@@ -306,6 +306,7 @@ struct rbc3d_ctx {
   DBuf<double> d_mxc, d_marea, d_mvol, d_mmesh, d_mmod;
   DBuf<int> d_moff, d_mnlat, d_mnlat0, d_mref;
   DBuf<double> d_vel;  // the cells' node velocities of the last Compute_Rbc_Vel, [3][np]
+  DBuf<double> d_vel_k1;  // an earlier velocity of the multi-stage integrators (RK2 k1, AB2 v_1)
   bool vel_ready = false;
   int mech_epoch = -1;  // geom_epoch the per-cell tables above were built for
   bool rbc_geom_ready = false;  // the block geometry above is that of d_x

@@ -14,6 +14,20 @@ __global__ void k_move(size_t n, double dt, const double *__restrict__ v, double
   if (i < n) x[i] = x[i] + dt * v[i];
 }
 
+// x = x + Ts (b1 k1 + b2 v): the update of TimeInt_RK2 (common/ModTimeInt.F90:246-250)
+__global__ void k_rk2_update(size_t n, double Ts, double b1, double b2, const double *__restrict__ k1,
+                             const double *__restrict__ v, double *x) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] = x[i] + Ts * (b1 * k1[i] + b2 * v[i]);
+}
+
+// x = x + Ts (3 v - v_1)/2: the update of TimeInt_AB2 (common/ModTimeInt.F90:314-317)
+__global__ void k_ab2_update(size_t n, double Ts, const double *__restrict__ v,
+                             const double *__restrict__ v1, double *x) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i < n) x[i] = x[i] + Ts * (3 * v[i] - v1[i]) / 2;
+}
+
 // ReboxRbcs (common/ModTimeInt.F90:1006-1020): per cell and axis the centre of the
 // coordinate range, and the cell shifted by whole boxes so that centre lies in [0, Lb)
 __global__ void __launch_bounds__(256) k_rebox(int nrbc, const CellRec *__restrict__ cells, int np,
@@ -197,5 +211,91 @@ extern "C" int rbc3d_timeint_euler(rbc3d_ctx *c, double Ts, int nsteps, double e
     if (its_cell) its_cell[lt] = ni;
     if (its_wall) its_wall[lt] = nw;
   }
+  return RBC3D_OK;
+}
+
+// Compute_Rbc_Vel, then NoSlipWall when a wall is set: the velocity evaluation of the
+// integrators below; iteration counts to *ni, *nw
+static int velocity_and_wall(rbc3d_ctx *c, double noslip_rtol, int noslip_maxit, int *ni, int *nw) {
+  double rn = 0, vres = 0;
+  int conv = 0;
+  *ni = *nw = 0;
+  RBC_TRY(rbc3d_compute_rbc_vel(c, nullptr, ni, &rn));
+  if (c->wall.nvert) {
+    const int r = rbc3d_noslip_wall(c, noslip_rtol, noslip_maxit, nullptr, nw, &rn, &conv, &vres);
+    if (r != RBC3D_OK && r != RBC3D_ENOCONV) return r;
+  }
+  return RBC3D_OK;
+}
+
+// TimeInt_RK2(alpha) (common/ModTimeInt.F90:183-271): per step k1 = Compute_Rbc_Vel +
+// NoSlipWall at x, the cells moved to x + alpha Ts k1 for the second evaluation k2 and moved
+// back by the same amount, then x + Ts (b1 k1 + b2 k2) with b2 = 1/(2 alpha), b1 = 1 - b2,
+// and ReboxRbcs.  No volume constraint, repulsion or filter, as in the reference.
+// its_cell / its_wall (optional) take 2 nsteps counts: the two evaluations of each step.
+extern "C" int rbc3d_timeint_rk2(rbc3d_ctx *c, double alpha, double Ts, int nsteps,
+                                 double noslip_rtol, int noslip_maxit, double *time, int *its_cell,
+                                 int *its_wall) {
+  RBC_TRY(require_cells(c));
+  if (nsteps < 0 || !(Ts > 0) || !(alpha != 0) || !std::isfinite(alpha) || !time) return RBC3D_EINVAL;
+  const double c2 = alpha, b2 = 1 / (2 * alpha), b1 = 1 - b2;
+  const size_t n = (size_t)3 * c->np;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  RBC_TRY(c->d_vel_k1.ensure(n + 1));
+  for (int lt = 0; lt < nsteps; lt++) {
+    int ni, nw;
+    RBC_TRY(velocity_and_wall(c, noslip_rtol, noslip_maxit, &ni, &nw));
+    if (its_cell) its_cell[2 * lt] = ni;
+    if (its_wall) its_wall[2 * lt] = nw;
+    HIP_TRY(hipMemcpyAsync(c->d_vel_k1.p, c->d_vel.p, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                           c->stream));
+    k_move<<<nb, 256, 0, c->stream>>>(n, c2 * Ts, c->d_vel_k1.p, c->d_x.p);
+    HIP_TRY(hipGetLastError());
+    RBC_TRY(velocity_and_wall(c, noslip_rtol, noslip_maxit, &ni, &nw));
+    if (its_cell) its_cell[2 * lt + 1] = ni;
+    if (its_wall) its_wall[2 * lt + 1] = nw;
+    k_move<<<nb, 256, 0, c->stream>>>(n, -(c2 * Ts), c->d_vel_k1.p, c->d_x.p);
+    k_rk2_update<<<nb, 256, 0, c->stream>>>(n, Ts, b1, b2, c->d_vel_k1.p, c->d_vel.p, c->d_x.p);
+    HIP_TRY(hipGetLastError());
+    c->rbc_geom_ready = false;
+    RBC_TRY(rbc3d_rebox_rbcs(c));
+    *time += Ts;
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
+  return RBC3D_OK;
+}
+
+// TimeInt_AB2 (common/ModTimeInt.F90:275-341): one Euler step x + Ts v from Compute_Rbc_Vel
+// alone (no NoSlipWall, no rebox, time not advanced, as in the reference), then nsteps - 1
+// steps of Compute_Rbc_Vel + NoSlipWall, x + Ts (3 v - v_1)/2 with v_1 the previous
+// velocity, ReboxRbcs, time + Ts.  its_cell / its_wall (optional): nsteps counts, the
+// first from the Euler step.
+extern "C" int rbc3d_timeint_ab2(rbc3d_ctx *c, double Ts, int nsteps, double noslip_rtol,
+                                 int noslip_maxit, double *time, int *its_cell, int *its_wall) {
+  RBC_TRY(require_cells(c));
+  if (nsteps < 1 || !(Ts > 0) || !time) return RBC3D_EINVAL;
+  const size_t n = (size_t)3 * c->np;
+  const unsigned nb = (unsigned)((n + 255) / 256);
+  RBC_TRY(c->d_vel_k1.ensure(n + 1));
+  int ni = 0;
+  double rn = 0;
+  RBC_TRY(rbc3d_compute_rbc_vel(c, nullptr, &ni, &rn));
+  if (its_cell) its_cell[0] = ni;
+  if (its_wall) its_wall[0] = 0;
+  RBC_TRY(rbc3d_move_rbcs(c, Ts));
+  for (int lt = 1; lt < nsteps; lt++) {
+    HIP_TRY(hipMemcpyAsync(c->d_vel_k1.p, c->d_vel.p, sizeof(double) * n, hipMemcpyDeviceToDevice,
+                           c->stream));
+    int nw;
+    RBC_TRY(velocity_and_wall(c, noslip_rtol, noslip_maxit, &ni, &nw));
+    if (its_cell) its_cell[lt] = ni;
+    if (its_wall) its_wall[lt] = nw;
+    k_ab2_update<<<nb, 256, 0, c->stream>>>(n, Ts, c->d_vel.p, c->d_vel_k1.p, c->d_x.p);
+    HIP_TRY(hipGetLastError());
+    c->rbc_geom_ready = false;
+    RBC_TRY(rbc3d_rebox_rbcs(c));
+    *time += Ts;
+  }
+  HIP_TRY(hipStreamSynchronize(c->stream));
   return RBC3D_OK;
 }

@@ -113,3 +113,46 @@ def test_tube_step_with_noslip_wall():
     print(f"tube step: coordinates {err:.2e}, wall traction {e_f:.2e}, iterations "
           f"{ic[0]}/{iw[0]} vs {its[0]}", flush=True)
     assert err < TOL and e_f < 1e-9 and iw[0] == its[0][1]
+
+
+def test_rk2_and_ab2_match_the_oracle():
+    import rbc3d_amd as R
+    rng = np.random.default_rng(3)
+    rots = [configs.random_rotation(rng) for _ in range(2)]
+    cells = deformed_cells([(2.5, 2.5, 2.5), (5.5, 5.6, 5.4)], seed=4, rots=rots)
+    w = workload(cells, 5.0)
+    ref = ref_cell()
+    g = R.from_workload(w)
+    g.set_ref_cells([NLAT], ref_x(), [ES], [ED], [EB])
+    t, ic, _ = g.timeint_rk2(2.0 / 3.0, 0.02, 2)
+    xo, its = T.timeint_rk2(O.from_workload(w), cells, [ref], [ES], [ED], [EB], 2.0 / 3.0, 0.02, 2,
+                            w.Lb)
+    e_rk = np.abs(g.rbc_coords() - xo).max()
+    assert t == pytest.approx(0.04, abs=1e-15)
+    g = R.from_workload(w)
+    g.set_ref_cells([NLAT], ref_x(), [ES], [ED], [EB])
+    t2, ic2, _ = g.timeint_ab2(0.02, 3)
+    xa, its2 = T.timeint_ab2(O.from_workload(w), cells, [ref], [ES], [ED], [EB], 0.02, 3, w.Lb)
+    e_ab = np.abs(g.rbc_coords() - xa).max()
+    moved = np.abs(xa - cells.x).max()
+    print(f"RK2 {e_rk:.2e} (iterations {list(ic)} vs {[i for i, _ in its]}), AB2 {e_ab:.2e} "
+          f"(moved {moved:.3e})", flush=True)
+    assert t2 == pytest.approx(0.04, abs=1e-15)  # the first (Euler) step does not advance time
+    assert e_rk < TOL and e_ab < TOL and moved > 1e-3
+    assert all(abs(a - b[0]) <= 1 for a, b in zip(ic, its))
+
+
+def test_ab2_tube_with_noslip_wall():
+    import rbc3d_amd as R
+    w = configs.make_workload("C3s", seed=2)
+    g = R.from_workload(w)
+    g.set_ref_cells([NLAT], ref_x(), [ES], [ED], [EB])
+    _, ic, iw = g.timeint_ab2(0.02, 2, noslip_rtol=1e-7, noslip_maxit=200)
+    o = O.from_workload(w)
+    xo, its = T.timeint_ab2(o, w.cells, [ref_cell()], [ES], [ED], [EB], 0.02, 2, w.Lb,
+                            noslip=(1e-7, 200))
+    err = np.abs(g.rbc_coords() - xo).max()
+    e_f = np.abs(g.wall_density() - o.wall_density()).max() / np.abs(o.wall_density()).max()
+    print(f"AB2 tube: coordinates {err:.2e}, wall traction {e_f:.2e}, wall iterations "
+          f"{list(iw)} vs {[b for _, b in its]}", flush=True)
+    assert err < TOL and e_f < 1e-9 and list(iw) == [b for _, b in its]

@@ -139,3 +139,35 @@ def test_vol_constrain():
     eps = 0.01
     d = (T.vol_constrain(x, ref["vol"], eps) - x).reshape(3, -1)
     assert np.allclose(d, -(eps / 20) * g["a3"].reshape(3, -1), rtol=0, atol=1e-14)
+
+
+def trajectories(method, hs, T):
+    """final coordinates of a relaxing deformed cell (viscosity ratio 1, EB 0) after time T
+    with steps hs"""
+    ES, ED, EB = 1.0, 2.0, 0.0
+    ref = ref_cell()
+    cells = deformed_cells([(4.0, 4.0, 4.0)], seed=5, amp=0.12)
+    w = workload(cells, 1.0)
+    out = []
+    for h in hs:
+        o = O.from_workload(w)
+        n = int(round(T / h))
+        if method == "rk2":
+            x, _ = T_.timeint_rk2(o, cells, [ref], [ES], [ED], [EB], 2.0 / 3.0, h, n, w.Lb)
+        else:
+            x, _ = T_.timeint_ab2(o, cells, [ref], [ES], [ED], [EB], h, n, w.Lb)
+        out.append(x)
+    return out
+
+
+T_ = T
+
+
+@pytest.mark.parametrize("method", ["rk2", "ab2"])
+def test_second_order_integrators(method):
+    """RK2 (Ralston, alpha = 2/3) and AB2 converge at second order: halving the step
+    quarters the difference between successive solutions (Euler would halve it)."""
+    x1, x2, x3 = trajectories(method, [0.01, 0.005, 0.0025], 0.02)
+    e1, e2 = np.abs(x1 - x2).max(), np.abs(x2 - x3).max()
+    print(f"{method}: successive differences {e1:.3e}, {e2:.3e}, ratio {e1 / e2:.2f}")
+    assert 3.5 < e1 / e2 < 4.6
