@@ -585,8 +585,8 @@ def main():
         "near_singular": {"projections": int(nproj), "corrections": int(nnear)},
         "rates": stage_rates(w, {k: v[0] / args.steps for k, v in stages.items() if v[1]}, False),
         "chosen": {
-            "what": "split form (FP32 pair sums with FP64 near pairs, FP32 PME, split-BF16 "
-                    "MFMA Legendre GEMMs; singular quadrature FP64), same matvec",
+            "what": "split form (FP32 PME, split-BF16 MFMA Legendre GEMMs; pair sums and "
+                    "singular quadrature FP64), same matvec",
             "ms_per_step": round(ms_split, 4),
             "cell_matvecs_per_s": round(ncell / (ms_split * 1e-3), 3),
             "geometry_reused_ms_per_step": round(ms_split_reuse, 4),

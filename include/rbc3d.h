@@ -295,7 +295,7 @@ int rbc3d_graph_ready(rbc3d_ctx *ctx, int *ready);
 /* Precision of the hot path (no counterpart in the reference, which is FP64
  * throughout).  RBC3D_PREC_FP64, the default, is the plain form: every stage in FP64,
  * the reference's arithmetic.  RBC3D_PREC_SPLIT is the split form: real-space pair
- * sums in FP32 on target-bin-relative coordinates with FP32 accumulation; PME
+ * sums in FP64 or FP32 (rbc3d_set_split_pairs); PME
  * spread, FFTs, Green's function and gather on FP32 grids; the Legendre GEMMs of the
  * global SH transforms as three-way split BF16 products on the matrix cores with
  * FP32 accumulation; the wall self-matrix stored in FP32.  The self-patch and
@@ -303,6 +303,12 @@ int rbc3d_graph_ready(rbc3d_ctx *ctx, int *ready);
  * linear term, the surface splines and the GMRES Arnoldi process stay FP64. */
 enum { RBC3D_PREC_FP64 = 0, RBC3D_PREC_SPLIT = 1 };
 int rbc3d_set_precision(rbc3d_ctx *ctx, int mode);
+/* The real-space pair sums of the split form (no counterpart in the reference): FP64, the
+ * default (fp32 = 0), or FP32 on target-bin-relative coordinates with FP32 accumulation
+ * and the pairs closer than rc/4 in FP64 (fp32 = 1).  On gfx950 the pair kernel is bound
+ * by instruction issue, and FP32 arithmetic without packed instructions issues at the FP64
+ * rate, so the FP32 kernel is not faster and the split form keeps FP64 pairs unless asked. */
+int rbc3d_set_split_pairs(rbc3d_ctx *ctx, int fp32);
 
 /* --- the two GMRES solves (PETSc KSPGMRES semantics: restart 30, classical
  *     Gram-Schmidt, no preconditioner, x += V y at restarts) ------------------ */

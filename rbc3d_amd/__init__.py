@@ -114,6 +114,7 @@ _SIG = {
     "rbc3d_filter_rbcs": [C.c_void_p],
     "rbc3d_get_rbc_coords": [C.c_void_p, dp],
     "rbc3d_set_repulsion": [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_int],
+    "rbc3d_set_split_pairs": [C.c_void_p, C.c_int],
     "rbc3d_timeint_rk2": [C.c_void_p, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int,
                           C.POINTER(C.c_double), C.c_void_p, C.c_void_p],
     "rbc3d_timeint_ab2": [C.c_void_p, C.c_double, C.c_int, C.c_double, C.c_int,
@@ -499,6 +500,10 @@ class System:
         """0: plain FP64 (default); 1: split form (FP32 pair sums and PME, split-BF16
         Legendre GEMMs, FP32 wall self-matrix; quadratures and GMRES in FP64)."""
         _call("rbc3d_set_precision", self.h, int(mode))
+
+    def set_split_pairs(self, fp32):
+        """The split form's pair sums: FP64 (False, the default) or FP32 (True)."""
+        _call("rbc3d_set_split_pairs", self.h, int(bool(fp32)))
 
     def set_gmres_restart(self, restart):
         """GMRES restart length of both solves (30, PETSc's default, unless set)."""

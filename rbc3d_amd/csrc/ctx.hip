@@ -809,6 +809,16 @@ extern "C" int rbc3d_set_precision(rbc3d_ctx *c, int mode) {
   return RBC3D_OK;
 }
 
+extern "C" int rbc3d_set_split_pairs(rbc3d_ctx *c, int fp32) {
+  if (!c) return RBC3D_EINVAL;
+  if (c->split_pair32 != (fp32 != 0)) {
+    HIP_TRY(hipStreamSynchronize(c->stream));
+    c->split_pair32 = fp32 != 0;
+    c->mv_epoch = -1;  // a recorded matvec graph holds the other pair kernel
+  }
+  return RBC3D_OK;
+}
+
 extern "C" int rbc3d_set_gmres_restart(rbc3d_ctx *c, int restart) {
   if (!c || restart < 1 || restart > RBC3D_GMRES_MAX_RESTART) {
     rbc_set_error("rbc3d_set_gmres_restart: restart must be in 1..255");

@@ -190,6 +190,7 @@ struct rbc3d_ctx {
   double Acoef[8], Bcoef[8];
   hipStream_t stream = nullptr;
   int prec = RBC3D_PREC_FP64;  // rbc3d_set_precision
+  bool split_pair32 = false;   // rbc3d_set_split_pairs: FP32 pair sums in the split form
   int gmres_restart = 30;      // rbc3d_set_gmres_restart (PETSc's default length)
   int ncu = 256;               // compute units of the device
   int lds_max = 65536;         // LDS bytes one workgroup may allocate on the device
@@ -346,7 +347,6 @@ __device__ __forceinline__ int imod_d(int a, int n) {
   return r < 0 ? r + n : r;
 }
 
-// EwaldCoeff_SL / EwaldCoeff_DL (common/ModEwaldFunc.F90:86-178) by table interpolation.
 // XCD-aware block order: workgroups are dealt to the 8 XCDs round-robin
 // (blockIdx % 8), so logical block (blockIdx % 8) * per + blockIdx / 8 gives each XCD
 // one contiguous range of the logical order and keeps that range's data in one L2.
@@ -379,6 +379,7 @@ __device__ __forceinline__ double rsq_nr(double x) {
   return fma(0.5 * y, e, y);
 }
 
+// EwaldCoeff_SL / EwaldCoeff_DL (common/ModEwaldFunc.F90:86-178) by table interpolation.
 // Ewald coefficients at r with 1/r supplied (r >= reps already established)
 __device__ __forceinline__ double ewald_dl_ir(const double *__restrict__ t, double r, double ir,
                                               double tabscale) {

@@ -2039,7 +2039,7 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   pa.stat = c->d_pairstat.p;
   int nblk = nb + (nT + PT - 1) / PT + 1;
   rbc_stage_begin(c, ST_PAIR);
-  if (c->prec == RBC3D_PREC_SPLIT) {
+  if (c->prec == RBC3D_PREC_SPLIT && c->split_pair32) {
     RBC_TRY(launch_pair32(c, pa, nblk, tp->maskf));
   } else if (c1 != 0 && c2 != 0)
     k_pair<true, true><<<nblk, PT * PW, 0, st>>>(pa);

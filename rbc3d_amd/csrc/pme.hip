@@ -525,7 +525,11 @@ __global__ void __launch_bounds__(64) k_wsp_brick(WspArgs<T> a) {
   __shared__ T sxy[WSB_CH][16 + 16 / sizeof(T)];  // rows padded: conflict-free 16-byte stores
   __shared__ V4 sz[WSB_CH];
   __shared__ V4 sf[WSB_CH];
-  const int lane = threadIdx.x, b = blockIdx.x;
+  // consecutive bricks of an x-row on one XCD: in FP32 a brick row is 16 bytes, half a
+  // write sector, and the neighbouring brick's half then merges in the same L2
+  const int b = xcd_block(a.nbr);
+  if (b < 0) return;
+  const int lane = threadIdx.x;
   const int bx = b % a.nbx, by = (b / a.nbx) % a.nby, bz = b / (a.nbx * a.nby);
   const int lxy = lane & 15, lz = lane >> 4;
   const int gi = bx * WSB + (lane & 3), gj = by * WSB + ((lane >> 2) & 3), gk = bz * WSB + lz;
@@ -668,7 +672,7 @@ static int wall_spread(rbc3d_ctx *c, double c1, const double *d_u = nullptr,
   a.grid = (T *)c->d_grid.p;
   rbc_stage_begin(c, ST_SPREAD);
   k_wsp_src<T><<<(w.nele + 255) / 256, 256, 0, c->stream>>>(a);
-  k_wsp_brick<T><<<a.nbr, 64, 0, c->stream>>>(a);
+  k_wsp_brick<T><<<xcd_grid(a.nbr), 64, 0, c->stream>>>(a);
   rbc_stage_end(c, ST_SPREAD);
   HIP_TRY(hipGetLastError());
   return RBC3D_OK;

@@ -41,7 +41,7 @@ module rbc3d_bind
        rbc3d_set_ref_cells, rbc3d_compute_rbc_vel, rbc3d_move_rbcs, rbc3d_rebox_rbcs, &
        rbc3d_vol_constrain_rbcs, rbc3d_filter_rbcs, rbc3d_get_rbc_coords, rbc3d_timeint_euler, &
        rbc3d_set_repulsion, rbc3d_intra_cell_repulsion, rbc3d_inter_cell_repulsion, &
-       rbc3d_wall_mesh_info, rbc3d_read_wall_mesh
+       rbc3d_wall_mesh_info, rbc3d_read_wall_mesh, rbc3d_timeint_rk2, rbc3d_timeint_ab2
 
   ! rbc3d_set_precision modes
   integer(c_int), parameter, public :: RBC3D_PREC_FP64 = 0, RBC3D_PREC_SPLIT = 1
@@ -311,6 +311,24 @@ module rbc3d_bind
       real(c_double), value :: eps_dist
       integer(c_int), intent(out) :: npoints
       real(c_double), intent(out) :: dist_min
+    end function
+    integer(c_int) function rbc3d_timeint_rk2(ctx, alpha, Ts, nsteps, noslip_rtol, noslip_maxit, &
+        time, its_cell, its_wall) bind(C, name="rbc3d_timeint_rk2")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: alpha, Ts, noslip_rtol
+      integer(c_int), value :: nsteps, noslip_maxit
+      real(c_double), intent(inout) :: time
+      integer(c_int), intent(out) :: its_cell(*), its_wall(*)
+    end function
+    integer(c_int) function rbc3d_timeint_ab2(ctx, Ts, nsteps, noslip_rtol, noslip_maxit, &
+        time, its_cell, its_wall) bind(C, name="rbc3d_timeint_ab2")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: ctx
+      real(c_double), value :: Ts, noslip_rtol
+      integer(c_int), value :: nsteps, noslip_maxit
+      real(c_double), intent(inout) :: time
+      integer(c_int), intent(out) :: its_cell(*), its_wall(*)
     end function
     ! file names are NUL-terminated: pass trim(fn)//c_null_char
     integer(c_int) function rbc3d_wall_mesh_info(fn, nvert, nele) bind(C, name="rbc3d_wall_mesh_info")

@@ -1,12 +1,14 @@
 """The split-precision form (rbc3d_set_precision(ctx, RBC3D_PREC_SPLIT)) against the
 FP64 oracle on identical inputs.
 
-The split form runs the real-space pair sums in FP32 (target-bin-relative
-coordinates, FP32 accumulation), the PME grids / FFTs / gather in FP32, the Legendre
+The split form runs the PME grids / FFTs / gather in FP32, the Legendre
 GEMMs of the global SH transforms as three-way split BF16 products with FP32
 accumulation, and stores the wall self-matrix in FP32; the singular and
-near-singular quadratures and GMRES stay FP64.  Bar: every output within 1e-6
-relative (max-norm) of the FP64 oracle; each test prints the error it measured."""
+near-singular quadratures and GMRES stay FP64.  Its real-space pair sums are FP64 by
+default; with rbc3d_set_split_pairs(ctx, 1) they run in FP32 (target-bin-relative
+coordinates, FP32 accumulation), which the pair-sum tests here switch on.  Bar: every
+output within 1e-6 relative (max-norm) of the FP64 oracle; each test prints the error it
+measured."""
 import numpy as np
 import pytest
 
@@ -29,6 +31,7 @@ def c1split(R, orc):
     w = configs.make_workload("C1", seed=1)
     g = R.from_workload(w)
     g.set_precision(g.PREC_SPLIT)
+    g.set_split_pairs(True)
     return g, orc.from_workload(w)
 
 
@@ -99,6 +102,7 @@ def test_split_dense_subset(R, orc):
     w = configs.make_workload("C2", seed=3, ncell=48, Nb=(48, 48, 48))
     g = R.from_workload(w)
     g.set_precision(g.PREC_SPLIT)
+    g.set_split_pairs(True)
     o = orc.from_workload(w)
     tl = o.tlist_rbc()
     c = 1 / (4 * np.pi)
@@ -108,6 +112,21 @@ def test_split_dense_subset(R, orc):
     e_mv = relerr(g.cell_matmult(u), o.cell_matmult(u))
     print(f"split dense 48 cells: AddIntOnRbcs {e_pair:.2e}, cell matvec {e_mv:.2e}")
     assert e_pair < TOL and e_mv < TOL
+
+
+def test_split_default_pairs_are_fp64(R):
+    """The split form's pair sums are FP64 unless FP32 pairs are asked for: AddIntOnRbcs
+    (pair sums and FP64 quadratures, no PME) equals the plain form's bit for bit."""
+    from rbc3d_amd import configs
+    w = configs.make_workload("C1", seed=1)
+    g0, g1 = R.from_workload(w), R.from_workload(w)
+    g1.set_precision(g1.PREC_SPLIT)
+    tl = g0.tlist_rbc()
+    c = 1 / (4 * np.pi)
+    v0 = g0.add_int_on_rbcs(c, c, tl)
+    assert np.array_equal(g1.add_int_on_rbcs(c, c, tl), v0)
+    g1.set_split_pairs(True)
+    assert not np.array_equal(g1.add_int_on_rbcs(c, c, tl), v0)
 
 
 def test_split_switch_back_is_exact(R, orc):
