@@ -159,8 +159,13 @@ def test_c4_committed_network_held_out_seed(R):
     """The committed network (rbc3d_amd/data/mlp_C3.npz) on a held-out placement (seed
     201), one step of the C4 protocol from zero traction: the warm-started solve in the
     chosen form converges to 1e-8 (with plain GMRES as the fallback), and its cell
-    velocity matches the plain FP64 solve's within 1e-6.  The iteration counts are
-    printed; the gain itself is measured by the benchmark."""
+    velocity matches the plain FP64 solve's within 1e-6.  Plain GMRES is the judge of the
+    network's worth: the test records the ratio of the warm-started solve's iterations to
+    the chosen form's from zero.  The network does not pay for itself: over five held-out
+    placements its iterations are 1.19x those from zero, and on this step 1.10x (222 vs
+    202; plain FP64 GMRES(30) takes 844).  The ratio must stay within [0.7, 1.3]: a
+    network that helped, or one that hurt more than that, changes the recorded result and
+    fails the test."""
     import os
 
     from rbc3d_amd import configs, learned
@@ -178,6 +183,9 @@ def test_c4_committed_network_held_out_seed(R):
               f"velocity {r['vel_diff']:.2e}", flush=True)
         assert r["plain_conv"] and r["learned_conv"] and r["cold_conv"]
         assert r["vel_diff"] <= 1e-6
+        ratio = r["learned_iters"] / r["cold_iters"]
+        print(f"C4 seed 201: warm-started / from-zero iterations {ratio:.3f}", flush=True)
+        assert 0.7 <= ratio <= 1.3
     finally:
         g.close()
         m.close()
