@@ -419,7 +419,11 @@ int rbc3d_restart_write(const char *fn, const rbc3d_restart_info *info, const in
                         const double *starting_area, const double *x, const int *nvert,
                         const int *nele, const double *wx, const double *wf, const int *e2v);
 /* The header and counts, to size the arrays of rbc3d_restart_read; that call checks the
- * file still has the counts given in *info, then fills *info and the arrays. */
+ * file still has the counts given in *info, then fills *info and the arrays.  Both refuse
+ * (RBC3D_EIO), as ReadRestart does (common/ModIO.F90:858-875), a cell whose grid
+ * RBC_Create would not re-create from its nlat0: nlat must be nlat0 + 2 when
+ * nlat/nlat0 < 1.5, else k nlat0 with k = nlat / nlat0 (integer quotient) > 1, and
+ * nlon = 2 nlat. */
 int rbc3d_restart_info_read(const char *fn, rbc3d_restart_info *info);
 int rbc3d_restart_read(const char *fn, rbc3d_restart_info *info, int *nlat0, int *nlon0, int *nlat,
                        int *nlon, int *celltype, double *starting_area, double *x, int *nvert,

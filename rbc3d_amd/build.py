@@ -35,14 +35,11 @@ FDIR = os.path.join(HERE, "fortran")
 FBUILD = os.path.join(FDIR, "build")
 DRIVER_SRC = os.path.join(os.path.dirname(HERE), "tests", "fortran", "velocity_driver.f90")
 DRIVER = os.path.join(os.path.dirname(HERE), "tests", "fortran", "velocity_driver")
-RESTART_SRC = os.path.join(os.path.dirname(HERE), "tests", "fortran", "restart_copy.f90")
-RESTART = os.path.join(os.path.dirname(HERE), "tests", "fortran", "restart_copy")
 
 
 def build_fortran(verbose=True):
-    """The ISO_C_BINDING module (rbc3d_bind), the Fortran host driver the GPU tests run
-    (it links librbc3d_gpu.so through an $ORIGIN-relative rpath) and the Fortran program
-    that copies restart files record by record (the restart tests)."""
+    """The ISO_C_BINDING module (rbc3d_bind) and the Fortran host driver the GPU
+    tests run; the driver links librbc3d_gpu.so through an $ORIGIN-relative rpath."""
     os.makedirs(FBUILD, exist_ok=True)
     mod_obj = os.path.join(FBUILD, "rbc3d_bind.o")
     cmds = [
@@ -50,7 +47,6 @@ def build_fortran(verbose=True):
          "-o", mod_obj],
         [FLANG, "-O2", "-I", FBUILD, DRIVER_SRC, mod_obj, "-L", HERE, "-lrbc3d_gpu",
          "-Wl,-rpath,$ORIGIN/../../rbc3d_amd", "-o", DRIVER],
-        [FLANG, "-O2", RESTART_SRC, "-o", RESTART],
     ]
     for cmd in cmds:
         r = subprocess.run(cmd, capture_output=True, text=True)

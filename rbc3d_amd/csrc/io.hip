@@ -243,6 +243,16 @@ struct RecReader {
 
 bool rec_size_ok(long long bytes) { return bytes >= 0 && bytes <= INT32_MAX; }
 
+// The grid ReadRestart re-creates a cell on: it infers the dealiasing factor from the
+// file's nlat0 and nlat (common/ModIO.F90:858-862: 100 when nlat/nlat0 < 1.5, else the
+// integer quotient) and RBC_Create sets nlat from it (common/ModRbc.F90:55-78: nlat0 + 2
+// for 100, factor x nlat0 above 1, nlat0 for 1), nlon = 2 nlat.
+int restart_created_nlat(int nlat0, int nlat) {
+  if ((double)nlat / (double)nlat0 < 1.5) return nlat0 + 2;
+  const int k = nlat / nlat0;
+  return k > 1 ? k * nlat0 : nlat0;
+}
+
 }  // namespace
 
 extern "C" int rbc3d_wall_mesh_info(const char *fn, int *nvert, int *nele) {
@@ -370,6 +380,13 @@ static int restart_parse(const char *fn, const std::vector<unsigned char> &buf, 
     if (b[0] < 1 || b[1] < 1 || !rec_size_ok(24LL * b[0] * b[1])) return io_error(bad);
     const size_t n = (size_t)3 * b[0] * b[1];
     if (!(p = r.rec(8 * n))) return io_error(bad);
+    // the reference stops here when the re-created grid differs (common/ModIO.F90:866-875)
+    const int made = a[0] >= 1 ? restart_created_nlat(a[0], b[0]) : -1;
+    if (made != b[0] || 2 * made != b[1])
+      return io_error(std::string("ReadRestart: invalid array dimension in ") + fn + ": cell " +
+                      std::to_string(i + 1) + " has nlat0 " + std::to_string(a[0]) + ", nlat " +
+                      std::to_string(b[0]) + ", nlon " + std::to_string(b[1]) +
+                      ", which RBC_Create does not reproduce");
     if (fill) {
       nlat0[i] = a[0];
       nlon0[i] = a[1];

@@ -9,7 +9,8 @@
 - WriteRestart/ReadRestart (:742-921): a file written by the library has exactly the bytes
   of the record layout restated with Python's struct module; a Fortran program (amdflang)
   reads it with unformatted sequential input and writes it back byte for byte identical;
-  the library reads back what it wrote; a truncated file and changed counts are errors.
+  the library reads back what it wrote; a truncated file and changed counts are errors,
+  and so is a cell grid ReadRestart's dealiasing inference would not re-create.
 """
 import os
 import shutil
@@ -106,7 +107,8 @@ def test_exodus_errors(tmp_path):
 
 def sample_state(seed=0):
     rng = np.random.default_rng(seed)
-    nlat = np.array([5, 7, 5], np.int32)
+    # grids ReadRestart re-creates: nlat0 + 2 (nlat/nlat0 < 1.5) or k nlat0
+    nlat = np.array([7, 8, 12], np.int32)
     cells = dict(nlat0=np.array([5, 6, 4], np.int32), nlon0=np.array([10, 12, 8], np.int32),
                  nlat=nlat, nlon=2 * nlat, celltype=np.array([1, 2, 1], np.int32),
                  x=rng.standard_normal(int((3 * nlat * 2 * nlat).sum())))
@@ -184,6 +186,24 @@ def test_restart_through_fortran(tmp_path):
     assert open(fn2, "rb").read() == open(fn, "rb").read()
     r = R.read_restart(fn2)
     assert r["x"].size == 0 and r["walls"] == []
+
+
+@pytest.mark.parametrize("nlat0,nlat,ok", [(13, 15, True), (6, 12, True), (4, 12, True),
+                                             (13, 13, False), (6, 10, False), (5, 16, False)])
+def test_restart_grid_inference(tmp_path, nlat0, nlat, ok):
+    """ReadRestart's dealiasing inference: 100 (nlat = nlat0 + 2) below a ratio of 1.5,
+    else the integer quotient; a grid RBC_Create would not re-create is refused, as the
+    reference stops on it (a cell with nlat = nlat0 among them)."""
+    cells = dict(nlat0=np.array([nlat0], np.int32), nlon0=np.array([2 * nlat0], np.int32),
+                 nlat=np.array([nlat], np.int32), nlon=np.array([2 * nlat], np.int32),
+                 celltype=np.array([1], np.int32), x=np.zeros(3 * 2 * nlat * nlat))
+    fn = str(tmp_path / "g.dat")
+    R.write_restart(fn, (8.0, 8.0, 8.0), 0, 0.0, (0, 0, 0), cells, np.array([12.0]))
+    if ok:
+        assert R.read_restart(fn)["nlat"][0] == nlat
+    else:
+        with pytest.raises(R.Rbc3dError, match="invalid array dimension"):
+            R.read_restart(fn)
 
 
 def test_restart_errors(tmp_path):
