@@ -170,7 +170,12 @@ int rbc3d_rbc_sph_project(int ncell, const int *nlat, const int *nlat0, int nvar
 
 /* --- time stepping of the context's cells (common/ModTimeInt.F90) ---------------
  * The cells' coordinates stay on the device between steps; rbc3d_set_cells gives the
- * initial state, rbc3d_get_rbc_coords reads it back in the same per-cell layout. */
+ * initial state, rbc3d_get_rbc_coords reads it back in the same per-cell layout.  After
+ * an entry that moves the cells (move, rebox, volume constraint, filter, inter-cell
+ * repulsion), the next operator call (AddIntOnRbcs, the cell matvec and right-hand side,
+ * the solves, the wall residual, CalcVelocityField, a PME spread of the cells) first
+ * rebuilds the surface sources of the new coordinates: geometry, splines, source lists
+ * and the f and g densities on the new dS. */
 /* rbcRefs (common/ModTimeInt.F90:63-76, common/ModRbc.F90 RBC_ComputeGeometry): one
  * reference cell per cell type k (1..ntypes), its coordinates x(nlat(k), 2 nlat(k), 3)
  * concatenated over types, and the membrane moduli ES(k), ED(k), EB(k).  A cell's grid

@@ -641,6 +641,7 @@ extern "C" int rbc3d_set_cells(rbc3d_ctx *c, int nrbc, const int *nlat0, const i
   }
   c->np = np;
   c->sbq_ready = false;
+  c->x_moved = false;
   c->nchunks = choff;
   c->geom_epoch++;
   // the previous cell-BIE solution is the next solve's initial guess only for the same
@@ -778,6 +779,7 @@ extern "C" int rbc3d_add_int_on_rbcs(rbc3d_ctx *c, double c1, double c2, int nT,
                                      const int *tindx, const int *active, const double *Acoef,
                                      double *v) {
   if (!c || nT < 0 || (nT > 0 && (!tx || !tindx || !active || !Acoef || !v))) return RBC3D_EINVAL;
+  RBC_TRY(rbc_refresh_moved(c));
   if (c->nrbc == 0 || nT == 0) return RBC3D_OK;
   RBC_TRY(upload_targets(c, nT, tx, tindx, active, Acoef, v));
   RBC_TRY(rbc_add_int_on_rbcs_dev(c, c1, c2, nT, c->d_tx.p, c->d_tindx.p, c->d_tact.p, c->d_tA.p,
@@ -844,6 +846,7 @@ extern "C" int rbc3d_near_stats(rbc3d_ctx *c, long long *nitems, long long *nwor
 
 extern "C" int rbc3d_pme_distrib_source(rbc3d_ctx *c, double c1, double c2, int cells, int walls) {
   if (!c) return RBC3D_EINVAL;
+  if (cells) RBC_TRY(rbc_refresh_moved(c));
   RBC_TRY(rbc_pme_init(c));
   RBC_TRY(rbc_pme_spread(c, c1, c2, cells, walls));
   HIP_TRY(hipStreamSynchronize(c->stream));
@@ -1151,6 +1154,7 @@ extern "C" int rbc3d_graph_ready(rbc3d_ctx *c, int *ready) {
 extern "C" int rbc3d_cell_matmult_dev(rbc3d_ctx *c, const double *d_u, double *d_b) {
   if (!c || !d_u || !d_b) return RBC3D_EINVAL;
   if (c->nrbc == 0) return RBC3D_ESTATE;
+  RBC_TRY(rbc_refresh_moved(c));
   if (c->use_graph && c->geom_reuse) return cell_matmult_graph(c, d_u, d_b);
   return cell_matmult_body(c, d_u, d_b);
 }
@@ -1171,6 +1175,7 @@ extern "C" int rbc3d_cell_matmult(rbc3d_ctx *c, const double *u, double *b) {
 extern "C" int rbc3d_cell_rhs_dev(rbc3d_ctx *c, double *d_rhs) {
   if (!c || !d_rhs) return RBC3D_EINVAL;
   if (c->nrbc == 0) return RBC3D_ESTATE;
+  RBC_TRY(rbc_refresh_moved(c));
   int np = c->np, dof;
   rbc3d_cell_dof(c, &dof);
   hipStream_t st = c->stream;
@@ -1208,6 +1213,7 @@ extern "C" int rbc3d_calc_velocity_field(rbc3d_ctx *c, int nT, const double *tx,
                                          const int *active, const double *Acoef, double *v) {
   if (!c || nT < 0 || (nT > 0 && (!tx || !tindx || !active || !Acoef || !v))) return RBC3D_EINVAL;
   if (nT == 0) return RBC3D_OK;
+  RBC_TRY(rbc_refresh_moved(c));
   for (int i = 0; i < 3 * nT; i++) v[i] = 0.;
   RBC_TRY(upload_targets(c, nT, tx, tindx, active, Acoef, v));
   double c1 = 1.0 / (4. * RBC_PI), c2 = 1.0 / (4. * RBC_PI);

@@ -536,6 +536,7 @@ extern "C" int rbc3d_noslip_wall(rbc3d_ctx *c, double rtol, int maxit, const dou
   if (!c || !niter || !rnorm || !converged) return RBC3D_EINVAL;
   WallData &w = c->wall;
   if (!w.nvert) return RBC3D_ESTATE;
+  RBC_TRY(rbc_refresh_moved(c));
   hipStream_t st = c->stream;
   int nv = w.nvert, nd = w.ndof, nu = nd / 3;
   DBuf<double> &f0 = c->gmres.f0, &rhs = c->gmres.rhs, &x = c->gmres.x, &tmpv = c->gmres.tmp;
@@ -672,6 +673,7 @@ extern "C" int rbc3d_noslip_wall_learned(rbc3d_ctx *c, rbc3d_mlp *m, double rtol
   if (!c || !m || !niter || !rnorm || !converged || !fallback) return RBC3D_EINVAL;
   WallData &w = c->wall;
   if (!w.nvert) return RBC3D_ESTATE;
+  RBC_TRY(rbc_refresh_moved(c));
   if (!w.prepared) RBC_TRY(rbc_wall_prepare(c));
   hipStream_t st = c->stream;
   const int nv = w.nvert, nd = w.ndof, nu = nd / 3;
@@ -1011,6 +1013,7 @@ extern "C" int rbc3d_sb_modes(rbc3d_ctx *c, double *modes) {
 extern "C" int rbc3d_solve_rbc_vel(rbc3d_ctx *c, double *v, int *niter, double *rnorm) {
   if (!c || !niter || !rnorm) return RBC3D_EINVAL;
   if (c->nrbc == 0) return RBC3D_ESTATE;
+  RBC_TRY(rbc_refresh_moved(c));
   hipStream_t st = c->stream;
   int dof;
   RBC_TRY(rbc3d_cell_dof(c, &dof));

@@ -311,6 +311,9 @@ struct rbc3d_ctx {
   bool vel_ready = false;
   int mech_epoch = -1;  // geom_epoch the per-cell tables above were built for
   bool rbc_geom_ready = false;  // the block geometry above is that of d_x
+  // d_x moved by a cell update (move, rebox, volume constraint, filter, repulsion) since
+  // the surface sources were last built: the operator entries rebuild them first
+  bool x_moved = false;
   // repulsion (rbc3d_set_repulsion, repulsion.hip): ForceCoef of the intra-cell force,
   // viscRatThresh and rigidsep of the inter-cell displacement; per-cell bounding spheres,
   // candidate neighbour cells and wall triangles (capacities grown on overflow), the
@@ -556,6 +559,9 @@ int rbc_update_density(rbc3d_ctx *c, int which);
 int rbc_cells_geometry(rbc3d_ctx *c, bool sources);
 int rbc_cells_force(rbc3d_ctx *c);
 int rbc_cells_project_x(rbc3d_ctx *c);
+// the surface sources of moved cells rebuilt (geometry, splines, source lists, both
+// densities on the new dS); nothing when the cells have not moved (timeint.hip)
+int rbc_refresh_moved(rbc3d_ctx *c);
 // AddIntraCellRepulsionForce on the block geometry (repulsion.hip): mode 0 writes df to
 // out, mode 1 turns the block force out = f into -f + df; count (optional, device) is
 // increased by the number of nodes that received a force

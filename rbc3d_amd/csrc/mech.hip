@@ -832,6 +832,7 @@ int rbc_cells_geometry(rbc3d_ctx *c, bool sources) {
   c->geom_epoch++;
   c->mech_epoch = c->geom_epoch;  // same cells: the tables stay valid
   c->rbc_geom_ready = true;
+  c->x_moved = false;
   HIP_TRY(hipStreamSynchronize(st));
   return RBC3D_OK;
 }
@@ -918,6 +919,7 @@ int rbc_cells_project_x(rbc3d_ctx *c) {
   k_blocks_to_nodes<<<nb, 256, 0, st>>>(np, c->d_node_cell.p, c->d_cells.p, c->d_mx.p, 3, 1.0, c->d_x.p);
   HIP_TRY(hipGetLastError());
   c->rbc_geom_ready = false;
+  c->x_moved = true;
   return RBC3D_OK;
 }
 

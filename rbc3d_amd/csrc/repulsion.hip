@@ -471,5 +471,6 @@ extern "C" int rbc3d_inter_cell_repulsion(rbc3d_ctx *c, double eps_dist, int *np
     *dist_min = v;
   }
   c->rbc_geom_ready = false;
+  c->x_moved = true;
   return RBC3D_OK;
 }

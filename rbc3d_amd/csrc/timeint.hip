@@ -117,6 +117,7 @@ extern "C" int rbc3d_move_rbcs(rbc3d_ctx *c, double dt) {
   k_move<<<(unsigned)((n + 255) / 256), 256, 0, c->stream>>>(n, dt, c->d_vel.p, c->d_x.p);
   HIP_TRY(hipGetLastError());
   c->rbc_geom_ready = false;
+  c->x_moved = true;
   return RBC3D_OK;
 }
 
@@ -126,6 +127,7 @@ extern "C" int rbc3d_rebox_rbcs(rbc3d_ctx *c) {
                                          c->iLb[0], c->iLb[1], c->iLb[2], c->d_x.p);
   HIP_TRY(hipGetLastError());
   c->rbc_geom_ready = false;
+  c->x_moved = true;
   return RBC3D_OK;
 }
 
@@ -157,6 +159,7 @@ extern "C" int rbc3d_vol_constrain_rbcs(rbc3d_ctx *c, double eps_dist) {
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(c->stream));
   c->rbc_geom_ready = false;
+  c->x_moved = true;
   return RBC3D_OK;
 }
 
@@ -258,6 +261,7 @@ extern "C" int rbc3d_timeint_rk2(rbc3d_ctx *c, double alpha, double Ts, int nste
     k_rk2_update<<<nb, 256, 0, c->stream>>>(n, Ts, b1, b2, c->d_vel_k1.p, c->d_vel.p, c->d_x.p);
     HIP_TRY(hipGetLastError());
     c->rbc_geom_ready = false;
+    c->x_moved = true;
     RBC_TRY(rbc3d_rebox_rbcs(c));
     *time += Ts;
   }
@@ -293,9 +297,18 @@ extern "C" int rbc3d_timeint_ab2(rbc3d_ctx *c, double Ts, int nsteps, double nos
     k_ab2_update<<<nb, 256, 0, c->stream>>>(n, Ts, c->d_vel.p, c->d_vel_k1.p, c->d_x.p);
     HIP_TRY(hipGetLastError());
     c->rbc_geom_ready = false;
+    c->x_moved = true;
     RBC_TRY(rbc3d_rebox_rbcs(c));
     *time += Ts;
   }
   HIP_TRY(hipStreamSynchronize(c->stream));
+  return RBC3D_OK;
+}
+
+int rbc_refresh_moved(rbc3d_ctx *c) {
+  if (!c->x_moved || c->nrbc == 0) return RBC3D_OK;
+  RBC_TRY(rbc_cells_geometry(c, true));
+  RBC_TRY(rbc_update_density(c, 1));
+  RBC_TRY(rbc_update_density(c, 2));
   return RBC3D_OK;
 }

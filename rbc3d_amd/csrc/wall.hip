@@ -1069,6 +1069,7 @@ int rbc_wall_cell_vel_dev(rbc3d_ctx *c, double *d_out) {
 
 extern "C" int rbc3d_wall_cell_velocity(rbc3d_ctx *c, double *v) {
   if (!c || !v) return RBC3D_EINVAL;
+  RBC_TRY(rbc_refresh_moved(c));
   RBC_TRY(c->d_shbuf2.ensure((size_t)3 * c->wall.nvert + 1));
   RBC_TRY(rbc_wall_cell_vel_dev(c, c->d_shbuf2.p));
   HIP_TRY(hipMemcpyAsync(v, c->d_shbuf2.p, sizeof(double) * 3 * c->wall.nvert,
@@ -1111,6 +1112,7 @@ int rbc_wall_residual_dev(rbc3d_ctx *c) {
 
 extern "C" int rbc3d_wall_residual_vel(rbc3d_ctx *c, double *v) {
   if (!c || !v) return RBC3D_EINVAL;
+  RBC_TRY(rbc_refresh_moved(c));
   RBC_TRY(rbc_wall_residual_dev(c));
   HIP_TRY(hipMemcpyAsync(v, c->d_v.p, sizeof(double) * 3 * c->wall.nvert, hipMemcpyDeviceToHost,
                          c->stream));

@@ -156,3 +156,27 @@ def test_ab2_tube_with_noslip_wall():
     print(f"AB2 tube: coordinates {err:.2e}, wall traction {e_f:.2e}, wall iterations "
           f"{list(iw)} vs {[b for _, b in its]}", flush=True)
     assert err < TOL and e_f < 1e-9 and list(iw) == [b for _, b in its]
+
+
+def test_operators_after_a_cell_update_see_the_moved_cells():
+    """A cell update (here ReboxRbcs bringing a cell back from one period away) leaves the
+    surface sources of the old coordinates; the next operator call rebuilds them first,
+    so AddIntOnRbcs and the cell matvec equal those of a context set up at the new
+    coordinates."""
+    import dataclasses
+    import rbc3d_amd as R
+    from tests.conftest import relerr
+    cells = deformed_cells([(2.5, 2.5, 2.5), (5.5, 5.6, 5.4)], seed=13)
+    w = workload(cells, 5.0)
+    X = cells.x.reshape(2, 3, -1).copy()
+    X[0] += np.array([8.0, 0.0, -8.0])[:, None]
+    g1 = R.from_workload(dataclasses.replace(w, cells=dataclasses.replace(cells, x=X.ravel())))
+    g1.rebox_rbcs()
+    g2 = R.from_workload(w)
+    tl = g2.tlist_rbc()
+    c = 1 / (4 * np.pi)
+    e_add = relerr(g1.add_int_on_rbcs(c, c, tl), g2.add_int_on_rbcs(c, c, tl))
+    u = np.random.default_rng(3).standard_normal(g2.cell_dof())
+    e_mv = relerr(g1.cell_matmult(u), g2.cell_matmult(u))
+    print(f"after ReboxRbcs: AddIntOnRbcs {e_add:.2e}, cell matvec {e_mv:.2e}", flush=True)
+    assert e_add < 1e-10 and e_mv < 1e-10
