@@ -349,6 +349,13 @@ __device__ __forceinline__ int imod_d(int a, int n) {
   int r = a % n;
   return r < 0 ? r + n : r;
 }
+// imod_d for indices that usually lie within one period of [0, n) (stencil points of a
+// point near the box): compares and one add or subtract, the division only beyond that
+__device__ __forceinline__ int wrap_idx(int a, int n) {
+  if ((unsigned)a < (unsigned)n) return a;
+  const int b = a < 0 ? a + n : a - n;
+  return (unsigned)b < (unsigned)n ? b : imod_d(a, n);
+}
 
 // XCD-aware block order: workgroups are dealt to the 8 XCDs round-robin
 // (blockIdx % 8), so logical block (blockIdx % 8) * per + blockIdx / 8 gives each XCD

@@ -1945,11 +1945,11 @@ __global__ void __launch_bounds__(256) k_gather(GatherArgs<T> a) {
     const size_t ng = (size_t)a.N1 * a.N2 * a.N3;
     for (int r = sub; r < P * P; r += GATHER_L) {
       const int j0 = r % P, k0 = r / P;
-      const int k = imod_d(kmin + k0, a.N3), j = imod_d(jmin + j0, a.N2);
+      const int k = wrap_idx(kmin + k0, a.N3), j = wrap_idx(jmin + j0, a.N2);
       const double wjk = wy[j0] * wz[k0];
       const size_t row = (size_t)a.N1 * (j + (size_t)a.N2 * k);
       for (int i0 = 0; i0 < P; i0++) {
-        const int i = imod_d(imin + i0, a.N1);
+        const int i = wrap_idx(imin + i0, a.N1);
         const double w = wx[i0] * wjk;
         const size_t o = row + i;
         v0 += w * a.vv[o];
@@ -2034,7 +2034,7 @@ __global__ void __launch_bounds__(256) k_gather_wall(int nT, int N1, int N2, int
     // lane per stencil point, x fastest: one load instruction covers a few rows
     for (int p = sub; p < P * P * P; p += GATHER_WL) {
       const int i0 = p % P, r = p / P, j0 = r % P, k0 = r / P;
-      const int k = imod_d(kmin + k0, N3), j = imod_d(jmin + j0, N2), i = imod_d(imin + i0, N1);
+      const int k = wrap_idx(kmin + k0, N3), j = wrap_idx(jmin + j0, N2), i = wrap_idx(imin + i0, N1);
       const T w = W[i0] * (W[P + j0] * W[2 * P + k0]);
       const size_t o = i + (size_t)N1 * (j + (size_t)N2 * k);
       v0 += w * vv[o];
@@ -2044,13 +2044,13 @@ __global__ void __launch_bounds__(256) k_gather_wall(int nT, int N1, int N2, int
 #else
     for (int r = sub; r < P * P; r += GATHER_WL) {
       const int j0 = r % P, k0 = r / P;
-      const int k = imod_d(kmin + k0, N3), j = imod_d(jmin + j0, N2);
+      const int k = wrap_idx(kmin + k0, N3), j = wrap_idx(jmin + j0, N2);
       const T wjk = W[P + j0] * W[2 * P + k0];
       const size_t row = (size_t)N1 * (j + (size_t)N2 * k);
 #pragma unroll
       for (int i0 = 0; i0 < (PC ? PC : 16); i0++) {
         if (!PC && i0 >= P) break;
-        const int i = imod_d(imin + i0, N1);
+        const int i = wrap_idx(imin + i0, N1);
         const T w = W[i0] * wjk;
         const size_t o = row + i;
         v0 += w * vv[o];
