@@ -222,7 +222,9 @@ int rbc3d_inter_cell_repulsion(rbc3d_ctx *ctx, double eps_dist, int *npoints, do
  * NoSlipWall when a wall is set (rtol, maxit as rbc3d_noslip_wall), x += Ts v,
  * ReboxRbcs, four rounds of VolConstrainRbcs, InterCellRepulsion (when the repulsion is
  * on, at eps_dist) and FilterRbcs; *time += Ts per step.  its_cell, its_wall (length
- * nsteps, optional): the GMRES iterations of each step's two solves. */
+ * nsteps, optional): the GMRES iterations of each step's two solves.  In all three
+ * integrators a solve that reaches its maxit does not stop the run (the reference prints
+ * the residual and goes on); its count then equals maxit. */
 int rbc3d_timeint_euler(rbc3d_ctx *ctx, double Ts, int nsteps, double eps_dist,
                         double noslip_rtol, int noslip_maxit, double *time, int *its_cell,
                         int *its_wall);

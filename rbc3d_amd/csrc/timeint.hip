@@ -189,7 +189,8 @@ extern "C" int rbc3d_get_rbc_coords(rbc3d_ctx *c, double *x) {
 // NoSlipWall (when a wall is set), x += Ts v, ReboxRbcs, then four rounds of
 // VolConstrainRbcs, InterCellRepulsion (when the repulsion is on) and FilterRbcs.  time
 // advances by Ts per step; per step the cell and wall GMRES iteration counts go to
-// its_cell / its_wall when given (length nsteps).
+// its_cell / its_wall when given (length nsteps).  A solve that reaches its maxit does
+// not stop the run (its count is maxit), as the reference prints the residual and goes on.
 extern "C" int rbc3d_timeint_euler(rbc3d_ctx *c, double Ts, int nsteps, double eps_dist,
                                    double noslip_rtol, int noslip_maxit, double *time,
                                    int *its_cell, int *its_wall) {
@@ -198,7 +199,8 @@ extern "C" int rbc3d_timeint_euler(rbc3d_ctx *c, double Ts, int nsteps, double e
   for (int lt = 0; lt < nsteps; lt++) {
     int ni = 0, nw = 0, conv = 0;
     double rn = 0, vres = 0;
-    RBC_TRY(rbc3d_compute_rbc_vel(c, nullptr, &ni, &rn));
+    const int rv = rbc3d_compute_rbc_vel(c, nullptr, &ni, &rn);
+    if (rv != RBC3D_OK && rv != RBC3D_ENOCONV) return rv;
     if (c->wall.nvert) {
       const int r = rbc3d_noslip_wall(c, noslip_rtol, noslip_maxit, nullptr, &nw, &rn, &conv, &vres);
       if (r != RBC3D_OK && r != RBC3D_ENOCONV) return r;
@@ -223,7 +225,8 @@ static int velocity_and_wall(rbc3d_ctx *c, double noslip_rtol, int noslip_maxit,
   double rn = 0, vres = 0;
   int conv = 0;
   *ni = *nw = 0;
-  RBC_TRY(rbc3d_compute_rbc_vel(c, nullptr, ni, &rn));
+  const int rv = rbc3d_compute_rbc_vel(c, nullptr, ni, &rn);
+  if (rv != RBC3D_OK && rv != RBC3D_ENOCONV) return rv;
   if (c->wall.nvert) {
     const int r = rbc3d_noslip_wall(c, noslip_rtol, noslip_maxit, nullptr, nw, &rn, &conv, &vres);
     if (r != RBC3D_OK && r != RBC3D_ENOCONV) return r;
@@ -283,7 +286,8 @@ extern "C" int rbc3d_timeint_ab2(rbc3d_ctx *c, double Ts, int nsteps, double nos
   RBC_TRY(c->d_vel_k1.ensure(n + 1));
   int ni = 0;
   double rn = 0;
-  RBC_TRY(rbc3d_compute_rbc_vel(c, nullptr, &ni, &rn));
+  const int rv = rbc3d_compute_rbc_vel(c, nullptr, &ni, &rn);
+  if (rv != RBC3D_OK && rv != RBC3D_ENOCONV) return rv;
   if (its_cell) its_cell[0] = ni;
   if (its_wall) its_wall[0] = 0;
   RBC_TRY(rbc3d_move_rbcs(c, Ts));

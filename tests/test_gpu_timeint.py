@@ -180,3 +180,16 @@ def test_operators_after_a_cell_update_see_the_moved_cells():
     e_mv = relerr(g1.cell_matmult(u), g2.cell_matmult(u))
     print(f"after ReboxRbcs: AddIntOnRbcs {e_add:.2e}, cell matvec {e_mv:.2e}", flush=True)
     assert e_add < 1e-10 and e_mv < 1e-10
+
+
+def test_maxit_does_not_stop_the_run():
+    """A NoSlipWall that reaches its maxit is reported through its count and the step
+    goes on (the reference prints the residual and continues)."""
+    import rbc3d_amd as R
+    w = configs.make_workload("C3s", seed=2)
+    g = R.from_workload(w)
+    g.set_ref_cells([NLAT], ref_x(), [ES], [ED], [EB])
+    t, ic, iw = g.timeint_euler(0.02, 2, 0.05, noslip_rtol=1e-14, noslip_maxit=3)
+    print(f"maxit 3: wall iterations {list(iw)}, time {t}", flush=True)
+    assert list(iw) == [3, 3] and t == pytest.approx(0.04)
+    assert np.all(np.isfinite(g.rbc_coords()))
