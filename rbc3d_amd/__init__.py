@@ -497,8 +497,9 @@ class System:
     PREC_SPLIT = 1
 
     def set_precision(self, mode):
-        """0: plain FP64 (default); 1: split form (FP32 pair sums and PME, split-BF16
-        Legendre GEMMs, FP32 wall self-matrix; quadratures and GMRES in FP64)."""
+        """0: plain FP64 (default); 1: split form (FP32 PME, split-BF16 Legendre GEMMs,
+        FP32 wall self-matrix; pair sums, quadratures and GMRES in FP64 -- FP32 pair sums
+        only after set_split_pairs(True))."""
         _call("rbc3d_set_precision", self.h, int(mode))
 
     def set_split_pairs(self, fp32):
