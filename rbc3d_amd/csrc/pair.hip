@@ -550,10 +550,26 @@ __global__ void k_chunks(int nbins, const int *__restrict__ tbin_start, int *chu
 // source inside the box) give d = fl(xs - xi) + fl((k_run + kt) Lb), the same two
 // roundings as the reference's fl(fl(xs - xi) - fl(NINT * Lb)) with NINT = -(k_run +
 // kt); a candidate whose image differs is farther than rc either way.  Tiles holding a
-// source outside the box take the reference's formula.  The tiles take 26 KB of LDS,
+// source outside the box take the reference's formula.  In the image path the loader
+// keeps only the sources whose image comes within rc (plus a margin) of the chunk's
+// bounding sphere, packed in their order: about 30% of the candidates of a dense
+// packing pair with some target of the chunk, and the rest would each cost every lane
+// a distance test.  The pair set is unchanged.  The tiles take 26 KB of LDS,
 // so six workgroups fit a compute unit; the register budget is set to match (six
 // wavefronts per SIMD with a few spills measured 10% faster than four without).
-template <bool SL, bool DL>
+// a value every lane of the wavefront holds, moved to scalar registers
+__device__ __forceinline__ double wave_uniform(double v) {
+  const unsigned long long u = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)u);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(u >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// FILT: the image-path loader drops the sources beyond reach of the chunk (above); it
+// pays where bins are dense (a 20% packing: 7.7% less time) and costs where they are
+// sparse (a tube with ~8 sources per fine bin: 6% more), so the host chooses it by the
+// mean number of sources per fine bin.
+template <bool SL, bool DL, bool FILT>
 __global__ void __launch_bounds__(PT *PW) __attribute__((amdgpu_waves_per_eu(6))) k_pair(PairArgs a) {
   constexpr int NP = (SL ? 3 : 0) + (DL ? 6 : 0);
   constexpr int PD = SL ? 3 : 0;  // first DL payload slot
@@ -561,7 +577,8 @@ __global__ void __launch_bounds__(PT *PW) __attribute__((amdgpu_waves_per_eu(6))
   __shared__ double sp[NP > 0 ? NP : 1][PS];
   __shared__ int snode[PS], scell[PS], sll[PS];
   __shared__ double red[PW - 1][3][PT];
-  __shared__ int tile_out;  // a source of the tile lies outside the box
+  __shared__ int wout[PW], wkeep[PW];  // per wavefront: a source outside the box; sources kept
+  static_assert(PS == PT * PW, "one source per thread and tile");
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
   const int nb = a.nfbins, blk = blockIdx.x;
   if (blk >= a.chunk_off[nb]) return;
@@ -608,6 +625,39 @@ __global__ void __launch_bounds__(PT *PW) __attribute__((amdgpu_waves_per_eu(6))
   }
   const double L0 = a.Lb[0], L1 = a.Lb[1], L2 = a.Lb[2];
   const double iL0 = a.iLb[0], iL1 = a.iLb[1], iL2 = a.iLb[2];
+  // The chunk's bounding sphere over its active targets, each taken back into the box
+  // (x - kt Lb): in the image path a candidate's d is (xs + k_run Lb) - (xi - kt Lb) up
+  // to a few roundings, so a source whose image lies farther than rc + rt (plus a margin
+  // far above those roundings) from the centre is beyond rc of every target and the
+  // tile loader leaves it out.  Every wavefront holds the same targets and computes the
+  // same sphere.
+  double ct0 = 0, ct1 = 0, ct2 = 0, cut2 = -1.;
+  if (FILT) {
+    const double w0 = act ? xi0 - kt0 * L0 : 0., w1 = act ? xi1 - kt1 * L1 : 0.,
+                 w2 = act ? xi2 - kt2 * L2 : 0.;
+    double s0 = w0, s1 = w1, s2 = w2, n = act ? 1. : 0.;
+    for (int o = 32; o > 0; o >>= 1) {
+      s0 += __shfl_xor(s0, o, 64);
+      s1 += __shfl_xor(s1, o, 64);
+      s2 += __shfl_xor(s2, o, 64);
+      n += __shfl_xor(n, o, 64);
+    }
+    if (n > 0.) {
+      ct0 = s0 / n;
+      ct1 = s1 / n;
+      ct2 = s2 / n;
+      double r2 = act ? (w0 - ct0) * (w0 - ct0) + (w1 - ct1) * (w1 - ct1) +
+                            (w2 - ct2) * (w2 - ct2)
+                      : 0.;
+      for (int o = 32; o > 0; o >>= 1) r2 = fmax(r2, __shfl_xor(r2, o, 64));
+      const double cut = a.rc * (1. + 1e-9) + sqrt(r2) + 1e-9 * (1. + L0 + L1 + L2);
+      cut2 = cut * cut;
+    }
+    ct0 = wave_uniform(ct0);
+    ct1 = wave_uniform(ct1);
+    ct2 = wave_uniform(ct2);
+    cut2 = wave_uniform(cut2);
+  }
   double acc0 = 0, acc1 = 0, acc2 = 0;
   // pair statistics: the count, and sum (node + 1) per target (the pair-set checksum
   // sum (ti + 1)(node + 1) is (ti + 1) times it, modulo 2^64)
@@ -690,43 +740,75 @@ __global__ void __launch_bounds__(PT *PW) __attribute__((amdgpu_waves_per_eu(6))
       }
     }
     const double sh1 = (double)(k2 + kt1) * L1, sh2 = (double)(k3 + kt2) * L2;
+    const double rsh1 = (double)k2 * L1, rsh2 = (double)k3 * L2;
     for (int ru = 0; ru < nrun; ru++) {
-      const double sh0 = (double)(rk[ru] + kt0) * L0;
+      const double sh0 = (double)(rk[ru] + kt0) * L0, rsh0 = (double)rk[ru] * L0;
       const int s0 = a.bin_start[rowbase + rlo[ru]], s1 = a.bin_start[rowbase + rhi[ru] + 1];
       for (int base = s0; base < s1; base += PS) {
         const int cnt = min(PS, s1 - base);
-        __syncthreads();
-        if (threadIdx.x == 0) tile_out = 0;
-        __syncthreads();
-        for (int t = threadIdx.x; t < cnt; t += PT * PW) {
-          const int q = base + t;
-          const double x0 = a.rec[q], x1 = a.rec[q + np], x2 = a.rec[q + 2 * np];
-          sx[t] = x0;
-          sy[t] = x1;
-          sz[t] = x2;
+        // one source per thread: whether it lies outside the box, and (image path)
+        // whether its image can come within rc of the chunk's targets
+        const int t = threadIdx.x;
+        const bool hv = t < cnt;
+        const int q = base + t;
+        double x0 = 0., x1 = 0., x2 = 0.;
+        bool out = false, keep = hv;
+        if (hv) {
+          x0 = a.rec[q];
+          x1 = a.rec[q + np];
+          x2 = a.rec[q + 2 * np];
           if (a.img) {
             const double f0 = floor(x0 * a.iLbNf[0]), f1 = floor(x1 * a.iLbNf[1]),
                          f2 = floor(x2 * a.iLbNf[2]);
-            if (f0 < 0 || f0 >= Nf0 || f1 < 0 || f1 >= Nf1 || f2 < 0 || f2 >= Nf2) tile_out = 1;
+            out = f0 < 0 || f0 >= Nf0 || f1 < 0 || f1 >= Nf1 || f2 < 0 || f2 >= Nf2;
           }
+          if (FILT && a.img) {
+            const double e0 = x0 + rsh0 - ct0, e1 = x1 + rsh1 - ct1, e2 = x2 + rsh2 - ct2;
+            keep = e0 * e0 + e1 * e1 + e2 * e2 <= cut2;
+          }
+        }
+        const unsigned long long bo = __ballot(out), bk = FILT ? __ballot(keep) : 0ull;
+        __syncthreads();  // the previous tile is consumed
+        if (lane == 0) {
+          wout[wv] = bo != 0ull;
+          if (FILT) wkeep[wv] = __popcll(bk);
+        }
+        __syncthreads();
+        int tout = 0, off = 0, nkeep = 0;
+        for (int w = 0; w < PW; w++) {
+          tout |= wout[w];
+          if (FILT) {
+            off += w < wv ? wkeep[w] : 0;
+            nkeep += wkeep[w];
+          }
+        }
+        // the image path keeps only the sources that can pair, in their order; a tile
+        // holding a source outside the box keeps all of them for the reference's formula
+        const bool compact = FILT && a.img && !tout;
+        const int pos = compact ? off + __popcll(bk & ((1ull << lane) - 1ull)) : t;
+        const int ncnt = compact ? nkeep : cnt;
+        if (hv && (!compact || keep)) {
+          sx[pos] = x0;
+          sy[pos] = x1;
+          sz[pos] = x2;
           if (SL)
-            for (int d = 0; d < 3; d++) sp[d][t] = a.rec[q + np * (3 + d)];
+            for (int d = 0; d < 3; d++) sp[d][pos] = a.rec[q + np * (3 + d)];
           if (DL) {
             const double Bj = a.rec[q + np * 12];
             for (int d = 0; d < 3; d++) {
-              sp[PD + d][t] = a.rec[q + np * (6 + d)] * Bj;
-              sp[PD + 3 + d][t] = a.rec[q + np * (9 + d)];
+              sp[PD + d][pos] = a.rec[q + np * (6 + d)] * Bj;
+              sp[PD + 3 + d][pos] = a.rec[q + np * (9 + d)];
             }
           }
-          snode[t] = a.src_node[q];
-          scell[t] = a.src_node[q + np];
-          sll[t] = a.src_node[q + 2 * np];
+          snode[pos] = a.src_node[q];
+          scell[pos] = a.src_node[q + np];
+          sll[pos] = a.src_node[q + 2 * np];
         }
         __syncthreads();
         if (!act) continue;
-        if (a.img && !tile_out) {
+        if (a.img && !tout) {
 #pragma unroll 1
-          for (int j = wv; j < cnt; j += PW) {
+          for (int j = wv; j < ncnt; j += PW) {
             double d0, d1, d2, s2;
             {
 #pragma clang fp contract(off)
@@ -1886,6 +1968,14 @@ static int exclusive_scan(rbc3d_ctx *c, const int *in, int *out, int n) {
 }
 
 // launch the split-form pair kernel with the arguments of the FP64 one
+template <bool SL, bool DL>
+static void launch_pair(bool filt, int nblk, hipStream_t st, const PairArgs &pa) {
+  if (filt)
+    k_pair<SL, DL, true><<<nblk, PT * PW, 0, st>>>(pa);
+  else
+    k_pair<SL, DL, false><<<nblk, PT * PW, 0, st>>>(pa);
+}
+
 static int launch_pair32(rbc3d_ctx *c, const PairArgs &pa, int nblk_max,
                          const float *const *maskf) {
   Pair32Args a;
@@ -2041,14 +2131,18 @@ int rbc_add_int_on_rbcs_dev(rbc3d_ctx *c, double c1, double c2, int nT, const do
   rbc_stage_begin(c, ST_PAIR);
   if (c->prec == RBC3D_PREC_SPLIT && c->split_pair32) {
     RBC_TRY(launch_pair32(c, pa, nblk, tp->maskf));
-  } else if (c1 != 0 && c2 != 0)
-    k_pair<true, true><<<nblk, PT * PW, 0, st>>>(pa);
-  else if (c1 != 0)
-    k_pair<true, false><<<nblk, PT * PW, 0, st>>>(pa);
-  else if (c2 != 0)
-    k_pair<false, true><<<nblk, PT * PW, 0, st>>>(pa);
-  else
-    k_pair<false, false><<<nblk, PT * PW, 0, st>>>(pa);
+  } else {
+    // the source filter where fine bins hold 32 sources or more on average
+    const bool filt = c->pair_img && c->np >= 32 * (long long)nb;
+    if (c1 != 0 && c2 != 0)
+      launch_pair<true, true>(filt, nblk, st, pa);
+    else if (c1 != 0)
+      launch_pair<true, false>(filt, nblk, st, pa);
+    else if (c2 != 0)
+      launch_pair<false, true>(filt, nblk, st, pa);
+    else
+      launch_pair<false, false>(filt, nblk, st, pa);
+  }
   rbc_stage_end(c, ST_PAIR);
   HIP_TRY(hipGetLastError());
   // neighbouring-cell lists

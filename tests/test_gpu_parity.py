@@ -179,6 +179,31 @@ def test_dense_subset(R, orc):
     assert relerr(g.cell_matmult(u), o.cell_matmult(u)) < TOL
 
 
+def test_dense_packing_source_filter(R, orc):
+    """128 cells of the dense 20% packing in a 10.0 box: ten fine bins per dimension (the
+    image path) holding 74 sources each on average, where the pair kernel's loader keeps
+    only the sources within reach of each 64-target chunk.  Every target active: pair
+    set bit-exact, velocities and the cell matvec to 1e-12."""
+    from rbc3d_amd import configs
+    w = configs.make_workload("C2", seed=4, ncell=128, Nb=(64, 64, 64))
+    nf = 2 * int(w.Lb[0] // 2.0)  # fine bins per dimension at rc = 2
+    assert nf >= 7 and w.cells.npoint >= 32 * nf ** 3  # image path, filter on
+    g = R.from_workload(w)
+    o = orc.from_workload(w)
+    tl = o.tlist_rbc()
+    c = 1 / (4 * np.pi)
+    vo, npairs = o.add_int_on_rbcs(c, -c, tl)
+    vg = g.add_int_on_rbcs(c, -c, tl)
+    ng, cs = g.pair_stats()
+    e_v = relerr(vg, vo)
+    u = np.random.default_rng(5).standard_normal(o.cell_dof())
+    e_m = relerr(g.cell_matmult(u), o.cell_matmult(u))
+    print(f"dense 128-cell packing: {ng} pairs, AddIntOnRbcs {e_v:.2e}, matvec {e_m:.2e}",
+          flush=True)
+    assert ng == npairs and cs == o.pair_checksum()
+    assert e_v < TOL and e_m < TOL
+
+
 def test_geometry_reuse(R, orc):
     """Matvecs that reuse the geometry-only results (neighbour lists, projections) of
     the first one give exactly the values of matvecs that recompute them; after the
