@@ -39,7 +39,7 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per k_pair launch on C2 from two rocprofv3 counter runs (FETCH_SIZE,
 # WRITE_SIZE), scaled by the counters' reading for a plain 512 MiB copy (fetch x2.0,
 # write x1.0); commands and table in profiles/cell_matvec_C2_fp64_hbm_traffic.md
-K_PAIR_C2_TRAFFIC_BYTES = int((6029.4 + 37.4) * 1e6)
+K_PAIR_C2_TRAFFIC_BYTES = int((5363.8 + 342.8) * 1e6)
 
 
 def progress(msg):
