@@ -566,7 +566,7 @@ __device__ __forceinline__ double wave_uniform(double v) {
 }
 
 // FILT: the image-path loader drops the sources beyond reach of the chunk (above); it
-// pays where bins are dense (a 20% packing: 7.7% less time) and costs where they are
+// pays where bins are dense (a 20% packing: 10% less time) and costs where they are
 // sparse (a tube with ~8 sources per fine bin: 6% more), so the host chooses it by the
 // mean number of sources per fine bin.
 template <bool SL, bool DL, bool FILT>
